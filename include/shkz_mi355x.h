@@ -1,0 +1,224 @@
+/*
+ * shkz_mi355x.h -- C ABI of the MI355X pressure projection for Shiokaze.
+ *
+ * One call of shkz_project() is one call of the reference's
+ *   macproject3_interface::project(dt, velocity, solid, fluid, surface_tension, rigidbodies)
+ *   (include/shiokaze/projection/macproject3_interface.h:67-72, implemented by
+ *    macpressuresolver3::project, src/projection/macpressuresolver3.cpp:50-272)
+ * on dense arrays.  Everything crosses the boundary as plain pointers and sizes.
+ *
+ * Array layouts (all dense, z index fastest):
+ *   cell  (w, h, d)            index (i*h + j)*d + k          fluid level set, pressure
+ *   u     (w+1, h, d)          index (i*h + j)*d + k
+ *   v     (w, h+1, d)          index (i*(h+1) + j)*d + k
+ *   w     (w, h, d+1)          index (i*h + j)*(d+1) + k
+ *   nodal (w+1, h+1, d+1)      index (i*(h+1) + j)*(d+1) + k  solid level set
+ * Face (i,j,k) of dimension dim separates cells (i,j,k) and (i,j,k) - e_dim
+ * (include/shiokaze/math/vec.h:353-381).
+ *
+ * Errors: every entry point returns an int status (SHKZ_OK = 0, negative = error) and
+ * records a message readable with shkz_last_error() on the calling thread.  A solve that
+ * reaches max_iterations is not an error (as in the reference, pcg_solver.h:292-294): the
+ * result carries converged = 0.
+ */
+#ifndef SHKZ_MI355X_H
+#define SHKZ_MI355X_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SHKZ_ABI_VERSION 1
+
+/* ---------------------------------------------------------------- enums */
+enum shkz_status {
+    SHKZ_OK = 0,
+    SHKZ_ERR_INVALID = -1,   /* bad argument or shape */
+    SHKZ_ERR_DEVICE = -2,    /* HIP runtime error */
+    SHKZ_ERR_NOMEM = -3,     /* device allocation failed */
+    SHKZ_ERR_BUDGET = -4,    /* request exceeds the context's HBM budget */
+    SHKZ_ERR_NOT_FOUND = -5, /* unknown scene or ticket */
+    SHKZ_ERR_NUMERIC = -6    /* NaN/Inf met in the solve (reference: pcg_solver.h:267-271) */
+};
+
+enum shkz_dtype { SHKZ_F32 = 0, SHKZ_F64 = 1 };
+
+/* Linear solver behind the projection. */
+enum shkz_solver {
+    SHKZ_SOLVER_MGPCG = 0, /* multigrid-preconditioned CG (V-cycle preconditioner) */
+    SHKZ_SOLVER_CG = 1     /* unpreconditioned CG: the iteration the reference "pcg" module
+                              actually runs (its MIC(0) output is discarded,
+                              local/include/pcgsolver/pcg_solver.h:379-384) */
+};
+
+enum shkz_precision {
+    SHKZ_PREC_MIXED = 0, /* FP32 V-cycle / inner CG inside an FP64 defect-correction loop */
+    SHKZ_PREC_FP64 = 1   /* every vector and operator in FP64 */
+};
+
+enum shkz_norm {
+    SHKZ_NORM_INF = 0, /* reference criterion: ||r||_inf <= residual * ||r0||_inf (pcg_solver.h:252-283) */
+    SHKZ_NORM_L2 = 1   /* ||r||_2 <= residual * ||r0||_2 */
+};
+
+/* ---------------------------------------------------------------- grid */
+typedef struct shkz_grid {
+    int32_t w, h, d;          /* cells per axis */
+    double dx;                /* 0 selects min(1/w,1/h,1/d) (shape3::dx, shape.h:755-759) */
+    double dt;                /* time step */
+    const void *fluid;        /* cell level set (w*h*d), negative inside liquid */
+    int32_t fluid_dtype;      /* SHKZ_F32 or SHKZ_F64 */
+    int32_t solid_dtype;
+    const void *solid;        /* nodal solid level set ((w+1)(h+1)(d+1)), negative inside the
+                                 solid, or NULL for walls only (no-solid branch,
+                                 macutility3.cpp:149-164) */
+    float *u, *v, *w_vel;     /* face velocities, updated in place */
+    const uint8_t *active_u;  /* optional per-face activity (NULL = every face active); inactive */
+    const uint8_t *active_v;  /* faces are neither read as nonzero by the caller's contract nor */
+    const uint8_t *active_w;  /* written, matching macarray3::parallel_actives */
+    float *pressure;          /* optional out: m_pressure (cells, float; 0 off the unknowns) */
+    double *pressure64;       /* optional out: the FP64 solution (cells) */
+    uint64_t scene_id;        /* 0 = stateless; otherwise keys warm-start pressure and the
+                                 volume-correction integrator kept on the device across calls */
+} shkz_grid;
+
+/* ---------------------------------------------------------------- parameters */
+typedef struct shkz_params {
+    double residual;              /* relative tolerance; reference key Residual (pcg.cpp:76, 1e-4) */
+    int32_t norm;                 /* shkz_norm */
+    int32_t max_iterations;       /* MaxIterations (pcg.cpp:77, 30000); counts CG iterations */
+    int32_t solver;               /* shkz_solver */
+    int32_t precision;            /* shkz_precision */
+    double eps_fluid;             /* EpsFluid (macutility3.cpp:409, 1e-2) */
+    double eps_solid;             /* EpsSolid (macutility3.cpp:410, 1e-2) */
+    int32_t second_order_fluid;   /* SecondOrderAccurateFluid (macpressuresolver3.cpp:275, 1) */
+    int32_t second_order_solid;   /* SecondOrderAccurateSolid (:276, 1) */
+    double surface_tension;       /* kappa; 0 = off (macpressuresolver3.cpp:85-115) */
+    double gain;                  /* Gain (:277, 1.0): volume-correction PI gain */
+    double current_volume;        /* set_target_volume() arguments (:45-48); target 0 = off */
+    double target_volume;
+    int32_t warm_start;           /* WarmStart (:278, 0): start from the scene's previous pressure */
+    /* multigrid controls (0 = library default / autotuned) */
+    int32_t mg_pre_sweeps;
+    int32_t mg_post_sweeps;
+    int32_t mg_coarse_sweeps;
+    double mg_omega;
+    double inner_residual;        /* mixed precision: relative reduction per inner FP32 solve */
+    int32_t reserved[8];
+} shkz_params;
+
+/* ---------------------------------------------------------------- result */
+typedef struct shkz_result {
+    int32_t status;               /* shkz_status of the call */
+    int32_t converged;            /* 1 when the criterion was met */
+    int32_t iterations;           /* CG iterations (all inner iterations in mixed mode) */
+    int32_t outer_iterations;     /* defect-correction passes (mixed mode), else 1 */
+    double reresid;               /* final relative residual in the requested norm, FP64 */
+    double reresid_inf;           /* ||b - A x||_inf / ||r0||_inf */
+    double reresid_l2;            /* ||b - A x||_2 / ||r0||_2 */
+    int64_t n_unknowns;           /* liquid unknowns (the reference's matrix size) */
+    int64_t n_bricks;             /* active 8^3 bricks on the finest level */
+    int32_t n_levels;             /* multigrid levels */
+    int32_t pad0;
+    double rhs_correct;           /* volume-correction term added to every row */
+    /* stage times in milliseconds, named after the reference's scoped_timer records
+       (macpressuresolver3.cpp:82,201,237,269,271) */
+    double t_solid_fluid_fractions;
+    double t_build_highres_linsystem;
+    double t_linsolve;
+    double t_update_velocity;
+    double t_projection;
+} shkz_result;
+
+/* ---------------------------------------------------------------- context */
+typedef struct shkz_ctx shkz_ctx;
+
+/* Library ABI version (SHKZ_ABI_VERSION). */
+int shkz_abi_version(void);
+
+/* Message of the last failed call on this thread ("" if none). */
+const char *shkz_last_error(void);
+
+/* Defaults equal the reference's configuration defaults (see the field comments). */
+void shkz_default_params(shkz_params *p);
+
+/* Creates a context on HIP device `device`.  hbm_budget_bytes bounds the device memory the
+ * context may hold (0 = 90% of the device).  Replaces module::alloc_module + the driver's
+ * initialize() (src/core/module.cpp:103-175, macproject3_interface.h:90-98). */
+int shkz_create(int device, uint64_t hbm_budget_bytes, shkz_ctx **out);
+void shkz_destroy(shkz_ctx *ctx);
+
+/* macproject3_interface::project on host arrays (macpressuresolver3.cpp:50-272).  Copies the
+ * inputs to the device, projects, copies velocities (and pressure if requested) back. */
+int shkz_project(shkz_ctx *ctx, const shkz_grid *grid, const shkz_params *params, shkz_result *result);
+
+/* The same projection on arrays already resident in device memory (every pointer in `grid`
+ * is a device pointer).  Work is issued on `stream` (a hipStream_t, NULL = the null stream);
+ * the call returns once the result is known. */
+int shkz_project_device(shkz_ctx *ctx, const shkz_grid *grid, const shkz_params *params,
+                        shkz_result *result, void *stream);
+
+/* Runs fractions, labelling and assembly only and writes the assembled FP64 system on dense
+ * cell arrays (host): diag (0 off the unknowns), the couplings of each cell's low x/y/z face
+ * (A[c, c-e] = -g) and the right-hand side.  The counterpart of inspecting the RCMatrix the
+ * reference assembles (macpressuresolver3.cpp:159-199, RCMatrix_utility::report at :219). */
+int shkz_export_system(shkz_ctx *ctx, const shkz_grid *grid, const shkz_params *params, double *diag, double *gx,
+                       double *gy, double *gz, double *rhs);
+
+/* Drops a scene's resident state (warm-start pressure, volume integrator). */
+int shkz_release_scene(shkz_ctx *ctx, uint64_t scene_id);
+
+/* Device bytes currently held by the context. */
+uint64_t shkz_device_bytes(const shkz_ctx *ctx);
+
+/* ------------------------------------------------------- kernel timing (measurement) */
+typedef struct shkz_kernel_stat {
+    char name[48];
+    int64_t launches;
+    double total_ms;          /* summed event-measured durations */
+    double bytes;             /* summed algorithmic bytes of those launches */
+} shkz_kernel_stat;
+
+/* When enabled, each kernel launch is bracketed by HIP events on its stream and its
+ * algorithmic byte count is recorded; shkz_get_kernel_stats() resolves the events. */
+int shkz_set_kernel_timing(shkz_ctx *ctx, int enable);
+int shkz_get_kernel_stats(shkz_ctx *ctx, shkz_kernel_stat *out, int max_entries, int *n_out);
+int shkz_reset_kernel_stats(shkz_ctx *ctx);
+
+/* ------------------------------------------------------- synthetic workloads
+ * Scene synthesis for benchmarks and tests (no counterpart in the reference, whose scenes
+ * are dylibs sampled by macutility3::assign_initial_variables, macutility3.cpp:308-400; the
+ * sampling positions follow it: level sets at cell centres (i+1/2)dx, solids at nodes i*dx,
+ * velocities at face centres).  RNG: std::mt19937_64 with a Box-Muller normal. */
+enum shkz_scene_kind {
+    SHKZ_SCENE_DROP = 0,        /* sphere r=0.15 at (0.5,0.7,0.5) over a pool of depth 0.3, walls */
+    SHKZ_SCENE_DROP_SOLIDS = 1, /* the same plus 3 solid spheres r=0.1, centres U[0.2,0.8]^3 */
+    SHKZ_SCENE_SPHERES_BOX = 2, /* 8 spheres r~U[0.05,0.15] over a pool of depth 0.25 and one
+                                   random solid box of volume 0.1 */
+    SHKZ_SCENE_RANDOM_DROP = 3  /* seeded sphere-drop (pool depth and sphere drawn from seed) */
+};
+
+typedef struct shkz_scene_desc {
+    int32_t kind;           /* shkz_scene_kind */
+    int32_t w, h, d;
+    uint64_t seed;
+    double dt;
+    double gravity;         /* added as gravity*dt to v (e.g. -9.8); 0 = none */
+    double shift[3];        /* rigid translation of the liquid level set (phi(x - shift)) */
+    int32_t fluid_dtype;    /* dtype written to fluid */
+    int32_t velocity_mode;  /* 0 = i.i.d. N(0,1), 1 = zero */
+    uint64_t velocity_seed; /* 0 = use seed */
+} shkz_scene_desc;
+
+/* Fills host arrays for a scene: fluid (cells, fluid_dtype), solid (nodal float, may be NULL;
+ * *has_solid tells whether the scene has a solid), u, v, w (faces, may be NULL). */
+int shkz_scene_fill(const shkz_scene_desc *desc, void *fluid, float *solid, int32_t *has_solid,
+                    float *u, float *v, float *w);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,233 @@
+"""ctypes binding of the CPU oracle (oracle/liboracle.so) -- TEST INFRASTRUCTURE ONLY.
+
+Only tests, the smoke check and the benchmark's CPU-baseline leg use this module; the product
+library never imports it.  See shkz_oracle.h for what each function restates.
+
+Also binds oracle/_ref/libshkz_ref.so when it exists: a few header-only pieces of the reference
+compiled from the reference's own headers (utility::fraction/get_area/box and Bridson's
+SparseMatrix/FixedSparseMatrix multiply), used to pin the restatement.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liboracle.so")
+REF_PATH = os.path.join(_HERE, "_ref", "libshkz_ref.so")
+
+MIC_FAITHFUL, MIC_APPLIED, MIC_NONE = 0, 1, 2
+
+_vp = ctypes.c_void_p
+_i64p = ctypes.POINTER(ctypes.c_int64)
+_dp = ctypes.POINTER(ctypes.c_double)
+
+
+class OGrid(ctypes.Structure):
+    _fields_ = [("w", ctypes.c_int), ("h", ctypes.c_int), ("d", ctypes.c_int), ("dx", ctypes.c_double),
+                ("dt", ctypes.c_double), ("fluid", _vp), ("fluid_f64", ctypes.c_int), ("solid", _vp),
+                ("solid_f64", ctypes.c_int), ("vel", _vp * 3), ("active", _vp * 3)]
+
+
+class OParams(ctypes.Structure):
+    _fields_ = [("residual", ctypes.c_double), ("max_iterations", ctypes.c_int), ("mic_mode", ctypes.c_int),
+                ("mic_omega", ctypes.c_double), ("mic_min_diag_ratio", ctypes.c_double),
+                ("eps_fluid", ctypes.c_double), ("eps_solid", ctypes.c_double),
+                ("second_order_fluid", ctypes.c_int), ("second_order_solid", ctypes.c_int),
+                ("surface_tension", ctypes.c_double), ("rhs_correct", ctypes.c_double),
+                ("warm_start", _vp), ("threads", ctypes.c_int)]
+
+
+class OResult(ctypes.Structure):
+    _fields_ = [("iterations", ctypes.c_int), ("reresid", ctypes.c_double), ("converged", ctypes.c_int),
+                ("n_unknowns", ctypes.c_int64), ("t_fractions", ctypes.c_double), ("t_build", ctypes.c_double),
+                ("t_linsolve", ctypes.c_double), ("t_update", ctypes.c_double), ("t_projection", ctypes.c_double)]
+
+
+_lib = None
+_ref = None
+
+
+def build():
+    subprocess.check_call(["make", "-s", "-C", _HERE, "all"])
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = ctypes.CDLL(LIB_PATH)
+        L.oracle_default_params.argtypes = [ctypes.POINTER(OParams)]
+        L.oracle_fraction.argtypes = [ctypes.c_double, ctypes.c_double]
+        L.oracle_fraction.restype = ctypes.c_double
+        L.oracle_get_area.argtypes = [_dp]
+        L.oracle_get_area.restype = ctypes.c_double
+        L.oracle_box_sdf.argtypes = [_dp, _dp, _dp]
+        L.oracle_box_sdf.restype = ctypes.c_double
+        L.oracle_face_fractions.argtypes = [ctypes.POINTER(OGrid), ctypes.POINTER(OParams), _vp * 3, _vp * 3]
+        L.oracle_surface_tension.argtypes = [ctypes.POINTER(OGrid), ctypes.c_double, _vp * 3]
+        L.oracle_label.argtypes = [ctypes.POINTER(OGrid), _vp * 3, _vp * 3, _vp]
+        L.oracle_label.restype = ctypes.c_int64
+        L.oracle_assemble_count.argtypes = [ctypes.POINTER(OGrid), _vp, _vp * 3, _vp * 3]
+        L.oracle_assemble_count.restype = ctypes.c_int64
+        L.oracle_assemble.argtypes = [ctypes.POINTER(OGrid), _vp, _vp * 3, _vp * 3, _vp, _vp, _vp, _vp]
+        L.oracle_volume_correction.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp]
+        L.oracle_volume_correction.restype = ctypes.c_double
+        L.oracle_spmv.argtypes = [ctypes.c_int64, _vp, _vp, _vp, _vp, _vp]
+        L.oracle_pcg.argtypes = [ctypes.c_int64, _vp, _vp, _vp, _vp, _vp, ctypes.c_double, ctypes.c_int, ctypes.c_int,
+                                 ctypes.c_double, ctypes.c_double, ctypes.POINTER(ctypes.c_int), _dp]
+        L.oracle_pcg.restype = ctypes.c_int
+        L.oracle_velocity_update.argtypes = [ctypes.POINTER(OGrid), _vp, _vp * 3, _vp * 3]
+        L.oracle_divergence.argtypes = [ctypes.POINTER(OGrid), _vp, _vp * 3, _vp]
+        L.oracle_divergence.restype = ctypes.c_double
+        L.oracle_project.argtypes = [ctypes.POINTER(OGrid), ctypes.POINTER(OParams), _vp, _vp, ctypes.POINTER(OResult)]
+        L.oracle_project.restype = ctypes.c_int
+        _lib = L
+    return _lib
+
+
+def ref():
+    """The reference-header driver, or None when it was not built (no reference tree)."""
+    global _ref
+    if _ref is None and os.path.exists(REF_PATH):
+        R = ctypes.CDLL(REF_PATH)
+        R.ref_fraction.argtypes = [ctypes.c_double, ctypes.c_double]
+        R.ref_fraction.restype = ctypes.c_double
+        R.ref_get_area.argtypes = [_dp]
+        R.ref_get_area.restype = ctypes.c_double
+        R.ref_box.argtypes = [_dp, _dp, _dp]
+        R.ref_box.restype = ctypes.c_double
+        R.ref_spmv.argtypes = [ctypes.c_int64, ctypes.c_int64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]
+        R.ref_spmv.restype = ctypes.c_int64
+        _ref = R
+    return _ref
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data
+
+
+def default_params(**kw) -> OParams:
+    p = OParams()
+    lib().oracle_default_params(ctypes.byref(p))
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+def make_grid(scene, dx=None, active=None) -> tuple[OGrid, list]:
+    """OGrid over a shiokaze_amd.Scene-like object (fluid, solid, u, v, wv, w, h, d, dt)."""
+    keep = [scene.fluid, scene.solid, scene.u, scene.v, scene.wv]
+    g = OGrid(w=scene.w, h=scene.h, d=scene.d,
+              dx=dx if dx is not None else min(1.0 / scene.w, 1.0 / scene.h, 1.0 / scene.d), dt=scene.dt,
+              fluid=_p(scene.fluid), fluid_f64=int(scene.fluid.dtype == np.float64), solid=_p(scene.solid),
+              solid_f64=int(scene.solid is not None and scene.solid.dtype == np.float64))
+    g.vel[0], g.vel[1], g.vel[2] = _p(scene.u), _p(scene.v), _p(scene.wv)
+    if active is not None:
+        for a in range(3):
+            g.active[a] = _p(active[a])
+        keep += list(active)
+    return g, keep
+
+
+def face_shapes(w, h, d):
+    return [(w + 1, h, d), (w, h + 1, d), (w, h, d + 1)]
+
+
+def fractions(scene, params=None):
+    """Dense (area[3], rho[3]) float face arrays."""
+    g, keep = make_grid(scene)
+    prm = params or default_params()
+    area = [np.zeros(s, np.float32) for s in face_shapes(scene.w, scene.h, scene.d)]
+    rho = [np.zeros(s, np.float32) for s in face_shapes(scene.w, scene.h, scene.d)]
+    A = (_vp * 3)(*[a.ctypes.data for a in area])
+    R = (_vp * 3)(*[a.ctypes.data for a in rho])
+    lib().oracle_face_fractions(ctypes.byref(g), ctypes.byref(prm), A, R)
+    return area, rho
+
+
+def assemble(scene, params=None):
+    """Labelling + CSR assembly.  Returns dict(index_map, rowptr, col, val, b, area, rho)."""
+    prm = params or default_params()
+    area, rho = fractions(scene, prm)
+    g, keep = make_grid(scene)
+    A = (_vp * 3)(*[a.ctypes.data for a in area])
+    R = (_vp * 3)(*[a.ctypes.data for a in rho])
+    if prm.surface_tension:
+        lib().oracle_surface_tension(ctypes.byref(g), prm.surface_tension, R)
+    index_map = np.empty((scene.w, scene.h, scene.d), np.int64)
+    n = lib().oracle_label(ctypes.byref(g), A, R, index_map.ctypes.data)
+    nnz = lib().oracle_assemble_count(ctypes.byref(g), index_map.ctypes.data, A, R)
+    rowptr = np.empty(n + 1, np.int64)
+    col = np.empty(max(nnz, 1), np.int64)
+    val = np.empty(max(nnz, 1), np.float64)
+    b = np.empty(max(n, 1), np.float64)
+    lib().oracle_assemble(ctypes.byref(g), index_map.ctypes.data, A, R, rowptr.ctypes.data, col.ctypes.data,
+                          val.ctypes.data, b.ctypes.data)
+    return dict(index_map=index_map, rowptr=rowptr, col=col[:nnz], val=val[:nnz], b=b[:n], area=area, rho=rho, n=n)
+
+
+def spmv(sys, x):
+    y = np.empty(sys["n"], np.float64)
+    lib().oracle_spmv(sys["n"], sys["rowptr"].ctypes.data, sys["col"].ctypes.data, sys["val"].ctypes.data,
+                      np.ascontiguousarray(x, np.float64).ctypes.data, y.ctypes.data)
+    return y
+
+
+def pcg(sys, b=None, residual=1e-8, max_iterations=30000, mic_mode=MIC_NONE):
+    n = sys["n"]
+    b = sys["b"] if b is None else np.ascontiguousarray(b, np.float64)
+    x = np.zeros(max(n, 1), np.float64)
+    it = ctypes.c_int(0)
+    rr = ctypes.c_double(0)
+    ok = lib().oracle_pcg(n, sys["rowptr"].ctypes.data, sys["col"].ctypes.data, sys["val"].ctypes.data, b.ctypes.data,
+                          x.ctypes.data, residual, max_iterations, mic_mode, 0.97, 0.25, ctypes.byref(it),
+                          ctypes.byref(rr))
+    return x[:n], it.value, rr.value, bool(ok)
+
+
+def project(scene, params=None, pressure64=True):
+    """Whole projection on a copy-free scene (velocities updated in place)."""
+    prm = params or default_params()
+    g, keep = make_grid(scene)
+    P = np.zeros((scene.w, scene.h, scene.d), np.float32)
+    P64 = np.zeros((scene.w, scene.h, scene.d), np.float64) if pressure64 else None
+    r = OResult()
+    lib().oracle_project(ctypes.byref(g), ctypes.byref(prm), P.ctypes.data, _p(P64), ctypes.byref(r))
+    return r, P, P64
+
+
+def velocity_update(scene, P, area, rho):
+    g, keep = make_grid(scene)
+    A = (_vp * 3)(*[a.ctypes.data for a in area])
+    R = (_vp * 3)(*[a.ctypes.data for a in rho])
+    lib().oracle_velocity_update(ctypes.byref(g), np.ascontiguousarray(P, np.float32).ctypes.data, A, R)
+
+
+def divergence(scene, index_map, area):
+    g, keep = make_grid(scene)
+    A = (_vp * 3)(*[a.ctypes.data for a in area])
+    div = np.zeros((scene.w, scene.h, scene.d), np.float64)
+    mx = lib().oracle_divergence(ctypes.byref(g), index_map.ctypes.data, A, div.ctypes.data)
+    return mx, div
+
+
+def scatter(sys, x, shape):
+    """Cell-indexed array from an unknown-indexed vector (0 off the unknowns)."""
+    out = np.zeros(shape, np.float64)
+    m = sys["index_map"]
+    mask = m >= 0
+    out[mask] = x[m[mask]]
+    return out
+
+
+def gather(sys, cells):
+    m = sys["index_map"]
+    x = np.zeros(sys["n"], np.float64)
+    mask = m >= 0
+    x[m[mask]] = cells[mask]
+    return x

@@ -1,0 +1,122 @@
+/*
+ * shkz_oracle.h -- CPU restatement of Shiokaze's MAC pressure projection.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP path:
+ * tests, the smoke check and the benchmark's CPU-baseline leg may call it;
+ * the product library never links it and never falls back to it.
+ *
+ * It restates, in plain C, the reference path
+ *   macpressuresolver3::project      src/projection/macpressuresolver3.cpp:50-272
+ *   macutility3 face fractions       src/utility/macutility3.cpp:94-194
+ *   utility::fraction / get_area     include/shiokaze/utility/utility.h:162-214
+ *   pcg_solver / PCGSolver<double>   src/linsolver/pcg.cpp:45-80,
+ *                                    local/include/pcgsolver/pcg_solver.h:89-295,379-384
+ * on dense arrays (z fastest):
+ *   cell  (w,h,d)        idx = (i*h + j)*d + k
+ *   u     (w+1,h,d)      v (w,h+1,d)      w (w,h,d+1)
+ *   nodal (w+1,h+1,d+1)  (solid level set)
+ */
+#ifndef SHKZ_ORACLE_H
+#define SHKZ_ORACLE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+    int w, h, d;            /* cells */
+    double dx;              /* min(1/w,1/h,1/d) as shape3::dx (shape.h:755-759) */
+    double dt;
+    const void *fluid;      /* cell level set, float or double */
+    int fluid_f64;
+    const void *solid;      /* nodal level set or NULL (walls only) */
+    int solid_f64;
+    float *vel[3];          /* face velocities (in/out) */
+    const uint8_t *active[3]; /* optional face activity (NULL = all active) */
+} oracle_grid;
+
+enum { ORACLE_MIC_FAITHFUL = 0, ORACLE_MIC_APPLIED = 1, ORACLE_MIC_NONE = 2 };
+
+typedef struct {
+    double residual;            /* pcg.cpp:76 default 1e-4 */
+    int max_iterations;         /* pcg.cpp:77 default 30000 */
+    int mic_mode;               /* ORACLE_MIC_* */
+    double mic_omega;           /* 0.97 (pcg_solver.h:89) */
+    double mic_min_diag_ratio;  /* 0.25 */
+    double eps_fluid, eps_solid;/* macutility3.cpp:408-422 defaults 1e-2 */
+    int second_order_fluid;     /* macpressuresolver3.cpp:302-303 default 1 */
+    int second_order_solid;
+    double surface_tension;     /* 0 = off */
+    double rhs_correct;         /* volume-correction term added to every row (0 = off) */
+    const double *warm_start;   /* cell-indexed previous pressure or NULL */
+    int threads;                /* threaded stages (fractions/assembly/update); 0 = all */
+} oracle_params;
+
+typedef struct {
+    int iterations;
+    double reresid;             /* ||r||_inf / ||r0||_inf as pcg_solver.h:283,293 */
+    int converged;
+    int64_t n_unknowns;
+    double t_fractions, t_build, t_linsolve, t_update, t_projection; /* seconds */
+} oracle_result;
+
+void oracle_default_params(oracle_params *p);
+
+/* utility.h:162-170 */
+double oracle_fraction(double phi0, double phi1);
+/* utility.h:179-214; q is isosurf[2][2] row-major: q[0]=[0][0] q[1]=[0][1] q[2]=[1][0] q[3]=[1][1] */
+double oracle_get_area(const double q[4]);
+/* utility.h:113-120 */
+double oracle_box_sdf(const double p[3], const double p0[3], const double p1[3]);
+
+/* macutility3.cpp:94-194 plus the first-order overrides of macpressuresolver3.cpp:70-80.
+ * area[dim], rho[dim] are dense face arrays (float). */
+void oracle_face_fractions(const oracle_grid *g, const oracle_params *p, float *area[3], float *rho[3]);
+
+/* macpressuresolver3.cpp:85-115: adds the surface-tension impulse to g->vel. */
+void oracle_surface_tension(const oracle_grid *g, double kappa, float *rho[3]);
+
+/* macpressuresolver3.cpp:121-156: index_map[c] = unknown number or -1. Returns N. */
+int64_t oracle_label(const oracle_grid *g, float *area[3], float *rho[3], int64_t *index_map);
+
+/* macpressuresolver3.cpp:159-199: CSR rows sorted by column (as SparseMatrix keeps them,
+ * sparse_matrix.h:75-90).  rowptr has N+1 entries; col/val have rowptr[N] entries. */
+int64_t oracle_assemble_count(const oracle_grid *g, const int64_t *index_map, float *area[3], float *rho[3]);
+void oracle_assemble(const oracle_grid *g, const int64_t *index_map, float *area[3], float *rho[3],
+                     int64_t *rowptr, int64_t *col, double *val, double *b);
+
+/* macpressuresolver3.cpp:207-211 PI controller; returns the term added to every rhs row and
+ * updates *y_prev. */
+double oracle_volume_correction(double current_volume, double target_volume, double dt,
+                                double gain, double *y_prev);
+
+/* sparse_matrix.h:265-275: y = A x */
+void oracle_spmv(int64_t n, const int64_t *rowptr, const int64_t *col, const double *val,
+                 const double *x, double *y);
+
+/* PCGSolver<double>::solve (pcg_solver.h:246-295) with the MIC(0) factor of :89-189 and the
+ * triangular solves of :196-223; mic_mode selects the reference behaviour (factor and solves
+ * computed, result discarded, :379-384), the applied preconditioner, or neither. */
+int oracle_pcg(int64_t n, const int64_t *rowptr, const int64_t *col, const double *val,
+               const double *b, double *x, double residual, int max_iterations, int mic_mode,
+               double omega, double min_diag_ratio, int *iterations, double *reresid);
+
+/* macpressuresolver3.cpp:252-268 velocity update with pressure stored in float. */
+void oracle_velocity_update(const oracle_grid *g, const float *pressure, float *area[3], float *rho[3]);
+
+/* Divergence sum_f sgn_f area_f u_f / dx over the faces of every unknown (negated rhs formula of
+ * macpressuresolver3.cpp:193).  div may be NULL; returns max |div|. */
+double oracle_divergence(const oracle_grid *g, const int64_t *index_map, float *area[3], double *div);
+
+/* The whole macpressuresolver3::project (macpressuresolver3.cpp:50-272).  pressure (float, cells)
+ * receives m_pressure; pressure64 (optional, cells) receives the FP64 solution. */
+int oracle_project(const oracle_grid *g, const oracle_params *p, float *pressure,
+                   double *pressure64, oracle_result *res);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,314 @@
+"""shiokaze_amd -- MI355X pressure projection for Shiokaze, Python binding of the C ABI.
+
+The product is ``shiokaze_amd/lib/libshkz_mi355x.so`` (C ABI declared in
+``include/shkz_mi355x.h``); this module only binds it with ctypes for tests and the
+benchmark.  It performs no computation of its own and has no fallback: if the library is
+missing, importing works but every call raises ``RuntimeError``.
+
+``Context.project`` mirrors ``macproject3_interface::project``
+(include/shiokaze/projection/macproject3_interface.h:67-72) on dense numpy arrays.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libshkz_mi355x.so")
+
+F32, F64 = 0, 1
+SOLVER_MGPCG, SOLVER_CG = 0, 1
+PREC_MIXED, PREC_FP64 = 0, 1
+NORM_INF, NORM_L2 = 0, 1
+SCENE_DROP, SCENE_DROP_SOLIDS, SCENE_SPHERES_BOX, SCENE_RANDOM_DROP = 0, 1, 2, 3
+
+STATUS = {0: "OK", -1: "INVALID", -2: "DEVICE", -3: "NOMEM", -4: "BUDGET", -5: "NOT_FOUND", -6: "NUMERIC"}
+
+_vp = ctypes.c_void_p
+
+
+class Grid(ctypes.Structure):
+    _fields_ = [
+        ("w", ctypes.c_int32), ("h", ctypes.c_int32), ("d", ctypes.c_int32),
+        ("dx", ctypes.c_double), ("dt", ctypes.c_double),
+        ("fluid", _vp), ("fluid_dtype", ctypes.c_int32), ("solid_dtype", ctypes.c_int32),
+        ("solid", _vp),
+        ("u", _vp), ("v", _vp), ("w_vel", _vp),
+        ("active_u", _vp), ("active_v", _vp), ("active_w", _vp),
+        ("pressure", _vp), ("pressure64", _vp),
+        ("scene_id", ctypes.c_uint64),
+    ]
+
+
+class Params(ctypes.Structure):
+    _fields_ = [
+        ("residual", ctypes.c_double), ("norm", ctypes.c_int32), ("max_iterations", ctypes.c_int32),
+        ("solver", ctypes.c_int32), ("precision", ctypes.c_int32),
+        ("eps_fluid", ctypes.c_double), ("eps_solid", ctypes.c_double),
+        ("second_order_fluid", ctypes.c_int32), ("second_order_solid", ctypes.c_int32),
+        ("surface_tension", ctypes.c_double), ("gain", ctypes.c_double),
+        ("current_volume", ctypes.c_double), ("target_volume", ctypes.c_double),
+        ("warm_start", ctypes.c_int32),
+        ("mg_pre_sweeps", ctypes.c_int32), ("mg_post_sweeps", ctypes.c_int32), ("mg_coarse_sweeps", ctypes.c_int32),
+        ("mg_omega", ctypes.c_double), ("inner_residual", ctypes.c_double),
+        ("reserved", ctypes.c_int32 * 8),
+    ]
+
+
+class Result(ctypes.Structure):
+    _fields_ = [
+        ("status", ctypes.c_int32), ("converged", ctypes.c_int32), ("iterations", ctypes.c_int32),
+        ("outer_iterations", ctypes.c_int32),
+        ("reresid", ctypes.c_double), ("reresid_inf", ctypes.c_double), ("reresid_l2", ctypes.c_double),
+        ("n_unknowns", ctypes.c_int64), ("n_bricks", ctypes.c_int64), ("n_levels", ctypes.c_int32),
+        ("pad0", ctypes.c_int32), ("rhs_correct", ctypes.c_double),
+        ("t_solid_fluid_fractions", ctypes.c_double), ("t_build_highres_linsystem", ctypes.c_double),
+        ("t_linsolve", ctypes.c_double), ("t_update_velocity", ctypes.c_double), ("t_projection", ctypes.c_double),
+    ]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_ if name != "pad0"}
+
+
+class KernelStat(ctypes.Structure):
+    _fields_ = [("name", ctypes.c_char * 48), ("launches", ctypes.c_int64), ("total_ms", ctypes.c_double),
+                ("bytes", ctypes.c_double)]
+
+
+class SceneDesc(ctypes.Structure):
+    _fields_ = [
+        ("kind", ctypes.c_int32), ("w", ctypes.c_int32), ("h", ctypes.c_int32), ("d", ctypes.c_int32),
+        ("seed", ctypes.c_uint64), ("dt", ctypes.c_double), ("gravity", ctypes.c_double),
+        ("shift", ctypes.c_double * 3), ("fluid_dtype", ctypes.c_int32), ("velocity_mode", ctypes.c_int32),
+        ("velocity_seed", ctypes.c_uint64),
+    ]
+
+
+# Every symbol declared in include/shkz_mi355x.h.
+EXPORTS = [
+    "shkz_abi_version", "shkz_last_error", "shkz_default_params", "shkz_create", "shkz_destroy",
+    "shkz_project", "shkz_project_device", "shkz_release_scene", "shkz_device_bytes",
+    "shkz_set_kernel_timing", "shkz_get_kernel_stats", "shkz_reset_kernel_stats", "shkz_scene_fill",
+    "shkz_export_system",
+]
+
+_lib = None
+
+
+def lib():
+    """Loads libshkz_mi355x.so (raises RuntimeError when it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
+                           "(make -C shiokaze_amd/csrc); there is no fallback implementation")
+    L = ctypes.CDLL(LIB_PATH)
+    L.shkz_abi_version.restype = ctypes.c_int
+    L.shkz_last_error.restype = ctypes.c_char_p
+    L.shkz_default_params.argtypes = [ctypes.POINTER(Params)]
+    L.shkz_create.argtypes = [ctypes.c_int, ctypes.c_uint64, ctypes.POINTER(_vp)]
+    L.shkz_destroy.argtypes = [_vp]
+    L.shkz_project.argtypes = [_vp, ctypes.POINTER(Grid), ctypes.POINTER(Params), ctypes.POINTER(Result)]
+    L.shkz_project_device.argtypes = [_vp, ctypes.POINTER(Grid), ctypes.POINTER(Params), ctypes.POINTER(Result), _vp]
+    L.shkz_release_scene.argtypes = [_vp, ctypes.c_uint64]
+    L.shkz_device_bytes.argtypes = [_vp]
+    L.shkz_device_bytes.restype = ctypes.c_uint64
+    L.shkz_set_kernel_timing.argtypes = [_vp, ctypes.c_int]
+    L.shkz_get_kernel_stats.argtypes = [_vp, ctypes.POINTER(KernelStat), ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
+    L.shkz_reset_kernel_stats.argtypes = [_vp]
+    L.shkz_scene_fill.argtypes = [ctypes.POINTER(SceneDesc), _vp, _vp, ctypes.POINTER(ctypes.c_int32), _vp, _vp, _vp]
+    L.shkz_export_system.argtypes = [_vp, ctypes.POINTER(Grid), ctypes.POINTER(Params), _vp, _vp, _vp, _vp, _vp]
+    _lib = L
+    return L
+
+
+def _ptr(a):
+    if a is None:
+        return None
+    if isinstance(a, np.ndarray):
+        assert a.flags["C_CONTIGUOUS"]
+        return a.ctypes.data
+    return int(a)  # device pointer (int) or torch tensor data_ptr()
+
+
+def last_error() -> str:
+    return lib().shkz_last_error().decode()
+
+
+class ShkzError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{STATUS.get(code, code)}: {msg}")
+        self.code = code
+
+
+def default_params(**kw) -> Params:
+    p = Params()
+    lib().shkz_default_params(ctypes.byref(p))
+    for k, v in kw.items():
+        if not hasattr(p, k):
+            raise KeyError(k)
+        setattr(p, k, v)
+    return p
+
+
+@dataclass
+class Scene:
+    """Dense host arrays of one projection request (layouts of include/shkz_mi355x.h)."""
+    w: int
+    h: int
+    d: int
+    dt: float
+    fluid: np.ndarray
+    solid: np.ndarray | None
+    u: np.ndarray
+    v: np.ndarray
+    wv: np.ndarray
+
+    @property
+    def dx(self) -> float:
+        return min(1.0 / self.w, 1.0 / self.h, 1.0 / self.d)
+
+    def copy(self) -> "Scene":
+        return Scene(self.w, self.h, self.d, self.dt, self.fluid.copy(),
+                     None if self.solid is None else self.solid.copy(), self.u.copy(), self.v.copy(), self.wv.copy())
+
+
+def make_scene(kind: int, w: int, h: int | None = None, d: int | None = None, seed: int = 0, dt: float = 1.0 / 120.0,
+               gravity: float = -9.8, fluid_dtype: int = F64, shift=(0.0, 0.0, 0.0), velocity_mode: int = 0,
+               velocity_seed: int = 0) -> Scene:
+    """Synthesises a seeded workload with shkz_scene_fill."""
+    h = w if h is None else h
+    d = w if d is None else d
+    desc = SceneDesc(kind=kind, w=w, h=h, d=d, seed=seed, dt=dt, gravity=gravity, fluid_dtype=fluid_dtype,
+                     velocity_mode=velocity_mode, velocity_seed=velocity_seed)
+    for a in range(3):
+        desc.shift[a] = shift[a]
+    fluid = np.empty((w, h, d), np.float64 if fluid_dtype == F64 else np.float32)
+    solid = np.empty((w + 1, h + 1, d + 1), np.float32)
+    u = np.empty((w + 1, h, d), np.float32)
+    v = np.empty((w, h + 1, d), np.float32)
+    wv = np.empty((w, h, d + 1), np.float32)
+    has = ctypes.c_int32(0)
+    rc = lib().shkz_scene_fill(ctypes.byref(desc), _ptr(fluid), _ptr(solid), ctypes.byref(has), _ptr(u), _ptr(v),
+                               _ptr(wv))
+    if rc != 0:
+        raise ShkzError(rc, "shkz_scene_fill failed")
+    return Scene(w, h, d, dt, fluid, solid if has.value else None, u, v, wv)
+
+
+class Context:
+    """One shkz_ctx on a HIP device (the module instance of macpressuresolver3)."""
+
+    def __init__(self, device: int = 0, hbm_budget_bytes: int = 0):
+        self._L = lib()
+        h = _vp()
+        rc = self._L.shkz_create(device, hbm_budget_bytes, ctypes.byref(h))
+        if rc != 0:
+            raise ShkzError(rc, last_error())
+        self.handle = h
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self._L.shkz_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # ------------------------------------------------------------------ projection
+    def project(self, scene: Scene, params: Params | None = None, pressure: bool = True, pressure64: bool = False,
+                scene_id: int = 0, active=None, check: bool = True):
+        """Projects scene's velocities in place (host arrays).  Returns (result, P32, P64)."""
+        g = self._grid(scene, scene_id, active)
+        P = np.zeros((scene.w, scene.h, scene.d), np.float32) if pressure else None
+        P64 = np.zeros((scene.w, scene.h, scene.d), np.float64) if pressure64 else None
+        g.pressure = _ptr(P)
+        g.pressure64 = _ptr(P64)
+        r = Result()
+        prm = params if params is not None else default_params()
+        rc = self._L.shkz_project(self.handle, ctypes.byref(g), ctypes.byref(prm), ctypes.byref(r))
+        if rc != 0 and check:
+            raise ShkzError(rc, last_error())
+        return r, P, P64
+
+    def project_device(self, w, h, d, dt, fluid, fluid_dtype, solid, solid_dtype, u, v, wv, params=None,
+                       pressure=None, pressure64=None, scene_id=0, stream=None, dx=0.0, check=True):
+        """Projection on device-resident buffers (pointers as ints or torch tensors)."""
+        def dp(x):
+            if x is None:
+                return None
+            return x.data_ptr() if hasattr(x, "data_ptr") else int(x)
+        g = Grid(w=w, h=h, d=d, dx=dx, dt=dt, fluid=dp(fluid), fluid_dtype=fluid_dtype, solid_dtype=solid_dtype,
+                 solid=dp(solid), u=dp(u), v=dp(v), w_vel=dp(wv), pressure=dp(pressure), pressure64=dp(pressure64),
+                 scene_id=scene_id)
+        r = Result()
+        prm = params if params is not None else default_params()
+        rc = self._L.shkz_project_device(self.handle, ctypes.byref(g), ctypes.byref(prm), ctypes.byref(r),
+                                         None if stream is None else int(stream))
+        if rc != 0 and check:
+            raise ShkzError(rc, last_error())
+        return r
+
+    def _grid(self, scene: Scene, scene_id: int, active) -> Grid:
+        for a in (scene.fluid, scene.u, scene.v, scene.wv):
+            assert a.flags["C_CONTIGUOUS"]
+        g = Grid(w=scene.w, h=scene.h, d=scene.d, dx=0.0, dt=scene.dt, fluid=_ptr(scene.fluid),
+                 fluid_dtype=F64 if scene.fluid.dtype == np.float64 else F32,
+                 solid_dtype=F64 if (scene.solid is not None and scene.solid.dtype == np.float64) else F32,
+                 solid=_ptr(scene.solid), u=_ptr(scene.u), v=_ptr(scene.v), w_vel=_ptr(scene.wv), scene_id=scene_id)
+        if active is not None:
+            g.active_u, g.active_v, g.active_w = (_ptr(a) for a in active)
+        return g
+
+    def export_system(self, scene: Scene, params: Params | None = None) -> dict:
+        """Assembled FP64 system on dense cell arrays: diag, gx, gy, gz (low-face couplings), rhs."""
+        g = self._grid(scene, 0, None)
+        shape = (scene.w, scene.h, scene.d)
+        out = {k: np.zeros(shape, np.float64) for k in ("diag", "gx", "gy", "gz", "rhs")}
+        prm = params if params is not None else default_params()
+        rc = self._L.shkz_export_system(self.handle, ctypes.byref(g), ctypes.byref(prm), *(out[k].ctypes.data for k in
+                                                                                         ("diag", "gx", "gy", "gz",
+                                                                                          "rhs")))
+        if rc != 0:
+            raise ShkzError(rc, last_error())
+        return out
+
+    def release_scene(self, scene_id: int):
+        rc = self._L.shkz_release_scene(self.handle, scene_id)
+        if rc != 0:
+            raise ShkzError(rc, last_error())
+
+    def device_bytes(self) -> int:
+        return int(self._L.shkz_device_bytes(self.handle))
+
+    # ------------------------------------------------------------------ kernel timers
+    def set_kernel_timing(self, on: bool):
+        self._L.shkz_set_kernel_timing(self.handle, 1 if on else 0)
+
+    def reset_kernel_stats(self):
+        self._L.shkz_reset_kernel_stats(self.handle)
+
+    def kernel_stats(self) -> dict:
+        n = ctypes.c_int(0)
+        buf = (KernelStat * 128)()
+        rc = self._L.shkz_get_kernel_stats(self.handle, buf, 128, ctypes.byref(n))
+        if rc != 0:
+            raise ShkzError(rc, last_error())
+        out = {}
+        for i in range(min(n.value, 128)):
+            s = buf[i]
+            out[s.name.decode()] = {"launches": int(s.launches), "total_ms": float(s.total_ms), "bytes": float(s.bytes)}
+        return out

@@ -1,0 +1,154 @@
+// shkz_geometry.hpp -- face fractions of the variational ghost-fluid operator, evaluated on the
+// device directly from the level sets (matrix-free: nothing per face is stored).
+//
+//   area  = open (non-solid) fraction of a face, macutility3::compute_area_fraction
+//           (src/utility/macutility3.cpp:94-165) with utility::get_area
+//           (include/shiokaze/utility/utility.h:179-214);
+//   rho   = liquid fraction of a face, macutility3::compute_fluid_fraction (:166-194) with
+//           utility::fraction (utility.h:162-170);
+//   both are rounded to float as the reference stores them (Real = float), and the
+//   first-order switches of macpressuresolver3.cpp:70-80 are applied.
+// Floating-point contraction is off here so that the values match the reference bit for bit.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace shkz {
+
+struct GridDesc {
+    int w, h, d;
+    int phi64, solid64;
+    double dx, dt;
+    const void *phi;     // cells
+    const void *solid;   // nodes or nullptr
+    const int *exist;    // device: [0] liquid level set has a negative value, [1] solid has one
+    double eps_fluid, eps_solid;
+    int so_fluid, so_solid;
+    float *vel[3];
+    const uint8_t *active[3];
+};
+
+__device__ __forceinline__ size_t cell_index(const GridDesc &G, int i, int j, int k) {
+    return ((size_t)i * G.h + j) * G.d + k;
+}
+__device__ __forceinline__ size_t face_index(const GridDesc &G, int dim, int i, int j, int k) {
+    const size_t H = G.h + (dim == 1), D = G.d + (dim == 2);
+    return ((size_t)i * H + j) * D + k;
+}
+__device__ __forceinline__ size_t node_index(const GridDesc &G, int i, int j, int k) {
+    return ((size_t)i * (G.h + 1) + j) * (G.d + 1) + k;
+}
+__device__ __forceinline__ double phi_at(const GridDesc &G, int i, int j, int k) {
+    const size_t c = cell_index(G, i, j, k);
+    return G.phi64 ? ((const double *)G.phi)[c] : (double)((const float *)G.phi)[c];
+}
+__device__ __forceinline__ double solid_at(const GridDesc &G, int i, int j, int k) {
+    const size_t n = node_index(G, i, j, k);
+    return G.solid64 ? ((const double *)G.solid)[n] : (double)((const float *)G.solid)[n];
+}
+__device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+// Liquid fraction between two level-set samples (utility.h:162-170).
+__device__ __forceinline__ double liquid_fraction(double phi0, double phi1) {
+#pragma clang fp contract(off)
+    if (phi0 * phi1 >= 0.0) return (phi0 < 0.0 || phi1 < 0.0) ? 1.0 : 0.0;
+    const double lo = phi0 < phi1 ? phi0 : phi1;
+    double gap = fabs(phi1 - phi0);
+    if (gap < 2.2250738585072014e-308) gap = 2.2250738585072014e-308;
+    return -lo / gap;
+}
+
+// Negative-region area of a unit square from corner samples v[0..3] at (0,0),(1,0),(1,1),(0,1):
+// polygon of negative corners and edge crossings, shoelace sum (utility.h:179-214).
+__device__ __forceinline__ double negative_area(const double v[4]) {
+#pragma clang fp contract(off)
+    const int cx[4] = {0, 1, 1, 0};
+    const int cy[4] = {0, 0, 1, 1};
+    double px[8], py[8];
+    int m = 0;
+#pragma unroll
+    for (int n = 0; n < 4; ++n) {
+        const int nn = (n + 1) & 3;
+        if (v[n] < 0.0) {
+            px[m] = cx[n];
+            py[m] = cy[n];
+            ++m;
+        }
+        if (v[n] * v[nn] < 0) {
+            const double y0 = v[n], y1 = v[nn];
+            if (y0 - y1) {
+                const double a = y0 / (y0 - y1);
+                px[m] = (1.0 - a) * cx[n] + a * cx[nn];
+                py[m] = (1.0 - a) * cy[n] + a * cy[nn];
+                ++m;
+            }
+        }
+    }
+    double sum = 0.0;
+    for (int a = 0; a < m; ++a) {
+        const int b = (a + 1) % m;
+        sum += px[a] * py[b] - py[a] * px[b];
+    }
+    return 0.5 * sum;
+}
+
+// Open fraction of face (dim; i,j,k), float as stored by the reference.
+__device__ __forceinline__ float face_area(const GridDesc &G, int dim, int i, int j, int k) {
+#pragma clang fp contract(off)
+    const int N[3] = {G.w, G.h, G.d};
+    const int pi[3] = {i, j, k};
+    const bool boundary = (pi[dim] == 0 || pi[dim] == N[dim]);
+    double a;
+    if (G.solid && G.exist[1]) {
+        if (boundary) {
+            a = 0.0;
+        } else {
+            // corner order of macutility3.cpp:123-138 mapped to (0,0),(1,0),(1,1),(0,1)
+            double v[4];
+            if (dim == 0) {
+                v[0] = solid_at(G, i, j, k);
+                v[1] = solid_at(G, i, j + 1, k);
+                v[2] = solid_at(G, i, j + 1, k + 1);
+                v[3] = solid_at(G, i, j, k + 1);
+            } else if (dim == 1) {
+                v[0] = solid_at(G, i, j, k);
+                v[1] = solid_at(G, i + 1, j, k);
+                v[2] = solid_at(G, i + 1, j, k + 1);
+                v[3] = solid_at(G, i, j, k + 1);
+            } else {
+                v[0] = solid_at(G, i, j, k);
+                v[1] = solid_at(G, i + 1, j, k);
+                v[2] = solid_at(G, i + 1, j + 1, k);
+                v[3] = solid_at(G, i, j + 1, k);
+            }
+            a = 1.0 - negative_area(v);
+        }
+        if (a && a < G.eps_solid) a = G.eps_solid;
+    } else {
+        a = boundary ? 0.0 : 1.0;
+    }
+    float af = (float)a;
+    if (!G.so_solid && af) af = 1.0f;
+    return af;
+}
+
+// Liquid fraction of face (dim; i,j,k) between cell (i,j,k) and (i,j,k)-e_dim, clamped into the
+// grid, float as stored by the reference.
+__device__ __forceinline__ float face_rho(const GridDesc &G, int dim, int i, int j, int k) {
+#pragma clang fp contract(off)
+    double r;
+    if (G.exist[0]) {
+        const int i0 = clampi(i, 0, G.w - 1), j0 = clampi(j, 0, G.h - 1), k0 = clampi(k, 0, G.d - 1);
+        const int i1 = clampi(i - (dim == 0), 0, G.w - 1), j1 = clampi(j - (dim == 1), 0, G.h - 1),
+                  k1 = clampi(k - (dim == 2), 0, G.d - 1);
+        r = liquid_fraction(phi_at(G, i0, j0, k0), phi_at(G, i1, j1, k1));
+        if (r && r < G.eps_fluid) r = G.eps_fluid;
+    } else {
+        r = 1.0;
+    }
+    float rf = (float)r;
+    if (!G.so_fluid && rf) rf = 1.0f;
+    return rf;
+}
+
+}  // namespace shkz

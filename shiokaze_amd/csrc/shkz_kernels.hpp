@@ -1,0 +1,739 @@
+// shkz_kernels.hpp -- HIP kernels of the MI355X pressure projection (gfx950).
+//
+// Stages, in the order of macpressuresolver3::project (src/projection/macpressuresolver3.cpp):
+//   k_any_negative     levelset_exist() of both level sets (array_utility3.h:112-122)
+//   k_curvature,
+//   k_surface_tension  surface-tension impulse (:85-115)
+//   k_flags            unknown labelling (:121-156) + which 8^3 bricks hold unknowns
+//   k_operator         matrix-free assembly (:159-199): conductances, Dirichlet sums, rhs
+//   k_coarsen          multigrid operators (no counterpart: stands in for MIC(0))
+//   k_defect ... k_update_p   Krylov solve (stands in for PCGSolver::solve, pcg_solver.h:246-295)
+//   k_scatter_pressure m_pressure (:245-248)
+//   k_velocity_update  velocity update (:252-268)
+//
+// All brick kernels run one 512-thread block per 8^3 brick (thread = cell), with the XCD-aware
+// block-to-brick remap; stencil neighbours in the same brick are read at +-1/8/64, the others
+// through the brick's neighbour table.  Reductions are deterministic (fixed block order).
+#pragma once
+#include "shkz_device.hpp"
+#include "shkz_geometry.hpp"
+
+namespace shkz {
+
+// ------------------------------------------------------------------------------ helpers
+struct Nbrs {
+    long base;
+    int n[6];
+};
+
+__device__ __forceinline__ Nbrs load_nbrs(const LevelDesc &L, int b) {
+    Nbrs N;
+    N.base = (long)b * BRICK_CELLS;
+#pragma unroll
+    for (int d = 0; d < 6; ++d) N.n[d] = L.bnbr[b * 6 + d];
+    return N;
+}
+
+// Storage index of the six face neighbours of local cell t, -1 where no brick is stored.
+__device__ __forceinline__ void nb_index(const Nbrs &N, int t, long idx[6]) {
+    const int li = t >> 6, lj = (t >> 3) & 7, lk = t & 7;
+    idx[NB_XM] = li > 0 ? N.base + t - 64 : (N.n[NB_XM] >= 0 ? (long)N.n[NB_XM] * BRICK_CELLS + t + 448 : -1);
+    idx[NB_XP] = li < 7 ? N.base + t + 64 : (N.n[NB_XP] >= 0 ? (long)N.n[NB_XP] * BRICK_CELLS + t - 448 : -1);
+    idx[NB_YM] = lj > 0 ? N.base + t - 8 : (N.n[NB_YM] >= 0 ? (long)N.n[NB_YM] * BRICK_CELLS + t + 56 : -1);
+    idx[NB_YP] = lj < 7 ? N.base + t + 8 : (N.n[NB_YP] >= 0 ? (long)N.n[NB_YP] * BRICK_CELLS + t - 56 : -1);
+    idx[NB_ZM] = lk > 0 ? N.base + t - 1 : (N.n[NB_ZM] >= 0 ? (long)N.n[NB_ZM] * BRICK_CELLS + t + 7 : -1);
+    idx[NB_ZP] = lk < 7 ? N.base + t + 1 : (N.n[NB_ZP] >= 0 ? (long)N.n[NB_ZP] * BRICK_CELLS + t - 7 : -1);
+}
+
+template <class T>
+__device__ __forceinline__ T ld(const T *a, long i) {
+    return i >= 0 ? a[i] : T(0);
+}
+
+// (A p)_c for an unknown c; also returns the diagonal.
+template <class T>
+__device__ __forceinline__ T apply_row(const OpDesc<T> &A, const T *__restrict__ p, long c, const long idx[6], T dirc,
+                                       T pc, T &diag) {
+    const T gxm = A.gx[c], gym = A.gy[c], gzm = A.gz[c];
+    const T gxp = ld(A.gx, idx[NB_XP]), gyp = ld(A.gy, idx[NB_YP]), gzp = ld(A.gz, idx[NB_ZP]);
+    const T off = gxm * ld(p, idx[NB_XM]) + gxp * ld(p, idx[NB_XP]) + gym * ld(p, idx[NB_YM]) +
+                  gyp * ld(p, idx[NB_YP]) + gzm * ld(p, idx[NB_ZM]) + gzp * ld(p, idx[NB_ZP]);
+    diag = dirc + ((gxm + gxp) + (gym + gyp) + (gzm + gzp));
+    return diag * pc - off;
+}
+
+__device__ __forceinline__ int brick_or_idle(const LevelDesc &L) {
+    const int b = xcd_brick(blockIdx.x, gridDim.x);
+    return b < L.nbricks ? b : -1;
+}
+
+// ------------------------------------------------------------------------------ build
+__global__ void k_reset_ints(int *a, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = 0;
+}
+
+// flag |= 1 when any of the n values is negative.
+__global__ void k_any_negative(const void *a, int is64, size_t n, int *flag) {
+    int found = 0;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const double v = is64 ? ((const double *)a)[i] : (double)((const float *)a)[i];
+        found |= (v < 0.0);
+    }
+    found = __syncthreads_or(found);
+    if (threadIdx.x == 0 && found) atomicOr(flag, 1);
+}
+
+// 7-point Laplacian of the level set with clamped neighbours (macpressuresolver3.cpp:92-100);
+// float level sets are summed in float as the reference sums Real values.
+__global__ void k_curvature(GridDesc G, float *curv) {
+    const size_t M = (size_t)G.w * G.h * G.d;
+    const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= M) return;
+    const int k = (int)(c % G.d), j = (int)((c / G.d) % G.h), i = (int)(c / ((size_t)G.d * G.h));
+    const int im = clampi(i - 1, 0, G.w - 1), ip = clampi(i + 1, 0, G.w - 1);
+    const int jm = clampi(j - 1, 0, G.h - 1), jp = clampi(j + 1, 0, G.h - 1);
+    const int km = clampi(k - 1, 0, G.d - 1), kp = clampi(k + 1, 0, G.d - 1);
+    double value;
+    if (G.phi64) {
+        const double s = +phi_at(G, im, j, k) + phi_at(G, ip, j, k) + phi_at(G, i, jm, k) + phi_at(G, i, jp, k) +
+                         phi_at(G, i, j, km) + phi_at(G, i, j, kp);
+        value = (s - 6.0 * phi_at(G, i, j, k)) / (G.dx * G.dx);
+    } else {
+        const float *f = (const float *)G.phi;
+        const float s = +f[cell_index(G, im, j, k)] + f[cell_index(G, ip, j, k)] + f[cell_index(G, i, jm, k)] +
+                        f[cell_index(G, i, jp, k)] + f[cell_index(G, i, j, km)] + f[cell_index(G, i, j, kp)];
+        value = (s - 6.0 * f[c]) / (G.dx * G.dx);
+    }
+    curv[c] = (float)value;
+}
+
+// Surface-tension impulse on faces with 0 < rho < 1 (macpressuresolver3.cpp:102-113).
+__global__ void k_surface_tension(GridDesc G, int dim, const float *curv, double kappa) {
+    const int W = G.w + (dim == 0), H = G.h + (dim == 1), D = G.d + (dim == 2);
+    const size_t n = (size_t)W * H * D;
+    const size_t f = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= n) return;
+    if (G.active[dim] && !G.active[dim][f]) return;
+    const int k = (int)(f % D), j = (int)((f / D) % H), i = (int)(f / ((size_t)D * H));
+    const double rho = face_rho(G, dim, i, j, k);
+    if (rho && rho < 1.0) {
+        const int i0 = clampi(i, 0, G.w - 1), j0 = clampi(j, 0, G.h - 1), k0 = clampi(k, 0, G.d - 1);
+        const int i1 = clampi(i - (dim == 0), 0, G.w - 1), j1 = clampi(j - (dim == 1), 0, G.h - 1),
+                  k1 = clampi(k - (dim == 2), 0, G.d - 1);
+        const double sgn = phi_at(G, i0, j0, k0) < 0.0 ? -1.0 : 1.0;
+        const double theta = sgn < 0 ? 1.0 - rho : rho;
+        const double face_c = theta * curv[cell_index(G, i0, j0, k0)] + (1.0 - theta) * curv[cell_index(G, i1, j1, k1)];
+        G.vel[dim][f] += (float)(-sgn * G.dt / (G.dx * rho) * kappa * face_c);
+    }
+}
+
+// Unknown labelling (macpressuresolver3.cpp:121-156): a liquid cell is an unknown when an
+// in-bounds liquid neighbour shares a face with nonzero area (rho is 1 between two liquid
+// cells).  One block per brick position of the dense grid; writes cell flags
+// (bit0 liquid, bit1 unknown) and whether the brick holds an unknown.
+__global__ __launch_bounds__(BLOCK) void k_flags(GridDesc G, int nbx, int nby, int nbz, uint8_t *cflag, uint8_t *bflag) {
+    const int bid = blockIdx.x;
+    const int bz = bid % nbz, by = (bid / nbz) % nby, bx = bid / (nbz * nby);
+    const int t = threadIdx.x;
+    const int i = bx * BRICK + (t >> 6), j = by * BRICK + ((t >> 3) & 7), k = bz * BRICK + (t & 7);
+    int unknown = 0;
+    if (i < G.w && j < G.h && k < G.d) {
+        const bool liquid = phi_at(G, i, j, k) < 0.0;
+        if (liquid) {
+            const int di[6] = {1, -1, 0, 0, 0, 0}, dj[6] = {0, 0, 1, -1, 0, 0}, dk[6] = {0, 0, 0, 0, 1, -1};
+            for (int nq = 0; nq < 6 && !unknown; ++nq) {
+                const int qi = i + di[nq], qj = j + dj[nq], qk = k + dk[nq];
+                if (qi < 0 || qi >= G.w || qj < 0 || qj >= G.h || qk < 0 || qk >= G.d) continue;
+                if (!(phi_at(G, qi, qj, qk) < 0.0)) continue;
+                const int dim = nq >> 1;
+                const int fi = (nq & 1) ? i : qi, fj = (nq & 1) ? j : qj, fk = (nq & 1) ? k : qk;
+                if (face_area(G, dim, fi, fj, fk) != 0.0f && face_rho(G, dim, fi, fj, fk) != 0.0f) unknown = 1;
+            }
+        }
+        cflag[cell_index(G, i, j, k)] = (uint8_t)((liquid ? 1 : 0) | (unknown ? 2 : 0));
+    }
+    unknown = __syncthreads_or(unknown);
+    if (t == 0) bflag[bid] = (uint8_t)unknown;
+}
+
+// Matrix-free assembly on the stored bricks of level 0 (macpressuresolver3.cpp:159-199).
+// For an unknown c and each in-bounds neighbour q (+x,-x,+y,-y,+z,-z): with area>0 and rho>0
+// the conductance w = dt*area/(dx*dx*rho) joins the diagonal, and couples c to q when q is
+// liquid (then q is an unknown too); with area>0 the rhs gains -sgn*area*u/dx.  Stored: the
+// low-face couplings gx/gy/gz and dir = the sum of conductances to non-unknown neighbours
+// (so diag = dir + sum of the six couplings); dir = -1 marks a non-unknown.
+__global__ __launch_bounds__(BLOCK) void k_operator(GridDesc G, LevelDesc L, const uint8_t *__restrict__ cflag,
+                                                    double *dir64, double *gx64, double *gy64, double *gz64,
+                                                    float *dir32, float *gx32, float *gy32, float *gz32, double *b,
+                                                    double *x, const double *__restrict__ warm, double rhs_correct) {
+#pragma clang fp contract(off)
+    const int br = brick_or_idle(L);
+    if (br < 0) return;
+    const int t = threadIdx.x;
+    const int bx = L.bxyz[3 * br], by = L.bxyz[3 * br + 1], bz = L.bxyz[3 * br + 2];
+    const int i = bx * BRICK + (t >> 6), j = by * BRICK + ((t >> 3) & 7), k = bz * BRICK + (t & 7);
+    const long c = (long)br * BRICK_CELLS + t;
+    const bool inside = i < G.w && j < G.h && k < G.d;
+    const bool unknown = inside && (cflag[cell_index(G, i, j, k)] & 2);
+    double dir = -1.0, g[3] = {0.0, 0.0, 0.0}, rhs = 0.0, x0 = 0.0;
+    if (unknown) {
+        const double dt = G.dt, dx = G.dx;
+        const int di[6] = {1, -1, 0, 0, 0, 0}, dj[6] = {0, 0, 1, -1, 0, 0}, dk[6] = {0, 0, 0, 0, 1, -1};
+        const int sgn[6] = {1, -1, 1, -1, 1, -1};
+        double dsum = 0.0;
+        for (int nq = 0; nq < 6; ++nq) {
+            const int qi = i + di[nq], qj = j + dj[nq], qk = k + dk[nq];
+            if (qi < 0 || qi >= G.w || qj < 0 || qj >= G.h || qk < 0 || qk >= G.d) continue;
+            const int dim = nq >> 1;
+            const int fi = (nq & 1) ? i : qi, fj = (nq & 1) ? j : qj, fk = (nq & 1) ? k : qk;
+            const double area = face_area(G, dim, fi, fj, fk);
+            if (area) {
+                const double rho = face_rho(G, dim, fi, fj, fk);
+                if (rho) {
+                    const double value = dt * area / (dx * dx * rho);
+                    if (phi_at(G, qi, qj, qk) < 0.0) {
+                        if (nq & 1) g[dim] = value;
+                    } else {
+                        dsum += value;
+                    }
+                }
+                const float u = G.vel[dim][face_index(G, dim, fi, fj, fk)];
+                rhs += -sgn[nq] * area * (double)u / dx;
+            }
+        }
+        dir = dsum;
+        rhs += rhs_correct;
+        if (warm) x0 = warm[cell_index(G, i, j, k)];
+    }
+    dir64[c] = dir;
+    gx64[c] = g[0];
+    gy64[c] = g[1];
+    gz64[c] = g[2];
+    b[c] = unknown ? rhs : 0.0;
+    x[c] = x0;
+    if (dir32) {
+        dir32[c] = (float)dir;
+        gx32[c] = (float)g[0];
+        gy32[c] = (float)g[1];
+        gz32[c] = (float)g[2];
+    }
+}
+
+// Coarse operator = 1/2 of the piecewise-constant Galerkin product: dir_H = 1/2 sum of the
+// children's dir, g_H = 1/2 sum of the four fine couplings across the coarse face.  Every child
+// of a coarse cell and every fine face it needs lie in one fine brick.
+template <class Ti, class To>
+__global__ __launch_bounds__(BLOCK) void k_coarsen(LevelDesc Lf, OpDesc<Ti> Af, LevelDesc Lc, To *dir, To *gx, To *gy,
+                                                   To *gz) {
+    const int br = brick_or_idle(Lc);
+    if (br < 0) return;
+    const int t = threadIdx.x;
+    const int I = Lc.bxyz[3 * br] * BRICK + (t >> 6), J = Lc.bxyz[3 * br + 1] * BRICK + ((t >> 3) & 7),
+              K = Lc.bxyz[3 * br + 2] * BRICK + (t & 7);
+    const long c = (long)br * BRICK_CELLS + t;
+    double sdir = 0.0, sg[3] = {0.0, 0.0, 0.0};
+    bool any = false;
+    const int fi0 = 2 * I, fj0 = 2 * J, fk0 = 2 * K;
+    const int fbx = fi0 >> 3, fby = fj0 >> 3, fbz = fk0 >> 3;
+    if (fi0 < Lf.ni && fj0 < Lf.nj && fk0 < Lf.nk) {
+        const int fb = Lf.bmap[((long)fbx * Lf.nby + fby) * Lf.nbz + fbz];
+        if (fb >= 0) {
+            for (int a = 0; a < 2; ++a)
+                for (int bb = 0; bb < 2; ++bb)
+                    for (int cc = 0; cc < 2; ++cc) {
+                        const int li = (fi0 + a) & 7, lj = (fj0 + bb) & 7, lk = (fk0 + cc) & 7;
+                        const long f = (long)fb * BRICK_CELLS + (li * 8 + lj) * 8 + lk;
+                        const double d = (double)Af.dir[f];
+                        if (d >= 0.0) {
+                            any = true;
+                            sdir += d;
+                        }
+                        if (a == 0) sg[0] += (double)Af.gx[f];
+                        if (bb == 0) sg[1] += (double)Af.gy[f];
+                        if (cc == 0) sg[2] += (double)Af.gz[f];
+                    }
+        }
+    }
+    dir[c] = any ? (To)(0.5 * sdir) : (To)(-1);
+    gx[c] = any ? (To)(0.5 * sg[0]) : (To)0;
+    gy[c] = any ? (To)(0.5 * sg[1]) : (To)0;
+    gz[c] = any ? (To)(0.5 * sg[2]) : (To)0;
+}
+
+// Number of stored cells that are unknowns (dir >= 0).
+__global__ void k_count_unknowns(const void *dir, int is64, size_t n, unsigned long long *out) {
+    unsigned long long c = 0;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const double v = is64 ? ((const double *)dir)[i] : (double)((const float *)dir)[i];
+        c += (v >= 0.0) ? 1ull : 0ull;
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off, 64);
+    __shared__ unsigned long long s[16];
+    if ((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long t = 0;
+        for (int i = 0; i < (int)(blockDim.x >> 6); ++i) t += s[i];
+        if (t) atomicAdd(out, t);
+    }
+}
+
+// ------------------------------------------------------------------------------ Krylov
+// r = b - A x (FP64) with ||r||_inf and ||r||_2.  `first` also fixes the tolerance from ||r0||.
+template <class T>
+__global__ __launch_bounds__(BLOCK) void k_defect(LevelDesc L, OpDesc<T> A, const T *__restrict__ b,
+                                                  const T *__restrict__ x, T *__restrict__ r, SolveScalars *S, RedBuf R,
+                                                  int first, double residual) {
+    __shared__ double lds[16];
+    __shared__ int flag;
+    const int br = brick_or_idle(L);
+    double mx = 0.0, sq = 0.0;
+    if (br >= 0) {
+        const int t = threadIdx.x;
+        const long c = (long)br * BRICK_CELLS + t;
+        const T dirc = A.dir[c];
+        T out = 0;
+        if (dirc >= 0) {
+            const Nbrs N = load_nbrs(L, br);
+            long idx[6];
+            nb_index(N, t, idx);
+            T diag;
+            out = b[c] - apply_row(A, x, c, idx, dirc, x[c], diag);
+        }
+        r[c] = out;
+        mx = fabs((double)out);
+        sq = (double)out * (double)out;
+    }
+    mx = block_max(mx, lds);
+    sq = block_sum(sq, lds);
+    if (publish_partials(R, mx, sq, &flag)) {
+        double rinf, rsq;
+        combine_partials(R, 1, 0, &rinf, &rsq, lds);
+        if (threadIdx.x == 0) {
+            const double rl2 = sqrt(rsq);
+            S->rinf = rinf;
+            S->rl2 = rl2;
+            if (first) {
+                S->r0inf = rinf;
+                S->r0l2 = rl2;
+                const double r0 = S->norm == 0 ? rinf : rl2;
+                S->tol = residual * r0;
+                S->iter = 0;
+                S->status = (rinf != rinf) ? 1 : 0;
+                S->converged = (r0 == 0.0) ? 1 : 0;
+                S->done = (r0 == 0.0 || rinf != rinf) ? 1 : 0;
+            }
+        }
+    }
+}
+
+// Inner (FP32) solve set-up of the defect correction: r32 = (float) r64, c = 0, tolerance
+// eta * ||r32||.
+__global__ __launch_bounds__(BLOCK) void k_inner_init(LevelDesc L, const double *__restrict__ r64, float *__restrict__ r32,
+                                                      float *__restrict__ corr, SolveScalars *S, RedBuf R, double eta,
+                                                      int maxit) {
+    __shared__ double lds[16];
+    __shared__ int flag;
+    const int br = brick_or_idle(L);
+    double mx = 0.0, sq = 0.0;
+    if (br >= 0) {
+        const long c = (long)br * BRICK_CELLS + threadIdx.x;
+        const float v = (float)r64[c];
+        r32[c] = v;
+        corr[c] = 0.0f;
+        mx = fabs((double)v);
+        sq = (double)v * (double)v;
+    }
+    mx = block_max(mx, lds);
+    sq = block_sum(sq, lds);
+    if (publish_partials(R, mx, sq, &flag)) {
+        double rinf, rsq;
+        combine_partials(R, 1, 0, &rinf, &rsq, lds);
+        if (threadIdx.x == 0) {
+            const double rl2 = sqrt(rsq);
+            S->rinf = rinf;
+            S->rl2 = rl2;
+            S->r0inf = rinf;
+            S->r0l2 = rl2;
+            S->tol = eta * (S->norm == 0 ? rinf : rl2);
+            S->iter = 0;
+            S->maxit = maxit;
+            S->status = 0;
+            S->converged = (rinf == 0.0) ? 1 : 0;
+            S->done = (rinf == 0.0 || maxit <= 0) ? 1 : 0;
+        }
+    }
+}
+
+// q = A p and p.q; the last block forms alpha = (r.z) / (p.q).
+template <class T>
+__global__ __launch_bounds__(BLOCK) void k_apply_dot(LevelDesc L, OpDesc<T> A, const T *__restrict__ p,
+                                                     T *__restrict__ q, SolveScalars *S, RedBuf R) {
+    __shared__ double lds[16];
+    __shared__ int flag;
+    if (S->done) return;
+    const int br = brick_or_idle(L);
+    double part = 0.0;
+    if (br >= 0) {
+        const int t = threadIdx.x;
+        const long c = (long)br * BRICK_CELLS + t;
+        const T dirc = A.dir[c];
+        T out = 0;
+        if (dirc >= 0) {
+            const Nbrs N = load_nbrs(L, br);
+            long idx[6];
+            nb_index(N, t, idx);
+            const T pc = p[c];
+            T diag;
+            out = apply_row(A, p, c, idx, dirc, pc, diag);
+            part = (double)pc * (double)out;
+        }
+        q[c] = out;
+    }
+    part = block_sum(part, lds);
+    if (publish_partials(R, part, 0.0, &flag)) {
+        double pq, unused;
+        combine_partials(R, 0, 0, &pq, &unused, lds);
+        if (threadIdx.x == 0) {
+            S->pq = pq;
+            const double a = S->rz / pq;
+            S->alpha = a;
+            if (!(a == a) || a == 1.0 / 0.0 || a == -1.0 / 0.0) {
+                S->status = 1;
+                S->done = 1;
+            }
+        }
+    }
+}
+
+// x += alpha p, r -= alpha q, then the convergence test on ||r|| (pcg_solver.h:278-285).
+// plain_cg: z = r, so r.r is the next r.z and beta follows here.
+template <class T>
+__global__ __launch_bounds__(BLOCK) void k_cg_update(LevelDesc L, T *__restrict__ x, T *__restrict__ r,
+                                                     const T *__restrict__ p, const T *__restrict__ q, SolveScalars *S,
+                                                     RedBuf R, int plain_cg) {
+    __shared__ double lds[16];
+    __shared__ int flag;
+    if (S->done) return;
+    const int br = brick_or_idle(L);
+    double mx = 0.0, sq = 0.0;
+    if (br >= 0) {
+        const long c = (long)br * BRICK_CELLS + threadIdx.x;
+        const T alpha = (T)S->alpha;
+        x[c] += alpha * p[c];
+        const T rv = r[c] - alpha * q[c];
+        r[c] = rv;
+        mx = fabs((double)rv);
+        sq = (double)rv * (double)rv;
+    }
+    mx = block_max(mx, lds);
+    sq = block_sum(sq, lds);
+    if (publish_partials(R, mx, sq, &flag)) {
+        double rinf, rsq;
+        combine_partials(R, 1, 0, &rinf, &rsq, lds);
+        if (threadIdx.x == 0) {
+            const double rl2 = sqrt(rsq);
+            S->rinf = rinf;
+            S->rl2 = rl2;
+            const int it = S->iter + 1;
+            S->iter = it;
+            const double cur = S->norm == 0 ? rinf : rl2;
+            if (!(cur == cur)) {
+                S->status = 1;
+                S->done = 1;
+            } else if (cur <= S->tol) {
+                S->converged = 1;
+                S->done = 1;
+            } else if (it >= S->maxit) {
+                S->done = 1;
+            }
+            if (plain_cg) {
+                S->beta = rsq / S->rz;
+                S->rz = rsq;
+            }
+        }
+    }
+}
+
+// r.z for the next direction: beta = (r.z)_new / (r.z)_old.  first: rz only.
+template <class T>
+__global__ __launch_bounds__(BLOCK) void k_dot_rz(LevelDesc L, const T *__restrict__ r, const T *__restrict__ z,
+                                                  SolveScalars *S, RedBuf R, int first) {
+    __shared__ double lds[16];
+    __shared__ int flag;
+    if (S->done) return;
+    const int br = brick_or_idle(L);
+    double part = 0.0;
+    if (br >= 0) {
+        const long c = (long)br * BRICK_CELLS + threadIdx.x;
+        part = (double)r[c] * (double)z[c];
+    }
+    part = block_sum(part, lds);
+    if (publish_partials(R, part, 0.0, &flag)) {
+        double rz, unused;
+        combine_partials(R, 0, 0, &rz, &unused, lds);
+        if (threadIdx.x == 0) {
+            S->beta = first ? 0.0 : rz / S->rz;
+            S->rz = rz;
+            if (!(rz == rz) || (first && rz == 0.0)) {
+                S->status = (rz == rz) ? 0 : 1;
+                S->done = 1;
+            }
+        }
+    }
+}
+
+// p = z + beta p (first: p = z).
+template <class T>
+__global__ __launch_bounds__(BLOCK) void k_update_p(LevelDesc L, const T *__restrict__ z, T *__restrict__ p,
+                                                    const SolveScalars *S, int first) {
+    if (S->done) return;
+    const int br = brick_or_idle(L);
+    if (br < 0) return;
+    const long c = (long)br * BRICK_CELLS + threadIdx.x;
+    p[c] = first ? z[c] : z[c] + (T)S->beta * p[c];
+}
+
+// x64 += (double) corr
+__global__ __launch_bounds__(BLOCK) void k_add_correction(LevelDesc L, double *__restrict__ x,
+                                                          const float *__restrict__ corr) {
+    const int br = brick_or_idle(L);
+    if (br < 0) return;
+    const long c = (long)br * BRICK_CELLS + threadIdx.x;
+    x[c] += (double)corr[c];
+}
+
+// ------------------------------------------------------------------------------ V-cycle
+// z = omega * rhs / diag (first damped-Jacobi sweep from a zero guess).
+template <class T>
+__global__ __launch_bounds__(BLOCK) void k_smooth_first(LevelDesc L, OpDesc<T> A, const T *__restrict__ rhs,
+                                                        T *__restrict__ z, T omega, const SolveScalars *S) {
+    if (S && S->done) return;
+    const int br = brick_or_idle(L);
+    if (br < 0) return;
+    const int t = threadIdx.x;
+    const long c = (long)br * BRICK_CELLS + t;
+    const T dirc = A.dir[c];
+    T out = 0;
+    if (dirc >= 0) {
+        const Nbrs N = load_nbrs(L, br);
+        long idx[6];
+        nb_index(N, t, idx);
+        const T diag = dirc + ((A.gx[c] + ld(A.gx, idx[NB_XP])) + (A.gy[c] + ld(A.gy, idx[NB_YP])) +
+                               (A.gz[c] + ld(A.gz, idx[NB_ZP])));
+        if (diag > 0) out = omega * rhs[c] / diag;
+    }
+    z[c] = out;
+}
+
+// zout = zin + omega (rhs - A zin) / diag
+template <class T>
+__global__ __launch_bounds__(BLOCK) void k_jacobi(LevelDesc L, OpDesc<T> A, const T *__restrict__ rhs,
+                                                  const T *__restrict__ zin, T *__restrict__ zout, T omega,
+                                                  const SolveScalars *S) {
+    if (S && S->done) return;
+    const int br = brick_or_idle(L);
+    if (br < 0) return;
+    const int t = threadIdx.x;
+    const long c = (long)br * BRICK_CELLS + t;
+    const T dirc = A.dir[c];
+    T out = 0;
+    if (dirc >= 0) {
+        const Nbrs N = load_nbrs(L, br);
+        long idx[6];
+        nb_index(N, t, idx);
+        const T zc = zin[c];
+        T diag;
+        const T az = apply_row(A, zin, c, idx, dirc, zc, diag);
+        out = diag > 0 ? zc + omega * (rhs[c] - az) / diag : T(0);
+    }
+    zout[c] = out;
+}
+
+// res = rhs - A z
+template <class T>
+__global__ __launch_bounds__(BLOCK) void k_residual(LevelDesc L, OpDesc<T> A, const T *__restrict__ rhs,
+                                                    const T *__restrict__ z, T *__restrict__ res, const SolveScalars *S) {
+    if (S && S->done) return;
+    const int br = brick_or_idle(L);
+    if (br < 0) return;
+    const int t = threadIdx.x;
+    const long c = (long)br * BRICK_CELLS + t;
+    const T dirc = A.dir[c];
+    T out = 0;
+    if (dirc >= 0) {
+        const Nbrs N = load_nbrs(L, br);
+        long idx[6];
+        nb_index(N, t, idx);
+        T diag;
+        out = rhs[c] - apply_row(A, z, c, idx, dirc, z[c], diag);
+    }
+    res[c] = out;
+}
+
+template <class T>
+__device__ __forceinline__ T level_value(const LevelDesc &L, const T *__restrict__ v, int i, int j, int k) {
+    if (i < 0 || j < 0 || k < 0 || i >= L.ni || j >= L.nj || k >= L.nk) return T(0);
+    const int b = L.bmap[((long)(i >> 3) * L.nby + (j >> 3)) * L.nbz + (k >> 3)];
+    if (b < 0) return T(0);
+    return v[(long)b * BRICK_CELLS + (((i & 7) * 8 + (j & 7)) * 8 + (k & 7))];
+}
+
+// Restriction = transpose of trilinear prolongation: coarse cell I gathers fine cells
+// 2I-1..2I+2 per axis with weights 1/4, 3/4, 3/4, 1/4.
+template <class T>
+__global__ __launch_bounds__(BLOCK) void k_restrict(LevelDesc Lf, LevelDesc Lc, OpDesc<T> Ac, const T *__restrict__ res,
+                                                    T *__restrict__ rhs_c, const SolveScalars *S) {
+    if (S && S->done) return;
+    const int br = brick_or_idle(Lc);
+    if (br < 0) return;
+    const int t = threadIdx.x;
+    const long c = (long)br * BRICK_CELLS + t;
+    T out = 0;
+    if (Ac.dir[c] >= 0) {
+        const int I = Lc.bxyz[3 * br] * BRICK + (t >> 6), J = Lc.bxyz[3 * br + 1] * BRICK + ((t >> 3) & 7),
+                  K = Lc.bxyz[3 * br + 2] * BRICK + (t & 7);
+        const T w[4] = {T(0.25), T(0.75), T(0.75), T(0.25)};
+        T acc = 0;
+        for (int a = 0; a < 4; ++a) {
+            T acc_a = 0;
+            for (int b = 0; b < 4; ++b) {
+                T acc_b = 0;
+                for (int cc = 0; cc < 4; ++cc)
+                    acc_b += w[cc] * level_value(Lf, res, 2 * I - 1 + a, 2 * J - 1 + b, 2 * K - 1 + cc);
+                acc_a += w[b] * acc_b;
+            }
+            acc += w[a] * acc_a;
+        }
+        out = acc;
+    }
+    rhs_c[c] = out;
+}
+
+// z_f += trilinear interpolation of the coarse correction (fine unknowns only).
+template <class T>
+__global__ __launch_bounds__(BLOCK) void k_prolong(LevelDesc Lf, OpDesc<T> Af, LevelDesc Lc, const T *__restrict__ zc,
+                                                   T *__restrict__ zf, const SolveScalars *S) {
+    if (S && S->done) return;
+    const int br = brick_or_idle(Lf);
+    if (br < 0) return;
+    const int t = threadIdx.x;
+    const long c = (long)br * BRICK_CELLS + t;
+    if (!(Af.dir[c] >= 0)) return;
+    const int i = Lf.bxyz[3 * br] * BRICK + (t >> 6), j = Lf.bxyz[3 * br + 1] * BRICK + ((t >> 3) & 7),
+              k = Lf.bxyz[3 * br + 2] * BRICK + (t & 7);
+    const int I0 = i >> 1, J0 = j >> 1, K0 = k >> 1;
+    const int I1 = (i & 1) ? I0 + 1 : I0 - 1, J1 = (j & 1) ? J0 + 1 : J0 - 1, K1 = (k & 1) ? K0 + 1 : K0 - 1;
+    const T n = T(0.75), f = T(0.25);
+    const T v = n * (n * (n * level_value(Lc, zc, I0, J0, K0) + f * level_value(Lc, zc, I0, J0, K1)) +
+                     f * (n * level_value(Lc, zc, I0, J1, K0) + f * level_value(Lc, zc, I0, J1, K1))) +
+                f * (n * (n * level_value(Lc, zc, I1, J0, K0) + f * level_value(Lc, zc, I1, J0, K1)) +
+                     f * (n * level_value(Lc, zc, I1, J1, K0) + f * level_value(Lc, zc, I1, J1, K1)));
+    zf[c] += v;
+}
+
+// Coarsest level (one brick): `sweeps` damped-Jacobi sweeps from zero, in LDS.
+template <class T>
+__global__ __launch_bounds__(BLOCK) void k_coarse_solve(LevelDesc L, OpDesc<T> A, const T *__restrict__ rhs,
+                                                        T *__restrict__ z, int sweeps, T omega, const SolveScalars *S) {
+    __shared__ T zs[2][BRICK_CELLS];
+    if (S && S->done) return;
+    if (L.nbricks < 1) return;
+    const int t = threadIdx.x;
+    const int li = t >> 6, lj = (t >> 3) & 7, lk = t & 7;
+    const long c = t;  // brick 0
+    const T dirc = A.dir[c];
+    const T gxm = A.gx[c], gym = A.gy[c], gzm = A.gz[c];
+    const T gxp = li < 7 ? A.gx[c + 64] : T(0), gyp = lj < 7 ? A.gy[c + 8] : T(0), gzp = lk < 7 ? A.gz[c + 1] : T(0);
+    const T diag = dirc + ((gxm + gxp) + (gym + gyp) + (gzm + gzp));
+    const bool unk = dirc >= 0 && diag > 0;
+    const T r = rhs[c];
+    const T inv = unk ? omega / diag : T(0);
+    zs[0][t] = unk ? inv * r : T(0);
+    __syncthreads();
+    int cur = 0;
+    for (int s = 1; s < sweeps; ++s) {
+        const T *zz = zs[cur];
+        T out = 0;
+        if (unk) {
+            const T off = gxm * (li > 0 ? zz[t - 64] : T(0)) + gxp * (li < 7 ? zz[t + 64] : T(0)) +
+                          gym * (lj > 0 ? zz[t - 8] : T(0)) + gyp * (lj < 7 ? zz[t + 8] : T(0)) +
+                          gzm * (lk > 0 ? zz[t - 1] : T(0)) + gzp * (lk < 7 ? zz[t + 1] : T(0));
+            const T zc = zz[t];
+            out = zc + inv * (r - (diag * zc - off));
+        }
+        zs[cur ^ 1][t] = out;
+        __syncthreads();
+        cur ^= 1;
+    }
+    z[c] = zs[cur][t];
+}
+
+// ------------------------------------------------------------------------------ output
+// m_pressure: dense float (and optional FP64) pressure, 0 off the unknowns
+// (macpressuresolver3.cpp:245-248).
+__global__ void k_scatter_pressure(GridDesc G, LevelDesc L, const double *__restrict__ x, float *P, double *P64) {
+    const size_t M = (size_t)G.w * G.h * G.d;
+    const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= M) return;
+    const int k = (int)(c % G.d), j = (int)((c / G.d) % G.h), i = (int)(c / ((size_t)G.d * G.h));
+    const double v = level_value(L, x, i, j, k);
+    if (P) P[c] = (float)v;
+    if (P64) P64[c] = v;
+}
+
+// Dense export of the level-0 system: diag (0 off the unknowns), the low-face couplings and the
+// right-hand side, all cell-indexed.
+__global__ void k_export_system(GridDesc G, LevelDesc L, OpDesc<double> A, const double *__restrict__ b, double *diag,
+                                double *gx, double *gy, double *gz, double *rhs) {
+    const size_t M = (size_t)G.w * G.h * G.d;
+    const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= M) return;
+    const int k = (int)(c % G.d), j = (int)((c / G.d) % G.h), i = (int)(c / ((size_t)G.d * G.h));
+    const double dirc = level_value(L, A.dir, i, j, k);
+    const bool unk = dirc >= 0.0 && L.bmap[((long)(i >> 3) * L.nby + (j >> 3)) * L.nbz + (k >> 3)] >= 0;
+    const double gxm = level_value(L, A.gx, i, j, k), gym = level_value(L, A.gy, i, j, k),
+                 gzm = level_value(L, A.gz, i, j, k);
+    const double gxp = level_value(L, A.gx, i + 1, j, k), gyp = level_value(L, A.gy, i, j + 1, k),
+                 gzp = level_value(L, A.gz, i, j, k + 1);
+    diag[c] = unk ? dirc + ((gxm + gxp) + (gym + gyp) + (gzm + gzp)) : 0.0;
+    gx[c] = gxm;
+    gy[c] = gym;
+    gz[c] = gzm;
+    rhs[c] = unk ? level_value(L, b, i, j, k) : 0.0;
+}
+
+// Velocity update over every (active) face (macpressuresolver3.cpp:252-268): open faces on the
+// domain boundary become 0, interior open faces lose dt*(P1-P0)/(rho*dx) with P stored in
+// float and P1-P0 taken in float; all other faces are switched off (0).
+__global__ void k_velocity_update(GridDesc G, int dim, LevelDesc L, const double *__restrict__ x) {
+#pragma clang fp contract(off)
+    const int W = G.w + (dim == 0), H = G.h + (dim == 1), D = G.d + (dim == 2);
+    const size_t n = (size_t)W * H * D;
+    const size_t f = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= n) return;
+    if (G.active[dim] && !G.active[dim][f]) return;
+    const int k = (int)(f % D), j = (int)((f / D) % H), i = (int)(f / ((size_t)D * H));
+    const int pi[3] = {i, j, k};
+    const int N[3] = {G.w, G.h, G.d};
+    float *U = G.vel[dim];
+    const float area = face_area(G, dim, i, j, k);
+    const double rho = face_rho(G, dim, i, j, k);
+    if (area != 0.0f && rho != 0.0) {
+        if (pi[dim] == 0 || pi[dim] == N[dim]) {
+            U[f] = 0.0f;
+        } else {
+            const float p1 = (float)level_value(L, x, i, j, k);
+            const float p0 = (float)level_value(L, x, i - (dim == 0), j - (dim == 1), k - (dim == 2));
+            const float diff = p1 - p0;
+            const float delta = (float)(G.dt * (double)diff / (rho * G.dx));
+            U[f] = U[f] - delta;
+        }
+    } else {
+        U[f] = 0.0f;
+    }
+}
+
+}  // namespace shkz

@@ -1,0 +1,190 @@
+// shkz_scenes.cpp -- seeded synthetic projection workloads (host only).
+//
+// Sampling conventions follow macutility3::assign_initial_variables
+// (src/utility/macutility3.cpp:336-374): the liquid level set is sampled at cell centres,
+// the solid level set at nodes, and where a solid exists the liquid is pushed out of it as
+// max(phi_fluid, -(phi_solid + dx)) (:358-364).  Shapes are unions (min) of signed distances;
+// the box uses the reference's utility::box (include/shiokaze/utility/utility.h:113-120).
+#include "../../include/shkz_mi355x.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <random>
+#include <vector>
+
+namespace {
+
+// std::mt19937_64 is fully specified by the C++ standard; the normal sampler is written here
+// (Box-Muller) because std::normal_distribution differs between standard libraries.
+struct SceneRng {
+    std::mt19937_64 g;
+    bool has_spare = false;
+    double spare = 0.0;
+    explicit SceneRng(uint64_t seed) : g(seed) {}
+    double uniform() { return (double)(g() >> 11) * (1.0 / 9007199254740992.0); }  // [0,1)
+    double uniform(double a, double b) { return a + (b - a) * uniform(); }
+    double normal() {
+        if (has_spare) {
+            has_spare = false;
+            return spare;
+        }
+        double u1 = 1.0 - uniform();  // (0,1]
+        double u2 = uniform();
+        double rad = std::sqrt(-2.0 * std::log(u1));
+        double ang = 6.283185307179586476925286766559 * u2;
+        spare = rad * std::sin(ang);
+        has_spare = true;
+        return rad * std::cos(ang);
+    }
+};
+
+struct Sphere {
+    double c[3];
+    double r;
+};
+
+struct Scene {
+    std::vector<Sphere> liquid_spheres;
+    double pool = -1.0;  // pool level (liquid below y < pool), <0 = none
+    std::vector<Sphere> solid_spheres;
+    bool solid_box = false;
+    double box0[3] = {0, 0, 0}, box1[3] = {0, 0, 0};
+    double shift[3] = {0, 0, 0};
+
+    bool has_solid() const { return !solid_spheres.empty() || solid_box; }
+
+    double fluid(const double p[3]) const {
+        const double q[3] = {p[0] - shift[0], p[1] - shift[1], p[2] - shift[2]};
+        double v = 1e9;
+        if (pool >= 0.0) v = std::min(v, q[1] - pool);
+        for (const Sphere &s : liquid_spheres) {
+            double dx = q[0] - s.c[0], dy = q[1] - s.c[1], dz = q[2] - s.c[2];
+            v = std::min(v, std::sqrt(dx * dx + dy * dy + dz * dz) - s.r);
+        }
+        return v;
+    }
+    double solid(const double p[3]) const {
+        double v = 1e9;
+        for (const Sphere &s : solid_spheres) {
+            double dx = p[0] - s.c[0], dy = p[1] - s.c[1], dz = p[2] - s.c[2];
+            v = std::min(v, std::sqrt(dx * dx + dy * dy + dz * dz) - s.r);
+        }
+        if (solid_box) {
+            double sd = -9999.0;
+            for (int dim = 0; dim < 3; ++dim) {
+                sd = std::max(sd, box0[dim] - p[dim]);
+                sd = std::max(sd, p[dim] - box1[dim]);
+            }
+            v = std::min(v, sd);
+        }
+        return v;
+    }
+};
+
+}  // namespace
+
+extern "C" int shkz_scene_fill(const shkz_scene_desc *desc, void *fluid, float *solid, int32_t *has_solid, float *u,
+                               float *v, float *w) {
+    if (!desc || desc->w <= 0 || desc->h <= 0 || desc->d <= 0) return SHKZ_ERR_INVALID;
+    const int W = desc->w, H = desc->h, D = desc->d;
+    const double dx = std::min(1.0 / W, std::min(1.0 / H, 1.0 / D));
+    SceneRng rng(desc->seed);
+    Scene sc;
+    for (int a = 0; a < 3; ++a) sc.shift[a] = desc->shift[a];
+    switch (desc->kind) {
+    case SHKZ_SCENE_DROP:
+        sc.liquid_spheres.push_back({{0.5, 0.7, 0.5}, 0.15});
+        sc.pool = 0.3;
+        break;
+    case SHKZ_SCENE_DROP_SOLIDS:
+        sc.liquid_spheres.push_back({{0.5, 0.7, 0.5}, 0.15});
+        sc.pool = 0.3;
+        for (int s = 0; s < 3; ++s) {
+            Sphere sp;
+            for (int a = 0; a < 3; ++a) sp.c[a] = rng.uniform(0.2, 0.8);
+            sp.r = 0.1;
+            sc.solid_spheres.push_back(sp);
+        }
+        break;
+    case SHKZ_SCENE_SPHERES_BOX: {
+        sc.pool = 0.25;
+        for (int s = 0; s < 8; ++s) {
+            Sphere sp;
+            sp.r = rng.uniform(0.05, 0.15);
+            sp.c[0] = rng.uniform(0.15, 0.85);
+            sp.c[1] = rng.uniform(0.3, 0.85);
+            sp.c[2] = rng.uniform(0.15, 0.85);
+            sc.liquid_spheres.push_back(sp);
+        }
+        // one axis-aligned box holding 10% of the unit cube
+        double sx = rng.uniform(0.35, 0.65), sy = rng.uniform(0.35, 0.65);
+        double sz = std::min(0.9, 0.1 / (sx * sy));
+        double s3[3] = {sx, sy, sz};
+        for (int a = 0; a < 3; ++a) {
+            sc.box0[a] = rng.uniform(0.05, std::max(0.05, 0.95 - s3[a]));
+            sc.box1[a] = sc.box0[a] + s3[a];
+        }
+        sc.solid_box = true;
+        break;
+    }
+    case SHKZ_SCENE_RANDOM_DROP: {
+        sc.pool = rng.uniform(0.2, 0.35);
+        Sphere sp;
+        sp.r = rng.uniform(0.08, 0.18);
+        sp.c[0] = rng.uniform(0.3, 0.7);
+        sp.c[2] = rng.uniform(0.3, 0.7);
+        double lo = sc.pool + sp.r + 0.05, hi = 0.95 - sp.r;
+        sp.c[1] = rng.uniform(lo, std::max(lo, hi));
+        sc.liquid_spheres.push_back(sp);
+        break;
+    }
+    default:
+        return SHKZ_ERR_INVALID;
+    }
+    const bool with_solid = sc.has_solid();
+    if (has_solid) *has_solid = with_solid ? 1 : 0;
+
+    // liquid level set at cell centres
+    if (fluid) {
+        for (int i = 0; i < W; ++i)
+            for (int j = 0; j < H; ++j)
+                for (int k = 0; k < D; ++k) {
+                    double p[3] = {(i + 0.5) * dx, (j + 0.5) * dx, (k + 0.5) * dx};
+                    double val = sc.fluid(p);
+                    if (with_solid) val = std::max(val, -(sc.solid(p) + dx));
+                    size_t c = ((size_t)i * H + j) * D + k;
+                    if (desc->fluid_dtype == SHKZ_F64) ((double *)fluid)[c] = val;
+                    else ((float *)fluid)[c] = (float)val;
+                }
+    }
+    // solid level set at nodes
+    if (solid) {
+        for (int i = 0; i <= W; ++i)
+            for (int j = 0; j <= H; ++j)
+                for (int k = 0; k <= D; ++k) {
+                    double p[3] = {i * dx, j * dx, k * dx};
+                    size_t n = ((size_t)i * (H + 1) + j) * (D + 1) + k;
+                    solid[n] = with_solid ? (float)sc.solid(p) : 1.0f;
+                }
+    }
+    // face velocities: i.i.d. N(0,1) over u, then v, then w (index order), gravity*dt on v
+    SceneRng vrng(desc->velocity_seed ? desc->velocity_seed : 0);
+    SceneRng &vr = desc->velocity_seed ? vrng : rng;
+    float *vel[3] = {u, v, w};
+    for (int dim = 0; dim < 3; ++dim) {
+        size_t n = (size_t)(W + (dim == 0)) * (H + (dim == 1)) * (D + (dim == 2));
+        float *a = vel[dim];
+        if (!a) continue;
+        if (desc->velocity_mode == 1) {
+            std::memset(a, 0, n * sizeof(float));
+        } else {
+            for (size_t f = 0; f < n; ++f) a[f] = (float)vr.normal();
+        }
+        if (dim == 1 && desc->gravity != 0.0) {
+            const float g = (float)(desc->gravity * desc->dt);
+            for (size_t f = 0; f < n; ++f) a[f] += g;
+        }
+    }
+    return SHKZ_OK;
+}

@@ -1,0 +1,262 @@
+"""Parity of the HIP projection (through the C ABI) with the CPU oracle.
+
+Bars:
+  * integer / structural work (unknown labelling, which cells are unknowns): identical;
+  * operator and right-hand side: couplings and rhs bit-identical to the oracle's CSR (same
+    arithmetic in the same order), diagonal within 1e-14 relative (summation order differs);
+  * pressure: relative L2 error against an FP64 oracle solve to 1e-12 within the tolerance each
+    test states (set by the requested residual);
+  * velocities: bit-identical to the oracle's velocity update applied to the GPU's own pressure;
+  * divergence: |max div_gpu - max div_oracle| <= 1e-5 (the 1e-5 gate of the projection target).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import shiokaze_amd as S
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(f"[gpu-case] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = S.Context(0)
+    yield c
+    c.close()
+
+
+def oracle_solution(sc, residual=1e-12, params=None):
+    A = O.assemble(sc, params)
+    x, it, rr, ok = O.pcg(A, residual=residual, mic_mode=O.MIC_APPLIED)
+    assert ok
+    return A, x
+
+
+def run_gpu(ctx, sc, **kw):
+    prm = S.default_params(**kw)
+    work = sc.copy()
+    r, P, P64 = ctx.project(work, prm, pressure64=True)
+    return r, P, P64, work
+
+
+def rel_l2(a, b):
+    nb = np.linalg.norm(b)
+    return np.linalg.norm(a - b) / (nb if nb else 1.0)
+
+
+SCENES = {
+    "drop64": lambda: S.make_scene(S.SCENE_DROP, 64, seed=0),
+    "solids48": lambda: S.make_scene(S.SCENE_DROP_SOLIDS, 48, seed=1),
+    "box40": lambda: S.make_scene(S.SCENE_SPHERES_BOX, 40, seed=2),
+    "odd": lambda: S.make_scene(S.SCENE_RANDOM_DROP, 21, 13, 30, seed=9),
+    "float_phi": lambda: S.make_scene(S.SCENE_DROP_SOLIDS, 33, seed=5, fluid_dtype=S.F32),
+}
+
+
+@pytest.mark.parametrize("name", list(SCENES))
+def test_operator_matches_oracle(ctx, name):
+    say(f"operator {name}")
+    sc = SCENES[name]()
+    exp = ctx.export_system(sc)
+    A = O.assemble(sc)
+    im = A["index_map"]
+    unk = im >= 0
+    assert np.array_equal(exp["diag"] > 0, unk), "unknown sets differ"
+    n = A["n"]
+    rows = np.repeat(np.arange(n), np.diff(A["rowptr"]))
+    cells = np.argwhere(unk)
+    cell_of_row = np.empty((n, 3), np.int64)
+    cell_of_row[im[unk]] = cells
+    diag = np.zeros(n)
+    off = {}
+    for r_, c_, v_ in zip(rows, A["col"], A["val"]):
+        if r_ == c_:
+            diag[r_] = v_
+        else:
+            off[(r_, c_)] = v_
+    # diagonal
+    gd = exp["diag"][unk][np.argsort(im[unk])]
+    assert np.allclose(gd, diag, rtol=1e-14, atol=0)
+    # couplings: every low-face coupling equals -A[c, c-e] exactly, and every matrix entry is seen
+    seen = 0
+    for dim, key in enumerate(("gx", "gy", "gz")):
+        g = exp[key]
+        e = np.zeros(3, np.int64)
+        e[dim] = 1
+        nz = np.argwhere(g != 0)
+        for (i, j, k) in nz:
+            q = (i - e[0], j - e[1], k - e[2])
+            r1, r2 = im[i, j, k], im[q]
+            assert r1 >= 0 and r2 >= 0
+            assert -g[i, j, k] == off[(r1, r2)] == off[(r2, r1)]
+            seen += 2
+    assert seen == len(off)
+    rhs = exp["rhs"][unk][np.argsort(im[unk])]
+    assert np.array_equal(rhs, A["b"])
+
+
+@pytest.mark.parametrize("name", ["drop64", "solids48", "odd"])
+def test_reference_cg_iteration_matches_oracle(ctx, name):
+    """SHKZ_SOLVER_CG runs the reference's iteration (plain CG, inf-norm test): same count."""
+    say(f"cg {name}")
+    sc = SCENES[name]()
+    A = O.assemble(sc)
+    x_o, it_o, rr_o, ok = O.pcg(A, residual=1e-8, mic_mode=O.MIC_NONE)
+    r, P, P64, work = run_gpu(ctx, sc, solver=S.SOLVER_CG, precision=S.PREC_FP64, residual=1e-8)
+    assert r.converged and r.status == 0
+    assert abs(r.iterations - it_o) <= max(3, 0.02 * it_o), (r.iterations, it_o)
+    x_g = O.gather(A, P64)
+    assert rel_l2(x_g, x_o) < 1e-6
+
+
+@pytest.mark.parametrize("name", list(SCENES))
+@pytest.mark.parametrize("prec", [S.PREC_FP64, S.PREC_MIXED])
+def test_mgpcg_pressure_and_velocity(ctx, name, prec):
+    say(f"mgpcg {name} prec={prec}")
+    sc = SCENES[name]()
+    A, x_ref = oracle_solution(sc)
+    r, P, P64, work = run_gpu(ctx, sc, precision=prec, residual=1e-9)
+    assert r.status == 0 and r.converged, r.as_dict()
+    x_g = O.gather(A, P64)
+    # the FP64 residual of the returned pressure meets the requested tolerance (inf norm, rel. b)
+    resid = A["b"] - O.spmv(A, x_g)
+    assert np.abs(resid).max() <= 1e-9 * np.abs(A["b"]).max() * 1.0001
+    assert rel_l2(x_g, x_ref) < 1e-5
+    assert np.array_equal(P, P64.astype(np.float32))
+    # velocity update: bit-identical to the oracle's update applied to the GPU pressure
+    ref = sc.copy()
+    O.velocity_update(ref, P, A["area"], A["rho"])
+    for a, b in ((work.u, ref.u), (work.v, ref.v), (work.wv, ref.wv)):
+        assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("name", ["drop64", "solids48"])
+def test_divergence_gate(ctx, name):
+    say(f"divergence {name}")
+    sc = SCENES[name]()
+    ref = sc.copy()
+    r_o, P_o, _ = O.project(ref, O.default_params(residual=1e-10, mic_mode=O.MIC_APPLIED))
+    r, P, P64, work = run_gpu(ctx, sc, residual=1e-6)
+    A = O.assemble(sc)
+    d_gpu, _ = O.divergence(work, A["index_map"], A["area"])
+    d_ref, _ = O.divergence(ref, A["index_map"], A["area"])
+    assert abs(d_gpu - d_ref) <= 1e-5 * max(1.0, 1.0), (d_gpu, d_ref)
+
+
+def test_no_liquid(ctx):
+    say("no liquid")
+    w = h = d = 10
+    fluid = np.full((w, h, d), 0.5)
+    sc = S.Scene(w, h, d, 1 / 120, fluid, None, *(np.ones(s, np.float32) for s in O.face_shapes(w, h, d)))
+    ref = sc.copy()
+    O.project(ref, O.default_params())
+    r, P, P64, work = run_gpu(ctx, sc)
+    assert r.status == 0 and r.n_unknowns == 0 and r.converged
+    assert not P.any()
+    for a, b in ((work.u, ref.u), (work.v, ref.v), (work.wv, ref.wv)):
+        assert np.array_equal(a, b)
+
+
+def test_all_liquid_tiny_grids(ctx):
+    for dims in [(1, 1, 1), (2, 1, 1), (2, 3, 5), (8, 8, 8), (9, 8, 17)]:
+        say(f"all liquid {dims}")
+        w, h, d = dims
+        fluid = np.full(dims, -0.5)
+        rng = np.random.default_rng(sum(dims))
+        vel = [rng.normal(size=s).astype(np.float32) for s in O.face_shapes(w, h, d)]
+        sc = S.Scene(w, h, d, 1 / 120, fluid, None, *vel)
+        ref = sc.copy()
+        r_o, P_o, P64_o = O.project(ref, O.default_params(residual=1e-10, mic_mode=O.MIC_APPLIED))
+        r, P, P64, work = run_gpu(ctx, sc, residual=1e-10, precision=S.PREC_FP64)
+        assert r.status == 0
+        assert r.n_unknowns == r_o.n_unknowns
+        # pure Neumann: pressure defined up to a constant; compare gradients via velocities
+        for a, b in ((work.u, ref.u), (work.v, ref.v), (work.wv, ref.wv)):
+            assert np.abs(a - b).max() <= 1e-4 * max(1.0, np.abs(b).max())
+
+
+def test_surface_tension_and_first_order(ctx):
+    for kw in [dict(surface_tension=0.05), dict(second_order_fluid=0), dict(second_order_solid=0)]:
+        say(f"options {kw}")
+        sc = S.make_scene(S.SCENE_DROP_SOLIDS, 32, seed=7, fluid_dtype=S.F32)
+        oprm = O.default_params(residual=1e-11, mic_mode=O.MIC_APPLIED)
+        for k, v in kw.items():
+            setattr(oprm, k, v)
+        ref = sc.copy()
+        r_o, P_o, P64_o = O.project(ref, oprm)
+        r, P, P64, work = run_gpu(ctx, sc, residual=1e-11, precision=S.PREC_FP64, **kw)
+        assert r.status == 0 and r.converged
+        assert r.n_unknowns == r_o.n_unknowns
+        assert rel_l2(P64, P64_o) < 1e-6
+        for a, b in ((work.u, ref.u), (work.v, ref.v), (work.wv, ref.wv)):
+            assert np.abs(a - b).max() <= 1e-4 * max(1.0, np.abs(b).max())
+
+
+def test_volume_correction_matches_oracle(ctx):
+    say("volume correction")
+    sc = S.make_scene(S.SCENE_DROP, 24, seed=2)
+    c2 = S.Context(0)
+    prm = S.default_params(residual=1e-10, precision=S.PREC_FP64, gain=1.0, current_volume=0.9, target_volume=1.0)
+    work = sc.copy()
+    r, P, P64 = c2.project(work, prm, pressure64=True)
+    c2.close()
+    y = ctypes_double(0.0)
+    rc = O.lib().oracle_volume_correction(0.9, 1.0, sc.dt, 1.0, y)
+    assert r.rhs_correct == rc
+    oprm = O.default_params(residual=1e-12, mic_mode=O.MIC_APPLIED)
+    oprm.rhs_correct = rc
+    ref = sc.copy()
+    r_o, P_o, P64_o = O.project(ref, oprm)
+    assert rel_l2(P64, P64_o) < 1e-6
+
+
+def ctypes_double(v):
+    import ctypes
+    return ctypes.byref(ctypes.c_double(v))
+
+
+def test_warm_start_reaches_same_answer(ctx):
+    say("warm start")
+    sc = S.make_scene(S.SCENE_DROP, 48, seed=3)
+    prm = S.default_params(residual=1e-9, precision=S.PREC_MIXED, warm_start=1)
+    r1, P1, Q1 = ctx.project(sc.copy(), prm, pressure64=True, scene_id=77)
+    sc2 = S.make_scene(S.SCENE_DROP, 48, seed=3, shift=(0.0, -0.01, 0.0), velocity_seed=99)
+    r2, P2, Q2 = ctx.project(sc2.copy(), prm, pressure64=True, scene_id=77)
+    r3, P3, Q3 = ctx.project(sc2.copy(), S.default_params(residual=1e-9, precision=S.PREC_MIXED), pressure64=True)
+    assert r2.converged and r3.converged
+    A = O.assemble(sc2)
+    assert rel_l2(O.gather(A, Q2), O.gather(A, Q3)) < 1e-5
+    ctx.release_scene(77)
+
+
+def test_determinism(ctx):
+    say("determinism")
+    sc = S.make_scene(S.SCENE_SPHERES_BOX, 48, seed=2)
+    a = run_gpu(ctx, sc, residual=1e-7)
+    b = run_gpu(ctx, sc, residual=1e-7)
+    assert a[0].iterations == b[0].iterations
+    assert np.array_equal(a[2], b[2]) and np.array_equal(a[3].u, b[3].u)
+
+
+def test_nan_velocity_is_reported(ctx):
+    say("nan input")
+    sc = S.make_scene(S.SCENE_DROP, 16, seed=0)
+    sc.u[8, 2, 8] = np.nan
+    r, P, P64 = ctx.project(sc, S.default_params(), check=False)
+    assert r.status == -6
+
+
+def test_budget_refusal():
+    say("budget")
+    c = S.Context(0, hbm_budget_bytes=1 << 20)
+    sc = S.make_scene(S.SCENE_DROP, 64, seed=0)
+    r, P, P64 = c.project(sc, S.default_params(), check=False)
+    assert r.status == -4
+    c.close()
