@@ -65,11 +65,11 @@ struct SolveScalars {
     int32_t pad[2];
 };
 
-// Deterministic grid-wide reduction: every block writes its partials, the last block to finish
-// (found with one atomic counter) sums them in block order and resets the counter.
+// Deterministic two-pass reduction: every block of a reducing kernel stores two partials (plain
+// stores, no atomics); a one-block finalize kernel combines them in block order and updates the
+// solve's scalars.
 struct RedBuf {
-    double *partials;   // 2 per block
-    unsigned *counter;  // 1
+    double *partials;  // 2 per block
 };
 
 // Remaps a linear block id so that each of the 8 XCDs (which receive blocks round-robin) gets a
@@ -114,39 +114,11 @@ __device__ __forceinline__ double block_max(double v, double *lds) {
     return s;
 }
 
-// Publishes this block's two partials and tells whether this block finished last.
-__device__ __forceinline__ bool publish_partials(const RedBuf &R, double a, double b, int *lds_flag) {
+__device__ __forceinline__ void store_partials(const RedBuf &R, double a, double b) {
     if (threadIdx.x == 0) {
         R.partials[2 * blockIdx.x + 0] = a;
         R.partials[2 * blockIdx.x + 1] = b;
-        __threadfence();
-        unsigned prev = atomicAdd(R.counter, 1u);
-        *lds_flag = (prev == gridDim.x - 1) ? 1 : 0;
     }
-    __syncthreads();
-    return *lds_flag != 0;
-}
-
-__device__ __forceinline__ double load_agent(const double *p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// In the last block: combine all partials in a fixed order.  kind 0 = sum, 1 = max.
-__device__ __forceinline__ void combine_partials(const RedBuf &R, int kind_a, int kind_b, double *out_a, double *out_b,
-                                                 double *lds) {
-    __threadfence();
-    double a = kind_a ? 0.0 : 0.0, b = 0.0;
-    for (int i = threadIdx.x; i < (int)gridDim.x; i += blockDim.x) {
-        double pa = load_agent(&R.partials[2 * i + 0]);
-        double pb = load_agent(&R.partials[2 * i + 1]);
-        a = kind_a ? fmax(a, pa) : a + pa;
-        b = kind_b ? fmax(b, pb) : b + pb;
-    }
-    a = kind_a ? block_max(a, lds) : block_sum(a, lds);
-    b = kind_b ? block_max(b, lds) : block_sum(b, lds);
-    *out_a = a;
-    *out_b = b;
-    if (threadIdx.x == 0) *R.counter = 0u;
 }
 
 }  // namespace shkz

@@ -281,13 +281,11 @@ __global__ void k_count_unknowns(const void *dir, int is64, size_t n, unsigned l
 }
 
 // ------------------------------------------------------------------------------ Krylov
-// r = b - A x (FP64) with ||r||_inf and ||r||_2.  `first` also fixes the tolerance from ||r0||.
+// r = b - A x with per-block ||r||_inf and ||r||^2 partials (finalized by FIN_DEFECT*).
 template <class T>
 __global__ __launch_bounds__(BLOCK) void k_defect(LevelDesc L, OpDesc<T> A, const T *__restrict__ b,
-                                                  const T *__restrict__ x, T *__restrict__ r, SolveScalars *S, RedBuf R,
-                                                  int first, double residual) {
+                                                  const T *__restrict__ x, T *__restrict__ r, RedBuf R) {
     __shared__ double lds[16];
-    __shared__ int flag;
     const int br = brick_or_idle(L);
     double mx = 0.0, sq = 0.0;
     if (br >= 0) {
@@ -308,34 +306,14 @@ __global__ __launch_bounds__(BLOCK) void k_defect(LevelDesc L, OpDesc<T> A, cons
     }
     mx = block_max(mx, lds);
     sq = block_sum(sq, lds);
-    if (publish_partials(R, mx, sq, &flag)) {
-        double rinf, rsq;
-        combine_partials(R, 1, 0, &rinf, &rsq, lds);
-        if (threadIdx.x == 0) {
-            const double rl2 = sqrt(rsq);
-            S->rinf = rinf;
-            S->rl2 = rl2;
-            if (first) {
-                S->r0inf = rinf;
-                S->r0l2 = rl2;
-                const double r0 = S->norm == 0 ? rinf : rl2;
-                S->tol = residual * r0;
-                S->iter = 0;
-                S->status = (rinf != rinf) ? 1 : 0;
-                S->converged = (r0 == 0.0) ? 1 : 0;
-                S->done = (r0 == 0.0 || rinf != rinf) ? 1 : 0;
-            }
-        }
-    }
+    store_partials(R, mx, sq);
 }
 
-// Inner (FP32) solve set-up of the defect correction: r32 = (float) r64, c = 0, tolerance
-// eta * ||r32||.
+// Inner (FP32) solve set-up of the defect correction: r32 = (float) r64, c = 0, with
+// ||r32|| partials (finalized by FIN_INNER_INIT).
 __global__ __launch_bounds__(BLOCK) void k_inner_init(LevelDesc L, const double *__restrict__ r64, float *__restrict__ r32,
-                                                      float *__restrict__ corr, SolveScalars *S, RedBuf R, double eta,
-                                                      int maxit) {
+                                                      float *__restrict__ corr, RedBuf R) {
     __shared__ double lds[16];
-    __shared__ int flag;
     const int br = brick_or_idle(L);
     double mx = 0.0, sq = 0.0;
     if (br >= 0) {
@@ -348,31 +326,14 @@ __global__ __launch_bounds__(BLOCK) void k_inner_init(LevelDesc L, const double 
     }
     mx = block_max(mx, lds);
     sq = block_sum(sq, lds);
-    if (publish_partials(R, mx, sq, &flag)) {
-        double rinf, rsq;
-        combine_partials(R, 1, 0, &rinf, &rsq, lds);
-        if (threadIdx.x == 0) {
-            const double rl2 = sqrt(rsq);
-            S->rinf = rinf;
-            S->rl2 = rl2;
-            S->r0inf = rinf;
-            S->r0l2 = rl2;
-            S->tol = eta * (S->norm == 0 ? rinf : rl2);
-            S->iter = 0;
-            S->maxit = maxit;
-            S->status = 0;
-            S->converged = (rinf == 0.0) ? 1 : 0;
-            S->done = (rinf == 0.0 || maxit <= 0) ? 1 : 0;
-        }
-    }
+    store_partials(R, mx, sq);
 }
 
-// q = A p and p.q; the last block forms alpha = (r.z) / (p.q).
+// q = A p and p.q partials (FIN_PQ forms alpha = (r.z) / (p.q)).
 template <class T>
 __global__ __launch_bounds__(BLOCK) void k_apply_dot(LevelDesc L, OpDesc<T> A, const T *__restrict__ p,
-                                                     T *__restrict__ q, SolveScalars *S, RedBuf R) {
+                                                     T *__restrict__ q, const SolveScalars *S, RedBuf R) {
     __shared__ double lds[16];
-    __shared__ int flag;
     if (S->done) return;
     const int br = brick_or_idle(L);
     double part = 0.0;
@@ -393,29 +354,16 @@ __global__ __launch_bounds__(BLOCK) void k_apply_dot(LevelDesc L, OpDesc<T> A, c
         q[c] = out;
     }
     part = block_sum(part, lds);
-    if (publish_partials(R, part, 0.0, &flag)) {
-        double pq, unused;
-        combine_partials(R, 0, 0, &pq, &unused, lds);
-        if (threadIdx.x == 0) {
-            S->pq = pq;
-            const double a = S->rz / pq;
-            S->alpha = a;
-            if (!(a == a) || a == 1.0 / 0.0 || a == -1.0 / 0.0) {
-                S->status = 1;
-                S->done = 1;
-            }
-        }
-    }
+    store_partials(R, part, 0.0);
 }
 
-// x += alpha p, r -= alpha q, then the convergence test on ||r|| (pcg_solver.h:278-285).
-// plain_cg: z = r, so r.r is the next r.z and beta follows here.
+// x += alpha p, r -= alpha q with ||r||_inf and r.r partials (FIN_UPDATE runs the convergence
+// test of pcg_solver.h:278-285).
 template <class T>
 __global__ __launch_bounds__(BLOCK) void k_cg_update(LevelDesc L, T *__restrict__ x, T *__restrict__ r,
-                                                     const T *__restrict__ p, const T *__restrict__ q, SolveScalars *S,
-                                                     RedBuf R, int plain_cg) {
+                                                     const T *__restrict__ p, const T *__restrict__ q,
+                                                     const SolveScalars *S, RedBuf R) {
     __shared__ double lds[16];
-    __shared__ int flag;
     if (S->done) return;
     const int br = brick_or_idle(L);
     double mx = 0.0, sq = 0.0;
@@ -430,39 +378,14 @@ __global__ __launch_bounds__(BLOCK) void k_cg_update(LevelDesc L, T *__restrict_
     }
     mx = block_max(mx, lds);
     sq = block_sum(sq, lds);
-    if (publish_partials(R, mx, sq, &flag)) {
-        double rinf, rsq;
-        combine_partials(R, 1, 0, &rinf, &rsq, lds);
-        if (threadIdx.x == 0) {
-            const double rl2 = sqrt(rsq);
-            S->rinf = rinf;
-            S->rl2 = rl2;
-            const int it = S->iter + 1;
-            S->iter = it;
-            const double cur = S->norm == 0 ? rinf : rl2;
-            if (!(cur == cur)) {
-                S->status = 1;
-                S->done = 1;
-            } else if (cur <= S->tol) {
-                S->converged = 1;
-                S->done = 1;
-            } else if (it >= S->maxit) {
-                S->done = 1;
-            }
-            if (plain_cg) {
-                S->beta = rsq / S->rz;
-                S->rz = rsq;
-            }
-        }
-    }
+    store_partials(R, mx, sq);
 }
 
-// r.z for the next direction: beta = (r.z)_new / (r.z)_old.  first: rz only.
+// r.z partials for the next direction (FIN_RZ forms beta).
 template <class T>
 __global__ __launch_bounds__(BLOCK) void k_dot_rz(LevelDesc L, const T *__restrict__ r, const T *__restrict__ z,
-                                                  SolveScalars *S, RedBuf R, int first) {
+                                                  const SolveScalars *S, RedBuf R) {
     __shared__ double lds[16];
-    __shared__ int flag;
     if (S->done) return;
     const int br = brick_or_idle(L);
     double part = 0.0;
@@ -471,17 +394,105 @@ __global__ __launch_bounds__(BLOCK) void k_dot_rz(LevelDesc L, const T *__restri
         part = (double)r[c] * (double)z[c];
     }
     part = block_sum(part, lds);
-    if (publish_partials(R, part, 0.0, &flag)) {
-        double rz, unused;
-        combine_partials(R, 0, 0, &rz, &unused, lds);
-        if (threadIdx.x == 0) {
-            S->beta = first ? 0.0 : rz / S->rz;
-            S->rz = rz;
-            if (!(rz == rz) || (first && rz == 0.0)) {
-                S->status = (rz == rz) ? 0 : 1;
-                S->done = 1;
-            }
+    store_partials(R, part, 0.0);
+}
+
+enum FinOp {
+    FIN_PQ = 0,           // alpha = rz / pq
+    FIN_UPDATE = 1,       // rinf, rl2, iteration count, convergence test
+    FIN_UPDATE_CG = 2,    // FIN_UPDATE plus beta = r.r / rz (unpreconditioned CG)
+    FIN_RZ_FIRST = 3,     // rz of the first direction
+    FIN_RZ = 4,           // beta = rz_new / rz
+    FIN_DEFECT_FIRST = 5, // ||r0|| and the absolute tolerance  (param = relative residual)
+    FIN_DEFECT = 6,       // ||r|| only
+    FIN_INNER_INIT = 7    // inner solve: ||r0||, tol = param * ||r0||, maxit = iparam
+};
+
+// One block combines the partials of `nblocks` blocks in block order and updates S.
+__global__ __launch_bounds__(1024) void k_finalize(const double *__restrict__ partials, int nblocks, SolveScalars *S,
+                                                   int op, double param, int iparam) {
+    __shared__ double lds[16];
+    const bool always = (op == FIN_DEFECT_FIRST || op == FIN_DEFECT || op == FIN_INNER_INIT);
+    if (!always && S->done) return;
+    const bool a_is_max = (op == FIN_UPDATE || op == FIN_UPDATE_CG || op == FIN_DEFECT_FIRST || op == FIN_DEFECT ||
+                           op == FIN_INNER_INIT);
+    double a = 0.0, b = 0.0;
+    for (int i = threadIdx.x; i < nblocks; i += blockDim.x) {
+        const double pa = partials[2 * i], pb = partials[2 * i + 1];
+        a = a_is_max ? fmax(a, pa) : a + pa;
+        b += pb;
+    }
+    a = a_is_max ? block_max(a, lds) : block_sum(a, lds);
+    b = block_sum(b, lds);
+    if (threadIdx.x != 0) return;
+    switch (op) {
+    case FIN_PQ: {
+        S->pq = a;
+        const double al = S->rz / a;
+        S->alpha = al;
+        if (!(al == al) || al - al != 0.0) {
+            S->status = 1;
+            S->done = 1;
         }
+        break;
+    }
+    case FIN_UPDATE:
+    case FIN_UPDATE_CG: {
+        const double rl2 = sqrt(b);
+        S->rinf = a;
+        S->rl2 = rl2;
+        const int it = S->iter + 1;
+        S->iter = it;
+        const double cur = S->norm == 0 ? a : rl2;
+        if (!(b == b)) {
+            S->status = 1;
+            S->done = 1;
+        } else if (cur <= S->tol) {
+            S->converged = 1;
+            S->done = 1;
+        } else if (it >= S->maxit) {
+            S->done = 1;
+        }
+        if (op == FIN_UPDATE_CG) {
+            S->beta = b / S->rz;
+            S->rz = b;
+        }
+        break;
+    }
+    case FIN_RZ_FIRST:
+    case FIN_RZ: {
+        S->beta = op == FIN_RZ_FIRST ? 0.0 : a / S->rz;
+        S->rz = a;
+        if (!(a == a)) {
+            S->status = 1;
+            S->done = 1;
+        } else if (op == FIN_RZ_FIRST && a == 0.0) {
+            S->done = 1;
+        }
+        break;
+    }
+    case FIN_DEFECT_FIRST:
+    case FIN_DEFECT:
+    case FIN_INNER_INIT: {
+        const double rl2 = sqrt(b);
+        S->rinf = a;
+        S->rl2 = rl2;
+        if (op != FIN_DEFECT) {
+            S->r0inf = a;
+            S->r0l2 = rl2;
+            const double r0 = S->norm == 0 ? a : rl2;
+            S->tol = param * r0;
+            S->iter = 0;
+            if (op == FIN_INNER_INIT) S->maxit = iparam;
+            const bool bad = !(b == b);
+            S->status = bad ? 1 : 0;
+            S->converged = (r0 == 0.0) ? 1 : 0;
+            S->done = (r0 == 0.0 || bad || S->maxit <= 0) ? 1 : 0;
+        }
+        break;
+    }
+    default:
+        break;
     }
 }
 
@@ -582,27 +593,37 @@ __device__ __forceinline__ T level_value(const LevelDesc &L, const T *__restrict
 }
 
 // Restriction = transpose of trilinear prolongation: coarse cell I gathers fine cells
-// 2I-1..2I+2 per axis with weights 1/4, 3/4, 3/4, 1/4.
+// 2I-1..2I+2 per axis with weights 1/4, 3/4, 3/4, 1/4.  One block per coarse brick stages the
+// 18^3 fine residuals it needs in LDS.
 template <class T>
 __global__ __launch_bounds__(BLOCK) void k_restrict(LevelDesc Lf, LevelDesc Lc, OpDesc<T> Ac, const T *__restrict__ res,
                                                     T *__restrict__ rhs_c, const SolveScalars *S) {
+    constexpr int E = 18;
+    __shared__ T tile[E * E * E];
     if (S && S->done) return;
     const int br = brick_or_idle(Lc);
     if (br < 0) return;
     const int t = threadIdx.x;
+    const int BX = Lc.bxyz[3 * br], BY = Lc.bxyz[3 * br + 1], BZ = Lc.bxyz[3 * br + 2];
+    const int f0 = 16 * BX - 1, g0 = 16 * BY - 1, h0 = 16 * BZ - 1;
+    for (int e = t; e < E * E * E; e += BLOCK) {
+        const int c = e % E, b = (e / E) % E, a = e / (E * E);
+        tile[e] = level_value(Lf, res, f0 + a, g0 + b, h0 + c);
+    }
+    __syncthreads();
     const long c = (long)br * BRICK_CELLS + t;
     T out = 0;
     if (Ac.dir[c] >= 0) {
-        const int I = Lc.bxyz[3 * br] * BRICK + (t >> 6), J = Lc.bxyz[3 * br + 1] * BRICK + ((t >> 3) & 7),
-                  K = Lc.bxyz[3 * br + 2] * BRICK + (t & 7);
+        const int li = t >> 6, lj = (t >> 3) & 7, lk = t & 7;
         const T w[4] = {T(0.25), T(0.75), T(0.75), T(0.25)};
         T acc = 0;
+#pragma unroll
         for (int a = 0; a < 4; ++a) {
             T acc_a = 0;
+#pragma unroll
             for (int b = 0; b < 4; ++b) {
-                T acc_b = 0;
-                for (int cc = 0; cc < 4; ++cc)
-                    acc_b += w[cc] * level_value(Lf, res, 2 * I - 1 + a, 2 * J - 1 + b, 2 * K - 1 + cc);
+                const T *row = &tile[((2 * li + a) * E + (2 * lj + b)) * E + 2 * lk];
+                const T acc_b = w[0] * row[0] + w[1] * row[1] + w[2] * row[2] + w[3] * row[3];
                 acc_a += w[b] * acc_b;
             }
             acc += w[a] * acc_a;
@@ -612,25 +633,32 @@ __global__ __launch_bounds__(BLOCK) void k_restrict(LevelDesc Lf, LevelDesc Lc, 
     rhs_c[c] = out;
 }
 
-// z_f += trilinear interpolation of the coarse correction (fine unknowns only).
+// z_f += trilinear interpolation of the coarse correction (fine unknowns only).  One block per
+// fine brick stages the 6^3 coarse values it needs in LDS.
 template <class T>
 __global__ __launch_bounds__(BLOCK) void k_prolong(LevelDesc Lf, OpDesc<T> Af, LevelDesc Lc, const T *__restrict__ zc,
                                                    T *__restrict__ zf, const SolveScalars *S) {
+    constexpr int E = 6;
+    __shared__ T tile[E * E * E];
     if (S && S->done) return;
     const int br = brick_or_idle(Lf);
     if (br < 0) return;
     const int t = threadIdx.x;
+    const int BX = Lf.bxyz[3 * br], BY = Lf.bxyz[3 * br + 1], BZ = Lf.bxyz[3 * br + 2];
+    if (t < E * E * E) {
+        const int c = t % E, b = (t / E) % E, a = t / (E * E);
+        tile[t] = level_value(Lc, zc, 4 * BX - 1 + a, 4 * BY - 1 + b, 4 * BZ - 1 + c);
+    }
+    __syncthreads();
     const long c = (long)br * BRICK_CELLS + t;
     if (!(Af.dir[c] >= 0)) return;
-    const int i = Lf.bxyz[3 * br] * BRICK + (t >> 6), j = Lf.bxyz[3 * br + 1] * BRICK + ((t >> 3) & 7),
-              k = Lf.bxyz[3 * br + 2] * BRICK + (t & 7);
-    const int I0 = i >> 1, J0 = j >> 1, K0 = k >> 1;
-    const int I1 = (i & 1) ? I0 + 1 : I0 - 1, J1 = (j & 1) ? J0 + 1 : J0 - 1, K1 = (k & 1) ? K0 + 1 : K0 - 1;
+    const int li = t >> 6, lj = (t >> 3) & 7, lk = t & 7;
+    const int a0 = (li >> 1) + 1, b0 = (lj >> 1) + 1, c0 = (lk >> 1) + 1;
+    const int a1 = (li & 1) ? a0 + 1 : a0 - 1, b1 = (lj & 1) ? b0 + 1 : b0 - 1, c1 = (lk & 1) ? c0 + 1 : c0 - 1;
+    auto at = [&](int a, int b, int cc) { return tile[(a * E + b) * E + cc]; };
     const T n = T(0.75), f = T(0.25);
-    const T v = n * (n * (n * level_value(Lc, zc, I0, J0, K0) + f * level_value(Lc, zc, I0, J0, K1)) +
-                     f * (n * level_value(Lc, zc, I0, J1, K0) + f * level_value(Lc, zc, I0, J1, K1))) +
-                f * (n * (n * level_value(Lc, zc, I1, J0, K0) + f * level_value(Lc, zc, I1, J0, K1)) +
-                     f * (n * level_value(Lc, zc, I1, J1, K0) + f * level_value(Lc, zc, I1, J1, K1)));
+    const T v = n * (n * (n * at(a0, b0, c0) + f * at(a0, b0, c1)) + f * (n * at(a0, b1, c0) + f * at(a0, b1, c1))) +
+                f * (n * (n * at(a1, b0, c0) + f * at(a1, b0, c1)) + f * (n * at(a1, b1, c0) + f * at(a1, b1, c1)));
     zf[c] += v;
 }
 

@@ -226,7 +226,6 @@ struct shkz_ctx {
     std::vector<TimedLaunch> pending;
     std::vector<hipEvent_t> event_pool;
     std::vector<KernelTotals> totals;
-    int red_blocks_cap = 0;
 };
 
 namespace shkz {
@@ -375,38 +374,48 @@ static int read_scalars(shkz_ctx *C, hipStream_t s, const SolveScalars *dS, Solv
     return SHKZ_OK;
 }
 
+static void finalize(Launcher &K, const Problem &P, unsigned nblocks, SolveScalars *S, int op, double param = 0.0,
+                     int iparam = 0) {
+    K.go("finalize", -1, 0, 16.0 * nblocks, dim3(1), dim3(1024), k_finalize, (const double *)P.red.partials,
+         (int)nblocks, S, op, param, iparam);
+}
+
 // Krylov loop on level 0: T = element type; precond = V-cycle or identity (plain CG).
 // Expects r (residual), and S with tol/maxit/iter set.  x is updated in place.
 template <class T>
 static int krylov(shkz_ctx *C, Launcher &K, Problem &P, T *x, T *r, T *p, T *q, T *z, T *tmp, SolveScalars *S,
                   bool precond, SolveScalars *hs) {
     LevelHost &L0 = C->levels[0];
-    const dim3 g(brick_grid(L0.nbricks)), b(BLOCK);
+    const unsigned nb = brick_grid(L0.nbricks);
+    const dim3 g(nb), b(BLOCK);
     const OpDesc<T> A = op_of<T>(L0);
     const double es = (double)sizeof(T);
     // first direction
     const T *zp = r;
     if (precond) zp = vcycle<T>(C, K, P, r, z, tmp, S);
-    K.go("dot_rz", 0, 2 * es, 0, g, b, k_dot_rz<T>, L0.desc, (const T *)r, zp, S, P.red, 1);
+    K.go("dot_rz", 0, 2 * es, 0, g, b, k_dot_rz<T>, L0.desc, (const T *)r, zp, (const SolveScalars *)S, P.red);
+    finalize(K, P, nb, S, FIN_RZ_FIRST);
     K.go("update_p", 0, 2 * es, 0, g, b, k_update_p<T>, L0.desc, zp, p, (const SolveScalars *)S, 1);
     int chunk = 4;
-    int launched = 0;
-    double last_res = -1.0;
     for (;;) {
         for (int it = 0; it < chunk; ++it) {
-            K.go("apply_dot", 0, 6 * es, 0, g, b, k_apply_dot<T>, L0.desc, A, (const T *)p, q, S, P.red);
-            K.go("cg_update", 0, 6 * es, 0, g, b, k_cg_update<T>, L0.desc, x, r, (const T *)p, (const T *)q, S, P.red,
-                 precond ? 0 : 1);
+            K.go("apply_dot", 0, 6 * es, 0, g, b, k_apply_dot<T>, L0.desc, A, (const T *)p, q, (const SolveScalars *)S,
+                 P.red);
+            finalize(K, P, nb, S, FIN_PQ);
+            K.go("cg_update", 0, 6 * es, 0, g, b, k_cg_update<T>, L0.desc, x, r, (const T *)p, (const T *)q,
+                 (const SolveScalars *)S, P.red);
+            finalize(K, P, nb, S, precond ? FIN_UPDATE : FIN_UPDATE_CG);
             if (precond) {
                 zp = vcycle<T>(C, K, P, r, z, tmp, S);
-                K.go("dot_rz", 0, 2 * es, 0, g, b, k_dot_rz<T>, L0.desc, (const T *)r, zp, S, P.red, 0);
+                K.go("dot_rz", 0, 2 * es, 0, g, b, k_dot_rz<T>, L0.desc, (const T *)r, zp, (const SolveScalars *)S,
+                     P.red);
+                finalize(K, P, nb, S, FIN_RZ);
                 K.go("update_p", 0, 3 * es, 0, g, b, k_update_p<T>, L0.desc, zp, p, (const SolveScalars *)S, 0);
             } else {
                 K.go("update_p", 0, 3 * es, 0, g, b, k_update_p<T>, L0.desc, (const T *)r, p, (const SolveScalars *)S,
                      0);
             }
         }
-        launched += chunk;
         SHKZ_TRY(read_scalars(C, K.s, S, hs));
         if (hs->done) break;
         // size the next chunk from the observed contraction rate
@@ -415,17 +424,11 @@ static int krylov(shkz_ctx *C, Launcher &K, Problem &P, T *x, T *r, T *p, T *q, 
         int next = chunk;
         if (hs->iter > 0 && cur > 0 && r0 > 0 && cur < r0) {
             const double rate = std::pow(cur / r0, 1.0 / hs->iter);
-            if (rate > 0 && rate < 1) {
-                const double need = std::log(hs->tol / cur) / std::log(rate);
-                next = (int)std::ceil(need);
-            }
+            if (rate > 0 && rate < 1) next = (int)std::ceil(std::log(hs->tol / cur) / std::log(rate));
         }
-        (void)last_res;
-        last_res = cur;
         chunk = std::max(1, std::min(next, 64));
         if (hs->iter + chunk > hs->maxit) chunk = std::max(1, hs->maxit - hs->iter);
     }
-    (void)launched;
     return SHKZ_OK;
 }
 
@@ -582,13 +585,6 @@ static int project_impl(shkz_ctx *C, const shkz_grid *grid, const shkz_params *p
     }
     // reduction scratch and scalars
     SHKZ_TRY(arena_get(AR, "partials", (size_t)2 * max_blocks, &P.red.partials));
-    if (C->red_blocks_cap == 0) {
-        SHKZ_TRY(arena_get(AR, "counter", 16, &P.red.counter));
-        SHKZ_HIP(hipMemsetAsync(P.red.counter, 0, 16 * sizeof(unsigned), s));
-        C->red_blocks_cap = 1;
-    } else {
-        SHKZ_TRY(arena_get(AR, "counter", 16, &P.red.counter));
-    }
     SolveScalars *Sbuf = nullptr;
     SHKZ_TRY(arena_get(AR, "scalars", 2, &Sbuf));
     P.S_out = Sbuf;
@@ -754,7 +750,8 @@ static int project_impl(shkz_ctx *C, const shkz_grid *grid, const shkz_params *p
     } else if (!P.mixed) {
         // one FP64 Krylov solve: CG (reference iteration) or MGPCG
         K.go("defect", 0, 7 * 8, 0, gb, bb, k_defect<double>, F0.desc, A64, (const double *)P.b64,
-             (const double *)P.x64, P.r64, P.S_out, P.red, 1, residual);
+             (const double *)P.x64, P.r64, P.red);
+        finalize(K, P, g0, P.S_out, FIN_DEFECT_FIRST, residual);
         SHKZ_TRY(read_scalars(C, s, P.S_out, &hs));
         if (!hs.done) {
             SHKZ_TRY(krylov<double>(C, K, P, P.x64, P.r64, (double *)P.kp, (double *)P.kq, (double *)P.kz,
@@ -772,7 +769,8 @@ static int project_impl(shkz_ctx *C, const shkz_grid *grid, const shkz_params *p
         double prev = -1.0;
         for (outer = 0; outer < 64; ++outer) {
             K.go("defect", 0, 7 * 8, 0, gb, bb, k_defect<double>, F0.desc, A64, (const double *)P.b64,
-                 (const double *)P.x64, P.r64, P.S_out, P.red, outer == 0 ? 1 : 0, residual);
+                 (const double *)P.x64, P.r64, P.red);
+            finalize(K, P, g0, P.S_out, outer == 0 ? FIN_DEFECT_FIRST : FIN_DEFECT, residual);
             SHKZ_TRY(read_scalars(C, s, P.S_out, &hs));
             if (hs.status) {
                 status = SHKZ_ERR_NUMERIC;
@@ -791,7 +789,9 @@ static int project_impl(shkz_ctx *C, const shkz_grid *grid, const shkz_params *p
             if (prev >= 0 && cur > 0.9 * prev) break;  // FP32 inner solves no longer reduce the defect
             prev = cur;
             K.go("inner_init", 0, 16, 0, gb, bb, k_inner_init, F0.desc, (const double *)P.r64, (float *)P.kr,
-                 (float *)P.kx, P.S_in, P.red, std::max(eta, hs.tol / std::max(cur, 1e-300)), maxit - total_iters);
+                 (float *)P.kx, P.red);
+            finalize(K, P, g0, P.S_in, FIN_INNER_INIT, std::max(eta, hs.tol / std::max(cur, 1e-300)),
+                     maxit - total_iters);
             SHKZ_TRY(read_scalars(C, s, P.S_in, &hi));
             if (!hi.done) {
                 SHKZ_TRY(krylov<float>(C, K, P, (float *)P.kx, (float *)P.kr, (float *)P.kp, (float *)P.kq,
@@ -810,7 +810,8 @@ static int project_impl(shkz_ctx *C, const shkz_grid *grid, const shkz_params *p
     if (res->n_unknowns > 0) {
         SolveScalars hf{};
         K.go("defect", 0, 7 * 8, 0, gb, bb, k_defect<double>, F0.desc, A64, (const double *)P.b64,
-             (const double *)P.x64, P.r64, P.S_in, P.red, 0, residual);
+             (const double *)P.x64, P.r64, P.red);
+        finalize(K, P, g0, P.S_in, FIN_DEFECT);
         SHKZ_TRY(read_scalars(C, s, P.S_in, &hf));
         const double r0i = hs.r0inf > 0 ? hs.r0inf : 1.0, r0l = hs.r0l2 > 0 ? hs.r0l2 : 1.0;
         res->reresid_inf = hf.rinf / r0i;

