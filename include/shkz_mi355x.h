@@ -107,7 +107,10 @@ typedef struct shkz_params {
     int32_t mg_coarse_sweeps;
     double mg_omega;
     double inner_residual;        /* mixed precision: relative reduction per inner FP32 solve */
-    int32_t reserved[8];
+    int32_t mg_coarse_pre_sweeps; /* sweeps on the coarser levels (mg_pre/post_sweeps: finest) */
+    int32_t mg_coarse_post_sweeps;
+    int32_t mg_bottom_bricks;     /* levels with at most this many bricks run in one workgroup */
+    int32_t reserved[5];
 } shkz_params;
 
 /* ---------------------------------------------------------------- result */

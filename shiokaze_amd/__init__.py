@@ -54,7 +54,9 @@ class Params(ctypes.Structure):
         ("warm_start", ctypes.c_int32),
         ("mg_pre_sweeps", ctypes.c_int32), ("mg_post_sweeps", ctypes.c_int32), ("mg_coarse_sweeps", ctypes.c_int32),
         ("mg_omega", ctypes.c_double), ("inner_residual", ctypes.c_double),
-        ("reserved", ctypes.c_int32 * 8),
+        ("mg_coarse_pre_sweeps", ctypes.c_int32), ("mg_coarse_post_sweeps", ctypes.c_int32),
+        ("mg_bottom_bricks", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 5),
     ]
 
 
@@ -132,7 +134,7 @@ def _ptr(a):
     if isinstance(a, np.ndarray):
         assert a.flags["C_CONTIGUOUS"]
         return a.ctypes.data
-    return int(a)  # device pointer (int) or torch tensor data_ptr()
+    return int(a)  # device pointer as an int
 
 
 def last_error() -> str:
@@ -246,7 +248,7 @@ class Context:
 
     def project_device(self, w, h, d, dt, fluid, fluid_dtype, solid, solid_dtype, u, v, wv, params=None,
                        pressure=None, pressure64=None, scene_id=0, stream=None, dx=0.0, check=True):
-        """Projection on device-resident buffers (pointers as ints or torch tensors)."""
+        """Projection on device-resident buffers (ints or objects with data_ptr(), e.g. devmem.DeviceArray)."""
         def dp(x):
             if x is None:
                 return None

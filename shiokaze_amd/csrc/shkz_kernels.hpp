@@ -417,10 +417,22 @@ __global__ __launch_bounds__(1024) void k_finalize(const double *__restrict__ pa
     const bool a_is_max = (op == FIN_UPDATE || op == FIN_UPDATE_CG || op == FIN_DEFECT_FIRST || op == FIN_DEFECT ||
                            op == FIN_INNER_INIT);
     double a = 0.0, b = 0.0;
-    for (int i = threadIdx.x; i < nblocks; i += blockDim.x) {
-        const double pa = partials[2 * i], pb = partials[2 * i + 1];
-        a = a_is_max ? fmax(a, pa) : a + pa;
-        b += pb;
+    const double2 *pp = reinterpret_cast<const double2 *>(partials);
+    int i = threadIdx.x;
+    for (; i + 3 * (int)blockDim.x < nblocks; i += 4 * blockDim.x) {
+        double2 v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) v[u] = pp[i + u * blockDim.x];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            a = a_is_max ? fmax(a, v[u].x) : a + v[u].x;
+            b += v[u].y;
+        }
+    }
+    for (; i < nblocks; i += blockDim.x) {
+        const double2 v = pp[i];
+        a = a_is_max ? fmax(a, v.x) : a + v.x;
+        b += v.y;
     }
     a = a_is_max ? block_max(a, lds) : block_sum(a, lds);
     b = block_sum(b, lds);
@@ -517,11 +529,11 @@ __global__ __launch_bounds__(BLOCK) void k_add_correction(LevelDesc L, double *_
 }
 
 // ------------------------------------------------------------------------------ V-cycle
-// z = omega * rhs / diag (first damped-Jacobi sweep from a zero guess).
+// The smoothers use a stored inverse diagonal: damped Jacobi is
+//   z' = (1 - omega) z + omega * idiag * (rhs + sum_f g_f z_q),
+// and idiag = 0 marks a non-unknown (or a cell whose diagonal vanished).
 template <class T>
-__global__ __launch_bounds__(BLOCK) void k_smooth_first(LevelDesc L, OpDesc<T> A, const T *__restrict__ rhs,
-                                                        T *__restrict__ z, T omega, const SolveScalars *S) {
-    if (S && S->done) return;
+__global__ __launch_bounds__(BLOCK) void k_idiag(LevelDesc L, OpDesc<T> A, T *__restrict__ idiag) {
     const int br = brick_or_idle(L);
     if (br < 0) return;
     const int t = threadIdx.x;
@@ -534,37 +546,95 @@ __global__ __launch_bounds__(BLOCK) void k_smooth_first(LevelDesc L, OpDesc<T> A
         nb_index(N, t, idx);
         const T diag = dirc + ((A.gx[c] + ld(A.gx, idx[NB_XP])) + (A.gy[c] + ld(A.gy, idx[NB_YP])) +
                                (A.gz[c] + ld(A.gz, idx[NB_ZP])));
-        if (diag > 0) out = omega * rhs[c] / diag;
+        if (diag > 0) out = T(1) / diag;
     }
-    z[c] = out;
+    idiag[c] = out;
 }
 
-// zout = zin + omega (rhs - A zin) / diag
+// Sum of g_f * v_q over the six faces of cell c.
 template <class T>
-__global__ __launch_bounds__(BLOCK) void k_jacobi(LevelDesc L, OpDesc<T> A, const T *__restrict__ rhs,
-                                                  const T *__restrict__ zin, T *__restrict__ zout, T omega,
-                                                  const SolveScalars *S) {
+__device__ __forceinline__ T offdiag_sum(const OpDesc<T> &A, const T *__restrict__ v, long c, const long idx[6]) {
+    const T gxp = ld(A.gx, idx[NB_XP]), gyp = ld(A.gy, idx[NB_YP]), gzp = ld(A.gz, idx[NB_ZP]);
+    return A.gx[c] * ld(v, idx[NB_XM]) + gxp * ld(v, idx[NB_XP]) + A.gy[c] * ld(v, idx[NB_YM]) +
+           gyp * ld(v, idx[NB_YP]) + A.gz[c] * ld(v, idx[NB_ZM]) + gzp * ld(v, idx[NB_ZP]);
+}
+
+// z = omega * idiag * rhs (one damped-Jacobi sweep from a zero guess).
+// FINEST: 1 on the finest level (a distinct symbol for profiling), 0 on coarse levels.
+template <class T, int FINEST>
+__global__ __launch_bounds__(BLOCK) void k_smooth_first(LevelDesc L, const T *__restrict__ idiag,
+                                                        const T *__restrict__ rhs, T *__restrict__ z, T omega,
+                                                        const SolveScalars *S) {
+    if (S && S->done) return;
+    const int br = brick_or_idle(L);
+    if (br < 0) return;
+    const long c = (long)br * BRICK_CELLS + threadIdx.x;
+    z[c] = omega * idiag[c] * rhs[c];
+}
+
+// Two damped-Jacobi sweeps from a zero guess in one pass: the first sweep's values at the
+// neighbours, omega * idiag_q * rhs_q, are formed on the fly.
+template <class T, int FINEST>
+__global__ __launch_bounds__(BLOCK) void k_smooth2(LevelDesc L, OpDesc<T> A, const T *__restrict__ idiag,
+                                                   const T *__restrict__ rhs, T *__restrict__ z, T omega,
+                                                   const SolveScalars *S) {
     if (S && S->done) return;
     const int br = brick_or_idle(L);
     if (br < 0) return;
     const int t = threadIdx.x;
     const long c = (long)br * BRICK_CELLS + t;
-    const T dirc = A.dir[c];
+    const T id = idiag[c];
     T out = 0;
-    if (dirc >= 0) {
+    if (id > 0) {
         const Nbrs N = load_nbrs(L, br);
         long idx[6];
         nb_index(N, t, idx);
-        const T zc = zin[c];
-        T diag;
-        const T az = apply_row(A, zin, c, idx, dirc, zc, diag);
-        out = diag > 0 ? zc + omega * (rhs[c] - az) / diag : T(0);
+        auto z1 = [&](long i) { return i >= 0 ? omega * idiag[i] * rhs[i] : T(0); };
+        const T gxp = ld(A.gx, idx[NB_XP]), gyp = ld(A.gy, idx[NB_YP]), gzp = ld(A.gz, idx[NB_ZP]);
+        const T off = A.gx[c] * z1(idx[NB_XM]) + gxp * z1(idx[NB_XP]) + A.gy[c] * z1(idx[NB_YM]) +
+                      gyp * z1(idx[NB_YP]) + A.gz[c] * z1(idx[NB_ZM]) + gzp * z1(idx[NB_ZP]);
+        const T r = rhs[c];
+        out = (T(1) - omega) * (omega * id * r) + omega * id * (r + off);
     }
-    zout[c] = out;
+    z[c] = out;
+}
+
+// zout = (1 - omega) zin + omega idiag (rhs + offdiag(zin)).  DOT: also rhs.zout partials (the
+// finest level's last post-smoothing sweep yields r.z of the preconditioned CG for free).
+template <class T, int FINEST, int DOT>
+__global__ __launch_bounds__(BLOCK) void k_jacobi(LevelDesc L, OpDesc<T> A, const T *__restrict__ idiag,
+                                                  const T *__restrict__ rhs, const T *__restrict__ zin,
+                                                  T *__restrict__ zout, T omega, const SolveScalars *S, RedBuf R) {
+    __shared__ double lds[16];
+    if (S && S->done) return;
+    const int br = brick_or_idle(L);
+    double part = 0.0;
+    if (br >= 0) {
+        const int t = threadIdx.x;
+        const long c = (long)br * BRICK_CELLS + t;
+        const T id = idiag[c];
+        T out = 0;
+        if (id > 0) {
+            const Nbrs N = load_nbrs(L, br);
+            long idx[6];
+            nb_index(N, t, idx);
+            const T r = rhs[c];
+            out = (T(1) - omega) * zin[c] + omega * id * (r + offdiag_sum(A, zin, c, idx));
+            if (DOT) part = (double)r * (double)out;
+        }
+        zout[c] = out;
+    } else if (!DOT) {
+        return;
+    }
+    if (DOT) {
+        part = block_sum(part, lds);
+        store_partials(R, part, 0.0);
+    }
 }
 
 // res = rhs - A z
-template <class T>
+// FINEST: 1 on the finest level (a distinct symbol for profiling), 0 on coarse levels.
+template <class T, int FINEST>
 __global__ __launch_bounds__(BLOCK) void k_residual(LevelDesc L, OpDesc<T> A, const T *__restrict__ rhs,
                                                     const T *__restrict__ z, T *__restrict__ res, const SolveScalars *S) {
     if (S && S->done) return;
@@ -595,7 +665,8 @@ __device__ __forceinline__ T level_value(const LevelDesc &L, const T *__restrict
 // Restriction = transpose of trilinear prolongation: coarse cell I gathers fine cells
 // 2I-1..2I+2 per axis with weights 1/4, 3/4, 3/4, 1/4.  One block per coarse brick stages the
 // 18^3 fine residuals it needs in LDS.
-template <class T>
+// FINEST: 1 on the finest level (a distinct symbol for profiling), 0 on coarse levels.
+template <class T, int FINEST>
 __global__ __launch_bounds__(BLOCK) void k_restrict(LevelDesc Lf, LevelDesc Lc, OpDesc<T> Ac, const T *__restrict__ res,
                                                     T *__restrict__ rhs_c, const SolveScalars *S) {
     constexpr int E = 18;
@@ -606,9 +677,24 @@ __global__ __launch_bounds__(BLOCK) void k_restrict(LevelDesc Lf, LevelDesc Lc, 
     const int t = threadIdx.x;
     const int BX = Lc.bxyz[3 * br], BY = Lc.bxyz[3 * br + 1], BZ = Lc.bxyz[3 * br + 2];
     const int f0 = 16 * BX - 1, g0 = 16 * BY - 1, h0 = 16 * BZ - 1;
+    // the 18^3 tile touches fine bricks 2B-1 .. 2B+2 per axis: look them up once
+    __shared__ int bid[64];
+    if (t < 64) {
+        const int a = t >> 4, b = (t >> 2) & 3, cc = t & 3;
+        const int fx = 2 * BX - 1 + a, fy = 2 * BY - 1 + b, fz = 2 * BZ - 1 + cc;
+        int id = -1;
+        if (fx >= 0 && fy >= 0 && fz >= 0 && fx < Lf.nbx && fy < Lf.nby && fz < Lf.nbz)
+            id = Lf.bmap[((long)fx * Lf.nby + fy) * Lf.nbz + fz];
+        bid[t] = id;
+    }
+    __syncthreads();
     for (int e = t; e < E * E * E; e += BLOCK) {
         const int c = e % E, b = (e / E) % E, a = e / (E * E);
-        tile[e] = level_value(Lf, res, f0 + a, g0 + b, h0 + c);
+        const int fi = f0 + a, fj = g0 + b, fk = h0 + c;
+        // (fi + 8) >> 3 avoids shifting -1; slot 0 is brick 2B-1
+        const int slot = ((((fi + 8) >> 3) - 2 * BX) * 4 + (((fj + 8) >> 3) - 2 * BY)) * 4 + (((fk + 8) >> 3) - 2 * BZ);
+        const int id = bid[slot];
+        tile[e] = id >= 0 ? res[(long)id * BRICK_CELLS + (((fi & 7) * 8 + (fj & 7)) * 8 + (fk & 7))] : T(0);
     }
     __syncthreads();
     const long c = (long)br * BRICK_CELLS + t;
@@ -635,7 +721,8 @@ __global__ __launch_bounds__(BLOCK) void k_restrict(LevelDesc Lf, LevelDesc Lc, 
 
 // z_f += trilinear interpolation of the coarse correction (fine unknowns only).  One block per
 // fine brick stages the 6^3 coarse values it needs in LDS.
-template <class T>
+// FINEST: 1 on the finest level (a distinct symbol for profiling), 0 on coarse levels.
+template <class T, int FINEST>
 __global__ __launch_bounds__(BLOCK) void k_prolong(LevelDesc Lf, OpDesc<T> Af, LevelDesc Lc, const T *__restrict__ zc,
                                                    T *__restrict__ zf, const SolveScalars *S) {
     constexpr int E = 6;
@@ -645,9 +732,28 @@ __global__ __launch_bounds__(BLOCK) void k_prolong(LevelDesc Lf, OpDesc<T> Af, L
     if (br < 0) return;
     const int t = threadIdx.x;
     const int BX = Lf.bxyz[3 * br], BY = Lf.bxyz[3 * br + 1], BZ = Lf.bxyz[3 * br + 2];
+    // the 6^3 tile (coarse cells 4B-1 .. 4B+4 per axis) touches coarse bricks (4B-1)>>3 and
+    // (4B+4)>>3 per axis: look them up once
+    __shared__ int bid[8];
+    const int cx0 = (4 * BX + 7) / 8 - 1, cy0 = (4 * BY + 7) / 8 - 1, cz0 = (4 * BZ + 7) / 8 - 1;
+    if (t < 8) {
+        const int fx = cx0 + (t >> 2), fy = cy0 + ((t >> 1) & 1), fz = cz0 + (t & 1);
+        int id = -1;
+        if (fx >= 0 && fy >= 0 && fz >= 0 && fx < Lc.nbx && fy < Lc.nby && fz < Lc.nbz)
+            id = Lc.bmap[((long)fx * Lc.nby + fy) * Lc.nbz + fz];
+        bid[t] = id;
+    }
+    __syncthreads();
     if (t < E * E * E) {
         const int c = t % E, b = (t / E) % E, a = t / (E * E);
-        tile[t] = level_value(Lc, zc, 4 * BX - 1 + a, 4 * BY - 1 + b, 4 * BZ - 1 + c);
+        const int I = 4 * BX - 1 + a, J = 4 * BY - 1 + b, K = 4 * BZ - 1 + c;
+        T v = 0;
+        if (I >= 0 && J >= 0 && K >= 0 && I < Lc.ni && J < Lc.nj && K < Lc.nk) {
+            const int slot = (((I >> 3) - cx0) * 2 + ((J >> 3) - cy0)) * 2 + ((K >> 3) - cz0);
+            const int id = bid[slot];
+            if (id >= 0) v = zc[(long)id * BRICK_CELLS + (((I & 7) * 8 + (J & 7)) * 8 + (K & 7))];
+        }
+        tile[t] = v;
     }
     __syncthreads();
     const long c = (long)br * BRICK_CELLS + t;
@@ -697,6 +803,214 @@ __global__ __launch_bounds__(BLOCK) void k_coarse_solve(LevelDesc L, OpDesc<T> A
         cur ^= 1;
     }
     z[c] = zs[cur][t];
+}
+
+// ------------------------------------------------------------------------------ bottom of the V-cycle
+// Levels with few bricks are latency-bound: one kernel launch per smoothing sweep costs more than
+// the sweep.  The bottom kernel runs the whole V-cycle below a given level in one workgroup of
+// 1024 threads, the phases separated by workgroup barriers.  Per-cell arithmetic is the same as
+// in the per-brick kernels.
+constexpr int BOTTOM_THREADS = 1024;
+constexpr int BOTTOM_MAX = 8;
+
+template <class T>
+struct BottomLevel {
+    LevelDesc L;
+    OpDesc<T> A;
+    const T *idiag;
+    const T *rhs;
+    T *z, *tmp, *res;
+};
+
+template <class T>
+struct BottomDesc {
+    BottomLevel<T> lv[BOTTOM_MAX];
+    int nlev;
+    int pre, post, coarse_sweeps;
+    T omega;
+};
+
+template <class T>
+__device__ __forceinline__ T cell_jacobi(const LevelDesc &L, const OpDesc<T> &A, const T *idiag, const T *rhs,
+                                         const T *zin, T omega, int b, int t) {
+    const long c = (long)b * BRICK_CELLS + t;
+    const T id = idiag[c];
+    if (!(id > 0)) return T(0);
+    const Nbrs N = load_nbrs(L, b);
+    long idx[6];
+    nb_index(N, t, idx);
+    return (T(1) - omega) * zin[c] + omega * id * (rhs[c] + offdiag_sum(A, zin, c, idx));
+}
+
+template <class T>
+__device__ __forceinline__ T cell_smooth2(const LevelDesc &L, const OpDesc<T> &A, const T *idiag, const T *rhs,
+                                          T omega, int b, int t) {
+    const long c = (long)b * BRICK_CELLS + t;
+    const T id = idiag[c];
+    if (!(id > 0)) return T(0);
+    const Nbrs N = load_nbrs(L, b);
+    long idx[6];
+    nb_index(N, t, idx);
+    auto z1 = [&](long i) { return i >= 0 ? omega * idiag[i] * rhs[i] : T(0); };
+    const T gxp = ld(A.gx, idx[NB_XP]), gyp = ld(A.gy, idx[NB_YP]), gzp = ld(A.gz, idx[NB_ZP]);
+    const T off = A.gx[c] * z1(idx[NB_XM]) + gxp * z1(idx[NB_XP]) + A.gy[c] * z1(idx[NB_YM]) + gyp * z1(idx[NB_YP]) +
+                  A.gz[c] * z1(idx[NB_ZM]) + gzp * z1(idx[NB_ZP]);
+    const T r = rhs[c];
+    return (T(1) - omega) * (omega * id * r) + omega * id * (r + off);
+}
+
+template <class T>
+__device__ __forceinline__ T cell_residual(const LevelDesc &L, const OpDesc<T> &A, const T *rhs, const T *z, int b,
+                                           int t) {
+    const long c = (long)b * BRICK_CELLS + t;
+    const T dirc = A.dir[c];
+    if (!(dirc >= 0)) return T(0);
+    const Nbrs N = load_nbrs(L, b);
+    long idx[6];
+    nb_index(N, t, idx);
+    T diag;
+    return rhs[c] - apply_row(A, z, c, idx, dirc, z[c], diag);
+}
+
+template <class T>
+__device__ __forceinline__ T cell_restrict(const LevelDesc &Lf, const LevelDesc &Lc, const OpDesc<T> &Ac,
+                                           const T *res, int b, int t) {
+    const long c = (long)b * BRICK_CELLS + t;
+    if (!(Ac.dir[c] >= 0)) return T(0);
+    const int I = Lc.bxyz[3 * b] * BRICK + (t >> 6), J = Lc.bxyz[3 * b + 1] * BRICK + ((t >> 3) & 7),
+              K = Lc.bxyz[3 * b + 2] * BRICK + (t & 7);
+    const T w[4] = {T(0.25), T(0.75), T(0.75), T(0.25)};
+    T acc = 0;
+    for (int a = 0; a < 4; ++a) {
+        T acc_a = 0;
+        for (int bb = 0; bb < 4; ++bb) {
+            T acc_b = 0;
+            for (int cc = 0; cc < 4; ++cc) acc_b += w[cc] * level_value(Lf, res, 2 * I - 1 + a, 2 * J - 1 + bb, 2 * K - 1 + cc);
+            acc_a += w[bb] * acc_b;
+        }
+        acc += w[a] * acc_a;
+    }
+    return acc;
+}
+
+template <class T>
+__device__ __forceinline__ T cell_prolong(const LevelDesc &Lf, const LevelDesc &Lc, const T *zc, int b, int t) {
+    const int i = Lf.bxyz[3 * b] * BRICK + (t >> 6), j = Lf.bxyz[3 * b + 1] * BRICK + ((t >> 3) & 7),
+              k = Lf.bxyz[3 * b + 2] * BRICK + (t & 7);
+    const int I0 = i >> 1, J0 = j >> 1, K0 = k >> 1;
+    const int I1 = (i & 1) ? I0 + 1 : I0 - 1, J1 = (j & 1) ? J0 + 1 : J0 - 1, K1 = (k & 1) ? K0 + 1 : K0 - 1;
+    const T n = T(0.75), f = T(0.25);
+    return n * (n * (n * level_value(Lc, zc, I0, J0, K0) + f * level_value(Lc, zc, I0, J0, K1)) +
+                f * (n * level_value(Lc, zc, I0, J1, K0) + f * level_value(Lc, zc, I0, J1, K1))) +
+           f * (n * (n * level_value(Lc, zc, I1, J0, K0) + f * level_value(Lc, zc, I1, J0, K1)) +
+                f * (n * level_value(Lc, zc, I1, J1, K0) + f * level_value(Lc, zc, I1, J1, K1)));
+}
+
+// V-cycle on levels lv[0..nlev-1] (lv[0].rhs given, result in lv[0].z or lv[0].tmp: the host
+// knows which from the sweep counts).  The last level holds a single brick and is solved with
+// coarse_sweeps damped-Jacobi sweeps in LDS.
+template <class T>
+__global__ __launch_bounds__(BOTTOM_THREADS) void k_bottom(BottomDesc<T> B, const SolveScalars *S) {
+    __shared__ T zs[2][BRICK_CELLS];
+    if (S && S->done) return;
+    const int tid = threadIdx.x;
+    const T om = B.omega;
+    T *zcur[BOTTOM_MAX], *zoth[BOTTOM_MAX];
+    for (int l = 0; l < B.nlev; ++l) {
+        const BottomLevel<T> &V = B.lv[l];
+        const int ncell = V.L.nbricks * BRICK_CELLS;
+        if (l == B.nlev - 1) {
+            // coarsest: one brick, LDS Jacobi (threads >= 512 only take part in the barriers)
+            const int t = tid;
+            const bool act = t < BRICK_CELLS && V.L.nbricks > 0;
+            const int li = t >> 6, lj = (t >> 3) & 7, lk = t & 7;
+            T gxm = 0, gym = 0, gzm = 0, gxp = 0, gyp = 0, gzp = 0, r = 0, inv = 0, diag = 0;
+            bool unk = false;
+            if (act) {
+                const T dirc = V.A.dir[t];
+                gxm = V.A.gx[t];
+                gym = V.A.gy[t];
+                gzm = V.A.gz[t];
+                gxp = li < 7 ? V.A.gx[t + 64] : T(0);
+                gyp = lj < 7 ? V.A.gy[t + 8] : T(0);
+                gzp = lk < 7 ? V.A.gz[t + 1] : T(0);
+                diag = dirc + ((gxm + gxp) + (gym + gyp) + (gzm + gzp));
+                unk = dirc >= 0 && diag > 0;
+                r = V.rhs[t];
+                inv = unk ? om / diag : T(0);
+                zs[0][t] = unk ? inv * r : T(0);
+            }
+            __syncthreads();
+            int cur = 0;
+            for (int s = 1; s < B.coarse_sweeps; ++s) {
+                if (act) {
+                    const T *zz = zs[cur];
+                    T out = 0;
+                    if (unk) {
+                        const T off = gxm * (li > 0 ? zz[t - 64] : T(0)) + gxp * (li < 7 ? zz[t + 64] : T(0)) +
+                                      gym * (lj > 0 ? zz[t - 8] : T(0)) + gyp * (lj < 7 ? zz[t + 8] : T(0)) +
+                                      gzm * (lk > 0 ? zz[t - 1] : T(0)) + gzp * (lk < 7 ? zz[t + 1] : T(0));
+                        const T zc = zz[t];
+                        out = zc + inv * (r - (diag * zc - off));
+                    }
+                    zs[cur ^ 1][t] = out;
+                }
+                __syncthreads();
+                cur ^= 1;
+            }
+            if (act) V.z[t] = zs[cur][t];
+            __syncthreads();
+            zcur[l] = V.z;
+            zoth[l] = V.tmp;
+            break;
+        }
+        T *cur = V.z, *oth = V.tmp;
+        if (B.pre <= 1) {
+            for (int c = tid; c < ncell; c += BOTTOM_THREADS) cur[c] = om * V.idiag[c] * V.rhs[c];
+        } else {
+            for (int c = tid; c < ncell; c += BOTTOM_THREADS)
+                cur[c] = cell_smooth2(V.L, V.A, V.idiag, V.rhs, om, c >> 9, c & 511);
+            for (int s = 2; s < B.pre; ++s) {
+                __syncthreads();
+                for (int c = tid; c < ncell; c += BOTTOM_THREADS)
+                    oth[c] = cell_jacobi(V.L, V.A, V.idiag, V.rhs, (const T *)cur, om, c >> 9, c & 511);
+                T *tt = cur;
+                cur = oth;
+                oth = tt;
+            }
+        }
+        __syncthreads();
+        for (int c = tid; c < ncell; c += BOTTOM_THREADS)
+            V.res[c] = cell_residual(V.L, V.A, V.rhs, (const T *)cur, c >> 9, c & 511);
+        __syncthreads();
+        const BottomLevel<T> &W = B.lv[l + 1];
+        const int nc = W.L.nbricks * BRICK_CELLS;
+        T *wrhs = const_cast<T *>(W.rhs);
+        for (int c = tid; c < nc; c += BOTTOM_THREADS)
+            wrhs[c] = cell_restrict(V.L, W.L, W.A, (const T *)V.res, c >> 9, c & 511);
+        __syncthreads();
+        zcur[l] = cur;
+        zoth[l] = oth;
+    }
+    for (int l = B.nlev - 2; l >= 0; --l) {
+        const BottomLevel<T> &V = B.lv[l];
+        const BottomLevel<T> &W = B.lv[l + 1];
+        const int ncell = V.L.nbricks * BRICK_CELLS;
+        T *cur = zcur[l], *oth = zoth[l];
+        for (int c = tid; c < ncell; c += BOTTOM_THREADS)
+            if (V.A.dir[c] >= 0) cur[c] += cell_prolong(V.L, W.L, (const T *)zcur[l + 1], c >> 9, c & 511);
+        __syncthreads();
+        for (int s = 0; s < B.post; ++s) {
+            for (int c = tid; c < ncell; c += BOTTOM_THREADS)
+                oth[c] = cell_jacobi(V.L, V.A, V.idiag, V.rhs, (const T *)cur, om, c >> 9, c & 511);
+            __syncthreads();
+            T *tt = cur;
+            cur = oth;
+            oth = tt;
+        }
+        zcur[l] = cur;
+        zoth[l] = oth;
+    }
 }
 
 // ------------------------------------------------------------------------------ output

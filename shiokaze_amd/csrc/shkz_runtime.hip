@@ -17,6 +17,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -123,7 +124,7 @@ struct LevelHost {
     std::vector<int> bmap, bxyz, bnbr;
     LevelDesc desc{};
     // operator and V-cycle vectors (element type = level precision)
-    void *dir = nullptr, *gx = nullptr, *gy = nullptr, *gz = nullptr;
+    void *dir = nullptr, *gx = nullptr, *gy = nullptr, *gz = nullptr, *idiag = nullptr;
     void *rhs = nullptr, *z = nullptr, *tmp = nullptr, *res = nullptr;
     int64_t unknowns = 0;
 };
@@ -226,6 +227,16 @@ struct shkz_ctx {
     std::vector<TimedLaunch> pending;
     std::vector<hipEvent_t> event_pool;
     std::vector<KernelTotals> totals;
+    // execution: one work stream (default priority) ordered after the caller's stream
+    hipStream_t work = nullptr;
+    hipEvent_t ev_in = nullptr, ev_out = nullptr;
+    bool use_graphs = true;
+    bool trace = false;
+    // replayable PCG iteration graphs, keyed by everything their kernel arguments depend on
+    struct GraphSlot {
+        std::string key;
+        hipGraphExec_t exec = nullptr;
+    } graphs[4];
 };
 
 namespace shkz {
@@ -234,6 +245,7 @@ namespace shkz {
 struct Launcher {
     shkz_ctx *C;
     hipStream_t s;
+    bool capturing = false;
 
     int kernel_id(const char *name) {
         auto it = C->kernel_ids.find(name);
@@ -258,13 +270,13 @@ struct Launcher {
             Args... args) {
         if (grid.x == 0) return;
         hipEvent_t e0 = nullptr, e1 = nullptr;
-        if (C->timing) {
+        if (C->timing && !capturing) {
             e0 = event();
             e1 = event();
             if (e0) (void)hipEventRecord(e0, s);
         }
         kernel<<<grid, block, 0, s>>>(args...);
-        if (C->timing && e0 && e1) {
+        if (C->timing && !capturing && e0 && e1) {
             (void)hipEventRecord(e1, s);
             C->pending.push_back({kernel_id(name), level, bytes_per_unknown, bytes_fixed, e0, e1});
         }
@@ -304,7 +316,8 @@ struct Problem {
     void *kr = nullptr, *kp = nullptr, *kq = nullptr, *kx = nullptr, *kz = nullptr, *ktmp = nullptr;
     SolveScalars *S_out = nullptr, *S_in = nullptr;
     RedBuf red{};
-    int pre = 2, post = 2, coarse_sweeps = 48;
+    int pre = 2, post = 2, pre_c = 2, post_c = 2, coarse_sweeps = 48;
+    int lb = 0;  // first level of the one-workgroup bottom cycle
     double omega = 0.8;
 };
 
@@ -314,8 +327,13 @@ static OpDesc<T> op_of(const LevelHost &L) {
 }
 
 // One V-cycle z = M r on the level-0 vectors (rhs0 -> returns the pointer holding z).
+// One V-cycle z = M r on the level-0 vectors; returns the pointer holding z.  Levels below P.lb
+// run as per-brick kernels, levels P.lb.. in one k_bottom launch.  With dot_partials, the last
+// post-smoothing sweep on the finest level also stores r.z partials in P.red (the caller then
+// finalizes them instead of running a separate dot kernel).
 template <class T>
-static T *vcycle(shkz_ctx *C, Launcher &K, Problem &P, const T *rhs0, T *z0, T *tmp0, const SolveScalars *S) {
+static T *vcycle(shkz_ctx *C, Launcher &K, Problem &P, const T *rhs0, T *z0, T *tmp0, const SolveScalars *S,
+                 bool dot_partials) {
     const int nl = P.nlev;
     std::vector<T *> zcur(nl), zoth(nl);
     const T om = (T)P.omega;
@@ -326,37 +344,76 @@ static T *vcycle(shkz_ctx *C, Launcher &K, Problem &P, const T *rhs0, T *z0, T *
         T *z = l == 0 ? z0 : (T *)L.z;
         T *tmp = l == 0 ? tmp0 : (T *)L.tmp;
         const OpDesc<T> A = op_of<T>(L);
+        const T *id = (const T *)L.idiag;
         const dim3 g(brick_grid(L.nbricks)), b(BLOCK);
-        if (l == nl - 1) {
-            K.go("coarse_solve", l, 6 * es, 0, dim3(1), b, k_coarse_solve<T>, L.desc, A, rhs, z, P.coarse_sweeps, om, S);
-            zcur[l] = z;
-            zoth[l] = tmp;
+        const bool f = l == 0;
+        const int pre = f ? P.pre : P.pre_c;
+        if (l == P.lb) {
+            BottomDesc<T> B{};
+            B.nlev = nl - l;
+            B.pre = P.pre_c;
+            B.post = P.post_c;
+            B.coarse_sweeps = P.coarse_sweeps;
+            B.omega = om;
+            for (int q = 0; q < B.nlev; ++q) {
+                LevelHost &Q = C->levels[l + q];
+                BottomLevel<T> &V = B.lv[q];
+                V.L = Q.desc;
+                V.A = op_of<T>(Q);
+                V.idiag = (const T *)Q.idiag;
+                V.rhs = q == 0 ? rhs : (const T *)Q.rhs;
+                V.z = q == 0 ? z : (T *)Q.z;
+                V.tmp = q == 0 ? tmp : (T *)Q.tmp;
+                V.res = (T *)Q.res;
+            }
+            K.go("bottom", -1, 0, 0, dim3(1), dim3(BOTTOM_THREADS), k_bottom<T>, B, S);
+            // where the bottom's first level ended: each Jacobi sweep after smooth2 swaps buffers
+            const int swaps = B.nlev == 1 ? 0 : (std::max(0, P.pre_c - 2) + P.post_c);
+            zcur[l] = (swaps & 1) ? tmp : z;
+            zoth[l] = (swaps & 1) ? z : tmp;
             break;
         }
-        K.go("smooth_first", l, 6 * es, 0, g, b, k_smooth_first<T>, L.desc, A, rhs, z, om, S);
         T *cur = z, *oth = tmp;
-        for (int s = 1; s < P.pre; ++s) {
-            K.go("jacobi", l, 7 * es, 0, g, b, k_jacobi<T>, L.desc, A, rhs, (const T *)cur, oth, om, S);
-            std::swap(cur, oth);
+        if (pre <= 1) {
+            K.go(f ? "smooth_first_L0" : "smooth_first_Lc", l, 3 * es, 0, g, b,
+                 f ? k_smooth_first<T, 1> : k_smooth_first<T, 0>, L.desc, id, rhs, z, om, S);
+        } else {
+            K.go(f ? "smooth2_L0" : "smooth2_Lc", l, 6 * es, 0, g, b, f ? k_smooth2<T, 1> : k_smooth2<T, 0>, L.desc, A,
+                 id, rhs, z, om, S);
+            for (int s = 2; s < pre; ++s) {
+                K.go(f ? "jacobi_L0" : "jacobi_Lc", l, 7 * es, 0, g, b, f ? k_jacobi<T, 1, 0> : k_jacobi<T, 0, 0>,
+                     L.desc, A, id, rhs, (const T *)cur, oth, om, S, P.red);
+                std::swap(cur, oth);
+            }
         }
-        K.go("residual", l, 7 * es, 0, g, b, k_residual<T>, L.desc, A, rhs, (const T *)cur, (T *)L.res, S);
+        K.go(f ? "residual_L0" : "residual_Lc", l, 7 * es, 0, g, b, f ? k_residual<T, 1> : k_residual<T, 0>, L.desc, A,
+             rhs, (const T *)cur, (T *)L.res, S);
         LevelHost &Lc = C->levels[l + 1];
-        K.go("restrict", l, es, 2 * es * (double)Lc.unknowns, dim3(brick_grid(Lc.nbricks)), b, k_restrict<T>, L.desc,
-             Lc.desc, op_of<T>(Lc), (const T *)L.res, (T *)Lc.rhs, S);
+        K.go(f ? "restrict_L0" : "restrict_Lc", l, es, 2 * es * (double)Lc.unknowns, dim3(brick_grid(Lc.nbricks)), b,
+             f ? k_restrict<T, 1> : k_restrict<T, 0>, L.desc, Lc.desc, op_of<T>(Lc), (const T *)L.res, (T *)Lc.rhs, S);
         zcur[l] = cur;
         zoth[l] = oth;
     }
-    for (int l = nl - 2; l >= 0; --l) {
+    for (int l = P.lb - 1; l >= 0; --l) {
         LevelHost &L = C->levels[l];
         LevelHost &Lc = C->levels[l + 1];
         const T *rhs = l == 0 ? rhs0 : (const T *)L.rhs;
         const OpDesc<T> A = op_of<T>(L);
+        const T *id = (const T *)L.idiag;
         const dim3 g(brick_grid(L.nbricks)), b(BLOCK);
-        K.go("prolong", l, 3 * es, es * (double)Lc.unknowns, g, b, k_prolong<T>, L.desc, A, Lc.desc,
-             (const T *)zcur[l + 1], zcur[l], S);
+        const bool f = l == 0;
+        const int post = f ? P.post : P.post_c;
+        K.go(f ? "prolong_L0" : "prolong_Lc", l, 3 * es, es * (double)Lc.unknowns, g, b,
+             f ? k_prolong<T, 1> : k_prolong<T, 0>, L.desc, A, Lc.desc, (const T *)zcur[l + 1], zcur[l], S);
         T *cur = zcur[l], *oth = zoth[l];
-        for (int s = 0; s < P.post; ++s) {
-            K.go("jacobi", l, 7 * es, 0, g, b, k_jacobi<T>, L.desc, A, rhs, (const T *)cur, oth, om, S);
+        for (int s = 0; s < post; ++s) {
+            const bool last_dot = f && dot_partials && s == post - 1;
+            if (last_dot)
+                K.go("jacobi_dot_L0", l, 7 * es, 0, g, b, k_jacobi<T, 1, 1>, L.desc, A, id, rhs, (const T *)cur, oth, om,
+                     S, P.red);
+            else
+                K.go(f ? "jacobi_L0" : "jacobi_Lc", l, 7 * es, 0, g, b, f ? k_jacobi<T, 1, 0> : k_jacobi<T, 0, 0>,
+                     L.desc, A, id, rhs, (const T *)cur, oth, om, S, P.red);
             std::swap(cur, oth);
         }
         zcur[l] = cur;
@@ -380,8 +437,33 @@ static void finalize(Launcher &K, const Problem &P, unsigned nblocks, SolveScala
          (int)nblocks, S, op, param, iparam);
 }
 
+template <class T>
+static std::string graph_key(shkz_ctx *C, const Problem &P, const void *const *vecs, int nvec, bool precond) {
+    std::string k;
+    char buf[160];
+    std::snprintf(buf, sizeof(buf), "T%zu pc%d pre%d post%d prc%d poc%d lb%d cs%d om%.17g red%p|", sizeof(T),
+                  precond ? 1 : 0, P.pre, P.post, P.pre_c, P.post_c, P.lb, P.coarse_sweeps, P.omega,
+                  (void *)P.red.partials);
+    k += buf;
+    for (int i = 0; i < nvec; ++i) {
+        std::snprintf(buf, sizeof(buf), "%p,", vecs[i]);
+        k += buf;
+    }
+    const int nl = precond ? P.nlev : 1;
+    for (int l = 0; l < nl; ++l) {
+        const LevelHost &L = C->levels[l];
+        std::snprintf(buf, sizeof(buf), "|%d %d %d %d %p %p %p %p %p %p %p %p %p %p %p", L.ni, L.nj, L.nk, L.nbricks,
+                      (const void *)L.desc.bmap, L.dir, L.gx, L.gy, L.gz, L.rhs, L.z, L.tmp, L.res,
+                      (const void *)L.desc.bxyz, (const void *)L.desc.bnbr);
+        k += buf;
+    }
+    return k;
+}
+
 // Krylov loop on level 0: T = element type; precond = V-cycle or identity (plain CG).
-// Expects r (residual), and S with tol/maxit/iter set.  x is updated in place.
+// Expects r (residual), and S with tol/maxit/iter set.  x is updated in place.  One iteration is
+// captured once into a hipGraph and replayed (kernels return at once after convergence, so a
+// chunk of replays may overshoot by a few no-op iterations).
 template <class T>
 static int krylov(shkz_ctx *C, Launcher &K, Problem &P, T *x, T *r, T *p, T *q, T *z, T *tmp, SolveScalars *S,
                   bool precond, SolveScalars *hs) {
@@ -390,31 +472,68 @@ static int krylov(shkz_ctx *C, Launcher &K, Problem &P, T *x, T *r, T *p, T *q, 
     const dim3 g(nb), b(BLOCK);
     const OpDesc<T> A = op_of<T>(L0);
     const double es = (double)sizeof(T);
+    // the finest level's last post-smoothing sweep forms r.z when there is one
+    const bool fused_dot = precond && P.lb > 0 && P.post > 0;
     // first direction
     const T *zp = r;
-    if (precond) zp = vcycle<T>(C, K, P, r, z, tmp, S);
-    K.go("dot_rz", 0, 2 * es, 0, g, b, k_dot_rz<T>, L0.desc, (const T *)r, zp, (const SolveScalars *)S, P.red);
+    if (precond) zp = vcycle<T>(C, K, P, r, z, tmp, S, fused_dot);
+    if (!fused_dot)
+        K.go("dot_rz", 0, 2 * es, 0, g, b, k_dot_rz<T>, L0.desc, (const T *)r, zp, (const SolveScalars *)S, P.red);
     finalize(K, P, nb, S, FIN_RZ_FIRST);
     K.go("update_p", 0, 2 * es, 0, g, b, k_update_p<T>, L0.desc, zp, p, (const SolveScalars *)S, 1);
+
+    auto iteration = [&]() {
+        K.go("apply_dot", 0, 6 * es, 0, g, b, k_apply_dot<T>, L0.desc, A, (const T *)p, q, (const SolveScalars *)S,
+             P.red);
+        finalize(K, P, nb, S, FIN_PQ);
+        K.go("cg_update", 0, 6 * es, 0, g, b, k_cg_update<T>, L0.desc, x, r, (const T *)p, (const T *)q,
+             (const SolveScalars *)S, P.red);
+        finalize(K, P, nb, S, precond ? FIN_UPDATE : FIN_UPDATE_CG);
+        if (precond) {
+            const T *zz = vcycle<T>(C, K, P, r, z, tmp, S, fused_dot);
+            if (!fused_dot)
+                K.go("dot_rz", 0, 2 * es, 0, g, b, k_dot_rz<T>, L0.desc, (const T *)r, zz, (const SolveScalars *)S,
+                     P.red);
+            finalize(K, P, nb, S, FIN_RZ);
+            K.go("update_p", 0, 3 * es, 0, g, b, k_update_p<T>, L0.desc, zz, p, (const SolveScalars *)S, 0);
+        } else {
+            K.go("update_p", 0, 3 * es, 0, g, b, k_update_p<T>, L0.desc, (const T *)r, p, (const SolveScalars *)S, 0);
+        }
+    };
+    hipGraphExec_t exec = nullptr;
+    if (C->use_graphs && !C->timing) {
+        const void *vecs[] = {x, r, p, q, z, tmp, S};
+        const std::string key = graph_key<T>(C, P, vecs, 7, precond);
+        auto &slot = C->graphs[(sizeof(T) == 8 ? 2 : 0) + (precond ? 1 : 0)];
+        if (slot.exec && slot.key == key) {
+            exec = slot.exec;
+        } else {
+            if (slot.exec) {
+                SHKZ_HIP(hipGraphExecDestroy(slot.exec));
+                slot.exec = nullptr;
+                slot.key.clear();
+            }
+            hipGraph_t graph = nullptr;
+            SHKZ_HIP(hipStreamBeginCapture(K.s, hipStreamCaptureModeThreadLocal));
+            K.capturing = true;
+            iteration();
+            K.capturing = false;
+            SHKZ_HIP(hipStreamEndCapture(K.s, &graph));
+            hipError_t e = hipGraphInstantiate(&slot.exec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            if (e != hipSuccess) {
+                slot.exec = nullptr;
+                return fail(SHKZ_ERR_DEVICE, "hipGraphInstantiate failed: %s", hipGetErrorString(e));
+            }
+            slot.key = key;
+            exec = slot.exec;
+        }
+    }
     int chunk = 4;
     for (;;) {
         for (int it = 0; it < chunk; ++it) {
-            K.go("apply_dot", 0, 6 * es, 0, g, b, k_apply_dot<T>, L0.desc, A, (const T *)p, q, (const SolveScalars *)S,
-                 P.red);
-            finalize(K, P, nb, S, FIN_PQ);
-            K.go("cg_update", 0, 6 * es, 0, g, b, k_cg_update<T>, L0.desc, x, r, (const T *)p, (const T *)q,
-                 (const SolveScalars *)S, P.red);
-            finalize(K, P, nb, S, precond ? FIN_UPDATE : FIN_UPDATE_CG);
-            if (precond) {
-                zp = vcycle<T>(C, K, P, r, z, tmp, S);
-                K.go("dot_rz", 0, 2 * es, 0, g, b, k_dot_rz<T>, L0.desc, (const T *)r, zp, (const SolveScalars *)S,
-                     P.red);
-                finalize(K, P, nb, S, FIN_RZ);
-                K.go("update_p", 0, 3 * es, 0, g, b, k_update_p<T>, L0.desc, zp, p, (const SolveScalars *)S, 0);
-            } else {
-                K.go("update_p", 0, 3 * es, 0, g, b, k_update_p<T>, L0.desc, (const T *)r, p, (const SolveScalars *)S,
-                     0);
-            }
+            if (exec) SHKZ_HIP(hipGraphLaunch(exec, K.s));
+            else iteration();
         }
         SHKZ_TRY(read_scalars(C, K.s, S, hs));
         if (hs->done) break;
@@ -447,8 +566,26 @@ struct ExportOut {
     double *diag, *gx, *gy, *gz, *rhs;  // device, cell-indexed
 };
 
+static int project_stages(shkz_ctx *C, const shkz_grid *grid, const shkz_params *prm, shkz_result *res,
+                          hipStream_t s, const ExportOut *exp);
+
+// Runs the projection on the context's work stream and orders the caller's stream after it, so
+// that work the caller queues next (and synchronous copies on the null stream) sees the outputs.
 static int project_impl(shkz_ctx *C, const shkz_grid *grid, const shkz_params *prm, shkz_result *res,
-                        hipStream_t s, const ExportOut *exp = nullptr) {
+                        hipStream_t caller, const ExportOut *exp = nullptr) {
+    const int rc = project_stages(C, grid, prm, res, caller, exp);
+    if (C->work) {
+        if (hipEventRecord(C->ev_out, C->work) != hipSuccess || hipStreamWaitEvent(caller, C->ev_out, 0) != hipSuccess) {
+            (void)hipStreamSynchronize(C->work);
+            return rc != SHKZ_OK ? rc : fail(SHKZ_ERR_DEVICE, "ordering the caller's stream after the projection failed");
+        }
+        if (rc != SHKZ_OK) (void)hipStreamSynchronize(C->work);
+    }
+    return rc;
+}
+
+static int project_stages(shkz_ctx *C, const shkz_grid *grid, const shkz_params *prm, shkz_result *res,
+                          hipStream_t s, const ExportOut *exp) {
     const double t_start = now_ms();
     std::memset(res, 0, sizeof(*res));
     const int w = grid->w, h = grid->h, d = grid->d;
@@ -460,6 +597,10 @@ static int project_impl(shkz_ctx *C, const shkz_grid *grid, const shkz_params *p
         return fail(SHKZ_ERR_INVALID, "unknown precision %d", prm->precision);
     if ((int64_t)w * h * d > (int64_t)1 << 31) return fail(SHKZ_ERR_INVALID, "grid too large for 32-bit brick ids");
     SHKZ_HIP(hipSetDevice(C->device));
+    // all work runs on the context's stream, ordered after what the caller queued on its stream
+    SHKZ_HIP(hipEventRecord(C->ev_in, s));
+    SHKZ_HIP(hipStreamWaitEvent(C->work, C->ev_in, 0));
+    s = C->work;
     Launcher K{C, s};
     Arena &AR = C->arena;
     const size_t M = (size_t)w * h * d;
@@ -468,10 +609,13 @@ static int project_impl(shkz_ctx *C, const shkz_grid *grid, const shkz_params *p
     Problem P;
     P.mixed = (prm->solver == SHKZ_SOLVER_MGPCG && prm->precision == SHKZ_PREC_MIXED);
     P.fp64_levels = !P.mixed;
-    P.pre = prm->mg_pre_sweeps > 0 ? prm->mg_pre_sweeps : 2;
-    P.post = prm->mg_post_sweeps > 0 ? prm->mg_post_sweeps : 2;
+    P.pre = prm->mg_pre_sweeps > 0 ? prm->mg_pre_sweeps : 4;
+    P.post = prm->mg_post_sweeps > 0 ? prm->mg_post_sweeps : 4;
+    P.pre_c = prm->mg_coarse_pre_sweeps > 0 ? prm->mg_coarse_pre_sweeps : P.pre;
+    P.post_c = prm->mg_coarse_post_sweeps > 0 ? prm->mg_coarse_post_sweeps : P.post;
     P.coarse_sweeps = prm->mg_coarse_sweeps > 0 ? prm->mg_coarse_sweeps : 48;
-    P.omega = prm->mg_omega > 0 ? prm->mg_omega : 0.8;
+    P.omega = prm->mg_omega > 0 ? prm->mg_omega : 0.9;
+    const int bottom_bricks = prm->mg_bottom_bricks > 0 ? prm->mg_bottom_bricks : 32;
 
     GridDesc &G = P.G;
     G.w = w;
@@ -557,6 +701,20 @@ static int project_impl(shkz_ctx *C, const shkz_grid *grid, const shkz_params *p
         }
     }
     P.nlev = (int)C->levels.size();
+    P.lb = P.nlev - 1;
+    for (int l = 1; l < P.nlev; ++l)
+        if (C->levels[l].nbricks <= bottom_bricks) {
+            P.lb = l;
+            break;
+        }
+    if (P.nlev - P.lb > BOTTOM_MAX) P.lb = P.nlev - BOTTOM_MAX;
+    if (C->trace) {
+        std::fprintf(stderr, "[shkz] levels:");
+        for (int l = 0; l < P.nlev; ++l)
+            std::fprintf(stderr, " %dx%dx%d/%d%s", C->levels[l].ni, C->levels[l].nj, C->levels[l].nk,
+                         C->levels[l].nbricks, l == P.lb ? "(bottom)" : "");
+        std::fprintf(stderr, "\n");
+    }
     if (P.nlev > 1 && C->levels.back().nbricks > 1) return fail(SHKZ_ERR_INVALID, "coarsest level holds more than one brick");
 
     // upload the brick tables (one buffer per level)
@@ -631,6 +789,7 @@ static int project_impl(shkz_ctx *C, const shkz_grid *grid, const shkz_params *p
         SHKZ_TRY(AR.get("gx_l" + sfx, n * es, &pp)); L.gx = pp;
         SHKZ_TRY(AR.get("gy_l" + sfx, n * es, &pp)); L.gy = pp;
         SHKZ_TRY(AR.get("gz_l" + sfx, n * es, &pp)); L.gz = pp;
+        SHKZ_TRY(AR.get("idiag_l" + sfx, n * es, &pp)); L.idiag = pp;
         SHKZ_TRY(AR.get("rhs_l" + sfx, n * es, &pp)); L.rhs = pp;
         SHKZ_TRY(AR.get("z_l" + sfx, n * es, &pp)); L.z = pp;
         SHKZ_TRY(AR.get("tmp_l" + sfx, n * es, &pp)); L.tmp = pp;
@@ -640,6 +799,8 @@ static int project_impl(shkz_ctx *C, const shkz_grid *grid, const shkz_params *p
         void *pp = nullptr;
         SHKZ_TRY(AR.get("res_l0", n0 * es, &pp));
         F0.res = pp;
+        SHKZ_TRY(AR.get("idiag_l0", n0 * es, &pp));
+        F0.idiag = pp;
     }
     // Krylov vectors on level 0
     const size_t ks = (prm->solver == SHKZ_SOLVER_MGPCG && P.mixed) ? 4 : 8;
@@ -697,6 +858,17 @@ static int project_impl(shkz_ctx *C, const shkz_grid *grid, const shkz_params *p
         } else {
             K.go("coarsen", l, 0, 0, g, b, k_coarsen<float, float>, Lf.desc, op_of<float>(Lf), Lc.desc,
                  (float *)Lc.dir, (float *)Lc.gx, (float *)Lc.gy, (float *)Lc.gz);
+        }
+    }
+    // inverse diagonals for the smoothers
+    if (multigrid) {
+        for (int l = 0; l < P.nlev; ++l) {
+            LevelHost &L = C->levels[l];
+            const dim3 g(brick_grid(L.nbricks)), b(BLOCK);
+            if (P.fp64_levels)
+                K.go("idiag", -1, 0, 0, g, b, k_idiag<double>, L.desc, op_of<double>(L), (double *)L.idiag);
+            else
+                K.go("idiag", -1, 0, 0, g, b, k_idiag<float>, L.desc, op_of<float>(L), (float *)L.idiag);
         }
     }
     // unknowns per level (result + byte model of the kernel timers)
@@ -798,6 +970,9 @@ static int project_impl(shkz_ctx *C, const shkz_grid *grid, const shkz_params *p
                                        (float *)P.kz, (float *)P.ktmp, P.S_in, true, &hi));
             }
             total_iters += hi.iter;
+            if (C->trace)
+                std::fprintf(stderr, "[shkz] outer %d: defect inf %.6e (tol %.3e)  inner it %d rinf %.6e r0 %.6e tol %.3e rz %.9e\n",
+                             outer, hs.rinf, hs.tol, hi.iter, hi.rinf, hi.r0inf, hi.tol, hi.rz);
             if (hi.status) {
                 status = SHKZ_ERR_NUMERIC;
                 break;
@@ -890,6 +1065,17 @@ int shkz_create(int device, uint64_t hbm_budget_bytes, shkz_ctx **out) {
         delete C;
         return fail(SHKZ_ERR_NOMEM, "pinned scalar buffer allocation failed");
     }
+    if (hipStreamCreateWithFlags(&C->work, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&C->ev_in, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&C->ev_out, hipEventDisableTiming) != hipSuccess) {
+        (void)hipHostFree(C->host_scalars);
+        delete C;
+        return fail(SHKZ_ERR_DEVICE, "stream/event creation failed");
+    }
+    const char *ng = std::getenv("SHKZ_NO_GRAPHS");
+    C->use_graphs = !(ng && ng[0] == '1');
+    const char *tr = std::getenv("SHKZ_TRACE");
+    C->trace = tr && tr[0] == '1';
     *out = C;
     return SHKZ_OK;
 }
@@ -903,6 +1089,11 @@ void shkz_destroy(shkz_ctx *C) {
         (void)hipEventDestroy(t.ev1);
     }
     for (hipEvent_t e : C->event_pool) (void)hipEventDestroy(e);
+    for (auto &g : C->graphs)
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    if (C->ev_in) (void)hipEventDestroy(C->ev_in);
+    if (C->ev_out) (void)hipEventDestroy(C->ev_out);
+    if (C->work) (void)hipStreamDestroy(C->work);
     C->arena.clear();
     if (C->host_scalars) (void)hipHostFree(C->host_scalars);
     delete C;

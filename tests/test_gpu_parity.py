@@ -268,3 +268,40 @@ def test_budget_refusal():
     r, P, P64 = c.project(sc, S.default_params(), check=False)
     assert r.status == -4
     c.close()
+
+
+def test_device_entry_orders_caller_stream():
+    """Back-to-back shkz_project_device calls with no host synchronisation: each call's outputs are
+    ordered on the caller's stream, so restoring the inputs on that stream before the next call
+    never races the previous velocity update (identical results every time)."""
+    say("device entry stream ordering")
+    from shiokaze_amd.devmem import DeviceArray, Stream
+    sc = S.make_scene(S.SCENE_SPHERES_BOX, 96, seed=2)
+    fl, so = DeviceArray(sc.fluid), DeviceArray(sc.solid)
+    src = [DeviceArray(a) for a in (sc.u, sc.v, sc.wv)]
+    vel = [DeviceArray(a) for a in (sc.u, sc.v, sc.wv)]
+    st = Stream()
+    c = S.Context(0)
+    prm = S.default_params(residual=1e-6)
+    its = []
+    for rep in range(4):
+        for a, b in zip(vel, src):
+            a.copy_from(b, st.handle)
+        r = c.project_device(sc.w, sc.h, sc.d, sc.dt, fl, S.F64, so, S.F32, *vel, params=prm, stream=st.handle)
+        its.append(r.iterations)
+    st.synchronize()
+    out = [v.download() for v in vel]
+    # one more, fully synchronised, as the reference result
+    for a, b in zip(vel, src):
+        a.copy_from(b, st.handle)
+    st.synchronize()
+    r = c.project_device(sc.w, sc.h, sc.d, sc.dt, fl, S.F64, so, S.F32, *vel, params=prm, stream=st.handle)
+    st.synchronize()
+    ref = [v.download() for v in vel]
+    c.close()
+    st.destroy()
+    for a in [fl, so] + src + vel:
+        a.free()
+    assert len(set(its)) == 1 and its[0] == r.iterations, (its, r.iterations)
+    for a, b in zip(out, ref):
+        assert np.array_equal(a, b)
