@@ -220,9 +220,12 @@ __global__ __launch_bounds__(BLOCK) void k_operator(GridDesc G, LevelDesc L, con
     }
 }
 
-// Coarse operator = 1/2 of the piecewise-constant Galerkin product: dir_H = 1/2 sum of the
-// children's dir, g_H = 1/2 sum of the four fine couplings across the coarse face.  Every child
-// of a coarse cell and every fine face it needs lie in one fine brick.
+// Coarse operator from the piecewise-constant aggregation P0 of the 2x2x2 children: the couplings
+// are 1/2 of the Galerkin product (g_H = 1/2 sum of the four fine couplings across the coarse
+// face, the usual rediscretisation factor for a 2h grid), the Dirichlet (air) part is kept whole
+// (dir_H = sum of the children's dir), which keeps the free-surface condition as strong on the
+// coarse grids as on the fine one.  Every child of a coarse cell and every fine face it needs lie
+// in one fine brick.
 template <class Ti, class To>
 __global__ __launch_bounds__(BLOCK) void k_coarsen(LevelDesc Lf, OpDesc<Ti> Af, LevelDesc Lc, To *dir, To *gx, To *gy,
                                                    To *gz) {
@@ -255,7 +258,7 @@ __global__ __launch_bounds__(BLOCK) void k_coarsen(LevelDesc Lf, OpDesc<Ti> Af, 
                     }
         }
     }
-    dir[c] = any ? (To)(0.5 * sdir) : (To)(-1);
+    dir[c] = any ? (To)sdir : (To)(-1);
     gx[c] = any ? (To)(0.5 * sg[0]) : (To)0;
     gy[c] = any ? (To)(0.5 * sg[1]) : (To)0;
     gz[c] = any ? (To)(0.5 * sg[2]) : (To)0;

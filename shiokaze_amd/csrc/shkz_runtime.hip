@@ -615,7 +615,7 @@ static int project_stages(shkz_ctx *C, const shkz_grid *grid, const shkz_params 
     P.post_c = prm->mg_coarse_post_sweeps > 0 ? prm->mg_coarse_post_sweeps : P.post;
     P.coarse_sweeps = prm->mg_coarse_sweeps > 0 ? prm->mg_coarse_sweeps : 48;
     P.omega = prm->mg_omega > 0 ? prm->mg_omega : 0.9;
-    const int bottom_bricks = prm->mg_bottom_bricks > 0 ? prm->mg_bottom_bricks : 32;
+    const int bottom_bricks = prm->mg_bottom_bricks > 0 ? prm->mg_bottom_bricks : 1;
 
     GridDesc &G = P.G;
     G.w = w;
