@@ -84,3 +84,15 @@ def test_scene_liquid_is_pushed_out_of_solids():
     # cell centres inside a solid are never liquid
     s_cell = sc.solid[:-1, :-1, :-1]  # nodal sample at the cell's low corner, inside test is loose
     assert not ((sc.fluid < 0) & (s_cell < -2.0 / 32)).any()
+
+
+@pytest.mark.skipif(not os.path.isdir("/root/reference/include"), reason="Shiokaze tree not present")
+def test_shiokaze_adapter_compiles_against_reference_headers():
+    """The drop-in module source type-checks against the reference's own interface headers."""
+    src = os.path.join(ROOT, "shiokaze_amd", "host", "macpressuresolver3_mi355x.cpp")
+    subprocess.check_call(["g++", "-std=c++14", "-fsyntax-only", "-DNUM_THREAD=8", "-I/root/reference/include",
+                           "-I/root/reference/local/include", src])
+
+
+def test_host_harness_is_built():
+    assert os.path.exists(os.path.join(ROOT, "shiokaze_amd", "lib", "shkz_harness"))
