@@ -39,6 +39,26 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+SYMBOL = {"jacobi_L0": "k_jacobi<float, 1, 0>", "jacobi_dot_L0": "k_jacobi<float, 1, 1>",
+          "residual_L0": "k_residual<float, 1>", "smooth2_L0": "k_smooth2<float, 1>"}
+
+
+def measured_traffic(kernel, config):
+    """HBM bytes per dispatch of `kernel` measured by the rocprofv3 FETCH_SIZE/WRITE_SIZE passes
+    summarised in profiles/ (copy-calibrated); None when no measurement covers it."""
+    sym = SYMBOL.get(kernel)
+    if sym is None:
+        return None
+    import glob
+    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*hbm_traffic*.md"))):
+        for line in open(path):
+            if line.startswith("TRAFFIC ") and sym in line and f" {config} " in line:
+                kv = dict(x.split("=") for x in line.split() if "=" in x)
+                return {"bytes": int(kv["read_bytes"]) + int(kv["write_bytes"]),
+                        "source": os.path.relpath(path, ROOT)}
+    return None
+
+
 def cpu_baseline(scene, n_iter_needed, residual):
     """Reference-faithful CPU projection (the oracle restating macpressuresolver3 + pcg with MIC(0)
     computed and discarded), timed on a bounded sample: the full fractions/labelling/assembly and
@@ -84,7 +104,21 @@ def main():
     ap.add_argument("--solver", default="mgpcg", choices=["mgpcg", "cg"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-kernel-timing", action="store_true")
+    ap.add_argument("--calibrate-copy-mb", type=int, default=0,
+                    help="only run 5 device-to-device copies of this many MiB (HBM counter calibration)")
     args = ap.parse_args()
+    if args.calibrate_copy_mb:
+        from shiokaze_amd.devmem import DeviceArray, Stream
+        nb = args.calibrate_copy_mb << 20
+        a = DeviceArray(shape=(nb,), dtype=np.uint8)
+        b = DeviceArray(shape=(nb,), dtype=np.uint8)
+        a.zero()
+        st = Stream()
+        for _ in range(5):
+            b.copy_from(a, st.handle)
+        st.synchronize()
+        print(json.dumps({"calibration_copy_bytes": nb, "copies": 5}))
+        return
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print(json.dumps({"error": "this benchmark runs on one GPU only: the projection of one scene is a "
@@ -156,7 +190,9 @@ def main():
     roofline = None
     if stats:
         kern = {k: v for k, v in stats.items() if v["launches"] > 0 and v["total_ms"] > 0}
-        top = max(kern.items(), key=lambda kv: kv[1]["total_ms"])
+        # dominant kernel = the HBM-bound kernel moving the most algorithmic bytes (the event-timed
+        # pass adds ~2 us per launch, which overstates the thousands of latency-bound coarse launches)
+        top = max(kern.items(), key=lambda kv: kv[1]["bytes"])
         name, st = top
         avg_ms = st["total_ms"] / st["launches"]
         bytes_per_launch = st["bytes"] / st["launches"]
@@ -164,6 +200,11 @@ def main():
         roofline = {"bound": "hbm", "kernel": name, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                     "avg_launch_us": round(avg_ms * 1e3, 2), "algorithmic_bytes_per_launch": int(bytes_per_launch)}
+        roofline["time_share"] = round(st["total_ms"] / sum(v["total_ms"] for v in kern.values()), 4)
+        tr = measured_traffic(name, args.config)
+        if tr is not None:
+            roofline["traffic"] = tr["bytes"]
+            roofline["traffic_source"] = tr["source"]
         total_ms = sum(v["total_ms"] for v in kern.values())
         total_bytes = sum(v["bytes"] for v in kern.values())
         roofline["all_kernels_gbs"] = round(total_bytes / (total_ms * 1e-3) / 1e9, 1)
