@@ -109,7 +109,8 @@ typedef struct shkz_params {
     double inner_residual;        /* mixed precision: relative reduction per inner FP32 solve */
     int32_t mg_coarse_pre_sweeps; /* sweeps on the coarser levels (mg_pre/post_sweeps: finest) */
     int32_t mg_coarse_post_sweeps;
-    int32_t mg_bottom_bricks;     /* levels with at most this many bricks run in one workgroup */
+    int32_t mg_bottom_bricks;     /* unused (kept for layout): every grid's coarsest level is one
+                                     brick, solved by one workgroup */
     int32_t reserved[5];
 } shkz_params;
 
@@ -163,6 +164,19 @@ int shkz_project(shkz_ctx *ctx, const shkz_grid *grid, const shkz_params *params
  * the call returns once the result is known. */
 int shkz_project_device(shkz_ctx *ctx, const shkz_grid *grid, const shkz_params *params,
                         shkz_result *result, void *stream);
+
+/* A batch of `ngrids` independent projections in one call (no counterpart in the reference,
+ * whose driver projects one grid per time step, macliquid3.cpp:321-367): every solver launch
+ * covers the stored bricks of all grids, while each grid keeps its own reductions, multigrid
+ * depth, defect-correction state and convergence test, so results[g] and grid g's outputs equal
+ * those of projecting grid g alone.  Grids may differ in size; scene ids must be distinct (0 =
+ * no scene).  Host arrays, like shkz_project. */
+int shkz_project_batch(shkz_ctx *ctx, const shkz_grid *grids, int ngrids, const shkz_params *params,
+                       shkz_result *results);
+
+/* The batch on device-resident arrays, issued on `stream`, like shkz_project_device. */
+int shkz_project_batch_device(shkz_ctx *ctx, const shkz_grid *grids, int ngrids, const shkz_params *params,
+                              shkz_result *results, void *stream);
 
 /* Runs fractions, labelling and assembly only and writes the assembled FP64 system on dense
  * cell arrays (host): diag (0 off the unknowns), the couplings of each cell's low x/y/z face

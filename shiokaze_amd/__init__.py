@@ -25,6 +25,7 @@ PREC_MIXED, PREC_FP64 = 0, 1
 NORM_INF, NORM_L2 = 0, 1
 SCENE_DROP, SCENE_DROP_SOLIDS, SCENE_SPHERES_BOX, SCENE_RANDOM_DROP = 0, 1, 2, 3
 
+OK, ERR_INVALID, ERR_DEVICE, ERR_NOMEM, ERR_BUDGET, ERR_NOT_FOUND, ERR_NUMERIC = 0, -1, -2, -3, -4, -5, -6
 STATUS = {0: "OK", -1: "INVALID", -2: "DEVICE", -3: "NOMEM", -4: "BUDGET", -5: "NOT_FOUND", -6: "NUMERIC"}
 
 _vp = ctypes.c_void_p
@@ -94,7 +95,7 @@ EXPORTS = [
     "shkz_abi_version", "shkz_last_error", "shkz_default_params", "shkz_create", "shkz_destroy",
     "shkz_project", "shkz_project_device", "shkz_release_scene", "shkz_device_bytes",
     "shkz_set_kernel_timing", "shkz_get_kernel_stats", "shkz_reset_kernel_stats", "shkz_scene_fill",
-    "shkz_export_system",
+    "shkz_export_system", "shkz_project_batch", "shkz_project_batch_device",
 ]
 
 _lib = None
@@ -124,6 +125,10 @@ def lib():
     L.shkz_reset_kernel_stats.argtypes = [_vp]
     L.shkz_scene_fill.argtypes = [ctypes.POINTER(SceneDesc), _vp, _vp, ctypes.POINTER(ctypes.c_int32), _vp, _vp, _vp]
     L.shkz_export_system.argtypes = [_vp, ctypes.POINTER(Grid), ctypes.POINTER(Params), _vp, _vp, _vp, _vp, _vp]
+    L.shkz_project_batch.argtypes = [_vp, ctypes.POINTER(Grid), ctypes.c_int, ctypes.POINTER(Params),
+                                     ctypes.POINTER(Result)]
+    L.shkz_project_batch_device.argtypes = [_vp, ctypes.POINTER(Grid), ctypes.c_int, ctypes.POINTER(Params),
+                                            ctypes.POINTER(Result), _vp]
     _lib = L
     return L
 
@@ -246,16 +251,57 @@ class Context:
             raise ShkzError(rc, last_error())
         return r, P, P64
 
-    def project_device(self, w, h, d, dt, fluid, fluid_dtype, solid, solid_dtype, u, v, wv, params=None,
-                       pressure=None, pressure64=None, scene_id=0, stream=None, dx=0.0, check=True):
-        """Projection on device-resident buffers (ints or objects with data_ptr(), e.g. devmem.DeviceArray)."""
+    def project_batch(self, scenes, params: Params | None = None, pressure: bool = True, pressure64: bool = False,
+                      scene_ids=None, check: bool = True):
+        """Projects a batch of scenes in one call (host arrays, velocities updated in place).
+        Returns (results, [P32], [P64])."""
+        n = len(scenes)
+        grids = (Grid * n)()
+        Ps, P64s = [], []
+        for i, sc in enumerate(scenes):
+            grids[i] = self._grid(sc, 0 if scene_ids is None else scene_ids[i], None)
+            P = np.zeros((sc.w, sc.h, sc.d), np.float32) if pressure else None
+            P64 = np.zeros((sc.w, sc.h, sc.d), np.float64) if pressure64 else None
+            grids[i].pressure = _ptr(P)
+            grids[i].pressure64 = _ptr(P64)
+            Ps.append(P)
+            P64s.append(P64)
+        res = (Result * n)()
+        prm = params if params is not None else default_params()
+        rc = self._L.shkz_project_batch(self.handle, grids, n, ctypes.byref(prm), res)
+        if rc != 0 and check:
+            raise ShkzError(rc, last_error())
+        return list(res), Ps, P64s
+
+    @staticmethod
+    def device_grid(w, h, d, dt, fluid, fluid_dtype, solid, solid_dtype, u, v, wv, pressure=None, pressure64=None,
+                    scene_id=0, dx=0.0) -> Grid:
+        """A Grid of device-resident buffers (ints or objects with data_ptr(), e.g. devmem.DeviceArray)."""
         def dp(x):
             if x is None:
                 return None
             return x.data_ptr() if hasattr(x, "data_ptr") else int(x)
-        g = Grid(w=w, h=h, d=d, dx=dx, dt=dt, fluid=dp(fluid), fluid_dtype=fluid_dtype, solid_dtype=solid_dtype,
-                 solid=dp(solid), u=dp(u), v=dp(v), w_vel=dp(wv), pressure=dp(pressure), pressure64=dp(pressure64),
-                 scene_id=scene_id)
+        return Grid(w=w, h=h, d=d, dx=dx, dt=dt, fluid=dp(fluid), fluid_dtype=fluid_dtype, solid_dtype=solid_dtype,
+                    solid=dp(solid), u=dp(u), v=dp(v), w_vel=dp(wv), pressure=dp(pressure),
+                    pressure64=dp(pressure64), scene_id=scene_id)
+
+    def project_batch_device(self, grids, params: Params | None = None, stream=None, check: bool = True):
+        """Batch projection of device-resident grids (a list of Grid from device_grid())."""
+        n = len(grids)
+        arr = (Grid * n)(*grids)
+        res = (Result * n)()
+        prm = params if params is not None else default_params()
+        rc = self._L.shkz_project_batch_device(self.handle, arr, n, ctypes.byref(prm), res,
+                                               None if stream is None else int(stream))
+        if rc != 0 and check:
+            raise ShkzError(rc, last_error())
+        return list(res)
+
+    def project_device(self, w, h, d, dt, fluid, fluid_dtype, solid, solid_dtype, u, v, wv, params=None,
+                       pressure=None, pressure64=None, scene_id=0, stream=None, dx=0.0, check=True):
+        """Projection on device-resident buffers (ints or objects with data_ptr(), e.g. devmem.DeviceArray)."""
+        g = self.device_grid(w, h, d, dt, fluid, fluid_dtype, solid, solid_dtype, u, v, wv, pressure, pressure64,
+                             scene_id, dx)
         r = Result()
         prm = params if params is not None else default_params()
         rc = self._L.shkz_project_device(self.handle, ctypes.byref(g), ctypes.byref(prm), ctypes.byref(r),

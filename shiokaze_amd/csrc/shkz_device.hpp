@@ -23,19 +23,33 @@ namespace shkz {
 constexpr int BRICK = 8;
 constexpr int BRICK_CELLS = 512;
 constexpr int BLOCK = 512;  // brick kernels: one thread per cell
-constexpr int MAX_LEVELS = 12;
+constexpr int MAX_LEVELS = 32;  // a grid side of 8 * 2^31 cells would need more
 
 // Neighbour slots of a brick in its neighbour table.
 enum { NB_XM = 0, NB_XP = 1, NB_YM = 2, NB_YP = 3, NB_ZM = 4, NB_ZP = 5 };
 
-struct LevelDesc {
+// Per grid and level: the grid's cell and brick extents, where its dense brick map starts in the
+// level's combined map, and its range of stored bricks in the level's combined brick list.
+struct GridLevel {
     int ni, nj, nk;     // cells
     int nbx, nby, nbz;  // bricks per axis
-    int nbricks;        // stored bricks
-    int pad;
-    const int *bmap;    // (bx*nby + by)*nbz + bz -> compact id or -1
-    const int *bxyz;    // compact id -> bx, by, bz
-    const int *bnbr;    // compact id -> 6 neighbour ids (NB_*), -1 = none
+    int bmap_off;       // offset of this grid's dense brick map
+    int brick_start;    // first stored brick of this grid
+    int brick_end;      // one past the last
+    int coarsest;       // 1: the grid fits one brick here, so this is its last level (coarse solve)
+    int pad[2];
+};
+
+// One multigrid level of a batch of grids: the stored bricks of every grid, concatenated
+// (grid-major, Morton order inside a grid).  A single grid is a batch of one.
+struct LevelDesc {
+    int nbricks;        // stored bricks, all grids
+    int ngrids;
+    const int *bmap;    // per grid g: bmap[gl[g].bmap_off + (bx*nby + by)*nbz + bz] -> brick or -1
+    const int *bxyz;    // brick -> bx, by, bz (in its grid)
+    const int *bnbr;    // brick -> 6 neighbour bricks (NB_*), -1 = none
+    const int *bgrid;   // brick -> grid
+    const GridLevel *gl;
 };
 
 template <class T>
@@ -46,7 +60,8 @@ struct OpDesc {
     const T *gz;
 };
 
-// Scalars of one Krylov solve, resident on the device so that no iteration waits on the host.
+// Scalars of one grid's Krylov solve, resident on the device so that no iteration waits on the
+// host.
 struct SolveScalars {
     double rz;        // r.z of the current iteration
     double pq;        // p.Ap
@@ -58,18 +73,34 @@ struct SolveScalars {
     double tol;       // absolute tolerance in the chosen norm
     int32_t iter;     // iterations done
     int32_t maxit;
-    int32_t done;     // 1 = stop (converged or out of iterations); kernels return at once
+    int32_t done;     // 1 = stop (converged or out of iterations)
     int32_t converged;
     int32_t norm;     // shkz_norm
     int32_t status;   // 0 ok, 1 = NaN met
     int32_t pad[2];
 };
 
-// Deterministic two-pass reduction: every block of a reducing kernel stores two partials (plain
-// stores, no atomics); a one-block finalize kernel combines them in block order and updates the
-// solve's scalars.
+// Batch-wide state of one Krylov solve: how many grids are done; every kernel returns at once when
+// all are.
+struct BatchCtl {
+    int32_t ngrids;
+    int32_t ndone;
+    int32_t all_done;
+    int32_t pad;
+};
+
+// What a V-cycle launch may skip: everything once all grids are done, and the bricks of a grid
+// that is done.  S may be NULL (no per-grid test).
+struct Gate {
+    const BatchCtl *ctl;
+    const SolveScalars *S;
+};
+
+// Deterministic two-pass reduction: every brick of a reducing kernel stores two partials (plain
+// stores, no atomics); a finalize kernel with one block per grid combines the partials of that
+// grid's bricks in brick order and updates the grid's scalars.
 struct RedBuf {
-    double *partials;  // 2 per block
+    double *partials;  // 2 per brick
 };
 
 // Remaps a linear block id so that each of the 8 XCDs (which receive blocks round-robin) gets a
@@ -114,10 +145,10 @@ __device__ __forceinline__ double block_max(double v, double *lds) {
     return s;
 }
 
-__device__ __forceinline__ void store_partials(const RedBuf &R, double a, double b) {
+__device__ __forceinline__ void store_partials(const RedBuf &R, int brick, double a, double b) {
     if (threadIdx.x == 0) {
-        R.partials[2 * blockIdx.x + 0] = a;
-        R.partials[2 * blockIdx.x + 1] = b;
+        R.partials[2 * brick + 0] = a;
+        R.partials[2 * brick + 1] = b;
     }
 }
 

@@ -6,14 +6,17 @@
 //   k_surface_tension  surface-tension impulse (:85-115)
 //   k_flags            unknown labelling (:121-156) + which 8^3 bricks hold unknowns
 //   k_operator         matrix-free assembly (:159-199): conductances, Dirichlet sums, rhs
-//   k_coarsen          multigrid operators (no counterpart: stands in for MIC(0))
-//   k_defect ... k_update_p   Krylov solve (stands in for PCGSolver::solve, pcg_solver.h:246-295)
+//   k_coarsen, k_idiag multigrid operators (no counterpart: they stand in for MIC(0))
+//   k_defect ... k_update_p, V-cycle kernels
+//                      Krylov solve (stands in for PCGSolver::solve, pcg_solver.h:246-295)
 //   k_scatter_pressure m_pressure (:245-248)
 //   k_velocity_update  velocity update (:252-268)
 //
-// All brick kernels run one 512-thread block per 8^3 brick (thread = cell), with the XCD-aware
-// block-to-brick remap; stencil neighbours in the same brick are read at +-1/8/64, the others
-// through the brick's neighbour table.  Reductions are deterministic (fixed block order).
+// Build and output kernels work on one grid's dense arrays.  Solver kernels work on a batch: the
+// stored bricks of every grid of the batch, concatenated, one 512-thread block per brick (thread =
+// cell) with an XCD-aware block-to-brick remap.  Stencil neighbours in the same brick are read at
+// +-1/8/64, the others through the brick's neighbour table.  Reductions are per grid and
+// deterministic (per-brick partials, combined in brick order).
 #pragma once
 #include "shkz_device.hpp"
 #include "shkz_geometry.hpp"
@@ -62,12 +65,39 @@ __device__ __forceinline__ T apply_row(const OpDesc<T> &A, const T *__restrict__
     return diag * pc - off;
 }
 
+// Sum of g_f * v_q over the six faces of cell c.
+template <class T>
+__device__ __forceinline__ T offdiag_sum(const OpDesc<T> &A, const T *__restrict__ v, long c, const long idx[6]) {
+    const T gxp = ld(A.gx, idx[NB_XP]), gyp = ld(A.gy, idx[NB_YP]), gzp = ld(A.gz, idx[NB_ZP]);
+    return A.gx[c] * ld(v, idx[NB_XM]) + gxp * ld(v, idx[NB_XP]) + A.gy[c] * ld(v, idx[NB_YM]) +
+           gyp * ld(v, idx[NB_YP]) + A.gz[c] * ld(v, idx[NB_ZM]) + gzp * ld(v, idx[NB_ZP]);
+}
+
 __device__ __forceinline__ int brick_or_idle(const LevelDesc &L) {
     const int b = xcd_brick(blockIdx.x, gridDim.x);
     return b < L.nbricks ? b : -1;
 }
 
-// ------------------------------------------------------------------------------ build
+__device__ __forceinline__ bool all_done(const Gate &Q) { return Q.ctl && Q.ctl->all_done; }
+
+// Smoothing-side V-cycle work on brick br of level L is skipped for a grid that is done or whose
+// last level this is (its coarse solve handles it).
+__device__ __forceinline__ bool skip_smooth(const Gate &Q, const LevelDesc &L, int br) {
+    const int g = L.bgrid[br];
+    return L.gl[g].coarsest || (Q.S && Q.S[g].done);
+}
+
+// Value of level vector v at cell (i,j,k) of the grid described by G (0 outside or unstored).
+template <class T>
+__device__ __forceinline__ T level_value(const LevelDesc &L, const GridLevel &G, const T *__restrict__ v, int i, int j,
+                                         int k) {
+    if (i < 0 || j < 0 || k < 0 || i >= G.ni || j >= G.nj || k >= G.nk) return T(0);
+    const int b = L.bmap[G.bmap_off + ((i >> 3) * G.nby + (j >> 3)) * G.nbz + (k >> 3)];
+    if (b < 0) return T(0);
+    return v[(long)b * BRICK_CELLS + (((i & 7) * 8 + (j & 7)) * 8 + (k & 7))];
+}
+
+// ------------------------------------------------------------------------------ build (one grid)
 __global__ void k_reset_ints(int *a, int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) a[i] = 0;
@@ -157,19 +187,20 @@ __global__ __launch_bounds__(BLOCK) void k_flags(GridDesc G, int nbx, int nby, i
     if (t == 0) bflag[bid] = (uint8_t)unknown;
 }
 
-// Matrix-free assembly on the stored bricks of level 0 (macpressuresolver3.cpp:159-199).
+// Matrix-free assembly on the stored bricks of one grid at level 0 (macpressuresolver3.cpp:159-199).
 // For an unknown c and each in-bounds neighbour q (+x,-x,+y,-y,+z,-z): with area>0 and rho>0
 // the conductance w = dt*area/(dx*dx*rho) joins the diagonal, and couples c to q when q is
 // liquid (then q is an unknown too); with area>0 the rhs gains -sgn*area*u/dx.  Stored: the
 // low-face couplings gx/gy/gz and dir = the sum of conductances to non-unknown neighbours
 // (so diag = dir + sum of the six couplings); dir = -1 marks a non-unknown.
-__global__ __launch_bounds__(BLOCK) void k_operator(GridDesc G, LevelDesc L, const uint8_t *__restrict__ cflag,
+__global__ __launch_bounds__(BLOCK) void k_operator(GridDesc G, LevelDesc L, GridLevel GL, const uint8_t *__restrict__ cflag,
                                                     double *dir64, double *gx64, double *gy64, double *gz64,
                                                     float *dir32, float *gx32, float *gy32, float *gz32, double *b,
                                                     double *x, const double *__restrict__ warm, double rhs_correct) {
 #pragma clang fp contract(off)
-    const int br = brick_or_idle(L);
-    if (br < 0) return;
+    const int local = xcd_brick(blockIdx.x, gridDim.x);
+    if (local >= GL.brick_end - GL.brick_start) return;
+    const int br = GL.brick_start + local;
     const int t = threadIdx.x;
     const int bx = L.bxyz[3 * br], by = L.bxyz[3 * br + 1], bz = L.bxyz[3 * br + 2];
     const int i = bx * BRICK + (t >> 6), j = by * BRICK + ((t >> 3) & 7), k = bz * BRICK + (t & 7);
@@ -220,6 +251,7 @@ __global__ __launch_bounds__(BLOCK) void k_operator(GridDesc G, LevelDesc L, con
     }
 }
 
+// ------------------------------------------------------------------------------ hierarchy (batch)
 // Coarse operator from the piecewise-constant aggregation P0 of the 2x2x2 children: the couplings
 // are 1/2 of the Galerkin product (g_H = 1/2 sum of the four fine couplings across the coarse
 // face, the usual rediscretisation factor for a 2h grid), the Dirichlet (air) part is kept whole
@@ -232,15 +264,15 @@ __global__ __launch_bounds__(BLOCK) void k_coarsen(LevelDesc Lf, OpDesc<Ti> Af, 
     const int br = brick_or_idle(Lc);
     if (br < 0) return;
     const int t = threadIdx.x;
+    const GridLevel F = Lf.gl[Lc.bgrid[br]];
     const int I = Lc.bxyz[3 * br] * BRICK + (t >> 6), J = Lc.bxyz[3 * br + 1] * BRICK + ((t >> 3) & 7),
               K = Lc.bxyz[3 * br + 2] * BRICK + (t & 7);
     const long c = (long)br * BRICK_CELLS + t;
     double sdir = 0.0, sg[3] = {0.0, 0.0, 0.0};
     bool any = false;
     const int fi0 = 2 * I, fj0 = 2 * J, fk0 = 2 * K;
-    const int fbx = fi0 >> 3, fby = fj0 >> 3, fbz = fk0 >> 3;
-    if (fi0 < Lf.ni && fj0 < Lf.nj && fk0 < Lf.nk) {
-        const int fb = Lf.bmap[((long)fbx * Lf.nby + fby) * Lf.nbz + fbz];
+    if (fi0 < F.ni && fj0 < F.nj && fk0 < F.nk) {
+        const int fb = Lf.bmap[F.bmap_off + ((fi0 >> 3) * F.nby + (fj0 >> 3)) * F.nbz + (fk0 >> 3)];
         if (fb >= 0) {
             for (int a = 0; a < 2; ++a)
                 for (int bb = 0; bb < 2; ++bb)
@@ -264,274 +296,6 @@ __global__ __launch_bounds__(BLOCK) void k_coarsen(LevelDesc Lf, OpDesc<Ti> Af, 
     gz[c] = any ? (To)(0.5 * sg[2]) : (To)0;
 }
 
-// Number of stored cells that are unknowns (dir >= 0).
-__global__ void k_count_unknowns(const void *dir, int is64, size_t n, unsigned long long *out) {
-    unsigned long long c = 0;
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-        const double v = is64 ? ((const double *)dir)[i] : (double)((const float *)dir)[i];
-        c += (v >= 0.0) ? 1ull : 0ull;
-    }
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off, 64);
-    __shared__ unsigned long long s[16];
-    if ((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = c;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        unsigned long long t = 0;
-        for (int i = 0; i < (int)(blockDim.x >> 6); ++i) t += s[i];
-        if (t) atomicAdd(out, t);
-    }
-}
-
-// ------------------------------------------------------------------------------ Krylov
-// r = b - A x with per-block ||r||_inf and ||r||^2 partials (finalized by FIN_DEFECT*).
-template <class T>
-__global__ __launch_bounds__(BLOCK) void k_defect(LevelDesc L, OpDesc<T> A, const T *__restrict__ b,
-                                                  const T *__restrict__ x, T *__restrict__ r, RedBuf R) {
-    __shared__ double lds[16];
-    const int br = brick_or_idle(L);
-    double mx = 0.0, sq = 0.0;
-    if (br >= 0) {
-        const int t = threadIdx.x;
-        const long c = (long)br * BRICK_CELLS + t;
-        const T dirc = A.dir[c];
-        T out = 0;
-        if (dirc >= 0) {
-            const Nbrs N = load_nbrs(L, br);
-            long idx[6];
-            nb_index(N, t, idx);
-            T diag;
-            out = b[c] - apply_row(A, x, c, idx, dirc, x[c], diag);
-        }
-        r[c] = out;
-        mx = fabs((double)out);
-        sq = (double)out * (double)out;
-    }
-    mx = block_max(mx, lds);
-    sq = block_sum(sq, lds);
-    store_partials(R, mx, sq);
-}
-
-// Inner (FP32) solve set-up of the defect correction: r32 = (float) r64, c = 0, with
-// ||r32|| partials (finalized by FIN_INNER_INIT).
-__global__ __launch_bounds__(BLOCK) void k_inner_init(LevelDesc L, const double *__restrict__ r64, float *__restrict__ r32,
-                                                      float *__restrict__ corr, RedBuf R) {
-    __shared__ double lds[16];
-    const int br = brick_or_idle(L);
-    double mx = 0.0, sq = 0.0;
-    if (br >= 0) {
-        const long c = (long)br * BRICK_CELLS + threadIdx.x;
-        const float v = (float)r64[c];
-        r32[c] = v;
-        corr[c] = 0.0f;
-        mx = fabs((double)v);
-        sq = (double)v * (double)v;
-    }
-    mx = block_max(mx, lds);
-    sq = block_sum(sq, lds);
-    store_partials(R, mx, sq);
-}
-
-// q = A p and p.q partials (FIN_PQ forms alpha = (r.z) / (p.q)).
-template <class T>
-__global__ __launch_bounds__(BLOCK) void k_apply_dot(LevelDesc L, OpDesc<T> A, const T *__restrict__ p,
-                                                     T *__restrict__ q, const SolveScalars *S, RedBuf R) {
-    __shared__ double lds[16];
-    if (S->done) return;
-    const int br = brick_or_idle(L);
-    double part = 0.0;
-    if (br >= 0) {
-        const int t = threadIdx.x;
-        const long c = (long)br * BRICK_CELLS + t;
-        const T dirc = A.dir[c];
-        T out = 0;
-        if (dirc >= 0) {
-            const Nbrs N = load_nbrs(L, br);
-            long idx[6];
-            nb_index(N, t, idx);
-            const T pc = p[c];
-            T diag;
-            out = apply_row(A, p, c, idx, dirc, pc, diag);
-            part = (double)pc * (double)out;
-        }
-        q[c] = out;
-    }
-    part = block_sum(part, lds);
-    store_partials(R, part, 0.0);
-}
-
-// x += alpha p, r -= alpha q with ||r||_inf and r.r partials (FIN_UPDATE runs the convergence
-// test of pcg_solver.h:278-285).
-template <class T>
-__global__ __launch_bounds__(BLOCK) void k_cg_update(LevelDesc L, T *__restrict__ x, T *__restrict__ r,
-                                                     const T *__restrict__ p, const T *__restrict__ q,
-                                                     const SolveScalars *S, RedBuf R) {
-    __shared__ double lds[16];
-    if (S->done) return;
-    const int br = brick_or_idle(L);
-    double mx = 0.0, sq = 0.0;
-    if (br >= 0) {
-        const long c = (long)br * BRICK_CELLS + threadIdx.x;
-        const T alpha = (T)S->alpha;
-        x[c] += alpha * p[c];
-        const T rv = r[c] - alpha * q[c];
-        r[c] = rv;
-        mx = fabs((double)rv);
-        sq = (double)rv * (double)rv;
-    }
-    mx = block_max(mx, lds);
-    sq = block_sum(sq, lds);
-    store_partials(R, mx, sq);
-}
-
-// r.z partials for the next direction (FIN_RZ forms beta).
-template <class T>
-__global__ __launch_bounds__(BLOCK) void k_dot_rz(LevelDesc L, const T *__restrict__ r, const T *__restrict__ z,
-                                                  const SolveScalars *S, RedBuf R) {
-    __shared__ double lds[16];
-    if (S->done) return;
-    const int br = brick_or_idle(L);
-    double part = 0.0;
-    if (br >= 0) {
-        const long c = (long)br * BRICK_CELLS + threadIdx.x;
-        part = (double)r[c] * (double)z[c];
-    }
-    part = block_sum(part, lds);
-    store_partials(R, part, 0.0);
-}
-
-enum FinOp {
-    FIN_PQ = 0,           // alpha = rz / pq
-    FIN_UPDATE = 1,       // rinf, rl2, iteration count, convergence test
-    FIN_UPDATE_CG = 2,    // FIN_UPDATE plus beta = r.r / rz (unpreconditioned CG)
-    FIN_RZ_FIRST = 3,     // rz of the first direction
-    FIN_RZ = 4,           // beta = rz_new / rz
-    FIN_DEFECT_FIRST = 5, // ||r0|| and the absolute tolerance  (param = relative residual)
-    FIN_DEFECT = 6,       // ||r|| only
-    FIN_INNER_INIT = 7    // inner solve: ||r0||, tol = param * ||r0||, maxit = iparam
-};
-
-// One block combines the partials of `nblocks` blocks in block order and updates S.
-__global__ __launch_bounds__(1024) void k_finalize(const double *__restrict__ partials, int nblocks, SolveScalars *S,
-                                                   int op, double param, int iparam) {
-    __shared__ double lds[16];
-    const bool always = (op == FIN_DEFECT_FIRST || op == FIN_DEFECT || op == FIN_INNER_INIT);
-    if (!always && S->done) return;
-    const bool a_is_max = (op == FIN_UPDATE || op == FIN_UPDATE_CG || op == FIN_DEFECT_FIRST || op == FIN_DEFECT ||
-                           op == FIN_INNER_INIT);
-    double a = 0.0, b = 0.0;
-    const double2 *pp = reinterpret_cast<const double2 *>(partials);
-    int i = threadIdx.x;
-    for (; i + 3 * (int)blockDim.x < nblocks; i += 4 * blockDim.x) {
-        double2 v[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) v[u] = pp[i + u * blockDim.x];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            a = a_is_max ? fmax(a, v[u].x) : a + v[u].x;
-            b += v[u].y;
-        }
-    }
-    for (; i < nblocks; i += blockDim.x) {
-        const double2 v = pp[i];
-        a = a_is_max ? fmax(a, v.x) : a + v.x;
-        b += v.y;
-    }
-    a = a_is_max ? block_max(a, lds) : block_sum(a, lds);
-    b = block_sum(b, lds);
-    if (threadIdx.x != 0) return;
-    switch (op) {
-    case FIN_PQ: {
-        S->pq = a;
-        const double al = S->rz / a;
-        S->alpha = al;
-        if (!(al == al) || al - al != 0.0) {
-            S->status = 1;
-            S->done = 1;
-        }
-        break;
-    }
-    case FIN_UPDATE:
-    case FIN_UPDATE_CG: {
-        const double rl2 = sqrt(b);
-        S->rinf = a;
-        S->rl2 = rl2;
-        const int it = S->iter + 1;
-        S->iter = it;
-        const double cur = S->norm == 0 ? a : rl2;
-        if (!(b == b)) {
-            S->status = 1;
-            S->done = 1;
-        } else if (cur <= S->tol) {
-            S->converged = 1;
-            S->done = 1;
-        } else if (it >= S->maxit) {
-            S->done = 1;
-        }
-        if (op == FIN_UPDATE_CG) {
-            S->beta = b / S->rz;
-            S->rz = b;
-        }
-        break;
-    }
-    case FIN_RZ_FIRST:
-    case FIN_RZ: {
-        S->beta = op == FIN_RZ_FIRST ? 0.0 : a / S->rz;
-        S->rz = a;
-        if (!(a == a)) {
-            S->status = 1;
-            S->done = 1;
-        } else if (op == FIN_RZ_FIRST && a == 0.0) {
-            S->done = 1;
-        }
-        break;
-    }
-    case FIN_DEFECT_FIRST:
-    case FIN_DEFECT:
-    case FIN_INNER_INIT: {
-        const double rl2 = sqrt(b);
-        S->rinf = a;
-        S->rl2 = rl2;
-        if (op != FIN_DEFECT) {
-            S->r0inf = a;
-            S->r0l2 = rl2;
-            const double r0 = S->norm == 0 ? a : rl2;
-            S->tol = param * r0;
-            S->iter = 0;
-            if (op == FIN_INNER_INIT) S->maxit = iparam;
-            const bool bad = !(b == b);
-            S->status = bad ? 1 : 0;
-            S->converged = (r0 == 0.0) ? 1 : 0;
-            S->done = (r0 == 0.0 || bad || S->maxit <= 0) ? 1 : 0;
-        }
-        break;
-    }
-    default:
-        break;
-    }
-}
-
-// p = z + beta p (first: p = z).
-template <class T>
-__global__ __launch_bounds__(BLOCK) void k_update_p(LevelDesc L, const T *__restrict__ z, T *__restrict__ p,
-                                                    const SolveScalars *S, int first) {
-    if (S->done) return;
-    const int br = brick_or_idle(L);
-    if (br < 0) return;
-    const long c = (long)br * BRICK_CELLS + threadIdx.x;
-    p[c] = first ? z[c] : z[c] + (T)S->beta * p[c];
-}
-
-// x64 += (double) corr
-__global__ __launch_bounds__(BLOCK) void k_add_correction(LevelDesc L, double *__restrict__ x,
-                                                          const float *__restrict__ corr) {
-    const int br = brick_or_idle(L);
-    if (br < 0) return;
-    const long c = (long)br * BRICK_CELLS + threadIdx.x;
-    x[c] += (double)corr[c];
-}
-
-// ------------------------------------------------------------------------------ V-cycle
 // The smoothers use a stored inverse diagonal: damped Jacobi is
 //   z' = (1 - omega) z + omega * idiag * (rhs + sum_f g_f z_q),
 // and idiag = 0 marks a non-unknown (or a cell whose diagonal vanished).
@@ -554,23 +318,319 @@ __global__ __launch_bounds__(BLOCK) void k_idiag(LevelDesc L, OpDesc<T> A, T *__
     idiag[c] = out;
 }
 
-// Sum of g_f * v_q over the six faces of cell c.
+// Number of unknowns (dir >= 0) in every stored brick.
 template <class T>
-__device__ __forceinline__ T offdiag_sum(const OpDesc<T> &A, const T *__restrict__ v, long c, const long idx[6]) {
-    const T gxp = ld(A.gx, idx[NB_XP]), gyp = ld(A.gy, idx[NB_YP]), gzp = ld(A.gz, idx[NB_ZP]);
-    return A.gx[c] * ld(v, idx[NB_XM]) + gxp * ld(v, idx[NB_XP]) + A.gy[c] * ld(v, idx[NB_YM]) +
-           gyp * ld(v, idx[NB_YP]) + A.gz[c] * ld(v, idx[NB_ZM]) + gzp * ld(v, idx[NB_ZP]);
+__global__ __launch_bounds__(BLOCK) void k_brick_counts(LevelDesc L, const T *__restrict__ dir, int *counts) {
+    const int br = brick_or_idle(L);
+    if (br < 0) return;
+    const int n = __syncthreads_count(dir[(long)br * BRICK_CELLS + threadIdx.x] >= 0);
+    if (threadIdx.x == 0) counts[br] = n;
 }
 
+// ------------------------------------------------------------------------------ Krylov (batch)
+__device__ __forceinline__ int grid_of(const LevelDesc &L, int br) { return L.bgrid[br]; }
+
+// r = b - A x with per-brick ||r||_inf and ||r||^2 partials (finalized by FIN_DEFECT*).
+template <class T>
+__global__ __launch_bounds__(BLOCK) void k_defect(LevelDesc L, OpDesc<T> A, const T *__restrict__ b,
+                                                  const T *__restrict__ x, T *__restrict__ r, RedBuf R) {
+    __shared__ double lds[16];
+    const int br = brick_or_idle(L);
+    if (br < 0) return;
+    const int t = threadIdx.x;
+    const long c = (long)br * BRICK_CELLS + t;
+    const T dirc = A.dir[c];
+    T out = 0;
+    if (dirc >= 0) {
+        const Nbrs N = load_nbrs(L, br);
+        long idx[6];
+        nb_index(N, t, idx);
+        T diag;
+        out = b[c] - apply_row(A, x, c, idx, dirc, x[c], diag);
+    }
+    r[c] = out;
+    const double mx = block_max(fabs((double)out), lds);
+    const double sq = block_sum((double)out * (double)out, lds);
+    store_partials(R, br, mx, sq);
+}
+
+// Inner (FP32) solve set-up of the defect correction: r32 = (float) r64, c = 0, with ||r32||
+// partials (finalized by FIN_INNER_INIT).
+__global__ __launch_bounds__(BLOCK) void k_inner_init(LevelDesc L, const double *__restrict__ r64, float *__restrict__ r32,
+                                                      float *__restrict__ corr, RedBuf R) {
+    __shared__ double lds[16];
+    const int br = brick_or_idle(L);
+    if (br < 0) return;
+    const long c = (long)br * BRICK_CELLS + threadIdx.x;
+    const float v = (float)r64[c];
+    r32[c] = v;
+    corr[c] = 0.0f;
+    const double mx = block_max(fabs((double)v), lds);
+    const double sq = block_sum((double)v * (double)v, lds);
+    store_partials(R, br, mx, sq);
+}
+
+// q = A p and p.q partials (FIN_PQ forms alpha = (r.z) / (p.q)).
+template <class T>
+__global__ __launch_bounds__(BLOCK) void k_apply_dot(LevelDesc L, OpDesc<T> A, const T *__restrict__ p,
+                                                     T *__restrict__ q, const SolveScalars *S, const BatchCtl *ctl,
+                                                     RedBuf R) {
+    __shared__ double lds[16];
+    if (ctl->all_done) return;
+    const int br = brick_or_idle(L);
+    if (br < 0 || S[grid_of(L, br)].done) return;
+    const int t = threadIdx.x;
+    const long c = (long)br * BRICK_CELLS + t;
+    const T dirc = A.dir[c];
+    T out = 0;
+    double part = 0.0;
+    if (dirc >= 0) {
+        const Nbrs N = load_nbrs(L, br);
+        long idx[6];
+        nb_index(N, t, idx);
+        const T pc = p[c];
+        T diag;
+        out = apply_row(A, p, c, idx, dirc, pc, diag);
+        part = (double)pc * (double)out;
+    }
+    q[c] = out;
+    part = block_sum(part, lds);
+    store_partials(R, br, part, 0.0);
+}
+
+// x += alpha p, r -= alpha q with ||r||_inf and r.r partials (FIN_UPDATE runs the convergence
+// test of pcg_solver.h:278-285).
+template <class T>
+__global__ __launch_bounds__(BLOCK) void k_cg_update(LevelDesc L, T *__restrict__ x, T *__restrict__ r,
+                                                     const T *__restrict__ p, const T *__restrict__ q,
+                                                     const SolveScalars *S, const BatchCtl *ctl, RedBuf R) {
+    __shared__ double lds[16];
+    if (ctl->all_done) return;
+    const int br = brick_or_idle(L);
+    if (br < 0) return;
+    const SolveScalars &Sg = S[grid_of(L, br)];
+    if (Sg.done) return;
+    const long c = (long)br * BRICK_CELLS + threadIdx.x;
+    const T alpha = (T)Sg.alpha;
+    x[c] += alpha * p[c];
+    const T rv = r[c] - alpha * q[c];
+    r[c] = rv;
+    const double mx = block_max(fabs((double)rv), lds);
+    const double sq = block_sum((double)rv * (double)rv, lds);
+    store_partials(R, br, mx, sq);
+}
+
+// r.z partials for the next direction (FIN_RZ forms beta).
+template <class T>
+__global__ __launch_bounds__(BLOCK) void k_dot_rz(LevelDesc L, const T *__restrict__ r, const T *__restrict__ z,
+                                                  const SolveScalars *S, const BatchCtl *ctl, RedBuf R) {
+    __shared__ double lds[16];
+    if (ctl->all_done) return;
+    const int br = brick_or_idle(L);
+    if (br < 0 || S[grid_of(L, br)].done) return;
+    const long c = (long)br * BRICK_CELLS + threadIdx.x;
+    const double part = block_sum((double)r[c] * (double)z[c], lds);
+    store_partials(R, br, part, 0.0);
+}
+
+// p = z + beta p (first: p = z).
+template <class T>
+__global__ __launch_bounds__(BLOCK) void k_update_p(LevelDesc L, const T *__restrict__ z, T *__restrict__ p,
+                                                    const SolveScalars *S, const BatchCtl *ctl, int first) {
+    if (ctl->all_done) return;
+    const int br = brick_or_idle(L);
+    if (br < 0) return;
+    const SolveScalars &Sg = S[grid_of(L, br)];
+    if (Sg.done) return;
+    const long c = (long)br * BRICK_CELLS + threadIdx.x;
+    p[c] = first ? z[c] : z[c] + (T)Sg.beta * p[c];
+}
+
+// x64 += (double) corr
+__global__ __launch_bounds__(BLOCK) void k_add_correction(LevelDesc L, double *__restrict__ x,
+                                                          const float *__restrict__ corr) {
+    const int br = brick_or_idle(L);
+    if (br < 0) return;
+    const long c = (long)br * BRICK_CELLS + threadIdx.x;
+    x[c] += (double)corr[c];
+}
+
+enum FinOp {
+    FIN_PQ = 0,           // alpha = rz / pq
+    FIN_UPDATE = 1,       // rinf, rl2, iteration count, convergence test
+    FIN_UPDATE_CG = 2,    // FIN_UPDATE plus beta = r.r / rz (unpreconditioned CG)
+    FIN_RZ_FIRST = 3,     // rz of the first direction
+    FIN_RZ = 4,           // beta = rz_new / rz
+    FIN_DEFECT_FIRST = 5, // ||r0|| and the absolute tolerance  (param = relative residual)
+    FIN_DEFECT = 6,       // ||r|| only
+    FIN_INNER_INIT = 7,   // inner solve: ||r0||, tol = max(param, tol_outer/||r_outer||) * ||r0||,
+                          // maxit = the grid's iterations left (0 when its defect correction is done)
+    FIN_OUTER = 8         // defect correction: ||r||, iterations += the inner solve's; done when the
+                          // defect converged, the iterations ran out or the defect fell by < 10 %
+};
+
+// Resets a control block for a new Krylov solve over ngrids grids (a kernel, so that it orders
+// with the work around it and needs no host buffer).
+__global__ void k_init_ctl(BatchCtl *ctl, int ngrids) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        ctl->ngrids = ngrids;
+        ctl->ndone = 0;
+        ctl->all_done = 0;
+        ctl->pad = 0;
+    }
+}
+
+__device__ __forceinline__ void mark_done(SolveScalars &S, BatchCtl *ctl) {
+    S.done = 1;
+    if (ctl) {
+        const int n = atomicAdd(&ctl->ndone, 1) + 1;
+        if (n >= ctl->ngrids) ctl->all_done = 1;
+    }
+}
+
+// One block per grid combines the partials of the grid's bricks in brick order and updates the
+// grid's scalars.  Grids that finished are left alone (their kernels stored no partials).
+__global__ __launch_bounds__(1024) void k_finalize(const double *__restrict__ partials, LevelDesc L0,
+                                                   SolveScalars *Sarr, BatchCtl *ctl, int op, double param, int iparam,
+                                                   const SolveScalars *outer) {
+    __shared__ double lds[16];
+    const int g = blockIdx.x;
+    SolveScalars &S = Sarr[g];
+    const bool always = (op == FIN_DEFECT_FIRST || op == FIN_DEFECT || op == FIN_INNER_INIT);
+    if (!always && (S.done || (ctl && ctl->all_done))) return;
+    const GridLevel G = L0.gl[g];
+    const bool a_is_max = (op == FIN_UPDATE || op == FIN_UPDATE_CG || op == FIN_DEFECT_FIRST || op == FIN_DEFECT ||
+                           op == FIN_INNER_INIT || op == FIN_OUTER);
+    double a = 0.0, b = 0.0;
+    const double2 *pp = reinterpret_cast<const double2 *>(partials);
+    int i = G.brick_start + threadIdx.x;
+    for (; i + 3 * (int)blockDim.x < G.brick_end; i += 4 * blockDim.x) {
+        double2 v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) v[u] = pp[i + u * blockDim.x];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            a = a_is_max ? fmax(a, v[u].x) : a + v[u].x;
+            b += v[u].y;
+        }
+    }
+    for (; i < G.brick_end; i += blockDim.x) {
+        const double2 v = pp[i];
+        a = a_is_max ? fmax(a, v.x) : a + v.x;
+        b += v.y;
+    }
+    a = a_is_max ? block_max(a, lds) : block_sum(a, lds);
+    b = block_sum(b, lds);
+    if (threadIdx.x != 0) return;
+    switch (op) {
+    case FIN_PQ: {
+        S.pq = a;
+        const double al = S.rz / a;
+        S.alpha = al;
+        if (!(al == al) || al - al != 0.0) {
+            S.status = 1;
+            mark_done(S, ctl);
+        }
+        break;
+    }
+    case FIN_UPDATE:
+    case FIN_UPDATE_CG: {
+        const double rl2 = sqrt(b);
+        S.rinf = a;
+        S.rl2 = rl2;
+        const int it = S.iter + 1;
+        S.iter = it;
+        const double cur = S.norm == 0 ? a : rl2;
+        if (op == FIN_UPDATE_CG) {
+            S.beta = b / S.rz;
+            S.rz = b;
+        }
+        if (!(b == b)) {
+            S.status = 1;
+            mark_done(S, ctl);
+        } else if (cur <= S.tol) {
+            S.converged = 1;
+            mark_done(S, ctl);
+        } else if (it >= S.maxit) {
+            mark_done(S, ctl);
+        }
+        break;
+    }
+    case FIN_RZ_FIRST:
+    case FIN_RZ: {
+        S.beta = op == FIN_RZ_FIRST ? 0.0 : a / S.rz;
+        S.rz = a;
+        if (!(a == a)) {
+            S.status = 1;
+            mark_done(S, ctl);
+        } else if (op == FIN_RZ_FIRST && a == 0.0) {
+            mark_done(S, ctl);
+        }
+        break;
+    }
+    case FIN_DEFECT_FIRST:
+    case FIN_DEFECT:
+    case FIN_INNER_INIT: {
+        const double rl2 = sqrt(b);
+        S.rinf = a;
+        S.rl2 = rl2;
+        if (op != FIN_DEFECT) {
+            S.r0inf = a;
+            S.r0l2 = rl2;
+            const double r0 = S.norm == 0 ? a : rl2;
+            double rel = param;
+            int maxit = S.maxit;
+            if (op == FIN_INNER_INIT) {
+                const SolveScalars &O = outer[g];
+                const double ocur = O.norm == 0 ? O.rinf : O.rl2;
+                rel = ocur > 0.0 ? fmax(param, O.tol / ocur) : param;
+                maxit = (O.done || ocur <= O.tol || O.status) ? 0 : max(1, O.maxit - O.iter);
+                S.maxit = maxit;
+            }
+            S.tol = rel * r0;
+            S.iter = 0;
+            S.done = 0;
+            const bool bad = !(b == b);
+            S.status = bad ? 1 : 0;
+            S.converged = (r0 == 0.0) ? 1 : 0;
+            if (r0 == 0.0 || bad || maxit <= 0) mark_done(S, ctl);
+        }
+        break;
+    }
+    case FIN_OUTER: {
+        const SolveScalars &I = outer[g];
+        const double prev = S.norm == 0 ? S.rinf : S.rl2;
+        const double rl2 = sqrt(b);
+        S.rinf = a;
+        S.rl2 = rl2;
+        S.iter += I.iter;
+        const double cur = S.norm == 0 ? a : rl2;
+        if (!(b == b) || I.status) {
+            S.status = 1;
+            mark_done(S, ctl);
+        } else if (cur <= S.tol) {
+            S.converged = 1;
+            mark_done(S, ctl);
+        } else if (S.iter >= S.maxit || cur > 0.9 * prev) {
+            mark_done(S, ctl);
+        }
+        break;
+    }
+    default:
+        break;
+    }
+}
+
+// ------------------------------------------------------------------------------ V-cycle (batch)
 // z = omega * idiag * rhs (one damped-Jacobi sweep from a zero guess).
 // FINEST: 1 on the finest level (a distinct symbol for profiling), 0 on coarse levels.
 template <class T, int FINEST>
 __global__ __launch_bounds__(BLOCK) void k_smooth_first(LevelDesc L, const T *__restrict__ idiag,
                                                         const T *__restrict__ rhs, T *__restrict__ z, T omega,
-                                                        const SolveScalars *S) {
-    if (S && S->done) return;
+                                                        Gate Q) {
+    if (all_done(Q)) return;
     const int br = brick_or_idle(L);
-    if (br < 0) return;
+    if (br < 0 || skip_smooth(Q, L, br)) return;
     const long c = (long)br * BRICK_CELLS + threadIdx.x;
     z[c] = omega * idiag[c] * rhs[c];
 }
@@ -580,10 +640,10 @@ __global__ __launch_bounds__(BLOCK) void k_smooth_first(LevelDesc L, const T *__
 template <class T, int FINEST>
 __global__ __launch_bounds__(BLOCK) void k_smooth2(LevelDesc L, OpDesc<T> A, const T *__restrict__ idiag,
                                                    const T *__restrict__ rhs, T *__restrict__ z, T omega,
-                                                   const SolveScalars *S) {
-    if (S && S->done) return;
+                                                   Gate Q) {
+    if (all_done(Q)) return;
     const int br = brick_or_idle(L);
-    if (br < 0) return;
+    if (br < 0 || skip_smooth(Q, L, br)) return;
     const int t = threadIdx.x;
     const long c = (long)br * BRICK_CELLS + t;
     const T id = idiag[c];
@@ -607,42 +667,38 @@ __global__ __launch_bounds__(BLOCK) void k_smooth2(LevelDesc L, OpDesc<T> A, con
 template <class T, int FINEST, int DOT>
 __global__ __launch_bounds__(BLOCK) void k_jacobi(LevelDesc L, OpDesc<T> A, const T *__restrict__ idiag,
                                                   const T *__restrict__ rhs, const T *__restrict__ zin,
-                                                  T *__restrict__ zout, T omega, const SolveScalars *S, RedBuf R) {
+                                                  T *__restrict__ zout, T omega, Gate Q, RedBuf R) {
     __shared__ double lds[16];
-    if (S && S->done) return;
+    if (all_done(Q)) return;
     const int br = brick_or_idle(L);
+    if (br < 0 || skip_smooth(Q, L, br)) return;
+    const int t = threadIdx.x;
+    const long c = (long)br * BRICK_CELLS + t;
+    const T id = idiag[c];
+    T out = 0;
     double part = 0.0;
-    if (br >= 0) {
-        const int t = threadIdx.x;
-        const long c = (long)br * BRICK_CELLS + t;
-        const T id = idiag[c];
-        T out = 0;
-        if (id > 0) {
-            const Nbrs N = load_nbrs(L, br);
-            long idx[6];
-            nb_index(N, t, idx);
-            const T r = rhs[c];
-            out = (T(1) - omega) * zin[c] + omega * id * (r + offdiag_sum(A, zin, c, idx));
-            if (DOT) part = (double)r * (double)out;
-        }
-        zout[c] = out;
-    } else if (!DOT) {
-        return;
+    if (id > 0) {
+        const Nbrs N = load_nbrs(L, br);
+        long idx[6];
+        nb_index(N, t, idx);
+        const T r = rhs[c];
+        out = (T(1) - omega) * zin[c] + omega * id * (r + offdiag_sum(A, zin, c, idx));
+        if (DOT) part = (double)r * (double)out;
     }
+    zout[c] = out;
     if (DOT) {
         part = block_sum(part, lds);
-        store_partials(R, part, 0.0);
+        store_partials(R, br, part, 0.0);
     }
 }
 
 // res = rhs - A z
-// FINEST: 1 on the finest level (a distinct symbol for profiling), 0 on coarse levels.
 template <class T, int FINEST>
 __global__ __launch_bounds__(BLOCK) void k_residual(LevelDesc L, OpDesc<T> A, const T *__restrict__ rhs,
-                                                    const T *__restrict__ z, T *__restrict__ res, const SolveScalars *S) {
-    if (S && S->done) return;
+                                                    const T *__restrict__ z, T *__restrict__ res, Gate Q) {
+    if (all_done(Q)) return;
     const int br = brick_or_idle(L);
-    if (br < 0) return;
+    if (br < 0 || skip_smooth(Q, L, br)) return;
     const int t = threadIdx.x;
     const long c = (long)br * BRICK_CELLS + t;
     const T dirc = A.dir[c];
@@ -657,37 +713,29 @@ __global__ __launch_bounds__(BLOCK) void k_residual(LevelDesc L, OpDesc<T> A, co
     res[c] = out;
 }
 
-template <class T>
-__device__ __forceinline__ T level_value(const LevelDesc &L, const T *__restrict__ v, int i, int j, int k) {
-    if (i < 0 || j < 0 || k < 0 || i >= L.ni || j >= L.nj || k >= L.nk) return T(0);
-    const int b = L.bmap[((long)(i >> 3) * L.nby + (j >> 3)) * L.nbz + (k >> 3)];
-    if (b < 0) return T(0);
-    return v[(long)b * BRICK_CELLS + (((i & 7) * 8 + (j & 7)) * 8 + (k & 7))];
-}
-
 // Restriction = transpose of trilinear prolongation: coarse cell I gathers fine cells
 // 2I-1..2I+2 per axis with weights 1/4, 3/4, 3/4, 1/4.  One block per coarse brick stages the
 // 18^3 fine residuals it needs in LDS.
-// FINEST: 1 on the finest level (a distinct symbol for profiling), 0 on coarse levels.
 template <class T, int FINEST>
 __global__ __launch_bounds__(BLOCK) void k_restrict(LevelDesc Lf, LevelDesc Lc, OpDesc<T> Ac, const T *__restrict__ res,
-                                                    T *__restrict__ rhs_c, const SolveScalars *S) {
+                                                    T *__restrict__ rhs_c, Gate Q) {
     constexpr int E = 18;
     __shared__ T tile[E * E * E];
-    if (S && S->done) return;
+    __shared__ int bid[64];
+    if (all_done(Q)) return;
     const int br = brick_or_idle(Lc);
-    if (br < 0) return;
+    if (br < 0 || (Q.S && Q.S[Lc.bgrid[br]].done)) return;
     const int t = threadIdx.x;
+    const GridLevel F = Lf.gl[Lc.bgrid[br]];
     const int BX = Lc.bxyz[3 * br], BY = Lc.bxyz[3 * br + 1], BZ = Lc.bxyz[3 * br + 2];
     const int f0 = 16 * BX - 1, g0 = 16 * BY - 1, h0 = 16 * BZ - 1;
     // the 18^3 tile touches fine bricks 2B-1 .. 2B+2 per axis: look them up once
-    __shared__ int bid[64];
     if (t < 64) {
         const int a = t >> 4, b = (t >> 2) & 3, cc = t & 3;
         const int fx = 2 * BX - 1 + a, fy = 2 * BY - 1 + b, fz = 2 * BZ - 1 + cc;
         int id = -1;
-        if (fx >= 0 && fy >= 0 && fz >= 0 && fx < Lf.nbx && fy < Lf.nby && fz < Lf.nbz)
-            id = Lf.bmap[((long)fx * Lf.nby + fy) * Lf.nbz + fz];
+        if (fx >= 0 && fy >= 0 && fz >= 0 && fx < F.nbx && fy < F.nby && fz < F.nbz)
+            id = Lf.bmap[F.bmap_off + (fx * F.nby + fy) * F.nbz + fz];
         bid[t] = id;
     }
     __syncthreads();
@@ -724,26 +772,26 @@ __global__ __launch_bounds__(BLOCK) void k_restrict(LevelDesc Lf, LevelDesc Lc, 
 
 // z_f += trilinear interpolation of the coarse correction (fine unknowns only).  One block per
 // fine brick stages the 6^3 coarse values it needs in LDS.
-// FINEST: 1 on the finest level (a distinct symbol for profiling), 0 on coarse levels.
 template <class T, int FINEST>
 __global__ __launch_bounds__(BLOCK) void k_prolong(LevelDesc Lf, OpDesc<T> Af, LevelDesc Lc, const T *__restrict__ zc,
-                                                   T *__restrict__ zf, const SolveScalars *S) {
+                                                   T *__restrict__ zf, Gate Q) {
     constexpr int E = 6;
     __shared__ T tile[E * E * E];
-    if (S && S->done) return;
+    __shared__ int bid[8];
+    if (all_done(Q)) return;
     const int br = brick_or_idle(Lf);
-    if (br < 0) return;
+    if (br < 0 || skip_smooth(Q, Lf, br)) return;
     const int t = threadIdx.x;
+    const GridLevel Cg = Lc.gl[Lf.bgrid[br]];
     const int BX = Lf.bxyz[3 * br], BY = Lf.bxyz[3 * br + 1], BZ = Lf.bxyz[3 * br + 2];
     // the 6^3 tile (coarse cells 4B-1 .. 4B+4 per axis) touches coarse bricks (4B-1)>>3 and
     // (4B+4)>>3 per axis: look them up once
-    __shared__ int bid[8];
     const int cx0 = (4 * BX + 7) / 8 - 1, cy0 = (4 * BY + 7) / 8 - 1, cz0 = (4 * BZ + 7) / 8 - 1;
     if (t < 8) {
         const int fx = cx0 + (t >> 2), fy = cy0 + ((t >> 1) & 1), fz = cz0 + (t & 1);
         int id = -1;
-        if (fx >= 0 && fy >= 0 && fz >= 0 && fx < Lc.nbx && fy < Lc.nby && fz < Lc.nbz)
-            id = Lc.bmap[((long)fx * Lc.nby + fy) * Lc.nbz + fz];
+        if (fx >= 0 && fy >= 0 && fz >= 0 && fx < Cg.nbx && fy < Cg.nby && fz < Cg.nbz)
+            id = Lc.bmap[Cg.bmap_off + (fx * Cg.nby + fy) * Cg.nbz + fz];
         bid[t] = id;
     }
     __syncthreads();
@@ -751,7 +799,7 @@ __global__ __launch_bounds__(BLOCK) void k_prolong(LevelDesc Lf, OpDesc<T> Af, L
         const int c = t % E, b = (t / E) % E, a = t / (E * E);
         const int I = 4 * BX - 1 + a, J = 4 * BY - 1 + b, K = 4 * BZ - 1 + c;
         T v = 0;
-        if (I >= 0 && J >= 0 && K >= 0 && I < Lc.ni && J < Lc.nj && K < Lc.nk) {
+        if (I >= 0 && J >= 0 && K >= 0 && I < Cg.ni && J < Cg.nj && K < Cg.nk) {
             const int slot = (((I >> 3) - cx0) * 2 + ((J >> 3) - cy0)) * 2 + ((K >> 3) - cz0);
             const int id = bid[slot];
             if (id >= 0) v = zc[(long)id * BRICK_CELLS + (((I & 7) * 8 + (J & 7)) * 8 + (K & 7))];
@@ -771,16 +819,21 @@ __global__ __launch_bounds__(BLOCK) void k_prolong(LevelDesc Lf, OpDesc<T> Af, L
     zf[c] += v;
 }
 
-// Coarsest level (one brick): `sweeps` damped-Jacobi sweeps from zero, in LDS.
-template <class T>
+// Coarse solve of every grid whose last level L is (one brick): block g handles grid g with
+// `sweeps` damped-Jacobi sweeps from zero, in LDS.  DOT (finest level only): also the r.z partial
+// of the grid's brick, as k_jacobi<..., DOT> stores it for grids with coarser levels.
+template <class T, int DOT>
 __global__ __launch_bounds__(BLOCK) void k_coarse_solve(LevelDesc L, OpDesc<T> A, const T *__restrict__ rhs,
-                                                        T *__restrict__ z, int sweeps, T omega, const SolveScalars *S) {
+                                                        T *__restrict__ z, int sweeps, T omega, Gate Q, RedBuf R) {
     __shared__ T zs[2][BRICK_CELLS];
-    if (S && S->done) return;
-    if (L.nbricks < 1) return;
+    __shared__ double lds[16];
+    if (all_done(Q)) return;
+    const int g = blockIdx.x;
+    const GridLevel G = L.gl[g];
+    if (!G.coarsest || G.brick_end <= G.brick_start || (Q.S && Q.S[g].done)) return;
     const int t = threadIdx.x;
     const int li = t >> 6, lj = (t >> 3) & 7, lk = t & 7;
-    const long c = t;  // brick 0
+    const long c = (long)G.brick_start * BRICK_CELLS + t;
     const T dirc = A.dir[c];
     const T gxm = A.gx[c], gym = A.gy[c], gzm = A.gz[c];
     const T gxp = li < 7 ? A.gx[c + 64] : T(0), gyp = lj < 7 ? A.gy[c + 8] : T(0), gzp = lk < 7 ? A.gz[c + 1] : T(0);
@@ -805,262 +858,58 @@ __global__ __launch_bounds__(BLOCK) void k_coarse_solve(LevelDesc L, OpDesc<T> A
         __syncthreads();
         cur ^= 1;
     }
-    z[c] = zs[cur][t];
-}
-
-// ------------------------------------------------------------------------------ bottom of the V-cycle
-// Levels with few bricks are latency-bound: one kernel launch per smoothing sweep costs more than
-// the sweep.  The bottom kernel runs the whole V-cycle below a given level in one workgroup of
-// 1024 threads, the phases separated by workgroup barriers.  Per-cell arithmetic is the same as
-// in the per-brick kernels.
-constexpr int BOTTOM_THREADS = 1024;
-constexpr int BOTTOM_MAX = 8;
-
-template <class T>
-struct BottomLevel {
-    LevelDesc L;
-    OpDesc<T> A;
-    const T *idiag;
-    const T *rhs;
-    T *z, *tmp, *res;
-};
-
-template <class T>
-struct BottomDesc {
-    BottomLevel<T> lv[BOTTOM_MAX];
-    int nlev;
-    int pre, post, coarse_sweeps;
-    T omega;
-};
-
-template <class T>
-__device__ __forceinline__ T cell_jacobi(const LevelDesc &L, const OpDesc<T> &A, const T *idiag, const T *rhs,
-                                         const T *zin, T omega, int b, int t) {
-    const long c = (long)b * BRICK_CELLS + t;
-    const T id = idiag[c];
-    if (!(id > 0)) return T(0);
-    const Nbrs N = load_nbrs(L, b);
-    long idx[6];
-    nb_index(N, t, idx);
-    return (T(1) - omega) * zin[c] + omega * id * (rhs[c] + offdiag_sum(A, zin, c, idx));
-}
-
-template <class T>
-__device__ __forceinline__ T cell_smooth2(const LevelDesc &L, const OpDesc<T> &A, const T *idiag, const T *rhs,
-                                          T omega, int b, int t) {
-    const long c = (long)b * BRICK_CELLS + t;
-    const T id = idiag[c];
-    if (!(id > 0)) return T(0);
-    const Nbrs N = load_nbrs(L, b);
-    long idx[6];
-    nb_index(N, t, idx);
-    auto z1 = [&](long i) { return i >= 0 ? omega * idiag[i] * rhs[i] : T(0); };
-    const T gxp = ld(A.gx, idx[NB_XP]), gyp = ld(A.gy, idx[NB_YP]), gzp = ld(A.gz, idx[NB_ZP]);
-    const T off = A.gx[c] * z1(idx[NB_XM]) + gxp * z1(idx[NB_XP]) + A.gy[c] * z1(idx[NB_YM]) + gyp * z1(idx[NB_YP]) +
-                  A.gz[c] * z1(idx[NB_ZM]) + gzp * z1(idx[NB_ZP]);
-    const T r = rhs[c];
-    return (T(1) - omega) * (omega * id * r) + omega * id * (r + off);
-}
-
-template <class T>
-__device__ __forceinline__ T cell_residual(const LevelDesc &L, const OpDesc<T> &A, const T *rhs, const T *z, int b,
-                                           int t) {
-    const long c = (long)b * BRICK_CELLS + t;
-    const T dirc = A.dir[c];
-    if (!(dirc >= 0)) return T(0);
-    const Nbrs N = load_nbrs(L, b);
-    long idx[6];
-    nb_index(N, t, idx);
-    T diag;
-    return rhs[c] - apply_row(A, z, c, idx, dirc, z[c], diag);
-}
-
-template <class T>
-__device__ __forceinline__ T cell_restrict(const LevelDesc &Lf, const LevelDesc &Lc, const OpDesc<T> &Ac,
-                                           const T *res, int b, int t) {
-    const long c = (long)b * BRICK_CELLS + t;
-    if (!(Ac.dir[c] >= 0)) return T(0);
-    const int I = Lc.bxyz[3 * b] * BRICK + (t >> 6), J = Lc.bxyz[3 * b + 1] * BRICK + ((t >> 3) & 7),
-              K = Lc.bxyz[3 * b + 2] * BRICK + (t & 7);
-    const T w[4] = {T(0.25), T(0.75), T(0.75), T(0.25)};
-    T acc = 0;
-    for (int a = 0; a < 4; ++a) {
-        T acc_a = 0;
-        for (int bb = 0; bb < 4; ++bb) {
-            T acc_b = 0;
-            for (int cc = 0; cc < 4; ++cc) acc_b += w[cc] * level_value(Lf, res, 2 * I - 1 + a, 2 * J - 1 + bb, 2 * K - 1 + cc);
-            acc_a += w[bb] * acc_b;
-        }
-        acc += w[a] * acc_a;
-    }
-    return acc;
-}
-
-template <class T>
-__device__ __forceinline__ T cell_prolong(const LevelDesc &Lf, const LevelDesc &Lc, const T *zc, int b, int t) {
-    const int i = Lf.bxyz[3 * b] * BRICK + (t >> 6), j = Lf.bxyz[3 * b + 1] * BRICK + ((t >> 3) & 7),
-              k = Lf.bxyz[3 * b + 2] * BRICK + (t & 7);
-    const int I0 = i >> 1, J0 = j >> 1, K0 = k >> 1;
-    const int I1 = (i & 1) ? I0 + 1 : I0 - 1, J1 = (j & 1) ? J0 + 1 : J0 - 1, K1 = (k & 1) ? K0 + 1 : K0 - 1;
-    const T n = T(0.75), f = T(0.25);
-    return n * (n * (n * level_value(Lc, zc, I0, J0, K0) + f * level_value(Lc, zc, I0, J0, K1)) +
-                f * (n * level_value(Lc, zc, I0, J1, K0) + f * level_value(Lc, zc, I0, J1, K1))) +
-           f * (n * (n * level_value(Lc, zc, I1, J0, K0) + f * level_value(Lc, zc, I1, J0, K1)) +
-                f * (n * level_value(Lc, zc, I1, J1, K0) + f * level_value(Lc, zc, I1, J1, K1)));
-}
-
-// V-cycle on levels lv[0..nlev-1] (lv[0].rhs given, result in lv[0].z or lv[0].tmp: the host
-// knows which from the sweep counts).  The last level holds a single brick and is solved with
-// coarse_sweeps damped-Jacobi sweeps in LDS.
-template <class T>
-__global__ __launch_bounds__(BOTTOM_THREADS) void k_bottom(BottomDesc<T> B, const SolveScalars *S) {
-    __shared__ T zs[2][BRICK_CELLS];
-    if (S && S->done) return;
-    const int tid = threadIdx.x;
-    const T om = B.omega;
-    T *zcur[BOTTOM_MAX], *zoth[BOTTOM_MAX];
-    for (int l = 0; l < B.nlev; ++l) {
-        const BottomLevel<T> &V = B.lv[l];
-        const int ncell = V.L.nbricks * BRICK_CELLS;
-        if (l == B.nlev - 1) {
-            // coarsest: one brick, LDS Jacobi (threads >= 512 only take part in the barriers)
-            const int t = tid;
-            const bool act = t < BRICK_CELLS && V.L.nbricks > 0;
-            const int li = t >> 6, lj = (t >> 3) & 7, lk = t & 7;
-            T gxm = 0, gym = 0, gzm = 0, gxp = 0, gyp = 0, gzp = 0, r = 0, inv = 0, diag = 0;
-            bool unk = false;
-            if (act) {
-                const T dirc = V.A.dir[t];
-                gxm = V.A.gx[t];
-                gym = V.A.gy[t];
-                gzm = V.A.gz[t];
-                gxp = li < 7 ? V.A.gx[t + 64] : T(0);
-                gyp = lj < 7 ? V.A.gy[t + 8] : T(0);
-                gzp = lk < 7 ? V.A.gz[t + 1] : T(0);
-                diag = dirc + ((gxm + gxp) + (gym + gyp) + (gzm + gzp));
-                unk = dirc >= 0 && diag > 0;
-                r = V.rhs[t];
-                inv = unk ? om / diag : T(0);
-                zs[0][t] = unk ? inv * r : T(0);
-            }
-            __syncthreads();
-            int cur = 0;
-            for (int s = 1; s < B.coarse_sweeps; ++s) {
-                if (act) {
-                    const T *zz = zs[cur];
-                    T out = 0;
-                    if (unk) {
-                        const T off = gxm * (li > 0 ? zz[t - 64] : T(0)) + gxp * (li < 7 ? zz[t + 64] : T(0)) +
-                                      gym * (lj > 0 ? zz[t - 8] : T(0)) + gyp * (lj < 7 ? zz[t + 8] : T(0)) +
-                                      gzm * (lk > 0 ? zz[t - 1] : T(0)) + gzp * (lk < 7 ? zz[t + 1] : T(0));
-                        const T zc = zz[t];
-                        out = zc + inv * (r - (diag * zc - off));
-                    }
-                    zs[cur ^ 1][t] = out;
-                }
-                __syncthreads();
-                cur ^= 1;
-            }
-            if (act) V.z[t] = zs[cur][t];
-            __syncthreads();
-            zcur[l] = V.z;
-            zoth[l] = V.tmp;
-            break;
-        }
-        T *cur = V.z, *oth = V.tmp;
-        if (B.pre <= 1) {
-            for (int c = tid; c < ncell; c += BOTTOM_THREADS) cur[c] = om * V.idiag[c] * V.rhs[c];
-        } else {
-            for (int c = tid; c < ncell; c += BOTTOM_THREADS)
-                cur[c] = cell_smooth2(V.L, V.A, V.idiag, V.rhs, om, c >> 9, c & 511);
-            for (int s = 2; s < B.pre; ++s) {
-                __syncthreads();
-                for (int c = tid; c < ncell; c += BOTTOM_THREADS)
-                    oth[c] = cell_jacobi(V.L, V.A, V.idiag, V.rhs, (const T *)cur, om, c >> 9, c & 511);
-                T *tt = cur;
-                cur = oth;
-                oth = tt;
-            }
-        }
-        __syncthreads();
-        for (int c = tid; c < ncell; c += BOTTOM_THREADS)
-            V.res[c] = cell_residual(V.L, V.A, V.rhs, (const T *)cur, c >> 9, c & 511);
-        __syncthreads();
-        const BottomLevel<T> &W = B.lv[l + 1];
-        const int nc = W.L.nbricks * BRICK_CELLS;
-        T *wrhs = const_cast<T *>(W.rhs);
-        for (int c = tid; c < nc; c += BOTTOM_THREADS)
-            wrhs[c] = cell_restrict(V.L, W.L, W.A, (const T *)V.res, c >> 9, c & 511);
-        __syncthreads();
-        zcur[l] = cur;
-        zoth[l] = oth;
-    }
-    for (int l = B.nlev - 2; l >= 0; --l) {
-        const BottomLevel<T> &V = B.lv[l];
-        const BottomLevel<T> &W = B.lv[l + 1];
-        const int ncell = V.L.nbricks * BRICK_CELLS;
-        T *cur = zcur[l], *oth = zoth[l];
-        for (int c = tid; c < ncell; c += BOTTOM_THREADS)
-            if (V.A.dir[c] >= 0) cur[c] += cell_prolong(V.L, W.L, (const T *)zcur[l + 1], c >> 9, c & 511);
-        __syncthreads();
-        for (int s = 0; s < B.post; ++s) {
-            for (int c = tid; c < ncell; c += BOTTOM_THREADS)
-                oth[c] = cell_jacobi(V.L, V.A, V.idiag, V.rhs, (const T *)cur, om, c >> 9, c & 511);
-            __syncthreads();
-            T *tt = cur;
-            cur = oth;
-            oth = tt;
-        }
-        zcur[l] = cur;
-        zoth[l] = oth;
+    const T zo = zs[cur][t];
+    z[c] = zo;
+    if (DOT) {
+        const double part = block_sum((double)r * (double)zo, lds);
+        store_partials(R, G.brick_start, part, 0.0);
     }
 }
 
-// ------------------------------------------------------------------------------ output
+// ------------------------------------------------------------------------------ output (one grid)
 // m_pressure: dense float (and optional FP64) pressure, 0 off the unknowns
 // (macpressuresolver3.cpp:245-248).
-__global__ void k_scatter_pressure(GridDesc G, LevelDesc L, const double *__restrict__ x, float *P, double *P64) {
+__global__ void k_scatter_pressure(GridDesc G, LevelDesc L, GridLevel GL, const double *__restrict__ x, float *P,
+                                   double *P64) {
     const size_t M = (size_t)G.w * G.h * G.d;
     const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= M) return;
     const int k = (int)(c % G.d), j = (int)((c / G.d) % G.h), i = (int)(c / ((size_t)G.d * G.h));
-    const double v = level_value(L, x, i, j, k);
+    const double v = level_value(L, GL, x, i, j, k);
     if (P) P[c] = (float)v;
     if (P64) P64[c] = v;
 }
 
 // Dense export of the level-0 system: diag (0 off the unknowns), the low-face couplings and the
 // right-hand side, all cell-indexed.
-__global__ void k_export_system(GridDesc G, LevelDesc L, OpDesc<double> A, const double *__restrict__ b, double *diag,
-                                double *gx, double *gy, double *gz, double *rhs) {
+__global__ void k_export_system(GridDesc G, LevelDesc L, GridLevel GL, OpDesc<double> A, const double *__restrict__ b,
+                                double *diag, double *gx, double *gy, double *gz, double *rhs) {
     const size_t M = (size_t)G.w * G.h * G.d;
     const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= M) return;
     const int k = (int)(c % G.d), j = (int)((c / G.d) % G.h), i = (int)(c / ((size_t)G.d * G.h));
-    const double dirc = level_value(L, A.dir, i, j, k);
-    const bool unk = dirc >= 0.0 && L.bmap[((long)(i >> 3) * L.nby + (j >> 3)) * L.nbz + (k >> 3)] >= 0;
-    const double gxm = level_value(L, A.gx, i, j, k), gym = level_value(L, A.gy, i, j, k),
-                 gzm = level_value(L, A.gz, i, j, k);
-    const double gxp = level_value(L, A.gx, i + 1, j, k), gyp = level_value(L, A.gy, i, j + 1, k),
-                 gzp = level_value(L, A.gz, i, j, k + 1);
+    const bool stored = L.bmap[GL.bmap_off + ((i >> 3) * GL.nby + (j >> 3)) * GL.nbz + (k >> 3)] >= 0;
+    const double dirc = stored ? level_value(L, GL, A.dir, i, j, k) : -1.0;
+    const bool unk = dirc >= 0.0;
+    const double gxm = level_value(L, GL, A.gx, i, j, k), gym = level_value(L, GL, A.gy, i, j, k),
+                 gzm = level_value(L, GL, A.gz, i, j, k);
+    const double gxp = level_value(L, GL, A.gx, i + 1, j, k), gyp = level_value(L, GL, A.gy, i, j + 1, k),
+                 gzp = level_value(L, GL, A.gz, i, j, k + 1);
     diag[c] = unk ? dirc + ((gxm + gxp) + (gym + gyp) + (gzm + gzp)) : 0.0;
     gx[c] = gxm;
     gy[c] = gym;
     gz[c] = gzm;
-    rhs[c] = unk ? level_value(L, b, i, j, k) : 0.0;
+    rhs[c] = unk ? level_value(L, GL, b, i, j, k) : 0.0;
 }
 
-// Velocity update over every (active) face (macpressuresolver3.cpp:252-268): open faces on the
-// domain boundary become 0, interior open faces lose dt*(P1-P0)/(rho*dx) with P stored in
-// float and P1-P0 taken in float; all other faces are switched off (0).
-__global__ void k_velocity_update(GridDesc G, int dim, LevelDesc L, const double *__restrict__ x) {
+// Velocity update (macpressuresolver3.cpp:252-268) of the three faces indexed at node position
+// (i,j,k) in one thread (each level-set sample is read once for the three faces): open faces on
+// the domain boundary become 0, interior open faces lose dt*(P1-P0)/(rho*dx) with P stored in
+// float and P1-P0 taken in float; all other (active) faces are switched off (0).
+__device__ __forceinline__ void update_face(const GridDesc &G, int dim, int i, int j, int k, float p1, float p0) {
 #pragma clang fp contract(off)
-    const int W = G.w + (dim == 0), H = G.h + (dim == 1), D = G.d + (dim == 2);
-    const size_t n = (size_t)W * H * D;
-    const size_t f = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (f >= n) return;
+    const size_t f = face_index(G, dim, i, j, k);
     if (G.active[dim] && !G.active[dim][f]) return;
-    const int k = (int)(f % D), j = (int)((f / D) % H), i = (int)(f / ((size_t)D * H));
     const int pi[3] = {i, j, k};
     const int N[3] = {G.w, G.h, G.d};
     float *U = G.vel[dim];
@@ -1070,8 +919,6 @@ __global__ void k_velocity_update(GridDesc G, int dim, LevelDesc L, const double
         if (pi[dim] == 0 || pi[dim] == N[dim]) {
             U[f] = 0.0f;
         } else {
-            const float p1 = (float)level_value(L, x, i, j, k);
-            const float p0 = (float)level_value(L, x, i - (dim == 0), j - (dim == 1), k - (dim == 2));
             const float diff = p1 - p0;
             const float delta = (float)(G.dt * (double)diff / (rho * G.dx));
             U[f] = U[f] - delta;
@@ -1079,6 +926,19 @@ __global__ void k_velocity_update(GridDesc G, int dim, LevelDesc L, const double
     } else {
         U[f] = 0.0f;
     }
+}
+
+__global__ void k_velocity_update(GridDesc G, LevelDesc L, GridLevel GL, const double *__restrict__ x) {
+    const int W = G.w + 1, H = G.h + 1, D = G.d + 1;
+    const size_t n = (size_t)W * H * D;
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    const int k = (int)(e % D), j = (int)((e / D) % H), i = (int)(e / ((size_t)D * H));
+    const bool in_i = i < G.w, in_j = j < G.h, in_k = k < G.d;
+    const float pc = (in_i && in_j && in_k) ? (float)level_value(L, GL, x, i, j, k) : 0.0f;
+    if (in_j && in_k) update_face(G, 0, i, j, k, pc, (float)level_value(L, GL, x, i - 1, j, k));
+    if (in_i && in_k) update_face(G, 1, i, j, k, pc, (float)level_value(L, GL, x, i, j - 1, k));
+    if (in_i && in_j) update_face(G, 2, i, j, k, pc, (float)level_value(L, GL, x, i, j, k - 1));
 }
 
 }  // namespace shkz

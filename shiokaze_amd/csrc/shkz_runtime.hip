@@ -1,14 +1,19 @@
 // shkz_runtime.hip -- host runtime of the MI355X pressure projection and its C ABI.
 //
-// One shkz_project*() call is one macpressuresolver3::project (src/projection/
+// One projection of one grid is one macpressuresolver3::project (src/projection/
 // macpressuresolver3.cpp:50-272):
-//   fractions + labelling          k_any_negative, k_flags            (:61-156)
-//   assembly (matrix-free)         k_operator, k_coarsen              (:159-199)
-//   volume correction, warm start  host PI controller, scene state    (:203-230, :239-242)
-//   linear solve                   CG / MGPCG / mixed-precision MGPCG (:233-237 -> pcg.cpp:45-73)
+//   fractions + labelling          k_any_negative, k_flags               (:61-156)
+//   assembly (matrix-free)         k_operator, k_coarsen, k_idiag        (:159-199)
+//   volume correction, warm start  host PI controller, scene state       (:203-230, :239-242)
+//   linear solve                   CG / MGPCG / mixed-precision MGPCG    (:233-237 -> pcg.cpp:45-73)
 //   scatter + velocity update      k_scatter_pressure, k_velocity_update (:245-268)
-// The context owns every device buffer (grown on demand, never freed per call), the brick
-// hierarchy of the last problem, the per-scene resident state and the kernel timers.
+// A call may project a batch of independent grids at once.  Their stored bricks are concatenated
+// into one tile table per multigrid level, so every solver launch covers every grid of the batch,
+// while reductions, scalars, convergence and the multigrid depth stay per grid: each grid's result
+// is the one it gets when projected alone.  A single projection is a batch of one.
+//
+// The context owns every device buffer (grown on demand, never freed per call), the hierarchy of
+// the last batch, per-scene resident state, the replayable iteration graphs and the kernel timers.
 #include "../../include/shkz_mi355x.h"
 #include "shkz_kernels.hpp"
 
@@ -60,6 +65,8 @@ struct DevBuf {
     size_t cap = 0;
 };
 
+// Named device buffers, grown with 1/8 headroom in 2 MiB steps and kept across calls, under a
+// byte budget.
 struct Arena {
     std::unordered_map<std::string, DevBuf> bufs;
     size_t total = 0;
@@ -116,19 +123,7 @@ static int arena_get(Arena &A, const std::string &name, size_t count, T **out) {
     return SHKZ_OK;
 }
 
-// ------------------------------------------------------------------------ hierarchy
-struct LevelHost {
-    int ni = 0, nj = 0, nk = 0;
-    int nbx = 0, nby = 0, nbz = 0;
-    int nbricks = 0;
-    std::vector<int> bmap, bxyz, bnbr;
-    LevelDesc desc{};
-    // operator and V-cycle vectors (element type = level precision)
-    void *dir = nullptr, *gx = nullptr, *gy = nullptr, *gz = nullptr, *idiag = nullptr;
-    void *rhs = nullptr, *z = nullptr, *tmp = nullptr, *res = nullptr;
-    int64_t unknowns = 0;
-};
-
+// ------------------------------------------------------------------------ hierarchy (host side)
 static inline uint64_t spread3(uint32_t v) {
     uint64_t x = v & 0x1fffff;
     x = (x | x << 32) & 0x1f00000000ffffULL;
@@ -142,39 +137,21 @@ static inline uint64_t morton3(int x, int y, int z) {
     return spread3((uint32_t)x) << 2 | spread3((uint32_t)y) << 1 | spread3((uint32_t)z);
 }
 
-// Orders the active bricks along a Morton curve, fills the dense map and the neighbour table.
-static void finish_level(LevelHost &L, const std::vector<uint8_t> &active) {
-    std::vector<std::pair<uint64_t, int>> keys;
-    const int nb = L.nbx * L.nby * L.nbz;
-    for (int b = 0; b < nb; ++b)
-        if (active[b]) {
-            const int bz = b % L.nbz, by = (b / L.nbz) % L.nby, bx = b / (L.nbz * L.nby);
-            keys.push_back({morton3(bx, by, bz), b});
-        }
-    std::sort(keys.begin(), keys.end());
-    L.nbricks = (int)keys.size();
-    L.bmap.assign(nb, -1);
-    L.bxyz.resize(3 * (size_t)L.nbricks);
-    for (int id = 0; id < L.nbricks; ++id) {
-        const int b = keys[id].second;
-        L.bmap[b] = id;
-        L.bxyz[3 * id] = b / (L.nbz * L.nby);
-        L.bxyz[3 * id + 1] = (b / L.nbz) % L.nby;
-        L.bxyz[3 * id + 2] = b % L.nbz;
+// One grid at one level, before concatenation.  A grid that ended on a coarser level is empty here.
+struct GridLevelHost {
+    int ni = 0, nj = 0, nk = 0, nbx = 0, nby = 0, nbz = 0;
+    std::vector<uint8_t> active;  // per brick position: holds unknowns
+    bool single() const { return nbx <= 1 && nby <= 1 && nbz <= 1; }
+    bool any_active() const {
+        for (uint8_t a : active)
+            if (a) return true;
+        return false;
     }
-    L.bnbr.resize(6 * (size_t)L.nbricks);
-    const int off[6][3] = {{-1, 0, 0}, {1, 0, 0}, {0, -1, 0}, {0, 1, 0}, {0, 0, -1}, {0, 0, 1}};
-    for (int id = 0; id < L.nbricks; ++id) {
-        for (int d = 0; d < 6; ++d) {
-            const int x = L.bxyz[3 * id] + off[d][0], y = L.bxyz[3 * id + 1] + off[d][1], z = L.bxyz[3 * id + 2] + off[d][2];
-            int n = -1;
-            if (x >= 0 && y >= 0 && z >= 0 && x < L.nbx && y < L.nby && z < L.nbz) n = L.bmap[(x * L.nby + y) * L.nbz + z];
-            L.bnbr[6 * id + d] = n;
-        }
-    }
-}
+    // a grid is coarsened further while it has unknowns and spans more than one brick
+    bool coarsens() const { return !single() && any_active(); }
+};
 
-static void level_dims(LevelHost &L, int ni, int nj, int nk) {
+static void grid_level_dims(GridLevelHost &L, int ni, int nj, int nk) {
     L.ni = ni;
     L.nj = nj;
     L.nk = nk;
@@ -183,9 +160,79 @@ static void level_dims(LevelHost &L, int ni, int nj, int nk) {
     L.nbz = (nk + BRICK - 1) / BRICK;
 }
 
+// One multigrid level of the batch: every grid's stored bricks, concatenated.
+struct LevelHost {
+    int nbricks = 0;
+    bool any_coarsest = false;  // some grid ends on this level
+    std::vector<int> bmap, bxyz, bnbr, bgrid;
+    std::vector<GridLevel> gl;
+    LevelDesc desc{};
+    // operator and V-cycle vectors (element type = level precision)
+    void *dir = nullptr, *gx = nullptr, *gy = nullptr, *gz = nullptr, *idiag = nullptr;
+    void *rhs = nullptr, *z = nullptr, *tmp = nullptr, *res = nullptr;
+    int64_t unknowns = 0;
+    std::vector<int64_t> unknowns_grid;
+};
+
+// Concatenates the grids' active bricks (Morton order inside each grid) and builds the dense maps
+// and the neighbour table of the level.
+static void build_level(LevelHost &L, const std::vector<GridLevelHost> &grids) {
+    const int G = (int)grids.size();
+    L.gl.assign(G, GridLevel{});
+    L.bmap.clear();
+    L.bxyz.clear();
+    L.bgrid.clear();
+    L.any_coarsest = false;
+    int nb = 0;
+    for (int g = 0; g < G; ++g) {
+        const GridLevelHost &H = grids[g];
+        GridLevel &gl = L.gl[g];
+        gl.ni = H.ni;
+        gl.nj = H.nj;
+        gl.nk = H.nk;
+        gl.nbx = H.nbx;
+        gl.nby = H.nby;
+        gl.nbz = H.nbz;
+        gl.bmap_off = (int)L.bmap.size();
+        gl.brick_start = nb;
+        const int npos = H.nbx * H.nby * H.nbz;
+        std::vector<std::pair<uint64_t, int>> keys;
+        for (int b = 0; b < npos; ++b)
+            if (H.active[b]) {
+                const int bz = b % H.nbz, by = (b / H.nbz) % H.nby, bx = b / (H.nbz * H.nby);
+                keys.push_back({morton3(bx, by, bz), b});
+            }
+        std::sort(keys.begin(), keys.end());
+        L.bmap.resize(L.bmap.size() + npos, -1);
+        for (size_t q = 0; q < keys.size(); ++q) {
+            const int b = keys[q].second, id = nb + (int)q;
+            L.bmap[gl.bmap_off + b] = id;
+            L.bxyz.push_back(b / (H.nbz * H.nby));
+            L.bxyz.push_back((b / H.nbz) % H.nby);
+            L.bxyz.push_back(b % H.nbz);
+            L.bgrid.push_back(g);
+        }
+        nb += (int)keys.size();
+        gl.brick_end = nb;
+        gl.coarsest = (npos > 0 && H.single()) ? 1 : 0;
+        if (gl.coarsest && gl.brick_end > gl.brick_start) L.any_coarsest = true;
+    }
+    L.nbricks = nb;
+    L.bnbr.assign(6 * (size_t)nb, -1);
+    const int off[6][3] = {{-1, 0, 0}, {1, 0, 0}, {0, -1, 0}, {0, 1, 0}, {0, 0, -1}, {0, 0, 1}};
+    for (int id = 0; id < nb; ++id) {
+        const GridLevel &gl = L.gl[L.bgrid[id]];
+        for (int d = 0; d < 6; ++d) {
+            const int x = L.bxyz[3 * id] + off[d][0], y = L.bxyz[3 * id + 1] + off[d][1], z = L.bxyz[3 * id + 2] + off[d][2];
+            if (x >= 0 && y >= 0 && z >= 0 && x < gl.nbx && y < gl.nby && z < gl.nbz)
+                L.bnbr[6 * id + d] = L.bmap[gl.bmap_off + (x * gl.nby + y) * gl.nbz + z];
+        }
+    }
+}
+
 // ------------------------------------------------------------------------ timing
 struct TimedLaunch {
-    int kernel;   // index into names
+    int kernel;  // index into names
     int level;
     double bytes_per_unknown;
     double bytes_fixed;
@@ -213,13 +260,15 @@ using namespace shkz;
 struct shkz_ctx {
     int device = 0;
     Arena arena;
-    std::mutex mu;  // one projection at a time per context
+    std::mutex mu;  // one projection call at a time per context
     std::unordered_map<uint64_t, SceneState> scenes;
     double y_prev = 0.0;  // stateless calls share the module-level integrator, as m_y_prev does
-    // hierarchy of the current problem
+    // hierarchy of the current batch
     std::vector<LevelHost> levels;
-    // pinned scalars read-back
+    // pinned read-back / upload of per-grid scalars
     SolveScalars *host_scalars = nullptr;
+    int host_scalars_cap = 0;
+    BatchCtl *host_ctl = nullptr;
     // timing
     bool timing = false;
     std::vector<std::string> kernel_names;
@@ -265,6 +314,9 @@ struct Launcher {
         if (hipEventCreate(&e) != hipSuccess) return nullptr;
         return e;
     }
+    // Launches kernel<<<grid, block>>> on the work stream; with kernel timing on (and outside a
+    // graph capture) brackets it with events.  The byte model (per unknown of `level`, plus fixed)
+    // feeds the per-kernel achieved-bandwidth statistics.
     template <class K, class... Args>
     void go(const char *name, int level, double bytes_per_unknown, double bytes_fixed, dim3 grid, dim3 block, K kernel,
             Args... args) {
@@ -305,7 +357,7 @@ static void resolve_timing(shkz_ctx *C) {
 
 // ------------------------------------------------------------------------ solve plumbing
 struct Problem {
-    GridDesc G{};
+    int ngrids = 0;
     int nlev = 0;
     bool mixed = false;
     bool fp64_levels = true;  // V-cycle element type
@@ -314,11 +366,12 @@ struct Problem {
     double *b64 = nullptr, *x64 = nullptr, *r64 = nullptr;
     // Krylov vectors of the level-0 solve (element type of the solve)
     void *kr = nullptr, *kp = nullptr, *kq = nullptr, *kx = nullptr, *kz = nullptr, *ktmp = nullptr;
-    SolveScalars *S_out = nullptr, *S_in = nullptr;
+    // per-grid scalars: outer (FP64 defect / FP64 solve), inner (FP32 solve), final check
+    SolveScalars *S_out = nullptr, *S_in = nullptr, *S_fin = nullptr;
+    BatchCtl *ctl = nullptr;  // [0] outer / FP64 solve, [1] inner solve
     RedBuf red{};
-    int pre = 2, post = 2, pre_c = 2, post_c = 2, coarse_sweeps = 48;
-    int lb = 0;  // first level of the one-workgroup bottom cycle
-    double omega = 0.8;
+    int pre = 4, post = 4, pre_c = 4, post_c = 4, coarse_sweeps = 48;
+    double omega = 0.9;
 };
 
 template <class T>
@@ -326,14 +379,20 @@ static OpDesc<T> op_of(const LevelHost &L) {
     return OpDesc<T>{(const T *)L.dir, (const T *)L.gx, (const T *)L.gy, (const T *)L.gz};
 }
 
-// One V-cycle z = M r on the level-0 vectors (rhs0 -> returns the pointer holding z).
-// One V-cycle z = M r on the level-0 vectors; returns the pointer holding z.  Levels below P.lb
-// run as per-brick kernels, levels P.lb.. in one k_bottom launch.  With dot_partials, the last
-// post-smoothing sweep on the finest level also stores r.z partials in P.red (the caller then
-// finalizes them instead of running a separate dot kernel).
+// Which of the level's two vectors (z or tmp) holds the smoothed result at the end of the level's
+// post-smoothing, given the ping-pong of the sweeps: 0 = z, 1 = tmp.
+static int final_slot(int pre, int post) {
+    const int after_pre = pre <= 1 ? 0 : ((pre - 2) & 1);
+    return after_pre ^ (post & 1);
+}
+
+// One V-cycle z = M r on the level-0 vectors of every grid not done; returns the pointer holding
+// z.  Each grid descends to its own coarsest level (one brick), solved there by k_coarse_solve;
+// the smoothing kernels skip the bricks of grids that end on their level.  With dot_partials, the
+// last post-smoothing sweep on the finest level (or the coarse solve of a one-brick grid) also
+// stores r.z partials in P.red.
 template <class T>
-static T *vcycle(shkz_ctx *C, Launcher &K, Problem &P, const T *rhs0, T *z0, T *tmp0, const SolveScalars *S,
-                 bool dot_partials) {
+static T *vcycle(shkz_ctx *C, Launcher &K, Problem &P, const T *rhs0, T *z0, T *tmp0, Gate Q, bool dot_partials) {
     const int nl = P.nlev;
     std::vector<T *> zcur(nl), zoth(nl);
     const T om = (T)P.omega;
@@ -347,54 +406,45 @@ static T *vcycle(shkz_ctx *C, Launcher &K, Problem &P, const T *rhs0, T *z0, T *
         const T *id = (const T *)L.idiag;
         const dim3 g(brick_grid(L.nbricks)), b(BLOCK);
         const bool f = l == 0;
-        const int pre = f ? P.pre : P.pre_c;
-        if (l == P.lb) {
-            BottomDesc<T> B{};
-            B.nlev = nl - l;
-            B.pre = P.pre_c;
-            B.post = P.post_c;
-            B.coarse_sweeps = P.coarse_sweeps;
-            B.omega = om;
-            for (int q = 0; q < B.nlev; ++q) {
-                LevelHost &Q = C->levels[l + q];
-                BottomLevel<T> &V = B.lv[q];
-                V.L = Q.desc;
-                V.A = op_of<T>(Q);
-                V.idiag = (const T *)Q.idiag;
-                V.rhs = q == 0 ? rhs : (const T *)Q.rhs;
-                V.z = q == 0 ? z : (T *)Q.z;
-                V.tmp = q == 0 ? tmp : (T *)Q.tmp;
-                V.res = (T *)Q.res;
-            }
-            K.go("bottom", -1, 0, 0, dim3(1), dim3(BOTTOM_THREADS), k_bottom<T>, B, S);
-            // where the bottom's first level ended: each Jacobi sweep after smooth2 swaps buffers
-            const int swaps = B.nlev == 1 ? 0 : (std::max(0, P.pre_c - 2) + P.post_c);
-            zcur[l] = (swaps & 1) ? tmp : z;
-            zoth[l] = (swaps & 1) ? z : tmp;
+        const int pre = f ? P.pre : P.pre_c, post = f ? P.post : P.post_c;
+        const bool last = l == nl - 1;
+        if (L.any_coarsest) {
+            // grids ending here: their z goes where the other grids' smoothed z ends up
+            T *dst = (last || final_slot(pre, post) == 0) ? z : tmp;
+            if (f && dot_partials)
+                K.go("coarse_solve_dot", l, 6 * es, 0, dim3((unsigned)P.ngrids), b, k_coarse_solve<T, 1>, L.desc, A,
+                     rhs, dst, P.coarse_sweeps, om, Q, P.red);
+            else
+                K.go("coarse_solve", l, 6 * es, 0, dim3((unsigned)P.ngrids), b, k_coarse_solve<T, 0>, L.desc, A, rhs,
+                     dst, P.coarse_sweeps, om, Q, P.red);
+        }
+        if (last) {
+            zcur[l] = z;
+            zoth[l] = tmp;
             break;
         }
         T *cur = z, *oth = tmp;
         if (pre <= 1) {
             K.go(f ? "smooth_first_L0" : "smooth_first_Lc", l, 3 * es, 0, g, b,
-                 f ? k_smooth_first<T, 1> : k_smooth_first<T, 0>, L.desc, id, rhs, z, om, S);
+                 f ? k_smooth_first<T, 1> : k_smooth_first<T, 0>, L.desc, id, rhs, z, om, Q);
         } else {
             K.go(f ? "smooth2_L0" : "smooth2_Lc", l, 6 * es, 0, g, b, f ? k_smooth2<T, 1> : k_smooth2<T, 0>, L.desc, A,
-                 id, rhs, z, om, S);
+                 id, rhs, z, om, Q);
             for (int s = 2; s < pre; ++s) {
                 K.go(f ? "jacobi_L0" : "jacobi_Lc", l, 7 * es, 0, g, b, f ? k_jacobi<T, 1, 0> : k_jacobi<T, 0, 0>,
-                     L.desc, A, id, rhs, (const T *)cur, oth, om, S, P.red);
+                     L.desc, A, id, rhs, (const T *)cur, oth, om, Q, P.red);
                 std::swap(cur, oth);
             }
         }
         K.go(f ? "residual_L0" : "residual_Lc", l, 7 * es, 0, g, b, f ? k_residual<T, 1> : k_residual<T, 0>, L.desc, A,
-             rhs, (const T *)cur, (T *)L.res, S);
+             rhs, (const T *)cur, (T *)L.res, Q);
         LevelHost &Lc = C->levels[l + 1];
         K.go(f ? "restrict_L0" : "restrict_Lc", l, es, 2 * es * (double)Lc.unknowns, dim3(brick_grid(Lc.nbricks)), b,
-             f ? k_restrict<T, 1> : k_restrict<T, 0>, L.desc, Lc.desc, op_of<T>(Lc), (const T *)L.res, (T *)Lc.rhs, S);
+             f ? k_restrict<T, 1> : k_restrict<T, 0>, L.desc, Lc.desc, op_of<T>(Lc), (const T *)L.res, (T *)Lc.rhs, Q);
         zcur[l] = cur;
         zoth[l] = oth;
     }
-    for (int l = P.lb - 1; l >= 0; --l) {
+    for (int l = nl - 2; l >= 0; --l) {
         LevelHost &L = C->levels[l];
         LevelHost &Lc = C->levels[l + 1];
         const T *rhs = l == 0 ? rhs0 : (const T *)L.rhs;
@@ -404,16 +454,16 @@ static T *vcycle(shkz_ctx *C, Launcher &K, Problem &P, const T *rhs0, T *z0, T *
         const bool f = l == 0;
         const int post = f ? P.post : P.post_c;
         K.go(f ? "prolong_L0" : "prolong_Lc", l, 3 * es, es * (double)Lc.unknowns, g, b,
-             f ? k_prolong<T, 1> : k_prolong<T, 0>, L.desc, A, Lc.desc, (const T *)zcur[l + 1], zcur[l], S);
+             f ? k_prolong<T, 1> : k_prolong<T, 0>, L.desc, A, Lc.desc, (const T *)zcur[l + 1], zcur[l], Q);
         T *cur = zcur[l], *oth = zoth[l];
         for (int s = 0; s < post; ++s) {
             const bool last_dot = f && dot_partials && s == post - 1;
             if (last_dot)
                 K.go("jacobi_dot_L0", l, 7 * es, 0, g, b, k_jacobi<T, 1, 1>, L.desc, A, id, rhs, (const T *)cur, oth, om,
-                     S, P.red);
+                     Q, P.red);
             else
                 K.go(f ? "jacobi_L0" : "jacobi_Lc", l, 7 * es, 0, g, b, f ? k_jacobi<T, 1, 0> : k_jacobi<T, 0, 0>,
-                     L.desc, A, id, rhs, (const T *)cur, oth, om, S, P.red);
+                     L.desc, A, id, rhs, (const T *)cur, oth, om, Q, P.red);
             std::swap(cur, oth);
         }
         zcur[l] = cur;
@@ -422,27 +472,56 @@ static T *vcycle(shkz_ctx *C, Launcher &K, Problem &P, const T *rhs0, T *z0, T *
     return zcur[0];
 }
 
-static int read_scalars(shkz_ctx *C, hipStream_t s, const SolveScalars *dS, SolveScalars *out) {
-    SHKZ_HIP(hipMemcpyAsync(C->host_scalars, dS, sizeof(SolveScalars), hipMemcpyDeviceToHost, s));
+static int ensure_host_scalars(shkz_ctx *C, int n) {
+    if (C->host_scalars_cap >= n) return SHKZ_OK;
+    if (C->host_scalars) (void)hipHostFree(C->host_scalars);
+    C->host_scalars = nullptr;
+    C->host_scalars_cap = 0;
+    SHKZ_HIP(hipHostMalloc((void **)&C->host_scalars, (size_t)n * sizeof(SolveScalars), hipHostMallocDefault));
+    C->host_scalars_cap = n;
+    return SHKZ_OK;
+}
+
+// Reads n per-grid scalars (and optionally a control block) back to the host.
+static int read_scalars(shkz_ctx *C, hipStream_t s, const SolveScalars *dS, int n, std::vector<SolveScalars> &out,
+                        const BatchCtl *dctl = nullptr, BatchCtl *ctl_out = nullptr) {
+    SHKZ_TRY(ensure_host_scalars(C, n));
+    SHKZ_HIP(hipMemcpyAsync(C->host_scalars, dS, (size_t)n * sizeof(SolveScalars), hipMemcpyDeviceToHost, s));
+    if (dctl) SHKZ_HIP(hipMemcpyAsync(C->host_ctl, dctl, sizeof(BatchCtl), hipMemcpyDeviceToHost, s));
     SHKZ_HIP(hipStreamSynchronize(s));
     SHKZ_HIP(hipGetLastError());
-    *out = *C->host_scalars;
+    out.assign(C->host_scalars, C->host_scalars + n);
+    if (ctl_out) *ctl_out = *C->host_ctl;
     if (C->timing) resolve_timing(C);
     return SHKZ_OK;
 }
 
-static void finalize(Launcher &K, const Problem &P, unsigned nblocks, SolveScalars *S, int op, double param = 0.0,
-                     int iparam = 0) {
-    K.go("finalize", -1, 0, 16.0 * nblocks, dim3(1), dim3(1024), k_finalize, (const double *)P.red.partials,
-         (int)nblocks, S, op, param, iparam);
+// Uploads n fresh scalars (norm, maxit) and waits for the copy (the pinned buffer is reused).
+static int upload_scalars(shkz_ctx *C, hipStream_t s, SolveScalars *dS, int n, int norm, int maxit) {
+    SHKZ_TRY(ensure_host_scalars(C, n));
+    for (int g = 0; g < n; ++g) {
+        SolveScalars h{};
+        h.norm = norm;
+        h.maxit = maxit;
+        C->host_scalars[g] = h;
+    }
+    SHKZ_HIP(hipMemcpyAsync(dS, C->host_scalars, (size_t)n * sizeof(SolveScalars), hipMemcpyHostToDevice, s));
+    SHKZ_HIP(hipStreamSynchronize(s));
+    return SHKZ_OK;
+}
+
+static void finalize(Launcher &K, const Problem &P, const LevelDesc &L0, SolveScalars *S, BatchCtl *ctl, int op,
+                     double param = 0.0, int iparam = 0, const SolveScalars *other = nullptr) {
+    K.go("finalize", -1, 0, 0, dim3((unsigned)P.ngrids), dim3(1024), k_finalize, (const double *)P.red.partials, L0, S,
+         ctl, op, param, iparam, other);
 }
 
 template <class T>
 static std::string graph_key(shkz_ctx *C, const Problem &P, const void *const *vecs, int nvec, bool precond) {
     std::string k;
-    char buf[160];
-    std::snprintf(buf, sizeof(buf), "T%zu pc%d pre%d post%d prc%d poc%d lb%d cs%d om%.17g red%p|", sizeof(T),
-                  precond ? 1 : 0, P.pre, P.post, P.pre_c, P.post_c, P.lb, P.coarse_sweeps, P.omega,
+    char buf[256];
+    std::snprintf(buf, sizeof(buf), "T%zu pc%d g%d nl%d pre%d post%d prc%d poc%d cs%d om%.17g red%p|", sizeof(T),
+                  precond ? 1 : 0, P.ngrids, P.nlev, P.pre, P.post, P.pre_c, P.post_c, P.coarse_sweeps, P.omega,
                   (void *)P.red.partials);
     k += buf;
     for (int i = 0; i < nvec; ++i) {
@@ -452,58 +531,66 @@ static std::string graph_key(shkz_ctx *C, const Problem &P, const void *const *v
     const int nl = precond ? P.nlev : 1;
     for (int l = 0; l < nl; ++l) {
         const LevelHost &L = C->levels[l];
-        std::snprintf(buf, sizeof(buf), "|%d %d %d %d %p %p %p %p %p %p %p %p %p %p %p", L.ni, L.nj, L.nk, L.nbricks,
-                      (const void *)L.desc.bmap, L.dir, L.gx, L.gy, L.gz, L.rhs, L.z, L.tmp, L.res,
-                      (const void *)L.desc.bxyz, (const void *)L.desc.bnbr);
+        std::snprintf(buf, sizeof(buf), "|%d %d %p %p %p %p %p %p %p %p %p %p %p %p %p %p", L.nbricks,
+                      L.any_coarsest ? 1 : 0, (const void *)L.desc.bmap, L.dir, L.gx, L.gy, L.gz, L.idiag, L.rhs, L.z,
+                      L.tmp, L.res, (const void *)L.desc.bxyz, (const void *)L.desc.bnbr, (const void *)L.desc.bgrid,
+                      (const void *)L.desc.gl);
         k += buf;
     }
     return k;
 }
 
-// Krylov loop on level 0: T = element type; precond = V-cycle or identity (plain CG).
-// Expects r (residual), and S with tol/maxit/iter set.  x is updated in place.  One iteration is
-// captured once into a hipGraph and replayed (kernels return at once after convergence, so a
-// chunk of replays may overshoot by a few no-op iterations).
+// Krylov loop on level 0 for every grid of the batch: T = element type; precond = V-cycle or
+// identity (plain CG).  Expects r (residual) and per-grid S with tol/maxit/iter set and the control
+// block counting the grids already done.  x is updated in place.  One iteration is captured once
+// into a hipGraph and replayed; every kernel returns at once when all grids are done and skips the
+// bricks of a grid that is done, so a chunk of replays may overshoot by a few no-op iterations
+// without changing any result.  `last` receives the final per-grid scalars.
 template <class T>
 static int krylov(shkz_ctx *C, Launcher &K, Problem &P, T *x, T *r, T *p, T *q, T *z, T *tmp, SolveScalars *S,
-                  bool precond, SolveScalars *hs) {
+                  BatchCtl *ctl, bool precond, std::vector<SolveScalars> &last) {
     LevelHost &L0 = C->levels[0];
     const unsigned nb = brick_grid(L0.nbricks);
     const dim3 g(nb), b(BLOCK);
     const OpDesc<T> A = op_of<T>(L0);
     const double es = (double)sizeof(T);
+    const Gate Q{ctl, S};
     // the finest level's last post-smoothing sweep forms r.z when there is one
-    const bool fused_dot = precond && P.lb > 0 && P.post > 0;
+    const bool fused_dot = precond && P.nlev > 1 && P.post > 0;
     // first direction
     const T *zp = r;
-    if (precond) zp = vcycle<T>(C, K, P, r, z, tmp, S, fused_dot);
+    if (precond) zp = vcycle<T>(C, K, P, r, z, tmp, Q, fused_dot);
     if (!fused_dot)
-        K.go("dot_rz", 0, 2 * es, 0, g, b, k_dot_rz<T>, L0.desc, (const T *)r, zp, (const SolveScalars *)S, P.red);
-    finalize(K, P, nb, S, FIN_RZ_FIRST);
-    K.go("update_p", 0, 2 * es, 0, g, b, k_update_p<T>, L0.desc, zp, p, (const SolveScalars *)S, 1);
+        K.go("dot_rz", 0, 2 * es, 0, g, b, k_dot_rz<T>, L0.desc, (const T *)r, zp, (const SolveScalars *)S,
+             (const BatchCtl *)ctl, P.red);
+    finalize(K, P, L0.desc, S, ctl, FIN_RZ_FIRST);
+    K.go("update_p", 0, 2 * es, 0, g, b, k_update_p<T>, L0.desc, zp, p, (const SolveScalars *)S, (const BatchCtl *)ctl,
+         1);
 
     auto iteration = [&]() {
         K.go("apply_dot", 0, 6 * es, 0, g, b, k_apply_dot<T>, L0.desc, A, (const T *)p, q, (const SolveScalars *)S,
-             P.red);
-        finalize(K, P, nb, S, FIN_PQ);
+             (const BatchCtl *)ctl, P.red);
+        finalize(K, P, L0.desc, S, ctl, FIN_PQ);
         K.go("cg_update", 0, 6 * es, 0, g, b, k_cg_update<T>, L0.desc, x, r, (const T *)p, (const T *)q,
-             (const SolveScalars *)S, P.red);
-        finalize(K, P, nb, S, precond ? FIN_UPDATE : FIN_UPDATE_CG);
+             (const SolveScalars *)S, (const BatchCtl *)ctl, P.red);
+        finalize(K, P, L0.desc, S, ctl, precond ? FIN_UPDATE : FIN_UPDATE_CG);
         if (precond) {
-            const T *zz = vcycle<T>(C, K, P, r, z, tmp, S, fused_dot);
+            const T *zz = vcycle<T>(C, K, P, r, z, tmp, Q, fused_dot);
             if (!fused_dot)
                 K.go("dot_rz", 0, 2 * es, 0, g, b, k_dot_rz<T>, L0.desc, (const T *)r, zz, (const SolveScalars *)S,
-                     P.red);
-            finalize(K, P, nb, S, FIN_RZ);
-            K.go("update_p", 0, 3 * es, 0, g, b, k_update_p<T>, L0.desc, zz, p, (const SolveScalars *)S, 0);
+                     (const BatchCtl *)ctl, P.red);
+            finalize(K, P, L0.desc, S, ctl, FIN_RZ);
+            K.go("update_p", 0, 3 * es, 0, g, b, k_update_p<T>, L0.desc, zz, p, (const SolveScalars *)S,
+                 (const BatchCtl *)ctl, 0);
         } else {
-            K.go("update_p", 0, 3 * es, 0, g, b, k_update_p<T>, L0.desc, (const T *)r, p, (const SolveScalars *)S, 0);
+            K.go("update_p", 0, 3 * es, 0, g, b, k_update_p<T>, L0.desc, (const T *)r, p, (const SolveScalars *)S,
+                 (const BatchCtl *)ctl, 0);
         }
     };
     hipGraphExec_t exec = nullptr;
     if (C->use_graphs && !C->timing) {
-        const void *vecs[] = {x, r, p, q, z, tmp, S};
-        const std::string key = graph_key<T>(C, P, vecs, 7, precond);
+        const void *vecs[] = {x, r, p, q, z, tmp, S, ctl};
+        const std::string key = graph_key<T>(C, P, vecs, 8, precond);
         auto &slot = C->graphs[(sizeof(T) == 8 ? 2 : 0) + (precond ? 1 : 0)];
         if (slot.exec && slot.key == key) {
             exec = slot.exec;
@@ -530,72 +617,94 @@ static int krylov(shkz_ctx *C, Launcher &K, Problem &P, T *x, T *r, T *p, T *q, 
         }
     }
     int chunk = 4;
+    BatchCtl hc{};
     for (;;) {
         for (int it = 0; it < chunk; ++it) {
             if (exec) SHKZ_HIP(hipGraphLaunch(exec, K.s));
             else iteration();
         }
-        SHKZ_TRY(read_scalars(C, K.s, S, hs));
-        if (hs->done) break;
-        // size the next chunk from the observed contraction rate
-        const double cur = hs->norm == 0 ? hs->rinf : hs->rl2;
-        const double r0 = hs->norm == 0 ? hs->r0inf : hs->r0l2;
-        int next = chunk;
-        if (hs->iter > 0 && cur > 0 && r0 > 0 && cur < r0) {
-            const double rate = std::pow(cur / r0, 1.0 / hs->iter);
-            if (rate > 0 && rate < 1) next = (int)std::ceil(std::log(hs->tol / cur) / std::log(rate));
+        SHKZ_TRY(read_scalars(C, K.s, S, P.ngrids, last, ctl, &hc));
+        if (hc.all_done) break;
+        // size the next chunk from the slowest grid's observed contraction rate
+        int next = 1;
+        for (const SolveScalars &s : last) {
+            if (s.done) continue;
+            const double cur = s.norm == 0 ? s.rinf : s.rl2;
+            const double r0 = s.norm == 0 ? s.r0inf : s.r0l2;
+            int need = chunk;
+            if (s.iter > 0 && cur > 0 && r0 > 0 && cur < r0) {
+                const double rate = std::pow(cur / r0, 1.0 / s.iter);
+                if (rate > 0 && rate < 1) need = (int)std::ceil(std::log(s.tol / cur) / std::log(rate));
+            }
+            need = std::min(need, s.maxit - s.iter);
+            next = std::max(next, need);
         }
         chunk = std::max(1, std::min(next, 64));
-        if (hs->iter + chunk > hs->maxit) chunk = std::max(1, hs->maxit - hs->iter);
     }
     return SHKZ_OK;
 }
 
-static int upload_scalars(shkz_ctx *C, hipStream_t s, SolveScalars *dS, int norm, int maxit) {
-    SolveScalars h{};
-    h.norm = norm;
-    h.maxit = maxit;
-    *C->host_scalars = h;
-    SHKZ_HIP(hipMemcpyAsync(dS, C->host_scalars, sizeof(SolveScalars), hipMemcpyHostToDevice, s));
-    SHKZ_HIP(hipStreamSynchronize(s));
-    return SHKZ_OK;
-}
-
-// ------------------------------------------------------------------------ the projection
+// ------------------------------------------------------------------------ the projection (batch)
 struct ExportOut {
-    double *diag, *gx, *gy, *gz, *rhs;  // device, cell-indexed
+    double *diag, *gx, *gy, *gz, *rhs;  // device, cell-indexed (grid 0)
 };
 
-static int project_stages(shkz_ctx *C, const shkz_grid *grid, const shkz_params *prm, shkz_result *res,
-                          hipStream_t s, const ExportOut *exp);
+struct GridJob {
+    const shkz_grid *in = nullptr;
+    GridDesc G{};
+    size_t M = 0;
+    int *exist = nullptr;
+    uint8_t *cflag = nullptr, *bflag = nullptr;
+    GridLevelHost l0;
+    SceneState *scene = nullptr;
+    double *scene_p = nullptr;
+    double rhs_correct = 0.0;
+};
 
-// Runs the projection on the context's work stream and orders the caller's stream after it, so
-// that work the caller queues next (and synchronous copies on the null stream) sees the outputs.
-static int project_impl(shkz_ctx *C, const shkz_grid *grid, const shkz_params *prm, shkz_result *res,
-                        hipStream_t caller, const ExportOut *exp = nullptr) {
-    const int rc = project_stages(C, grid, prm, res, caller, exp);
-    if (C->work) {
-        if (hipEventRecord(C->ev_out, C->work) != hipSuccess || hipStreamWaitEvent(caller, C->ev_out, 0) != hipSuccess) {
-            (void)hipStreamSynchronize(C->work);
-            return rc != SHKZ_OK ? rc : fail(SHKZ_ERR_DEVICE, "ordering the caller's stream after the projection failed");
-        }
-        if (rc != SHKZ_OK) (void)hipStreamSynchronize(C->work);
-    }
-    return rc;
-}
-
-static int project_stages(shkz_ctx *C, const shkz_grid *grid, const shkz_params *prm, shkz_result *res,
-                          hipStream_t s, const ExportOut *exp) {
-    const double t_start = now_ms();
-    std::memset(res, 0, sizeof(*res));
+static int validate_grid(const shkz_grid *grid) {
     const int w = grid->w, h = grid->h, d = grid->d;
     if (w <= 0 || h <= 0 || d <= 0) return fail(SHKZ_ERR_INVALID, "grid dimensions must be positive (%d,%d,%d)", w, h, d);
     if (!grid->fluid || !grid->u || !grid->v || !grid->w_vel) return fail(SHKZ_ERR_INVALID, "fluid and velocities are required");
     if (!(grid->dt > 0)) return fail(SHKZ_ERR_INVALID, "dt must be positive");
+    if ((int64_t)w * h * d > (int64_t)1 << 31) return fail(SHKZ_ERR_INVALID, "grid too large for 32-bit brick ids");
+    return SHKZ_OK;
+}
+
+static int project_stages(shkz_ctx *C, const shkz_grid *grids, int ngrids, const shkz_params *prm, shkz_result *res,
+                          hipStream_t s, const ExportOut *exp);
+
+// Runs the projection on the context's work stream and orders the caller's stream after it, so
+// that work the caller queues next (and synchronous copies on the null stream) sees the outputs.
+static int project_impl(shkz_ctx *C, const shkz_grid *grids, int ngrids, const shkz_params *prm, shkz_result *res,
+                        hipStream_t caller, const ExportOut *exp = nullptr) {
+    int rc = project_stages(C, grids, ngrids, prm, res, caller, exp);
+    if (C->work) {
+        if (hipEventRecord(C->ev_out, C->work) != hipSuccess || hipStreamWaitEvent(caller, C->ev_out, 0) != hipSuccess) {
+            (void)hipStreamSynchronize(C->work);
+            if (rc == SHKZ_OK) rc = fail(SHKZ_ERR_DEVICE, "ordering the caller's stream after the projection failed");
+        } else if (rc != SHKZ_OK) {
+            (void)hipStreamSynchronize(C->work);
+        }
+    }
+    if (rc != SHKZ_OK)
+        for (int g = 0; g < ngrids; ++g)
+            if (res[g].status == SHKZ_OK && rc != SHKZ_ERR_NUMERIC) res[g].status = rc;
+    return rc;
+}
+
+static int project_stages(shkz_ctx *C, const shkz_grid *grids, int ngrids, const shkz_params *prm, shkz_result *res,
+                          hipStream_t s, const ExportOut *exp) {
+    const double t_start = now_ms();
+    for (int g = 0; g < ngrids; ++g) std::memset(&res[g], 0, sizeof(shkz_result));
+    if (ngrids <= 0) return fail(SHKZ_ERR_INVALID, "empty batch");
+    for (int g = 0; g < ngrids; ++g) SHKZ_TRY(validate_grid(&grids[g]));
     if (prm->solver != SHKZ_SOLVER_MGPCG && prm->solver != SHKZ_SOLVER_CG) return fail(SHKZ_ERR_INVALID, "unknown solver %d", prm->solver);
     if (prm->precision != SHKZ_PREC_MIXED && prm->precision != SHKZ_PREC_FP64)
         return fail(SHKZ_ERR_INVALID, "unknown precision %d", prm->precision);
-    if ((int64_t)w * h * d > (int64_t)1 << 31) return fail(SHKZ_ERR_INVALID, "grid too large for 32-bit brick ids");
+    for (int a = 0; a < ngrids; ++a)
+        for (int b = a + 1; b < ngrids; ++b)
+            if (grids[a].scene_id && grids[a].scene_id == grids[b].scene_id)
+                return fail(SHKZ_ERR_INVALID, "scene %llu appears twice in one batch", (unsigned long long)grids[a].scene_id);
     SHKZ_HIP(hipSetDevice(C->device));
     // all work runs on the context's stream, ordered after what the caller queued on its stream
     SHKZ_HIP(hipEventRecord(C->ev_in, s));
@@ -603,10 +712,9 @@ static int project_stages(shkz_ctx *C, const shkz_grid *grid, const shkz_params 
     s = C->work;
     Launcher K{C, s};
     Arena &AR = C->arena;
-    const size_t M = (size_t)w * h * d;
-    const double dx = grid->dx > 0 ? grid->dx : std::min(1.0 / w, std::min(1.0 / h, 1.0 / d));
 
     Problem P;
+    P.ngrids = ngrids;
     P.mixed = (prm->solver == SHKZ_SOLVER_MGPCG && prm->precision == SHKZ_PREC_MIXED);
     P.fp64_levels = !P.mixed;
     P.pre = prm->mg_pre_sweeps > 0 ? prm->mg_pre_sweeps : 4;
@@ -615,144 +723,155 @@ static int project_stages(shkz_ctx *C, const shkz_grid *grid, const shkz_params 
     P.post_c = prm->mg_coarse_post_sweeps > 0 ? prm->mg_coarse_post_sweeps : P.post;
     P.coarse_sweeps = prm->mg_coarse_sweeps > 0 ? prm->mg_coarse_sweeps : 48;
     P.omega = prm->mg_omega > 0 ? prm->mg_omega : 0.9;
-    const int bottom_bricks = prm->mg_bottom_bricks > 0 ? prm->mg_bottom_bricks : 1;
+    const bool multigrid = prm->solver == SHKZ_SOLVER_MGPCG;
 
-    GridDesc &G = P.G;
-    G.w = w;
-    G.h = h;
-    G.d = d;
-    G.phi64 = grid->fluid_dtype == SHKZ_F64;
-    G.solid64 = grid->solid_dtype == SHKZ_F64;
-    G.dx = dx;
-    G.dt = grid->dt;
-    G.phi = grid->fluid;
-    G.solid = grid->solid;
-    G.eps_fluid = prm->eps_fluid;
-    G.eps_solid = prm->eps_solid;
-    G.so_fluid = prm->second_order_fluid;
-    G.so_solid = prm->second_order_solid;
-    G.vel[0] = grid->u;
-    G.vel[1] = grid->v;
-    G.vel[2] = grid->w_vel;
-    G.active[0] = grid->active_u;
-    G.active[1] = grid->active_v;
-    G.active[2] = grid->active_w;
-
-    int *exist = nullptr;
-    SHKZ_TRY(arena_get(AR, "exist", 4, &exist));
-    G.exist = exist;
-
-    // ---------------------------------------------------------------- fractions + labelling
-    K.go("reset", -1, 0, 8, dim3(1), dim3(64), k_reset_ints, exist, 4);
-    {
-        const unsigned nbk = (unsigned)std::min<size_t>((M + 255) / 256, 4096);
-        K.go("any_negative", -1, 0, (double)M * (G.phi64 ? 8 : 4), dim3(nbk), dim3(256), k_any_negative, grid->fluid,
-             G.phi64, M, exist + 0);
+    // ---------------------------------------------------------------- per grid: fractions + labelling
+    std::vector<GridJob> jobs(ngrids);
+    for (int g = 0; g < ngrids; ++g) {
+        GridJob &J = jobs[g];
+        const shkz_grid *grid = &grids[g];
+        J.in = grid;
+        const int w = grid->w, h = grid->h, d = grid->d;
+        J.M = (size_t)w * h * d;
+        GridDesc &G = J.G;
+        G.w = w;
+        G.h = h;
+        G.d = d;
+        G.phi64 = grid->fluid_dtype == SHKZ_F64;
+        G.solid64 = grid->solid_dtype == SHKZ_F64;
+        G.dx = grid->dx > 0 ? grid->dx : std::min(1.0 / w, std::min(1.0 / h, 1.0 / d));
+        G.dt = grid->dt;
+        G.phi = grid->fluid;
+        G.solid = grid->solid;
+        G.eps_fluid = prm->eps_fluid;
+        G.eps_solid = prm->eps_solid;
+        G.so_fluid = prm->second_order_fluid;
+        G.so_solid = prm->second_order_solid;
+        G.vel[0] = grid->u;
+        G.vel[1] = grid->v;
+        G.vel[2] = grid->w_vel;
+        G.active[0] = grid->active_u;
+        G.active[1] = grid->active_v;
+        G.active[2] = grid->active_w;
+        const std::string sfx = std::to_string(g);
+        SHKZ_TRY(arena_get(AR, "exist" + sfx, 4, &J.exist));
+        G.exist = J.exist;
+        K.go("reset", -1, 0, 8, dim3(1), dim3(64), k_reset_ints, J.exist, 4);
+        const unsigned nbk = (unsigned)std::min<size_t>((J.M + 255) / 256, 4096);
+        K.go("any_negative", -1, 0, (double)J.M * (G.phi64 ? 8 : 4), dim3(nbk), dim3(256), k_any_negative, grid->fluid,
+             (int)G.phi64, J.M, J.exist + 0);
         if (grid->solid) {
             const size_t Nn = (size_t)(w + 1) * (h + 1) * (d + 1);
             const unsigned nbs = (unsigned)std::min<size_t>((Nn + 255) / 256, 4096);
             K.go("any_negative", -1, 0, (double)Nn * (G.solid64 ? 8 : 4), dim3(nbs), dim3(256), k_any_negative,
-                 grid->solid, G.solid64, Nn, exist + 1);
+                 grid->solid, (int)G.solid64, Nn, J.exist + 1);
         }
-    }
-    if (prm->surface_tension != 0.0) {
-        float *curv = nullptr;
-        SHKZ_TRY(arena_get(AR, "curvature", M, &curv));
-        K.go("curvature", -1, 0, (double)M * ((G.phi64 ? 8 : 4) + 4), dim3((unsigned)((M + 255) / 256)), dim3(256),
-             k_curvature, G, curv);
-        for (int dim = 0; dim < 3; ++dim) {
-            const size_t nf = (size_t)(w + (dim == 0)) * (h + (dim == 1)) * (d + (dim == 2));
-            K.go("surface_tension", -1, 0, (double)nf * 8, dim3((unsigned)((nf + 255) / 256)), dim3(256),
-                 k_surface_tension, G, dim, (const float *)curv, prm->surface_tension);
+        if (prm->surface_tension != 0.0) {
+            float *curv = nullptr;
+            SHKZ_TRY(arena_get(AR, "curvature", J.M, &curv));
+            K.go("curvature", -1, 0, (double)J.M * ((G.phi64 ? 8 : 4) + 4), dim3((unsigned)((J.M + 255) / 256)),
+                 dim3(256), k_curvature, G, curv);
+            for (int dim = 0; dim < 3; ++dim) {
+                const size_t nf = (size_t)(w + (dim == 0)) * (h + (dim == 1)) * (d + (dim == 2));
+                K.go("surface_tension", -1, 0, (double)nf * 8, dim3((unsigned)((nf + 255) / 256)), dim3(256),
+                     k_surface_tension, G, dim, (const float *)curv, prm->surface_tension);
+            }
         }
+        grid_level_dims(J.l0, w, h, d);
+        const int npos = J.l0.nbx * J.l0.nby * J.l0.nbz;
+        SHKZ_TRY(arena_get(AR, "cflag" + sfx, J.M, &J.cflag));
+        SHKZ_TRY(arena_get(AR, "bflag" + sfx, (size_t)npos, &J.bflag));
+        K.go("flags", -1, 0, (double)J.M * ((G.phi64 ? 8 : 4) + 1 + (grid->solid ? 4 : 0)), dim3((unsigned)npos),
+             dim3(BLOCK), k_flags, G, J.l0.nbx, J.l0.nby, J.l0.nbz, J.cflag, J.bflag);
+        J.l0.active.resize(npos);
+        SHKZ_HIP(hipMemcpyAsync(J.l0.active.data(), J.bflag, npos, hipMemcpyDeviceToHost, s));
     }
-    LevelHost L0;
-    level_dims(L0, w, h, d);
-    const int nb0 = L0.nbx * L0.nby * L0.nbz;
-    uint8_t *cflag = nullptr, *bflag = nullptr;
-    SHKZ_TRY(arena_get(AR, "cflag", M, &cflag));
-    SHKZ_TRY(arena_get(AR, "bflag", (size_t)nb0, &bflag));
-    K.go("flags", -1, 0, (double)M * ((G.phi64 ? 8 : 4) + 1 + (grid->solid ? 4 : 0)), dim3((unsigned)nb0), dim3(BLOCK),
-         k_flags, G, L0.nbx, L0.nby, L0.nbz, cflag, bflag);
-    std::vector<uint8_t> hflag(nb0);
-    SHKZ_HIP(hipMemcpyAsync(hflag.data(), bflag, nb0, hipMemcpyDeviceToHost, s));
     SHKZ_HIP(hipStreamSynchronize(s));
     SHKZ_HIP(hipGetLastError());
     const double t_frac = now_ms();
 
     // ---------------------------------------------------------------- hierarchy (host)
-    C->levels.clear();
-    finish_level(L0, hflag);
-    C->levels.push_back(std::move(L0));
-    const bool multigrid = prm->solver == SHKZ_SOLVER_MGPCG;
+    // each grid is coarsened (cells halved, a coarse brick active when any child brick is) until
+    // it fits one brick; the batch has as many levels as its deepest grid
+    std::vector<std::vector<GridLevelHost>> glevels(1);
+    for (int g = 0; g < ngrids; ++g) glevels[0].push_back(jobs[g].l0);
     if (multigrid) {
-        while ((int)C->levels.size() < MAX_LEVELS) {
-            const LevelHost &F = C->levels.back();
-            if (F.nbx <= 1 && F.nby <= 1 && F.nbz <= 1) break;
-            LevelHost Lc;
-            level_dims(Lc, (F.ni + 1) / 2, (F.nj + 1) / 2, (F.nk + 1) / 2);
-            std::vector<uint8_t> act((size_t)Lc.nbx * Lc.nby * Lc.nbz, 0);
-            for (int id = 0; id < F.nbricks; ++id) {
-                const int bx = F.bxyz[3 * id] / 2, by = F.bxyz[3 * id + 1] / 2, bz = F.bxyz[3 * id + 2] / 2;
-                act[((size_t)bx * Lc.nby + by) * Lc.nbz + bz] = 1;
+        for (;;) {
+            const std::vector<GridLevelHost> &F = glevels.back();
+            bool any = false;
+            for (const GridLevelHost &H : F) any |= H.coarsens();
+            if (!any) break;
+            if ((int)glevels.size() >= MAX_LEVELS) return fail(SHKZ_ERR_INVALID, "more than %d multigrid levels", MAX_LEVELS);
+            std::vector<GridLevelHost> next(ngrids);
+            for (int g = 0; g < ngrids; ++g) {
+                const GridLevelHost &H = F[g];
+                GridLevelHost &N = next[g];
+                if (!H.coarsens()) continue;  // ended on this or a finer level
+                grid_level_dims(N, (H.ni + 1) / 2, (H.nj + 1) / 2, (H.nk + 1) / 2);
+                N.active.assign((size_t)N.nbx * N.nby * N.nbz, 0);
+                for (int b = 0; b < H.nbx * H.nby * H.nbz; ++b)
+                    if (H.active[b]) {
+                        const int bz = b % H.nbz, by = (b / H.nbz) % H.nby, bx = b / (H.nbz * H.nby);
+                        N.active[((size_t)(bx / 2) * N.nby + by / 2) * N.nbz + bz / 2] = 1;
+                    }
             }
-            finish_level(Lc, act);
-            C->levels.push_back(std::move(Lc));
+            glevels.push_back(std::move(next));
         }
     }
-    P.nlev = (int)C->levels.size();
-    P.lb = P.nlev - 1;
-    for (int l = 1; l < P.nlev; ++l)
-        if (C->levels[l].nbricks <= bottom_bricks) {
-            P.lb = l;
-            break;
-        }
-    if (P.nlev - P.lb > BOTTOM_MAX) P.lb = P.nlev - BOTTOM_MAX;
-    if (C->trace) {
-        std::fprintf(stderr, "[shkz] levels:");
-        for (int l = 0; l < P.nlev; ++l)
-            std::fprintf(stderr, " %dx%dx%d/%d%s", C->levels[l].ni, C->levels[l].nj, C->levels[l].nk,
-                         C->levels[l].nbricks, l == P.lb ? "(bottom)" : "");
-        std::fprintf(stderr, "\n");
-    }
-    if (P.nlev > 1 && C->levels.back().nbricks > 1) return fail(SHKZ_ERR_INVALID, "coarsest level holds more than one brick");
-
-    // upload the brick tables (one buffer per level)
-    int max_blocks = 8;
+    P.nlev = (int)glevels.size();
+    C->levels.assign(P.nlev, LevelHost{});
+    int max_bricks = 8;
     for (int l = 0; l < P.nlev; ++l) {
         LevelHost &L = C->levels[l];
-        const size_t nmap = L.bmap.size(), nxyz = L.bxyz.size(), nnbr = L.bnbr.size();
+        build_level(L, glevels[l]);
+        if (!multigrid) {
+            // plain CG: no coarse solve anywhere
+            L.any_coarsest = false;
+            for (GridLevel &gl : L.gl) gl.coarsest = 0;
+        }
+        max_bricks = std::max(max_bricks, (int)brick_grid(L.nbricks));
+        // upload the tables: bmap | bxyz | bnbr | bgrid | gl (16-byte aligned)
+        const size_t nmap = L.bmap.size(), nxyz = L.bxyz.size(), nnbr = L.bnbr.size(), ngr = L.bgrid.size();
+        const size_t ngl = L.gl.size() * (sizeof(GridLevel) / sizeof(int));
+        const size_t o_xyz = nmap, o_nbr = o_xyz + nxyz, o_gr = o_nbr + nnbr;
+        const size_t o_gl = (o_gr + ngr + 3) & ~(size_t)3;
+        const size_t total = o_gl + ngl;
         int *meta = nullptr;
-        SHKZ_TRY(arena_get(AR, "meta" + std::to_string(l), nmap + nxyz + nnbr + 4, &meta));
-        std::vector<int> host(nmap + nxyz + nnbr + 4, -1);
+        SHKZ_TRY(arena_get(AR, "meta" + std::to_string(l), total, &meta));
+        std::vector<int> host(total, -1);
         std::copy(L.bmap.begin(), L.bmap.end(), host.begin());
-        std::copy(L.bxyz.begin(), L.bxyz.end(), host.begin() + nmap);
-        std::copy(L.bnbr.begin(), L.bnbr.end(), host.begin() + nmap + nxyz);
+        std::copy(L.bxyz.begin(), L.bxyz.end(), host.begin() + o_xyz);
+        std::copy(L.bnbr.begin(), L.bnbr.end(), host.begin() + o_nbr);
+        std::copy(L.bgrid.begin(), L.bgrid.end(), host.begin() + o_gr);
+        std::memcpy(host.data() + o_gl, L.gl.data(), L.gl.size() * sizeof(GridLevel));
+        // synchronous: the host vector goes out of scope
         SHKZ_HIP(hipMemcpyAsync(meta, host.data(), host.size() * sizeof(int), hipMemcpyHostToDevice, s));
-        L.desc.ni = L.ni;
-        L.desc.nj = L.nj;
-        L.desc.nk = L.nk;
-        L.desc.nbx = L.nbx;
-        L.desc.nby = L.nby;
-        L.desc.nbz = L.nbz;
+        SHKZ_HIP(hipStreamSynchronize(s));
         L.desc.nbricks = L.nbricks;
+        L.desc.ngrids = ngrids;
         L.desc.bmap = meta;
-        L.desc.bxyz = meta + nmap;
-        L.desc.bnbr = meta + nmap + nxyz;
-        max_blocks = std::max(max_blocks, (int)brick_grid(L.nbricks));
+        L.desc.bxyz = meta + o_xyz;
+        L.desc.bnbr = meta + o_nbr;
+        L.desc.bgrid = meta + o_gr;
+        L.desc.gl = reinterpret_cast<const GridLevel *>(meta + o_gl);
     }
-    // reduction scratch and scalars
-    SHKZ_TRY(arena_get(AR, "partials", (size_t)2 * max_blocks, &P.red.partials));
+    if (C->trace) {
+        std::fprintf(stderr, "[shkz] %d grid(s), levels:", ngrids);
+        for (int l = 0; l < P.nlev; ++l) std::fprintf(stderr, " %d%s", C->levels[l].nbricks, C->levels[l].any_coarsest ? "*" : "");
+        std::fprintf(stderr, " bricks\n");
+    }
+    // reduction scratch, scalars, control blocks
+    SHKZ_TRY(arena_get(AR, "partials", (size_t)2 * max_bricks, &P.red.partials));
     SolveScalars *Sbuf = nullptr;
-    SHKZ_TRY(arena_get(AR, "scalars", 2, &Sbuf));
+    SHKZ_TRY(arena_get(AR, "scalars", (size_t)3 * ngrids, &Sbuf));
     P.S_out = Sbuf;
-    P.S_in = Sbuf + 1;
+    P.S_in = Sbuf + ngrids;
+    P.S_fin = Sbuf + 2 * ngrids;
+    SHKZ_TRY(arena_get(AR, "ctl", 2, &P.ctl));
 
     // ---------------------------------------------------------------- operator + vectors
     LevelHost &F0 = C->levels[0];
-    const size_t n0 = (size_t)F0.nbricks * BRICK_CELLS;
-    res->n_bricks = F0.nbricks;
-    res->n_levels = P.nlev;
+    const size_t n0 = (size_t)std::max(F0.nbricks, 1) * BRICK_CELLS;
     SHKZ_TRY(arena_get(AR, "dir64", n0, &P.dir64));
     SHKZ_TRY(arena_get(AR, "gx64", n0, &P.gx64));
     SHKZ_TRY(arena_get(AR, "gy64", n0, &P.gy64));
@@ -766,9 +885,6 @@ static int project_stages(shkz_ctx *C, const shkz_grid *grid, const shkz_params 
         SHKZ_TRY(arena_get(AR, "gx32", n0, &gx32));
         SHKZ_TRY(arena_get(AR, "gy32", n0, &gy32));
         SHKZ_TRY(arena_get(AR, "gz32", n0, &gz32));
-    }
-    // level-0 operator as seen by the V-cycle / Krylov loop
-    if (P.mixed) {
         F0.dir = dir32;
         F0.gx = gx32;
         F0.gy = gy32;
@@ -782,18 +898,27 @@ static int project_stages(shkz_ctx *C, const shkz_grid *grid, const shkz_params 
     const size_t es = P.fp64_levels ? 8 : 4;
     for (int l = 1; l < P.nlev; ++l) {
         LevelHost &L = C->levels[l];
-        const size_t n = (size_t)L.nbricks * BRICK_CELLS;
+        const size_t n = (size_t)std::max(L.nbricks, 1) * BRICK_CELLS;
         const std::string sfx = std::to_string(l);
         void *pp = nullptr;
-        SHKZ_TRY(AR.get("dir_l" + sfx, n * es, &pp)); L.dir = pp;
-        SHKZ_TRY(AR.get("gx_l" + sfx, n * es, &pp)); L.gx = pp;
-        SHKZ_TRY(AR.get("gy_l" + sfx, n * es, &pp)); L.gy = pp;
-        SHKZ_TRY(AR.get("gz_l" + sfx, n * es, &pp)); L.gz = pp;
-        SHKZ_TRY(AR.get("idiag_l" + sfx, n * es, &pp)); L.idiag = pp;
-        SHKZ_TRY(AR.get("rhs_l" + sfx, n * es, &pp)); L.rhs = pp;
-        SHKZ_TRY(AR.get("z_l" + sfx, n * es, &pp)); L.z = pp;
-        SHKZ_TRY(AR.get("tmp_l" + sfx, n * es, &pp)); L.tmp = pp;
-        SHKZ_TRY(AR.get("res_l" + sfx, n * es, &pp)); L.res = pp;
+        SHKZ_TRY(AR.get("dir_l" + sfx, n * es, &pp));
+        L.dir = pp;
+        SHKZ_TRY(AR.get("gx_l" + sfx, n * es, &pp));
+        L.gx = pp;
+        SHKZ_TRY(AR.get("gy_l" + sfx, n * es, &pp));
+        L.gy = pp;
+        SHKZ_TRY(AR.get("gz_l" + sfx, n * es, &pp));
+        L.gz = pp;
+        SHKZ_TRY(AR.get("idiag_l" + sfx, n * es, &pp));
+        L.idiag = pp;
+        SHKZ_TRY(AR.get("rhs_l" + sfx, n * es, &pp));
+        L.rhs = pp;
+        SHKZ_TRY(AR.get("z_l" + sfx, n * es, &pp));
+        L.z = pp;
+        SHKZ_TRY(AR.get("tmp_l" + sfx, n * es, &pp));
+        L.tmp = pp;
+        SHKZ_TRY(AR.get("res_l" + sfx, n * es, &pp));
+        L.res = pp;
     }
     {
         void *pp = nullptr;
@@ -802,8 +927,7 @@ static int project_stages(shkz_ctx *C, const shkz_grid *grid, const shkz_params 
         SHKZ_TRY(AR.get("idiag_l0", n0 * es, &pp));
         F0.idiag = pp;
     }
-    // Krylov vectors on level 0
-    const size_t ks = (prm->solver == SHKZ_SOLVER_MGPCG && P.mixed) ? 4 : 8;
+    const size_t ks = P.mixed ? 4 : 8;
     SHKZ_TRY(AR.get("kp", n0 * ks, &P.kp));
     SHKZ_TRY(AR.get("kq", n0 * ks, &P.kq));
     SHKZ_TRY(AR.get("kz", n0 * ks, &P.kz));
@@ -813,50 +937,52 @@ static int project_stages(shkz_ctx *C, const shkz_grid *grid, const shkz_params 
         SHKZ_TRY(AR.get("kx32", n0 * 4, &P.kx));
     }
 
-    // volume correction (macpressuresolver3.cpp:203-217) and warm start state
-    SceneState *scene = nullptr;
-    if (grid->scene_id) {
-        scene = &C->scenes[grid->scene_id];
-        if (scene->w != w || scene->h != h || scene->d != d) {
-            scene->w = w;
-            scene->h = h;
-            scene->d = d;
-            scene->has_pressure = false;
+    // per grid: volume correction (macpressuresolver3.cpp:203-217), warm start, assembly
+    for (int g = 0; g < ngrids; ++g) {
+        GridJob &J = jobs[g];
+        const shkz_grid *grid = J.in;
+        if (grid->scene_id) {
+            J.scene = &C->scenes[grid->scene_id];
+            if (J.scene->w != grid->w || J.scene->h != grid->h || J.scene->d != grid->d) {
+                J.scene->w = grid->w;
+                J.scene->h = grid->h;
+                J.scene->d = grid->d;
+                J.scene->has_pressure = false;
+            }
+            SHKZ_TRY(arena_get(AR, "scene" + std::to_string(grid->scene_id), J.M, &J.scene_p));
         }
+        if (prm->gain && prm->target_volume) {
+            double &yprev = J.scene ? J.scene->y_prev : C->y_prev;
+            const double xv = (prm->current_volume - prm->target_volume) / prm->target_volume;
+            const double y = yprev + xv * grid->dt;
+            yprev = y;
+            const double kp = prm->gain * 2.3 / (25.0 * 0.01);
+            const double ki = kp * kp / 16.0;
+            J.rhs_correct = -(kp * xv + ki * y) / (xv + 1.0);
+        }
+        res[g].rhs_correct = J.rhs_correct;
+        const double *warm = (prm->warm_start && J.scene && J.scene->has_pressure) ? J.scene_p : nullptr;
+        const GridLevel &GL = F0.gl[g];
+        const int nbg = GL.brick_end - GL.brick_start;
+        K.go("operator", -1, 0,
+             (double)nbg * BRICK_CELLS *
+                 ((J.G.phi64 ? 8 : 4) + (grid->solid ? 4 : 0) + 12 + 1 + 6 * 8 + (P.mixed ? 16 : 0)),
+             dim3(brick_grid(nbg)), dim3(BLOCK), k_operator, J.G, F0.desc, GL, (const uint8_t *)J.cflag, P.dir64,
+             P.gx64, P.gy64, P.gz64, dir32, gx32, gy32, gz32, P.b64, P.x64, warm, J.rhs_correct);
     }
-    double rhs_correct = 0.0;
-    if (prm->gain && prm->target_volume) {
-        double &yprev = scene ? scene->y_prev : C->y_prev;
-        const double xv = (prm->current_volume - prm->target_volume) / prm->target_volume;
-        const double y = yprev + xv * grid->dt;
-        yprev = y;
-        const double kp = prm->gain * 2.3 / (25.0 * 0.01);
-        const double ki = kp * kp / 16.0;
-        rhs_correct = -(kp * xv + ki * y) / (xv + 1.0);
-    }
-    res->rhs_correct = rhs_correct;
-    double *scene_p = nullptr;
-    if (scene) SHKZ_TRY(arena_get(AR, "scene" + std::to_string(grid->scene_id), M, &scene_p));
-    const double *warm = (prm->warm_start && scene && scene->has_pressure) ? scene_p : nullptr;
-
-    const unsigned g0 = brick_grid(F0.nbricks);
-    K.go("operator", -1, 0,
-         (double)n0 * ((G.phi64 ? 8 : 4) + (grid->solid ? 4 : 0) + 12 + 1 + 6 * 8 + (P.mixed ? 16 : 0)), dim3(g0),
-         dim3(BLOCK), k_operator, G, F0.desc, (const uint8_t *)cflag, P.dir64, P.gx64, P.gy64, P.gz64, dir32, gx32,
-         gy32, gz32, P.b64, P.x64, warm, rhs_correct);
     for (int l = 1; l < P.nlev; ++l) {
         LevelHost &Lf = C->levels[l - 1];
         LevelHost &Lc = C->levels[l];
         const dim3 g(brick_grid(Lc.nbricks)), b(BLOCK);
         if (P.fp64_levels) {
-            K.go("coarsen", l, 0, 0, g, b, k_coarsen<double, double>, Lf.desc, op_of<double>(Lf), Lc.desc,
+            K.go("coarsen", -1, 0, 0, g, b, k_coarsen<double, double>, Lf.desc, op_of<double>(Lf), Lc.desc,
                  (double *)Lc.dir, (double *)Lc.gx, (double *)Lc.gy, (double *)Lc.gz);
         } else if (l == 1) {
             OpDesc<double> A64{P.dir64, P.gx64, P.gy64, P.gz64};
-            K.go("coarsen", l, 0, 0, g, b, k_coarsen<double, float>, Lf.desc, A64, Lc.desc, (float *)Lc.dir,
+            K.go("coarsen", -1, 0, 0, g, b, k_coarsen<double, float>, Lf.desc, A64, Lc.desc, (float *)Lc.dir,
                  (float *)Lc.gx, (float *)Lc.gy, (float *)Lc.gz);
         } else {
-            K.go("coarsen", l, 0, 0, g, b, k_coarsen<float, float>, Lf.desc, op_of<float>(Lf), Lc.desc,
+            K.go("coarsen", -1, 0, 0, g, b, k_coarsen<float, float>, Lf.desc, op_of<float>(Lf), Lc.desc,
                  (float *)Lc.dir, (float *)Lc.gx, (float *)Lc.gy, (float *)Lc.gz);
         }
     }
@@ -871,159 +997,288 @@ static int project_stages(shkz_ctx *C, const shkz_grid *grid, const shkz_params 
                 K.go("idiag", -1, 0, 0, g, b, k_idiag<float>, L.desc, op_of<float>(L), (float *)L.idiag);
         }
     }
-    // unknowns per level (result + byte model of the kernel timers)
+    // unknowns per level and grid (results + byte model of the kernel timers)
     {
-        int64_t *cnt = nullptr;
-        SHKZ_TRY(arena_get(AR, "counts", MAX_LEVELS, &cnt));
-        K.go("reset", -1, 0, 0, dim3(1), dim3(64), k_reset_ints, (int *)cnt, 2 * MAX_LEVELS);
+        int total = 0;
+        for (const LevelHost &L : C->levels) total += L.nbricks;
+        int *cnt = nullptr;
+        SHKZ_TRY(arena_get(AR, "counts", (size_t)std::max(total, 1), &cnt));
+        int off = 0;
         for (int l = 0; l < P.nlev; ++l) {
             LevelHost &L = C->levels[l];
-            const bool f64 = (l == 0) || P.fp64_levels;
-            const void *dp = (l == 0) ? (const void *)P.dir64 : L.dir;
-            const size_t n = (size_t)L.nbricks * BRICK_CELLS;
-            if (n == 0) continue;
-            K.go("count", -1, 0, (double)n * (f64 ? 8 : 4), dim3((unsigned)std::min<size_t>((n + 255) / 256, 2048)),
-                 dim3(256), k_count_unknowns, dp, f64 ? 1 : 0, n, (unsigned long long *)(cnt + l));
+            const dim3 g(brick_grid(L.nbricks)), b(BLOCK);
+            if (l == 0)
+                K.go("count", -1, 0, 0, g, b, k_brick_counts<double>, L.desc, (const double *)P.dir64, cnt + off);
+            else if (P.fp64_levels)
+                K.go("count", -1, 0, 0, g, b, k_brick_counts<double>, L.desc, (const double *)L.dir, cnt + off);
+            else
+                K.go("count", -1, 0, 0, g, b, k_brick_counts<float>, L.desc, (const float *)L.dir, cnt + off);
+            off += L.nbricks;
         }
-        std::vector<int64_t> hc(MAX_LEVELS, 0);
-        SHKZ_HIP(hipMemcpyAsync(hc.data(), cnt, MAX_LEVELS * sizeof(int64_t), hipMemcpyDeviceToHost, s));
+        std::vector<int> hc(std::max(total, 1), 0);
+        if (total) SHKZ_HIP(hipMemcpyAsync(hc.data(), cnt, (size_t)total * sizeof(int), hipMemcpyDeviceToHost, s));
         SHKZ_HIP(hipStreamSynchronize(s));
         SHKZ_HIP(hipGetLastError());
-        for (int l = 0; l < P.nlev; ++l) C->levels[l].unknowns = hc[l];
-        res->n_unknowns = hc[0];
+        off = 0;
+        for (int l = 0; l < P.nlev; ++l) {
+            LevelHost &L = C->levels[l];
+            L.unknowns_grid.assign(ngrids, 0);
+            L.unknowns = 0;
+            for (int bk = 0; bk < L.nbricks; ++bk) {
+                L.unknowns_grid[L.bgrid[bk]] += hc[off + bk];
+                L.unknowns += hc[off + bk];
+            }
+            off += L.nbricks;
+        }
+        for (int g = 0; g < ngrids; ++g) {
+            res[g].n_unknowns = F0.unknowns_grid[g];
+            res[g].n_bricks = F0.gl[g].brick_end - F0.gl[g].brick_start;
+            int depth = 0;
+            for (int l = 0; l < P.nlev; ++l) depth += C->levels[l].gl[g].brick_end > C->levels[l].gl[g].brick_start;
+            res[g].n_levels = depth;
+        }
     }
-    const double t_build_issue = now_ms();
     if (exp) {
-        K.go("export_system", -1, 0, 0, dim3((unsigned)((M + 255) / 256)), dim3(256), k_export_system, G, F0.desc,
-             OpDesc<double>{P.dir64, P.gx64, P.gy64, P.gz64}, (const double *)P.b64, exp->diag, exp->gx, exp->gy,
-             exp->gz, exp->rhs);
+        const GridJob &J = jobs[0];
+        K.go("export_system", -1, 0, 0, dim3((unsigned)((J.M + 255) / 256)), dim3(256), k_export_system, J.G, F0.desc,
+             F0.gl[0], OpDesc<double>{P.dir64, P.gx64, P.gy64, P.gz64}, (const double *)P.b64, exp->diag, exp->gx,
+             exp->gy, exp->gz, exp->rhs);
         SHKZ_HIP(hipStreamSynchronize(s));
         SHKZ_HIP(hipGetLastError());
         if (C->timing) resolve_timing(C);
         return SHKZ_OK;
     }
+    const double t_build = now_ms();
 
     // ---------------------------------------------------------------- solve
-    SolveScalars hs{};
     const int norm = prm->norm == SHKZ_NORM_L2 ? 1 : 0;
     const int maxit = prm->max_iterations > 0 ? prm->max_iterations : 30000;
     const double residual = prm->residual < 1e-30 ? 1e-30 : prm->residual;
     OpDesc<double> A64{P.dir64, P.gx64, P.gy64, P.gz64};
+    const unsigned g0 = brick_grid(F0.nbricks);
     const dim3 gb(g0), bb(BLOCK);
-    SHKZ_TRY(upload_scalars(C, s, P.S_out, norm, maxit));
-    const double t_build = now_ms();
-    int total_iters = 0, outer = 0;
-    bool converged = false;
-    int status = SHKZ_OK;
-    if (res->n_unknowns == 0) {
-        converged = true;
-        outer = 0;
-        hs.r0inf = hs.r0l2 = 0.0;
+    std::vector<int> total_iters(ngrids, 0), outer_passes(ngrids, 0), status(ngrids, SHKZ_OK);
+    std::vector<SolveScalars> hs(ngrids), hi, hf;
+    if (F0.nbricks > 0) {
+        SHKZ_TRY(upload_scalars(C, s, P.S_out, ngrids, norm, maxit));
+        K.go("init_ctl", -1, 0, 0, dim3(1), dim3(64), k_init_ctl, P.ctl, ngrids);
+    }
+    if (F0.nbricks == 0) {
+        // no unknowns anywhere: nothing to solve
     } else if (!P.mixed) {
-        // one FP64 Krylov solve: CG (reference iteration) or MGPCG
-        K.go("defect", 0, 7 * 8, 0, gb, bb, k_defect<double>, F0.desc, A64, (const double *)P.b64,
-             (const double *)P.x64, P.r64, P.red);
-        finalize(K, P, g0, P.S_out, FIN_DEFECT_FIRST, residual);
-        SHKZ_TRY(read_scalars(C, s, P.S_out, &hs));
-        if (!hs.done) {
+        // one FP64 Krylov solve per grid: CG (the reference iteration) or MGPCG
+        K.go("defect", 0, 7 * 8, 0, gb, bb, k_defect<double>, F0.desc, A64, (const double *)P.b64, (const double *)P.x64,
+             P.r64, P.red);
+        finalize(K, P, F0.desc, P.S_out, P.ctl, FIN_DEFECT_FIRST, residual);
+        BatchCtl hc{};
+        SHKZ_TRY(read_scalars(C, s, P.S_out, ngrids, hs, P.ctl, &hc));
+        if (!hc.all_done)
             SHKZ_TRY(krylov<double>(C, K, P, P.x64, P.r64, (double *)P.kp, (double *)P.kq, (double *)P.kz,
-                                    (double *)P.ktmp, P.S_out, prm->solver == SHKZ_SOLVER_MGPCG, &hs));
+                                    (double *)P.ktmp, P.S_out, P.ctl, multigrid, hs));
+        for (int g = 0; g < ngrids; ++g) {
+            total_iters[g] = hs[g].iter;
+            outer_passes[g] = F0.gl[g].brick_end > F0.gl[g].brick_start ? 1 : 0;
+            if (hs[g].status) status[g] = SHKZ_ERR_NUMERIC;
         }
-        total_iters = hs.iter;
-        converged = hs.converged != 0;
-        outer = 1;
-        if (hs.status) status = SHKZ_ERR_NUMERIC;
     } else {
-        // FP64 defect correction around FP32 MGPCG solves
+        // FP64 defect correction around FP32 MGPCG solves, per grid: the outer scalars decide on
+        // the device when a grid is done (converged, out of iterations, or no longer reduced)
         const double eta = prm->inner_residual > 0 ? prm->inner_residual : 1e-3;
-        SolveScalars hi{};
-        SHKZ_TRY(upload_scalars(C, s, P.S_in, norm, maxit));
-        double prev = -1.0;
-        for (outer = 0; outer < 64; ++outer) {
+        SHKZ_TRY(upload_scalars(C, s, P.S_in, ngrids, norm, maxit));
+        for (int outer = 0; outer < 64; ++outer) {
             K.go("defect", 0, 7 * 8, 0, gb, bb, k_defect<double>, F0.desc, A64, (const double *)P.b64,
                  (const double *)P.x64, P.r64, P.red);
-            finalize(K, P, g0, P.S_out, outer == 0 ? FIN_DEFECT_FIRST : FIN_DEFECT, residual);
-            SHKZ_TRY(read_scalars(C, s, P.S_out, &hs));
-            if (hs.status) {
-                status = SHKZ_ERR_NUMERIC;
-                break;
-            }
-            const double cur = norm == 0 ? hs.rinf : hs.rl2;
-            if (outer == 0 && hs.done) {
-                converged = hs.converged != 0;
-                break;
-            }
-            if (cur <= hs.tol) {
-                converged = true;
-                break;
-            }
-            if (total_iters >= maxit) break;
-            if (prev >= 0 && cur > 0.9 * prev) break;  // FP32 inner solves no longer reduce the defect
-            prev = cur;
+            if (outer == 0)
+                finalize(K, P, F0.desc, P.S_out, nullptr, FIN_DEFECT_FIRST, residual);
+            else
+                finalize(K, P, F0.desc, P.S_out, nullptr, FIN_OUTER, 0.0, 0, P.S_in);
+            K.go("init_ctl", -1, 0, 0, dim3(1), dim3(64), k_init_ctl, P.ctl + 1, ngrids);
             K.go("inner_init", 0, 16, 0, gb, bb, k_inner_init, F0.desc, (const double *)P.r64, (float *)P.kr,
                  (float *)P.kx, P.red);
-            finalize(K, P, g0, P.S_in, FIN_INNER_INIT, std::max(eta, hs.tol / std::max(cur, 1e-300)),
-                     maxit - total_iters);
-            SHKZ_TRY(read_scalars(C, s, P.S_in, &hi));
-            if (!hi.done) {
-                SHKZ_TRY(krylov<float>(C, K, P, (float *)P.kx, (float *)P.kr, (float *)P.kp, (float *)P.kq,
-                                       (float *)P.kz, (float *)P.ktmp, P.S_in, true, &hi));
-            }
-            total_iters += hi.iter;
+            finalize(K, P, F0.desc, P.S_in, P.ctl + 1, FIN_INNER_INIT, eta, 0, P.S_out);
+            BatchCtl hc{};
+            SHKZ_TRY(read_scalars(C, s, P.S_out, ngrids, hs, P.ctl + 1, &hc));
+            if (outer > 0)
+                for (int g = 0; g < ngrids; ++g) total_iters[g] = hs[g].iter;
+            if (hc.all_done) break;
+            for (int g = 0; g < ngrids; ++g)
+                if (!hs[g].done) outer_passes[g] = outer + 1;
+            SHKZ_TRY(krylov<float>(C, K, P, (float *)P.kx, (float *)P.kr, (float *)P.kp, (float *)P.kq, (float *)P.kz,
+                                   (float *)P.ktmp, P.S_in, P.ctl + 1, true, hi));
             if (C->trace)
-                std::fprintf(stderr, "[shkz] outer %d: defect inf %.6e (tol %.3e)  inner it %d rinf %.6e r0 %.6e tol %.3e rz %.9e\n",
-                             outer, hs.rinf, hs.tol, hi.iter, hi.rinf, hi.r0inf, hi.tol, hi.rz);
-            if (hi.status) {
-                status = SHKZ_ERR_NUMERIC;
-                break;
-            }
+                for (int g = 0; g < ngrids; ++g)
+                    if (!hs[g].done)
+                        std::fprintf(stderr, "[shkz] grid %d outer %d: defect %.6e (tol %.3e) inner it %d rinf %.6e\n",
+                                     g, outer, norm == 0 ? hs[g].rinf : hs[g].rl2, hs[g].tol, hi[g].iter, hi[g].rinf);
             K.go("add_correction", 0, 20, 0, gb, bb, k_add_correction, F0.desc, P.x64, (const float *)P.kx);
+            if (outer == 63)
+                for (int g = 0; g < ngrids; ++g)
+                    if (!hs[g].done) total_iters[g] += hi[g].iter;
         }
-        outer = std::max(outer, 1);
+        for (int g = 0; g < ngrids; ++g)
+            if (hs[g].status) status[g] = SHKZ_ERR_NUMERIC;
     }
-    // true FP64 residual of the returned pressure
-    if (res->n_unknowns > 0) {
-        SolveScalars hf{};
-        K.go("defect", 0, 7 * 8, 0, gb, bb, k_defect<double>, F0.desc, A64, (const double *)P.b64,
-             (const double *)P.x64, P.r64, P.red);
-        finalize(K, P, g0, P.S_in, FIN_DEFECT);
-        SHKZ_TRY(read_scalars(C, s, P.S_in, &hf));
-        const double r0i = hs.r0inf > 0 ? hs.r0inf : 1.0, r0l = hs.r0l2 > 0 ? hs.r0l2 : 1.0;
-        res->reresid_inf = hf.rinf / r0i;
-        res->reresid_l2 = hf.rl2 / r0l;
-        res->reresid = norm == 0 ? res->reresid_inf : res->reresid_l2;
-        if (!P.mixed) converged = hs.converged != 0;
+    // true FP64 residual of the returned pressure, per grid
+    if (F0.nbricks > 0) {
+        K.go("defect", 0, 7 * 8, 0, gb, bb, k_defect<double>, F0.desc, A64, (const double *)P.b64, (const double *)P.x64,
+             P.r64, P.red);
+        finalize(K, P, F0.desc, P.S_fin, nullptr, FIN_DEFECT);
+        SHKZ_TRY(read_scalars(C, s, P.S_fin, ngrids, hf));
+    }
+    for (int g = 0; g < ngrids; ++g) {
+        if (F0.gl[g].brick_end == F0.gl[g].brick_start) {
+            res[g].converged = 1;  // no unknowns
+            continue;
+        }
+        const SolveScalars &o = hs[g];
+        res[g].reresid_inf = o.r0inf > 0 ? hf[g].rinf / o.r0inf : 0.0;
+        res[g].reresid_l2 = o.r0l2 > 0 ? hf[g].rl2 / o.r0l2 : 0.0;
+        res[g].reresid = norm == 0 ? res[g].reresid_inf : res[g].reresid_l2;
+        const double cur = norm == 0 ? hf[g].rinf : hf[g].rl2;
+        const double r0 = norm == 0 ? o.r0inf : o.r0l2;
+        res[g].converged = (r0 == 0.0 || cur <= o.tol * (1.0 + 1e-12)) ? 1 : 0;
+        if (!P.mixed && !o.converged && r0 != 0.0) res[g].converged = 0;
     }
     const double t_solve = now_ms();
 
-    // ---------------------------------------------------------------- scatter + update
-    if (grid->pressure || grid->pressure64)
-        K.go("scatter_pressure", -1, 0, (double)M * ((grid->pressure ? 4 : 0) + (grid->pressure64 ? 8 : 0)),
-             dim3((unsigned)((M + 255) / 256)), dim3(256), k_scatter_pressure, G, F0.desc, (const double *)P.x64,
-             grid->pressure, grid->pressure64);
-    if (scene)
-        K.go("scatter_pressure", -1, 0, (double)M * 8, dim3((unsigned)((M + 255) / 256)), dim3(256), k_scatter_pressure,
-             G, F0.desc, (const double *)P.x64, (float *)nullptr, scene_p);
-    for (int dim = 0; dim < 3; ++dim) {
-        const size_t nf = (size_t)(w + (dim == 0)) * (h + (dim == 1)) * (d + (dim == 2));
-        K.go("velocity_update", -1, 0, (double)nf * (8 + (G.phi64 ? 16 : 8) + (grid->solid ? 4 : 0)),
-             dim3((unsigned)((nf + 255) / 256)), dim3(256), k_velocity_update, G, dim, F0.desc, (const double *)P.x64);
+    // ---------------------------------------------------------------- per grid: scatter + update
+    for (int g = 0; g < ngrids; ++g) {
+        GridJob &J = jobs[g];
+        const shkz_grid *grid = J.in;
+        const GridLevel &GL = F0.gl[g];
+        if (grid->pressure || grid->pressure64)
+            K.go("scatter_pressure", -1, 0, (double)J.M * ((grid->pressure ? 4 : 0) + (grid->pressure64 ? 8 : 0)),
+                 dim3((unsigned)((J.M + 255) / 256)), dim3(256), k_scatter_pressure, J.G, F0.desc, GL,
+                 (const double *)P.x64, grid->pressure, grid->pressure64);
+        if (J.scene)
+            K.go("scatter_pressure", -1, 0, (double)J.M * 8, dim3((unsigned)((J.M + 255) / 256)), dim3(256),
+                 k_scatter_pressure, J.G, F0.desc, GL, (const double *)P.x64, (float *)nullptr, J.scene_p);
+        const size_t ne = (size_t)(grid->w + 1) * (grid->h + 1) * (grid->d + 1);
+        K.go("velocity_update", -1, 0, (double)J.M * (24 + (J.G.phi64 ? 8 : 4) + (grid->solid ? 4 : 0)),
+             dim3((unsigned)((ne + 255) / 256)), dim3(256), k_velocity_update, J.G, F0.desc, GL, (const double *)P.x64);
+        if (J.scene) J.scene->has_pressure = true;
+        res[g].iterations = total_iters[g];
+        res[g].outer_iterations = outer_passes[g];
+        res[g].status = status[g];
     }
-    if (scene) scene->has_pressure = true;
-    if (C->timing) resolve_timing(C);
+    if (C->timing) {
+        SHKZ_HIP(hipStreamSynchronize(s));
+        resolve_timing(C);
+    }
     const double t_end = now_ms();
-    res->converged = converged ? 1 : 0;
-    res->iterations = total_iters;
-    res->outer_iterations = outer;
-    res->status = status;
-    res->t_solid_fluid_fractions = t_frac - t_start;
-    res->t_build_highres_linsystem = t_build - t_frac;
-    res->t_linsolve = t_solve - t_build;
-    res->t_update_velocity = t_end - t_solve;
-    res->t_projection = t_end - t_start;
-    (void)t_build_issue;
-    if (status != SHKZ_OK) return fail(status, "non-finite value met in the linear solve");
+    int rc = SHKZ_OK;
+    for (int g = 0; g < ngrids; ++g) {
+        shkz_result &r = res[g];
+        r.t_solid_fluid_fractions = t_frac - t_start;
+        r.t_build_highres_linsystem = t_build - t_frac;
+        r.t_linsolve = t_solve - t_build;
+        r.t_update_velocity = t_end - t_solve;
+        r.t_projection = t_end - t_start;
+        if (status[g] != SHKZ_OK) rc = status[g];
+    }
+    if (rc != SHKZ_OK) return fail(rc, "non-finite value met in the linear solve");
     return SHKZ_OK;
+}
+
+// Host-array path: stages every grid's arrays on the device (buffers named per batch slot),
+// projects, copies velocities and requested pressures back.
+static int project_host(shkz_ctx *C, const shkz_grid *grids, int ngrids, const shkz_params *params, shkz_result *res,
+                        double *const *export_host) {
+    for (int g = 0; g < ngrids; ++g) SHKZ_TRY(validate_grid(&grids[g]));
+    SHKZ_HIP(hipSetDevice(C->device));
+    Arena &AR = C->arena;
+    std::vector<shkz_grid> dg(grids, grids + ngrids);
+    struct Back {
+        float *dv[3];
+        size_t nf[3];
+        float *dp;
+        double *dp64;
+        size_t M;
+    };
+    std::vector<Back> back(ngrids);
+    for (int g = 0; g < ngrids; ++g) {
+        const shkz_grid *grid = &grids[g];
+        const std::string sfx = std::to_string(g);
+        const size_t w = grid->w, h = grid->h, d = grid->d;
+        const size_t M = w * h * d, Nn = (w + 1) * (h + 1) * (d + 1);
+        const size_t nf[3] = {(w + 1) * h * d, w * (h + 1) * d, w * h * (d + 1)};
+        void *phi = nullptr, *solid = nullptr;
+        const size_t pe = grid->fluid_dtype == SHKZ_F64 ? 8 : 4, se = grid->solid_dtype == SHKZ_F64 ? 8 : 4;
+        SHKZ_TRY(AR.get("h_phi" + sfx, M * pe, &phi));
+        SHKZ_HIP(hipMemcpy(phi, grid->fluid, M * pe, hipMemcpyHostToDevice));
+        dg[g].fluid = phi;
+        if (grid->solid) {
+            SHKZ_TRY(AR.get("h_solid" + sfx, Nn * se, &solid));
+            SHKZ_HIP(hipMemcpy(solid, grid->solid, Nn * se, hipMemcpyHostToDevice));
+            dg[g].solid = solid;
+        }
+        float *hv[3] = {grid->u, grid->v, grid->w_vel};
+        const uint8_t *ha[3] = {grid->active_u, grid->active_v, grid->active_w};
+        uint8_t *da[3] = {nullptr, nullptr, nullptr};
+        const char *vn[3] = {"h_u", "h_v", "h_w"}, *an[3] = {"h_au", "h_av", "h_aw"};
+        Back &B = back[g];
+        for (int dim = 0; dim < 3; ++dim) {
+            void *pp = nullptr;
+            SHKZ_TRY(AR.get(vn[dim] + sfx, nf[dim] * 4, &pp));
+            B.dv[dim] = (float *)pp;
+            B.nf[dim] = nf[dim];
+            SHKZ_HIP(hipMemcpy(B.dv[dim], hv[dim], nf[dim] * 4, hipMemcpyHostToDevice));
+            if (ha[dim]) {
+                SHKZ_TRY(AR.get(an[dim] + sfx, nf[dim], &pp));
+                da[dim] = (uint8_t *)pp;
+                SHKZ_HIP(hipMemcpy(da[dim], ha[dim], nf[dim], hipMemcpyHostToDevice));
+            }
+        }
+        dg[g].u = B.dv[0];
+        dg[g].v = B.dv[1];
+        dg[g].w_vel = B.dv[2];
+        dg[g].active_u = da[0];
+        dg[g].active_v = da[1];
+        dg[g].active_w = da[2];
+        B.dp = nullptr;
+        B.dp64 = nullptr;
+        B.M = M;
+        if (grid->pressure) {
+            void *pp = nullptr;
+            SHKZ_TRY(AR.get("h_p" + sfx, M * 4, &pp));
+            B.dp = (float *)pp;
+        }
+        if (grid->pressure64) {
+            void *pp = nullptr;
+            SHKZ_TRY(AR.get("h_p64" + sfx, M * 8, &pp));
+            B.dp64 = (double *)pp;
+        }
+        dg[g].pressure = B.dp;
+        dg[g].pressure64 = B.dp64;
+    }
+    shkz_params def;
+    shkz_default_params(&def);
+    if (export_host) {
+        const size_t M = back[0].M;
+        double *ex[5];
+        for (int q = 0; q < 5; ++q) {
+            void *pp = nullptr;
+            SHKZ_TRY(AR.get("h_export" + std::to_string(q), M * 8, &pp));
+            ex[q] = (double *)pp;
+        }
+        ExportOut eo{ex[0], ex[1], ex[2], ex[3], ex[4]};
+        const int rc = project_impl(C, dg.data(), 1, params ? params : &def, res, nullptr, &eo);
+        if (rc != SHKZ_OK) return rc;
+        for (int q = 0; q < 5; ++q) SHKZ_HIP(hipMemcpy(export_host[q], ex[q], M * 8, hipMemcpyDeviceToHost));
+        return SHKZ_OK;
+    }
+    const int rc = project_impl(C, dg.data(), ngrids, params ? params : &def, res, nullptr);
+    if (rc != SHKZ_OK && rc != SHKZ_ERR_NUMERIC) return rc;
+    for (int g = 0; g < ngrids; ++g) {
+        const shkz_grid *grid = &grids[g];
+        float *hv[3] = {grid->u, grid->v, grid->w_vel};
+        for (int dim = 0; dim < 3; ++dim)
+            SHKZ_HIP(hipMemcpy(hv[dim], back[g].dv[dim], back[g].nf[dim] * 4, hipMemcpyDeviceToHost));
+        if (grid->pressure) SHKZ_HIP(hipMemcpy(grid->pressure, back[g].dp, back[g].M * 4, hipMemcpyDeviceToHost));
+        if (grid->pressure64)
+            SHKZ_HIP(hipMemcpy(grid->pressure64, back[g].dp64, back[g].M * 8, hipMemcpyDeviceToHost));
+    }
+    return rc;
 }
 
 }  // namespace shkz
@@ -1061,7 +1316,9 @@ int shkz_create(int device, uint64_t hbm_budget_bytes, shkz_ctx **out) {
     shkz_ctx *C = new shkz_ctx();
     C->device = device;
     C->arena.budget = hbm_budget_bytes ? (size_t)hbm_budget_bytes : (size_t)(0.9 * (double)total_b);
-    if (hipHostMalloc((void **)&C->host_scalars, sizeof(SolveScalars), hipHostMallocDefault) != hipSuccess) {
+    if (ensure_host_scalars(C, 16) != SHKZ_OK ||
+        hipHostMalloc((void **)&C->host_ctl, sizeof(BatchCtl), hipHostMallocDefault) != hipSuccess) {
+        if (C->host_scalars) (void)hipHostFree(C->host_scalars);
         delete C;
         return fail(SHKZ_ERR_NOMEM, "pinned scalar buffer allocation failed");
     }
@@ -1069,6 +1326,7 @@ int shkz_create(int device, uint64_t hbm_budget_bytes, shkz_ctx **out) {
         hipEventCreateWithFlags(&C->ev_in, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&C->ev_out, hipEventDisableTiming) != hipSuccess) {
         (void)hipHostFree(C->host_scalars);
+        (void)hipHostFree(C->host_ctl);
         delete C;
         return fail(SHKZ_ERR_DEVICE, "stream/event creation failed");
     }
@@ -1083,7 +1341,7 @@ int shkz_create(int device, uint64_t hbm_budget_bytes, shkz_ctx **out) {
 void shkz_destroy(shkz_ctx *C) {
     if (!C) return;
     (void)hipSetDevice(C->device);
-    (void)hipDeviceSynchronize();
+    if (C->work) (void)hipStreamSynchronize(C->work);
     for (auto &t : C->pending) {
         (void)hipEventDestroy(t.ev0);
         (void)hipEventDestroy(t.ev1);
@@ -1096,120 +1354,50 @@ void shkz_destroy(shkz_ctx *C) {
     if (C->work) (void)hipStreamDestroy(C->work);
     C->arena.clear();
     if (C->host_scalars) (void)hipHostFree(C->host_scalars);
+    if (C->host_ctl) (void)hipHostFree(C->host_ctl);
     delete C;
+}
+
+int shkz_project_batch_device(shkz_ctx *C, const shkz_grid *grids, int ngrids, const shkz_params *params,
+                              shkz_result *results, void *stream) {
+    if (!C || !grids || !results || ngrids <= 0) return fail(SHKZ_ERR_INVALID, "NULL argument or empty batch");
+    shkz_params def;
+    shkz_default_params(&def);
+    const shkz_params *p = params ? params : &def;
+    std::lock_guard<std::mutex> lock(C->mu);
+    return project_impl(C, grids, ngrids, p, results, (hipStream_t)stream);
 }
 
 int shkz_project_device(shkz_ctx *C, const shkz_grid *grid, const shkz_params *params, shkz_result *result,
                         void *stream) {
     if (!C || !grid || !result) return fail(SHKZ_ERR_INVALID, "NULL argument");
-    shkz_params def;
-    shkz_default_params(&def);
-    const shkz_params *p = params ? params : &def;
-    std::memset(result, 0, sizeof(*result));
+    return shkz_project_batch_device(C, grid, 1, params, result, stream);
+}
+
+int shkz_project_batch(shkz_ctx *C, const shkz_grid *grids, int ngrids, const shkz_params *params,
+                       shkz_result *results) {
+    if (!C || !grids || !results || ngrids <= 0) return fail(SHKZ_ERR_INVALID, "NULL argument or empty batch");
+    for (int g = 0; g < ngrids; ++g) std::memset(&results[g], 0, sizeof(shkz_result));
     std::lock_guard<std::mutex> lock(C->mu);
-    int rc = project_impl(C, grid, p, result, (hipStream_t)stream);
-    result->status = rc;
+    const int rc = project_host(C, grids, ngrids, params, results, nullptr);
+    for (int g = 0; g < ngrids; ++g)
+        if (rc != SHKZ_OK && rc != SHKZ_ERR_NUMERIC && results[g].status == SHKZ_OK) results[g].status = rc;
     return rc;
 }
 
-static int project_host(shkz_ctx *C, const shkz_grid *grid, const shkz_params *params, shkz_result *result,
-                        double *const *export_host);
-
 int shkz_project(shkz_ctx *C, const shkz_grid *grid, const shkz_params *params, shkz_result *result) {
-    if (result) std::memset(result, 0, sizeof(*result));
-    const int rc = project_host(C, grid, params, result, nullptr);
-    if (result) result->status = rc;
-    return rc;
+    if (!C || !grid || !result) return fail(SHKZ_ERR_INVALID, "NULL argument");
+    return shkz_project_batch(C, grid, 1, params, result);
 }
 
 int shkz_export_system(shkz_ctx *C, const shkz_grid *grid, const shkz_params *params, double *diag, double *gx,
                        double *gy, double *gz, double *rhs) {
+    if (!C || !grid) return fail(SHKZ_ERR_INVALID, "NULL argument");
     if (!diag || !gx || !gy || !gz || !rhs) return fail(SHKZ_ERR_INVALID, "export arrays are required");
     double *outs[5] = {diag, gx, gy, gz, rhs};
     shkz_result r;
-    return project_host(C, grid, params, &r, outs);
-}
-
-static int project_host(shkz_ctx *C, const shkz_grid *grid, const shkz_params *params, shkz_result *result,
-                        double *const *export_host) {
-    if (!C || !grid || !result) return fail(SHKZ_ERR_INVALID, "NULL argument");
-    if (grid->w <= 0 || grid->h <= 0 || grid->d <= 0) return fail(SHKZ_ERR_INVALID, "grid dimensions must be positive");
-    if (!grid->fluid || !grid->u || !grid->v || !grid->w_vel) return fail(SHKZ_ERR_INVALID, "fluid and velocities are required");
     std::lock_guard<std::mutex> lock(C->mu);
-    SHKZ_HIP(hipSetDevice(C->device));
-    Arena &AR = C->arena;
-    const size_t w = grid->w, h = grid->h, d = grid->d;
-    const size_t M = w * h * d, Nn = (w + 1) * (h + 1) * (d + 1);
-    const size_t nf[3] = {(w + 1) * h * d, w * (h + 1) * d, w * h * (d + 1)};
-    shkz_grid dg = *grid;
-    void *phi = nullptr, *solid = nullptr;
-    const size_t pe = grid->fluid_dtype == SHKZ_F64 ? 8 : 4, se = grid->solid_dtype == SHKZ_F64 ? 8 : 4;
-    SHKZ_TRY(AR.get("h_phi", M * pe, &phi));
-    SHKZ_HIP(hipMemcpy(phi, grid->fluid, M * pe, hipMemcpyHostToDevice));
-    dg.fluid = phi;
-    if (grid->solid) {
-        SHKZ_TRY(AR.get("h_solid", Nn * se, &solid));
-        SHKZ_HIP(hipMemcpy(solid, grid->solid, Nn * se, hipMemcpyHostToDevice));
-        dg.solid = solid;
-    }
-    float *hv[3] = {grid->u, grid->v, grid->w_vel};
-    const uint8_t *ha[3] = {grid->active_u, grid->active_v, grid->active_w};
-    float *dv[3];
-    uint8_t *da[3] = {nullptr, nullptr, nullptr};
-    const char *vn[3] = {"h_u", "h_v", "h_w"}, *an[3] = {"h_au", "h_av", "h_aw"};
-    for (int dim = 0; dim < 3; ++dim) {
-        void *pp = nullptr;
-        SHKZ_TRY(AR.get(vn[dim], nf[dim] * 4, &pp));
-        dv[dim] = (float *)pp;
-        SHKZ_HIP(hipMemcpy(dv[dim], hv[dim], nf[dim] * 4, hipMemcpyHostToDevice));
-        if (ha[dim]) {
-            SHKZ_TRY(AR.get(an[dim], nf[dim], &pp));
-            da[dim] = (uint8_t *)pp;
-            SHKZ_HIP(hipMemcpy(da[dim], ha[dim], nf[dim], hipMemcpyHostToDevice));
-        }
-    }
-    dg.u = dv[0];
-    dg.v = dv[1];
-    dg.w_vel = dv[2];
-    dg.active_u = da[0];
-    dg.active_v = da[1];
-    dg.active_w = da[2];
-    float *dp = nullptr;
-    double *dp64 = nullptr;
-    if (grid->pressure) {
-        void *pp = nullptr;
-        SHKZ_TRY(AR.get("h_p", M * 4, &pp));
-        dp = (float *)pp;
-    }
-    if (grid->pressure64) {
-        void *pp = nullptr;
-        SHKZ_TRY(AR.get("h_p64", M * 8, &pp));
-        dp64 = (double *)pp;
-    }
-    dg.pressure = dp;
-    dg.pressure64 = dp64;
-    shkz_params def;
-    shkz_default_params(&def);
-    if (export_host) {
-        double *ex[5];
-        for (int q = 0; q < 5; ++q) {
-            void *pp = nullptr;
-            SHKZ_TRY(AR.get("h_export" + std::to_string(q), M * 8, &pp));
-            ex[q] = (double *)pp;
-        }
-        ExportOut eo{ex[0], ex[1], ex[2], ex[3], ex[4]};
-        int rc = project_impl(C, &dg, params ? params : &def, result, nullptr, &eo);
-        if (rc != SHKZ_OK) return rc;
-        for (int q = 0; q < 5; ++q) SHKZ_HIP(hipMemcpy(export_host[q], ex[q], M * 8, hipMemcpyDeviceToHost));
-        return SHKZ_OK;
-    }
-    int rc = project_impl(C, &dg, params ? params : &def, result, nullptr);
-    result->status = rc;
-    if (rc != SHKZ_OK && rc != SHKZ_ERR_NUMERIC) return rc;
-    for (int dim = 0; dim < 3; ++dim) SHKZ_HIP(hipMemcpy(hv[dim], dv[dim], nf[dim] * 4, hipMemcpyDeviceToHost));
-    if (grid->pressure) SHKZ_HIP(hipMemcpy(grid->pressure, dp, M * 4, hipMemcpyDeviceToHost));
-    if (grid->pressure64) SHKZ_HIP(hipMemcpy(grid->pressure64, dp64, M * 8, hipMemcpyDeviceToHost));
-    return rc;
+    return project_host(C, grid, 1, params, &r, outs);
 }
 
 int shkz_release_scene(shkz_ctx *C, uint64_t scene_id) {
@@ -1235,7 +1423,7 @@ int shkz_get_kernel_stats(shkz_ctx *C, shkz_kernel_stat *out, int max_entries, i
     if (!C) return fail(SHKZ_ERR_INVALID, "NULL context");
     std::lock_guard<std::mutex> lock(C->mu);
     if (!C->pending.empty()) {
-        SHKZ_HIP(hipDeviceSynchronize());
+        SHKZ_HIP(hipStreamSynchronize(C->work));
         resolve_timing(C);
     }
     int n = 0;
@@ -1258,7 +1446,7 @@ int shkz_reset_kernel_stats(shkz_ctx *C) {
     if (!C) return fail(SHKZ_ERR_INVALID, "NULL context");
     std::lock_guard<std::mutex> lock(C->mu);
     if (!C->pending.empty()) {
-        SHKZ_HIP(hipDeviceSynchronize());
+        SHKZ_HIP(hipStreamSynchronize(C->work));
         resolve_timing(C);
     }
     C->totals.clear();
