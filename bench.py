@@ -33,6 +33,8 @@ CONFIGS = {
     "c3": ("spheres_box", 256, 2, 1e-6),
 }
 HBM_PEAK_GBS = 8000.0
+C4_SIDES = (32, 48, 64, 96, 128, 160)
+C4_GRIDS = 64
 
 
 def log(*a):
@@ -94,12 +96,150 @@ def cpu_baseline(scene, n_iter_needed, residual):
     }
 
 
+def kernel_roofline(stats, inst_steps, config):
+    """Roofline entry of the dominant kernel from the event-timed pass (per-kernel durations on the
+    library's stream and the algorithmic byte model), plus a per-kernel time-share log."""
+    if not stats:
+        return None
+    kern = {k: v for k, v in stats.items() if v["launches"] > 0 and v["total_ms"] > 0}
+    # dominant kernel = the HBM-bound kernel moving the most algorithmic bytes (the event-timed
+    # pass adds ~2 us per launch, which overstates the thousands of latency-bound coarse launches)
+    name, st = max(kern.items(), key=lambda kv: kv[1]["bytes"])
+    avg_ms = st["total_ms"] / st["launches"]
+    bytes_per_launch = st["bytes"] / st["launches"]
+    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "kernel": name, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "avg_launch_us": round(avg_ms * 1e3, 2), "algorithmic_bytes_per_launch": int(bytes_per_launch)}
+    roofline["time_share"] = round(st["total_ms"] / sum(v["total_ms"] for v in kern.values()), 4)
+    tr = measured_traffic(name, config)
+    if tr is not None:
+        roofline["traffic"] = tr["bytes"]
+        roofline["traffic_source"] = tr["source"]
+    total_ms = sum(v["total_ms"] for v in kern.values())
+    total_bytes = sum(v["bytes"] for v in kern.values())
+    roofline["all_kernels_gbs"] = round(total_bytes / (total_ms * 1e-3) / 1e9, 1)
+    log("[bench] kernel time share:")
+    for k, v in sorted(kern.items(), key=lambda kv: -kv[1]["total_ms"])[:14]:
+        gbs = v["bytes"] / (v["total_ms"] * 1e-3) / 1e9 if v["total_ms"] else 0
+        log(f"    {k:18s} {v['total_ms'] / inst_steps:8.3f} ms/step  {v['launches'] / inst_steps:7.1f} launches/step"
+            f"  {gbs:8.1f} GB/s")
+    return roofline
+
+
+def c4_scenes(S):
+    """64 cubes, sides drawn uniformly from {32,48,64,96,128,160} (numpy default_rng(3)), each a
+    seeded sphere-drop (scene seed 3000+i) with its own N(0,1) velocities."""
+    from concurrent.futures import ThreadPoolExecutor
+    sides = np.random.default_rng(3).choice(C4_SIDES, size=C4_GRIDS)
+    # the generator runs in the library (ctypes releases the GIL): synthesise in parallel
+    with ThreadPoolExecutor(max_workers=min(16, os.cpu_count() or 1)) as ex:
+        return list(ex.map(lambda a: S.make_scene(S.SCENE_RANDOM_DROP, int(a[1]), seed=3000 + a[0]),
+                           enumerate(sides)))
+
+
+def bench_batch(args):
+    """C4: 64 uneven grids projected in one batched call per step (mixed precision, 1e-6), against
+    the plain single-solve FP64 GPU path over the same grids one by one."""
+    import shiokaze_amd as S
+    from shiokaze_amd.devmem import DeviceArray, Stream, device_synchronize
+
+    residual = 1e-6
+    log(f"[bench] synthesising c4: {C4_GRIDS} grids")
+    scenes = c4_scenes(S)
+    cells = sum(sc.w * sc.h * sc.d for sc in scenes)
+    log(f"[bench] sides {sorted(sc.w for sc in scenes)}; {cells / 1e6:.1f} M cells")
+    dev = []
+    for sc in scenes:
+        dev.append(dict(fl=DeviceArray(sc.fluid), src=[DeviceArray(a) for a in (sc.u, sc.v, sc.wv)],
+                        vel=[DeviceArray(a) for a in (sc.u, sc.v, sc.wv)]))
+    st = Stream()
+    stream = st.handle
+    device_synchronize()
+    grids = [S.Context.device_grid(sc.w, sc.h, sc.d, sc.dt, b["fl"], S.F64, None, S.F32, *b["vel"])
+             for sc, b in zip(scenes, dev)]
+    ctx = S.Context(0)
+    prm_mixed = S.default_params(residual=residual)
+    prm_fp64 = S.default_params(residual=residual, precision=S.PREC_FP64)
+
+    def restore():
+        for b in dev:
+            for a, c in zip(b["vel"], b["src"]):
+                a.copy_from(c, stream)
+
+    def batched(prm):
+        restore()
+        return ctx.project_batch_device(grids, prm, stream=stream)
+
+    def single(prm):
+        out = []
+        for g, b in zip(grids, dev):
+            for a, c in zip(b["vel"], b["src"]):
+                a.copy_from(c, stream)
+            out.append(ctx.project_batch_device([g], prm, stream=stream)[0])
+        return out
+
+    def timed(fn, prm, steps, label):
+        for _ in range(args.warmup):
+            fn(prm)
+        st.synchronize()
+        lat, res = [], None
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            ts = time.perf_counter()
+            res = fn(prm)
+            lat.append((time.perf_counter() - ts) * 1e3)
+        st.synchronize()
+        el = time.perf_counter() - t0
+        worst = max(r.reresid_inf for r in res)
+        conv = all(r.converged for r in res)
+        its = [r.iterations for r in res]
+        log(f"[bench] {label}: {steps} x {len(res)} solves in {el:.3f}s -> {steps * len(res) / el:.1f} solves/s; "
+            f"iterations {min(its)}..{max(its)}; worst reresid_inf {worst:.2e}; all converged {conv}")
+        return steps * len(res) / el, el / steps * 1e3, lat, worst, conv
+
+    v_b, ms_b, lat_b, worst_b, conv_b = timed(batched, prm_mixed, args.steps, "batched mixed")
+    v_f, ms_f, _, worst_f, conv_f = timed(single, prm_fp64, max(1, args.steps // 2), "single FP64")
+    v_m, _, _, _, _ = timed(single, prm_mixed, max(1, args.steps // 2), "single mixed")
+    v_bf, _, _, _, _ = timed(batched, prm_fp64, max(1, args.steps // 2), "batched FP64")
+    stats, inst_steps = {}, 0
+    if not args.no_kernel_timing:
+        ctx.set_kernel_timing(True)
+        ctx.reset_kernel_stats()
+        inst_steps = max(1, min(args.steps, 3))
+        for _ in range(inst_steps):
+            batched(prm_mixed)
+        st.synchronize()
+        stats = ctx.kernel_stats()
+        ctx.set_kernel_timing(False)
+    roofline = kernel_roofline(stats, inst_steps, "c4")
+    ctx.close()
+    out = {
+        "metric": f"batched pressure solves/sec ({C4_GRIDS} uneven grids to 1e-6 rel-res)",
+        "value": round(v_b, 3), "unit": "solves/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(ms_b, 3), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32+f64", "data": "synthetic",
+        "config": {"workload": f"{C4_GRIDS} sphere-drop cubes, sides from {list(C4_SIDES)} (seed 3), mgpcg mixed "
+                               f"to {residual:g} relative residual (inf norm), one batched call per step",
+                   "grids": C4_GRIDS, "cells": int(cells), "sides": sorted(int(sc.w) for sc in scenes),
+                   "worst_reresid_inf": worst_b, "all_converged": conv_b,
+                   "cell_updates_per_s": round(v_b / C4_GRIDS * cells, 1)},
+        "single_fp64_solves_per_s": round(v_f, 3), "single_fp64_worst_reresid_inf": worst_f,
+        "single_fp64_all_converged": conv_f, "single_mixed_solves_per_s": round(v_m, 3),
+        "batched_fp64_solves_per_s": round(v_bf, 3), "speedup_vs_single_fp64": round(v_b / v_f, 3),
+        "latency_ms": {"p50": round(float(np.percentile(lat_b, 50)), 3),
+                       "p99": round(float(np.percentile(lat_b, 99)), 3)},
+        "roofline": roofline, "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--config", default="c3", choices=sorted(CONFIGS))
+    ap.add_argument("--config", default="c3", choices=sorted(CONFIGS) + ["c4"])
     ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64"])
     ap.add_argument("--solver", default="mgpcg", choices=["mgpcg", "cg"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -124,6 +264,10 @@ def main():
         print(json.dumps({"error": "this benchmark runs on one GPU only: the projection of one scene is a "
                                    "single-GPU kernel pipeline and no multi-GPU path is built; run with --gpus 1"}))
         sys.exit(2)
+
+    if args.config == "c4":
+        bench_batch(args)
+        return
 
     import shiokaze_amd as S
     from shiokaze_amd.devmem import DeviceArray, Stream, device_synchronize
@@ -187,32 +331,7 @@ def main():
         log(f"[bench] instrumented pass: {inst_steps} steps, {(time.perf_counter() - t1) / inst_steps * 1e3:.2f} ms/step")
         stats = ctx.kernel_stats()
         ctx.set_kernel_timing(False)
-    roofline = None
-    if stats:
-        kern = {k: v for k, v in stats.items() if v["launches"] > 0 and v["total_ms"] > 0}
-        # dominant kernel = the HBM-bound kernel moving the most algorithmic bytes (the event-timed
-        # pass adds ~2 us per launch, which overstates the thousands of latency-bound coarse launches)
-        top = max(kern.items(), key=lambda kv: kv[1]["bytes"])
-        name, st = top
-        avg_ms = st["total_ms"] / st["launches"]
-        bytes_per_launch = st["bytes"] / st["launches"]
-        achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
-        roofline = {"bound": "hbm", "kernel": name, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                    "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                    "avg_launch_us": round(avg_ms * 1e3, 2), "algorithmic_bytes_per_launch": int(bytes_per_launch)}
-        roofline["time_share"] = round(st["total_ms"] / sum(v["total_ms"] for v in kern.values()), 4)
-        tr = measured_traffic(name, args.config)
-        if tr is not None:
-            roofline["traffic"] = tr["bytes"]
-            roofline["traffic_source"] = tr["source"]
-        total_ms = sum(v["total_ms"] for v in kern.values())
-        total_bytes = sum(v["bytes"] for v in kern.values())
-        roofline["all_kernels_gbs"] = round(total_bytes / (total_ms * 1e-3) / 1e9, 1)
-        log("[bench] kernel time share:")
-        for k, v in sorted(kern.items(), key=lambda kv: -kv[1]["total_ms"])[:14]:
-            gbs = v["bytes"] / (v["total_ms"] * 1e-3) / 1e9 if v["total_ms"] else 0
-            log(f"    {k:18s} {v['total_ms'] / inst_steps:8.3f} ms/step  {v['launches'] / inst_steps:7.1f} launches/step"
-                f"  {gbs:8.1f} GB/s")
+    roofline = kernel_roofline(stats, inst_steps, args.config)
 
     value = args.steps / elapsed
     out = {
