@@ -234,12 +234,60 @@ def bench_batch(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_serve(args):
+    """C5: Poisson-arrival stream of time-stepped projection requests against the server
+    (continuous batching, HBM-budget admission, warm start from resident pressure)."""
+    import shiokaze_amd as S
+    from shiokaze_amd import loadgen
+
+    specs = loadgen.make_requests(args.c5_requests, rate=200.0, seed=4)
+    steps = args.c5_steps
+    prm = S.default_params(residual=1e-6, warm_start=1)
+    cells = [s.w * s.h * s.d for s in specs]
+    log(f"[bench] c5: {len(specs)} requests x {steps} steps, mean {np.mean(cells) / 1e6:.2f} M cells, "
+        f"arrivals over {specs[-1].t_arrival:.2f}s")
+    # warm-up: a few requests through a separate server (code paths, allocations, graphs)
+    with S.Server(0, executors=args.c5_executors) as srv:
+        loadgen.run_served(srv, loadgen.make_requests(4, rate=1e6, seed=99), steps=2, params=prm)
+    with S.Server(0, executors=args.c5_executors, max_batch_grids=64, hbm_budget_bytes=200 * 10 ** 9) as srv:
+        res = loadgen.run_served(srv, specs, steps=steps, params=prm, log=log)
+        st = srv.stats()
+    ok = sum(1 for x in res["statuses"] if x == 0)
+    value = res["solves"] / res["elapsed_s"]
+    rl, sl = res["request_latency_ms"], res["solve_latency_ms"]
+    log(f"[bench] c5: {res['solves']} solves ({ok} ok) in {res['elapsed_s']:.2f}s -> {value:.1f} solves/s; "
+        f"request p50 {np.percentile(rl, 50):.0f} ms p99 {np.percentile(rl, 99):.0f} ms; "
+        f"solve p50 {np.percentile(sl, 50):.1f} ms p99 {np.percentile(sl, 99):.1f} ms; server {st}")
+    out = {
+        "metric": "served pressure solves/sec (Poisson 200 req/s, 20-step requests, 1e-6 rel-res)",
+        "value": round(value, 3), "unit": "solves/s", "n_gpus": 1, "steps": res["solves"], "warmup": 8,
+        "ms_per_step": round(res["elapsed_s"] / max(res["solves"], 1) * 1e3, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic",
+        "config": {"workload": f"{len(specs)} requests (Poisson arrivals, 200/s, seed 4) x {steps} steps, sides "
+                               f"U[48,192] aspect <= 2, translated sphere-drop, warm start, mgpcg mixed to 1e-6",
+                   "requests": len(specs), "steps_per_request": steps, "executors": args.c5_executors,
+                   "hbm_budget_bytes": 200 * 10 ** 9, "mean_cells": float(np.mean(cells)),
+                   "ok_solves": ok, "cell_updates_per_s": round(sum(cells) * steps / res["elapsed_s"], 1),
+                   "median_iterations": float(np.median(res["iterations"])) if res["iterations"] else None,
+                   "server": st},
+        "latency_ms": {"p50": round(float(np.percentile(rl, 50)), 3), "p99": round(float(np.percentile(rl, 99)), 3),
+                       "what": "request (arrival to last step done)",
+                       "solve_p50": round(float(np.percentile(sl, 50)), 3),
+                       "solve_p99": round(float(np.percentile(sl, 99)), 3)},
+        "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--config", default="c3", choices=sorted(CONFIGS) + ["c4"])
+    ap.add_argument("--config", default="c3", choices=sorted(CONFIGS) + ["c4", "c5"])
+    ap.add_argument("--c5-requests", type=int, default=200, help="c5: requests in the sample")
+    ap.add_argument("--c5-steps", type=int, default=20, help="c5: steps per request")
+    ap.add_argument("--c5-executors", type=int, default=2, help="c5: server executors")
     ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64"])
     ap.add_argument("--solver", default="mgpcg", choices=["mgpcg", "cg"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -267,6 +315,9 @@ def main():
 
     if args.config == "c4":
         bench_batch(args)
+        return
+    if args.config == "c5":
+        bench_serve(args)
         return
 
     import shiokaze_amd as S

@@ -31,11 +31,12 @@
 extern "C" {
 #endif
 
-#define SHKZ_ABI_VERSION 1
+#define SHKZ_ABI_VERSION 2
 
 /* ---------------------------------------------------------------- enums */
 enum shkz_status {
     SHKZ_OK = 0,
+    SHKZ_PENDING = 1,        /* shkz_poll / shkz_wait: the request is queued or running */
     SHKZ_ERR_INVALID = -1,   /* bad argument or shape */
     SHKZ_ERR_DEVICE = -2,    /* HIP runtime error */
     SHKZ_ERR_NOMEM = -3,     /* device allocation failed */
@@ -126,7 +127,7 @@ typedef struct shkz_result {
     int64_t n_unknowns;           /* liquid unknowns (the reference's matrix size) */
     int64_t n_bricks;             /* active 8^3 bricks on the finest level */
     int32_t n_levels;             /* multigrid levels */
-    int32_t pad0;
+    int32_t batch_size;           /* grids in the batch this projection ran in */
     double rhs_correct;           /* volume-correction term added to every row */
     /* stage times in milliseconds, named after the reference's scoped_timer records
        (macpressuresolver3.cpp:82,201,237,269,271) */
@@ -135,6 +136,10 @@ typedef struct shkz_result {
     double t_linsolve;
     double t_update_velocity;
     double t_projection;
+    /* server requests only (shkz_submit): milliseconds from submission to the start of its batch,
+       and to its completion */
+    double t_queue;
+    double t_latency;
 } shkz_result;
 
 /* ---------------------------------------------------------------- context */
@@ -191,6 +196,54 @@ int shkz_release_scene(shkz_ctx *ctx, uint64_t scene_id);
 /* Device bytes currently held by the context. */
 uint64_t shkz_device_bytes(const shkz_ctx *ctx);
 
+/* ------------------------------------------------------- server (continuous batching)
+ * No counterpart in the reference (its driver projects one grid per time step on one thread,
+ * ui.cpp:703-710).  A server owns `executors` workspaces, each with its own HIP stream; every
+ * executor that is free takes the next batch of queued requests, so requests that arrive while
+ * others run join the next batch and batches overlap on the device.  Admission: the estimated
+ * working set of the batches in flight plus the resident scene state stays within
+ * hbm_budget_bytes; memory is granted in arrival order; each scene's requests run in order and
+ * never two at once; only requests with equal shkz_params share a batch.  Every request's result
+ * equals that of projecting it alone with shkz_project_device.
+ * Requests carry device pointers (as shkz_project_device): inputs must be complete when submitted
+ * and stay valid and unmodified until the request completes; outputs are complete when
+ * shkz_poll / shkz_wait report it done.  All server calls are thread-safe. */
+typedef struct shkz_server shkz_server;
+
+typedef struct shkz_server_config {
+    int32_t device;
+    int32_t executors;            /* batches in flight at once (0 = 2) */
+    int32_t max_batch_grids;      /* 0 = 64 */
+    int32_t reserved0;
+    int64_t max_batch_cells;      /* box cells per batch (0 = 2^26) */
+    uint64_t hbm_budget_bytes;    /* workspaces + resident scene state (0 = 90% of the device) */
+    int32_t reserved[8];
+} shkz_server_config;
+
+typedef struct shkz_server_stats {
+    uint64_t submitted, completed, rejected;
+    uint64_t batches, batched_requests;
+    int32_t max_batch, max_in_flight_batches;
+    uint64_t max_in_flight_bytes;  /* largest estimated working set in flight */
+    uint64_t resident_bytes;       /* scene state held now */
+    uint64_t device_bytes;         /* workspaces + scene state held now */
+} shkz_server_stats;
+
+void shkz_server_default_config(shkz_server_config *cfg);
+int shkz_server_create(const shkz_server_config *cfg, shkz_server **out);
+/* Completes every queued request, then frees everything. */
+void shkz_server_destroy(shkz_server *srv);
+/* Queues one projection (params NULL = defaults); *ticket identifies it. */
+int shkz_submit(shkz_server *srv, const shkz_grid *grid, const shkz_params *params, uint64_t *ticket);
+/* SHKZ_PENDING while queued or running; SHKZ_OK once done (result filled, ticket retired); else the
+ * request's error (result filled, ticket retired) or SHKZ_ERR_NOT_FOUND for an unknown ticket. */
+int shkz_poll(shkz_server *srv, uint64_t ticket, shkz_result *result);
+/* shkz_poll after waiting up to timeout_ms (< 0: no limit) for the request. */
+int shkz_wait(shkz_server *srv, uint64_t ticket, double timeout_ms, shkz_result *result);
+/* Drops a scene's resident state; SHKZ_ERR_INVALID while a request of the scene is queued or running. */
+int shkz_server_release_scene(shkz_server *srv, uint64_t scene_id);
+int shkz_server_get_stats(shkz_server *srv, shkz_server_stats *stats);
+
 /* ------------------------------------------------------- kernel timing (measurement) */
 typedef struct shkz_kernel_stat {
     char name[48];
@@ -234,6 +287,14 @@ typedef struct shkz_scene_desc {
  * *has_solid tells whether the scene has a solid), u, v, w (faces, may be NULL). */
 int shkz_scene_fill(const shkz_scene_desc *desc, void *fluid, float *solid, int32_t *has_solid,
                     float *u, float *v, float *w);
+
+/* The same scenes generated on the device into device buffers, on `stream` (synthetic inputs
+ * for serving workloads, where host generation would dominate).  The level sets equal
+ * shkz_scene_fill's; velocities are i.i.d. N(0,1) from a counter-based generator (splitmix64 of
+ * seed, array and face index, Box-Muller), so they differ from shkz_scene_fill's mt19937_64
+ * stream but are reproducible for a seed. */
+int shkz_scene_fill_device(const shkz_scene_desc *desc, void *fluid, float *solid, int32_t *has_solid, float *u,
+                           float *v, float *w, void *stream);
 
 #ifdef __cplusplus
 }

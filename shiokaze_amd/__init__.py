@@ -19,6 +19,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "lib", "libshkz_mi355x.so")
 
+ABI_VERSION = 2  # the layout of the structs below
 F32, F64 = 0, 1
 SOLVER_MGPCG, SOLVER_CG = 0, 1
 PREC_MIXED, PREC_FP64 = 0, 1
@@ -67,13 +68,34 @@ class Result(ctypes.Structure):
         ("outer_iterations", ctypes.c_int32),
         ("reresid", ctypes.c_double), ("reresid_inf", ctypes.c_double), ("reresid_l2", ctypes.c_double),
         ("n_unknowns", ctypes.c_int64), ("n_bricks", ctypes.c_int64), ("n_levels", ctypes.c_int32),
-        ("pad0", ctypes.c_int32), ("rhs_correct", ctypes.c_double),
+        ("batch_size", ctypes.c_int32), ("rhs_correct", ctypes.c_double),
         ("t_solid_fluid_fractions", ctypes.c_double), ("t_build_highres_linsystem", ctypes.c_double),
         ("t_linsolve", ctypes.c_double), ("t_update_velocity", ctypes.c_double), ("t_projection", ctypes.c_double),
+        ("t_queue", ctypes.c_double), ("t_latency", ctypes.c_double),
     ]
 
     def as_dict(self) -> dict:
-        return {name: getattr(self, name) for name, _ in self._fields_ if name != "pad0"}
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class ServerConfig(ctypes.Structure):
+    _fields_ = [
+        ("device", ctypes.c_int32), ("executors", ctypes.c_int32), ("max_batch_grids", ctypes.c_int32),
+        ("reserved0", ctypes.c_int32), ("max_batch_cells", ctypes.c_int64), ("hbm_budget_bytes", ctypes.c_uint64),
+        ("reserved", ctypes.c_int32 * 8),
+    ]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [
+        ("submitted", ctypes.c_uint64), ("completed", ctypes.c_uint64), ("rejected", ctypes.c_uint64),
+        ("batches", ctypes.c_uint64), ("batched_requests", ctypes.c_uint64), ("max_batch", ctypes.c_int32),
+        ("max_in_flight_batches", ctypes.c_int32), ("max_in_flight_bytes", ctypes.c_uint64),
+        ("resident_bytes", ctypes.c_uint64), ("device_bytes", ctypes.c_uint64),
+    ]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
 
 
 class KernelStat(ctypes.Structure):
@@ -96,6 +118,8 @@ EXPORTS = [
     "shkz_project", "shkz_project_device", "shkz_release_scene", "shkz_device_bytes",
     "shkz_set_kernel_timing", "shkz_get_kernel_stats", "shkz_reset_kernel_stats", "shkz_scene_fill",
     "shkz_export_system", "shkz_project_batch", "shkz_project_batch_device",
+    "shkz_server_default_config", "shkz_server_create", "shkz_server_destroy", "shkz_submit", "shkz_poll",
+    "shkz_wait", "shkz_server_release_scene", "shkz_server_get_stats", "shkz_scene_fill_device",
 ]
 
 _lib = None
@@ -111,6 +135,8 @@ def lib():
                            "(make -C shiokaze_amd/csrc); there is no fallback implementation")
     L = ctypes.CDLL(LIB_PATH)
     L.shkz_abi_version.restype = ctypes.c_int
+    if L.shkz_abi_version() != ABI_VERSION:
+        raise RuntimeError(f"{LIB_PATH} has ABI {L.shkz_abi_version()}, these bindings expect {ABI_VERSION}: rebuild it")
     L.shkz_last_error.restype = ctypes.c_char_p
     L.shkz_default_params.argtypes = [ctypes.POINTER(Params)]
     L.shkz_create.argtypes = [ctypes.c_int, ctypes.c_uint64, ctypes.POINTER(_vp)]
@@ -129,6 +155,16 @@ def lib():
                                      ctypes.POINTER(Result)]
     L.shkz_project_batch_device.argtypes = [_vp, ctypes.POINTER(Grid), ctypes.c_int, ctypes.POINTER(Params),
                                             ctypes.POINTER(Result), _vp]
+    L.shkz_server_default_config.argtypes = [ctypes.POINTER(ServerConfig)]
+    L.shkz_server_create.argtypes = [ctypes.POINTER(ServerConfig), ctypes.POINTER(_vp)]
+    L.shkz_server_destroy.argtypes = [_vp]
+    L.shkz_submit.argtypes = [_vp, ctypes.POINTER(Grid), ctypes.POINTER(Params), ctypes.POINTER(ctypes.c_uint64)]
+    L.shkz_poll.argtypes = [_vp, ctypes.c_uint64, ctypes.POINTER(Result)]
+    L.shkz_wait.argtypes = [_vp, ctypes.c_uint64, ctypes.c_double, ctypes.POINTER(Result)]
+    L.shkz_server_release_scene.argtypes = [_vp, ctypes.c_uint64]
+    L.shkz_server_get_stats.argtypes = [_vp, ctypes.POINTER(ServerStats)]
+    L.shkz_scene_fill_device.argtypes = [ctypes.POINTER(SceneDesc), _vp, _vp, ctypes.POINTER(ctypes.c_int32), _vp,
+                                         _vp, _vp, _vp]
     _lib = L
     return L
 
@@ -205,6 +241,99 @@ def make_scene(kind: int, w: int, h: int | None = None, d: int | None = None, se
     if rc != 0:
         raise ShkzError(rc, "shkz_scene_fill failed")
     return Scene(w, h, d, dt, fluid, solid if has.value else None, u, v, wv)
+
+
+def scene_fill_device(kind: int, w: int, h: int, d: int, fluid, u, v, wv, seed: int = 0, dt: float = 1.0 / 120.0,
+                      gravity: float = -9.8, fluid_dtype: int = F64, shift=(0.0, 0.0, 0.0), velocity_mode: int = 0,
+                      velocity_seed: int = 0, solid=None, stream=None) -> bool:
+    """Synthesises a scene into device buffers (objects with data_ptr() or ints) on `stream` with
+    shkz_scene_fill_device; returns whether the scene has a solid."""
+    def dp(x):
+        if x is None:
+            return None
+        return x.data_ptr() if hasattr(x, "data_ptr") else int(x)
+    desc = SceneDesc(kind=kind, w=w, h=h, d=d, seed=seed, dt=dt, gravity=gravity, fluid_dtype=fluid_dtype,
+                     velocity_mode=velocity_mode, velocity_seed=velocity_seed)
+    for a in range(3):
+        desc.shift[a] = shift[a]
+    has = ctypes.c_int32(0)
+    rc = lib().shkz_scene_fill_device(ctypes.byref(desc), dp(fluid), dp(solid), ctypes.byref(has), dp(u), dp(v),
+                                      dp(wv), None if stream is None else int(stream))
+    if rc != 0:
+        raise ShkzError(rc, "shkz_scene_fill_device failed")
+    return bool(has.value)
+
+
+PENDING = 1
+
+
+class Server:
+    """A projection server (shkz_server): continuous batching of submitted device-resident requests."""
+
+    def __init__(self, device: int = 0, executors: int = 2, max_batch_grids: int = 64, max_batch_cells: int = 0,
+                 hbm_budget_bytes: int = 0):
+        self._L = lib()
+        cfg = ServerConfig()
+        self._L.shkz_server_default_config(ctypes.byref(cfg))
+        cfg.device = device
+        cfg.executors = executors
+        cfg.max_batch_grids = max_batch_grids
+        if max_batch_cells:
+            cfg.max_batch_cells = max_batch_cells
+        cfg.hbm_budget_bytes = hbm_budget_bytes
+        h = _vp()
+        rc = self._L.shkz_server_create(ctypes.byref(cfg), ctypes.byref(h))
+        if rc != 0:
+            raise ShkzError(rc, last_error())
+        self.handle = h
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self._L.shkz_server_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def submit(self, grid: "Grid", params: "Params | None" = None) -> int:
+        t = ctypes.c_uint64(0)
+        rc = self._L.shkz_submit(self.handle, ctypes.byref(grid), None if params is None else ctypes.byref(params),
+                                 ctypes.byref(t))
+        if rc != 0:
+            raise ShkzError(rc, last_error())
+        return int(t.value)
+
+    def poll(self, ticket: int):
+        """(status, Result): status PENDING while queued or running."""
+        r = Result()
+        rc = self._L.shkz_poll(self.handle, ticket, ctypes.byref(r))
+        return rc, r
+
+    def wait(self, ticket: int, timeout_ms: float = -1.0):
+        r = Result()
+        rc = self._L.shkz_wait(self.handle, ticket, timeout_ms, ctypes.byref(r))
+        return rc, r
+
+    def release_scene(self, scene_id: int):
+        rc = self._L.shkz_server_release_scene(self.handle, scene_id)
+        if rc != 0:
+            raise ShkzError(rc, last_error())
+
+    def stats(self) -> dict:
+        st = ServerStats()
+        rc = self._L.shkz_server_get_stats(self.handle, ctypes.byref(st))
+        if rc != 0:
+            raise ShkzError(rc, last_error())
+        return st.as_dict()
 
 
 class Context:

@@ -16,9 +16,12 @@
 // the last batch, per-scene resident state, the replayable iteration graphs and the kernel timers.
 #include "../../include/shkz_mi355x.h"
 #include "shkz_kernels.hpp"
+#include "shkz_serve.hpp"
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
+#include <functional>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -66,11 +69,12 @@ struct DevBuf {
 };
 
 // Named device buffers, grown with 1/8 headroom in 2 MiB steps and kept across calls, under a
-// byte budget.
+// byte budget shared with what `extra` reports (resident scene state, other workspaces).
 struct Arena {
     std::unordered_map<std::string, DevBuf> bufs;
-    size_t total = 0;
-    size_t budget = 0;
+    std::atomic<uint64_t> total{0};  // read by other threads of a server
+    uint64_t budget = 0;
+    std::function<uint64_t()> extra;
 
     int get(const std::string &name, size_t bytes, void **out) {
         DevBuf &b = bufs[name];
@@ -78,9 +82,11 @@ struct Arena {
         if (b.cap < bytes) {
             size_t want = bytes + bytes / 8;
             want = (want + ((size_t)2 << 20) - 1) & ~(((size_t)2 << 20) - 1);
-            if (total - b.cap + want > budget)
-                return fail(SHKZ_ERR_BUDGET, "device budget exceeded: %s needs %zu bytes (held %zu of %zu)", name.c_str(),
-                            want, total, budget);
+            const uint64_t other = extra ? extra() : 0;
+            if (total - b.cap + want + other > budget)
+                return fail(SHKZ_ERR_BUDGET, "device budget exceeded: %s needs %zu bytes (held %llu + %llu of %llu)",
+                            name.c_str(), want, (unsigned long long)total.load(), (unsigned long long)other,
+                            (unsigned long long)budget);
             if (b.ptr) {
                 SHKZ_HIP(hipFree(b.ptr));
                 total -= b.cap;
@@ -247,10 +253,70 @@ struct KernelTotals {
 };
 
 // ------------------------------------------------------------------------ scenes
+// State a scene keeps on the device between projections: the previous pressure (warm start,
+// macpressuresolver3.cpp:239-242, kept cell-indexed) and the volume-correction integrator (:208).
 struct SceneState {
     int w = 0, h = 0, d = 0;
     bool has_pressure = false;
-    double y_prev = 0.0;  // volume-correction integrator (macpressuresolver3.cpp:208)
+    double y_prev = 0.0;
+    double *p = nullptr;  // device, w*h*d
+    size_t cap = 0;       // bytes held at p
+};
+
+// Scene states by id.  A context owns one; the contexts of a server share the server's (a scene is
+// used by one projection at a time: the server never runs two requests of a scene together).
+struct SceneStore {
+    std::mutex mu;
+    std::unordered_map<uint64_t, SceneState> map;  // node-based: element addresses are stable
+    std::atomic<uint64_t> bytes{0};
+    std::function<bool(uint64_t)> may_grow;  // budget test for `want` more bytes
+
+    // The scene's state with a pressure buffer for a w*h*d grid (reset when the shape changed).
+    int acquire(uint64_t id, int w, int h, int d, SceneState **out) {
+        std::lock_guard<std::mutex> g(mu);
+        SceneState &S = map[id];
+        if (S.w != w || S.h != h || S.d != d) {
+            S.w = w;
+            S.h = h;
+            S.d = d;
+            S.has_pressure = false;
+        }
+        const size_t want = (size_t)w * h * d * sizeof(double);
+        if (S.cap < want) {
+            if (may_grow && !may_grow(want - S.cap))
+                return fail(SHKZ_ERR_BUDGET, "device budget exceeded by the state of scene %llu (%zu bytes)",
+                            (unsigned long long)id, want);
+            if (S.p) {
+                SHKZ_HIP(hipFree(S.p));
+                bytes -= S.cap;
+                S.p = nullptr;
+                S.cap = 0;
+            }
+            SHKZ_HIP(hipMalloc(&S.p, want));
+            S.cap = want;
+            bytes += want;
+        }
+        *out = &S;
+        return SHKZ_OK;
+    }
+    int release(uint64_t id) {
+        std::lock_guard<std::mutex> g(mu);
+        auto it = map.find(id);
+        if (it == map.end()) return fail(SHKZ_ERR_NOT_FOUND, "scene %llu not found", (unsigned long long)id);
+        if (it->second.p) {
+            (void)hipFree(it->second.p);
+            bytes -= it->second.cap;
+        }
+        map.erase(it);
+        return SHKZ_OK;
+    }
+    void clear() {
+        std::lock_guard<std::mutex> g(mu);
+        for (auto &kv : map)
+            if (kv.second.p) (void)hipFree(kv.second.p);
+        map.clear();
+        bytes = 0;
+    }
 };
 
 }  // namespace shkz
@@ -261,7 +327,8 @@ struct shkz_ctx {
     int device = 0;
     Arena arena;
     std::mutex mu;  // one projection call at a time per context
-    std::unordered_map<uint64_t, SceneState> scenes;
+    SceneStore own_scenes;
+    SceneStore *scenes = &own_scenes;  // a server points its contexts at its shared store
     double y_prev = 0.0;  // stateless calls share the module-level integrator, as m_y_prev does
     // hierarchy of the current batch
     std::vector<LevelHost> levels;
@@ -657,7 +724,6 @@ struct GridJob {
     uint8_t *cflag = nullptr, *bflag = nullptr;
     GridLevelHost l0;
     SceneState *scene = nullptr;
-    double *scene_p = nullptr;
     double rhs_correct = 0.0;
 };
 
@@ -678,7 +744,7 @@ static int project_stages(shkz_ctx *C, const shkz_grid *grids, int ngrids, const
 static int project_impl(shkz_ctx *C, const shkz_grid *grids, int ngrids, const shkz_params *prm, shkz_result *res,
                         hipStream_t caller, const ExportOut *exp = nullptr) {
     int rc = project_stages(C, grids, ngrids, prm, res, caller, exp);
-    if (C->work) {
+    if (C->work && caller != C->work) {
         if (hipEventRecord(C->ev_out, C->work) != hipSuccess || hipStreamWaitEvent(caller, C->ev_out, 0) != hipSuccess) {
             (void)hipStreamSynchronize(C->work);
             if (rc == SHKZ_OK) rc = fail(SHKZ_ERR_DEVICE, "ordering the caller's stream after the projection failed");
@@ -707,9 +773,11 @@ static int project_stages(shkz_ctx *C, const shkz_grid *grids, int ngrids, const
                 return fail(SHKZ_ERR_INVALID, "scene %llu appears twice in one batch", (unsigned long long)grids[a].scene_id);
     SHKZ_HIP(hipSetDevice(C->device));
     // all work runs on the context's stream, ordered after what the caller queued on its stream
-    SHKZ_HIP(hipEventRecord(C->ev_in, s));
-    SHKZ_HIP(hipStreamWaitEvent(C->work, C->ev_in, 0));
-    s = C->work;
+    if (s != C->work) {
+        SHKZ_HIP(hipEventRecord(C->ev_in, s));
+        SHKZ_HIP(hipStreamWaitEvent(C->work, C->ev_in, 0));
+        s = C->work;
+    }
     Launcher K{C, s};
     Arena &AR = C->arena;
 
@@ -727,6 +795,25 @@ static int project_stages(shkz_ctx *C, const shkz_grid *grids, int ngrids, const
 
     // ---------------------------------------------------------------- per grid: fractions + labelling
     std::vector<GridJob> jobs(ngrids);
+    // per-grid dense flags live in three combined buffers (256-byte aligned slices)
+    size_t cflag_total = 0, bflag_total = 0;
+    std::vector<size_t> cflag_off(ngrids), bflag_off(ngrids);
+    for (int g = 0; g < ngrids; ++g) {
+        const size_t M = (size_t)grids[g].w * grids[g].h * grids[g].d;
+        const size_t npos = (size_t)((grids[g].w + BRICK - 1) / BRICK) * ((grids[g].h + BRICK - 1) / BRICK) *
+                            ((grids[g].d + BRICK - 1) / BRICK);
+        cflag_off[g] = cflag_total;
+        bflag_off[g] = bflag_total;
+        cflag_total += (M + 255) & ~(size_t)255;
+        bflag_total += (npos + 255) & ~(size_t)255;
+    }
+    uint8_t *cflag_all = nullptr, *bflag_all = nullptr;
+    int *exist_all = nullptr;
+    SHKZ_TRY(arena_get(AR, "cflag", cflag_total, &cflag_all));
+    SHKZ_TRY(arena_get(AR, "bflag", bflag_total, &bflag_all));
+    SHKZ_TRY(arena_get(AR, "exist", (size_t)4 * ngrids, &exist_all));
+    K.go("reset", -1, 0, 16.0 * ngrids, dim3((unsigned)((4 * ngrids + 255) / 256)), dim3(256), k_reset_ints, exist_all,
+         4 * ngrids);
     for (int g = 0; g < ngrids; ++g) {
         GridJob &J = jobs[g];
         const shkz_grid *grid = &grids[g];
@@ -753,10 +840,8 @@ static int project_stages(shkz_ctx *C, const shkz_grid *grids, int ngrids, const
         G.active[0] = grid->active_u;
         G.active[1] = grid->active_v;
         G.active[2] = grid->active_w;
-        const std::string sfx = std::to_string(g);
-        SHKZ_TRY(arena_get(AR, "exist" + sfx, 4, &J.exist));
+        J.exist = exist_all + 4 * g;
         G.exist = J.exist;
-        K.go("reset", -1, 0, 8, dim3(1), dim3(64), k_reset_ints, J.exist, 4);
         const unsigned nbk = (unsigned)std::min<size_t>((J.M + 255) / 256, 4096);
         K.go("any_negative", -1, 0, (double)J.M * (G.phi64 ? 8 : 4), dim3(nbk), dim3(256), k_any_negative, grid->fluid,
              (int)G.phi64, J.M, J.exist + 0);
@@ -779,8 +864,8 @@ static int project_stages(shkz_ctx *C, const shkz_grid *grids, int ngrids, const
         }
         grid_level_dims(J.l0, w, h, d);
         const int npos = J.l0.nbx * J.l0.nby * J.l0.nbz;
-        SHKZ_TRY(arena_get(AR, "cflag" + sfx, J.M, &J.cflag));
-        SHKZ_TRY(arena_get(AR, "bflag" + sfx, (size_t)npos, &J.bflag));
+        J.cflag = cflag_all + cflag_off[g];
+        J.bflag = bflag_all + bflag_off[g];
         K.go("flags", -1, 0, (double)J.M * ((G.phi64 ? 8 : 4) + 1 + (grid->solid ? 4 : 0)), dim3((unsigned)npos),
              dim3(BLOCK), k_flags, G, J.l0.nbx, J.l0.nby, J.l0.nbz, J.cflag, J.bflag);
         J.l0.active.resize(npos);
@@ -941,16 +1026,7 @@ static int project_stages(shkz_ctx *C, const shkz_grid *grids, int ngrids, const
     for (int g = 0; g < ngrids; ++g) {
         GridJob &J = jobs[g];
         const shkz_grid *grid = J.in;
-        if (grid->scene_id) {
-            J.scene = &C->scenes[grid->scene_id];
-            if (J.scene->w != grid->w || J.scene->h != grid->h || J.scene->d != grid->d) {
-                J.scene->w = grid->w;
-                J.scene->h = grid->h;
-                J.scene->d = grid->d;
-                J.scene->has_pressure = false;
-            }
-            SHKZ_TRY(arena_get(AR, "scene" + std::to_string(grid->scene_id), J.M, &J.scene_p));
-        }
+        if (grid->scene_id) SHKZ_TRY(C->scenes->acquire(grid->scene_id, grid->w, grid->h, grid->d, &J.scene));
         if (prm->gain && prm->target_volume) {
             double &yprev = J.scene ? J.scene->y_prev : C->y_prev;
             const double xv = (prm->current_volume - prm->target_volume) / prm->target_volume;
@@ -961,7 +1037,7 @@ static int project_stages(shkz_ctx *C, const shkz_grid *grids, int ngrids, const
             J.rhs_correct = -(kp * xv + ki * y) / (xv + 1.0);
         }
         res[g].rhs_correct = J.rhs_correct;
-        const double *warm = (prm->warm_start && J.scene && J.scene->has_pressure) ? J.scene_p : nullptr;
+        const double *warm = (prm->warm_start && J.scene && J.scene->has_pressure) ? J.scene->p : nullptr;
         const GridLevel &GL = F0.gl[g];
         const int nbg = GL.brick_end - GL.brick_start;
         K.go("operator", -1, 0,
@@ -1152,7 +1228,7 @@ static int project_stages(shkz_ctx *C, const shkz_grid *grids, int ngrids, const
                  (const double *)P.x64, grid->pressure, grid->pressure64);
         if (J.scene)
             K.go("scatter_pressure", -1, 0, (double)J.M * 8, dim3((unsigned)((J.M + 255) / 256)), dim3(256),
-                 k_scatter_pressure, J.G, F0.desc, GL, (const double *)P.x64, (float *)nullptr, J.scene_p);
+                 k_scatter_pressure, J.G, F0.desc, GL, (const double *)P.x64, (float *)nullptr, J.scene->p);
         const size_t ne = (size_t)(grid->w + 1) * (grid->h + 1) * (grid->d + 1);
         K.go("velocity_update", -1, 0, (double)J.M * (24 + (J.G.phi64 ? 8 : 4) + (grid->solid ? 4 : 0)),
              dim3((unsigned)((ne + 255) / 256)), dim3(256), k_velocity_update, J.G, F0.desc, GL, (const double *)P.x64);
@@ -1283,6 +1359,78 @@ static int project_host(shkz_ctx *C, const shkz_grid *grids, int ngrids, const s
 
 }  // namespace shkz
 
+// ============================================================================ server
+namespace shkz {
+
+// One submitted request: its grid (device pointers), parameters and, once done, its result.
+struct ServerJob {
+    shkz_grid grid;
+    shkz_params params;
+    shkz_result result;
+    std::string error;
+};
+
+// Upper bound of the device working set of one grid in a batch: every brick of the box stored,
+// level-0 operator and vectors of the solve, coarse levels (at most 1/7 of level 0), dense flags,
+// and the scene's resident pressure.
+static uint64_t estimate_bytes(const shkz_grid &g, const shkz_params &p) {
+    const uint64_t M = (uint64_t)g.w * g.h * g.d;
+    const uint64_t nb = (uint64_t)((g.w + BRICK - 1) / BRICK) * ((g.h + BRICK - 1) / BRICK) * ((g.d + BRICK - 1) / BRICK);
+    const bool mixed = p.solver == SHKZ_SOLVER_MGPCG && p.precision == SHKZ_PREC_MIXED;
+    // FP64: dir,gx,gy,gz,b,x,r (56) + p,q,z,tmp (32) + res,idiag (16) + coarse 9 arrays x 8 / 7
+    // mixed: FP64 operator, b, x, r (56) + FP32 operator (16) + r,x,p,q,z,tmp (24) + res,idiag (8)
+    //        + coarse 9 arrays x 4 / 7
+    const uint64_t per_cell = mixed ? 56 + 16 + 24 + 8 + 6 : 56 + 32 + 16 + 11;
+    return nb * BRICK_CELLS * per_cell + nb * 64 + 2 * M + (g.scene_id ? M * 8 : 0) + ((uint64_t)1 << 20);
+}
+
+}  // namespace shkz
+
+struct shkz_server {
+    int device = 0;
+    shkz_server_config cfg{};
+    uint64_t budget = 0;
+    SceneStore store;
+    std::vector<shkz_ctx *> exec;
+    std::unique_ptr<ServeCore> core;
+    std::mutex mu;  // tickets and parameter groups
+    uint64_t next_ticket = 1;
+    std::vector<shkz_params> groups;
+
+    uint64_t held_except(int k) const {
+        uint64_t b = store.bytes.load();
+        for (int j = 0; j < (int)exec.size(); ++j)
+            if (j != k) b += exec[j]->arena.total.load();
+        return b;
+    }
+    void run_batch(int k, std::vector<ServeRequest> &batch, std::vector<int> &status) {
+        shkz_ctx *C = exec[k];
+        const int n = (int)batch.size();
+        std::vector<shkz_grid> grids(n);
+        std::vector<shkz_result> res(n);
+        for (int i = 0; i < n; ++i) grids[i] = ((ServerJob *)batch[i].payload)->grid;
+        const shkz_params prm = ((ServerJob *)batch[0].payload)->params;
+        int rc;
+        {
+            std::lock_guard<std::mutex> g(C->mu);
+            rc = hipSetDevice(device) == hipSuccess ? SHKZ_OK : fail(SHKZ_ERR_DEVICE, "hipSetDevice failed");
+            if (rc == SHKZ_OK) rc = project_impl(C, grids.data(), n, &prm, res.data(), C->work);
+            // outputs are complete when the request is reported done
+            if (hipStreamSynchronize(C->work) != hipSuccess && rc == SHKZ_OK)
+                rc = fail(SHKZ_ERR_DEVICE, "server stream synchronisation failed");
+        }
+        const std::string err = rc != SHKZ_OK ? shkz_last_error() : "";
+        for (int i = 0; i < n; ++i) {
+            ServerJob *J = (ServerJob *)batch[i].payload;
+            J->result = res[i];
+            J->result.batch_size = n;
+            if (rc != SHKZ_OK && rc != SHKZ_ERR_NUMERIC) J->result.status = rc;
+            status[i] = J->result.status;
+            if (status[i] != SHKZ_OK) J->error = err;
+        }
+    }
+};
+
 // ============================================================================ C ABI
 extern "C" {
 
@@ -1316,6 +1464,10 @@ int shkz_create(int device, uint64_t hbm_budget_bytes, shkz_ctx **out) {
     shkz_ctx *C = new shkz_ctx();
     C->device = device;
     C->arena.budget = hbm_budget_bytes ? (size_t)hbm_budget_bytes : (size_t)(0.9 * (double)total_b);
+    C->arena.extra = [C] { return C->own_scenes.bytes.load(); };
+    C->own_scenes.may_grow = [C](uint64_t want) {
+        return C->arena.total.load() + C->own_scenes.bytes.load() + want <= C->arena.budget;
+    };
     if (ensure_host_scalars(C, 16) != SHKZ_OK ||
         hipHostMalloc((void **)&C->host_ctl, sizeof(BatchCtl), hipHostMallocDefault) != hipSuccess) {
         if (C->host_scalars) (void)hipHostFree(C->host_scalars);
@@ -1353,6 +1505,7 @@ void shkz_destroy(shkz_ctx *C) {
     if (C->ev_out) (void)hipEventDestroy(C->ev_out);
     if (C->work) (void)hipStreamDestroy(C->work);
     C->arena.clear();
+    C->own_scenes.clear();
     if (C->host_scalars) (void)hipHostFree(C->host_scalars);
     if (C->host_ctl) (void)hipHostFree(C->host_ctl);
     delete C;
@@ -1403,14 +1556,12 @@ int shkz_export_system(shkz_ctx *C, const shkz_grid *grid, const shkz_params *pa
 int shkz_release_scene(shkz_ctx *C, uint64_t scene_id) {
     if (!C) return fail(SHKZ_ERR_INVALID, "NULL context");
     std::lock_guard<std::mutex> lock(C->mu);
-    auto it = C->scenes.find(scene_id);
-    if (it == C->scenes.end()) return fail(SHKZ_ERR_NOT_FOUND, "scene %llu not found", (unsigned long long)scene_id);
-    C->scenes.erase(it);
-    C->arena.release("scene" + std::to_string(scene_id));
-    return SHKZ_OK;
+    return C->scenes->release(scene_id);
 }
 
-uint64_t shkz_device_bytes(const shkz_ctx *C) { return C ? (uint64_t)C->arena.total : 0; }
+uint64_t shkz_device_bytes(const shkz_ctx *C) {
+    return C ? (uint64_t)C->arena.total.load() + (C->scenes == &C->own_scenes ? C->own_scenes.bytes.load() : 0) : 0;
+}
 
 int shkz_set_kernel_timing(shkz_ctx *C, int enable) {
     if (!C) return fail(SHKZ_ERR_INVALID, "NULL context");
@@ -1450,6 +1601,162 @@ int shkz_reset_kernel_stats(shkz_ctx *C) {
         resolve_timing(C);
     }
     C->totals.clear();
+    return SHKZ_OK;
+}
+
+}  // extern "C"
+
+// ============================================================================ C ABI: server
+extern "C" {
+
+void shkz_server_default_config(shkz_server_config *cfg) {
+    if (!cfg) return;
+    std::memset(cfg, 0, sizeof(*cfg));
+    cfg->executors = 2;
+    cfg->max_batch_grids = 64;
+    cfg->max_batch_cells = (int64_t)1 << 26;
+}
+
+int shkz_server_create(const shkz_server_config *cfg_in, shkz_server **out) {
+    if (!out) return fail(SHKZ_ERR_INVALID, "out is NULL");
+    *out = nullptr;
+    shkz_server_config cfg;
+    shkz_server_default_config(&cfg);
+    if (cfg_in) {
+        cfg = *cfg_in;
+        if (cfg.executors <= 0) cfg.executors = 2;
+        if (cfg.max_batch_grids <= 0) cfg.max_batch_grids = 64;
+        if (cfg.max_batch_cells <= 0) cfg.max_batch_cells = (int64_t)1 << 26;
+    }
+    int ndev = 0;
+    SHKZ_HIP(hipGetDeviceCount(&ndev));
+    if (cfg.device < 0 || cfg.device >= ndev)
+        return fail(SHKZ_ERR_INVALID, "device %d out of range (%d devices)", cfg.device, ndev);
+    SHKZ_HIP(hipSetDevice(cfg.device));
+    size_t free_b = 0, total_b = 0;
+    SHKZ_HIP(hipMemGetInfo(&free_b, &total_b));
+    auto *srv = new shkz_server();
+    srv->device = cfg.device;
+    srv->cfg = cfg;
+    srv->budget = cfg.hbm_budget_bytes ? cfg.hbm_budget_bytes : (uint64_t)(0.9 * (double)total_b);
+    srv->store.may_grow = [srv](uint64_t want) { return srv->held_except(-1) + want <= srv->budget; };
+    for (int k = 0; k < cfg.executors; ++k) {
+        shkz_ctx *C = nullptr;
+        const int rc = shkz_create(cfg.device, srv->budget, &C);
+        if (rc != SHKZ_OK) {
+            for (shkz_ctx *E : srv->exec) shkz_destroy(E);
+            delete srv;
+            return rc;
+        }
+        C->scenes = &srv->store;
+        C->arena.extra = [srv, k] { return srv->held_except(k); };
+        srv->exec.push_back(C);
+    }
+    ServeLimits lim;
+    lim.executors = cfg.executors;
+    lim.max_batch_grids = cfg.max_batch_grids;
+    lim.max_batch_cells = cfg.max_batch_cells;
+    lim.budget = srv->budget;
+    srv->core.reset(new ServeCore(
+        lim, [srv](int k, std::vector<ServeRequest> &b, std::vector<int> &st) { srv->run_batch(k, b, st); },
+        [srv] { return srv->store.bytes.load(); }));
+    *out = srv;
+    return SHKZ_OK;
+}
+
+void shkz_server_destroy(shkz_server *srv) {
+    if (!srv) return;
+    srv->core->shutdown();
+    for (const ServeOutcome &o : srv->core->take_all()) delete (ServerJob *)o.payload;  // never collected
+    srv->core.reset();
+    for (shkz_ctx *C : srv->exec) shkz_destroy(C);
+    srv->store.clear();
+    delete srv;
+}
+
+int shkz_submit(shkz_server *srv, const shkz_grid *grid, const shkz_params *params, uint64_t *ticket) {
+    if (!srv || !grid || !ticket) return fail(SHKZ_ERR_INVALID, "NULL argument");
+    SHKZ_TRY(validate_grid(grid));
+    auto *J = new ServerJob();
+    J->grid = *grid;
+    if (params) J->params = *params;
+    else shkz_default_params(&J->params);
+    std::memset(&J->result, 0, sizeof(J->result));
+    ServeRequest r;
+    {
+        std::lock_guard<std::mutex> g(srv->mu);
+        size_t gi = 0;
+        for (; gi < srv->groups.size(); ++gi)
+            if (std::memcmp(&srv->groups[gi], &J->params, sizeof(shkz_params)) == 0) break;
+        if (gi == srv->groups.size()) srv->groups.push_back(J->params);
+        r.ticket = srv->next_ticket++;
+        r.group = gi;
+    }
+    r.scene_id = grid->scene_id;
+    r.est_bytes = estimate_bytes(*grid, J->params);
+    r.cells = (int64_t)grid->w * grid->h * grid->d;
+    r.payload = J;
+    if (srv->core->submit(r) != 0) {
+        delete J;
+        return fail(SHKZ_ERR_INVALID, "server is shutting down");
+    }
+    *ticket = r.ticket;
+    return SHKZ_OK;
+}
+
+static int server_take(shkz_server *srv, int got, const ServeOutcome &o, shkz_result *result) {
+    if (got == 1) return SHKZ_PENDING;
+    if (got != 0) return fail(SHKZ_ERR_NOT_FOUND, "unknown ticket");
+    auto *J = (ServerJob *)o.payload;
+    shkz_result r = J->result;
+    if (o.status == SHKZ_ERR_BUDGET && r.status == SHKZ_OK) {
+        r.status = SHKZ_ERR_BUDGET;
+        J->error = "the request's estimated working set exceeds the server's device budget";
+    }
+    r.t_queue = o.t_start_ms - o.t_submit_ms;
+    r.t_latency = o.t_end_ms - o.t_submit_ms;
+    if (result) *result = r;
+    const int st = r.status;
+    if (st != SHKZ_OK) g_last_error = J->error;
+    delete J;
+    return st;
+}
+
+int shkz_poll(shkz_server *srv, uint64_t ticket, shkz_result *result) {
+    if (!srv) return fail(SHKZ_ERR_INVALID, "NULL server");
+    ServeOutcome o;
+    const int got = srv->core->poll(ticket, &o);
+    return server_take(srv, got, o, result);
+}
+
+int shkz_wait(shkz_server *srv, uint64_t ticket, double timeout_ms, shkz_result *result) {
+    if (!srv) return fail(SHKZ_ERR_INVALID, "NULL server");
+    ServeOutcome o;
+    const int got = srv->core->wait(ticket, timeout_ms, &o);
+    return server_take(srv, got, o, result);
+}
+
+int shkz_server_release_scene(shkz_server *srv, uint64_t scene_id) {
+    if (!srv) return fail(SHKZ_ERR_INVALID, "NULL server");
+    if (srv->core->scene_busy(scene_id))
+        return fail(SHKZ_ERR_INVALID, "scene %llu has queued or running requests", (unsigned long long)scene_id);
+    return srv->store.release(scene_id);
+}
+
+int shkz_server_get_stats(shkz_server *srv, shkz_server_stats *out) {
+    if (!srv || !out) return fail(SHKZ_ERR_INVALID, "NULL argument");
+    const ServeStats s = srv->core->stats();
+    std::memset(out, 0, sizeof(*out));
+    out->submitted = s.submitted;
+    out->completed = s.completed;
+    out->rejected = s.rejected;
+    out->batches = s.batches;
+    out->batched_requests = s.batched_requests;
+    out->max_batch = s.max_batch;
+    out->max_in_flight_batches = s.max_in_flight_batches;
+    out->max_in_flight_bytes = s.max_in_flight_bytes;
+    out->resident_bytes = srv->store.bytes.load();
+    out->device_bytes = srv->held_except(-1);
     return SHKZ_OK;
 }
 

@@ -6,6 +6,7 @@
 // max(phi_fluid, -(phi_solid + dx)) (:358-364).  Shapes are unions (min) of signed distances;
 // the box uses the reference's utility::box (include/shiokaze/utility/utility.h:113-120).
 #include "../../include/shkz_mi355x.h"
+#include "shkz_scene_params.h"
 
 #include <algorithm>
 #include <cmath>
@@ -82,15 +83,8 @@ struct Scene {
     }
 };
 
-}  // namespace
-
-extern "C" int shkz_scene_fill(const shkz_scene_desc *desc, void *fluid, float *solid, int32_t *has_solid, float *u,
-                               float *v, float *w) {
-    if (!desc || desc->w <= 0 || desc->h <= 0 || desc->d <= 0) return SHKZ_ERR_INVALID;
-    const int W = desc->w, H = desc->h, D = desc->d;
-    const double dx = std::min(1.0 / W, std::min(1.0 / H, 1.0 / D));
-    SceneRng rng(desc->seed);
-    Scene sc;
+// Draws the scene's shapes from rng (the velocity stream continues from where this leaves it).
+static int build_scene(const shkz_scene_desc *desc, SceneRng &rng, Scene &sc) {
     for (int a = 0; a < 3; ++a) sc.shift[a] = desc->shift[a];
     switch (desc->kind) {
     case SHKZ_SCENE_DROP:
@@ -142,6 +136,46 @@ extern "C" int shkz_scene_fill(const shkz_scene_desc *desc, void *fluid, float *
     default:
         return SHKZ_ERR_INVALID;
     }
+    return SHKZ_OK;
+}
+
+}  // namespace
+
+extern "C" int shkz_scene_params_internal(const shkz_scene_desc *desc, ShkzSceneParams *out) {
+    if (!desc || !out || desc->w <= 0 || desc->h <= 0 || desc->d <= 0) return SHKZ_ERR_INVALID;
+    SceneRng rng(desc->seed);
+    Scene sc;
+    if (build_scene(desc, rng, sc) != SHKZ_OK) return SHKZ_ERR_INVALID;
+    std::memset(out, 0, sizeof(*out));
+    if (sc.liquid_spheres.size() > 8 || sc.solid_spheres.size() > 4) return SHKZ_ERR_INVALID;
+    out->n_liquid = (int32_t)sc.liquid_spheres.size();
+    out->n_solid = (int32_t)sc.solid_spheres.size();
+    out->has_box = sc.solid_box ? 1 : 0;
+    out->pool = sc.pool;
+    for (int q = 0; q < out->n_liquid; ++q) {
+        for (int a = 0; a < 3; ++a) out->liquid[q][a] = sc.liquid_spheres[q].c[a];
+        out->liquid[q][3] = sc.liquid_spheres[q].r;
+    }
+    for (int q = 0; q < out->n_solid; ++q) {
+        for (int a = 0; a < 3; ++a) out->solid[q][a] = sc.solid_spheres[q].c[a];
+        out->solid[q][3] = sc.solid_spheres[q].r;
+    }
+    for (int a = 0; a < 3; ++a) {
+        out->box0[a] = sc.box0[a];
+        out->box1[a] = sc.box1[a];
+        out->shift[a] = sc.shift[a];
+    }
+    return SHKZ_OK;
+}
+
+extern "C" int shkz_scene_fill(const shkz_scene_desc *desc, void *fluid, float *solid, int32_t *has_solid, float *u,
+                               float *v, float *w) {
+    if (!desc || desc->w <= 0 || desc->h <= 0 || desc->d <= 0) return SHKZ_ERR_INVALID;
+    const int W = desc->w, H = desc->h, D = desc->d;
+    const double dx = std::min(1.0 / W, std::min(1.0 / H, 1.0 / D));
+    SceneRng rng(desc->seed);
+    Scene sc;
+    if (build_scene(desc, rng, sc) != SHKZ_OK) return SHKZ_ERR_INVALID;
     const bool with_solid = sc.has_solid();
     if (has_solid) *has_solid = with_solid ? 1 : 0;
 
