@@ -127,6 +127,13 @@ def kernel_roofline(stats, inst_steps, config):
     return roofline
 
 
+def mg_controls(r):
+    """The multigrid controls a result reports (autotuned or default)."""
+    return {"pre": r.mg_pre_sweeps, "post": r.mg_post_sweeps, "coarse_pre": r.mg_coarse_pre_sweeps,
+            "coarse_post": r.mg_coarse_post_sweeps, "coarse_solve_sweeps": r.mg_coarse_sweeps,
+            "omega": r.mg_omega, "inner_residual": r.inner_residual, "tuned": r.tuned}
+
+
 def c4_scenes(S):
     """64 cubes, sides drawn uniformly from {32,48,64,96,128,160} (numpy default_rng(3)), each a
     seeded sphere-drop (scene seed 3000+i) with its own N(0,1) velocities."""
@@ -159,8 +166,9 @@ def bench_batch(args):
     grids = [S.Context.device_grid(sc.w, sc.h, sc.d, sc.dt, b["fl"], S.F64, None, S.F32, *b["vel"])
              for sc, b in zip(scenes, dev)]
     ctx = S.Context(0)
-    prm_mixed = S.default_params(residual=residual)
-    prm_fp64 = S.default_params(residual=residual, precision=S.PREC_FP64)
+    tune = 0 if args.no_autotune else 1
+    prm_mixed = S.default_params(residual=residual, autotune=tune)
+    prm_fp64 = S.default_params(residual=residual, precision=S.PREC_FP64, autotune=tune)
 
     def restore():
         for b in dev:
@@ -196,6 +204,7 @@ def bench_batch(args):
         its = [r.iterations for r in res]
         log(f"[bench] {label}: {steps} x {len(res)} solves in {el:.3f}s -> {steps * len(res) / el:.1f} solves/s; "
             f"iterations {min(its)}..{max(its)}; worst reresid_inf {worst:.2e}; all converged {conv}")
+        log(f"[bench] {label}: controls {mg_controls(res[-1])}")
         return steps * len(res) / el, el / steps * 1e3, lat, worst, conv
 
     v_b, ms_b, lat_b, worst_b, conv_b = timed(batched, prm_mixed, args.steps, "batched mixed")
@@ -242,6 +251,8 @@ def bench_serve(args):
 
     specs = loadgen.make_requests(args.c5_requests, rate=200.0, seed=4)
     steps = args.c5_steps
+    # library default controls: tuning on the first batch of each new shape class would land in
+    # the served latencies
     prm = S.default_params(residual=1e-6, warm_start=1)
     cells = [s.w * s.h * s.d for s in specs]
     log(f"[bench] c5: {len(specs)} requests x {steps} steps, mean {np.mean(cells) / 1e6:.2f} M cells, "
@@ -291,6 +302,7 @@ def main():
     ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64"])
     ap.add_argument("--solver", default="mgpcg", choices=["mgpcg", "cg"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true", help="library default multigrid controls")
     ap.add_argument("--no-kernel-timing", action="store_true")
     ap.add_argument("--calibrate-copy-mb", type=int, default=0,
                     help="only run 5 device-to-device copies of this many MiB (HBM counter calibration)")
@@ -338,7 +350,7 @@ def main():
 
     prec = S.PREC_MIXED if args.precision == "mixed" else S.PREC_FP64
     solver = S.SOLVER_MGPCG if args.solver == "mgpcg" else S.SOLVER_CG
-    prm = S.default_params(residual=residual, precision=prec, solver=solver)
+    prm = S.default_params(residual=residual, precision=prec, solver=solver, autotune=0 if args.no_autotune else 1)
     ctx = S.Context(0)
 
     def step():
@@ -396,6 +408,7 @@ def main():
                    "grid": [n, n, n], "unknowns": int(r.n_unknowns), "bricks": int(r.n_bricks),
                    "levels": int(r.n_levels), "iterations": int(np.median(its)),
                    "outer_iterations": int(r.outer_iterations), "reresid_inf": float(max(x[2] for x in results)),
+                   "mg_controls": mg_controls(r),
                    "cell_updates_per_s": round(value * n ** 3, 1)},
         "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3), "p99": round(float(np.percentile(lat, 99)), 3)},
         "roofline": roofline,

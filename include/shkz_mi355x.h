@@ -112,7 +112,12 @@ typedef struct shkz_params {
     int32_t mg_coarse_post_sweeps;
     int32_t mg_bottom_bricks;     /* unused (kept for layout): every grid's coarsest level is one
                                      brick, solved by one workgroup */
-    int32_t reserved[5];
+    int32_t autotune;             /* 1: multigrid controls left at 0 come from the autotuner: looked
+                                     up per (device, shape class), tuned on the first call of a class
+                                     (repeated solves of the built system; the returned projection
+                                     uses the best controls).  Cached per process, and in the file
+                                     named by the environment variable SHKZ_TUNE_CACHE if set. */
+    int32_t reserved[4];
 } shkz_params;
 
 /* ---------------------------------------------------------------- result */
@@ -140,6 +145,11 @@ typedef struct shkz_result {
        and to its completion */
     double t_queue;
     double t_latency;
+    /* the multigrid controls the solve used (defaults, the caller's, or autotuned) */
+    int32_t mg_pre_sweeps, mg_post_sweeps, mg_coarse_pre_sweeps, mg_coarse_post_sweeps, mg_coarse_sweeps;
+    int32_t tuned;                /* 0 = defaults / given, 1 = from the tuning cache, 2 = tuned now */
+    double mg_omega;
+    double inner_residual;
 } shkz_result;
 
 /* ---------------------------------------------------------------- context */

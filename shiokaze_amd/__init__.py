@@ -57,8 +57,8 @@ class Params(ctypes.Structure):
         ("mg_pre_sweeps", ctypes.c_int32), ("mg_post_sweeps", ctypes.c_int32), ("mg_coarse_sweeps", ctypes.c_int32),
         ("mg_omega", ctypes.c_double), ("inner_residual", ctypes.c_double),
         ("mg_coarse_pre_sweeps", ctypes.c_int32), ("mg_coarse_post_sweeps", ctypes.c_int32),
-        ("mg_bottom_bricks", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 5),
+        ("mg_bottom_bricks", ctypes.c_int32), ("autotune", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 4),
     ]
 
 
@@ -72,6 +72,10 @@ class Result(ctypes.Structure):
         ("t_solid_fluid_fractions", ctypes.c_double), ("t_build_highres_linsystem", ctypes.c_double),
         ("t_linsolve", ctypes.c_double), ("t_update_velocity", ctypes.c_double), ("t_projection", ctypes.c_double),
         ("t_queue", ctypes.c_double), ("t_latency", ctypes.c_double),
+        ("mg_pre_sweeps", ctypes.c_int32), ("mg_post_sweeps", ctypes.c_int32),
+        ("mg_coarse_pre_sweeps", ctypes.c_int32), ("mg_coarse_post_sweeps", ctypes.c_int32),
+        ("mg_coarse_sweeps", ctypes.c_int32), ("tuned", ctypes.c_int32),
+        ("mg_omega", ctypes.c_double), ("inner_residual", ctypes.c_double),
     ]
 
     def as_dict(self) -> dict:

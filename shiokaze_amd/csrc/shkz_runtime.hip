@@ -348,6 +348,7 @@ struct shkz_ctx {
     hipEvent_t ev_in = nullptr, ev_out = nullptr;
     bool use_graphs = true;
     bool trace = false;
+    std::string device_key;  // device identity of the tuning cache
     // replayable PCG iteration graphs, keyed by everything their kernel arguments depend on
     struct GraphSlot {
         std::string key;
@@ -439,6 +440,7 @@ struct Problem {
     RedBuf red{};
     int pre = 4, post = 4, pre_c = 4, post_c = 4, coarse_sweeps = 48;
     double omega = 0.9;
+    double eta = 1e-3;  // mixed precision: relative defect reduction asked of each inner solve
 };
 
 template <class T>
@@ -758,6 +760,285 @@ static int project_impl(shkz_ctx *C, const shkz_grid *grids, int ngrids, const s
     return rc;
 }
 
+// Output of one solve phase, per grid.
+struct SolveOut {
+    std::vector<int> total_iters, outer_passes, status;
+};
+
+// The linear solve of every grid of the built batch (macpressuresolver3.cpp:233-237): FP64
+// CG / MGPCG, or FP64 defect correction around FP32 MGPCG.  Starts from P.x64 and leaves the
+// solution there; fills the per-grid residual and convergence fields of res.
+static int solve_phase(shkz_ctx *C, Launcher &K, Problem &P, const shkz_params *prm, bool multigrid, SolveOut &O,
+                       shkz_result *res) {
+    const int ngrids = P.ngrids;
+    hipStream_t s = K.s;
+    LevelHost &F0 = C->levels[0];
+    const int norm = prm->norm == SHKZ_NORM_L2 ? 1 : 0;
+    const int maxit = prm->max_iterations > 0 ? prm->max_iterations : 30000;
+    const double residual = prm->residual < 1e-30 ? 1e-30 : prm->residual;
+    OpDesc<double> A64{P.dir64, P.gx64, P.gy64, P.gz64};
+    const unsigned g0 = brick_grid(F0.nbricks);
+    const dim3 gb(g0), bb(BLOCK);
+    std::vector<int> &total_iters = O.total_iters, &outer_passes = O.outer_passes, &status = O.status;
+    total_iters.assign(ngrids, 0);
+    outer_passes.assign(ngrids, 0);
+    status.assign(ngrids, SHKZ_OK);
+    std::vector<SolveScalars> hs(ngrids), hi, hf;
+    if (F0.nbricks > 0) {
+        SHKZ_TRY(upload_scalars(C, s, P.S_out, ngrids, norm, maxit));
+        K.go("init_ctl", -1, 0, 0, dim3(1), dim3(64), k_init_ctl, P.ctl, ngrids);
+    }
+    if (F0.nbricks == 0) {
+        // no unknowns anywhere: nothing to solve
+    } else if (!P.mixed) {
+        // one FP64 Krylov solve per grid: CG (the reference iteration) or MGPCG
+        K.go("defect", 0, 7 * 8, 0, gb, bb, k_defect<double>, F0.desc, A64, (const double *)P.b64, (const double *)P.x64,
+             P.r64, P.red);
+        finalize(K, P, F0.desc, P.S_out, P.ctl, FIN_DEFECT_FIRST, residual);
+        BatchCtl hc{};
+        SHKZ_TRY(read_scalars(C, s, P.S_out, ngrids, hs, P.ctl, &hc));
+        if (!hc.all_done)
+            SHKZ_TRY(krylov<double>(C, K, P, P.x64, P.r64, (double *)P.kp, (double *)P.kq, (double *)P.kz,
+                                    (double *)P.ktmp, P.S_out, P.ctl, multigrid, hs));
+        for (int g = 0; g < ngrids; ++g) {
+            total_iters[g] = hs[g].iter;
+            outer_passes[g] = F0.gl[g].brick_end > F0.gl[g].brick_start ? 1 : 0;
+            if (hs[g].status) status[g] = SHKZ_ERR_NUMERIC;
+        }
+    } else {
+        // FP64 defect correction around FP32 MGPCG solves, per grid: the outer scalars decide on
+        // the device when a grid is done (converged, out of iterations, or no longer reduced)
+        const double eta = P.eta;
+        SHKZ_TRY(upload_scalars(C, s, P.S_in, ngrids, norm, maxit));
+        for (int outer = 0; outer < 64; ++outer) {
+            K.go("defect", 0, 7 * 8, 0, gb, bb, k_defect<double>, F0.desc, A64, (const double *)P.b64,
+                 (const double *)P.x64, P.r64, P.red);
+            if (outer == 0)
+                finalize(K, P, F0.desc, P.S_out, nullptr, FIN_DEFECT_FIRST, residual);
+            else
+                finalize(K, P, F0.desc, P.S_out, nullptr, FIN_OUTER, 0.0, 0, P.S_in);
+            K.go("init_ctl", -1, 0, 0, dim3(1), dim3(64), k_init_ctl, P.ctl + 1, ngrids);
+            K.go("inner_init", 0, 16, 0, gb, bb, k_inner_init, F0.desc, (const double *)P.r64, (float *)P.kr,
+                 (float *)P.kx, P.red);
+            finalize(K, P, F0.desc, P.S_in, P.ctl + 1, FIN_INNER_INIT, eta, 0, P.S_out);
+            BatchCtl hc{};
+            SHKZ_TRY(read_scalars(C, s, P.S_out, ngrids, hs, P.ctl + 1, &hc));
+            if (outer > 0)
+                for (int g = 0; g < ngrids; ++g) total_iters[g] = hs[g].iter;
+            if (hc.all_done) break;
+            for (int g = 0; g < ngrids; ++g)
+                if (!hs[g].done) outer_passes[g] = outer + 1;
+            SHKZ_TRY(krylov<float>(C, K, P, (float *)P.kx, (float *)P.kr, (float *)P.kp, (float *)P.kq, (float *)P.kz,
+                                   (float *)P.ktmp, P.S_in, P.ctl + 1, true, hi));
+            if (C->trace)
+                for (int g = 0; g < ngrids; ++g)
+                    if (!hs[g].done)
+                        std::fprintf(stderr, "[shkz] grid %d outer %d: defect %.6e (tol %.3e) inner it %d rinf %.6e\n",
+                                     g, outer, norm == 0 ? hs[g].rinf : hs[g].rl2, hs[g].tol, hi[g].iter, hi[g].rinf);
+            K.go("add_correction", 0, 20, 0, gb, bb, k_add_correction, F0.desc, P.x64, (const float *)P.kx);
+            if (outer == 63)
+                for (int g = 0; g < ngrids; ++g)
+                    if (!hs[g].done) total_iters[g] += hi[g].iter;
+        }
+        for (int g = 0; g < ngrids; ++g)
+            if (hs[g].status) status[g] = SHKZ_ERR_NUMERIC;
+    }
+    // true FP64 residual of the returned pressure, per grid
+    if (F0.nbricks > 0) {
+        K.go("defect", 0, 7 * 8, 0, gb, bb, k_defect<double>, F0.desc, A64, (const double *)P.b64, (const double *)P.x64,
+             P.r64, P.red);
+        finalize(K, P, F0.desc, P.S_fin, nullptr, FIN_DEFECT);
+        SHKZ_TRY(read_scalars(C, s, P.S_fin, ngrids, hf));
+    }
+    for (int g = 0; g < ngrids; ++g) {
+        if (F0.gl[g].brick_end == F0.gl[g].brick_start) {
+            res[g].converged = 1;  // no unknowns
+            continue;
+        }
+        const SolveScalars &o = hs[g];
+        res[g].reresid_inf = o.r0inf > 0 ? hf[g].rinf / o.r0inf : 0.0;
+        res[g].reresid_l2 = o.r0l2 > 0 ? hf[g].rl2 / o.r0l2 : 0.0;
+        res[g].reresid = norm == 0 ? res[g].reresid_inf : res[g].reresid_l2;
+        const double cur = norm == 0 ? hf[g].rinf : hf[g].rl2;
+        const double r0 = norm == 0 ? o.r0inf : o.r0l2;
+        res[g].converged = (r0 == 0.0 || cur <= o.tol * (1.0 + 1e-12)) ? 1 : 0;
+        if (!P.mixed && !o.converged && r0 != 0.0) res[g].converged = 0;
+    }
+    return SHKZ_OK;
+}
+
+// ------------------------------------------------------------------------ autotuner
+// Multigrid controls picked per (device, shape class) by timing repeated solves of the built
+// system.  Coordinate search from the current controls over each control the caller left at 0:
+// finest-level sweeps (pre = post, keeping the V-cycle symmetric), coarse-level sweeps, the Jacobi
+// weight, the coarse-solve sweeps, and (mixed precision) the inner-solve reduction.  A candidate
+// counts only if every grid of the batch converges; it replaces the current one when it is at
+// least 2% faster (best of two timed solves, so graph capture is not timed).
+struct Knobs {
+    int pre, post, pre_c, post_c, coarse;
+    double omega, eta;
+};
+
+static Knobs knobs_of(const Problem &P) { return Knobs{P.pre, P.post, P.pre_c, P.post_c, P.coarse_sweeps, P.omega, P.eta}; }
+
+// Sets P's controls to k except where the caller fixed a control in prm.
+static void apply_knobs(Problem &P, const Knobs &k, const shkz_params *prm) {
+    if (prm->mg_pre_sweeps <= 0) P.pre = k.pre;
+    if (prm->mg_post_sweeps <= 0) P.post = k.post;
+    if (prm->mg_coarse_pre_sweeps <= 0) P.pre_c = k.pre_c;
+    if (prm->mg_coarse_post_sweeps <= 0) P.post_c = k.post_c;
+    if (prm->mg_coarse_sweeps <= 0) P.coarse_sweeps = k.coarse;
+    if (!(prm->mg_omega > 0)) P.omega = k.omega;
+    if (!(prm->inner_residual > 0)) P.eta = k.eta;
+}
+
+struct TuneCache {
+    std::mutex mu;
+    std::unordered_map<std::string, Knobs> map;
+    bool loaded = false;
+    std::string path;
+};
+static TuneCache g_tune;
+
+static void tune_load_locked() {
+    if (g_tune.loaded) return;
+    g_tune.loaded = true;
+    const char *p = std::getenv("SHKZ_TUNE_CACHE");
+    if (!p || !*p) return;
+    g_tune.path = p;
+    FILE *f = std::fopen(p, "r");
+    if (!f) return;
+    char line[512];
+    while (std::fgets(line, sizeof(line), f)) {
+        char key[320];
+        Knobs k;
+        if (std::sscanf(line, "%319s %d %d %d %d %d %lf %lf", key, &k.pre, &k.post, &k.pre_c, &k.post_c, &k.coarse,
+                        &k.omega, &k.eta) == 8 &&
+            k.pre > 0 && k.post > 0 && k.pre_c > 0 && k.post_c > 0 && k.coarse > 0 && k.omega > 0 && k.omega <= 1.0 &&
+            k.eta > 0)
+            g_tune.map[key] = k;
+    }
+    std::fclose(f);
+}
+
+static bool tune_find(const std::string &key, Knobs *k) {
+    std::lock_guard<std::mutex> g(g_tune.mu);
+    tune_load_locked();
+    auto it = g_tune.map.find(key);
+    if (it == g_tune.map.end()) return false;
+    *k = it->second;
+    return true;
+}
+
+static void tune_store(const std::string &key, const Knobs &k) {
+    std::lock_guard<std::mutex> g(g_tune.mu);
+    tune_load_locked();
+    g_tune.map[key] = k;
+    if (g_tune.path.empty()) return;
+    FILE *f = std::fopen(g_tune.path.c_str(), "a");
+    if (!f) return;
+    std::fprintf(f, "%s %d %d %d %d %d %.17g %.17g\n", key.c_str(), k.pre, k.post, k.pre_c, k.post_c, k.coarse, k.omega,
+                 k.eta);
+    std::fclose(f);
+}
+
+// Shape class: device, solver, precision, norm, residual decade, stored bricks (half-octaves),
+// multigrid depth, batch size class, and which controls the caller fixed.
+static std::string tune_key(const shkz_ctx *C, const Problem &P, const shkz_params *prm) {
+    const int nb = C->levels[0].nbricks;
+    const int bclass = (int)std::lround(2.0 * std::log2((double)std::max(nb, 1)));
+    const int rclass = (int)std::lround(std::log10(std::max(prm->residual, 1e-30)));
+    const int gclass = P.ngrids == 1 ? 0 : (int)std::ceil(std::log2((double)P.ngrids) / 3.0);
+    const int fixed = (prm->mg_pre_sweeps > 0) | (prm->mg_post_sweeps > 0) << 1 | (prm->mg_coarse_pre_sweeps > 0) << 2 |
+                      (prm->mg_coarse_post_sweeps > 0) << 3 | (prm->mg_coarse_sweeps > 0) << 4 |
+                      (prm->mg_omega > 0) << 5 | (prm->inner_residual > 0) << 6;
+    char buf[320];
+    std::snprintf(buf, sizeof(buf), "%s|s%d|p%d|n%d|r%d|b%d|l%d|g%d|f%d", C->device_key.c_str(), prm->solver,
+                  prm->precision, prm->norm, rclass, bclass, P.nlev, gclass, fixed);
+    return buf;
+}
+
+static int autotune(shkz_ctx *C, Launcher &K, Problem &P, const shkz_params *prm, bool multigrid) {
+    const size_t n0 = (size_t)std::max(C->levels[0].nbricks, 1) * BRICK_CELLS;
+    double *x0 = nullptr;
+    SHKZ_TRY(arena_get(C->arena, "tune_x0", n0, &x0));
+    SHKZ_HIP(hipMemcpyAsync(x0, P.x64, n0 * sizeof(double), hipMemcpyDeviceToDevice, K.s));
+    std::vector<shkz_result> scratch(P.ngrids);
+    auto trial = [&](const Knobs &k, double *ms) -> int {
+        apply_knobs(P, k, prm);
+        double best = 1e300;
+        for (int rep = 0; rep < 2; ++rep) {
+            SHKZ_HIP(hipMemcpyAsync(P.x64, x0, n0 * sizeof(double), hipMemcpyDeviceToDevice, K.s));
+            SHKZ_HIP(hipStreamSynchronize(K.s));
+            SolveOut o;
+            const double t0 = now_ms();
+            SHKZ_TRY(solve_phase(C, K, P, prm, multigrid, o, scratch.data()));
+            const double t1 = now_ms();
+            for (int g = 0; g < P.ngrids; ++g)
+                if (!scratch[g].converged || o.status[g] != SHKZ_OK) {
+                    *ms = 1e300;
+                    return SHKZ_OK;
+                }
+            best = std::min(best, t1 - t0);
+        }
+        *ms = best;
+        return SHKZ_OK;
+    };
+    Knobs cur = knobs_of(P);
+    double cur_ms = 1e300;
+    SHKZ_TRY(trial(cur, &cur_ms));
+    auto consider = [&](Knobs k) -> int {
+        double ms = 1e300;
+        SHKZ_TRY(trial(k, &ms));
+        if (C->trace)
+            std::fprintf(stderr, "[shkz] tune pre/post %d/%d coarse %d/%d cs %d omega %.2f eta %.0e: %.3f ms\n", k.pre,
+                         k.post, k.pre_c, k.post_c, k.coarse, k.omega, k.eta, ms);
+        if (ms < 0.98 * cur_ms) {
+            cur = k;
+            cur_ms = ms;
+        }
+        return SHKZ_OK;
+    };
+    if (prm->mg_pre_sweeps <= 0 && prm->mg_post_sweeps <= 0)
+        for (int v : {2, 3, 4, 6}) {
+            if (v == cur.pre && v == cur.post) continue;
+            Knobs k = cur;
+            k.pre = k.post = v;
+            SHKZ_TRY(consider(k));
+        }
+    if (prm->mg_coarse_pre_sweeps <= 0 && prm->mg_coarse_post_sweeps <= 0 && P.nlev > 2)
+        for (int v : {2, 4, 6}) {
+            if (v == cur.pre_c && v == cur.post_c) continue;
+            Knobs k = cur;
+            k.pre_c = k.post_c = v;
+            SHKZ_TRY(consider(k));
+        }
+    if (!(prm->mg_omega > 0))
+        for (double v : {0.8, 0.9, 1.0}) {
+            if (v == cur.omega) continue;
+            Knobs k = cur;
+            k.omega = v;
+            SHKZ_TRY(consider(k));
+        }
+    if (prm->mg_coarse_sweeps <= 0)
+        for (int v : {16, 32, 48, 96}) {
+            if (v == cur.coarse) continue;
+            Knobs k = cur;
+            k.coarse = v;
+            SHKZ_TRY(consider(k));
+        }
+    if (P.mixed && !(prm->inner_residual > 0))
+        for (double v : {1e-2, 1e-3, 1e-4}) {
+            if (v == cur.eta) continue;
+            Knobs k = cur;
+            k.eta = v;
+            SHKZ_TRY(consider(k));
+        }
+    apply_knobs(P, cur, prm);
+    SHKZ_HIP(hipMemcpyAsync(P.x64, x0, n0 * sizeof(double), hipMemcpyDeviceToDevice, K.s));
+    return SHKZ_OK;
+}
+
 static int project_stages(shkz_ctx *C, const shkz_grid *grids, int ngrids, const shkz_params *prm, shkz_result *res,
                           hipStream_t s, const ExportOut *exp) {
     const double t_start = now_ms();
@@ -791,6 +1072,7 @@ static int project_stages(shkz_ctx *C, const shkz_grid *grids, int ngrids, const
     P.post_c = prm->mg_coarse_post_sweeps > 0 ? prm->mg_coarse_post_sweeps : P.post;
     P.coarse_sweeps = prm->mg_coarse_sweeps > 0 ? prm->mg_coarse_sweeps : 48;
     P.omega = prm->mg_omega > 0 ? prm->mg_omega : 0.9;
+    P.eta = prm->inner_residual > 0 ? prm->inner_residual : 1e-3;
     const bool multigrid = prm->solver == SHKZ_SOLVER_MGPCG;
 
     // ---------------------------------------------------------------- per grid: fractions + labelling
@@ -1127,94 +1409,32 @@ static int project_stages(shkz_ctx *C, const shkz_grid *grids, int ngrids, const
     const double t_build = now_ms();
 
     // ---------------------------------------------------------------- solve
-    const int norm = prm->norm == SHKZ_NORM_L2 ? 1 : 0;
-    const int maxit = prm->max_iterations > 0 ? prm->max_iterations : 30000;
-    const double residual = prm->residual < 1e-30 ? 1e-30 : prm->residual;
-    OpDesc<double> A64{P.dir64, P.gx64, P.gy64, P.gz64};
-    const unsigned g0 = brick_grid(F0.nbricks);
-    const dim3 gb(g0), bb(BLOCK);
-    std::vector<int> total_iters(ngrids, 0), outer_passes(ngrids, 0), status(ngrids, SHKZ_OK);
-    std::vector<SolveScalars> hs(ngrids), hi, hf;
-    if (F0.nbricks > 0) {
-        SHKZ_TRY(upload_scalars(C, s, P.S_out, ngrids, norm, maxit));
-        K.go("init_ctl", -1, 0, 0, dim3(1), dim3(64), k_init_ctl, P.ctl, ngrids);
-    }
-    if (F0.nbricks == 0) {
-        // no unknowns anywhere: nothing to solve
-    } else if (!P.mixed) {
-        // one FP64 Krylov solve per grid: CG (the reference iteration) or MGPCG
-        K.go("defect", 0, 7 * 8, 0, gb, bb, k_defect<double>, F0.desc, A64, (const double *)P.b64, (const double *)P.x64,
-             P.r64, P.red);
-        finalize(K, P, F0.desc, P.S_out, P.ctl, FIN_DEFECT_FIRST, residual);
-        BatchCtl hc{};
-        SHKZ_TRY(read_scalars(C, s, P.S_out, ngrids, hs, P.ctl, &hc));
-        if (!hc.all_done)
-            SHKZ_TRY(krylov<double>(C, K, P, P.x64, P.r64, (double *)P.kp, (double *)P.kq, (double *)P.kz,
-                                    (double *)P.ktmp, P.S_out, P.ctl, multigrid, hs));
-        for (int g = 0; g < ngrids; ++g) {
-            total_iters[g] = hs[g].iter;
-            outer_passes[g] = F0.gl[g].brick_end > F0.gl[g].brick_start ? 1 : 0;
-            if (hs[g].status) status[g] = SHKZ_ERR_NUMERIC;
+    int tuned = 0;
+    if (prm->autotune && multigrid && F0.nbricks > 0) {
+        const std::string key = tune_key(C, P, prm);
+        Knobs k;
+        if (tune_find(key, &k)) {
+            apply_knobs(P, k, prm);
+            tuned = 1;
+        } else {
+            SHKZ_TRY(autotune(C, K, P, prm, multigrid));
+            tune_store(key, knobs_of(P));
+            tuned = 2;
         }
-    } else {
-        // FP64 defect correction around FP32 MGPCG solves, per grid: the outer scalars decide on
-        // the device when a grid is done (converged, out of iterations, or no longer reduced)
-        const double eta = prm->inner_residual > 0 ? prm->inner_residual : 1e-3;
-        SHKZ_TRY(upload_scalars(C, s, P.S_in, ngrids, norm, maxit));
-        for (int outer = 0; outer < 64; ++outer) {
-            K.go("defect", 0, 7 * 8, 0, gb, bb, k_defect<double>, F0.desc, A64, (const double *)P.b64,
-                 (const double *)P.x64, P.r64, P.red);
-            if (outer == 0)
-                finalize(K, P, F0.desc, P.S_out, nullptr, FIN_DEFECT_FIRST, residual);
-            else
-                finalize(K, P, F0.desc, P.S_out, nullptr, FIN_OUTER, 0.0, 0, P.S_in);
-            K.go("init_ctl", -1, 0, 0, dim3(1), dim3(64), k_init_ctl, P.ctl + 1, ngrids);
-            K.go("inner_init", 0, 16, 0, gb, bb, k_inner_init, F0.desc, (const double *)P.r64, (float *)P.kr,
-                 (float *)P.kx, P.red);
-            finalize(K, P, F0.desc, P.S_in, P.ctl + 1, FIN_INNER_INIT, eta, 0, P.S_out);
-            BatchCtl hc{};
-            SHKZ_TRY(read_scalars(C, s, P.S_out, ngrids, hs, P.ctl + 1, &hc));
-            if (outer > 0)
-                for (int g = 0; g < ngrids; ++g) total_iters[g] = hs[g].iter;
-            if (hc.all_done) break;
-            for (int g = 0; g < ngrids; ++g)
-                if (!hs[g].done) outer_passes[g] = outer + 1;
-            SHKZ_TRY(krylov<float>(C, K, P, (float *)P.kx, (float *)P.kr, (float *)P.kp, (float *)P.kq, (float *)P.kz,
-                                   (float *)P.ktmp, P.S_in, P.ctl + 1, true, hi));
-            if (C->trace)
-                for (int g = 0; g < ngrids; ++g)
-                    if (!hs[g].done)
-                        std::fprintf(stderr, "[shkz] grid %d outer %d: defect %.6e (tol %.3e) inner it %d rinf %.6e\n",
-                                     g, outer, norm == 0 ? hs[g].rinf : hs[g].rl2, hs[g].tol, hi[g].iter, hi[g].rinf);
-            K.go("add_correction", 0, 20, 0, gb, bb, k_add_correction, F0.desc, P.x64, (const float *)P.kx);
-            if (outer == 63)
-                for (int g = 0; g < ngrids; ++g)
-                    if (!hs[g].done) total_iters[g] += hi[g].iter;
-        }
-        for (int g = 0; g < ngrids; ++g)
-            if (hs[g].status) status[g] = SHKZ_ERR_NUMERIC;
     }
-    // true FP64 residual of the returned pressure, per grid
-    if (F0.nbricks > 0) {
-        K.go("defect", 0, 7 * 8, 0, gb, bb, k_defect<double>, F0.desc, A64, (const double *)P.b64, (const double *)P.x64,
-             P.r64, P.red);
-        finalize(K, P, F0.desc, P.S_fin, nullptr, FIN_DEFECT);
-        SHKZ_TRY(read_scalars(C, s, P.S_fin, ngrids, hf));
-    }
+    SolveOut SO;
+    SHKZ_TRY(solve_phase(C, K, P, prm, multigrid, SO, res));
     for (int g = 0; g < ngrids; ++g) {
-        if (F0.gl[g].brick_end == F0.gl[g].brick_start) {
-            res[g].converged = 1;  // no unknowns
-            continue;
-        }
-        const SolveScalars &o = hs[g];
-        res[g].reresid_inf = o.r0inf > 0 ? hf[g].rinf / o.r0inf : 0.0;
-        res[g].reresid_l2 = o.r0l2 > 0 ? hf[g].rl2 / o.r0l2 : 0.0;
-        res[g].reresid = norm == 0 ? res[g].reresid_inf : res[g].reresid_l2;
-        const double cur = norm == 0 ? hf[g].rinf : hf[g].rl2;
-        const double r0 = norm == 0 ? o.r0inf : o.r0l2;
-        res[g].converged = (r0 == 0.0 || cur <= o.tol * (1.0 + 1e-12)) ? 1 : 0;
-        if (!P.mixed && !o.converged && r0 != 0.0) res[g].converged = 0;
+        res[g].mg_pre_sweeps = P.pre;
+        res[g].mg_post_sweeps = P.post;
+        res[g].mg_coarse_pre_sweeps = P.pre_c;
+        res[g].mg_coarse_post_sweeps = P.post_c;
+        res[g].mg_coarse_sweeps = P.coarse_sweeps;
+        res[g].mg_omega = P.omega;
+        res[g].inner_residual = P.eta;
+        res[g].tuned = tuned;
     }
+    const std::vector<int> &total_iters = SO.total_iters, &outer_passes = SO.outer_passes, &status = SO.status;
     const double t_solve = now_ms();
 
     // ---------------------------------------------------------------- per grid: scatter + update
@@ -1481,6 +1701,19 @@ int shkz_create(int device, uint64_t hbm_budget_bytes, shkz_ctx **out) {
         (void)hipHostFree(C->host_ctl);
         delete C;
         return fail(SHKZ_ERR_DEVICE, "stream/event creation failed");
+    }
+    {
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, device) == hipSuccess) {
+            std::string k = prop.gcnArchName;
+            k += "/cu";
+            k += std::to_string(prop.multiProcessorCount);
+            for (char &c : k)
+                if (c == ' ' || c == '|' || c == '\t') c = '_';
+            C->device_key = k;
+        } else {
+            C->device_key = "unknown";
+        }
     }
     const char *ng = std::getenv("SHKZ_NO_GRAPHS");
     C->use_graphs = !(ng && ng[0] == '1');
