@@ -127,6 +127,16 @@ def kernel_roofline(stats, inst_steps, config):
     return roofline
 
 
+def explicit_mg(spec):
+    """Params fields of an explicit --mg pre,post,coarse_pre,coarse_post,coarse_sweeps,omega,eta."""
+    if not spec:
+        return {}
+    v = spec.split(",")
+    return dict(mg_pre_sweeps=int(v[0]), mg_post_sweeps=int(v[1]), mg_coarse_pre_sweeps=int(v[2]),
+                mg_coarse_post_sweeps=int(v[3]), mg_coarse_sweeps=int(v[4]), mg_omega=float(v[5]),
+                inner_residual=float(v[6]))
+
+
 def mg_controls(r):
     """The multigrid controls a result reports (autotuned or default)."""
     return {"pre": r.mg_pre_sweeps, "post": r.mg_post_sweeps, "coarse_pre": r.mg_coarse_pre_sweeps,
@@ -303,6 +313,8 @@ def main():
     ap.add_argument("--solver", default="mgpcg", choices=["mgpcg", "cg"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true", help="library default multigrid controls")
+    ap.add_argument("--mg", default="", help="explicit multigrid controls pre,post,coarse_pre,coarse_post,"
+                                             "coarse_solve_sweeps,omega,inner_residual (implies --no-autotune)")
     ap.add_argument("--no-kernel-timing", action="store_true")
     ap.add_argument("--calibrate-copy-mb", type=int, default=0,
                     help="only run 5 device-to-device copies of this many MiB (HBM counter calibration)")
@@ -350,7 +362,8 @@ def main():
 
     prec = S.PREC_MIXED if args.precision == "mixed" else S.PREC_FP64
     solver = S.SOLVER_MGPCG if args.solver == "mgpcg" else S.SOLVER_CG
-    prm = S.default_params(residual=residual, precision=prec, solver=solver, autotune=0 if args.no_autotune else 1)
+    prm = S.default_params(residual=residual, precision=prec, solver=solver,
+                           autotune=0 if (args.no_autotune or args.mg) else 1, **explicit_mg(args.mg))
     ctx = S.Context(0)
 
     def step():

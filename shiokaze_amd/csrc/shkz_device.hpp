@@ -40,16 +40,26 @@ struct GridLevel {
     int pad[2];
 };
 
+// Per stored brick: its six neighbour bricks (NB_*, -1 = none), its grid and flags, in one
+// 32-byte record (one scalar load per workgroup).
+enum { BM_COARSEST = 1 };  // the brick's grid ends on this level (its coarse solve handles it)
+struct __align__(32) BrickMeta {
+    int nbr[6];
+    int grid;
+    int flags;
+};
+
 // One multigrid level of a batch of grids: the stored bricks of every grid, concatenated
 // (grid-major, Morton order inside a grid).  A single grid is a batch of one.
 struct LevelDesc {
-    int nbricks;        // stored bricks, all grids
+    int nbricks;             // stored bricks, all grids
     int ngrids;
-    const int *bmap;    // per grid g: bmap[gl[g].bmap_off + (bx*nby + by)*nbz + bz] -> brick or -1
-    const int *bxyz;    // brick -> bx, by, bz (in its grid)
-    const int *bnbr;    // brick -> 6 neighbour bricks (NB_*), -1 = none
-    const int *bgrid;   // brick -> grid
+    const int *bmap;         // per grid g: bmap[gl[g].bmap_off + (bx*nby + by)*nbz + bz] -> brick or -1
+    const int *bxyz;         // brick -> bx, by, bz (in its grid)
+    const BrickMeta *meta;   // brick -> neighbours, grid, flags
     const GridLevel *gl;
+    const int *child64;      // brick -> the 4x4x4 finer-level bricks 2B-1 .. 2B+2 per axis (-1 = none)
+    const int *parent8;      // brick -> the 2x2x2 coarser-level bricks (B+1)/2 - 1 + {0,1} per axis
 };
 
 template <class T>

@@ -29,13 +29,15 @@ struct Nbrs {
     int n[6];
 };
 
-__device__ __forceinline__ Nbrs load_nbrs(const LevelDesc &L, int b) {
+__device__ __forceinline__ Nbrs nbrs_of(const BrickMeta &m, int b) {
     Nbrs N;
     N.base = (long)b * BRICK_CELLS;
 #pragma unroll
-    for (int d = 0; d < 6; ++d) N.n[d] = L.bnbr[b * 6 + d];
+    for (int d = 0; d < 6; ++d) N.n[d] = m.nbr[d];
     return N;
 }
+
+__device__ __forceinline__ Nbrs load_nbrs(const LevelDesc &L, int b) { return nbrs_of(L.meta[b], b); }
 
 // Storage index of the six face neighbours of local cell t, -1 where no brick is stored.
 __device__ __forceinline__ void nb_index(const Nbrs &N, int t, long idx[6]) {
@@ -80,11 +82,15 @@ __device__ __forceinline__ int brick_or_idle(const LevelDesc &L) {
 
 __device__ __forceinline__ bool all_done(const Gate &Q) { return Q.ctl && Q.ctl->all_done; }
 
-// Smoothing-side V-cycle work on brick br of level L is skipped for a grid that is done or whose
-// last level this is (its coarse solve handles it).
-__device__ __forceinline__ bool skip_smooth(const Gate &Q, const LevelDesc &L, int br) {
-    const int g = L.bgrid[br];
-    return L.gl[g].coarsest || (Q.S && Q.S[g].done);
+// The brick's grid is done (in a batch of one, all_done says so already: no load).
+__device__ __forceinline__ bool grid_done(const Gate &Q, const LevelDesc &L, const BrickMeta &m) {
+    return L.ngrids > 1 && Q.S && Q.S[m.grid].done;
+}
+
+// Smoothing-side V-cycle work on a brick is skipped for a grid that is done or whose last level
+// this is (its coarse solve handles it).
+__device__ __forceinline__ bool skip_smooth(const Gate &Q, const LevelDesc &L, const BrickMeta &m) {
+    return (m.flags & BM_COARSEST) || grid_done(Q, L, m);
 }
 
 // Value of level vector v at cell (i,j,k) of the grid described by G (0 outside or unstored).
@@ -264,7 +270,7 @@ __global__ __launch_bounds__(BLOCK) void k_coarsen(LevelDesc Lf, OpDesc<Ti> Af, 
     const int br = brick_or_idle(Lc);
     if (br < 0) return;
     const int t = threadIdx.x;
-    const GridLevel F = Lf.gl[Lc.bgrid[br]];
+    const GridLevel F = Lf.gl[Lc.meta[br].grid];
     const int I = Lc.bxyz[3 * br] * BRICK + (t >> 6), J = Lc.bxyz[3 * br + 1] * BRICK + ((t >> 3) & 7),
               K = Lc.bxyz[3 * br + 2] * BRICK + (t & 7);
     const long c = (long)br * BRICK_CELLS + t;
@@ -328,7 +334,8 @@ __global__ __launch_bounds__(BLOCK) void k_brick_counts(LevelDesc L, const T *__
 }
 
 // ------------------------------------------------------------------------------ Krylov (batch)
-__device__ __forceinline__ int grid_of(const LevelDesc &L, int br) { return L.bgrid[br]; }
+// Grid of brick br (no load in a batch of one).
+__device__ __forceinline__ int grid_of(const LevelDesc &L, int br) { return L.ngrids > 1 ? L.meta[br].grid : 0; }
 
 // r = b - A x with per-brick ||r||_inf and ||r||^2 partials (finalized by FIN_DEFECT*).
 template <class T>
@@ -376,16 +383,17 @@ __global__ __launch_bounds__(BLOCK) void k_apply_dot(LevelDesc L, OpDesc<T> A, c
                                                      T *__restrict__ q, const SolveScalars *S, const BatchCtl *ctl,
                                                      RedBuf R) {
     __shared__ double lds[16];
-    if (ctl->all_done) return;
     const int br = brick_or_idle(L);
-    if (br < 0 || S[grid_of(L, br)].done) return;
+    if (br < 0) return;
+    const BrickMeta m = L.meta[br];
+    if (ctl->all_done || (L.ngrids > 1 && S[m.grid].done)) return;
     const int t = threadIdx.x;
     const long c = (long)br * BRICK_CELLS + t;
     const T dirc = A.dir[c];
     T out = 0;
     double part = 0.0;
     if (dirc >= 0) {
-        const Nbrs N = load_nbrs(L, br);
+        const Nbrs N = nbrs_of(m, br);
         long idx[6];
         nb_index(N, t, idx);
         const T pc = p[c];
@@ -405,11 +413,10 @@ __global__ __launch_bounds__(BLOCK) void k_cg_update(LevelDesc L, T *__restrict_
                                                      const T *__restrict__ p, const T *__restrict__ q,
                                                      const SolveScalars *S, const BatchCtl *ctl, RedBuf R) {
     __shared__ double lds[16];
-    if (ctl->all_done) return;
     const int br = brick_or_idle(L);
     if (br < 0) return;
     const SolveScalars &Sg = S[grid_of(L, br)];
-    if (Sg.done) return;
+    if (ctl->all_done || Sg.done) return;
     const long c = (long)br * BRICK_CELLS + threadIdx.x;
     const T alpha = (T)Sg.alpha;
     x[c] += alpha * p[c];
@@ -425,9 +432,9 @@ template <class T>
 __global__ __launch_bounds__(BLOCK) void k_dot_rz(LevelDesc L, const T *__restrict__ r, const T *__restrict__ z,
                                                   const SolveScalars *S, const BatchCtl *ctl, RedBuf R) {
     __shared__ double lds[16];
-    if (ctl->all_done) return;
     const int br = brick_or_idle(L);
-    if (br < 0 || S[grid_of(L, br)].done) return;
+    if (br < 0) return;
+    if (ctl->all_done || S[grid_of(L, br)].done) return;
     const long c = (long)br * BRICK_CELLS + threadIdx.x;
     const double part = block_sum((double)r[c] * (double)z[c], lds);
     store_partials(R, br, part, 0.0);
@@ -437,11 +444,10 @@ __global__ __launch_bounds__(BLOCK) void k_dot_rz(LevelDesc L, const T *__restri
 template <class T>
 __global__ __launch_bounds__(BLOCK) void k_update_p(LevelDesc L, const T *__restrict__ z, T *__restrict__ p,
                                                     const SolveScalars *S, const BatchCtl *ctl, int first) {
-    if (ctl->all_done) return;
     const int br = brick_or_idle(L);
     if (br < 0) return;
     const SolveScalars &Sg = S[grid_of(L, br)];
-    if (Sg.done) return;
+    if (ctl->all_done || Sg.done) return;
     const long c = (long)br * BRICK_CELLS + threadIdx.x;
     p[c] = first ? z[c] : z[c] + (T)Sg.beta * p[c];
 }
@@ -628,9 +634,10 @@ template <class T, int FINEST>
 __global__ __launch_bounds__(BLOCK) void k_smooth_first(LevelDesc L, const T *__restrict__ idiag,
                                                         const T *__restrict__ rhs, T *__restrict__ z, T omega,
                                                         Gate Q) {
-    if (all_done(Q)) return;
     const int br = brick_or_idle(L);
-    if (br < 0 || skip_smooth(Q, L, br)) return;
+    if (br < 0) return;
+    const BrickMeta m = L.meta[br];
+    if (all_done(Q) || skip_smooth(Q, L, m)) return;
     const long c = (long)br * BRICK_CELLS + threadIdx.x;
     z[c] = omega * idiag[c] * rhs[c];
 }
@@ -641,15 +648,16 @@ template <class T, int FINEST>
 __global__ __launch_bounds__(BLOCK) void k_smooth2(LevelDesc L, OpDesc<T> A, const T *__restrict__ idiag,
                                                    const T *__restrict__ rhs, T *__restrict__ z, T omega,
                                                    Gate Q) {
-    if (all_done(Q)) return;
     const int br = brick_or_idle(L);
-    if (br < 0 || skip_smooth(Q, L, br)) return;
+    if (br < 0) return;
+    const BrickMeta m = L.meta[br];
+    if (all_done(Q) || skip_smooth(Q, L, m)) return;
     const int t = threadIdx.x;
     const long c = (long)br * BRICK_CELLS + t;
     const T id = idiag[c];
     T out = 0;
     if (id > 0) {
-        const Nbrs N = load_nbrs(L, br);
+        const Nbrs N = nbrs_of(m, br);
         long idx[6];
         nb_index(N, t, idx);
         auto z1 = [&](long i) { return i >= 0 ? omega * idiag[i] * rhs[i] : T(0); };
@@ -669,16 +677,17 @@ __global__ __launch_bounds__(BLOCK) void k_jacobi(LevelDesc L, OpDesc<T> A, cons
                                                   const T *__restrict__ rhs, const T *__restrict__ zin,
                                                   T *__restrict__ zout, T omega, Gate Q, RedBuf R) {
     __shared__ double lds[16];
-    if (all_done(Q)) return;
     const int br = brick_or_idle(L);
-    if (br < 0 || skip_smooth(Q, L, br)) return;
+    if (br < 0) return;
+    const BrickMeta m = L.meta[br];
+    if (all_done(Q) || skip_smooth(Q, L, m)) return;
     const int t = threadIdx.x;
     const long c = (long)br * BRICK_CELLS + t;
     const T id = idiag[c];
     T out = 0;
     double part = 0.0;
     if (id > 0) {
-        const Nbrs N = load_nbrs(L, br);
+        const Nbrs N = nbrs_of(m, br);
         long idx[6];
         nb_index(N, t, idx);
         const T r = rhs[c];
@@ -696,15 +705,16 @@ __global__ __launch_bounds__(BLOCK) void k_jacobi(LevelDesc L, OpDesc<T> A, cons
 template <class T, int FINEST>
 __global__ __launch_bounds__(BLOCK) void k_residual(LevelDesc L, OpDesc<T> A, const T *__restrict__ rhs,
                                                     const T *__restrict__ z, T *__restrict__ res, Gate Q) {
-    if (all_done(Q)) return;
     const int br = brick_or_idle(L);
-    if (br < 0 || skip_smooth(Q, L, br)) return;
+    if (br < 0) return;
+    const BrickMeta m = L.meta[br];
+    if (all_done(Q) || skip_smooth(Q, L, m)) return;
     const int t = threadIdx.x;
     const long c = (long)br * BRICK_CELLS + t;
     const T dirc = A.dir[c];
     T out = 0;
     if (dirc >= 0) {
-        const Nbrs N = load_nbrs(L, br);
+        const Nbrs N = nbrs_of(m, br);
         long idx[6];
         nb_index(N, t, idx);
         T diag;
@@ -714,38 +724,29 @@ __global__ __launch_bounds__(BLOCK) void k_residual(LevelDesc L, OpDesc<T> A, co
 }
 
 // Restriction = transpose of trilinear prolongation: coarse cell I gathers fine cells
-// 2I-1..2I+2 per axis with weights 1/4, 3/4, 3/4, 1/4.  One block per coarse brick stages the
-// 18^3 fine residuals it needs in LDS.
+// 2I-1..2I+2 per axis with weights 1/4, 3/4, 3/4, 1/4.  One block per coarse brick B stages the
+// 18^3 fine residuals it needs (fine cells 16B-1 .. 16B+16 per axis, in the fine bricks 2B-1 ..
+// 2B+2 that the brick's child table lists) in LDS.
 template <class T, int FINEST>
-__global__ __launch_bounds__(BLOCK) void k_restrict(LevelDesc Lf, LevelDesc Lc, OpDesc<T> Ac, const T *__restrict__ res,
+__global__ __launch_bounds__(BLOCK) void k_restrict(LevelDesc Lc, OpDesc<T> Ac, const T *__restrict__ res,
                                                     T *__restrict__ rhs_c, Gate Q) {
     constexpr int E = 18;
     __shared__ T tile[E * E * E];
     __shared__ int bid[64];
-    if (all_done(Q)) return;
     const int br = brick_or_idle(Lc);
-    if (br < 0 || (Q.S && Q.S[Lc.bgrid[br]].done)) return;
+    if (br < 0) return;
+    const BrickMeta m = Lc.meta[br];
+    if (all_done(Q) || grid_done(Q, Lc, m)) return;
     const int t = threadIdx.x;
-    const GridLevel F = Lf.gl[Lc.bgrid[br]];
-    const int BX = Lc.bxyz[3 * br], BY = Lc.bxyz[3 * br + 1], BZ = Lc.bxyz[3 * br + 2];
-    const int f0 = 16 * BX - 1, g0 = 16 * BY - 1, h0 = 16 * BZ - 1;
-    // the 18^3 tile touches fine bricks 2B-1 .. 2B+2 per axis: look them up once
-    if (t < 64) {
-        const int a = t >> 4, b = (t >> 2) & 3, cc = t & 3;
-        const int fx = 2 * BX - 1 + a, fy = 2 * BY - 1 + b, fz = 2 * BZ - 1 + cc;
-        int id = -1;
-        if (fx >= 0 && fy >= 0 && fz >= 0 && fx < F.nbx && fy < F.nby && fz < F.nbz)
-            id = Lf.bmap[F.bmap_off + (fx * F.nby + fy) * F.nbz + fz];
-        bid[t] = id;
-    }
+    if (t < 64) bid[t] = Lc.child64[64 * br + t];
     __syncthreads();
     for (int e = t; e < E * E * E; e += BLOCK) {
         const int c = e % E, b = (e / E) % E, a = e / (E * E);
-        const int fi = f0 + a, fj = g0 + b, fk = h0 + c;
-        // (fi + 8) >> 3 avoids shifting -1; slot 0 is brick 2B-1
-        const int slot = ((((fi + 8) >> 3) - 2 * BX) * 4 + (((fj + 8) >> 3) - 2 * BY)) * 4 + (((fk + 8) >> 3) - 2 * BZ);
+        // tile offset a is fine cell 16B - 1 + a: fine brick 2B - 1 + ((a + 7) >> 3), local (a + 7) & 7
+        const int slot = (((a + 7) >> 3) * 4 + ((b + 7) >> 3)) * 4 + ((c + 7) >> 3);
         const int id = bid[slot];
-        tile[e] = id >= 0 ? res[(long)id * BRICK_CELLS + (((fi & 7) * 8 + (fj & 7)) * 8 + (fk & 7))] : T(0);
+        tile[e] = id >= 0 ? res[(long)id * BRICK_CELLS + ((((a + 7) & 7) * 8 + ((b + 7) & 7)) * 8 + ((c + 7) & 7))]
+                          : T(0);
     }
     __syncthreads();
     const long c = (long)br * BRICK_CELLS + t;
@@ -771,40 +772,31 @@ __global__ __launch_bounds__(BLOCK) void k_restrict(LevelDesc Lf, LevelDesc Lc, 
 }
 
 // z_f += trilinear interpolation of the coarse correction (fine unknowns only).  One block per
-// fine brick stages the 6^3 coarse values it needs in LDS.
+// fine brick B stages the 6^3 coarse values it needs (coarse cells 4B-1 .. 4B+4 per axis, in the
+// coarse bricks its parent table lists) in LDS.  Coarse cells outside the coarse grid read 0: they
+// lie in unstored bricks or are non-unknowns, whose correction every V-cycle kernel keeps at 0.
 template <class T, int FINEST>
-__global__ __launch_bounds__(BLOCK) void k_prolong(LevelDesc Lf, OpDesc<T> Af, LevelDesc Lc, const T *__restrict__ zc,
+__global__ __launch_bounds__(BLOCK) void k_prolong(LevelDesc Lf, OpDesc<T> Af, const T *__restrict__ zc,
                                                    T *__restrict__ zf, Gate Q) {
     constexpr int E = 6;
     __shared__ T tile[E * E * E];
     __shared__ int bid[8];
-    if (all_done(Q)) return;
     const int br = brick_or_idle(Lf);
-    if (br < 0 || skip_smooth(Q, Lf, br)) return;
+    if (br < 0) return;
+    const BrickMeta m = Lf.meta[br];
+    if (all_done(Q) || skip_smooth(Q, Lf, m)) return;
     const int t = threadIdx.x;
-    const GridLevel Cg = Lc.gl[Lf.bgrid[br]];
-    const int BX = Lf.bxyz[3 * br], BY = Lf.bxyz[3 * br + 1], BZ = Lf.bxyz[3 * br + 2];
-    // the 6^3 tile (coarse cells 4B-1 .. 4B+4 per axis) touches coarse bricks (4B-1)>>3 and
-    // (4B+4)>>3 per axis: look them up once
-    const int cx0 = (4 * BX + 7) / 8 - 1, cy0 = (4 * BY + 7) / 8 - 1, cz0 = (4 * BZ + 7) / 8 - 1;
-    if (t < 8) {
-        const int fx = cx0 + (t >> 2), fy = cy0 + ((t >> 1) & 1), fz = cz0 + (t & 1);
-        int id = -1;
-        if (fx >= 0 && fy >= 0 && fz >= 0 && fx < Cg.nbx && fy < Cg.nby && fz < Cg.nbz)
-            id = Lc.bmap[Cg.bmap_off + (fx * Cg.nby + fy) * Cg.nbz + fz];
-        bid[t] = id;
-    }
+    if (t < 8) bid[t] = Lf.parent8[8 * br + t];
+    // with B = 2M + p, tile offset a is coarse cell 8M + 4p - 1 + a: coarse brick
+    // M - 1 + p + slot with slot = ((4p + 7 + a) >> 3) - p, local (4p + 7 + a) & 7
+    const int px = Lf.bxyz[3 * br] & 1, py = Lf.bxyz[3 * br + 1] & 1, pz = Lf.bxyz[3 * br + 2] & 1;
     __syncthreads();
     if (t < E * E * E) {
         const int c = t % E, b = (t / E) % E, a = t / (E * E);
-        const int I = 4 * BX - 1 + a, J = 4 * BY - 1 + b, K = 4 * BZ - 1 + c;
-        T v = 0;
-        if (I >= 0 && J >= 0 && K >= 0 && I < Cg.ni && J < Cg.nj && K < Cg.nk) {
-            const int slot = (((I >> 3) - cx0) * 2 + ((J >> 3) - cy0)) * 2 + ((K >> 3) - cz0);
-            const int id = bid[slot];
-            if (id >= 0) v = zc[(long)id * BRICK_CELLS + (((I & 7) * 8 + (J & 7)) * 8 + (K & 7))];
-        }
-        tile[t] = v;
+        const int ia = 4 * px + 7 + a, ib = 4 * py + 7 + b, ic = 4 * pz + 7 + c;
+        const int slot = (((ia >> 3) - px) * 2 + ((ib >> 3) - py)) * 2 + ((ic >> 3) - pz);
+        const int id = bid[slot];
+        tile[t] = id >= 0 ? zc[(long)id * BRICK_CELLS + (((ia & 7) * 8 + (ib & 7)) * 8 + (ic & 7))] : T(0);
     }
     __syncthreads();
     const long c = (long)br * BRICK_CELLS + t;

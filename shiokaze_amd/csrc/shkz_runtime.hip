@@ -172,6 +172,8 @@ struct LevelHost {
     bool any_coarsest = false;  // some grid ends on this level
     std::vector<int> bmap, bxyz, bnbr, bgrid;
     std::vector<GridLevel> gl;
+    std::vector<BrickMeta> meta;
+    std::vector<int> child64, parent8;  // restriction / prolongation brick tables (multigrid only)
     LevelDesc desc{};
     // operator and V-cycle vectors (element type = level precision)
     void *dir = nullptr, *gx = nullptr, *gy = nullptr, *gz = nullptr, *idiag = nullptr;
@@ -225,14 +227,55 @@ static void build_level(LevelHost &L, const std::vector<GridLevelHost> &grids) {
     }
     L.nbricks = nb;
     L.bnbr.assign(6 * (size_t)nb, -1);
+    L.meta.assign(nb, BrickMeta{});
     const int off[6][3] = {{-1, 0, 0}, {1, 0, 0}, {0, -1, 0}, {0, 1, 0}, {0, 0, -1}, {0, 0, 1}};
     for (int id = 0; id < nb; ++id) {
         const GridLevel &gl = L.gl[L.bgrid[id]];
+        BrickMeta &m = L.meta[id];
         for (int d = 0; d < 6; ++d) {
             const int x = L.bxyz[3 * id] + off[d][0], y = L.bxyz[3 * id + 1] + off[d][1], z = L.bxyz[3 * id + 2] + off[d][2];
             if (x >= 0 && y >= 0 && z >= 0 && x < gl.nbx && y < gl.nby && z < gl.nbz)
                 L.bnbr[6 * id + d] = L.bmap[gl.bmap_off + (x * gl.nby + y) * gl.nbz + z];
+            m.nbr[d] = L.bnbr[6 * id + d];
         }
+        m.grid = L.bgrid[id];
+        m.flags = gl.coarsest ? BM_COARSEST : 0;
+    }
+    L.child64.clear();
+    L.parent8.clear();
+}
+
+// Brick of grid g at brick coordinates (x, y, z) of level L, -1 outside the grid or unstored.
+static int brick_at(const LevelHost &L, int g, int x, int y, int z) {
+    const GridLevel &gl = L.gl[g];
+    if (x < 0 || y < 0 || z < 0 || x >= gl.nbx || y >= gl.nby || z >= gl.nbz) return -1;
+    return L.bmap[gl.bmap_off + (x * gl.nby + y) * gl.nbz + z];
+}
+
+// Restriction and prolongation tables between level F and the next coarser level Cl: every
+// coarse brick B lists the fine bricks 2B-1 .. 2B+2 per axis, every fine brick B the coarse
+// bricks (B+1)/2 - 1 + {0,1} per axis (bricks of the same grid; -1 where none is stored).
+static void link_levels(LevelHost &F, LevelHost &Cl) {
+    Cl.child64.assign((size_t)64 * Cl.nbricks, -1);
+    for (int id = 0; id < Cl.nbricks; ++id) {
+        const int g = Cl.bgrid[id];
+        const int X = Cl.bxyz[3 * id], Y = Cl.bxyz[3 * id + 1], Z = Cl.bxyz[3 * id + 2];
+        for (int a = 0; a < 4; ++a)
+            for (int b = 0; b < 4; ++b)
+                for (int c = 0; c < 4; ++c)
+                    Cl.child64[64 * (size_t)id + (a * 4 + b) * 4 + c] =
+                        brick_at(F, g, 2 * X - 1 + a, 2 * Y - 1 + b, 2 * Z - 1 + c);
+    }
+    F.parent8.assign((size_t)8 * F.nbricks, -1);
+    for (int id = 0; id < F.nbricks; ++id) {
+        const int g = F.bgrid[id];
+        if (F.gl[g].coarsest) continue;
+        const int X = F.bxyz[3 * id], Y = F.bxyz[3 * id + 1], Z = F.bxyz[3 * id + 2];
+        const int x0 = (X + 1) / 2 - 1, y0 = (Y + 1) / 2 - 1, z0 = (Z + 1) / 2 - 1;
+        for (int a = 0; a < 2; ++a)
+            for (int b = 0; b < 2; ++b)
+                for (int c = 0; c < 2; ++c)
+                    F.parent8[8 * (size_t)id + (a * 2 + b) * 2 + c] = brick_at(Cl, g, x0 + a, y0 + b, z0 + c);
     }
 }
 
@@ -509,7 +552,7 @@ static T *vcycle(shkz_ctx *C, Launcher &K, Problem &P, const T *rhs0, T *z0, T *
              rhs, (const T *)cur, (T *)L.res, Q);
         LevelHost &Lc = C->levels[l + 1];
         K.go(f ? "restrict_L0" : "restrict_Lc", l, es, 2 * es * (double)Lc.unknowns, dim3(brick_grid(Lc.nbricks)), b,
-             f ? k_restrict<T, 1> : k_restrict<T, 0>, L.desc, Lc.desc, op_of<T>(Lc), (const T *)L.res, (T *)Lc.rhs, Q);
+             f ? k_restrict<T, 1> : k_restrict<T, 0>, Lc.desc, op_of<T>(Lc), (const T *)L.res, (T *)Lc.rhs, Q);
         zcur[l] = cur;
         zoth[l] = oth;
     }
@@ -523,7 +566,7 @@ static T *vcycle(shkz_ctx *C, Launcher &K, Problem &P, const T *rhs0, T *z0, T *
         const bool f = l == 0;
         const int post = f ? P.post : P.post_c;
         K.go(f ? "prolong_L0" : "prolong_Lc", l, 3 * es, es * (double)Lc.unknowns, g, b,
-             f ? k_prolong<T, 1> : k_prolong<T, 0>, L.desc, A, Lc.desc, (const T *)zcur[l + 1], zcur[l], Q);
+             f ? k_prolong<T, 1> : k_prolong<T, 0>, L.desc, A, (const T *)zcur[l + 1], zcur[l], Q);
         T *cur = zcur[l], *oth = zoth[l];
         for (int s = 0; s < post; ++s) {
             const bool last_dot = f && dot_partials && s == post - 1;
@@ -600,10 +643,10 @@ static std::string graph_key(shkz_ctx *C, const Problem &P, const void *const *v
     const int nl = precond ? P.nlev : 1;
     for (int l = 0; l < nl; ++l) {
         const LevelHost &L = C->levels[l];
-        std::snprintf(buf, sizeof(buf), "|%d %d %p %p %p %p %p %p %p %p %p %p %p %p %p %p", L.nbricks,
+        std::snprintf(buf, sizeof(buf), "|%d %d %p %p %p %p %p %p %p %p %p %p %p %p %p %p %p", L.nbricks,
                       L.any_coarsest ? 1 : 0, (const void *)L.desc.bmap, L.dir, L.gx, L.gy, L.gz, L.idiag, L.rhs, L.z,
-                      L.tmp, L.res, (const void *)L.desc.bxyz, (const void *)L.desc.bnbr, (const void *)L.desc.bgrid,
-                      (const void *)L.desc.gl);
+                      L.tmp, L.res, (const void *)L.desc.bxyz, (const void *)L.desc.meta, (const void *)L.desc.gl,
+                      (const void *)L.desc.child64, (const void *)L.desc.parent8);
         k += buf;
     }
     return k;
@@ -1195,22 +1238,30 @@ static int project_stages(shkz_ctx *C, const shkz_grid *grids, int ngrids, const
             // plain CG: no coarse solve anywhere
             L.any_coarsest = false;
             for (GridLevel &gl : L.gl) gl.coarsest = 0;
+            for (BrickMeta &m : L.meta) m.flags = 0;
         }
         max_bricks = std::max(max_bricks, (int)brick_grid(L.nbricks));
-        // upload the tables: bmap | bxyz | bnbr | bgrid | gl (16-byte aligned)
-        const size_t nmap = L.bmap.size(), nxyz = L.bxyz.size(), nnbr = L.bnbr.size(), ngr = L.bgrid.size();
-        const size_t ngl = L.gl.size() * (sizeof(GridLevel) / sizeof(int));
-        const size_t o_xyz = nmap, o_nbr = o_xyz + nxyz, o_gr = o_nbr + nnbr;
-        const size_t o_gl = (o_gr + ngr + 3) & ~(size_t)3;
-        const size_t total = o_gl + ngl;
+    }
+    for (int l = 0; l + 1 < P.nlev; ++l) link_levels(C->levels[l], C->levels[l + 1]);
+    for (int l = 0; l < P.nlev; ++l) {
+        LevelHost &L = C->levels[l];
+        // upload the tables: bmap | bxyz | meta | gl | child64 | parent8, each 32-byte aligned
+        auto al = [](size_t n) { return (n + 7) & ~(size_t)7; };
+        const size_t o_xyz = al(L.bmap.size());
+        const size_t o_meta = o_xyz + al(L.bxyz.size());
+        const size_t o_gl = o_meta + al(L.meta.size() * (sizeof(BrickMeta) / sizeof(int)));
+        const size_t o_c64 = o_gl + al(L.gl.size() * (sizeof(GridLevel) / sizeof(int)));
+        const size_t o_p8 = o_c64 + al(L.child64.size());
+        const size_t total = o_p8 + al(L.parent8.size()) + 8;
         int *meta = nullptr;
         SHKZ_TRY(arena_get(AR, "meta" + std::to_string(l), total, &meta));
         std::vector<int> host(total, -1);
         std::copy(L.bmap.begin(), L.bmap.end(), host.begin());
         std::copy(L.bxyz.begin(), L.bxyz.end(), host.begin() + o_xyz);
-        std::copy(L.bnbr.begin(), L.bnbr.end(), host.begin() + o_nbr);
-        std::copy(L.bgrid.begin(), L.bgrid.end(), host.begin() + o_gr);
+        std::memcpy(host.data() + o_meta, L.meta.data(), L.meta.size() * sizeof(BrickMeta));
         std::memcpy(host.data() + o_gl, L.gl.data(), L.gl.size() * sizeof(GridLevel));
+        std::copy(L.child64.begin(), L.child64.end(), host.begin() + o_c64);
+        std::copy(L.parent8.begin(), L.parent8.end(), host.begin() + o_p8);
         // synchronous: the host vector goes out of scope
         SHKZ_HIP(hipMemcpyAsync(meta, host.data(), host.size() * sizeof(int), hipMemcpyHostToDevice, s));
         SHKZ_HIP(hipStreamSynchronize(s));
@@ -1218,9 +1269,10 @@ static int project_stages(shkz_ctx *C, const shkz_grid *grids, int ngrids, const
         L.desc.ngrids = ngrids;
         L.desc.bmap = meta;
         L.desc.bxyz = meta + o_xyz;
-        L.desc.bnbr = meta + o_nbr;
-        L.desc.bgrid = meta + o_gr;
+        L.desc.meta = reinterpret_cast<const BrickMeta *>(meta + o_meta);
         L.desc.gl = reinterpret_cast<const GridLevel *>(meta + o_gl);
+        L.desc.child64 = L.child64.empty() ? nullptr : meta + o_c64;
+        L.desc.parent8 = L.parent8.empty() ? nullptr : meta + o_p8;
     }
     if (C->trace) {
         std::fprintf(stderr, "[shkz] %d grid(s), levels:", ngrids);
