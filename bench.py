@@ -380,6 +380,7 @@ def main():
     # ---- timed region: whole projections, graphs on, no per-kernel events
     lat = []
     results = []
+    stages = []
     st.synchronize()
     t0 = time.perf_counter()
     for i in range(args.steps):
@@ -387,6 +388,7 @@ def main():
         r = step()
         lat.append((time.perf_counter() - ts) * 1e3)
         results.append((r.iterations, r.outer_iterations, r.reresid_inf, r.converged))
+        stages.append((r.t_solid_fluid_fractions, r.t_build_highres_linsystem, r.t_linsolve, r.t_update_velocity))
     st.synchronize()
     elapsed = time.perf_counter() - t0
     its = [x[0] for x in results]
@@ -422,6 +424,8 @@ def main():
                    "levels": int(r.n_levels), "iterations": int(np.median(its)),
                    "outer_iterations": int(r.outer_iterations), "reresid_inf": float(max(x[2] for x in results)),
                    "mg_controls": mg_controls(r),
+                   "stage_ms_median": dict(zip(("fractions_labelling", "build", "solve", "update"),
+                                               (round(float(x), 3) for x in np.median(np.array(stages), axis=0)))),
                    "cell_updates_per_s": round(value * n ** 3, 1)},
         "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3), "p99": round(float(np.percentile(lat, 99)), 3)},
         "roofline": roofline,

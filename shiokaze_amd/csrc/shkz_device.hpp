@@ -155,6 +155,30 @@ __device__ __forceinline__ double block_max(double v, double *lds) {
     return s;
 }
 
+// Two block-wide reductions in one LDS round: a by max (AMAX) or sum, b by sum.  Same operations
+// in the same order as block_max / block_sum, so the results are identical to theirs.  lds holds
+// 32 doubles.
+template <bool AMAX>
+__device__ __forceinline__ void block_reduce2(double &a, double &b, double *lds) {
+    a = AMAX ? wave_max(a) : wave_sum(a);
+    b = wave_sum(b);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) {
+        lds[wid] = a;
+        lds[16 + wid] = b;
+    }
+    __syncthreads();
+    double sa = 0.0, sb = 0.0;
+    const int nw = blockDim.x >> 6;
+    for (int i = 0; i < nw; ++i) {
+        sa = AMAX ? fmax(sa, lds[i]) : sa + lds[i];
+        sb += lds[16 + i];
+    }
+    a = sa;
+    b = sb;
+}
+
 __device__ __forceinline__ void store_partials(const RedBuf &R, int brick, double a, double b) {
     if (threadIdx.x == 0) {
         R.partials[2 * brick + 0] = a;

@@ -341,7 +341,7 @@ __device__ __forceinline__ int grid_of(const LevelDesc &L, int br) { return L.ng
 template <class T>
 __global__ __launch_bounds__(BLOCK) void k_defect(LevelDesc L, OpDesc<T> A, const T *__restrict__ b,
                                                   const T *__restrict__ x, T *__restrict__ r, RedBuf R) {
-    __shared__ double lds[16];
+    __shared__ double lds[32];
     const int br = brick_or_idle(L);
     if (br < 0) return;
     const int t = threadIdx.x;
@@ -356,8 +356,8 @@ __global__ __launch_bounds__(BLOCK) void k_defect(LevelDesc L, OpDesc<T> A, cons
         out = b[c] - apply_row(A, x, c, idx, dirc, x[c], diag);
     }
     r[c] = out;
-    const double mx = block_max(fabs((double)out), lds);
-    const double sq = block_sum((double)out * (double)out, lds);
+    double mx = fabs((double)out), sq = (double)out * (double)out;
+    block_reduce2<true>(mx, sq, lds);
     store_partials(R, br, mx, sq);
 }
 
@@ -365,15 +365,15 @@ __global__ __launch_bounds__(BLOCK) void k_defect(LevelDesc L, OpDesc<T> A, cons
 // partials (finalized by FIN_INNER_INIT).
 __global__ __launch_bounds__(BLOCK) void k_inner_init(LevelDesc L, const double *__restrict__ r64, float *__restrict__ r32,
                                                       float *__restrict__ corr, RedBuf R) {
-    __shared__ double lds[16];
+    __shared__ double lds[32];
     const int br = brick_or_idle(L);
     if (br < 0) return;
     const long c = (long)br * BRICK_CELLS + threadIdx.x;
     const float v = (float)r64[c];
     r32[c] = v;
     corr[c] = 0.0f;
-    const double mx = block_max(fabs((double)v), lds);
-    const double sq = block_sum((double)v * (double)v, lds);
+    double mx = fabs((double)v), sq = (double)v * (double)v;
+    block_reduce2<true>(mx, sq, lds);
     store_partials(R, br, mx, sq);
 }
 
@@ -382,7 +382,7 @@ template <class T>
 __global__ __launch_bounds__(BLOCK) void k_apply_dot(LevelDesc L, OpDesc<T> A, const T *__restrict__ p,
                                                      T *__restrict__ q, const SolveScalars *S, const BatchCtl *ctl,
                                                      RedBuf R) {
-    __shared__ double lds[16];
+    __shared__ double lds[32];
     const int br = brick_or_idle(L);
     if (br < 0) return;
     const BrickMeta m = L.meta[br];
@@ -412,7 +412,7 @@ template <class T>
 __global__ __launch_bounds__(BLOCK) void k_cg_update(LevelDesc L, T *__restrict__ x, T *__restrict__ r,
                                                      const T *__restrict__ p, const T *__restrict__ q,
                                                      const SolveScalars *S, const BatchCtl *ctl, RedBuf R) {
-    __shared__ double lds[16];
+    __shared__ double lds[32];
     const int br = brick_or_idle(L);
     if (br < 0) return;
     const SolveScalars &Sg = S[grid_of(L, br)];
@@ -422,8 +422,8 @@ __global__ __launch_bounds__(BLOCK) void k_cg_update(LevelDesc L, T *__restrict_
     x[c] += alpha * p[c];
     const T rv = r[c] - alpha * q[c];
     r[c] = rv;
-    const double mx = block_max(fabs((double)rv), lds);
-    const double sq = block_sum((double)rv * (double)rv, lds);
+    double mx = fabs((double)rv), sq = (double)rv * (double)rv;
+    block_reduce2<true>(mx, sq, lds);
     store_partials(R, br, mx, sq);
 }
 
@@ -431,7 +431,7 @@ __global__ __launch_bounds__(BLOCK) void k_cg_update(LevelDesc L, T *__restrict_
 template <class T>
 __global__ __launch_bounds__(BLOCK) void k_dot_rz(LevelDesc L, const T *__restrict__ r, const T *__restrict__ z,
                                                   const SolveScalars *S, const BatchCtl *ctl, RedBuf R) {
-    __shared__ double lds[16];
+    __shared__ double lds[32];
     const int br = brick_or_idle(L);
     if (br < 0) return;
     if (ctl->all_done || S[grid_of(L, br)].done) return;
@@ -475,6 +475,16 @@ enum FinOp {
                           // defect converged, the iterations ran out or the defect fell by < 10 %
 };
 
+// Fresh per-grid scalars (norm, iteration limit; everything else 0) for n grids.
+__global__ void k_init_scalars(SolveScalars *S, int n, int norm, int maxit) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n) return;
+    SolveScalars v{};
+    v.norm = norm;
+    v.maxit = maxit;
+    S[g] = v;
+}
+
 // Resets a control block for a new Krylov solve over ngrids grids (a kernel, so that it orders
 // with the work around it and needs no host buffer).
 __global__ void k_init_ctl(BatchCtl *ctl, int ngrids) {
@@ -499,7 +509,7 @@ __device__ __forceinline__ void mark_done(SolveScalars &S, BatchCtl *ctl) {
 __global__ __launch_bounds__(1024) void k_finalize(const double *__restrict__ partials, LevelDesc L0,
                                                    SolveScalars *Sarr, BatchCtl *ctl, int op, double param, int iparam,
                                                    const SolveScalars *outer) {
-    __shared__ double lds[16];
+    __shared__ double lds[32];
     const int g = blockIdx.x;
     SolveScalars &S = Sarr[g];
     const bool always = (op == FIN_DEFECT_FIRST || op == FIN_DEFECT || op == FIN_INNER_INIT);
@@ -525,8 +535,8 @@ __global__ __launch_bounds__(1024) void k_finalize(const double *__restrict__ pa
         a = a_is_max ? fmax(a, v.x) : a + v.x;
         b += v.y;
     }
-    a = a_is_max ? block_max(a, lds) : block_sum(a, lds);
-    b = block_sum(b, lds);
+    if (a_is_max) block_reduce2<true>(a, b, lds);
+    else block_reduce2<false>(a, b, lds);
     if (threadIdx.x != 0) return;
     switch (op) {
     case FIN_PQ: {
@@ -676,7 +686,7 @@ template <class T, int FINEST, int DOT>
 __global__ __launch_bounds__(BLOCK) void k_jacobi(LevelDesc L, OpDesc<T> A, const T *__restrict__ idiag,
                                                   const T *__restrict__ rhs, const T *__restrict__ zin,
                                                   T *__restrict__ zout, T omega, Gate Q, RedBuf R) {
-    __shared__ double lds[16];
+    __shared__ double lds[32];
     const int br = brick_or_idle(L);
     if (br < 0) return;
     const BrickMeta m = L.meta[br];
@@ -818,7 +828,7 @@ template <class T, int DOT>
 __global__ __launch_bounds__(BLOCK) void k_coarse_solve(LevelDesc L, OpDesc<T> A, const T *__restrict__ rhs,
                                                         T *__restrict__ z, int sweeps, T omega, Gate Q, RedBuf R) {
     __shared__ T zs[2][BRICK_CELLS];
-    __shared__ double lds[16];
+    __shared__ double lds[32];
     if (all_done(Q)) return;
     const int g = blockIdx.x;
     const GridLevel G = L.gl[g];

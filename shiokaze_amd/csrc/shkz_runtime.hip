@@ -379,6 +379,11 @@ struct shkz_ctx {
     SolveScalars *host_scalars = nullptr;
     int host_scalars_cap = 0;
     BatchCtl *host_ctl = nullptr;
+    // pinned staging for the per-call transfers (brick flags down, level tables up, counts down)
+    struct Pinned {
+        void *p = nullptr;
+        size_t cap = 0;
+    } pin_flags, pin_meta, pin_counts;
     // timing
     bool timing = false;
     std::vector<std::string> kernel_names;
@@ -594,6 +599,21 @@ static int ensure_host_scalars(shkz_ctx *C, int n) {
     return SHKZ_OK;
 }
 
+// Grows a pinned staging buffer (callers only grow it between calls' synchronisations, so no
+// copy is in flight from the old one).
+static int ensure_pinned(shkz_ctx::Pinned &b, size_t bytes, void **out) {
+    if (b.cap < bytes) {
+        if (b.p) (void)hipHostFree(b.p);
+        b.p = nullptr;
+        b.cap = 0;
+        const size_t want = std::max<size_t>(bytes + bytes / 4, 4096);
+        SHKZ_HIP(hipHostMalloc(&b.p, want, hipHostMallocDefault));
+        b.cap = want;
+    }
+    *out = b.p;
+    return SHKZ_OK;
+}
+
 // Reads n per-grid scalars (and optionally a control block) back to the host.
 static int read_scalars(shkz_ctx *C, hipStream_t s, const SolveScalars *dS, int n, std::vector<SolveScalars> &out,
                         const BatchCtl *dctl = nullptr, BatchCtl *ctl_out = nullptr) {
@@ -605,20 +625,6 @@ static int read_scalars(shkz_ctx *C, hipStream_t s, const SolveScalars *dS, int 
     out.assign(C->host_scalars, C->host_scalars + n);
     if (ctl_out) *ctl_out = *C->host_ctl;
     if (C->timing) resolve_timing(C);
-    return SHKZ_OK;
-}
-
-// Uploads n fresh scalars (norm, maxit) and waits for the copy (the pinned buffer is reused).
-static int upload_scalars(shkz_ctx *C, hipStream_t s, SolveScalars *dS, int n, int norm, int maxit) {
-    SHKZ_TRY(ensure_host_scalars(C, n));
-    for (int g = 0; g < n; ++g) {
-        SolveScalars h{};
-        h.norm = norm;
-        h.maxit = maxit;
-        C->host_scalars[g] = h;
-    }
-    SHKZ_HIP(hipMemcpyAsync(dS, C->host_scalars, (size_t)n * sizeof(SolveScalars), hipMemcpyHostToDevice, s));
-    SHKZ_HIP(hipStreamSynchronize(s));
     return SHKZ_OK;
 }
 
@@ -828,7 +834,8 @@ static int solve_phase(shkz_ctx *C, Launcher &K, Problem &P, const shkz_params *
     status.assign(ngrids, SHKZ_OK);
     std::vector<SolveScalars> hs(ngrids), hi, hf;
     if (F0.nbricks > 0) {
-        SHKZ_TRY(upload_scalars(C, s, P.S_out, ngrids, norm, maxit));
+        K.go("init_scalars", -1, 0, 0, dim3((unsigned)((ngrids + 255) / 256)), dim3(256), k_init_scalars, P.S_out,
+             ngrids, norm, maxit);
         K.go("init_ctl", -1, 0, 0, dim3(1), dim3(64), k_init_ctl, P.ctl, ngrids);
     }
     if (F0.nbricks == 0) {
@@ -852,7 +859,8 @@ static int solve_phase(shkz_ctx *C, Launcher &K, Problem &P, const shkz_params *
         // FP64 defect correction around FP32 MGPCG solves, per grid: the outer scalars decide on
         // the device when a grid is done (converged, out of iterations, or no longer reduced)
         const double eta = P.eta;
-        SHKZ_TRY(upload_scalars(C, s, P.S_in, ngrids, norm, maxit));
+        K.go("init_scalars", -1, 0, 0, dim3((unsigned)((ngrids + 255) / 256)), dim3(256), k_init_scalars, P.S_in,
+             ngrids, norm, maxit);
         for (int outer = 0; outer < 64; ++outer) {
             K.go("defect", 0, 7 * 8, 0, gb, bb, k_defect<double>, F0.desc, A64, (const double *)P.b64,
                  (const double *)P.x64, P.r64, P.red);
@@ -1193,11 +1201,19 @@ static int project_stages(shkz_ctx *C, const shkz_grid *grids, int ngrids, const
         J.bflag = bflag_all + bflag_off[g];
         K.go("flags", -1, 0, (double)J.M * ((G.phi64 ? 8 : 4) + 1 + (grid->solid ? 4 : 0)), dim3((unsigned)npos),
              dim3(BLOCK), k_flags, G, J.l0.nbx, J.l0.nby, J.l0.nbz, J.cflag, J.bflag);
-        J.l0.active.resize(npos);
-        SHKZ_HIP(hipMemcpyAsync(J.l0.active.data(), J.bflag, npos, hipMemcpyDeviceToHost, s));
     }
-    SHKZ_HIP(hipStreamSynchronize(s));
-    SHKZ_HIP(hipGetLastError());
+    {
+        void *hflags = nullptr;
+        SHKZ_TRY(ensure_pinned(C->pin_flags, bflag_total, &hflags));
+        SHKZ_HIP(hipMemcpyAsync(hflags, bflag_all, bflag_total, hipMemcpyDeviceToHost, s));
+        SHKZ_HIP(hipStreamSynchronize(s));
+        SHKZ_HIP(hipGetLastError());
+        for (int g = 0; g < ngrids; ++g) {
+            GridJob &J = jobs[g];
+            const uint8_t *src = (const uint8_t *)hflags + bflag_off[g];
+            J.l0.active.assign(src, src + (size_t)J.l0.nbx * J.l0.nby * J.l0.nbz);
+        }
+    }
     const double t_frac = now_ms();
 
     // ---------------------------------------------------------------- hierarchy (host)
@@ -1243,36 +1259,53 @@ static int project_stages(shkz_ctx *C, const shkz_grid *grids, int ngrids, const
         max_bricks = std::max(max_bricks, (int)brick_grid(L.nbricks));
     }
     for (int l = 0; l + 1 < P.nlev; ++l) link_levels(C->levels[l], C->levels[l + 1]);
-    for (int l = 0; l < P.nlev; ++l) {
-        LevelHost &L = C->levels[l];
-        // upload the tables: bmap | bxyz | meta | gl | child64 | parent8, each 32-byte aligned
+    {
+        // every level's tables in one upload: per level bmap | bxyz | meta | gl | child64 | parent8,
+        // each 32-byte aligned
         auto al = [](size_t n) { return (n + 7) & ~(size_t)7; };
-        const size_t o_xyz = al(L.bmap.size());
-        const size_t o_meta = o_xyz + al(L.bxyz.size());
-        const size_t o_gl = o_meta + al(L.meta.size() * (sizeof(BrickMeta) / sizeof(int)));
-        const size_t o_c64 = o_gl + al(L.gl.size() * (sizeof(GridLevel) / sizeof(int)));
-        const size_t o_p8 = o_c64 + al(L.child64.size());
-        const size_t total = o_p8 + al(L.parent8.size()) + 8;
-        int *meta = nullptr;
-        SHKZ_TRY(arena_get(AR, "meta" + std::to_string(l), total, &meta));
-        std::vector<int> host(total, -1);
-        std::copy(L.bmap.begin(), L.bmap.end(), host.begin());
-        std::copy(L.bxyz.begin(), L.bxyz.end(), host.begin() + o_xyz);
-        std::memcpy(host.data() + o_meta, L.meta.data(), L.meta.size() * sizeof(BrickMeta));
-        std::memcpy(host.data() + o_gl, L.gl.data(), L.gl.size() * sizeof(GridLevel));
-        std::copy(L.child64.begin(), L.child64.end(), host.begin() + o_c64);
-        std::copy(L.parent8.begin(), L.parent8.end(), host.begin() + o_p8);
-        // synchronous: the host vector goes out of scope
-        SHKZ_HIP(hipMemcpyAsync(meta, host.data(), host.size() * sizeof(int), hipMemcpyHostToDevice, s));
-        SHKZ_HIP(hipStreamSynchronize(s));
-        L.desc.nbricks = L.nbricks;
-        L.desc.ngrids = ngrids;
-        L.desc.bmap = meta;
-        L.desc.bxyz = meta + o_xyz;
-        L.desc.meta = reinterpret_cast<const BrickMeta *>(meta + o_meta);
-        L.desc.gl = reinterpret_cast<const GridLevel *>(meta + o_gl);
-        L.desc.child64 = L.child64.empty() ? nullptr : meta + o_c64;
-        L.desc.parent8 = L.parent8.empty() ? nullptr : meta + o_p8;
+        struct Offs {
+            size_t base, xyz, meta, gl, c64, p8;
+        };
+        std::vector<Offs> offs(P.nlev);
+        size_t total = 0;
+        for (int l = 0; l < P.nlev; ++l) {
+            const LevelHost &L = C->levels[l];
+            Offs &o = offs[l];
+            o.base = total;
+            o.xyz = o.base + al(L.bmap.size());
+            o.meta = o.xyz + al(L.bxyz.size());
+            o.gl = o.meta + al(L.meta.size() * (sizeof(BrickMeta) / sizeof(int)));
+            o.c64 = o.gl + al(L.gl.size() * (sizeof(GridLevel) / sizeof(int)));
+            o.p8 = o.c64 + al(L.child64.size());
+            total = o.p8 + al(L.parent8.size());
+        }
+        total += 8;
+        int *dmeta = nullptr;
+        SHKZ_TRY(arena_get(AR, "meta", total, &dmeta));
+        void *hmeta = nullptr;
+        SHKZ_TRY(ensure_pinned(C->pin_meta, total * sizeof(int), &hmeta));
+        int *host = (int *)hmeta;
+        std::fill(host, host + total, -1);
+        for (int l = 0; l < P.nlev; ++l) {
+            LevelHost &L = C->levels[l];
+            const Offs &o = offs[l];
+            std::copy(L.bmap.begin(), L.bmap.end(), host + o.base);
+            std::copy(L.bxyz.begin(), L.bxyz.end(), host + o.xyz);
+            std::memcpy(host + o.meta, L.meta.data(), L.meta.size() * sizeof(BrickMeta));
+            std::memcpy(host + o.gl, L.gl.data(), L.gl.size() * sizeof(GridLevel));
+            std::copy(L.child64.begin(), L.child64.end(), host + o.c64);
+            std::copy(L.parent8.begin(), L.parent8.end(), host + o.p8);
+            L.desc.nbricks = L.nbricks;
+            L.desc.ngrids = ngrids;
+            L.desc.bmap = dmeta + o.base;
+            L.desc.bxyz = dmeta + o.xyz;
+            L.desc.meta = reinterpret_cast<const BrickMeta *>(dmeta + o.meta);
+            L.desc.gl = reinterpret_cast<const GridLevel *>(dmeta + o.gl);
+            L.desc.child64 = L.child64.empty() ? nullptr : dmeta + o.c64;
+            L.desc.parent8 = L.parent8.empty() ? nullptr : dmeta + o.p8;
+        }
+        // the pinned staging stays untouched until this call's next synchronisation
+        SHKZ_HIP(hipMemcpyAsync(dmeta, host, total * sizeof(int), hipMemcpyHostToDevice, s));
     }
     if (C->trace) {
         std::fprintf(stderr, "[shkz] %d grid(s), levels:", ngrids);
@@ -1425,11 +1458,17 @@ static int project_stages(shkz_ctx *C, const shkz_grid *grids, int ngrids, const
                 K.go("count", -1, 0, 0, g, b, k_brick_counts<float>, L.desc, (const float *)L.dir, cnt + off);
             off += L.nbricks;
         }
-        std::vector<int> hc(std::max(total, 1), 0);
-        if (total) SHKZ_HIP(hipMemcpyAsync(hc.data(), cnt, (size_t)total * sizeof(int), hipMemcpyDeviceToHost, s));
+        void *hcounts = nullptr;
+        SHKZ_TRY(ensure_pinned(C->pin_counts, (size_t)std::max(total, 1) * sizeof(int), &hcounts));
+        if (total) SHKZ_HIP(hipMemcpyAsync(hcounts, cnt, (size_t)total * sizeof(int), hipMemcpyDeviceToHost, s));
+    }
+    // the counts arrive with the next synchronisation; the kernel timers' byte model needs them
+    // before the solve's first read-back, the results only at the end
+    auto take_counts = [&]() -> int {
         SHKZ_HIP(hipStreamSynchronize(s));
         SHKZ_HIP(hipGetLastError());
-        off = 0;
+        const int *hc = (const int *)C->pin_counts.p;
+        int off = 0;
         for (int l = 0; l < P.nlev; ++l) {
             LevelHost &L = C->levels[l];
             L.unknowns_grid.assign(ngrids, 0);
@@ -1447,7 +1486,9 @@ static int project_stages(shkz_ctx *C, const shkz_grid *grids, int ngrids, const
             for (int l = 0; l < P.nlev; ++l) depth += C->levels[l].gl[g].brick_end > C->levels[l].gl[g].brick_start;
             res[g].n_levels = depth;
         }
-    }
+        return SHKZ_OK;
+    };
+    if (C->timing || exp) SHKZ_TRY(take_counts());
     if (exp) {
         const GridJob &J = jobs[0];
         K.go("export_system", -1, 0, 0, dim3((unsigned)((J.M + 255) / 256)), dim3(256), k_export_system, J.G, F0.desc,
@@ -1476,6 +1517,7 @@ static int project_stages(shkz_ctx *C, const shkz_grid *grids, int ngrids, const
     }
     SolveOut SO;
     SHKZ_TRY(solve_phase(C, K, P, prm, multigrid, SO, res));
+    if (!C->timing) SHKZ_TRY(take_counts());
     for (int g = 0; g < ngrids; ++g) {
         res[g].mg_pre_sweeps = P.pre;
         res[g].mg_post_sweeps = P.post;
@@ -1793,6 +1835,8 @@ void shkz_destroy(shkz_ctx *C) {
     C->own_scenes.clear();
     if (C->host_scalars) (void)hipHostFree(C->host_scalars);
     if (C->host_ctl) (void)hipHostFree(C->host_ctl);
+    for (shkz_ctx::Pinned *b : {&C->pin_flags, &C->pin_meta, &C->pin_counts})
+        if (b->p) (void)hipHostFree(b->p);
     delete C;
 }
 
