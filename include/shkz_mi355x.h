@@ -206,6 +206,39 @@ int shkz_release_scene(shkz_ctx *ctx, uint64_t scene_id);
 /* Device bytes currently held by the context. */
 uint64_t shkz_device_bytes(const shkz_ctx *ctx);
 
+/* ------------------------------------------------------- 2D (macpressuresolver2)
+ * macproject2_interface::project (include/shiokaze/projection/macproject2_interface.h:67-72, implemented
+ * by macpressuresolver2::project, src/projection/macpressuresolver2.cpp:48-232) on dense 2D arrays
+ * (y fastest): cells (w, h) index i*h + j; u (w+1, h); v (w, h+1); nodal solid (w+1, h+1).  Face
+ * areas follow macutility2 (src/utility/macutility2.cpp:89-125: 1 - fraction of the two solid
+ * samples at the ends of the face); assembly, solver and update are the 2D module's.  The
+ * update writes 0 to every face the reference sets to 0 or switches off; the reference keeps a
+ * closed boundary face with liquid inside active only while its velocity points out of the
+ * domain (:224-226), which a caller of the dense API applies to its own activity flags. */
+typedef struct shkz_grid2d {
+    int32_t w, h;
+    double dx;                    /* 0 = min(1/w, 1/h) */
+    double dt;
+    const void *fluid;
+    int32_t fluid_dtype, solid_dtype;
+    const void *solid;            /* nodal (w+1, h+1) or NULL */
+    float *u, *v;                 /* faces, updated in place */
+    const uint8_t *active_u, *active_v;
+    float *pressure;              /* optional outputs (cells) */
+    double *pressure64;
+    uint64_t scene_id;
+} shkz_grid2d;
+
+int shkz_project2d(shkz_ctx *ctx, const shkz_grid2d *grid, const shkz_params *params, shkz_result *result);
+int shkz_project2d_device(shkz_ctx *ctx, const shkz_grid2d *grid, const shkz_params *params, shkz_result *result,
+                          void *stream);
+/* A batch of 2D grids (device arrays), like shkz_project_batch_device. */
+int shkz_project2d_batch_device(shkz_ctx *ctx, const shkz_grid2d *grids, int ngrids, const shkz_params *params,
+                                shkz_result *results, void *stream);
+/* The assembled FP64 2D system on dense cell arrays (host): diag, low-face couplings, rhs. */
+int shkz_export_system2d(shkz_ctx *ctx, const shkz_grid2d *grid, const shkz_params *params, double *diag, double *gx,
+                         double *gy, double *rhs);
+
 /* ------------------------------------------------------- server (continuous batching)
  * No counterpart in the reference (its driver projects one grid per time step on one thread,
  * ui.cpp:703-710).  A server owns `executors` workspaces, each with its own HIP stream; every
@@ -297,6 +330,11 @@ typedef struct shkz_scene_desc {
  * *has_solid tells whether the scene has a solid), u, v, w (faces, may be NULL). */
 int shkz_scene_fill(const shkz_scene_desc *desc, void *fluid, float *solid, int32_t *has_solid,
                     float *u, float *v, float *w);
+
+/* 2D scenes for shkz_grid2d workloads: the same seeded shapes in the plane z = sphere centre
+ * (circles, the pool, the box's x-y rectangle); fluid (w, h), solid nodal (w+1, h+1) float (may be
+ * NULL), u (w+1, h) and v (w, h+1) (may be NULL).  d and shift[2] are ignored. */
+int shkz_scene_fill2d(const shkz_scene_desc *desc, void *fluid, float *solid, int32_t *has_solid, float *u, float *v);
 
 /* The same scenes generated on the device into device buffers, on `stream` (synthetic inputs
  * for serving workloads, where host generation would dominate).  The level sets equal

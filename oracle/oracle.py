@@ -32,6 +32,12 @@ class OGrid(ctypes.Structure):
                 ("solid_f64", ctypes.c_int), ("vel", _vp * 3), ("active", _vp * 3)]
 
 
+class OGrid2(ctypes.Structure):
+    _fields_ = [("w", ctypes.c_int), ("h", ctypes.c_int), ("dx", ctypes.c_double), ("dt", ctypes.c_double),
+                ("fluid", _vp), ("fluid_f64", ctypes.c_int), ("solid", _vp), ("solid_f64", ctypes.c_int),
+                ("vel", _vp * 2), ("active", _vp * 2)]
+
+
 class OParams(ctypes.Structure):
     _fields_ = [("residual", ctypes.c_double), ("max_iterations", ctypes.c_int), ("mic_mode", ctypes.c_int),
                 ("mic_omega", ctypes.c_double), ("mic_min_diag_ratio", ctypes.c_double),
@@ -85,6 +91,17 @@ def lib():
         L.oracle_divergence.argtypes = [ctypes.POINTER(OGrid), _vp, _vp * 3, _vp]
         L.oracle_divergence.restype = ctypes.c_double
         L.oracle_project.argtypes = [ctypes.POINTER(OGrid), ctypes.POINTER(OParams), _vp, _vp, ctypes.POINTER(OResult)]
+        L.oracle2_face_fractions.argtypes = [ctypes.POINTER(OGrid2), ctypes.POINTER(OParams), _vp * 2, _vp * 2]
+        L.oracle2_surface_tension.argtypes = [ctypes.POINTER(OGrid2), ctypes.c_double, _vp * 2]
+        L.oracle2_label.argtypes = [ctypes.POINTER(OGrid2), _vp * 2, _vp * 2, _vp]
+        L.oracle2_label.restype = ctypes.c_int64
+        L.oracle2_assemble_count.argtypes = [ctypes.POINTER(OGrid2), _vp, _vp * 2, _vp * 2]
+        L.oracle2_assemble_count.restype = ctypes.c_int64
+        L.oracle2_assemble.argtypes = [ctypes.POINTER(OGrid2), _vp, _vp * 2, _vp * 2, _vp, _vp, _vp, _vp]
+        L.oracle2_velocity_update.argtypes = [ctypes.POINTER(OGrid2), _vp, _vp * 2, _vp * 2, _vp]
+        L.oracle2_divergence.argtypes = [ctypes.POINTER(OGrid2), _vp, _vp * 2, _vp]
+        L.oracle2_divergence.restype = ctypes.c_double
+        L.oracle2_project.argtypes = [ctypes.POINTER(OGrid2), ctypes.POINTER(OParams), _vp, _vp, ctypes.POINTER(OResult)]
         L.oracle_project.restype = ctypes.c_int
         _lib = L
     return _lib
@@ -231,3 +248,83 @@ def gather(sys, cells):
     mask = m >= 0
     x[m[mask]] = cells[mask]
     return x
+
+
+# ---------------------------------------------------------------------------------------- 2D
+def make_grid2(scene, dx=None, active=None) -> tuple[OGrid2, list]:
+    """OGrid2 over a shiokaze_amd.Scene2D-like object (fluid, solid, u, v, w, h, dt)."""
+    keep = [scene.fluid, scene.solid, scene.u, scene.v]
+    g = OGrid2(w=scene.w, h=scene.h, dx=dx if dx is not None else min(1.0 / scene.w, 1.0 / scene.h), dt=scene.dt,
+               fluid=_p(scene.fluid), fluid_f64=int(scene.fluid.dtype == np.float64), solid=_p(scene.solid),
+               solid_f64=int(scene.solid is not None and scene.solid.dtype == np.float64))
+    g.vel[0], g.vel[1] = _p(scene.u), _p(scene.v)
+    if active is not None:
+        for a in range(2):
+            g.active[a] = _p(active[a])
+        keep += list(active)
+    return g, keep
+
+
+def face_shapes2(w, h):
+    return [(w + 1, h), (w, h + 1)]
+
+
+def fractions2(scene, params=None):
+    g, keep = make_grid2(scene)
+    prm = params or default_params()
+    area = [np.zeros(s, np.float32) for s in face_shapes2(scene.w, scene.h)]
+    rho = [np.zeros(s, np.float32) for s in face_shapes2(scene.w, scene.h)]
+    A = (_vp * 2)(*[a.ctypes.data for a in area])
+    R = (_vp * 2)(*[a.ctypes.data for a in rho])
+    lib().oracle2_face_fractions(ctypes.byref(g), ctypes.byref(prm), A, R)
+    return area, rho
+
+
+def assemble2(scene, params=None):
+    """2D labelling + CSR assembly.  Returns dict(index_map, rowptr, col, val, b, area, rho, n)."""
+    prm = params or default_params()
+    area, rho = fractions2(scene, prm)
+    g, keep = make_grid2(scene)
+    A = (_vp * 2)(*[a.ctypes.data for a in area])
+    R = (_vp * 2)(*[a.ctypes.data for a in rho])
+    if prm.surface_tension:
+        lib().oracle2_surface_tension(ctypes.byref(g), prm.surface_tension, R)
+    index_map = np.empty((scene.w, scene.h), np.int64)
+    n = lib().oracle2_label(ctypes.byref(g), A, R, index_map.ctypes.data)
+    nnz = lib().oracle2_assemble_count(ctypes.byref(g), index_map.ctypes.data, A, R)
+    rowptr = np.empty(n + 1, np.int64)
+    col = np.empty(max(nnz, 1), np.int64)
+    val = np.empty(max(nnz, 1), np.float64)
+    b = np.empty(max(n, 1), np.float64)
+    lib().oracle2_assemble(ctypes.byref(g), index_map.ctypes.data, A, R, rowptr.ctypes.data, col.ctypes.data,
+                           val.ctypes.data, b.ctypes.data)
+    return dict(index_map=index_map, rowptr=rowptr, col=col[:nnz], val=val[:nnz], b=b[:n], area=area, rho=rho, n=n)
+
+
+def project2(scene, params=None, pressure64=True):
+    """Whole 2D projection (velocities updated in place)."""
+    prm = params or default_params()
+    g, keep = make_grid2(scene)
+    P = np.zeros((scene.w, scene.h), np.float32)
+    P64 = np.zeros((scene.w, scene.h), np.float64) if pressure64 else None
+    r = OResult()
+    lib().oracle2_project(ctypes.byref(g), ctypes.byref(prm), P.ctypes.data, _p(P64), ctypes.byref(r))
+    return r, P, P64
+
+
+def velocity_update2(scene, P, area, rho, active_out=False):
+    g, keep = make_grid2(scene)
+    A = (_vp * 2)(*[a.ctypes.data for a in area])
+    R = (_vp * 2)(*[a.ctypes.data for a in rho])
+    act = [np.zeros(s, np.uint8) for s in face_shapes2(scene.w, scene.h)] if active_out else None
+    AO = (_vp * 2)(*[a.ctypes.data for a in act]) if act else None
+    lib().oracle2_velocity_update(ctypes.byref(g), np.ascontiguousarray(P, np.float32).ctypes.data, A, R, AO)
+    return act
+
+
+def divergence2(scene, index_map, area):
+    g, keep = make_grid2(scene)
+    A = (_vp * 2)(*[a.ctypes.data for a in area])
+    div = np.zeros((scene.w, scene.h), np.float64)
+    mx = lib().oracle2_divergence(ctypes.byref(g), index_map.ctypes.data, A, div.ctypes.data)
+    return mx, div

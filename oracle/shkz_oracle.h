@@ -116,6 +116,35 @@ double oracle_divergence(const oracle_grid *g, const int64_t *index_map, float *
 int oracle_project(const oracle_grid *g, const oracle_params *p, float *pressure,
                    double *pressure64, oracle_result *res);
 
+/* ------------------------------------------------------------------ 2D
+ * The 2D module macpressuresolver2 (src/projection/macpressuresolver2.cpp:48-232) with
+ * macutility2's fractions (src/utility/macutility2.cpp:89-150), on dense arrays (y fastest):
+ *   cell (w,h) idx = i*h + j;  u (w+1,h);  v (w,h+1);  nodal solid (w+1,h+1). */
+typedef struct {
+    int w, h;
+    double dx;              /* min(1/w,1/h) (shape2::dx) */
+    double dt;
+    const void *fluid;
+    int fluid_f64;
+    const void *solid;      /* nodal level set or NULL (walls only) */
+    int solid_f64;
+    float *vel[2];
+    const uint8_t *active[2];
+} oracle_grid2;
+
+void oracle2_face_fractions(const oracle_grid2 *g, const oracle_params *p, float *area[2], float *rho[2]);
+void oracle2_surface_tension(const oracle_grid2 *g, double kappa, float *rho[2]);
+int64_t oracle2_label(const oracle_grid2 *g, float *area[2], float *rho[2], int64_t *index_map);
+int64_t oracle2_assemble_count(const oracle_grid2 *g, const int64_t *index_map, float *area[2], float *rho[2]);
+void oracle2_assemble(const oracle_grid2 *g, const int64_t *index_map, float *area[2], float *rho[2],
+                      int64_t *rowptr, int64_t *col, double *val, double *b);
+/* active_out (optional, per face): 1 where the face stays active after the update. */
+void oracle2_velocity_update(const oracle_grid2 *g, const float *pressure, float *area[2], float *rho[2],
+                             uint8_t *active_out[2]);
+double oracle2_divergence(const oracle_grid2 *g, const int64_t *index_map, float *area[2], double *div);
+int oracle2_project(const oracle_grid2 *g, const oracle_params *p, float *pressure, double *pressure64,
+                    oracle_result *res);
+
 #ifdef __cplusplus
 }
 #endif

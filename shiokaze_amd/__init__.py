@@ -82,6 +82,15 @@ class Result(ctypes.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_}
 
 
+class Grid2D(ctypes.Structure):
+    _fields_ = [
+        ("w", ctypes.c_int32), ("h", ctypes.c_int32), ("dx", ctypes.c_double), ("dt", ctypes.c_double),
+        ("fluid", _vp), ("fluid_dtype", ctypes.c_int32), ("solid_dtype", ctypes.c_int32), ("solid", _vp),
+        ("u", _vp), ("v", _vp), ("active_u", _vp), ("active_v", _vp), ("pressure", _vp), ("pressure64", _vp),
+        ("scene_id", ctypes.c_uint64),
+    ]
+
+
 class ServerConfig(ctypes.Structure):
     _fields_ = [
         ("device", ctypes.c_int32), ("executors", ctypes.c_int32), ("max_batch_grids", ctypes.c_int32),
@@ -124,6 +133,8 @@ EXPORTS = [
     "shkz_export_system", "shkz_project_batch", "shkz_project_batch_device",
     "shkz_server_default_config", "shkz_server_create", "shkz_server_destroy", "shkz_submit", "shkz_poll",
     "shkz_wait", "shkz_server_release_scene", "shkz_server_get_stats", "shkz_scene_fill_device",
+    "shkz_project2d", "shkz_project2d_device", "shkz_project2d_batch_device", "shkz_export_system2d",
+    "shkz_scene_fill2d",
 ]
 
 _lib = None
@@ -169,6 +180,13 @@ def lib():
     L.shkz_server_get_stats.argtypes = [_vp, ctypes.POINTER(ServerStats)]
     L.shkz_scene_fill_device.argtypes = [ctypes.POINTER(SceneDesc), _vp, _vp, ctypes.POINTER(ctypes.c_int32), _vp,
                                          _vp, _vp, _vp]
+    L.shkz_project2d.argtypes = [_vp, ctypes.POINTER(Grid2D), ctypes.POINTER(Params), ctypes.POINTER(Result)]
+    L.shkz_project2d_device.argtypes = [_vp, ctypes.POINTER(Grid2D), ctypes.POINTER(Params), ctypes.POINTER(Result),
+                                        _vp]
+    L.shkz_project2d_batch_device.argtypes = [_vp, ctypes.POINTER(Grid2D), ctypes.c_int, ctypes.POINTER(Params),
+                                              ctypes.POINTER(Result), _vp]
+    L.shkz_export_system2d.argtypes = [_vp, ctypes.POINTER(Grid2D), ctypes.POINTER(Params), _vp, _vp, _vp, _vp]
+    L.shkz_scene_fill2d.argtypes = [ctypes.POINTER(SceneDesc), _vp, _vp, ctypes.POINTER(ctypes.c_int32), _vp, _vp]
     _lib = L
     return L
 
@@ -245,6 +263,46 @@ def make_scene(kind: int, w: int, h: int | None = None, d: int | None = None, se
     if rc != 0:
         raise ShkzError(rc, "shkz_scene_fill failed")
     return Scene(w, h, d, dt, fluid, solid if has.value else None, u, v, wv)
+
+
+@dataclass
+class Scene2D:
+    """Dense host arrays of one 2D projection request (y fastest): fluid (w, h), solid (w+1, h+1),
+    u (w+1, h), v (w, h+1)."""
+    w: int
+    h: int
+    dt: float
+    fluid: np.ndarray
+    solid: np.ndarray | None
+    u: np.ndarray
+    v: np.ndarray
+
+    @property
+    def dx(self) -> float:
+        return min(1.0 / self.w, 1.0 / self.h)
+
+    def copy(self) -> "Scene2D":
+        return Scene2D(self.w, self.h, self.dt, self.fluid.copy(), None if self.solid is None else self.solid.copy(),
+                       self.u.copy(), self.v.copy())
+
+
+def make_scene2d(kind: int, w: int, h: int | None = None, seed: int = 0, dt: float = 1.0 / 120.0,
+                 gravity: float = -9.8, fluid_dtype: int = F64, shift=(0.0, 0.0), velocity_mode: int = 0,
+                 velocity_seed: int = 0) -> Scene2D:
+    """Synthesises a seeded 2D workload with shkz_scene_fill2d."""
+    h = w if h is None else h
+    desc = SceneDesc(kind=kind, w=w, h=h, d=1, seed=seed, dt=dt, gravity=gravity, fluid_dtype=fluid_dtype,
+                     velocity_mode=velocity_mode, velocity_seed=velocity_seed)
+    desc.shift[0], desc.shift[1] = shift[0], shift[1]
+    fluid = np.empty((w, h), np.float64 if fluid_dtype == F64 else np.float32)
+    solid = np.empty((w + 1, h + 1), np.float32)
+    u = np.empty((w + 1, h), np.float32)
+    v = np.empty((w, h + 1), np.float32)
+    has = ctypes.c_int32(0)
+    rc = lib().shkz_scene_fill2d(ctypes.byref(desc), _ptr(fluid), _ptr(solid), ctypes.byref(has), _ptr(u), _ptr(v))
+    if rc != 0:
+        raise ShkzError(rc, "shkz_scene_fill2d failed")
+    return Scene2D(w, h, dt, fluid, solid if has.value else None, u, v)
 
 
 def scene_fill_device(kind: int, w: int, h: int, d: int, fluid, u, v, wv, seed: int = 0, dt: float = 1.0 / 120.0,
@@ -453,6 +511,58 @@ class Context:
         if active is not None:
             g.active_u, g.active_v, g.active_w = (_ptr(a) for a in active)
         return g
+
+    # ------------------------------------------------------------------ 2D
+    @staticmethod
+    def grid2d(scene: Scene2D, scene_id: int = 0, active=None) -> Grid2D:
+        for a in (scene.fluid, scene.u, scene.v):
+            assert a.flags["C_CONTIGUOUS"]
+        g = Grid2D(w=scene.w, h=scene.h, dx=0.0, dt=scene.dt, fluid=_ptr(scene.fluid),
+                   fluid_dtype=F64 if scene.fluid.dtype == np.float64 else F32,
+                   solid_dtype=F64 if (scene.solid is not None and scene.solid.dtype == np.float64) else F32,
+                   solid=_ptr(scene.solid), u=_ptr(scene.u), v=_ptr(scene.v), scene_id=scene_id)
+        if active is not None:
+            g.active_u, g.active_v = (_ptr(a) for a in active)
+        return g
+
+    def project2d(self, scene: Scene2D, params: Params | None = None, pressure: bool = True,
+                  pressure64: bool = False, scene_id: int = 0, active=None, check: bool = True):
+        """Projects a 2D scene's velocities in place (host arrays).  Returns (result, P32, P64)."""
+        g = self.grid2d(scene, scene_id, active)
+        P = np.zeros((scene.w, scene.h), np.float32) if pressure else None
+        P64 = np.zeros((scene.w, scene.h), np.float64) if pressure64 else None
+        g.pressure = _ptr(P)
+        g.pressure64 = _ptr(P64)
+        r = Result()
+        prm = params if params is not None else default_params()
+        rc = self._L.shkz_project2d(self.handle, ctypes.byref(g), ctypes.byref(prm), ctypes.byref(r))
+        if rc != 0 and check:
+            raise ShkzError(rc, last_error())
+        return r, P, P64
+
+    def project2d_batch_device(self, grids, params: Params | None = None, stream=None, check: bool = True):
+        """Batch of 2D grids whose pointers are device addresses (Grid2D)."""
+        n = len(grids)
+        arr = (Grid2D * n)(*grids)
+        res = (Result * n)()
+        prm = params if params is not None else default_params()
+        rc = self._L.shkz_project2d_batch_device(self.handle, arr, n, ctypes.byref(prm), res,
+                                                 None if stream is None else int(stream))
+        if rc != 0 and check:
+            raise ShkzError(rc, last_error())
+        return list(res)
+
+    def export_system2d(self, scene: Scene2D, params: Params | None = None) -> dict:
+        """Assembled FP64 2D system on dense cell arrays: diag, gx, gy (low-face couplings), rhs."""
+        g = self.grid2d(scene)
+        shape = (scene.w, scene.h)
+        out = {k: np.zeros(shape, np.float64) for k in ("diag", "gx", "gy", "rhs")}
+        prm = params if params is not None else default_params()
+        rc = self._L.shkz_export_system2d(self.handle, ctypes.byref(g), ctypes.byref(prm),
+                                          *(out[k].ctypes.data for k in ("diag", "gx", "gy", "rhs")))
+        if rc != 0:
+            raise ShkzError(rc, last_error())
+        return out
 
     def export_system(self, scene: Scene, params: Params | None = None) -> dict:
         """Assembled FP64 system on dense cell arrays: diag, gx, gy, gz (low-face couplings), rhs."""

@@ -54,6 +54,8 @@ struct __align__(32) BrickMeta {
 struct LevelDesc {
     int nbricks;             // stored bricks, all grids
     int ngrids;
+    int flat;                // 2D grids (one cell layer): bilinear transfers
+    int pad;
     const int *bmap;         // per grid g: bmap[gl[g].bmap_off + (bx*nby + by)*nbz + bz] -> brick or -1
     const int *bxyz;         // brick -> bx, by, bz (in its grid)
     const BrickMeta *meta;   // brick -> neighbours, grid, flags

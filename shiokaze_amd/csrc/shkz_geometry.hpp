@@ -26,6 +26,7 @@ struct GridDesc {
     int so_fluid, so_solid;
     float *vel[3];
     const uint8_t *active[3];
+    int flat;            // a 2D grid (w, h) held as d = 1: solid nodes (w+1, h+1), no z faces
 };
 
 __device__ __forceinline__ size_t cell_index(const GridDesc &G, int i, int j, int k) {
@@ -43,7 +44,7 @@ __device__ __forceinline__ double phi_at(const GridDesc &G, int i, int j, int k)
     return G.phi64 ? ((const double *)G.phi)[c] : (double)((const float *)G.phi)[c];
 }
 __device__ __forceinline__ double solid_at(const GridDesc &G, int i, int j, int k) {
-    const size_t n = node_index(G, i, j, k);
+    const size_t n = G.flat ? (size_t)i * (G.h + 1) + j : node_index(G, i, j, k);
     return G.solid64 ? ((const double *)G.solid)[n] : (double)((const float *)G.solid)[n];
 }
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
@@ -102,6 +103,10 @@ __device__ __forceinline__ float face_area(const GridDesc &G, int dim, int i, in
     if (G.solid && G.exist[1]) {
         if (boundary) {
             a = 0.0;
+        } else if (G.flat) {
+            // 2D face = an edge between two nodes: 1 - fraction of the solid samples at its ends
+            // (macutility2.cpp:104)
+            a = 1.0 - liquid_fraction(solid_at(G, i, j, 0), solid_at(G, i + (dim != 0), j + (dim != 1), 0));
         } else {
             // corner order of macutility3.cpp:123-138 mapped to (0,0),(1,0),(1,1),(0,1)
             double v[4];

@@ -131,7 +131,19 @@ __global__ void k_curvature(GridDesc G, float *curv) {
     const int jm = clampi(j - 1, 0, G.h - 1), jp = clampi(j + 1, 0, G.h - 1);
     const int km = clampi(k - 1, 0, G.d - 1), kp = clampi(k + 1, 0, G.d - 1);
     double value;
-    if (G.phi64) {
+    if (G.flat) {
+        // 5-point Laplacian of the 2D module (macpressuresolver2.cpp:82-89), summed in the level
+        // set's precision
+        if (G.phi64) {
+            const double s = +phi_at(G, im, j, k) + phi_at(G, ip, j, k) + phi_at(G, i, jm, k) + phi_at(G, i, jp, k);
+            value = (s - 4.0 * phi_at(G, i, j, k)) / (G.dx * G.dx);
+        } else {
+            const float *f = (const float *)G.phi;
+            const float s = +f[cell_index(G, im, j, k)] + f[cell_index(G, ip, j, k)] + f[cell_index(G, i, jm, k)] +
+                            f[cell_index(G, i, jp, k)];
+            value = (s - 4.0 * f[c]) / (G.dx * G.dx);
+        }
+    } else if (G.phi64) {
         const double s = +phi_at(G, im, j, k) + phi_at(G, ip, j, k) + phi_at(G, i, jm, k) + phi_at(G, i, jp, k) +
                          phi_at(G, i, j, km) + phi_at(G, i, j, kp);
         value = (s - 6.0 * phi_at(G, i, j, k)) / (G.dx * G.dx);
@@ -235,6 +247,13 @@ __global__ __launch_bounds__(BLOCK) void k_operator(GridDesc G, LevelDesc L, Gri
                         dsum += value;
                     }
                 }
+                if (!G.flat) {
+                    const float u = G.vel[dim][face_index(G, dim, fi, fj, fk)];
+                    rhs += -sgn[nq] * area * (double)u / dx;
+                }
+            }
+            if (G.flat) {
+                // the 2D module adds the term for every in-bounds neighbour (macpressuresolver2.cpp:176)
                 const float u = G.vel[dim][face_index(G, dim, fi, fj, fk)];
                 rhs += -sgn[nq] * area * (double)u / dx;
             }
@@ -764,6 +783,8 @@ __global__ __launch_bounds__(BLOCK) void k_restrict(LevelDesc Lc, OpDesc<T> Ac, 
     if (Ac.dir[c] >= 0) {
         const int li = t >> 6, lj = (t >> 3) & 7, lk = t & 7;
         const T w[4] = {T(0.25), T(0.75), T(0.75), T(0.25)};
+        // a flat (2D) level has one cell layer: the transfer is bilinear, weight 1 on that layer
+        const T wk[4] = {Lc.flat ? T(0) : w[0], Lc.flat ? T(1) : w[1], Lc.flat ? T(0) : w[2], Lc.flat ? T(0) : w[3]};
         T acc = 0;
 #pragma unroll
         for (int a = 0; a < 4; ++a) {
@@ -771,7 +792,7 @@ __global__ __launch_bounds__(BLOCK) void k_restrict(LevelDesc Lc, OpDesc<T> Ac, 
 #pragma unroll
             for (int b = 0; b < 4; ++b) {
                 const T *row = &tile[((2 * li + a) * E + (2 * lj + b)) * E + 2 * lk];
-                const T acc_b = w[0] * row[0] + w[1] * row[1] + w[2] * row[2] + w[3] * row[3];
+                const T acc_b = wk[0] * row[0] + wk[1] * row[1] + wk[2] * row[2] + wk[3] * row[3];
                 acc_a += w[b] * acc_b;
             }
             acc += w[a] * acc_a;
@@ -816,9 +837,107 @@ __global__ __launch_bounds__(BLOCK) void k_prolong(LevelDesc Lf, OpDesc<T> Af, c
     const int a1 = (li & 1) ? a0 + 1 : a0 - 1, b1 = (lj & 1) ? b0 + 1 : b0 - 1, c1 = (lk & 1) ? c0 + 1 : c0 - 1;
     auto at = [&](int a, int b, int cc) { return tile[(a * E + b) * E + cc]; };
     const T n = T(0.75), f = T(0.25);
-    const T v = n * (n * (n * at(a0, b0, c0) + f * at(a0, b0, c1)) + f * (n * at(a0, b1, c0) + f * at(a0, b1, c1))) +
-                f * (n * (n * at(a1, b0, c0) + f * at(a1, b0, c1)) + f * (n * at(a1, b1, c0) + f * at(a1, b1, c1)));
+    // a flat (2D) level has one cell layer: weight 1 on it, none on the empty neighbour layer
+    const T nk = Lf.flat ? T(1) : n, fk = Lf.flat ? T(0) : f;
+    const T v = n * (n * (nk * at(a0, b0, c0) + fk * at(a0, b0, c1)) + f * (nk * at(a0, b1, c0) + fk * at(a0, b1, c1))) +
+                f * (n * (nk * at(a1, b0, c0) + fk * at(a1, b0, c1)) + f * (nk * at(a1, b1, c0) + fk * at(a1, b1, c1)));
     zf[c] += v;
+}
+
+// K damped-Jacobi sweeps in one launch (coarse levels, where every launch is latency-bound).  The
+// block of brick B stages the cube of cells within distance K of B (bricks B-1 .. B+1 per axis,
+// found through the grid's brick map) and runs all K sweeps on it in LDS; values near the cube's
+// faces go stale by one cell per sweep, which after K sweeps has not reached B, so B's cells are
+// exactly K sweeps of k_jacobi (same arithmetic).  FROM_ZERO: the first sweep starts from z = 0
+// (pre-smoothing), else from zin.  1024 threads; the couplings sit in LDS, idiag and rhs in
+// registers.
+template <class T, int K>
+struct SmoothK {
+    static constexpr int R = BRICK + 2 * K;          // cube side
+    static constexpr int NR = R * R * R;
+    static constexpr int THREADS = 1024;
+    static constexpr int CPT = (NR + THREADS - 1) / THREADS;  // cube cells per thread
+};
+
+template <class T, int K, int FROM_ZERO>
+__global__ __launch_bounds__(1024) void k_smooth_k(LevelDesc L, OpDesc<T> A, const T *__restrict__ idiag,
+                                                   const T *__restrict__ rhs, const T *__restrict__ zin,
+                                                   T *__restrict__ zout, T omega, Gate Q) {
+    using C = SmoothK<T, K>;
+    constexpr int R = C::R, NR = C::NR, CPT = C::CPT;
+    __shared__ T zs[2][NR];
+    __shared__ T gxs[NR], gys[NR], gzs[NR];
+    __shared__ int nb27[27];
+    const int br = brick_or_idle(L);
+    if (br < 0) return;
+    const BrickMeta m = L.meta[br];
+    if (all_done(Q) || skip_smooth(Q, L, m)) return;
+    const int t = threadIdx.x;
+    if (t < 27) {
+        const int dx = t / 9 - 1, dy = (t / 3) % 3 - 1, dz = t % 3 - 1;
+        const GridLevel G = L.gl[m.grid];
+        const int x = L.bxyz[3 * br] + dx, y = L.bxyz[3 * br + 1] + dy, z = L.bxyz[3 * br + 2] + dz;
+        int id = -1;
+        if (x >= 0 && y >= 0 && z >= 0 && x < G.nbx && y < G.nby && z < G.nbz)
+            id = L.bmap[G.bmap_off + (x * G.nby + y) * G.nbz + z];
+        nb27[t] = id;
+    }
+    __syncthreads();
+    T rid[CPT], rr[CPT];
+#pragma unroll
+    for (int q = 0; q < CPT; ++q) {
+        const int e = t + q * C::THREADS;
+        rid[q] = 0;
+        rr[q] = 0;
+        if (e >= NR) continue;
+        const int a = e / (R * R), b = (e / R) % R, c = e % R;
+        // cube offset a is brick-local cell a - K, in brick (-1, 0 or +1) + local (0..7)
+        const int xa = a - K + BRICK, xb = b - K + BRICK, xc = c - K + BRICK;  // 0 .. 23
+        const int bi = xa >> 3, bj = xb >> 3, bk = xc >> 3;
+        const int id = nb27[(bi * 3 + bj) * 3 + bk];
+        T g0 = 0, g1 = 0, g2 = 0, z0 = 0;
+        if (id >= 0) {
+            const long cell = (long)id * BRICK_CELLS + (((xa & 7) * 8 + (xb & 7)) * 8 + (xc & 7));
+            rid[q] = idiag[cell];
+            rr[q] = rhs[cell];
+            g0 = A.gx[cell];
+            g1 = A.gy[cell];
+            g2 = A.gz[cell];
+            if (!FROM_ZERO) z0 = zin[cell];
+        }
+        gxs[e] = g0;
+        gys[e] = g1;
+        gzs[e] = g2;
+        zs[0][e] = z0;
+    }
+    __syncthreads();
+#pragma unroll 1
+    for (int s = 0; s < K; ++s) {
+        const T *zc = zs[s & 1];
+        T *zn = zs[(s & 1) ^ 1];
+#pragma unroll
+        for (int q = 0; q < CPT; ++q) {
+            const int e = t + q * C::THREADS;
+            if (e >= NR) continue;
+            const int a = e / (R * R), b = (e / R) % R, c = e % R;
+            T out = 0;
+            if (rid[q] > T(0)) {
+                const T zxm = a > 0 ? zc[e - R * R] : T(0), zxp = a < R - 1 ? zc[e + R * R] : T(0);
+                const T zym = b > 0 ? zc[e - R] : T(0), zyp = b < R - 1 ? zc[e + R] : T(0);
+                const T zzm = c > 0 ? zc[e - 1] : T(0), zzp = c < R - 1 ? zc[e + 1] : T(0);
+                const T gxp = a < R - 1 ? gxs[e + R * R] : T(0), gyp = b < R - 1 ? gys[e + R] : T(0),
+                        gzp = c < R - 1 ? gzs[e + 1] : T(0);
+                const T off = gxs[e] * zxm + gxp * zxp + gys[e] * zym + gyp * zyp + gzs[e] * zzm + gzp * zzp;
+                out = (T(1) - omega) * zc[e] + omega * rid[q] * (rr[q] + off);
+            }
+            zn[e] = out;
+        }
+        __syncthreads();
+    }
+    if (t < BRICK_CELLS) {
+        const int li = t >> 6, lj = (t >> 3) & 7, lk = t & 7;
+        zout[(long)br * BRICK_CELLS + t] = zs[K & 1][((li + K) * R + (lj + K)) * R + (lk + K)];
+    }
 }
 
 // Coarse solve of every grid whose last level L is (one brick): block g handles grid g with
@@ -930,17 +1049,19 @@ __device__ __forceinline__ void update_face(const GridDesc &G, int dim, int i, i
     }
 }
 
-__global__ void k_velocity_update(GridDesc G, LevelDesc L, GridLevel GL, const double *__restrict__ x) {
+// P is the grid's dense float pressure (0 off the unknowns), as k_scatter_pressure writes it.
+__global__ void k_velocity_update(GridDesc G, const float *__restrict__ P) {
     const int W = G.w + 1, H = G.h + 1, D = G.d + 1;
     const size_t n = (size_t)W * H * D;
     const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= n) return;
     const int k = (int)(e % D), j = (int)((e / D) % H), i = (int)(e / ((size_t)D * H));
     const bool in_i = i < G.w, in_j = j < G.h, in_k = k < G.d;
-    const float pc = (in_i && in_j && in_k) ? (float)level_value(L, GL, x, i, j, k) : 0.0f;
-    if (in_j && in_k) update_face(G, 0, i, j, k, pc, (float)level_value(L, GL, x, i - 1, j, k));
-    if (in_i && in_k) update_face(G, 1, i, j, k, pc, (float)level_value(L, GL, x, i, j - 1, k));
-    if (in_i && in_j) update_face(G, 2, i, j, k, pc, (float)level_value(L, GL, x, i, j, k - 1));
+    auto p_at = [&](int a, int b, int c) { return (a >= 0 && b >= 0 && c >= 0) ? P[cell_index(G, a, b, c)] : 0.0f; };
+    const float pc = (in_i && in_j && in_k) ? P[cell_index(G, i, j, k)] : 0.0f;
+    if (in_j && in_k) update_face(G, 0, i, j, k, pc, i > 0 ? p_at(i - 1, j, k) : 0.0f);
+    if (in_i && in_k) update_face(G, 1, i, j, k, pc, j > 0 ? p_at(i, j - 1, k) : 0.0f);
+    if (in_i && in_j && !G.flat) update_face(G, 2, i, j, k, pc, k > 0 ? p_at(i, j, k - 1) : 0.0f);
 }
 
 }  // namespace shkz

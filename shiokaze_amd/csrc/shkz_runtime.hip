@@ -488,6 +488,7 @@ struct Problem {
     RedBuf red{};
     int pre = 4, post = 4, pre_c = 4, post_c = 4, coarse_sweeps = 48;
     double omega = 0.9;
+    int multi_max_bricks = 1024;  // coarse levels up to this size smooth with multi-sweep launches
     double eta = 1e-3;  // mixed precision: relative defect reduction asked of each inner solve
 };
 
@@ -498,9 +499,51 @@ static OpDesc<T> op_of(const LevelHost &L) {
 
 // Which of the level's two vectors (z or tmp) holds the smoothed result at the end of the level's
 // post-smoothing, given the ping-pong of the sweeps: 0 = z, 1 = tmp.
-static int final_slot(int pre, int post) {
-    const int after_pre = pre <= 1 ? 0 : ((pre - 2) & 1);
-    return after_pre ^ (post & 1);
+// multi_pre / multi_post: the level smooths with one k_smooth_k launch (which writes the other
+// vector once) instead of per-sweep launches.
+static int final_slot(int pre, int post, bool multi_pre, bool multi_post) {
+    const int after_pre = (multi_pre || pre <= 1) ? 0 : ((pre - 2) & 1);
+    const int post_swaps = multi_post ? (post > 0 ? 1 : 0) : post;
+    return after_pre ^ (post_swaps & 1);
+}
+
+// Sweep counts k_smooth_k is built for (its cube of cells must fit LDS).
+template <class T>
+static bool smooth_k_supported(int k) {
+    return k >= 2 && k <= (sizeof(T) == 4 ? 5 : 3);
+}
+
+// k damped-Jacobi sweeps of one coarse level in one launch (from zero or from zin).
+template <class T>
+static void smooth_k(Launcher &K, int l, int k, bool from_zero, const LevelDesc &L, OpDesc<T> A, const T *id,
+                     const T *rhs, const T *zin, T *zout, T om, Gate Q) {
+    const dim3 g(brick_grid(L.nbricks)), b(1024);
+    const double bytes = 7.0 * sizeof(T);  // per unknown, the arrays it reads once and writes once
+    switch (k) {
+    case 2:
+        if (from_zero) K.go("smooth_k_Lc", l, bytes, 0, g, b, k_smooth_k<T, 2, 1>, L, A, id, rhs, zin, zout, om, Q);
+        else K.go("smooth_k_Lc", l, bytes, 0, g, b, k_smooth_k<T, 2, 0>, L, A, id, rhs, zin, zout, om, Q);
+        break;
+    case 3:
+        if (from_zero) K.go("smooth_k_Lc", l, bytes, 0, g, b, k_smooth_k<T, 3, 1>, L, A, id, rhs, zin, zout, om, Q);
+        else K.go("smooth_k_Lc", l, bytes, 0, g, b, k_smooth_k<T, 3, 0>, L, A, id, rhs, zin, zout, om, Q);
+        break;
+    default:
+        if constexpr (sizeof(T) == 4) {
+            if (k == 4) {
+                if (from_zero)
+                    K.go("smooth_k_Lc", l, bytes, 0, g, b, k_smooth_k<T, 4, 1>, L, A, id, rhs, zin, zout, om, Q);
+                else
+                    K.go("smooth_k_Lc", l, bytes, 0, g, b, k_smooth_k<T, 4, 0>, L, A, id, rhs, zin, zout, om, Q);
+            } else if (k == 5) {
+                if (from_zero)
+                    K.go("smooth_k_Lc", l, bytes, 0, g, b, k_smooth_k<T, 5, 1>, L, A, id, rhs, zin, zout, om, Q);
+                else
+                    K.go("smooth_k_Lc", l, bytes, 0, g, b, k_smooth_k<T, 5, 0>, L, A, id, rhs, zin, zout, om, Q);
+            }
+        }
+        break;
+    }
 }
 
 // One V-cycle z = M r on the level-0 vectors of every grid not done; returns the pointer holding
@@ -514,6 +557,13 @@ static T *vcycle(shkz_ctx *C, Launcher &K, Problem &P, const T *rhs0, T *z0, T *
     std::vector<T *> zcur(nl), zoth(nl);
     const T om = (T)P.omega;
     const double es = (double)sizeof(T);
+    // coarse levels small enough smooth with one multi-sweep launch per smoothing step
+    std::vector<char> mpre(nl, 0), mpost(nl, 0);
+    for (int l = 1; l < nl; ++l) {
+        const bool small = C->levels[l].nbricks <= P.multi_max_bricks;
+        mpre[l] = small && smooth_k_supported<T>(P.pre_c);
+        mpost[l] = small && smooth_k_supported<T>(P.post_c);
+    }
     for (int l = 0; l < nl; ++l) {
         LevelHost &L = C->levels[l];
         const T *rhs = l == 0 ? rhs0 : (const T *)L.rhs;
@@ -527,7 +577,7 @@ static T *vcycle(shkz_ctx *C, Launcher &K, Problem &P, const T *rhs0, T *z0, T *
         const bool last = l == nl - 1;
         if (L.any_coarsest) {
             // grids ending here: their z goes where the other grids' smoothed z ends up
-            T *dst = (last || final_slot(pre, post) == 0) ? z : tmp;
+            T *dst = (last || final_slot(pre, post, mpre[l], mpost[l]) == 0) ? z : tmp;
             if (f && dot_partials)
                 K.go("coarse_solve_dot", l, 6 * es, 0, dim3((unsigned)P.ngrids), b, k_coarse_solve<T, 1>, L.desc, A,
                      rhs, dst, P.coarse_sweeps, om, Q, P.red);
@@ -541,7 +591,9 @@ static T *vcycle(shkz_ctx *C, Launcher &K, Problem &P, const T *rhs0, T *z0, T *
             break;
         }
         T *cur = z, *oth = tmp;
-        if (pre <= 1) {
+        if (mpre[l]) {
+            smooth_k<T>(K, l, pre, true, L.desc, A, id, rhs, (const T *)nullptr, z, om, Q);
+        } else if (pre <= 1) {
             K.go(f ? "smooth_first_L0" : "smooth_first_Lc", l, 3 * es, 0, g, b,
                  f ? k_smooth_first<T, 1> : k_smooth_first<T, 0>, L.desc, id, rhs, z, om, Q);
         } else {
@@ -573,6 +625,13 @@ static T *vcycle(shkz_ctx *C, Launcher &K, Problem &P, const T *rhs0, T *z0, T *
         K.go(f ? "prolong_L0" : "prolong_Lc", l, 3 * es, es * (double)Lc.unknowns, g, b,
              f ? k_prolong<T, 1> : k_prolong<T, 0>, L.desc, A, (const T *)zcur[l + 1], zcur[l], Q);
         T *cur = zcur[l], *oth = zoth[l];
+        if (mpost[l]) {
+            smooth_k<T>(K, l, post, false, L.desc, A, id, rhs, (const T *)cur, oth, om, Q);
+            std::swap(cur, oth);
+            zcur[l] = cur;
+            zoth[l] = oth;
+            continue;
+        }
         for (int s = 0; s < post; ++s) {
             const bool last_dot = f && dot_partials && s == post - 1;
             if (last_dot)
@@ -638,9 +697,9 @@ template <class T>
 static std::string graph_key(shkz_ctx *C, const Problem &P, const void *const *vecs, int nvec, bool precond) {
     std::string k;
     char buf[256];
-    std::snprintf(buf, sizeof(buf), "T%zu pc%d g%d nl%d pre%d post%d prc%d poc%d cs%d om%.17g red%p|", sizeof(T),
-                  precond ? 1 : 0, P.ngrids, P.nlev, P.pre, P.post, P.pre_c, P.post_c, P.coarse_sweeps, P.omega,
-                  (void *)P.red.partials);
+    std::snprintf(buf, sizeof(buf), "T%zu pc%d g%d nl%d pre%d post%d prc%d poc%d cs%d mm%d om%.17g red%p|", sizeof(T),
+                  precond ? 1 : 0, P.ngrids, P.nlev, P.pre, P.post, P.pre_c, P.post_c, P.coarse_sweeps,
+                  P.multi_max_bricks, P.omega, (void *)P.red.partials);
     k += buf;
     for (int i = 0; i < nvec; ++i) {
         std::snprintf(buf, sizeof(buf), "%p,", vecs[i]);
@@ -778,10 +837,15 @@ struct GridJob {
     double rhs_correct = 0.0;
 };
 
+// A 2D grid (w, h) travels inside the runtime as a 3D grid with d = 1 and no w velocity; its solid
+// level set is the 2D nodal array (w+1, h+1).
+static inline bool is_flat(const shkz_grid *grid) { return grid->w_vel == nullptr; }
+
 static int validate_grid(const shkz_grid *grid) {
     const int w = grid->w, h = grid->h, d = grid->d;
     if (w <= 0 || h <= 0 || d <= 0) return fail(SHKZ_ERR_INVALID, "grid dimensions must be positive (%d,%d,%d)", w, h, d);
-    if (!grid->fluid || !grid->u || !grid->v || !grid->w_vel) return fail(SHKZ_ERR_INVALID, "fluid and velocities are required");
+    if (!grid->fluid || !grid->u || !grid->v || (!grid->w_vel && d != 1))
+        return fail(SHKZ_ERR_INVALID, "fluid and velocities are required");
     if (!(grid->dt > 0)) return fail(SHKZ_ERR_INVALID, "dt must be positive");
     if ((int64_t)w * h * d > (int64_t)1 << 31) return fail(SHKZ_ERR_INVALID, "grid too large for 32-bit brick ids");
     return SHKZ_OK;
@@ -926,11 +990,13 @@ static int solve_phase(shkz_ctx *C, Launcher &K, Problem &P, const shkz_params *
 // counts only if every grid of the batch converges; it replaces the current one when it is at
 // least 2% faster (best of two timed solves, so graph capture is not timed).
 struct Knobs {
-    int pre, post, pre_c, post_c, coarse;
+    int pre, post, pre_c, post_c, coarse, multi;
     double omega, eta;
 };
 
-static Knobs knobs_of(const Problem &P) { return Knobs{P.pre, P.post, P.pre_c, P.post_c, P.coarse_sweeps, P.omega, P.eta}; }
+static Knobs knobs_of(const Problem &P) {
+    return Knobs{P.pre, P.post, P.pre_c, P.post_c, P.coarse_sweeps, P.multi_max_bricks, P.omega, P.eta};
+}
 
 // Sets P's controls to k except where the caller fixed a control in prm.
 static void apply_knobs(Problem &P, const Knobs &k, const shkz_params *prm) {
@@ -941,6 +1007,7 @@ static void apply_knobs(Problem &P, const Knobs &k, const shkz_params *prm) {
     if (prm->mg_coarse_sweeps <= 0) P.coarse_sweeps = k.coarse;
     if (!(prm->mg_omega > 0)) P.omega = k.omega;
     if (!(prm->inner_residual > 0)) P.eta = k.eta;
+    P.multi_max_bricks = k.multi;
 }
 
 struct TuneCache {
@@ -963,10 +1030,10 @@ static void tune_load_locked() {
     while (std::fgets(line, sizeof(line), f)) {
         char key[320];
         Knobs k;
-        if (std::sscanf(line, "%319s %d %d %d %d %d %lf %lf", key, &k.pre, &k.post, &k.pre_c, &k.post_c, &k.coarse,
-                        &k.omega, &k.eta) == 8 &&
-            k.pre > 0 && k.post > 0 && k.pre_c > 0 && k.post_c > 0 && k.coarse > 0 && k.omega > 0 && k.omega <= 1.0 &&
-            k.eta > 0)
+        if (std::sscanf(line, "%319s %d %d %d %d %d %d %lf %lf", key, &k.pre, &k.post, &k.pre_c, &k.post_c, &k.coarse,
+                        &k.multi, &k.omega, &k.eta) == 9 &&
+            k.pre > 0 && k.post > 0 && k.pre_c > 0 && k.post_c > 0 && k.coarse > 0 && k.multi >= 0 && k.omega > 0 &&
+            k.omega <= 1.0 && k.eta > 0)
             g_tune.map[key] = k;
     }
     std::fclose(f);
@@ -988,8 +1055,8 @@ static void tune_store(const std::string &key, const Knobs &k) {
     if (g_tune.path.empty()) return;
     FILE *f = std::fopen(g_tune.path.c_str(), "a");
     if (!f) return;
-    std::fprintf(f, "%s %d %d %d %d %d %.17g %.17g\n", key.c_str(), k.pre, k.post, k.pre_c, k.post_c, k.coarse, k.omega,
-                 k.eta);
+    std::fprintf(f, "%s %d %d %d %d %d %d %.17g %.17g\n", key.c_str(), k.pre, k.post, k.pre_c, k.post_c, k.coarse,
+                 k.multi, k.omega, k.eta);
     std::fclose(f);
 }
 
@@ -1042,8 +1109,8 @@ static int autotune(shkz_ctx *C, Launcher &K, Problem &P, const shkz_params *prm
         double ms = 1e300;
         SHKZ_TRY(trial(k, &ms));
         if (C->trace)
-            std::fprintf(stderr, "[shkz] tune pre/post %d/%d coarse %d/%d cs %d omega %.2f eta %.0e: %.3f ms\n", k.pre,
-                         k.post, k.pre_c, k.post_c, k.coarse, k.omega, k.eta, ms);
+            std::fprintf(stderr, "[shkz] tune pre/post %d/%d coarse %d/%d cs %d multi %d omega %.2f eta %.0e: %.3f ms\n",
+                         k.pre, k.post, k.pre_c, k.post_c, k.coarse, k.multi, k.omega, k.eta, ms);
         if (ms < 0.98 * cur_ms) {
             cur = k;
             cur_ms = ms;
@@ -1058,7 +1125,7 @@ static int autotune(shkz_ctx *C, Launcher &K, Problem &P, const shkz_params *prm
             SHKZ_TRY(consider(k));
         }
     if (prm->mg_coarse_pre_sweeps <= 0 && prm->mg_coarse_post_sweeps <= 0 && P.nlev > 2)
-        for (int v : {2, 4, 6}) {
+        for (int v : {2, 3, 4, 5, 6}) {
             if (v == cur.pre_c && v == cur.post_c) continue;
             Knobs k = cur;
             k.pre_c = k.post_c = v;
@@ -1076,6 +1143,13 @@ static int autotune(shkz_ctx *C, Launcher &K, Problem &P, const shkz_params *prm
             if (v == cur.coarse) continue;
             Knobs k = cur;
             k.coarse = v;
+            SHKZ_TRY(consider(k));
+        }
+    if (P.nlev > 2)
+        for (int v : {0, 256, 4096}) {
+            if (v == cur.multi) continue;
+            Knobs k = cur;
+            k.multi = v;
             SHKZ_TRY(consider(k));
         }
     if (P.mixed && !(prm->inner_residual > 0))
@@ -1099,6 +1173,9 @@ static int project_stages(shkz_ctx *C, const shkz_grid *grids, int ngrids, const
     if (prm->solver != SHKZ_SOLVER_MGPCG && prm->solver != SHKZ_SOLVER_CG) return fail(SHKZ_ERR_INVALID, "unknown solver %d", prm->solver);
     if (prm->precision != SHKZ_PREC_MIXED && prm->precision != SHKZ_PREC_FP64)
         return fail(SHKZ_ERR_INVALID, "unknown precision %d", prm->precision);
+    const bool flat = is_flat(&grids[0]);
+    for (int g = 1; g < ngrids; ++g)
+        if (is_flat(&grids[g]) != flat) return fail(SHKZ_ERR_INVALID, "a batch cannot mix 2D and 3D grids");
     for (int a = 0; a < ngrids; ++a)
         for (int b = a + 1; b < ngrids; ++b)
             if (grids[a].scene_id && grids[a].scene_id == grids[b].scene_id)
@@ -1172,24 +1249,28 @@ static int project_stages(shkz_ctx *C, const shkz_grid *grids, int ngrids, const
         G.vel[2] = grid->w_vel;
         G.active[0] = grid->active_u;
         G.active[1] = grid->active_v;
-        G.active[2] = grid->active_w;
+        G.active[2] = flat ? nullptr : grid->active_w;
+        G.flat = flat ? 1 : 0;
         J.exist = exist_all + 4 * g;
         G.exist = J.exist;
         const unsigned nbk = (unsigned)std::min<size_t>((J.M + 255) / 256, 4096);
         K.go("any_negative", -1, 0, (double)J.M * (G.phi64 ? 8 : 4), dim3(nbk), dim3(256), k_any_negative, grid->fluid,
              (int)G.phi64, J.M, J.exist + 0);
         if (grid->solid) {
-            const size_t Nn = (size_t)(w + 1) * (h + 1) * (d + 1);
+            const size_t Nn = (size_t)(w + 1) * (h + 1) * (flat ? 1 : d + 1);
             const unsigned nbs = (unsigned)std::min<size_t>((Nn + 255) / 256, 4096);
             K.go("any_negative", -1, 0, (double)Nn * (G.solid64 ? 8 : 4), dim3(nbs), dim3(256), k_any_negative,
                  grid->solid, (int)G.solid64, Nn, J.exist + 1);
         }
         if (prm->surface_tension != 0.0) {
+            // sized for the batch's largest grid once, so no buffer is reallocated under queued work
+            size_t maxM = 1;
+            for (int q = 0; q < ngrids; ++q) maxM = std::max(maxM, (size_t)grids[q].w * grids[q].h * grids[q].d);
             float *curv = nullptr;
-            SHKZ_TRY(arena_get(AR, "curvature", J.M, &curv));
+            SHKZ_TRY(arena_get(AR, "curvature", maxM, &curv));
             K.go("curvature", -1, 0, (double)J.M * ((G.phi64 ? 8 : 4) + 4), dim3((unsigned)((J.M + 255) / 256)),
                  dim3(256), k_curvature, G, curv);
-            for (int dim = 0; dim < 3; ++dim) {
+            for (int dim = 0; dim < (flat ? 2 : 3); ++dim) {
                 const size_t nf = (size_t)(w + (dim == 0)) * (h + (dim == 1)) * (d + (dim == 2));
                 K.go("surface_tension", -1, 0, (double)nf * 8, dim3((unsigned)((nf + 255) / 256)), dim3(256),
                      k_surface_tension, G, dim, (const float *)curv, prm->surface_tension);
@@ -1297,6 +1378,7 @@ static int project_stages(shkz_ctx *C, const shkz_grid *grids, int ngrids, const
             std::copy(L.parent8.begin(), L.parent8.end(), host + o.p8);
             L.desc.nbricks = L.nbricks;
             L.desc.ngrids = ngrids;
+            L.desc.flat = flat ? 1 : 0;
             L.desc.bmap = dmeta + o.base;
             L.desc.bxyz = dmeta + o.xyz;
             L.desc.meta = reinterpret_cast<const BrickMeta *>(dmeta + o.meta);
@@ -1532,20 +1614,27 @@ static int project_stages(shkz_ctx *C, const shkz_grid *grids, int ngrids, const
     const double t_solve = now_ms();
 
     // ---------------------------------------------------------------- per grid: scatter + update
+    float *pdense = nullptr;
+    {
+        size_t maxM = 1;
+        for (const GridJob &J : jobs) maxM = std::max(maxM, J.M);
+        SHKZ_TRY(arena_get(AR, "pdense", maxM, &pdense));
+    }
     for (int g = 0; g < ngrids; ++g) {
         GridJob &J = jobs[g];
         const shkz_grid *grid = J.in;
         const GridLevel &GL = F0.gl[g];
-        if (grid->pressure || grid->pressure64)
-            K.go("scatter_pressure", -1, 0, (double)J.M * ((grid->pressure ? 4 : 0) + (grid->pressure64 ? 8 : 0)),
-                 dim3((unsigned)((J.M + 255) / 256)), dim3(256), k_scatter_pressure, J.G, F0.desc, GL,
-                 (const double *)P.x64, grid->pressure, grid->pressure64);
+        // the velocity update reads a dense float pressure: the caller's, or the scratch array
+        float *P32 = grid->pressure ? grid->pressure : pdense;
+        K.go("scatter_pressure", -1, 0, (double)J.M * (4 + (grid->pressure64 ? 8 : 0)),
+             dim3((unsigned)((J.M + 255) / 256)), dim3(256), k_scatter_pressure, J.G, F0.desc, GL,
+             (const double *)P.x64, P32, grid->pressure64);
         if (J.scene)
             K.go("scatter_pressure", -1, 0, (double)J.M * 8, dim3((unsigned)((J.M + 255) / 256)), dim3(256),
                  k_scatter_pressure, J.G, F0.desc, GL, (const double *)P.x64, (float *)nullptr, J.scene->p);
         const size_t ne = (size_t)(grid->w + 1) * (grid->h + 1) * (grid->d + 1);
-        K.go("velocity_update", -1, 0, (double)J.M * (24 + (J.G.phi64 ? 8 : 4) + (grid->solid ? 4 : 0)),
-             dim3((unsigned)((ne + 255) / 256)), dim3(256), k_velocity_update, J.G, F0.desc, GL, (const double *)P.x64);
+        K.go("velocity_update", -1, 0, (double)J.M * (24 + 4 + (J.G.phi64 ? 8 : 4) + (grid->solid ? 4 : 0)),
+             dim3((unsigned)((ne + 255) / 256)), dim3(256), k_velocity_update, J.G, (const float *)P32);
         if (J.scene) J.scene->has_pressure = true;
         res[g].iterations = total_iters[g];
         res[g].outer_iterations = outer_passes[g];
@@ -1590,8 +1679,9 @@ static int project_host(shkz_ctx *C, const shkz_grid *grids, int ngrids, const s
         const shkz_grid *grid = &grids[g];
         const std::string sfx = std::to_string(g);
         const size_t w = grid->w, h = grid->h, d = grid->d;
-        const size_t M = w * h * d, Nn = (w + 1) * (h + 1) * (d + 1);
-        const size_t nf[3] = {(w + 1) * h * d, w * (h + 1) * d, w * h * (d + 1)};
+        const bool flat = is_flat(grid);
+        const size_t M = w * h * d, Nn = (w + 1) * (h + 1) * (flat ? 1 : d + 1);
+        const size_t nf[3] = {(w + 1) * h * d, w * (h + 1) * d, flat ? 0 : w * h * (d + 1)};
         void *phi = nullptr, *solid = nullptr;
         const size_t pe = grid->fluid_dtype == SHKZ_F64 ? 8 : 4, se = grid->solid_dtype == SHKZ_F64 ? 8 : 4;
         SHKZ_TRY(AR.get("h_phi" + sfx, M * pe, &phi));
@@ -1608,10 +1698,12 @@ static int project_host(shkz_ctx *C, const shkz_grid *grids, int ngrids, const s
         const char *vn[3] = {"h_u", "h_v", "h_w"}, *an[3] = {"h_au", "h_av", "h_aw"};
         Back &B = back[g];
         for (int dim = 0; dim < 3; ++dim) {
+            B.dv[dim] = nullptr;
+            B.nf[dim] = nf[dim];
+            if (!hv[dim]) continue;  // no w velocity in 2D
             void *pp = nullptr;
             SHKZ_TRY(AR.get(vn[dim] + sfx, nf[dim] * 4, &pp));
             B.dv[dim] = (float *)pp;
-            B.nf[dim] = nf[dim];
             SHKZ_HIP(hipMemcpy(B.dv[dim], hv[dim], nf[dim] * 4, hipMemcpyHostToDevice));
             if (ha[dim]) {
                 SHKZ_TRY(AR.get(an[dim] + sfx, nf[dim], &pp));
@@ -1663,7 +1755,7 @@ static int project_host(shkz_ctx *C, const shkz_grid *grids, int ngrids, const s
         const shkz_grid *grid = &grids[g];
         float *hv[3] = {grid->u, grid->v, grid->w_vel};
         for (int dim = 0; dim < 3; ++dim)
-            SHKZ_HIP(hipMemcpy(hv[dim], back[g].dv[dim], back[g].nf[dim] * 4, hipMemcpyDeviceToHost));
+            if (hv[dim]) SHKZ_HIP(hipMemcpy(hv[dim], back[g].dv[dim], back[g].nf[dim] * 4, hipMemcpyDeviceToHost));
         if (grid->pressure) SHKZ_HIP(hipMemcpy(grid->pressure, back[g].dp, back[g].M * 4, hipMemcpyDeviceToHost));
         if (grid->pressure64)
             SHKZ_HIP(hipMemcpy(grid->pressure64, back[g].dp64, back[g].M * 8, hipMemcpyDeviceToHost));
@@ -1840,9 +1932,17 @@ void shkz_destroy(shkz_ctx *C) {
     delete C;
 }
 
+// The 3D entry points take 3D grids only (a 2D grid goes through the *2d entry points).
+static int require_3d(const shkz_grid *grids, int n) {
+    for (int g = 0; g < n; ++g)
+        if (!grids[g].w_vel) return fail(SHKZ_ERR_INVALID, "the w velocity is required (2D grids: shkz_project2d*)");
+    return SHKZ_OK;
+}
+
 int shkz_project_batch_device(shkz_ctx *C, const shkz_grid *grids, int ngrids, const shkz_params *params,
                               shkz_result *results, void *stream) {
     if (!C || !grids || !results || ngrids <= 0) return fail(SHKZ_ERR_INVALID, "NULL argument or empty batch");
+    SHKZ_TRY(require_3d(grids, ngrids));
     shkz_params def;
     shkz_default_params(&def);
     const shkz_params *p = params ? params : &def;
@@ -1859,6 +1959,7 @@ int shkz_project_device(shkz_ctx *C, const shkz_grid *grid, const shkz_params *p
 int shkz_project_batch(shkz_ctx *C, const shkz_grid *grids, int ngrids, const shkz_params *params,
                        shkz_result *results) {
     if (!C || !grids || !results || ngrids <= 0) return fail(SHKZ_ERR_INVALID, "NULL argument or empty batch");
+    SHKZ_TRY(require_3d(grids, ngrids));
     for (int g = 0; g < ngrids; ++g) std::memset(&results[g], 0, sizeof(shkz_result));
     std::lock_guard<std::mutex> lock(C->mu);
     const int rc = project_host(C, grids, ngrids, params, results, nullptr);
@@ -1876,6 +1977,7 @@ int shkz_export_system(shkz_ctx *C, const shkz_grid *grid, const shkz_params *pa
                        double *gy, double *gz, double *rhs) {
     if (!C || !grid) return fail(SHKZ_ERR_INVALID, "NULL argument");
     if (!diag || !gx || !gy || !gz || !rhs) return fail(SHKZ_ERR_INVALID, "export arrays are required");
+    SHKZ_TRY(require_3d(grid, 1));
     double *outs[5] = {diag, gx, gy, gz, rhs};
     shkz_result r;
     std::lock_guard<std::mutex> lock(C->mu);
@@ -1931,6 +2033,76 @@ int shkz_reset_kernel_stats(shkz_ctx *C) {
     }
     C->totals.clear();
     return SHKZ_OK;
+}
+
+}  // extern "C"
+
+// ============================================================================ C ABI: 2D
+// macpressuresolver2::project (src/projection/macpressuresolver2.cpp:48-232) on the same engine:
+// the 2D grid is a one-cell-thick 3D grid (d = 1, no z faces), with macutility2's edge area
+// fractions and the 2D module's assembly order.
+static shkz_grid grid_of_2d(const shkz_grid2d *g) {
+    shkz_grid q;
+    std::memset(&q, 0, sizeof(q));
+    q.w = g->w;
+    q.h = g->h;
+    q.d = 1;
+    q.dx = g->dx > 0 ? g->dx : std::min(1.0 / g->w, 1.0 / g->h);
+    q.dt = g->dt;
+    q.fluid = g->fluid;
+    q.fluid_dtype = g->fluid_dtype;
+    q.solid = g->solid;
+    q.solid_dtype = g->solid_dtype;
+    q.u = g->u;
+    q.v = g->v;
+    q.w_vel = nullptr;
+    q.active_u = g->active_u;
+    q.active_v = g->active_v;
+    q.pressure = g->pressure;
+    q.pressure64 = g->pressure64;
+    q.scene_id = g->scene_id;
+    return q;
+}
+
+extern "C" {
+
+int shkz_project2d(shkz_ctx *C, const shkz_grid2d *grid, const shkz_params *params, shkz_result *result) {
+    if (!C || !grid || !result) return fail(SHKZ_ERR_INVALID, "NULL argument");
+    const shkz_grid q = grid_of_2d(grid);
+    std::memset(result, 0, sizeof(*result));
+    std::lock_guard<std::mutex> lock(C->mu);
+    const int rc = project_host(C, &q, 1, params, result, nullptr);
+    if (rc != SHKZ_OK && rc != SHKZ_ERR_NUMERIC && result->status == SHKZ_OK) result->status = rc;
+    return rc;
+}
+
+int shkz_project2d_batch_device(shkz_ctx *C, const shkz_grid2d *grids, int ngrids, const shkz_params *params,
+                                shkz_result *results, void *stream) {
+    if (!C || !grids || !results || ngrids <= 0) return fail(SHKZ_ERR_INVALID, "NULL argument or empty batch");
+    std::vector<shkz_grid> q(ngrids);
+    for (int g = 0; g < ngrids; ++g) q[g] = grid_of_2d(&grids[g]);
+    shkz_params def;
+    shkz_default_params(&def);
+    std::lock_guard<std::mutex> lock(C->mu);
+    return project_impl(C, q.data(), ngrids, params ? params : &def, results, (hipStream_t)stream);
+}
+
+int shkz_project2d_device(shkz_ctx *C, const shkz_grid2d *grid, const shkz_params *params, shkz_result *result,
+                          void *stream) {
+    if (!C || !grid || !result) return fail(SHKZ_ERR_INVALID, "NULL argument");
+    return shkz_project2d_batch_device(C, grid, 1, params, result, stream);
+}
+
+int shkz_export_system2d(shkz_ctx *C, const shkz_grid2d *grid, const shkz_params *params, double *diag, double *gx,
+                         double *gy, double *rhs) {
+    if (!C || !grid) return fail(SHKZ_ERR_INVALID, "NULL argument");
+    if (!diag || !gx || !gy || !rhs) return fail(SHKZ_ERR_INVALID, "export arrays are required");
+    const shkz_grid q = grid_of_2d(grid);
+    std::vector<double> gz((size_t)grid->w * grid->h);
+    double *outs[5] = {diag, gx, gy, gz.data(), rhs};
+    shkz_result r;
+    std::lock_guard<std::mutex> lock(C->mu);
+    return project_host(C, &q, 1, params, &r, outs);
 }
 
 }  // extern "C"
@@ -2005,6 +2177,7 @@ void shkz_server_destroy(shkz_server *srv) {
 
 int shkz_submit(shkz_server *srv, const shkz_grid *grid, const shkz_params *params, uint64_t *ticket) {
     if (!srv || !grid || !ticket) return fail(SHKZ_ERR_INVALID, "NULL argument");
+    SHKZ_TRY(require_3d(grid, 1));
     SHKZ_TRY(validate_grid(grid));
     auto *J = new ServerJob();
     J->grid = *grid;

@@ -222,3 +222,75 @@ extern "C" int shkz_scene_fill(const shkz_scene_desc *desc, void *fluid, float *
     }
     return SHKZ_OK;
 }
+
+// 2D scenes: the same seeded shapes cut at z = 0 (circles, pool, the box's x-y rectangle) on a
+// (w, h) grid with dx = min(1/w, 1/h): level sets at cell centres, solid at nodes, N(0,1)
+// velocities over u then v (index order), gravity*dt added to v.
+extern "C" int shkz_scene_fill2d(const shkz_scene_desc *desc, void *fluid, float *solid, int32_t *has_solid, float *u,
+                                 float *v) {
+    if (!desc || desc->w <= 0 || desc->h <= 0) return SHKZ_ERR_INVALID;
+    const int W = desc->w, H = desc->h;
+    const double dx = std::min(1.0 / W, 1.0 / H);
+    SceneRng rng(desc->seed);
+    Scene sc;
+    if (build_scene(desc, rng, sc) != SHKZ_OK) return SHKZ_ERR_INVALID;
+    // the 3D shapes evaluated in the plane through their centres (z of each sphere, box at its middle)
+    auto fluid2 = [&](double x, double y) {
+        const double q[2] = {x - sc.shift[0], y - sc.shift[1]};
+        double val = 1e9;
+        if (sc.pool >= 0.0) val = std::min(val, q[1] - sc.pool);
+        for (const Sphere &s : sc.liquid_spheres) {
+            const double ddx = q[0] - s.c[0], ddy = q[1] - s.c[1];
+            val = std::min(val, std::sqrt(ddx * ddx + ddy * ddy) - s.r);
+        }
+        return val;
+    };
+    auto solid2 = [&](double x, double y) {
+        double val = 1e9;
+        for (const Sphere &s : sc.solid_spheres) {
+            const double ddx = x - s.c[0], ddy = y - s.c[1];
+            val = std::min(val, std::sqrt(ddx * ddx + ddy * ddy) - s.r);
+        }
+        if (sc.solid_box) {
+            double sd = -9999.0;
+            const double p[2] = {x, y};
+            for (int dim = 0; dim < 2; ++dim) {
+                sd = std::max(sd, sc.box0[dim] - p[dim]);
+                sd = std::max(sd, p[dim] - sc.box1[dim]);
+            }
+            val = std::min(val, sd);
+        }
+        return val;
+    };
+    const bool with_solid = sc.has_solid();
+    if (has_solid) *has_solid = with_solid ? 1 : 0;
+    if (fluid)
+        for (int i = 0; i < W; ++i)
+            for (int j = 0; j < H; ++j) {
+                const double x = (i + 0.5) * dx, y = (j + 0.5) * dx;
+                double val = fluid2(x, y);
+                if (with_solid) val = std::max(val, -(solid2(x, y) + dx));
+                const size_t c = (size_t)i * H + j;
+                if (desc->fluid_dtype == SHKZ_F64) ((double *)fluid)[c] = val;
+                else ((float *)fluid)[c] = (float)val;
+            }
+    if (solid)
+        for (int i = 0; i <= W; ++i)
+            for (int j = 0; j <= H; ++j) solid[(size_t)i * (H + 1) + j] = with_solid ? (float)solid2(i * dx, j * dx) : 1.0f;
+    SceneRng vrng(desc->velocity_seed ? desc->velocity_seed : 0);
+    SceneRng &vr = desc->velocity_seed ? vrng : rng;
+    float *vel[2] = {u, v};
+    for (int dim = 0; dim < 2; ++dim) {
+        const size_t n = (size_t)(W + (dim == 0)) * (H + (dim == 1));
+        float *a = vel[dim];
+        if (!a) continue;
+        if (desc->velocity_mode == 1) std::memset(a, 0, n * sizeof(float));
+        else
+            for (size_t f = 0; f < n; ++f) a[f] = (float)vr.normal();
+        if (dim == 1 && desc->gravity != 0.0) {
+            const float g = (float)(desc->gravity * desc->dt);
+            for (size_t f = 0; f < n; ++f) a[f] += g;
+        }
+    }
+    return SHKZ_OK;
+}
