@@ -239,6 +239,43 @@ int shkz_project2d_batch_device(shkz_ctx *ctx, const shkz_grid2d *grids, int ngr
 int shkz_export_system2d(shkz_ctx *ctx, const shkz_grid2d *grid, const shkz_params *params, double *diag, double *gx,
                          double *gy, double *rhs);
 
+/* ------------------------------------------------------- LinSolver drop-in (CSR systems)
+ * Stands in for the pcg LinSolver module, RCMatrix_solver_interface<size_t,double>::solve
+ * (include/shiokaze/linsolver/RCMatrix_solver.h:77, implemented by pcg_solver::solve,
+ * src/linsolver/pcg.cpp:45-73 -> PCGSolver<double>::solve, pcg_solver.h:246-295), for any
+ * symmetric positive (semi)definite system the caller assembled: x0 = 0, stop at
+ * ||r||_inf <= residual * ||b||_inf or max_iterations.  SHKZ_LS_NONE is the reference's
+ * iteration (its MIC(0) output is discarded, :379-384: plain CG); SHKZ_LS_JACOBI applies the
+ * inverse diagonal.  FP64 throughout. */
+enum shkz_ls_preconditioner { SHKZ_LS_NONE = 0, SHKZ_LS_JACOBI = 1 };
+
+typedef struct shkz_linsolve_params {
+    double residual;          /* Residual (pcg.cpp:76, 1e-4) */
+    int32_t max_iterations;   /* MaxIterations (:77, 30000) */
+    int32_t preconditioner;   /* shkz_ls_preconditioner */
+    int32_t reserved[6];
+} shkz_linsolve_params;
+
+typedef struct shkz_linsolve_result {
+    int32_t status;
+    int32_t converged;
+    int32_t iterations;       /* Result::count */
+    int32_t pad;
+    double reresid;           /* Result::reresid: ||r||_inf / ||b||_inf */
+    double t_upload, t_solve; /* milliseconds */
+} shkz_linsolve_result;
+
+typedef struct shkz_linsolver shkz_linsolver;
+
+void shkz_linsolve_default_params(shkz_linsolve_params *p);
+int shkz_linsolver_create(int device, shkz_linsolver **out);
+void shkz_linsolver_destroy(shkz_linsolver *solver);
+/* Solves A x = b for an n x n CSR matrix on host arrays (rows may be in any column order; indices
+ * are checked).  x receives the solution. */
+int shkz_linsolve_csr(shkz_linsolver *solver, int64_t n, const int64_t *rowptr, const int64_t *col,
+                      const double *val, const double *b, double *x, const shkz_linsolve_params *params,
+                      shkz_linsolve_result *result);
+
 /* ------------------------------------------------------- server (continuous batching)
  * No counterpart in the reference (its driver projects one grid per time step on one thread,
  * ui.cpp:703-710).  A server owns `executors` workspaces, each with its own HIP stream; every

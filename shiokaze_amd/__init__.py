@@ -91,6 +91,17 @@ class Grid2D(ctypes.Structure):
     ]
 
 
+class LinsolveParams(ctypes.Structure):
+    _fields_ = [("residual", ctypes.c_double), ("max_iterations", ctypes.c_int32), ("preconditioner", ctypes.c_int32),
+                ("reserved", ctypes.c_int32 * 6)]
+
+
+class LinsolveResult(ctypes.Structure):
+    _fields_ = [("status", ctypes.c_int32), ("converged", ctypes.c_int32), ("iterations", ctypes.c_int32),
+                ("pad", ctypes.c_int32), ("reresid", ctypes.c_double), ("t_upload", ctypes.c_double),
+                ("t_solve", ctypes.c_double)]
+
+
 class ServerConfig(ctypes.Structure):
     _fields_ = [
         ("device", ctypes.c_int32), ("executors", ctypes.c_int32), ("max_batch_grids", ctypes.c_int32),
@@ -134,7 +145,8 @@ EXPORTS = [
     "shkz_server_default_config", "shkz_server_create", "shkz_server_destroy", "shkz_submit", "shkz_poll",
     "shkz_wait", "shkz_server_release_scene", "shkz_server_get_stats", "shkz_scene_fill_device",
     "shkz_project2d", "shkz_project2d_device", "shkz_project2d_batch_device", "shkz_export_system2d",
-    "shkz_scene_fill2d",
+    "shkz_scene_fill2d", "shkz_linsolve_default_params", "shkz_linsolver_create", "shkz_linsolver_destroy",
+    "shkz_linsolve_csr",
 ]
 
 _lib = None
@@ -187,6 +199,11 @@ def lib():
                                               ctypes.POINTER(Result), _vp]
     L.shkz_export_system2d.argtypes = [_vp, ctypes.POINTER(Grid2D), ctypes.POINTER(Params), _vp, _vp, _vp, _vp]
     L.shkz_scene_fill2d.argtypes = [ctypes.POINTER(SceneDesc), _vp, _vp, ctypes.POINTER(ctypes.c_int32), _vp, _vp]
+    L.shkz_linsolve_default_params.argtypes = [ctypes.POINTER(LinsolveParams)]
+    L.shkz_linsolver_create.argtypes = [ctypes.c_int, ctypes.POINTER(_vp)]
+    L.shkz_linsolver_destroy.argtypes = [_vp]
+    L.shkz_linsolve_csr.argtypes = [_vp, ctypes.c_int64, _vp, _vp, _vp, _vp, _vp, ctypes.POINTER(LinsolveParams),
+                                    ctypes.POINTER(LinsolveResult)]
     _lib = L
     return L
 
@@ -327,6 +344,56 @@ def scene_fill_device(kind: int, w: int, h: int, d: int, fluid, u, v, wv, seed: 
 
 
 PENDING = 1
+LS_NONE, LS_JACOBI = 0, 1
+
+
+class LinSolver:
+    """The LinSolver drop-in (shkz_linsolver): CG / Jacobi-PCG on a CSR system in FP64."""
+
+    def __init__(self, device: int = 0):
+        self._L = lib()
+        h = _vp()
+        rc = self._L.shkz_linsolver_create(device, ctypes.byref(h))
+        if rc != 0:
+            raise ShkzError(rc, last_error())
+        self.handle = h
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self._L.shkz_linsolver_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def solve(self, rowptr, col, val, b, residual=1e-4, max_iterations=30000, preconditioner=LS_NONE,
+              check=True):
+        """Solves A x = b (CSR, int64 indices, float64 values).  Returns (x, result)."""
+        rowptr = np.ascontiguousarray(rowptr, np.int64)
+        col = np.ascontiguousarray(col, np.int64)
+        val = np.ascontiguousarray(val, np.float64)
+        b = np.ascontiguousarray(b, np.float64)
+        n = len(rowptr) - 1
+        x = np.zeros(max(n, 1), np.float64)
+        prm = LinsolveParams()
+        self._L.shkz_linsolve_default_params(ctypes.byref(prm))
+        prm.residual, prm.max_iterations, prm.preconditioner = residual, max_iterations, preconditioner
+        res = LinsolveResult()
+        rc = self._L.shkz_linsolve_csr(self.handle, n, rowptr.ctypes.data, col.ctypes.data if len(col) else None,
+                                       val.ctypes.data if len(val) else None, b.ctypes.data, x.ctypes.data,
+                                       ctypes.byref(prm), ctypes.byref(res))
+        if rc != 0 and check:
+            raise ShkzError(rc, last_error())
+        return x[:n], res
 
 
 class Server:

@@ -17,6 +17,7 @@
 #include "../../include/shkz_mi355x.h"
 #include "shkz_kernels.hpp"
 #include "shkz_serve.hpp"
+#include "shkz_error.hpp"
 
 #include <algorithm>
 #include <atomic>
@@ -45,6 +46,16 @@ static int fail(int code, const char *fmt, ...) {
     g_last_error = buf;
     return code;
 }
+
+}  // namespace shkz
+
+// Error recording for the library's other translation units (shkz_error.hpp).
+int shkz_set_error(int code, const char *msg) {
+    shkz::g_last_error = msg;
+    return code;
+}
+
+namespace shkz {
 
 #define SHKZ_HIP(expr)                                                                                                 \
     do {                                                                                                               \

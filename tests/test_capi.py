@@ -92,10 +92,12 @@ def test_scene_liquid_is_pushed_out_of_solids():
 
 
 @pytest.mark.skipif(not os.path.isdir("/root/reference/include"), reason="Shiokaze tree not present")
-def test_shiokaze_adapter_compiles_against_reference_headers():
-    """The drop-in module source type-checks against the reference's own interface headers."""
-    src = os.path.join(ROOT, "shiokaze_amd", "host", "macpressuresolver3_mi355x.cpp")
-    subprocess.check_call(["g++", "-std=c++14", "-fsyntax-only", "-DNUM_THREAD=8", "-I/root/reference/include",
+@pytest.mark.parametrize("module", ["macpressuresolver3_mi355x", "macpressuresolver2_mi355x", "pcg_mi355x"])
+def test_shiokaze_adapter_compiles_against_reference_headers(module):
+    """The drop-in module sources type-check against the reference's own interface headers."""
+    src = os.path.join(ROOT, "shiokaze_amd", "host", module + ".cpp")
+    subprocess.check_call(["g++", "-std=c++14", "-fsyntax-only", "-DNUM_THREAD=8", "-DINDEX_TYPE=size_t",
+                           "-DFLOAT_TYPE=double", "-I/root/reference/include",
                            "-I/root/reference/local/include", src])
 
 
