@@ -435,7 +435,8 @@ int shkz_linsolve_csr(shkz_linsolver *L, int64_t n, const int64_t *rowptr, const
     LS_HIP(hipStreamSynchronize(L->stream));  // the host staging vectors go out of scope
     const double t1 = now_ms();
     const int rc = ls_solve_device(L, (int)n, prm, result);
-    LS_HIP(hipMemcpy(x, L->x.p, n * sizeof(double), hipMemcpyDeviceToHost));
+    LS_HIP(hipMemcpyAsync(x, L->x.p, n * sizeof(double), hipMemcpyDeviceToHost, L->stream));
+    LS_HIP(hipStreamSynchronize(L->stream));
     const double t2 = now_ms();
     result->t_upload = t1 - t0;
     result->t_solve = t2 - t1;
