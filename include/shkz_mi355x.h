@@ -348,7 +348,10 @@ enum shkz_scene_kind {
     SHKZ_SCENE_DROP_SOLIDS = 1, /* the same plus 3 solid spheres r=0.1, centres U[0.2,0.8]^3 */
     SHKZ_SCENE_SPHERES_BOX = 2, /* 8 spheres r~U[0.05,0.15] over a pool of depth 0.25 and one
                                    random solid box of volume 0.1 */
-    SHKZ_SCENE_RANDOM_DROP = 3  /* seeded sphere-drop (pool depth and sphere drawn from seed) */
+    SHKZ_SCENE_RANDOM_DROP = 3, /* seeded sphere-drop (pool depth and sphere drawn from seed) */
+    SHKZ_SCENE_SMOKE = 4        /* all-fluid domain (fluid = -1 everywhere, as macsmoke3 initialises
+                                   it, macsmoke3.cpp:106) around 2 solid spheres r~U[0.08,0.15],
+                                   centres U[0.25,0.75]^3; the liquid is not pushed out of the solid */
 };
 
 typedef struct shkz_scene_desc {

@@ -24,7 +24,7 @@ F32, F64 = 0, 1
 SOLVER_MGPCG, SOLVER_CG = 0, 1
 PREC_MIXED, PREC_FP64 = 0, 1
 NORM_INF, NORM_L2 = 0, 1
-SCENE_DROP, SCENE_DROP_SOLIDS, SCENE_SPHERES_BOX, SCENE_RANDOM_DROP = 0, 1, 2, 3
+SCENE_DROP, SCENE_DROP_SOLIDS, SCENE_SPHERES_BOX, SCENE_RANDOM_DROP, SCENE_SMOKE = 0, 1, 2, 3, 4
 
 OK, ERR_INVALID, ERR_DEVICE, ERR_NOMEM, ERR_BUDGET, ERR_NOT_FOUND, ERR_NUMERIC = 0, -1, -2, -3, -4, -5, -6
 STATUS = {0: "OK", -1: "INVALID", -2: "DEVICE", -3: "NOMEM", -4: "BUDGET", -5: "NOT_FOUND", -6: "NUMERIC"}

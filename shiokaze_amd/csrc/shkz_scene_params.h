@@ -6,7 +6,7 @@
 #include "../../include/shkz_mi355x.h"
 
 struct ShkzSceneParams {
-    int32_t n_liquid, n_solid, has_box, pad;
+    int32_t n_liquid, n_solid, has_box, all_fluid;  // all_fluid: phi = -1 everywhere, no push-out
     double pool;            // liquid below y < pool; < 0 = no pool
     double liquid[8][4];    // sphere centre xyz, radius
     double solid[4][4];

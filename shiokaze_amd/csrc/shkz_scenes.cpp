@@ -52,10 +52,12 @@ struct Scene {
     bool solid_box = false;
     double box0[3] = {0, 0, 0}, box1[3] = {0, 0, 0};
     double shift[3] = {0, 0, 0};
+    bool all_fluid = false;  // every cell liquid (phi = -1), the liquid not pushed out of solids
 
     bool has_solid() const { return !solid_spheres.empty() || solid_box; }
 
     double fluid(const double p[3]) const {
+        if (all_fluid) return -1.0;
         const double q[3] = {p[0] - shift[0], p[1] - shift[1], p[2] - shift[2]};
         double v = 1e9;
         if (pool >= 0.0) v = std::min(v, q[1] - pool);
@@ -133,6 +135,15 @@ static int build_scene(const shkz_scene_desc *desc, SceneRng &rng, Scene &sc) {
         sc.liquid_spheres.push_back(sp);
         break;
     }
+    case SHKZ_SCENE_SMOKE:
+        sc.all_fluid = true;
+        for (int s = 0; s < 2; ++s) {
+            Sphere sp;
+            for (int a = 0; a < 3; ++a) sp.c[a] = rng.uniform(0.25, 0.75);
+            sp.r = rng.uniform(0.08, 0.15);
+            sc.solid_spheres.push_back(sp);
+        }
+        break;
     default:
         return SHKZ_ERR_INVALID;
     }
@@ -151,6 +162,7 @@ extern "C" int shkz_scene_params_internal(const shkz_scene_desc *desc, ShkzScene
     out->n_liquid = (int32_t)sc.liquid_spheres.size();
     out->n_solid = (int32_t)sc.solid_spheres.size();
     out->has_box = sc.solid_box ? 1 : 0;
+    out->all_fluid = sc.all_fluid ? 1 : 0;
     out->pool = sc.pool;
     for (int q = 0; q < out->n_liquid; ++q) {
         for (int a = 0; a < 3; ++a) out->liquid[q][a] = sc.liquid_spheres[q].c[a];
@@ -186,7 +198,7 @@ extern "C" int shkz_scene_fill(const shkz_scene_desc *desc, void *fluid, float *
                 for (int k = 0; k < D; ++k) {
                     double p[3] = {(i + 0.5) * dx, (j + 0.5) * dx, (k + 0.5) * dx};
                     double val = sc.fluid(p);
-                    if (with_solid) val = std::max(val, -(sc.solid(p) + dx));
+                    if (with_solid && !sc.all_fluid) val = std::max(val, -(sc.solid(p) + dx));
                     size_t c = ((size_t)i * H + j) * D + k;
                     if (desc->fluid_dtype == SHKZ_F64) ((double *)fluid)[c] = val;
                     else ((float *)fluid)[c] = (float)val;
@@ -236,6 +248,7 @@ extern "C" int shkz_scene_fill2d(const shkz_scene_desc *desc, void *fluid, float
     if (build_scene(desc, rng, sc) != SHKZ_OK) return SHKZ_ERR_INVALID;
     // the 3D shapes evaluated in the plane through their centres (z of each sphere, box at its middle)
     auto fluid2 = [&](double x, double y) {
+        if (sc.all_fluid) return -1.0;
         const double q[2] = {x - sc.shift[0], y - sc.shift[1]};
         double val = 1e9;
         if (sc.pool >= 0.0) val = std::min(val, q[1] - sc.pool);
@@ -269,7 +282,7 @@ extern "C" int shkz_scene_fill2d(const shkz_scene_desc *desc, void *fluid, float
             for (int j = 0; j < H; ++j) {
                 const double x = (i + 0.5) * dx, y = (j + 0.5) * dx;
                 double val = fluid2(x, y);
-                if (with_solid) val = std::max(val, -(solid2(x, y) + dx));
+                if (with_solid && !sc.all_fluid) val = std::max(val, -(solid2(x, y) + dx));
                 const size_t c = (size_t)i * H + j;
                 if (desc->fluid_dtype == SHKZ_F64) ((double *)fluid)[c] = val;
                 else ((float *)fluid)[c] = (float)val;

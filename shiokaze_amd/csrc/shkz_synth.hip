@@ -25,6 +25,7 @@ __device__ __forceinline__ uint64_t splitmix64(uint64_t x) {
 
 __device__ double fluid_sdf(const ShkzSceneParams &S, const double p[3]) {
 #pragma clang fp contract(off)
+    if (S.all_fluid) return -1.0;
     const double q[3] = {p[0] - S.shift[0], p[1] - S.shift[1], p[2] - S.shift[2]};
     double v = 1e9;
     if (S.pool >= 0.0) v = fmin(v, q[1] - S.pool);
@@ -103,7 +104,7 @@ extern "C" int shkz_scene_fill_device(const shkz_scene_desc *desc, void *fluid, 
     hipStream_t st = (hipStream_t)stream;
     const size_t M = (size_t)W * H * D, Nn = (size_t)(W + 1) * (H + 1) * (D + 1);
     if (fluid)
-        k_fill_fluid<<<blocks(M), 256, 0, st>>>(S, W, H, D, dx, with_solid, desc->fluid_dtype == SHKZ_F64 ? 1 : 0,
+        k_fill_fluid<<<blocks(M), 256, 0, st>>>(S, W, H, D, dx, with_solid && !S.all_fluid, desc->fluid_dtype == SHKZ_F64 ? 1 : 0,
                                                 fluid);
     if (solid) k_fill_solid<<<blocks(Nn), 256, 0, st>>>(S, W, H, D, dx, with_solid, solid);
     const uint64_t seed = desc->velocity_seed ? desc->velocity_seed : desc->seed;

@@ -91,6 +91,22 @@ def test_scene_liquid_is_pushed_out_of_solids():
     assert not ((sc.fluid < 0) & (s_cell < -2.0 / 32)).any()
 
 
+def test_smoke_scene_is_all_fluid():
+    """SCENE_SMOKE: every cell liquid (macsmoke3.cpp:106), a consistent pure-Neumann system."""
+    import scipy.sparse as sp
+    from scipy.sparse.csgraph import connected_components
+    from oracle import oracle as O
+    sc = S.make_scene(S.SCENE_SMOKE, 24, seed=3)
+    assert (sc.fluid == -1.0).all() and sc.solid is not None and (sc.solid < 0).any()
+    A = O.assemble(sc)
+    M = sp.csr_matrix((A["val"], A["col"], A["rowptr"]), shape=(A["n"], A["n"]))
+    nc, lab = connected_components(M, directed=False)
+    for c in range(nc):  # consistent: the rhs sums to zero on every component
+        assert abs(A["b"][lab == c].sum()) <= 1e-9 * np.abs(A["b"]).sum()
+    assert np.allclose(np.asarray(M.sum(axis=1)).ravel(), 0.0, atol=1e-9)  # no Dirichlet term
+    assert (S.make_scene2d(S.SCENE_SMOKE, 30, 20, seed=3).fluid == -1.0).all()
+
+
 @pytest.mark.skipif(not os.path.isdir("/root/reference/include"), reason="Shiokaze tree not present")
 @pytest.mark.parametrize("module", ["macpressuresolver3_mi355x", "macpressuresolver2_mi355x", "pcg_mi355x"])
 def test_shiokaze_adapter_compiles_against_reference_headers(module):

@@ -34,6 +34,7 @@ SCENES = {
     "box70x90": lambda: S.make_scene2d(S.SCENE_SPHERES_BOX, 70, 90, seed=2),
     "odd": lambda: S.make_scene2d(S.SCENE_RANDOM_DROP, 37, 23, seed=9, fluid_dtype=S.F32),
 }
+OPERATOR_SCENES = {**SCENES, "smoke60x50": lambda: S.make_scene2d(S.SCENE_SMOKE, 60, 50, seed=3)}
 
 
 def rel_l2(a, b):
@@ -41,10 +42,10 @@ def rel_l2(a, b):
     return np.linalg.norm(a - b) / (nb if nb else 1.0)
 
 
-@pytest.mark.parametrize("name", list(SCENES))
+@pytest.mark.parametrize("name", list(OPERATOR_SCENES))
 def test_2d_operator_matches_oracle(ctx, name):
     say(f"2d operator {name}")
-    sc = SCENES[name]()
+    sc = OPERATOR_SCENES[name]()
     exp = ctx.export_system2d(sc)
     A = O.assemble2(sc)
     im = A["index_map"]

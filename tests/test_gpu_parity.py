@@ -58,12 +58,14 @@ SCENES = {
     "odd": lambda: S.make_scene(S.SCENE_RANDOM_DROP, 21, 13, 30, seed=9),
     "float_phi": lambda: S.make_scene(S.SCENE_DROP_SOLIDS, 33, seed=5, fluid_dtype=S.F32),
 }
+# the all-fluid (smoke) branch: solved in test_gpu_smoke.py, its operator checked here
+OPERATOR_SCENES = {**SCENES, "smoke40": lambda: S.make_scene(S.SCENE_SMOKE, 40, seed=3)}
 
 
-@pytest.mark.parametrize("name", list(SCENES))
+@pytest.mark.parametrize("name", list(OPERATOR_SCENES))
 def test_operator_matches_oracle(ctx, name):
     say(f"operator {name}")
-    sc = SCENES[name]()
+    sc = OPERATOR_SCENES[name]()
     exp = ctx.export_system(sc)
     A = O.assemble(sc)
     im = A["index_map"]
