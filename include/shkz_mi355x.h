@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define SHKZ_ABI_VERSION 2
+#define SHKZ_ABI_VERSION 3
 
 /* ---------------------------------------------------------------- enums */
 enum shkz_status {
@@ -84,6 +84,11 @@ typedef struct shkz_grid {
     double *pressure64;       /* optional out: the FP64 solution (cells) */
     uint64_t scene_id;        /* 0 = stateless; otherwise keys warm-start pressure and the
                                  volume-correction integrator kept on the device across calls */
+    uint8_t *active_out_u;    /* optional out (3D grids): 1 where the face is active after the */
+    uint8_t *active_out_v;    /* projection, 0 where it is switched off or was inactive: open */
+    uint8_t *active_out_w;    /* faces and closed boundary faces with liquid inside stay active
+                                 (macpressuresolver3.cpp:252-266).  The input of the velocity
+                                 extrapolation (shkz_extrapolate_velocity). */
 } shkz_grid;
 
 /* ---------------------------------------------------------------- parameters */
@@ -205,6 +210,38 @@ int shkz_release_scene(shkz_ctx *ctx, uint64_t scene_id);
 
 /* Device bytes currently held by the context. */
 uint64_t shkz_device_bytes(const shkz_ctx *ctx);
+
+/* ------------------------------------------------------- after the projection
+ * Velocity extrapolation and the solid constraint: macutility3::extrapolate_and_constrain_velocity
+ * (src/utility/macutility3.cpp:89-93), as macliquid3::extend_both (src/liquid/macliquid3.cpp:309-319)
+ * and macsmoke3 (src/smoke/macsmoke3.cpp:294) call it on the velocity the projection left.
+ *  - `width` passes of macarray_extrapolator3::extrapolate (include/shiokaze/array/
+ *    array_extrapolator3.h:47-80): each pass gives every inactive face with an active
+ *    6-neighbour the mean of its active neighbours as they were before the pass, and activates it;
+ *  - with `constrain`, macutility3::constrain_velocity (:61-88): where the solid level set has a
+ *    negative value, an active face whose centre samples the solid below 0 loses the component of
+ *    the velocity along the solid's normal when the velocity points into the solid, and active
+ *    boundary faces pointing out of the domain become 0.  Nothing is constrained when `solid` is
+ *    NULL or has no negative value.
+ * Inactive faces read as 0 wherever the reference reads an inactive face (its background value).
+ * The activity arrays are the projection's active_out_* (or the caller's own), updated in place.
+ * Results are bit-identical to the reference's arithmetic (float velocities and level set,
+ * double normals).  With `constrain`, every axis needs at least 2 cells. */
+typedef struct shkz_velocity_field {
+    int32_t w, h, d;              /* cells */
+    int32_t solid_dtype;          /* SHKZ_F32 (the reference's Real) or SHKZ_F64 */
+    double dx;                    /* 0 = min(1/w, 1/h, 1/d) */
+    const void *solid;            /* nodal solid level set ((w+1)(h+1)(d+1)) or NULL */
+    float *u, *v, *w_vel;         /* face velocities, updated in place */
+    uint8_t *active_u, *active_v, *active_w; /* face activity (0/1), updated in place */
+} shkz_velocity_field;
+
+/* Host arrays.  width: 0..254 passes. */
+int shkz_extrapolate_velocity(shkz_ctx *ctx, const shkz_velocity_field *field, int32_t width, int32_t constrain);
+/* Device arrays, ordered on `stream` like shkz_project_device (so it may directly follow a
+ * projection of the same arrays on the same stream). */
+int shkz_extrapolate_velocity_device(shkz_ctx *ctx, const shkz_velocity_field *field, int32_t width,
+                                     int32_t constrain, void *stream);
 
 /* ------------------------------------------------------- 2D (macpressuresolver2)
  * macproject2_interface::project (include/shiokaze/projection/macproject2_interface.h:67-72, implemented

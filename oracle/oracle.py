@@ -38,6 +38,11 @@ class OGrid2(ctypes.Structure):
                 ("vel", _vp * 2), ("active", _vp * 2)]
 
 
+class OVField(ctypes.Structure):
+    _fields_ = [("w", ctypes.c_int), ("h", ctypes.c_int), ("d", ctypes.c_int), ("dx", ctypes.c_double),
+                ("solid", _vp), ("solid_f64", ctypes.c_int), ("vel", _vp * 3), ("active", _vp * 3)]
+
+
 class OParams(ctypes.Structure):
     _fields_ = [("residual", ctypes.c_double), ("max_iterations", ctypes.c_int), ("mic_mode", ctypes.c_int),
                 ("mic_omega", ctypes.c_double), ("mic_min_diag_ratio", ctypes.c_double),
@@ -87,7 +92,7 @@ def lib():
         L.oracle_pcg.argtypes = [ctypes.c_int64, _vp, _vp, _vp, _vp, _vp, ctypes.c_double, ctypes.c_int, ctypes.c_int,
                                  ctypes.c_double, ctypes.c_double, ctypes.POINTER(ctypes.c_int), _dp]
         L.oracle_pcg.restype = ctypes.c_int
-        L.oracle_velocity_update.argtypes = [ctypes.POINTER(OGrid), _vp, _vp * 3, _vp * 3]
+        L.oracle_velocity_update.argtypes = [ctypes.POINTER(OGrid), _vp, _vp * 3, _vp * 3, _vp]
         L.oracle_divergence.argtypes = [ctypes.POINTER(OGrid), _vp, _vp * 3, _vp]
         L.oracle_divergence.restype = ctypes.c_double
         L.oracle_project.argtypes = [ctypes.POINTER(OGrid), ctypes.POINTER(OParams), _vp, _vp, ctypes.POINTER(OResult)]
@@ -103,6 +108,12 @@ def lib():
         L.oracle2_divergence.restype = ctypes.c_double
         L.oracle2_project.argtypes = [ctypes.POINTER(OGrid2), ctypes.POINTER(OParams), _vp, _vp, ctypes.POINTER(OResult)]
         L.oracle_project.restype = ctypes.c_int
+        L.oracle_interp_coef.argtypes = [_vp, _vp, _vp, _vp]
+        L.oracle_deriv_coef.argtypes = [_vp, _vp, _vp, _vp]
+        L.oracle_extrapolate.argtypes = [ctypes.POINTER(OVField), ctypes.c_int]
+        L.oracle_extrapolate.restype = ctypes.c_int64
+        L.oracle_constrain_velocity.argtypes = [ctypes.POINTER(OVField)]
+        L.oracle_constrain_velocity.restype = ctypes.c_int
         _lib = L
     return _lib
 
@@ -120,6 +131,8 @@ def ref():
         R.ref_box.restype = ctypes.c_double
         R.ref_spmv.argtypes = [ctypes.c_int64, ctypes.c_int64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]
         R.ref_spmv.restype = ctypes.c_int64
+        R.ref_interp_coef.argtypes = [_vp, _vp, _vp, _vp]
+        R.ref_deriv_coef.argtypes = [_vp, _vp, _vp, _vp]
         _ref = R
     return _ref
 
@@ -218,11 +231,15 @@ def project(scene, params=None, pressure64=True):
     return r, P, P64
 
 
-def velocity_update(scene, P, area, rho):
-    g, keep = make_grid(scene)
+def velocity_update(scene, P, area, rho, active=None, active_out=False):
+    """The velocity update in place; with active_out, returns the post-update face activity."""
+    g, keep = make_grid(scene, active=active)
     A = (_vp * 3)(*[a.ctypes.data for a in area])
     R = (_vp * 3)(*[a.ctypes.data for a in rho])
-    lib().oracle_velocity_update(ctypes.byref(g), np.ascontiguousarray(P, np.float32).ctypes.data, A, R)
+    act = [np.zeros(s, np.uint8) for s in face_shapes(scene.w, scene.h, scene.d)] if active_out else None
+    AO = (_vp * 3)(*[a.ctypes.data for a in act]) if act else None
+    lib().oracle_velocity_update(ctypes.byref(g), np.ascontiguousarray(P, np.float32).ctypes.data, A, R, AO)
+    return act
 
 
 def divergence(scene, index_map, area):
@@ -328,3 +345,58 @@ def divergence2(scene, index_map, area):
     div = np.zeros((scene.w, scene.h), np.float64)
     mx = lib().oracle2_divergence(ctypes.byref(g), index_map.ctypes.data, A, div.ctypes.data)
     return mx, div
+
+
+# ------------------------------------------------ velocity extrapolation and solid constraint
+def _coef(fn, shape, p, ncoef):
+    sh = np.ascontiguousarray(shape, np.int32)
+    pp = np.ascontiguousarray(p, np.float64)
+    idx = np.zeros(24, np.int32)
+    coef = np.zeros(ncoef, np.float64)
+    fn(sh.ctypes.data, pp.ctypes.data, idx.ctypes.data, coef.ctypes.data)
+    return idx.reshape(8, 3), coef
+
+
+def interp_coef(shape, p):
+    """array_interpolator3::interpolate_coef: (8x3 corner indices, 8 weights)."""
+    return _coef(lib().oracle_interp_coef, shape, p, 8)
+
+
+def deriv_coef(shape, p):
+    """array_derivative3::derivative_interpolate_coef: (8x3 corner indices, 3x8 weights)."""
+    idx, c = _coef(lib().oracle_deriv_coef, shape, p, 24)
+    return idx, c.reshape(3, 8)
+
+
+def ref_interp_coef(shape, p):
+    return _coef(ref().ref_interp_coef, shape, p, 8)
+
+
+def ref_deriv_coef(shape, p):
+    idx, c = _coef(ref().ref_deriv_coef, shape, p, 24)
+    return idx, c.reshape(3, 8)
+
+
+def make_vfield(w, h, d, vel, active, solid=None, dx=None):
+    """OVField over dense face arrays (float32) and activity (uint8), modified in place."""
+    for a in list(vel) + list(active):
+        assert a.flags["C_CONTIGUOUS"]
+    f = OVField(w=w, h=h, d=d, dx=dx if dx is not None else min(1.0 / w, 1.0 / h, 1.0 / d), solid=_p(solid),
+                solid_f64=int(solid is not None and solid.dtype == np.float64))
+    for a in range(3):
+        assert vel[a].dtype == np.float32 and active[a].dtype == np.uint8
+        f.vel[a] = _p(vel[a])
+        f.active[a] = _p(active[a])
+    return f
+
+
+def extrapolate(w, h, d, vel, active, width):
+    """macarray_extrapolator3::extrapolate(velocity, width) in place; returns faces activated."""
+    f = make_vfield(w, h, d, vel, active)
+    return lib().oracle_extrapolate(ctypes.byref(f), int(width))
+
+
+def constrain_velocity(w, h, d, vel, active, solid, dx=None):
+    """macutility3::constrain_velocity in place; returns whether the solid exists."""
+    f = make_vfield(w, h, d, vel, active, solid, dx)
+    return lib().oracle_constrain_velocity(ctypes.byref(f))

@@ -5,10 +5,16 @@
 //   SparseMatrix::add_to_element,
 //   FixedSparseMatrix::construct_from_matrix, multiply
 //                                        local/include/pcgsolver/sparse_matrix.h:75-90,217-275
+//   array_interpolator3::interpolate_coef
+//                                        include/shiokaze/array/array_interpolator3.h:40-62
+//   array_derivative3::derivative_interpolate_coef
+//                                        include/shiokaze/array/array_derivative3.h:40-85
 // The reference's solver header (pcg_solver.h) needs GSL's cblas headers, which this image
 // lacks, so it is not built here.
 #include <shiokaze/utility/utility.h>
 #include <pcgsolver/sparse_matrix.h>
+#include <shiokaze/array/array_interpolator3.h>
+#include <shiokaze/array/array_derivative3.h>
 #include <cstdint>
 #include <vector>
 
@@ -44,4 +50,25 @@ extern "C" int64_t ref_spmv(int64_t n, int64_t nt, const int64_t *ti, const int6
         }
     }
     return (int64_t)f.colindex.size();
+}
+
+// Corner indices (idx[3n+a]) and trilinear weights at p for an array of the given shape.
+extern "C" void ref_interp_coef(const int shape[3], const double p[3], int idx[24], double coef[8]) {
+    vec3i ind[8];
+    array_interpolator3::interpolate_coef(shape3(shape[0], shape[1], shape[2]), vec3d(p[0], p[1], p[2]), ind, coef);
+    for (int n = 0; n < 8; ++n)
+        for (int a = 0; a < 3; ++a) idx[3 * n + a] = ind[n][a];
+}
+
+// Corner indices and the derivative weights coef[8*dim+n] at p.
+extern "C" void ref_deriv_coef(const int shape[3], const double p[3], int idx[24], double coef[24]) {
+    vec3i ind[8];
+    double c[DIM3][8];
+    array_derivative3::derivative_interpolate_coef(shape3(shape[0], shape[1], shape[2]), vec3d(p[0], p[1], p[2]),
+                                                   ind, c);
+    for (int n = 0; n < 8; ++n)
+        for (int a = 0; a < 3; ++a) {
+            idx[3 * n + a] = ind[n][a];
+            coef[8 * a + n] = c[a][n];
+        }
 }

@@ -698,9 +698,12 @@ done:
 
 /* macpressuresolver3.cpp:252-268 on dense faces.  Open faces (area>0 and rho>0) on the domain
  * boundary become 0; interior ones lose dt*(P(c)-P(c-e))/(rho*dx), the difference taken in
- * float as the reference subtracts two Real values; every other face is switched off and
- * reads back 0. */
-void oracle_velocity_update(const oracle_grid *g, const float *P, float *area[3], float *rho[3])
+ * float as the reference subtracts two Real values; every other face becomes 0 and is switched
+ * off, except a boundary face whose inside cell holds liquid, which stays active at 0
+ * (:263-265).  active_out (optional) receives the activity after the update: 1 for open faces
+ * and those boundary faces, 0 otherwise and for faces that were inactive. */
+void oracle_velocity_update(const oracle_grid *g, const float *P, float *area[3], float *rho[3],
+                            uint8_t *active_out[3])
 {
     const int N[3] = {g->w, g->h, g->d};
     const double dt = g->dt, dx = g->dx;
@@ -714,8 +717,13 @@ void oracle_velocity_update(const oracle_grid *g, const float *P, float *area[3]
                 for (int k = 0; k < D; ++k) {
                     const int pi[3] = {i, j, k};
                     size_t f = fidx(g, dim, i, j, k);
-                    if (act && !act[f]) continue;
+                    uint8_t *aout = active_out ? active_out[dim] : NULL;
+                    if (act && !act[f]) {
+                        if (aout) aout[f] = 0;
+                        continue;
+                    }
                     double r = rho[dim][f];
+                    uint8_t keep = 1;
                     if (area[dim][f] && r) {
                         if (pi[dim] == 0 || pi[dim] == N[dim]) U[f] = 0.0f;
                         else {
@@ -725,7 +733,11 @@ void oracle_velocity_update(const oracle_grid *g, const float *P, float *area[3]
                         }
                     } else {
                         U[f] = 0.0f;
+                        if (pi[dim] == 0) keep = phi_at(g, cidx(g, i, j, k)) < 0.0;
+                        else if (pi[dim] == N[dim]) keep = phi_at(g, cidx(g, i - (dim == 0), j - (dim == 1), k - (dim == 2))) < 0.0;
+                        else keep = 0;
                     }
+                    if (aout) aout[f] = keep;
                 }
     }
 }
@@ -815,7 +827,7 @@ int oracle_project(const oracle_grid *g, const oracle_params *p, float *pressure
         pressure[c] = r >= 0 ? (float)x[r] : 0.0f;
         if (pressure64) pressure64[c] = r >= 0 ? x[r] : 0.0;
     }
-    oracle_velocity_update(g, pressure, area, rho);
+    oracle_velocity_update(g, pressure, area, rho, NULL);
     const double t4 = now_s();
     res->iterations = iters;
     res->reresid = reresid;

@@ -105,7 +105,8 @@ int oracle_pcg(int64_t n, const int64_t *rowptr, const int64_t *col, const doubl
                double omega, double min_diag_ratio, int *iterations, double *reresid);
 
 /* macpressuresolver3.cpp:252-268 velocity update with pressure stored in float. */
-void oracle_velocity_update(const oracle_grid *g, const float *pressure, float *area[3], float *rho[3]);
+void oracle_velocity_update(const oracle_grid *g, const float *pressure, float *area[3], float *rho[3],
+                            uint8_t *active_out[3]);
 
 /* Divergence sum_f sgn_f area_f u_f / dx over the faces of every unknown (negated rhs formula of
  * macpressuresolver3.cpp:193).  div may be NULL; returns max |div|. */
@@ -144,6 +145,27 @@ void oracle2_velocity_update(const oracle_grid2 *g, const float *pressure, float
 double oracle2_divergence(const oracle_grid2 *g, const int64_t *index_map, float *area[2], double *div);
 int oracle2_project(const oracle_grid2 *g, const oracle_params *p, float *pressure, double *pressure64,
                     oracle_result *res);
+
+/* ---------------------------------------- velocity extrapolation and solid constraint
+ * (shkz_oracle_vel.c): the step after a projection, macutility3.cpp:61-93. */
+typedef struct {
+    int w, h, d;            /* cells */
+    double dx;
+    const void *solid;      /* nodal level set or NULL */
+    int solid_f64;          /* 0: float (Real), 1: double */
+    float *vel[3];          /* face velocities (in/out) */
+    uint8_t *active[3];     /* face activity (in/out), required */
+} oracle_vfield;
+
+/* array_interpolator3.h:40-62: the 8 corner indices (idx[3n+a]) and weights at p, index space of
+ * an array of the given shape */
+void oracle_interp_coef(const int shape[3], const double p[3], int idx[24], double coef[8]);
+/* array_derivative3.h:40-85: corner indices and the three derivative weight sets coef[8*dim+n] */
+void oracle_deriv_coef(const int shape[3], const double p[3], int idx[24], double coef[24]);
+/* macarray_extrapolator3::extrapolate(velocity, width); returns the faces activated */
+int64_t oracle_extrapolate(const oracle_vfield *f, int width);
+/* macutility3::constrain_velocity; returns 1 when the solid level set exists */
+int oracle_constrain_velocity(const oracle_vfield *f);
 
 #ifdef __cplusplus
 }

@@ -19,7 +19,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "lib", "libshkz_mi355x.so")
 
-ABI_VERSION = 2  # the layout of the structs below
+ABI_VERSION = 3  # the layout of the structs below
 F32, F64 = 0, 1
 SOLVER_MGPCG, SOLVER_CG = 0, 1
 PREC_MIXED, PREC_FP64 = 0, 1
@@ -42,6 +42,7 @@ class Grid(ctypes.Structure):
         ("active_u", _vp), ("active_v", _vp), ("active_w", _vp),
         ("pressure", _vp), ("pressure64", _vp),
         ("scene_id", ctypes.c_uint64),
+        ("active_out_u", _vp), ("active_out_v", _vp), ("active_out_w", _vp),
     ]
 
 
@@ -80,6 +81,15 @@ class Result(ctypes.Structure):
 
     def as_dict(self) -> dict:
         return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class VelocityField(ctypes.Structure):
+    _fields_ = [
+        ("w", ctypes.c_int32), ("h", ctypes.c_int32), ("d", ctypes.c_int32), ("solid_dtype", ctypes.c_int32),
+        ("dx", ctypes.c_double), ("solid", _vp),
+        ("u", _vp), ("v", _vp), ("w_vel", _vp),
+        ("active_u", _vp), ("active_v", _vp), ("active_w", _vp),
+    ]
 
 
 class Grid2D(ctypes.Structure):
@@ -146,7 +156,7 @@ EXPORTS = [
     "shkz_wait", "shkz_server_release_scene", "shkz_server_get_stats", "shkz_scene_fill_device",
     "shkz_project2d", "shkz_project2d_device", "shkz_project2d_batch_device", "shkz_export_system2d",
     "shkz_scene_fill2d", "shkz_linsolve_default_params", "shkz_linsolver_create", "shkz_linsolver_destroy",
-    "shkz_linsolve_csr",
+    "shkz_linsolve_csr", "shkz_extrapolate_velocity", "shkz_extrapolate_velocity_device",
 ]
 
 _lib = None
@@ -204,8 +214,18 @@ def lib():
     L.shkz_linsolver_destroy.argtypes = [_vp]
     L.shkz_linsolve_csr.argtypes = [_vp, ctypes.c_int64, _vp, _vp, _vp, _vp, _vp, ctypes.POINTER(LinsolveParams),
                                     ctypes.POINTER(LinsolveResult)]
+    L.shkz_extrapolate_velocity.argtypes = [_vp, ctypes.POINTER(VelocityField), ctypes.c_int32, ctypes.c_int32]
+    L.shkz_extrapolate_velocity_device.argtypes = [_vp, ctypes.POINTER(VelocityField), ctypes.c_int32, ctypes.c_int32,
+                                                   _vp]
     _lib = L
     return L
+
+
+def _dp(x):
+    """Device pointer of an int or an object with data_ptr() (None stays None)."""
+    if x is None:
+        return None
+    return x.data_ptr() if hasattr(x, "data_ptr") else int(x)
 
 
 def _ptr(a):
@@ -495,9 +515,14 @@ class Context:
 
     # ------------------------------------------------------------------ projection
     def project(self, scene: Scene, params: Params | None = None, pressure: bool = True, pressure64: bool = False,
-                scene_id: int = 0, active=None, check: bool = True):
-        """Projects scene's velocities in place (host arrays).  Returns (result, P32, P64)."""
+                scene_id: int = 0, active=None, active_out=None, check: bool = True):
+        """Projects scene's velocities in place (host arrays).  Returns (result, P32, P64).
+        active_out: optional three uint8 face arrays that receive the post-projection activity."""
         g = self._grid(scene, scene_id, active)
+        if active_out is not None:
+            for a in active_out:
+                assert a.dtype == np.uint8 and a.flags["C_CONTIGUOUS"]
+            g.active_out_u, g.active_out_v, g.active_out_w = (_ptr(a) for a in active_out)
         P = np.zeros((scene.w, scene.h, scene.d), np.float32) if pressure else None
         P64 = np.zeros((scene.w, scene.h, scene.d), np.float64) if pressure64 else None
         g.pressure = _ptr(P)
@@ -533,15 +558,16 @@ class Context:
 
     @staticmethod
     def device_grid(w, h, d, dt, fluid, fluid_dtype, solid, solid_dtype, u, v, wv, pressure=None, pressure64=None,
-                    scene_id=0, dx=0.0) -> Grid:
+                    scene_id=0, dx=0.0, active=None, active_out=None) -> Grid:
         """A Grid of device-resident buffers (ints or objects with data_ptr(), e.g. devmem.DeviceArray)."""
-        def dp(x):
-            if x is None:
-                return None
-            return x.data_ptr() if hasattr(x, "data_ptr") else int(x)
-        return Grid(w=w, h=h, d=d, dx=dx, dt=dt, fluid=dp(fluid), fluid_dtype=fluid_dtype, solid_dtype=solid_dtype,
-                    solid=dp(solid), u=dp(u), v=dp(v), w_vel=dp(wv), pressure=dp(pressure),
-                    pressure64=dp(pressure64), scene_id=scene_id)
+        g = Grid(w=w, h=h, d=d, dx=dx, dt=dt, fluid=_dp(fluid), fluid_dtype=fluid_dtype, solid_dtype=solid_dtype,
+                 solid=_dp(solid), u=_dp(u), v=_dp(v), w_vel=_dp(wv), pressure=_dp(pressure),
+                 pressure64=_dp(pressure64), scene_id=scene_id)
+        if active is not None:
+            g.active_u, g.active_v, g.active_w = (_dp(a) for a in active)
+        if active_out is not None:
+            g.active_out_u, g.active_out_v, g.active_out_w = (_dp(a) for a in active_out)
+        return g
 
     def project_batch_device(self, grids, params: Params | None = None, stream=None, check: bool = True):
         """Batch projection of device-resident grids (a list of Grid from device_grid())."""
@@ -556,10 +582,11 @@ class Context:
         return list(res)
 
     def project_device(self, w, h, d, dt, fluid, fluid_dtype, solid, solid_dtype, u, v, wv, params=None,
-                       pressure=None, pressure64=None, scene_id=0, stream=None, dx=0.0, check=True):
+                       pressure=None, pressure64=None, scene_id=0, stream=None, dx=0.0, active=None, active_out=None,
+                       check=True):
         """Projection on device-resident buffers (ints or objects with data_ptr(), e.g. devmem.DeviceArray)."""
         g = self.device_grid(w, h, d, dt, fluid, fluid_dtype, solid, solid_dtype, u, v, wv, pressure, pressure64,
-                             scene_id, dx)
+                             scene_id, dx, active, active_out)
         r = Result()
         prm = params if params is not None else default_params()
         rc = self._L.shkz_project_device(self.handle, ctypes.byref(g), ctypes.byref(prm), ctypes.byref(r),
@@ -567,6 +594,38 @@ class Context:
         if rc != 0 and check:
             raise ShkzError(rc, last_error())
         return r
+
+    # ------------------------------------------------------------ after the projection
+    def extrapolate_velocity(self, w, h, d, vel, active, solid=None, width=1, constrain=True, dx=0.0,
+                             check=True) -> int:
+        """Velocity extrapolation (`width` passes) and, with `constrain`, the solid constraint on host
+        arrays, in place: vel = (u, v, w) float32, active = three uint8 arrays, solid nodal or None."""
+        for a in list(vel) + list(active):
+            assert a.flags["C_CONTIGUOUS"]
+        for a in vel:
+            assert a.dtype == np.float32
+        for a in active:
+            assert a.dtype == np.uint8
+        f = VelocityField(w=w, h=h, d=d, dx=dx, solid=_ptr(solid),
+                          solid_dtype=F64 if (solid is not None and solid.dtype == np.float64) else F32,
+                          u=_ptr(vel[0]), v=_ptr(vel[1]), w_vel=_ptr(vel[2]), active_u=_ptr(active[0]),
+                          active_v=_ptr(active[1]), active_w=_ptr(active[2]))
+        rc = self._L.shkz_extrapolate_velocity(self.handle, ctypes.byref(f), int(width), int(bool(constrain)))
+        if rc != 0 and check:
+            raise ShkzError(rc, last_error())
+        return rc
+
+    def extrapolate_velocity_device(self, w, h, d, vel, active, solid=None, solid_dtype=F32, width=1,
+                                    constrain=True, dx=0.0, stream=None, check=True) -> int:
+        """The same on device buffers (ints or objects with data_ptr()), ordered on `stream`."""
+        f = VelocityField(w=w, h=h, d=d, dx=dx, solid=_dp(solid), solid_dtype=solid_dtype, u=_dp(vel[0]),
+                          v=_dp(vel[1]), w_vel=_dp(vel[2]), active_u=_dp(active[0]), active_v=_dp(active[1]),
+                          active_w=_dp(active[2]))
+        rc = self._L.shkz_extrapolate_velocity_device(self.handle, ctypes.byref(f), int(width), int(bool(constrain)),
+                                                      None if stream is None else int(stream))
+        if rc != 0 and check:
+            raise ShkzError(rc, last_error())
+        return rc
 
     def _grid(self, scene: Scene, scene_id: int, active) -> Grid:
         for a in (scene.fluid, scene.u, scene.v, scene.wv):

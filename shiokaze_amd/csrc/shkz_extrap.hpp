@@ -1,0 +1,282 @@
+// shkz_extrap.hpp -- the step after a projection in Shiokaze's time steppers, on the device:
+// velocity extrapolation and the solid constraint, macutility3::extrapolate_and_constrain_velocity
+// (src/utility/macutility3.cpp:89-93), which macliquid3::extend_both (macliquid3.cpp:309-319) and
+// macsmoke3 (macsmoke3.cpp:294) call on the velocity the projection left.
+//
+//   k_extrap_pass   one pass of array_extrapolator3::extrapolate (array_extrapolator3.h:47-80,
+//                   over dilate3.h:43-101) on all three face arrays
+//   k_constrain     macutility3::constrain_velocity (macutility3.cpp:61-88) with the
+//                   interpolation of array_interpolator3.h:40-97, array_derivative3.h:40-103 and
+//                   macarray_interpolator3.h:43-50
+//
+// Layout: the dense face arrays of the projection (u (w+1,h,d), v (w,h+1,d), w (w,h,d+1), z
+// fastest) with one activity byte per face.  One launch covers the three arrays (a flat index
+// over u, then v, then w).  A pass works in place: an inactive face reads only neighbours that
+// were active before the pass, which the activity byte records as the pass that activated it
+// (0 inactive, 1 active at the start, p+1 activated by pass p), so a pass needs no copy; the
+// constraint kernel turns the codes back into 0/1.
+//
+// Arithmetic matches the reference's types operation by operation (no contraction): float sums
+// of float velocities, interpolation into the array's element type after a double product, vec3d
+// normals and dot products, the result rounded to float.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace shkz {
+
+struct VelField {
+    int w, h, d;
+    int solid64;          // solid level set in double (else float, the reference's Real)
+    double dx;
+    const void *solid;    // nodal (w+1,h+1,d+1) or nullptr
+    float *vel[3];
+    uint8_t *act[3];
+    size_t n[3];          // faces per array
+};
+
+__device__ __forceinline__ void vf_shape(const VelField &F, int a, int s[3]) {
+    s[0] = F.w + (a == 0);
+    s[1] = F.h + (a == 1);
+    s[2] = F.d + (a == 2);
+}
+
+// flat index over the three arrays -> (array, local index, i, j, k); false past the end
+__device__ __forceinline__ bool vf_decode(const VelField &F, size_t e, int &a, size_t &f, int &i, int &j, int &k) {
+    if (e < F.n[0]) {
+        a = 0;
+        f = e;
+    } else if (e < F.n[0] + F.n[1]) {
+        a = 1;
+        f = e - F.n[0];
+    } else if (e < F.n[0] + F.n[1] + F.n[2]) {
+        a = 2;
+        f = e - F.n[0] - F.n[1];
+    } else {
+        return false;
+    }
+    int s[3];
+    vf_shape(F, a, s);
+    k = (int)(f % (size_t)s[2]);
+    j = (int)((f / (size_t)s[2]) % (size_t)s[1]);
+    i = (int)(f / ((size_t)s[2] * s[1]));
+    return true;
+}
+
+// One extrapolation pass (1-based `pass`): every inactive face with an in-bounds neighbour that
+// was active before the pass takes the mean of those neighbours, summed in float in the order
+// +x, -x, +y, -y, -z, +z (array_extrapolator3.h:57-72), and is marked activated by this pass.
+__global__ void k_extrap_pass(VelField F, int pass) {
+#pragma clang fp contract(off)
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int a, i, j, k;
+    size_t f;
+    if (!vf_decode(F, e, a, f, i, j, k)) return;
+    const uint8_t *A = F.act[a];
+    if (A[f] != 0) return;
+    int s[3];
+    vf_shape(F, a, s);
+    const size_t sx = (size_t)s[1] * s[2], sy = (size_t)s[2];
+    const float *U = F.vel[a];
+    const bool ok[6] = {i + 1 < s[0], i > 0, j + 1 < s[1], j > 0, k > 0, k + 1 < s[2]};
+    const long off[6] = {(long)sx, -(long)sx, (long)sy, -(long)sy, -1, 1};
+    float sum = 0.0f;
+    int weight = 0;
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+        if (!ok[q]) continue;
+        const size_t g = (size_t)((long)f + off[q]);
+        const int t = A[g];
+        if (t != 0 && t <= pass) {
+            sum += U[g];
+            weight++;
+        }
+    }
+    if (weight) {
+        // float / int in the reference; the double quotient rounded once to float is the
+        // correctly rounded float quotient
+        F.vel[a][f] = (float)((double)sum / (double)weight);
+        F.act[a][f] = (uint8_t)(pass + 1);
+    }
+}
+
+__device__ __forceinline__ double dmin_(double a, double b) { return b < a ? b : a; }  // std::min
+__device__ __forceinline__ double dmax_(double a, double b) { return a < b ? b : a; }  // std::max
+
+// Corner base index and clamped position of array_interpolator3::interpolate_coef (:40-62).
+struct Corner {
+    int i, j, k;
+    double x, y, z;
+};
+__device__ __forceinline__ Corner corner_of(const int s[3], const double p[3]) {
+#pragma clang fp contract(off)
+    Corner c;
+    c.x = dmax_(0.0, dmin_(s[0] - 1., p[0]));
+    c.y = dmax_(0.0, dmin_(s[1] - 1., p[1]));
+    c.z = dmax_(0.0, dmin_(s[2] - 1., p[2]));
+    c.i = (int)dmin_(c.x, s[0] - 2.);
+    c.j = (int)dmin_(c.y, s[1] - 2.);
+    c.k = (int)dmin_(c.z, s[2] - 2.);
+    return c;
+}
+
+// Trilinear weights in corner order (i,j,k), (i+1,j,k), (i,j+1,k), (i+1,j+1,k), then k+1.
+__device__ __forceinline__ void interp_weights(const Corner &c, double w[8]) {
+#pragma clang fp contract(off)
+    const double zk1 = (c.k + 1) - c.z, zk = c.z - c.k, xi1 = (c.i + 1) - c.x, xi = c.x - c.i, yj1 = (c.j + 1) - c.y,
+                 yj = c.y - c.j;
+    w[0] = zk1 * xi1 * yj1;
+    w[1] = zk1 * xi * yj1;
+    w[2] = zk1 * xi1 * yj;
+    w[3] = zk1 * xi * yj;
+    w[4] = zk * xi1 * yj1;
+    w[5] = zk * xi * yj1;
+    w[6] = zk * xi1 * yj;
+    w[7] = zk * xi * yj;
+}
+
+// Derivative weights of array_derivative3::derivative_interpolate_coef (:40-85), dw[8*dim+n].
+__device__ __forceinline__ void deriv_weights(const Corner &c, double dw[24]) {
+#pragma clang fp contract(off)
+    const double zk1 = (c.k + 1) - c.z, zk = c.z - c.k, xi1 = (c.i + 1) - c.x, xi = c.x - c.i, yj1 = (c.j + 1) - c.y,
+                 yj = c.y - c.j;
+    dw[0] = -zk1 * yj1;
+    dw[1] = zk1 * yj1;
+    dw[2] = -zk1 * yj;
+    dw[3] = zk1 * yj;
+    dw[4] = -zk * yj1;
+    dw[5] = zk * yj1;
+    dw[6] = -zk * yj;
+    dw[7] = zk * yj;
+    dw[8] = -zk1 * xi1;
+    dw[9] = -zk1 * xi;
+    dw[10] = zk1 * xi1;
+    dw[11] = zk1 * xi;
+    dw[12] = -zk * xi1;
+    dw[13] = -zk * xi;
+    dw[14] = zk * xi1;
+    dw[15] = zk * xi;
+    dw[16] = -xi1 * yj1;
+    dw[17] = -xi * yj1;
+    dw[18] = -xi1 * yj;
+    dw[19] = -xi * yj;
+    dw[20] = xi1 * yj1;
+    dw[21] = xi * yj1;
+    dw[22] = xi1 * yj;
+    dw[23] = xi * yj;
+}
+
+__device__ __forceinline__ size_t corner_index(const Corner &c, int n, const int s[3]) {
+    const int i = c.i + (n & 1), j = c.j + ((n >> 1) & 1), k = c.k + ((n >> 2) & 1);
+    return ((size_t)i * s[1] + j) * s[2] + k;
+}
+
+// Accumulation in the array's element type T after a double product (value += array(idx)*coef).
+template <class T>
+__device__ __forceinline__ T acc(T v, double term) {
+#pragma clang fp contract(off)
+    return (T)((double)v + term);
+}
+
+// The solid sample and its derivative at p (nodal array), in the level set's type T.
+template <class T>
+__device__ __forceinline__ double solid_sample(const VelField &F, const double p[3], double der[3], bool &inside) {
+#pragma clang fp contract(off)
+    const int s[3] = {F.w + 1, F.h + 1, F.d + 1};
+    const Corner c = corner_of(s, p);
+    const T *S = (const T *)F.solid;
+    T v[8];
+#pragma unroll
+    for (int n = 0; n < 8; ++n) v[n] = S[corner_index(c, n, s)];
+    double w[8];
+    interp_weights(c, w);
+    T val = T(0);
+#pragma unroll
+    for (int n = 0; n < 8; ++n)
+        if (w[n] != 0.0) val = acc<T>(val, (double)v[n] * w[n]);
+    inside = (double)val < 0.0;
+    if (inside) {
+        double dw[24];
+        deriv_weights(c, dw);
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            T r = T(0);
+#pragma unroll
+            for (int n = 0; n < 8; ++n) r = acc<T>(r, dw[8 * a + n] * (double)v[n]);
+            der[a] = (double)r;
+        }
+    }
+    return (double)val;
+}
+
+// Component b of the saved velocity at p (macarray_interpolator3.h:43-50): inactive faces read 0.
+__device__ __forceinline__ double vel_sample(const VelField &F, const float *const Sv[3], int b, const double p[3]) {
+#pragma clang fp contract(off)
+    int s[3];
+    vf_shape(F, b, s);
+    const double pos[3] = {p[0] - 0.5 * (b != 0), p[1] - 0.5 * (b != 1), p[2] - 0.5 * (b != 2)};
+    const Corner c = corner_of(s, pos);
+    double w[8];
+    interp_weights(c, w);
+    float val = 0.0f;
+#pragma unroll
+    for (int n = 0; n < 8; ++n)
+        if (w[n] != 0.0) {
+            const size_t q = corner_index(c, n, s);
+            const float u = F.act[b][q] ? Sv[b][q] : 0.0f;
+            val = acc<float>(val, (double)u * w[n]);
+        }
+    return (double)val;
+}
+
+// Normalises the activity codes of the passes to 0/1 and, when `constrain` is set and the solid
+// level set has a negative value (*exist), applies macutility3::constrain_velocity to every active
+// face: a face whose centre samples the solid below 0 loses the component of the (saved, pre-
+// constraint) velocity along the normal when that velocity points into the solid, then an active
+// face on the domain boundary pointing out of the domain becomes 0.  Sv holds a copy of the
+// velocities taken before this kernel (the reference's velocity_save).
+__global__ void k_constrain(VelField F, const float *S0, const float *S1, const float *S2, const int *exist,
+                            int constrain) {
+#pragma clang fp contract(off)
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int a, i, j, k;
+    size_t f;
+    if (!vf_decode(F, e, a, f, i, j, k)) return;
+    const int t = F.act[a][f];
+    if (!t) return;
+    if (!constrain || !F.solid || !*exist) {
+        if (t > 1) F.act[a][f] = 1;
+        return;
+    }
+    const float *const Sv[3] = {S0, S1, S2};
+    const double p[3] = {i + 0.5 * (a != 0), j + 0.5 * (a != 1), k + 0.5 * (a != 2)};
+    double der[3] = {0.0, 0.0, 0.0};
+    bool inside;
+    if (F.solid64) solid_sample<double>(F, p, der, inside);
+    else solid_sample<float>(F, p, der, inside);
+    float out = Sv[a][f];
+    if (inside) {
+        double nrm[3];
+#pragma unroll
+        for (int b = 0; b < 3; ++b) nrm[b] = der[b] / F.dx;
+        double n2 = 0.0;
+#pragma unroll
+        for (int b = 0; b < 3; ++b) n2 += nrm[b] * nrm[b];
+        if (n2 != 0.0) {
+            double u[3];
+#pragma unroll
+            for (int b = 0; b < 3; ++b) u[b] = vel_sample(F, Sv, b, p);
+            double un = 0.0;
+#pragma unroll
+            for (int b = 0; b < 3; ++b) un += u[b] * nrm[b];
+            if (un < 0.0) out = (float)(u[a] - nrm[a] * un);
+        }
+    }
+    const int pi[3] = {i, j, k}, N[3] = {F.w, F.h, F.d};
+    if (pi[a] == 0 && out < 0.0f) out = 0.0f;
+    if (pi[a] == N[a] && out > 0.0f) out = 0.0f;
+    F.vel[a][f] = out;
+    if (t > 1) F.act[a][f] = 1;
+}
+
+}  // namespace shkz

@@ -26,6 +26,7 @@ struct GridDesc {
     int so_fluid, so_solid;
     float *vel[3];
     const uint8_t *active[3];
+    uint8_t *active_out[3]; // optional: activity after the update (nullptr = not wanted)
     int flat;            // a 2D grid (w, h) held as d = 1: solid nodes (w+1, h+1), no z faces
 };
 

@@ -1030,12 +1030,17 @@ __global__ void k_export_system(GridDesc G, LevelDesc L, GridLevel GL, OpDesc<do
 __device__ __forceinline__ void update_face(const GridDesc &G, int dim, int i, int j, int k, float p1, float p0) {
 #pragma clang fp contract(off)
     const size_t f = face_index(G, dim, i, j, k);
-    if (G.active[dim] && !G.active[dim][f]) return;
+    uint8_t *aout = G.active_out[dim];
+    if (G.active[dim] && !G.active[dim][f]) {
+        if (aout) aout[f] = 0;
+        return;
+    }
     const int pi[3] = {i, j, k};
     const int N[3] = {G.w, G.h, G.d};
     float *U = G.vel[dim];
     const float area = face_area(G, dim, i, j, k);
     const double rho = face_rho(G, dim, i, j, k);
+    uint8_t keep = 1;
     if (area != 0.0f && rho != 0.0) {
         if (pi[dim] == 0 || pi[dim] == N[dim]) {
             U[f] = 0.0f;
@@ -1046,7 +1051,12 @@ __device__ __forceinline__ void update_face(const GridDesc &G, int dim, int i, i
         }
     } else {
         U[f] = 0.0f;
+        // a closed boundary face stays active (at 0) where the cell inside holds liquid
+        if (pi[dim] == 0) keep = phi_at(G, i, j, k) < 0.0;
+        else if (pi[dim] == N[dim]) keep = phi_at(G, i - (dim == 0), j - (dim == 1), k - (dim == 2)) < 0.0;
+        else keep = 0;
     }
+    if (aout) aout[f] = keep;
 }
 
 // P is the grid's dense float pressure (0 off the unknowns), as k_scatter_pressure writes it.

@@ -16,6 +16,7 @@
 // the last batch, per-scene resident state, the replayable iteration graphs and the kernel timers.
 #include "../../include/shkz_mi355x.h"
 #include "shkz_kernels.hpp"
+#include "shkz_extrap.hpp"
 #include "shkz_serve.hpp"
 #include "shkz_error.hpp"
 
@@ -1261,6 +1262,9 @@ static int project_stages(shkz_ctx *C, const shkz_grid *grids, int ngrids, const
         G.active[0] = grid->active_u;
         G.active[1] = grid->active_v;
         G.active[2] = flat ? nullptr : grid->active_w;
+        G.active_out[0] = flat ? nullptr : grid->active_out_u;
+        G.active_out[1] = flat ? nullptr : grid->active_out_v;
+        G.active_out[2] = flat ? nullptr : grid->active_out_w;
         G.flat = flat ? 1 : 0;
         J.exist = exist_all + 4 * g;
         G.exist = J.exist;
@@ -1680,6 +1684,7 @@ static int project_host(shkz_ctx *C, const shkz_grid *grids, int ngrids, const s
     std::vector<shkz_grid> dg(grids, grids + ngrids);
     struct Back {
         float *dv[3];
+        uint8_t *dao[3];
         size_t nf[3];
         float *dp;
         double *dp64;
@@ -1728,6 +1733,18 @@ static int project_host(shkz_ctx *C, const shkz_grid *grids, int ngrids, const s
         dg[g].active_u = da[0];
         dg[g].active_v = da[1];
         dg[g].active_w = da[2];
+        uint8_t *hao[3] = {grid->active_out_u, grid->active_out_v, grid->active_out_w};
+        const char *aon[3] = {"h_aou", "h_aov", "h_aow"};
+        for (int dim = 0; dim < 3; ++dim) {
+            B.dao[dim] = nullptr;
+            if (!hao[dim] || !hv[dim] || flat) continue;
+            void *pp = nullptr;
+            SHKZ_TRY(AR.get(aon[dim] + sfx, nf[dim], &pp));
+            B.dao[dim] = (uint8_t *)pp;
+        }
+        dg[g].active_out_u = B.dao[0];
+        dg[g].active_out_v = B.dao[1];
+        dg[g].active_out_w = B.dao[2];
         B.dp = nullptr;
         B.dp64 = nullptr;
         B.M = M;
@@ -1765,8 +1782,11 @@ static int project_host(shkz_ctx *C, const shkz_grid *grids, int ngrids, const s
     for (int g = 0; g < ngrids; ++g) {
         const shkz_grid *grid = &grids[g];
         float *hv[3] = {grid->u, grid->v, grid->w_vel};
-        for (int dim = 0; dim < 3; ++dim)
+        uint8_t *hao[3] = {grid->active_out_u, grid->active_out_v, grid->active_out_w};
+        for (int dim = 0; dim < 3; ++dim) {
             if (hv[dim]) SHKZ_HIP(hipMemcpy(hv[dim], back[g].dv[dim], back[g].nf[dim] * 4, hipMemcpyDeviceToHost));
+            if (back[g].dao[dim]) SHKZ_HIP(hipMemcpy(hao[dim], back[g].dao[dim], back[g].nf[dim], hipMemcpyDeviceToHost));
+        }
         if (grid->pressure) SHKZ_HIP(hipMemcpy(grid->pressure, back[g].dp, back[g].M * 4, hipMemcpyDeviceToHost));
         if (grid->pressure64)
             SHKZ_HIP(hipMemcpy(grid->pressure64, back[g].dp64, back[g].M * 8, hipMemcpyDeviceToHost));
@@ -1849,6 +1869,71 @@ struct shkz_server {
 };
 
 // ============================================================================ C ABI
+// ============================================================================ after the projection
+// macutility3::extrapolate_and_constrain_velocity (macutility3.cpp:89-93) on the work stream:
+// `width` in-place extrapolation passes over the three face arrays, then the constraint (which
+// also normalises the activity codes of the passes).  vel/act/solid are device arrays.
+static int extrapolate_stages(shkz_ctx *C, const shkz_velocity_field *f, int width, int constrain, hipStream_t s,
+                              float *const vel[3], uint8_t *const act[3], const void *solid) {
+    Arena &AR = C->arena;
+    Launcher K{C, s};
+    VelField F;
+    F.w = f->w;
+    F.h = f->h;
+    F.d = f->d;
+    F.solid64 = f->solid_dtype == SHKZ_F64;
+    F.dx = f->dx > 0 ? f->dx : std::min(1.0 / f->w, std::min(1.0 / f->h, 1.0 / f->d));
+    F.solid = solid;
+    for (int a = 0; a < 3; ++a) {
+        F.vel[a] = vel[a];
+        F.act[a] = act[a];
+        F.n[a] = (size_t)(f->w + (a == 0)) * (f->h + (a == 1)) * (f->d + (a == 2));
+    }
+    const size_t total = F.n[0] + F.n[1] + F.n[2];
+    const size_t nodes = (size_t)(f->w + 1) * (f->h + 1) * (f->d + 1);
+    const dim3 grid((unsigned)((total + 255) / 256)), block(256);
+    int *exist = nullptr;
+    SHKZ_TRY(arena_get(AR, "vx_exist", 1, &exist));
+    K.go("reset", -1, 0, 4.0, dim3(1), dim3(256), k_reset_ints, exist, 1);
+    const bool with_solid = constrain && solid;
+    if (with_solid) {
+        const unsigned nbs = (unsigned)std::min<size_t>((nodes + 255) / 256, 4096);
+        K.go("any_negative", -1, 0, (double)nodes * (F.solid64 ? 8 : 4), dim3(nbs), dim3(256), k_any_negative, solid,
+             (int)F.solid64, nodes, exist);
+    }
+    // byte models: a pass scans the activity bytes (neighbour reads hit the cache); the
+    // constraint reads activity, the saved and the live velocity and writes the velocity, plus
+    // the nodal solid once
+    for (int p = 1; p <= width; ++p) K.go("extrap_pass", -1, 0, (double)total, grid, block, k_extrap_pass, F, p);
+    float *sv[3] = {vel[0], vel[1], vel[2]};
+    if (with_solid) {
+        const char *names[3] = {"vx_save0", "vx_save1", "vx_save2"};
+        for (int a = 0; a < 3; ++a) {
+            SHKZ_TRY(arena_get(AR, names[a], F.n[a], &sv[a]));
+            SHKZ_HIP(hipMemcpyAsync(sv[a], vel[a], F.n[a] * sizeof(float), hipMemcpyDeviceToDevice, s));
+        }
+    }
+    K.go("constrain", -1, 0, with_solid ? (double)total * 9 + (double)nodes * (F.solid64 ? 8 : 4) : (double)total,
+         grid, block, k_constrain, F, (const float *)sv[0], (const float *)sv[1], (const float *)sv[2],
+         (const int *)exist, with_solid ? 1 : 0);
+    SHKZ_HIP(hipGetLastError());
+    return SHKZ_OK;
+}
+
+static int validate_vfield(const shkz_velocity_field *f, int width, int constrain, bool device) {
+    if (!f) return fail(SHKZ_ERR_INVALID, "NULL field");
+    if (f->w <= 0 || f->h <= 0 || f->d <= 0) return fail(SHKZ_ERR_INVALID, "bad grid %dx%dx%d", f->w, f->h, f->d);
+    if (width < 0 || width > 254) return fail(SHKZ_ERR_INVALID, "width %d outside 0..254", width);
+    if (!f->u || !f->v || !f->w_vel || !f->active_u || !f->active_v || !f->active_w)
+        return fail(SHKZ_ERR_INVALID, "velocity and activity arrays are required");
+    if (f->solid_dtype != SHKZ_F32 && f->solid_dtype != SHKZ_F64)
+        return fail(SHKZ_ERR_INVALID, "unknown solid dtype %d", f->solid_dtype);
+    if (constrain && f->solid && (f->w < 2 || f->h < 2 || f->d < 2))
+        return fail(SHKZ_ERR_INVALID, "the constraint needs at least 2 cells per axis (got %dx%dx%d)", f->w, f->h, f->d);
+    (void)device;
+    return SHKZ_OK;
+}
+
 extern "C" {
 
 int shkz_abi_version(void) { return SHKZ_ABI_VERSION; }
@@ -1959,6 +2044,67 @@ int shkz_project_batch_device(shkz_ctx *C, const shkz_grid *grids, int ngrids, c
     const shkz_params *p = params ? params : &def;
     std::lock_guard<std::mutex> lock(C->mu);
     return project_impl(C, grids, ngrids, p, results, (hipStream_t)stream);
+}
+
+int shkz_extrapolate_velocity_device(shkz_ctx *C, const shkz_velocity_field *f, int32_t width, int32_t constrain,
+                                     void *stream) {
+    if (!C) return fail(SHKZ_ERR_INVALID, "NULL context");
+    SHKZ_TRY(validate_vfield(f, width, constrain, true));
+    std::lock_guard<std::mutex> lock(C->mu);
+    SHKZ_HIP(hipSetDevice(C->device));
+    hipStream_t caller = (hipStream_t)stream, s = C->work;
+    if (caller != s) {
+        SHKZ_HIP(hipEventRecord(C->ev_in, caller));
+        SHKZ_HIP(hipStreamWaitEvent(s, C->ev_in, 0));
+    }
+    float *const vel[3] = {f->u, f->v, f->w_vel};
+    uint8_t *const act[3] = {f->active_u, f->active_v, f->active_w};
+    int rc = extrapolate_stages(C, f, width, constrain, s, vel, act, f->solid);
+    if (caller != s) {
+        if (hipEventRecord(C->ev_out, s) != hipSuccess || hipStreamWaitEvent(caller, C->ev_out, 0) != hipSuccess) {
+            (void)hipStreamSynchronize(s);
+            if (rc == SHKZ_OK) rc = fail(SHKZ_ERR_DEVICE, "ordering the caller's stream after the extrapolation failed");
+        }
+    }
+    if (rc != SHKZ_OK) (void)hipStreamSynchronize(s);
+    return rc;
+}
+
+int shkz_extrapolate_velocity(shkz_ctx *C, const shkz_velocity_field *f, int32_t width, int32_t constrain) {
+    if (!C) return fail(SHKZ_ERR_INVALID, "NULL context");
+    SHKZ_TRY(validate_vfield(f, width, constrain, false));
+    std::lock_guard<std::mutex> lock(C->mu);
+    SHKZ_HIP(hipSetDevice(C->device));
+    hipStream_t s = C->work;
+    Arena &AR = C->arena;
+    float *const hv[3] = {f->u, f->v, f->w_vel};
+    uint8_t *const ha[3] = {f->active_u, f->active_v, f->active_w};
+    float *vel[3];
+    uint8_t *act[3];
+    size_t n[3];
+    const char *vn[3] = {"hx_u", "hx_v", "hx_w"}, *an[3] = {"hx_au", "hx_av", "hx_aw"};
+    for (int a = 0; a < 3; ++a) {
+        n[a] = (size_t)(f->w + (a == 0)) * (f->h + (a == 1)) * (f->d + (a == 2));
+        SHKZ_TRY(arena_get(AR, vn[a], n[a], &vel[a]));
+        SHKZ_TRY(arena_get(AR, an[a], n[a], &act[a]));
+        SHKZ_HIP(hipMemcpyAsync(vel[a], hv[a], n[a] * sizeof(float), hipMemcpyHostToDevice, s));
+        SHKZ_HIP(hipMemcpyAsync(act[a], ha[a], n[a], hipMemcpyHostToDevice, s));
+    }
+    void *solid = nullptr;
+    if (f->solid && constrain) {
+        const size_t nodes = (size_t)(f->w + 1) * (f->h + 1) * (f->d + 1);
+        const size_t es = f->solid_dtype == SHKZ_F64 ? 8 : 4;
+        SHKZ_TRY(AR.get("hx_solid", nodes * es, &solid));
+        SHKZ_HIP(hipMemcpyAsync(solid, f->solid, nodes * es, hipMemcpyHostToDevice, s));
+    }
+    int rc = extrapolate_stages(C, f, width, constrain, s, vel, act, solid);
+    if (rc == SHKZ_OK)
+        for (int a = 0; a < 3; ++a) {
+            SHKZ_HIP(hipMemcpyAsync(hv[a], vel[a], n[a] * sizeof(float), hipMemcpyDeviceToHost, s));
+            SHKZ_HIP(hipMemcpyAsync(ha[a], act[a], n[a], hipMemcpyDeviceToHost, s));
+        }
+    SHKZ_HIP(hipStreamSynchronize(s));
+    return rc;
 }
 
 int shkz_project_device(shkz_ctx *C, const shkz_grid *grid, const shkz_params *params, shkz_result *result,

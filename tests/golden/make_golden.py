@@ -1,4 +1,5 @@
-"""Writes tests/golden/ref_vectors.npz from the reference's own header code.
+"""Writes tests/golden/ref_vectors.npz and tests/golden/ref_interp_vectors.npz from the
+reference's own header code.
 
 Run in a container that holds the reference tree: it builds oracle/_ref/libshkz_ref.so
 (oracle/Makefile `ref` target: the reference's include/shiokaze/utility/utility.h and
@@ -8,6 +9,8 @@ inputs and the reference's outputs for
   utility::get_area        (utility.h:179-214)
   utility::box             (utility.h:113-120)
   SparseMatrix -> FixedSparseMatrix -> multiply (local/include/pcgsolver/sparse_matrix.h)
+  array_interpolator3::interpolate_coef            (include/shiokaze/array/array_interpolator3.h:40-62)
+  array_derivative3::derivative_interpolate_coef   (include/shiokaze/array/array_derivative3.h:40-85)
 so that the tests can pin the oracle where the reference is absent.
 
     python tests/golden/make_golden.py
@@ -23,6 +26,7 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
 OUT = os.path.join(ROOT, "tests", "golden", "ref_vectors.npz")
+OUT_INTERP = os.path.join(ROOT, "tests", "golden", "ref_interp_vectors.npz")
 
 
 def special_values():
@@ -81,6 +85,32 @@ def main():
                         box=box, sp_n=np.int64(n), sp_ti=ti, sp_tj=tj, sp_tv=tv, sp_x=x, sp_y=y,
                         sp_rowstart=rowstart, sp_colindex=colindex, sp_value=value)
     print("wrote", OUT, os.path.getsize(OUT), "bytes")
+    interp_vectors(O)
+
+
+def interp_vectors(O):
+    """Sample positions inside, on and outside the array (the clamps), on integer and half-integer
+    coordinates, for random shapes of 2..40 per axis."""
+    rng = np.random.default_rng(20240602)
+    m = 3000
+    shapes = rng.integers(2, 41, size=(m, 3)).astype(np.int32)
+    pos = np.zeros((m, 3))
+    for a in range(3):
+        pos[:, a] = rng.uniform(-1.5, shapes[:, a] + 0.5)
+    snap = rng.integers(0, 4, size=(m, 3))  # 0 free, 1 integer, 2 half-integer, 3 the far edge
+    pos = np.where(snap == 1, np.round(pos), pos)
+    pos = np.where(snap == 2, np.floor(pos) + 0.5, pos)
+    pos = np.where(snap == 3, shapes - 1.0, pos)
+    ii = np.zeros((m, 8, 3), np.int32)
+    ci = np.zeros((m, 8), np.float64)
+    di = np.zeros((m, 8, 3), np.int32)
+    cd = np.zeros((m, 3, 8), np.float64)
+    for q in range(m):
+        ii[q], ci[q] = O.ref_interp_coef(shapes[q], pos[q])
+        di[q], cd[q] = O.ref_deriv_coef(shapes[q], pos[q])
+    np.savez_compressed(OUT_INTERP, shapes=shapes, pos=pos, interp_idx=ii, interp_coef=ci, deriv_idx=di,
+                        deriv_coef=cd)
+    print("wrote", OUT_INTERP, os.path.getsize(OUT_INTERP), "bytes")
 
 
 if __name__ == "__main__":
