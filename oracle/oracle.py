@@ -4,8 +4,9 @@ Only tests, the smoke check and the benchmark's CPU-baseline leg use this module
 library never imports it.  See shkz_oracle.h for what each function restates.
 
 Also binds oracle/_ref/libshkz_ref.so when it exists: a few header-only pieces of the reference
-compiled from the reference's own headers (utility::fraction/get_area/box and Bridson's
-SparseMatrix/FixedSparseMatrix multiply), used to pin the restatement.
+compiled from the reference's own headers (utility::fraction/get_area/box, Bridson's
+SparseMatrix/FixedSparseMatrix multiply, the trilinear interpolation and derivative weights),
+used to pin the restatement.
 """
 from __future__ import annotations
 
@@ -114,6 +115,10 @@ def lib():
         L.oracle_extrapolate.restype = ctypes.c_int64
         L.oracle_constrain_velocity.argtypes = [ctypes.POINTER(OVField)]
         L.oracle_constrain_velocity.restype = ctypes.c_int
+        L.oracle_advect_velocity.argtypes = [ctypes.POINTER(OVField), _vp, ctypes.c_int, ctypes.c_double, ctypes.c_int,
+                                             ctypes.c_int]
+        L.oracle_advect_scalar.argtypes = [ctypes.POINTER(OVField), _vp, _vp, _vp, ctypes.c_int, ctypes.c_double,
+                                           ctypes.c_int, ctypes.c_int, ctypes.c_float]
         _lib = L
     return _lib
 
@@ -400,3 +405,23 @@ def constrain_velocity(w, h, d, vel, active, solid, dx=None):
     """macutility3::constrain_velocity in place; returns whether the solid exists."""
     f = make_vfield(w, h, d, vel, active, solid, dx)
     return lib().oracle_constrain_velocity(ctypes.byref(f))
+
+
+# ------------------------------------------------------------------------------ advection
+def advect_velocity(w, h, d, vel, active, fluid, dt, dx=None, maccormack=True, trim=1):
+    """macadvection3::advect_vector(u, u, fluid, dt) in place on vel (float32 faces, uint8 activity)."""
+    f = make_vfield(w, h, d, vel, active, None, dx)
+    fl = np.ascontiguousarray(fluid)
+    lib().oracle_advect_velocity(ctypes.byref(f), fl.ctypes.data, int(fl.dtype == np.float64), float(dt),
+                                 int(bool(maccormack)), int(trim))
+
+
+def advect_scalar(w, h, d, vel, active, q, dt, active_q=None, fluid=None, dx=None, maccormack=True, trim=1,
+                  background=0.0):
+    """macadvection3::advect_scalar(q, velocity, fluid, dt) in place on q (float32 cells)."""
+    assert q.dtype == np.float32 and q.flags["C_CONTIGUOUS"]
+    f = make_vfield(w, h, d, vel, active, None, dx)
+    fl = None if fluid is None else np.ascontiguousarray(fluid)
+    lib().oracle_advect_scalar(ctypes.byref(f), q.ctypes.data, _p(active_q), None if fl is None else fl.ctypes.data,
+                               int(fl is not None and fl.dtype == np.float64), float(dt), int(bool(maccormack)),
+                               int(trim), float(background))

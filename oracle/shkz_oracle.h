@@ -167,6 +167,15 @@ int64_t oracle_extrapolate(const oracle_vfield *f, int width);
 /* macutility3::constrain_velocity; returns 1 when the solid level set exists */
 int oracle_constrain_velocity(const oracle_vfield *f);
 
+/* ---------------------------------------- advection (shkz_oracle_adv.c): macadvection3.cpp
+ * advect_vector(u, u, fluid, dt) in place on f->vel (f->active required; solid unused) */
+void oracle_advect_velocity(const oracle_vfield *f, const void *fluid, int fluid_f64, double dt, int maccormack,
+                            int trim);
+/* advect_scalar(q, velocity f, fluid, dt) in place on q (cells); active_q NULL = all active;
+ * fluid NULL = a copy of q; background = the value inactive cells of the backward pass read */
+void oracle_advect_scalar(const oracle_vfield *f, float *q, const uint8_t *active_q, const void *fluid, int fluid_f64,
+                          double dt, int maccormack, int trim, float background);
+
 #ifdef __cplusplus
 }
 #endif
