@@ -243,6 +243,41 @@ int shkz_extrapolate_velocity(shkz_ctx *ctx, const shkz_velocity_field *field, i
 int shkz_extrapolate_velocity_device(shkz_ctx *ctx, const shkz_velocity_field *field, int32_t width,
                                      int32_t constrain, void *stream);
 
+/* Advection between projections: macadvection3 (src/advection/macadvection3.cpp) with its
+ * defaults (MacCormack, trilinear interpolation, TrimNarrowBand 1; the WENO option is not
+ * provided).
+ *  - shkz_advect_velocity*: advect_vector(u, u, fluid, dt) (:47-53 -> advect_u, :140-177): each
+ *    active face moves back along its own full velocity (convert_to_full, macarray3.h:350-372);
+ *    MacCormack's correction is clamped to the forward stencil's min/max and replaced by the
+ *    semi-Lagrangian value where the liquid level set at the face exceeds
+ *    -dx * trim_narrowband.  Inactive faces read 0 and are not written.
+ *  - shkz_advect_scalar*: advect_scalar(q, velocity, fluid, dt) (:38-45 -> advect_cell,
+ *    :239-288) on a float cell array; the cell's full velocity is the mean of its two faces per
+ *    axis when all six are active, else zero (convert_to_full, macarray3.h:305-343).  With
+ *    fluid NULL the narrow band is measured on q itself, as the level-set tracker calls it
+ *    (maclevelsetsurfacetracker3.cpp:50-51).  active_q (optional) marks the cells to advect;
+ *    the others are not written, the forward pass reads their stored value and the backward
+ *    pass reads `background` (a fresh array's background value, as in the reference).
+ * Results are bit-identical to the reference's arithmetic.  Every axis needs at least 2 cells. */
+typedef struct shkz_advect_params {
+    double dt;
+    int32_t maccormack;           /* 1 (default) MacCormack, 0 semi-Lagrangian */
+    int32_t trim_narrowband;      /* TrimNarrowBand (default 1) */
+    const void *fluid;            /* liquid level set (cells) for the narrow-band test */
+    int32_t fluid_dtype;          /* SHKZ_F32 or SHKZ_F64 */
+    float background;             /* scalars: see above */
+    int32_t reserved[4];
+} shkz_advect_params;
+
+void shkz_advect_default_params(shkz_advect_params *params);
+int shkz_advect_velocity(shkz_ctx *ctx, const shkz_velocity_field *field, const shkz_advect_params *params);
+int shkz_advect_velocity_device(shkz_ctx *ctx, const shkz_velocity_field *field, const shkz_advect_params *params,
+                                void *stream);
+int shkz_advect_scalar(shkz_ctx *ctx, const shkz_velocity_field *field, float *q, const uint8_t *active_q,
+                       const shkz_advect_params *params);
+int shkz_advect_scalar_device(shkz_ctx *ctx, const shkz_velocity_field *field, float *q, const uint8_t *active_q,
+                              const shkz_advect_params *params, void *stream);
+
 /* ------------------------------------------------------- 2D (macpressuresolver2)
  * macproject2_interface::project (include/shiokaze/projection/macproject2_interface.h:67-72, implemented
  * by macpressuresolver2::project, src/projection/macpressuresolver2.cpp:48-232) on dense 2D arrays

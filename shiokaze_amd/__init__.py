@@ -92,6 +92,12 @@ class VelocityField(ctypes.Structure):
     ]
 
 
+class AdvectParams(ctypes.Structure):
+    _fields_ = [("dt", ctypes.c_double), ("maccormack", ctypes.c_int32), ("trim_narrowband", ctypes.c_int32),
+                ("fluid", _vp), ("fluid_dtype", ctypes.c_int32), ("background", ctypes.c_float),
+                ("reserved", ctypes.c_int32 * 4)]
+
+
 class Grid2D(ctypes.Structure):
     _fields_ = [
         ("w", ctypes.c_int32), ("h", ctypes.c_int32), ("dx", ctypes.c_double), ("dt", ctypes.c_double),
@@ -157,6 +163,8 @@ EXPORTS = [
     "shkz_project2d", "shkz_project2d_device", "shkz_project2d_batch_device", "shkz_export_system2d",
     "shkz_scene_fill2d", "shkz_linsolve_default_params", "shkz_linsolver_create", "shkz_linsolver_destroy",
     "shkz_linsolve_csr", "shkz_extrapolate_velocity", "shkz_extrapolate_velocity_device",
+    "shkz_advect_default_params", "shkz_advect_velocity", "shkz_advect_velocity_device", "shkz_advect_scalar",
+    "shkz_advect_scalar_device",
 ]
 
 _lib = None
@@ -217,6 +225,12 @@ def lib():
     L.shkz_extrapolate_velocity.argtypes = [_vp, ctypes.POINTER(VelocityField), ctypes.c_int32, ctypes.c_int32]
     L.shkz_extrapolate_velocity_device.argtypes = [_vp, ctypes.POINTER(VelocityField), ctypes.c_int32, ctypes.c_int32,
                                                    _vp]
+    L.shkz_advect_default_params.argtypes = [ctypes.POINTER(AdvectParams)]
+    L.shkz_advect_velocity.argtypes = [_vp, ctypes.POINTER(VelocityField), ctypes.POINTER(AdvectParams)]
+    L.shkz_advect_velocity_device.argtypes = [_vp, ctypes.POINTER(VelocityField), ctypes.POINTER(AdvectParams), _vp]
+    L.shkz_advect_scalar.argtypes = [_vp, ctypes.POINTER(VelocityField), _vp, _vp, ctypes.POINTER(AdvectParams)]
+    L.shkz_advect_scalar_device.argtypes = [_vp, ctypes.POINTER(VelocityField), _vp, _vp, ctypes.POINTER(AdvectParams),
+                                            _vp]
     _lib = L
     return L
 
@@ -623,6 +637,71 @@ class Context:
                           active_w=_dp(active[2]))
         rc = self._L.shkz_extrapolate_velocity_device(self.handle, ctypes.byref(f), int(width), int(bool(constrain)),
                                                       None if stream is None else int(stream))
+        if rc != 0 and check:
+            raise ShkzError(rc, last_error())
+        return rc
+
+    @staticmethod
+    def _advect_params(dt, fluid, fluid_dtype, maccormack, trim, background, device):
+        p = AdvectParams()
+        lib().shkz_advect_default_params(ctypes.byref(p))
+        p.dt = dt
+        p.maccormack = int(bool(maccormack))
+        p.trim_narrowband = int(trim)
+        p.background = background
+        if fluid is not None:
+            if device:
+                p.fluid, p.fluid_dtype = _dp(fluid), fluid_dtype
+            else:
+                assert fluid.flags["C_CONTIGUOUS"]
+                p.fluid, p.fluid_dtype = _ptr(fluid), F64 if fluid.dtype == np.float64 else F32
+        return p
+
+    @staticmethod
+    def _vfield(w, h, d, vel, active, dx, device):
+        conv = _dp if device else _ptr
+        if not device:
+            for a in list(vel) + list(active):
+                assert a.flags["C_CONTIGUOUS"]
+        return VelocityField(w=w, h=h, d=d, dx=dx, u=conv(vel[0]), v=conv(vel[1]), w_vel=conv(vel[2]),
+                             active_u=conv(active[0]), active_v=conv(active[1]), active_w=conv(active[2]))
+
+    def advect_velocity(self, w, h, d, vel, active, fluid, dt, maccormack=True, trim=1, dx=0.0, check=True) -> int:
+        """macadvection3::advect_vector(u, u, fluid, dt) on host arrays, in place on vel."""
+        f = self._vfield(w, h, d, vel, active, dx, False)
+        p = self._advect_params(dt, fluid, None, maccormack, trim, 0.0, False)
+        rc = self._L.shkz_advect_velocity(self.handle, ctypes.byref(f), ctypes.byref(p))
+        if rc != 0 and check:
+            raise ShkzError(rc, last_error())
+        return rc
+
+    def advect_velocity_device(self, w, h, d, vel, active, fluid, fluid_dtype, dt, maccormack=True, trim=1, dx=0.0,
+                               stream=None, check=True) -> int:
+        f = self._vfield(w, h, d, vel, active, dx, True)
+        p = self._advect_params(dt, fluid, fluid_dtype, maccormack, trim, 0.0, True)
+        rc = self._L.shkz_advect_velocity_device(self.handle, ctypes.byref(f), ctypes.byref(p),
+                                                 None if stream is None else int(stream))
+        if rc != 0 and check:
+            raise ShkzError(rc, last_error())
+        return rc
+
+    def advect_scalar(self, w, h, d, vel, active, q, dt, active_q=None, fluid=None, maccormack=True, trim=1,
+                      background=0.0, dx=0.0, check=True) -> int:
+        """macadvection3::advect_scalar(q, velocity, fluid, dt) on host arrays, in place on q (float32)."""
+        assert q.dtype == np.float32 and q.flags["C_CONTIGUOUS"]
+        f = self._vfield(w, h, d, vel, active, dx, False)
+        p = self._advect_params(dt, fluid, None, maccormack, trim, background, False)
+        rc = self._L.shkz_advect_scalar(self.handle, ctypes.byref(f), _ptr(q), _ptr(active_q), ctypes.byref(p))
+        if rc != 0 and check:
+            raise ShkzError(rc, last_error())
+        return rc
+
+    def advect_scalar_device(self, w, h, d, vel, active, q, dt, active_q=None, fluid=None, fluid_dtype=F32,
+                             maccormack=True, trim=1, background=0.0, dx=0.0, stream=None, check=True) -> int:
+        f = self._vfield(w, h, d, vel, active, dx, True)
+        p = self._advect_params(dt, fluid, fluid_dtype, maccormack, trim, background, True)
+        rc = self._L.shkz_advect_scalar_device(self.handle, ctypes.byref(f), _dp(q), _dp(active_q), ctypes.byref(p),
+                                               None if stream is None else int(stream))
         if rc != 0 and check:
             raise ShkzError(rc, last_error())
         return rc

@@ -1,0 +1,332 @@
+// shkz_advect.hpp -- advection on the device: macadvection3 (src/advection/macadvection3.cpp) with
+// its defaults (MacCormack, trilinear interpolation, TrimNarrowBand 1), for the face velocity
+// (advect_vector -> advect_u, :47-53,140-177) and for cell scalars such as the liquid level set
+// (advect_scalar -> advect_cell, :38-45,239-288).
+//
+//   k_adv_face_fwd   forward semi-Lagrangian pass of advect_semiLagrangian_u (:68-137) with the
+//                    MacCormack min/max and narrow-band record
+//   k_adv_face_bwd   backward pass from the forward result, fused with MacCormack's final
+//                    combination (:152-168), written in place over the input
+//   k_adv_cell_fwd,
+//   k_adv_cell_bwd   the same for cells (advect_semiLagrangian_cell, :179-237; :255-270)
+//   k_adv_*_copy     semi-Lagrangian only: the forward result copied over the active entries
+//
+// The full velocity at a face (macarray3::convert_to_full into a face array, macarray3.h:350-372)
+// and at a cell (:305-343) is computed where it is needed instead of being stored.  A face
+// velocity field samples inactive faces as 0, its background value.  Cell scalars sample their
+// stored value in the forward pass; in the backward pass inactive cells read `background`, the
+// value a fresh array of the scalar's type holds (:246).
+//
+// Arithmetic follows the reference's types operation by operation (no contraction): the value
+// pass of the face version computes the departure point from a float vector (vec3r), its min/max
+// pass from a double one (vec3d); interpolation accumulates in the array's type; MacCormack's
+// correction is 0.5 times a float difference.
+#pragma once
+#include <float.h>
+
+#include "shkz_extrap.hpp"
+
+namespace shkz {
+
+struct AdvScratch {
+    float *v0[3];   // forward result (faces) or v0[0] (cells)
+    float *lo[3];   // MacCormack bounds
+    float *hi[3];
+    uint8_t *nb[3]; // inside the narrow band: keep the semi-Lagrangian value
+};
+
+__device__ __forceinline__ float face_val(const float *U, const uint8_t *A, size_t q) { return A[q] ? U[q] : 0.0f; }
+
+__device__ __forceinline__ int clamp_i(int v, int n) { return v < 0 ? 0 : (v > n - 1 ? n - 1 : v); }
+
+// macarray3::convert_to_full at face (dim; i,j,k) of the fields U (activity F.act).
+__device__ __forceinline__ void full_face(const VelField &F, const float *const U[3], int dim, int i, int j, int k,
+                                          float out[3]) {
+#pragma clang fp contract(off)
+    const int pc[3] = {i, j, k};
+#pragma unroll
+    for (int ud = 0; ud < 3; ++ud) {
+        int s[3];
+        vf_shape(F, ud, s);
+        if (ud == dim) {
+            out[ud] = face_val(U[ud], F.act[ud], ((size_t)i * s[1] + j) * s[2] + k);
+            continue;
+        }
+        double u = 0.0;
+#pragma unroll
+        for (int ii = 0; ii < 2; ++ii)
+#pragma unroll
+            for (int jj = 0; jj < 2; ++jj) {
+                int q[3];
+#pragma unroll
+                for (int a = 0; a < 3; ++a) q[a] = clamp_i(pc[a] - (a == dim) + ii * (a == dim) + jj * (a == ud), s[a]);
+                u += (double)face_val(U[ud], F.act[ud], ((size_t)q[0] * s[1] + q[1]) * s[2] + q[2]);
+            }
+        u /= 4.0;
+        out[ud] = (float)u;
+    }
+}
+
+// macarray3::convert_to_full at cell (i,j,k): valid when both faces of every axis are active.
+__device__ __forceinline__ void full_cell(const VelField &F, int i, int j, int k, double out[3]) {
+#pragma clang fp contract(off)
+    int valid = 0;
+    float v[3] = {0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        int s[3];
+        vf_shape(F, a, s);
+        const size_t q0 = ((size_t)i * s[1] + j) * s[2] + k;
+        const size_t q1 = ((size_t)(i + (a == 0)) * s[1] + (j + (a == 1))) * s[2] + (k + (a == 2));
+        int wsum = 0;
+        double value = 0.0;
+        if (F.act[a][q0]) {
+            value += (double)F.vel[a][q0];
+            wsum++;
+        }
+        if (F.act[a][q1]) {
+            value += (double)F.vel[a][q1];
+            wsum++;
+        }
+        if (wsum == 2) {
+            v[a] = (float)(0.5 * value);
+            valid++;
+        }
+    }
+#pragma unroll
+    for (int a = 0; a < 3; ++a) out[a] = valid == 3 ? (double)v[a] : 0.0;
+}
+
+// interpolate<Real> over a float array of shape s; inactive entries read 0 when A is given
+__device__ __forceinline__ float interp_arr(const float *U, const uint8_t *A, const int s[3], const double p[3]) {
+#pragma clang fp contract(off)
+    const Corner c = corner_of(s, p);
+    double w[8];
+    interp_weights(c, w);
+    float val = 0.0f;
+#pragma unroll
+    for (int n = 0; n < 8; ++n)
+        if (w[n] != 0.0) {
+            const size_t q = corner_index(c, n, s);
+            const float u = A ? face_val(U, A, q) : U[q];
+            val = acc<float>(val, (double)u * w[n]);
+        }
+    return val;
+}
+
+// min and max of the 8 corner values at p (interpolate_coef's stencil), from the given initial
+// bounds (the reference's numeric_limits initialisers)
+__device__ __forceinline__ void corner_bounds(const float *U, const uint8_t *A, const int s[3], const double p[3],
+                                              double &lo, double &hi) {
+    const Corner c = corner_of(s, p);
+#pragma unroll
+    for (int n = 0; n < 8; ++n) {
+        const size_t q = corner_index(c, n, s);
+        const double v = A ? (double)face_val(U, A, q) : (double)U[q];
+        lo = v < lo ? v : lo;  // std::min
+        hi = hi < v ? v : hi;  // std::max
+    }
+}
+
+// the liquid level set at p (cell index space): float accumulation for a float level set
+__device__ __forceinline__ double interp_level(const void *phi, int f64, const int s[3], const double p[3]) {
+#pragma clang fp contract(off)
+    const Corner c = corner_of(s, p);
+    double w[8];
+    interp_weights(c, w);
+    if (f64) {
+        double val = 0.0;
+#pragma unroll
+        for (int n = 0; n < 8; ++n)
+            if (w[n] != 0.0) val += ((const double *)phi)[corner_index(c, n, s)] * w[n];
+        return val;
+    }
+    float val = 0.0f;
+#pragma unroll
+    for (int n = 0; n < 8; ++n)
+        if (w[n] != 0.0) val = acc<float>(val, (double)((const float *)phi)[corner_index(c, n, s)] * w[n]);
+    return (double)val;
+}
+
+// Forward pass over every active face: S.v0 = the semi-Lagrangian value; with `minmax`, the
+// MacCormack bounds and the narrow-band flag.  `in` is the field being advected (its own full
+// velocity moves it, :72-73).
+__global__ void k_adv_face_fwd(VelField F, const float *in0, const float *in1, const float *in2, AdvScratch S,
+                               const void *fluid, int fluid64, double dt, int trim, int minmax) {
+#pragma clang fp contract(off)
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int a, i, j, k;
+    size_t f;
+    if (!vf_decode(F, e, a, f, i, j, k)) return;
+    if (!F.act[a][f]) return;
+    const float *const in[3] = {in0, in1, in2};
+    float u[3];
+    full_face(F, in, a, i, j, k, u);
+    const bool empty = u[0] == 0.0f && u[1] == 0.0f && u[2] == 0.0f;
+    int s[3];
+    vf_shape(F, a, s);
+    const int pc[3] = {i, j, k};
+    float value;
+    if (!empty) {
+        double p[3];
+#pragma unroll
+        for (int b = 0; b < 3; ++b) {
+            const float t = (float)((double)u[b] * dt);   // dt*u on a float vector
+            const float sft = (float)((double)t / F.dx);  // /dx on a float vector
+            p[b] = (double)pc[b] - (double)sft;
+        }
+        value = interp_arr(in[a], F.act[a], s, p);
+    } else {
+        value = face_val(in[a], F.act[a], f);
+    }
+    S.v0[a][f] = value;
+    if (!minmax) return;
+    double fp[3];
+    float lo, hi;
+    if (!empty) {
+        double p[3];
+#pragma unroll
+        for (int b = 0; b < 3; ++b) {
+            const double sft = (dt * (double)u[b]) / F.dx;  // dt*u/dx on a double vector
+            p[b] = (double)pc[b] - sft;
+            fp[b] = (pc[b] + 0.5 * (a != b)) - sft;
+        }
+        double dlo = DBL_MAX, dhi = DBL_MIN;
+        corner_bounds(in[a], F.act[a], s, p, dlo, dhi);
+        lo = (float)dlo;
+        hi = (float)dhi;
+    } else {
+#pragma unroll
+        for (int b = 0; b < 3; ++b) fp[b] = pc[b] + 0.5 * (a != b);
+        lo = hi = face_val(in[a], F.act[a], f);
+    }
+    const int sc[3] = {F.w, F.h, F.d};
+    const double q[3] = {fp[0] - 0.5, fp[1] - 0.5, fp[2] - 0.5};
+    S.lo[a][f] = lo;
+    S.hi[a][f] = hi;
+    S.nb[a][f] = interp_level(fluid, fluid64, sc, q) > -F.dx * (double)trim ? 1 : 0;
+}
+
+// Backward pass from S.v0 (-dt) fused with MacCormack's final step, written over F.vel (the
+// input field, read only at the thread's own face).
+__global__ void k_adv_face_bwd(VelField F, AdvScratch S, double dt_back) {
+#pragma clang fp contract(off)
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int a, i, j, k;
+    size_t f;
+    if (!vf_decode(F, e, a, f, i, j, k)) return;
+    if (!F.act[a][f]) return;
+    const float v0 = S.v0[a][f];
+    if (S.nb[a][f]) {
+        F.vel[a][f] = v0;
+        return;
+    }
+    const float *const in[3] = {S.v0[0], S.v0[1], S.v0[2]};
+    float u[3];
+    full_face(F, in, a, i, j, k, u);
+    int s[3];
+    vf_shape(F, a, s);
+    const int pc[3] = {i, j, k};
+    float v1;
+    if (u[0] != 0.0f || u[1] != 0.0f || u[2] != 0.0f) {
+        double p[3];
+#pragma unroll
+        for (int b = 0; b < 3; ++b) {
+            const float t = (float)((double)u[b] * dt_back);
+            const float sft = (float)((double)t / F.dx);
+            p[b] = (double)pc[b] - (double)sft;
+        }
+        v1 = interp_arr(in[a], F.act[a], s, p);
+    } else {
+        v1 = v0;
+    }
+    const double lo = S.lo[a][f], hi = S.hi[a][f];
+    const float diff = F.vel[a][f] - v1;
+    const double r = (double)v0 + 0.5 * (double)diff;
+    F.vel[a][f] = r < lo ? (float)lo : (r > hi ? (float)hi : (float)r);
+}
+
+// semi-Lagrangian only: the forward result over the active faces
+__global__ void k_adv_face_copy(VelField F, AdvScratch S) {
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int a, i, j, k;
+    size_t f;
+    if (!vf_decode(F, e, a, f, i, j, k)) return;
+    if (F.act[a][f]) F.vel[a][f] = S.v0[a][f];
+}
+
+// Forward pass over the cells of q (advected by F): S.v0[0] = the semi-Lagrangian value, or
+// `background` on inactive cells (what the backward pass reads there); with `minmax`, bounds
+// and the narrow-band flag against `fluid` (cells).
+__global__ void k_adv_cell_fwd(VelField F, const float *q, const uint8_t *aq, AdvScratch S, const void *fluid,
+                               int fluid64, double dt, int trim, float background, int minmax) {
+#pragma clang fp contract(off)
+    const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t M = (size_t)F.w * F.h * F.d;
+    if (c >= M) return;
+    if (aq && !aq[c]) {
+        S.v0[0][c] = background;
+        return;
+    }
+    const int k = (int)(c % F.d), j = (int)((c / F.d) % F.h), i = (int)(c / ((size_t)F.d * F.h));
+    double u[3];
+    full_cell(F, i, j, k, u);
+    const bool empty = u[0] == 0.0 && u[1] == 0.0 && u[2] == 0.0;
+    const int s[3] = {F.w, F.h, F.d};
+    const int pc[3] = {i, j, k};
+    double p[3];
+#pragma unroll
+    for (int b = 0; b < 3; ++b) p[b] = (double)pc[b] - (dt * u[b]) / F.dx;
+    S.v0[0][c] = empty ? q[c] : interp_arr(q, nullptr, s, p);
+    if (!minmax) return;
+    if (!empty) {
+        double lo = (double)FLT_MAX, hi = (double)FLT_MIN;
+        corner_bounds(q, nullptr, s, p, lo, hi);
+        S.lo[0][c] = (float)lo;
+        S.hi[0][c] = (float)hi;
+        S.nb[0][c] = interp_level(fluid, fluid64, s, p) > -F.dx * (double)trim ? 1 : 0;
+    } else {
+        S.lo[0][c] = S.hi[0][c] = q[c];
+        const double fc = fluid64 ? ((const double *)fluid)[c] : (double)((const float *)fluid)[c];
+        S.nb[0][c] = fc > -F.dx * (double)trim ? 1 : 0;
+    }
+}
+
+// Backward pass from S.v0[0] fused with the final MacCormack step, written over q.
+__global__ void k_adv_cell_bwd(VelField F, float *q, const uint8_t *aq, AdvScratch S, double dt_back) {
+#pragma clang fp contract(off)
+    const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t M = (size_t)F.w * F.h * F.d;
+    if (c >= M) return;
+    if (aq && !aq[c]) return;
+    const float v0 = S.v0[0][c];
+    if (S.nb[0][c]) {
+        q[c] = v0;
+        return;
+    }
+    const int k = (int)(c % F.d), j = (int)((c / F.d) % F.h), i = (int)(c / ((size_t)F.d * F.h));
+    double u[3];
+    full_cell(F, i, j, k, u);
+    const int s[3] = {F.w, F.h, F.d};
+    const int pc[3] = {i, j, k};
+    float v1;
+    if (u[0] != 0.0 || u[1] != 0.0 || u[2] != 0.0) {
+        double p[3];
+#pragma unroll
+        for (int b = 0; b < 3; ++b) p[b] = (double)pc[b] - (dt_back * u[b]) / F.dx;
+        v1 = interp_arr(S.v0[0], nullptr, s, p);
+    } else {
+        v1 = v0;
+    }
+    const double lo = S.lo[0][c], hi = S.hi[0][c];
+    const float diff = q[c] - v1;
+    const double r = (double)v0 + 0.5 * (double)diff;
+    q[c] = r < lo ? (float)lo : (r > hi ? (float)hi : (float)r);
+}
+
+__global__ void k_adv_cell_copy(size_t M, float *q, const uint8_t *aq, AdvScratch S) {
+    const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < M && (!aq || aq[c])) q[c] = S.v0[0][c];
+}
+
+}  // namespace shkz

@@ -17,6 +17,7 @@
 #include "../../include/shkz_mi355x.h"
 #include "shkz_kernels.hpp"
 #include "shkz_extrap.hpp"
+#include "shkz_advect.hpp"
 #include "shkz_serve.hpp"
 #include "shkz_error.hpp"
 
@@ -1934,6 +1935,122 @@ static int validate_vfield(const shkz_velocity_field *f, int width, int constrai
     return SHKZ_OK;
 }
 
+// Runs fn(stream) on the context's work stream ordered after the caller's stream, and orders the
+// caller's stream after it (as project_impl does for the projection).
+template <class Fn>
+static int on_work_stream(shkz_ctx *C, hipStream_t caller, const char *what, Fn fn) {
+    hipStream_t s = C->work;
+    if (caller != s) {
+        SHKZ_HIP(hipEventRecord(C->ev_in, caller));
+        SHKZ_HIP(hipStreamWaitEvent(s, C->ev_in, 0));
+    }
+    int rc = fn(s);
+    if (caller != s) {
+        if (hipEventRecord(C->ev_out, s) != hipSuccess || hipStreamWaitEvent(caller, C->ev_out, 0) != hipSuccess) {
+            (void)hipStreamSynchronize(s);
+            if (rc == SHKZ_OK) rc = fail(SHKZ_ERR_DEVICE, "ordering the caller's stream after the %s failed", what);
+        }
+    }
+    if (rc != SHKZ_OK) (void)hipStreamSynchronize(s);
+    return rc;
+}
+
+static VelField vel_field_of(const shkz_velocity_field *f, float *const vel[3], uint8_t *const act[3]) {
+    VelField F;
+    F.w = f->w;
+    F.h = f->h;
+    F.d = f->d;
+    F.solid64 = f->solid_dtype == SHKZ_F64;
+    F.dx = f->dx > 0 ? f->dx : std::min(1.0 / f->w, std::min(1.0 / f->h, 1.0 / f->d));
+    F.solid = nullptr;
+    for (int a = 0; a < 3; ++a) {
+        F.vel[a] = vel[a];
+        F.act[a] = act[a];
+        F.n[a] = (size_t)(f->w + (a == 0)) * (f->h + (a == 1)) * (f->d + (a == 2));
+    }
+    return F;
+}
+
+static int validate_advect(const shkz_velocity_field *f, const shkz_advect_params *p, bool scalar) {
+    if (!f || !p) return fail(SHKZ_ERR_INVALID, "NULL field or parameters");
+    if (f->w < 2 || f->h < 2 || f->d < 2)
+        return fail(SHKZ_ERR_INVALID, "advection needs at least 2 cells per axis (got %dx%dx%d)", f->w, f->h, f->d);
+    if (!f->u || !f->v || !f->w_vel || !f->active_u || !f->active_v || !f->active_w)
+        return fail(SHKZ_ERR_INVALID, "velocity and activity arrays are required");
+    if (!std::isfinite(p->dt)) return fail(SHKZ_ERR_INVALID, "dt is not finite");
+    if (p->trim_narrowband < 0) return fail(SHKZ_ERR_INVALID, "negative TrimNarrowBand");
+    if (!scalar && !p->fluid) return fail(SHKZ_ERR_INVALID, "velocity advection needs the liquid level set");
+    if (p->fluid && p->fluid_dtype != SHKZ_F32 && p->fluid_dtype != SHKZ_F64)
+        return fail(SHKZ_ERR_INVALID, "unknown fluid dtype %d", p->fluid_dtype);
+    return SHKZ_OK;
+}
+
+// macadvection3::advect_vector(u, u, fluid, dt) on device arrays, on stream s.
+static int advect_velocity_stages(shkz_ctx *C, const shkz_velocity_field *f, const shkz_advect_params *p,
+                                  hipStream_t s, float *const vel[3], uint8_t *const act[3], const void *fluid) {
+    Arena &AR = C->arena;
+    Launcher K{C, s};
+    VelField F = vel_field_of(f, vel, act);
+    const size_t total = F.n[0] + F.n[1] + F.n[2];
+    const dim3 grid((unsigned)((total + 255) / 256)), block(256);
+    AdvScratch S;
+    const char *nv[3] = {"adv_v0", "adv_v1", "adv_v2"}, *nl[3] = {"adv_lo0", "adv_lo1", "adv_lo2"},
+               *nh[3] = {"adv_hi0", "adv_hi1", "adv_hi2"}, *nn[3] = {"adv_nb0", "adv_nb1", "adv_nb2"};
+    const bool mc = p->maccormack != 0;
+    for (int a = 0; a < 3; ++a) {
+        SHKZ_TRY(arena_get(AR, nv[a], F.n[a], &S.v0[a]));
+        S.lo[a] = S.hi[a] = nullptr;
+        S.nb[a] = nullptr;
+        if (mc) {
+            SHKZ_TRY(arena_get(AR, nl[a], F.n[a], &S.lo[a]));
+            SHKZ_TRY(arena_get(AR, nh[a], F.n[a], &S.hi[a]));
+            SHKZ_TRY(arena_get(AR, nn[a], F.n[a], &S.nb[a]));
+        }
+    }
+    const int f64 = p->fluid_dtype == SHKZ_F64;
+    // byte models per face: forward reads the face and its activity (neighbours hit the cache)
+    // and writes v0 (+ bounds and flag); backward reads v0, bounds, flag, the input and writes it
+    K.go("adv_face_fwd", -1, 0, (double)total * (mc ? 18 : 9), grid, block, k_adv_face_fwd, F, (const float *)vel[0],
+         (const float *)vel[1], (const float *)vel[2], S, fluid, f64, p->dt, (int)p->trim_narrowband, mc ? 1 : 0);
+    if (mc)
+        K.go("adv_face_bwd", -1, 0, (double)total * 22, grid, block, k_adv_face_bwd, F, S, -p->dt);
+    else
+        K.go("adv_face_copy", -1, 0, (double)total * 9, grid, block, k_adv_face_copy, F, S);
+    SHKZ_HIP(hipGetLastError());
+    return SHKZ_OK;
+}
+
+// macadvection3::advect_scalar(q, velocity, fluid, dt) on device arrays, on stream s.
+static int advect_scalar_stages(shkz_ctx *C, const shkz_velocity_field *f, float *q, const uint8_t *aq,
+                                const shkz_advect_params *p, hipStream_t s, float *const vel[3],
+                                uint8_t *const act[3], const void *fluid, int fluid64) {
+    Arena &AR = C->arena;
+    Launcher K{C, s};
+    VelField F = vel_field_of(f, vel, act);
+    const size_t M = (size_t)f->w * f->h * f->d;
+    const dim3 grid((unsigned)((M + 255) / 256)), block(256);
+    AdvScratch S = {};
+    const bool mc = p->maccormack != 0;
+    SHKZ_TRY(arena_get(AR, "adv_q0", M, &S.v0[0]));
+    if (mc) {
+        SHKZ_TRY(arena_get(AR, "adv_qlo", M, &S.lo[0]));
+        SHKZ_TRY(arena_get(AR, "adv_qhi", M, &S.hi[0]));
+        SHKZ_TRY(arena_get(AR, "adv_qnb", M, &S.nb[0]));
+    }
+    if (!fluid) {  // the narrow band of the scalar itself (the level-set tracker's call)
+        fluid = q;
+        fluid64 = 0;
+    }
+    K.go("adv_cell_fwd", -1, 0, (double)M * (mc ? 18 : 8), grid, block, k_adv_cell_fwd, F, (const float *)q, aq, S,
+         fluid, fluid64, p->dt, (int)p->trim_narrowband, p->background, mc ? 1 : 0);
+    if (mc)
+        K.go("adv_cell_bwd", -1, 0, (double)M * 21, grid, block, k_adv_cell_bwd, F, q, aq, S, -p->dt);
+    else
+        K.go("adv_cell_copy", -1, 0, (double)M * 8, grid, block, k_adv_cell_copy, M, q, aq, S);
+    SHKZ_HIP(hipGetLastError());
+    return SHKZ_OK;
+}
+
 extern "C" {
 
 int shkz_abi_version(void) { return SHKZ_ABI_VERSION; }
@@ -2052,21 +2169,113 @@ int shkz_extrapolate_velocity_device(shkz_ctx *C, const shkz_velocity_field *f, 
     SHKZ_TRY(validate_vfield(f, width, constrain, true));
     std::lock_guard<std::mutex> lock(C->mu);
     SHKZ_HIP(hipSetDevice(C->device));
-    hipStream_t caller = (hipStream_t)stream, s = C->work;
-    if (caller != s) {
-        SHKZ_HIP(hipEventRecord(C->ev_in, caller));
-        SHKZ_HIP(hipStreamWaitEvent(s, C->ev_in, 0));
-    }
     float *const vel[3] = {f->u, f->v, f->w_vel};
     uint8_t *const act[3] = {f->active_u, f->active_v, f->active_w};
-    int rc = extrapolate_stages(C, f, width, constrain, s, vel, act, f->solid);
-    if (caller != s) {
-        if (hipEventRecord(C->ev_out, s) != hipSuccess || hipStreamWaitEvent(caller, C->ev_out, 0) != hipSuccess) {
-            (void)hipStreamSynchronize(s);
-            if (rc == SHKZ_OK) rc = fail(SHKZ_ERR_DEVICE, "ordering the caller's stream after the extrapolation failed");
-        }
+    return on_work_stream(C, (hipStream_t)stream, "extrapolation",
+                          [&](hipStream_t s) { return extrapolate_stages(C, f, width, constrain, s, vel, act, f->solid); });
+}
+
+void shkz_advect_default_params(shkz_advect_params *p) {
+    if (!p) return;
+    std::memset(p, 0, sizeof(*p));
+    p->maccormack = 1;       // macadvection3.cpp:291
+    p->trim_narrowband = 1;  // :293
+    p->fluid_dtype = SHKZ_F32;
+}
+
+int shkz_advect_velocity_device(shkz_ctx *C, const shkz_velocity_field *f, const shkz_advect_params *p, void *stream) {
+    if (!C) return fail(SHKZ_ERR_INVALID, "NULL context");
+    SHKZ_TRY(validate_advect(f, p, false));
+    std::lock_guard<std::mutex> lock(C->mu);
+    SHKZ_HIP(hipSetDevice(C->device));
+    float *const vel[3] = {f->u, f->v, f->w_vel};
+    uint8_t *const act[3] = {f->active_u, f->active_v, f->active_w};
+    return on_work_stream(C, (hipStream_t)stream, "advection",
+                          [&](hipStream_t s) { return advect_velocity_stages(C, f, p, s, vel, act, p->fluid); });
+}
+
+int shkz_advect_scalar_device(shkz_ctx *C, const shkz_velocity_field *f, float *q, const uint8_t *active_q,
+                              const shkz_advect_params *p, void *stream) {
+    if (!C) return fail(SHKZ_ERR_INVALID, "NULL context");
+    SHKZ_TRY(validate_advect(f, p, true));
+    if (!q) return fail(SHKZ_ERR_INVALID, "NULL scalar");
+    std::lock_guard<std::mutex> lock(C->mu);
+    SHKZ_HIP(hipSetDevice(C->device));
+    float *const vel[3] = {f->u, f->v, f->w_vel};
+    uint8_t *const act[3] = {f->active_u, f->active_v, f->active_w};
+    return on_work_stream(C, (hipStream_t)stream, "advection", [&](hipStream_t s) {
+        return advect_scalar_stages(C, f, q, active_q, p, s, vel, act, p->fluid, p->fluid_dtype == SHKZ_F64);
+    });
+}
+
+// Host-array forms: stage the arrays in the arena, run, copy the results back.
+static int stage_field(shkz_ctx *C, const shkz_velocity_field *f, hipStream_t s, float *vel[3], uint8_t *act[3],
+                       size_t n[3]) {
+    Arena &AR = C->arena;
+    float *const hv[3] = {f->u, f->v, f->w_vel};
+    uint8_t *const ha[3] = {f->active_u, f->active_v, f->active_w};
+    const char *vn[3] = {"hx_u", "hx_v", "hx_w"}, *an[3] = {"hx_au", "hx_av", "hx_aw"};
+    for (int a = 0; a < 3; ++a) {
+        n[a] = (size_t)(f->w + (a == 0)) * (f->h + (a == 1)) * (f->d + (a == 2));
+        SHKZ_TRY(arena_get(AR, vn[a], n[a], &vel[a]));
+        SHKZ_TRY(arena_get(AR, an[a], n[a], &act[a]));
+        SHKZ_HIP(hipMemcpyAsync(vel[a], hv[a], n[a] * sizeof(float), hipMemcpyHostToDevice, s));
+        SHKZ_HIP(hipMemcpyAsync(act[a], ha[a], n[a], hipMemcpyHostToDevice, s));
     }
-    if (rc != SHKZ_OK) (void)hipStreamSynchronize(s);
+    return SHKZ_OK;
+}
+
+static int stage_cells(shkz_ctx *C, const char *name, const void *host, size_t bytes, hipStream_t s, void **dev) {
+    *dev = nullptr;
+    if (!host) return SHKZ_OK;
+    SHKZ_TRY(C->arena.get(name, bytes, dev));
+    SHKZ_HIP(hipMemcpyAsync(*dev, host, bytes, hipMemcpyHostToDevice, s));
+    return SHKZ_OK;
+}
+
+int shkz_advect_velocity(shkz_ctx *C, const shkz_velocity_field *f, const shkz_advect_params *p) {
+    if (!C) return fail(SHKZ_ERR_INVALID, "NULL context");
+    SHKZ_TRY(validate_advect(f, p, false));
+    std::lock_guard<std::mutex> lock(C->mu);
+    SHKZ_HIP(hipSetDevice(C->device));
+    hipStream_t s = C->work;
+    float *vel[3];
+    uint8_t *act[3];
+    size_t n[3];
+    SHKZ_TRY(stage_field(C, f, s, vel, act, n));
+    const size_t M = (size_t)f->w * f->h * f->d;
+    void *fluid = nullptr;
+    SHKZ_TRY(stage_cells(C, "hx_fluid", p->fluid, M * (p->fluid_dtype == SHKZ_F64 ? 8 : 4), s, &fluid));
+    const int rc = advect_velocity_stages(C, f, p, s, vel, act, fluid);
+    float *const hv[3] = {f->u, f->v, f->w_vel};
+    if (rc == SHKZ_OK)
+        for (int a = 0; a < 3; ++a)
+            SHKZ_HIP(hipMemcpyAsync(hv[a], vel[a], n[a] * sizeof(float), hipMemcpyDeviceToHost, s));
+    SHKZ_HIP(hipStreamSynchronize(s));
+    return rc;
+}
+
+int shkz_advect_scalar(shkz_ctx *C, const shkz_velocity_field *f, float *q, const uint8_t *active_q,
+                       const shkz_advect_params *p) {
+    if (!C) return fail(SHKZ_ERR_INVALID, "NULL context");
+    SHKZ_TRY(validate_advect(f, p, true));
+    if (!q) return fail(SHKZ_ERR_INVALID, "NULL scalar");
+    std::lock_guard<std::mutex> lock(C->mu);
+    SHKZ_HIP(hipSetDevice(C->device));
+    hipStream_t s = C->work;
+    float *vel[3];
+    uint8_t *act[3];
+    size_t n[3];
+    SHKZ_TRY(stage_field(C, f, s, vel, act, n));
+    const size_t M = (size_t)f->w * f->h * f->d;
+    void *dq = nullptr, *daq = nullptr, *fluid = nullptr;
+    SHKZ_TRY(stage_cells(C, "hx_q", q, M * 4, s, &dq));
+    SHKZ_TRY(stage_cells(C, "hx_aq", active_q, M, s, &daq));
+    SHKZ_TRY(stage_cells(C, "hx_fluid", p->fluid, M * (p->fluid_dtype == SHKZ_F64 ? 8 : 4), s, &fluid));
+    const int rc = advect_scalar_stages(C, f, (float *)dq, (const uint8_t *)daq, p, s, vel, act, fluid,
+                                        p->fluid_dtype == SHKZ_F64);
+    if (rc == SHKZ_OK) SHKZ_HIP(hipMemcpyAsync(q, dq, M * 4, hipMemcpyDeviceToHost, s));
+    SHKZ_HIP(hipStreamSynchronize(s));
     return rc;
 }
 

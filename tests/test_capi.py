@@ -36,13 +36,14 @@ def test_every_declared_symbol_is_exported():
 def test_struct_layouts_match_c(tmp_path):
     src = tmp_path / "sz.c"
     src.write_text('#include "shkz_mi355x.h"\n#include <stdio.h>\n#include <stddef.h>\n'
-                   'int main(){printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu\\n",sizeof(shkz_grid),'
+                   'int main(){printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu\\n",sizeof(shkz_grid),'
                    'sizeof(shkz_params),sizeof(shkz_result),sizeof(shkz_scene_desc),sizeof(shkz_kernel_stat),'
                    'offsetof(shkz_grid,scene_id),offsetof(shkz_result,t_projection),sizeof(shkz_server_config),'
                    'sizeof(shkz_server_stats),offsetof(shkz_result,t_latency),'
                    'offsetof(shkz_server_config,hbm_budget_bytes),sizeof(shkz_grid2d),'
                    'offsetof(shkz_grid2d,scene_id),offsetof(shkz_grid,active_out_w),sizeof(shkz_velocity_field),'
-                   'offsetof(shkz_velocity_field,active_w));return 0;}\n')
+                   'offsetof(shkz_velocity_field,active_w),sizeof(shkz_advect_params),'
+                   'offsetof(shkz_advect_params,background));return 0;}\n')
     exe = tmp_path / "sz"
     subprocess.check_call(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)])
     got = [int(x) for x in subprocess.check_output([str(exe)]).split()]
@@ -50,7 +51,8 @@ def test_struct_layouts_match_c(tmp_path):
             ctypes.sizeof(S.KernelStat), S.Grid.scene_id.offset, S.Result.t_projection.offset,
             ctypes.sizeof(S.ServerConfig), ctypes.sizeof(S.ServerStats), S.Result.t_latency.offset,
             S.ServerConfig.hbm_budget_bytes.offset, ctypes.sizeof(S.Grid2D), S.Grid2D.scene_id.offset,
-            S.Grid.active_out_w.offset, ctypes.sizeof(S.VelocityField), S.VelocityField.active_w.offset]
+            S.Grid.active_out_w.offset, ctypes.sizeof(S.VelocityField), S.VelocityField.active_w.offset,
+            ctypes.sizeof(S.AdvectParams), S.AdvectParams.background.offset]
     assert got == want
 
 
