@@ -302,10 +302,12 @@ def bench_serve(args):
 
 
 def bench_post(args):
-    """The step after the projection (velocity extrapolation over `--post-width` passes and the
-    solid constraint, shkz_extrapolate_velocity_device) on the c3 scene's projected velocity and
-    post-projection activity, both resident in HBM; each step restores them from device copies
-    (inside the timed region) and runs the extrapolation and constraint."""
+    """The rest of a time step around the projection, on the c3 scene's projected velocity and
+    post-projection activity, all resident in HBM: velocity extrapolation over `--post-width`
+    passes and the solid constraint (shkz_extrapolate_velocity_device), then MacCormack advection
+    of the liquid level set and of the velocity (shkz_advect_scalar_device,
+    shkz_advect_velocity_device) with dt at CFL 2.  Each step restores the inputs from device
+    copies (inside the timed region)."""
     import shiokaze_amd as S
     from shiokaze_amd.devmem import DeviceArray, Stream, device_synchronize
     kind_name, n, seed, residual = CONFIGS["c3"]
@@ -321,16 +323,23 @@ def bench_post(args):
     r = ctx.project_device(n, n, n, sc.dt, d_fluid, S.F64, d_solid, S.F32, *vel,
                            params=S.default_params(residual=residual), stream=st.handle, active_out=act)
     st.synchronize()
-    vel0 = [DeviceArray(v.download()) for v in vel]
+    vel0_host = [v.download() for v in vel]
+    vel0 = [DeviceArray(v) for v in vel0_host]
     act0_host = [a.download() for a in act]
     act0 = [DeviceArray(a) for a in act0_host]
+    phi_host = sc.fluid.astype(np.float32)
+    phi0 = DeviceArray(phi_host)
+    phi = DeviceArray(phi_host)
+    dt = 2.0 * sc.dx / max(float(np.abs(v).max()) for v in vel0_host)
     device_synchronize()
 
     def step():
-        for a, b in zip(vel + act, vel0 + act0):
+        for a, b in zip(vel + act + [phi], vel0 + act0 + [phi0]):
             a.copy_from(b, st.handle)
         ctx.extrapolate_velocity_device(n, n, n, vel, act, solid=d_solid, solid_dtype=S.F32, width=width,
                                         stream=st.handle)
+        ctx.advect_scalar_device(n, n, n, vel, act, phi, dt, stream=st.handle)
+        ctx.advect_velocity_device(n, n, n, vel, act, phi, S.F32, dt, stream=st.handle)
 
     for _ in range(args.warmup):
         step()
@@ -354,26 +363,36 @@ def bench_post(args):
         ctx.set_kernel_timing(False)
     roofline = kernel_roofline(stats, inst, "post")
     faces = sum(a * b * c for a, b, c in shapes)
-    out = {"metric": f"post-projection steps/sec @{n}^3 (extrapolation width {width} + solid constraint)",
+    stage_ms = {}
+    for k, v in stats.items():
+        stage_ms[k] = round(v["total_ms"] / max(inst, 1), 4)
+    out = {"metric": f"post-projection steps/sec @{n}^3 (extrapolation width {width}, solid constraint, "
+                     f"level-set and velocity advection)",
            "value": round(args.steps / elapsed, 3), "unit": "steps/s", "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "ms_per_step": round(elapsed / args.steps * 1e3, 3), "higher_is_better": True,
            "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
            "config": {"workload": f"{n}^3 ({kind_name}, seed {seed}) projected velocity: {width} extrapolation "
-                                  f"passes and the solid constraint, velocity and activity restored on the device "
-                                  f"each step", "faces": int(faces), "faces_activated": activated,
+                                  f"passes, the solid constraint, MacCormack advection of the level set and the "
+                                  f"velocity at CFL 2; inputs restored on the device each step",
+                      "faces": int(faces), "faces_activated": activated, "kernel_ms_per_step": stage_ms,
                       "active_faces_before": int(sum(int(a.sum()) for a in act0_host)),
                       "projection_iterations": int(r.iterations)},
            "roofline": roofline, "cpu_baseline": None}
     if not args.no_cpu_baseline:
         from oracle import oracle as O
-        vh = [v.download() for v in vel0]
+        vh = [v.copy() for v in vel0_host]
         ah = [a.copy() for a in act0_host]
+        ph = phi_host.copy()
         t = time.perf_counter()
         O.extrapolate(n, n, n, vh, ah, width)
         O.constrain_velocity(n, n, n, vh, ah, sc.solid)
-        dt = time.perf_counter() - t
-        out["cpu_baseline"] = {"value": round(1.0 / dt, 4), "unit": "steps/s", "cores": 1, "kind": "port",
-                               "sample": f"one whole step at {n}^3 by the oracle (serial C), {dt:.2f} s"}
+        O.advect_scalar(n, n, n, vh, ah, ph, dt)
+        O.advect_velocity(n, n, n, vh, ah, ph, dt)
+        el = time.perf_counter() - t
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count()
+        out["cpu_baseline"] = {"value": round(1.0 / el, 4), "unit": "steps/s", "cores": threads, "kind": "port",
+                               "sample": f"one whole step at {n}^3 by the oracle (C; the advection loops use "
+                                         f"OpenMP), {el:.2f} s"}
     ctx.close()
     print(json.dumps(out), flush=True)
 

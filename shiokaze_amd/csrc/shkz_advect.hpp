@@ -25,6 +25,7 @@
 #include <float.h>
 
 #include "shkz_extrap.hpp"
+#include "shkz_geometry.hpp"
 
 namespace shkz {
 
@@ -268,7 +269,8 @@ __global__ void k_adv_cell_fwd(VelField F, const float *q, const uint8_t *aq, Ad
         S.v0[0][c] = background;
         return;
     }
-    const int k = (int)(c % F.d), j = (int)((c / F.d) % F.h), i = (int)(c / ((size_t)F.d * F.h));
+    int i, j, k;
+    unflatten((unsigned)c, (unsigned)F.h, (unsigned)F.d, i, j, k);
     double u[3];
     full_cell(F, i, j, k, u);
     const bool empty = u[0] == 0.0 && u[1] == 0.0 && u[2] == 0.0;
@@ -304,7 +306,8 @@ __global__ void k_adv_cell_bwd(VelField F, float *q, const uint8_t *aq, AdvScrat
         q[c] = v0;
         return;
     }
-    const int k = (int)(c % F.d), j = (int)((c / F.d) % F.h), i = (int)(c / ((size_t)F.d * F.h));
+    int i, j, k;
+    unflatten((unsigned)c, (unsigned)F.h, (unsigned)F.d, i, j, k);
     double u[3];
     full_cell(F, i, j, k, u);
     const int s[3] = {F.w, F.h, F.d};

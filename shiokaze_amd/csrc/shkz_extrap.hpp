@@ -5,9 +5,12 @@
 //
 //   k_extrap_pass   one pass of array_extrapolator3::extrapolate (array_extrapolator3.h:47-80,
 //                   over dilate3.h:43-101) on all three face arrays
-//   k_constrain     macutility3::constrain_velocity (macutility3.cpp:61-88) with the
+//   k_constrain,
+//   k_constrain_apply  macutility3::constrain_velocity (macutility3.cpp:61-88) with the
 //                   interpolation of array_interpolator3.h:40-97, array_derivative3.h:40-103 and
-//                   macarray_interpolator3.h:43-50
+//                   macarray_interpolator3.h:43-50; the first kernel writes constrained values to
+//                   a side buffer, the second applies them and the boundary rule, so the velocity
+//                   needs no full copy (the reference samples a copy, velocity_save)
 //
 // Layout: the dense face arrays of the projection (u (w+1,h,d), v (w,h+1,d), w (w,h,d+1), z
 // fastest) with one activity byte per face.  One launch covers the three arrays (a flat index
@@ -57,9 +60,11 @@ __device__ __forceinline__ bool vf_decode(const VelField &F, size_t e, int &a, s
     }
     int s[3];
     vf_shape(F, a, s);
-    k = (int)(f % (size_t)s[2]);
-    j = (int)((f / (size_t)s[2]) % (size_t)s[1]);
-    i = (int)(f / ((size_t)s[2] * s[1]));
+    // 32-bit decode (each face array holds fewer than 2^32 faces, checked at entry)
+    const unsigned u = (unsigned)f, t = u / (unsigned)s[2], ii = t / (unsigned)s[1];
+    k = (int)(u - t * (unsigned)s[2]);
+    j = (int)(t - ii * (unsigned)s[1]);
+    i = (int)ii;
     return true;
 }
 
@@ -229,14 +234,16 @@ __device__ __forceinline__ double vel_sample(const VelField &F, const float *con
     return (double)val;
 }
 
-// Normalises the activity codes of the passes to 0/1 and, when `constrain` is set and the solid
-// level set has a negative value (*exist), applies macutility3::constrain_velocity to every active
-// face: a face whose centre samples the solid below 0 loses the component of the (saved, pre-
-// constraint) velocity along the normal when that velocity points into the solid, then an active
-// face on the domain boundary pointing out of the domain becomes 0.  Sv holds a copy of the
-// velocities taken before this kernel (the reference's velocity_save).
-__global__ void k_constrain(VelField F, const float *S0, const float *S1, const float *S2, const int *exist,
-                            int constrain) {
+// Activity code of a face whose constrained value waits in the side buffer (k_constrain_apply).
+constexpr uint8_t ACT_CONSTRAINED = 255;
+
+// macutility3::constrain_velocity, first half.  For every active face: the activity codes of the
+// passes become 1; when `constrain` is set and the solid level set has a negative value (*exist),
+// a face whose centre samples the solid below 0 and whose velocity points into the solid gets
+// the velocity minus its normal component, written to X (not to the velocity, which neighbouring
+// threads still sample, as the reference samples its velocity_save copy) and marked
+// ACT_CONSTRAINED.  The velocity arrays are only read here.
+__global__ void k_constrain(VelField F, float *X0, float *X1, float *X2, const int *exist, int constrain) {
 #pragma clang fp contract(off)
     const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     int a, i, j, k;
@@ -244,39 +251,58 @@ __global__ void k_constrain(VelField F, const float *S0, const float *S1, const 
     if (!vf_decode(F, e, a, f, i, j, k)) return;
     const int t = F.act[a][f];
     if (!t) return;
-    if (!constrain || !F.solid || !*exist) {
-        if (t > 1) F.act[a][f] = 1;
-        return;
-    }
-    const float *const Sv[3] = {S0, S1, S2};
-    const double p[3] = {i + 0.5 * (a != 0), j + 0.5 * (a != 1), k + 0.5 * (a != 2)};
-    double der[3] = {0.0, 0.0, 0.0};
-    bool inside;
-    if (F.solid64) solid_sample<double>(F, p, der, inside);
-    else solid_sample<float>(F, p, der, inside);
-    float out = Sv[a][f];
-    if (inside) {
-        double nrm[3];
+    if (constrain && F.solid && *exist) {
+        const float *const Sv[3] = {F.vel[0], F.vel[1], F.vel[2]};
+        const double p[3] = {i + 0.5 * (a != 0), j + 0.5 * (a != 1), k + 0.5 * (a != 2)};
+        double der[3] = {0.0, 0.0, 0.0};
+        bool inside;
+        if (F.solid64) solid_sample<double>(F, p, der, inside);
+        else solid_sample<float>(F, p, der, inside);
+        if (inside) {
+            double nrm[3];
 #pragma unroll
-        for (int b = 0; b < 3; ++b) nrm[b] = der[b] / F.dx;
-        double n2 = 0.0;
+            for (int b = 0; b < 3; ++b) nrm[b] = der[b] / F.dx;
+            double n2 = 0.0;
 #pragma unroll
-        for (int b = 0; b < 3; ++b) n2 += nrm[b] * nrm[b];
-        if (n2 != 0.0) {
-            double u[3];
+            for (int b = 0; b < 3; ++b) n2 += nrm[b] * nrm[b];
+            if (n2 != 0.0) {
+                double u[3];
 #pragma unroll
-            for (int b = 0; b < 3; ++b) u[b] = vel_sample(F, Sv, b, p);
-            double un = 0.0;
+                for (int b = 0; b < 3; ++b) u[b] = vel_sample(F, Sv, b, p);
+                double un = 0.0;
 #pragma unroll
-            for (int b = 0; b < 3; ++b) un += u[b] * nrm[b];
-            if (un < 0.0) out = (float)(u[a] - nrm[a] * un);
+                for (int b = 0; b < 3; ++b) un += u[b] * nrm[b];
+                if (un < 0.0) {
+                    float *const X[3] = {X0, X1, X2};
+                    X[a][f] = (float)(u[a] - nrm[a] * un);
+                    F.act[a][f] = ACT_CONSTRAINED;  // nonzero: neighbours still read the face as active
+                    return;
+                }
+            }
         }
     }
+    if (t > 1) F.act[a][f] = 1;
+}
+
+// Second half: constrained faces take their value from X, then (with the constraint on and a
+// solid present) active faces on the domain boundary pointing out of the domain become 0.
+__global__ void k_constrain_apply(VelField F, const float *X0, const float *X1, const float *X2, const int *exist,
+                                  int constrain) {
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int a, i, j, k;
+    size_t f;
+    if (!vf_decode(F, e, a, f, i, j, k)) return;
+    const int t = F.act[a][f];
     const int pi[3] = {i, j, k}, N[3] = {F.w, F.h, F.d};
+    const bool boundary = pi[a] == 0 || pi[a] == N[a];
+    if (t != ACT_CONSTRAINED && !(t && boundary)) return;
+    if (!(constrain && F.solid && *exist)) return;
+    const float *const X[3] = {X0, X1, X2};
+    float out = t == ACT_CONSTRAINED ? X[a][f] : F.vel[a][f];
     if (pi[a] == 0 && out < 0.0f) out = 0.0f;
     if (pi[a] == N[a] && out > 0.0f) out = 0.0f;
     F.vel[a][f] = out;
-    if (t > 1) F.act[a][f] = 1;
+    if (t == ACT_CONSTRAINED) F.act[a][f] = 1;
 }
 
 }  // namespace shkz

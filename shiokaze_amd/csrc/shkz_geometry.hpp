@@ -50,6 +50,17 @@ __device__ __forceinline__ double solid_at(const GridDesc &G, int i, int j, int 
 }
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
+// (i, j, k) of flat index e of a z-fastest (ni, nj, nk) box, in 32-bit arithmetic: every dense box
+// here holds fewer than 2^32 entries (checked at entry), and 64-bit division has no hardware
+// support (a long software sequence per thread).
+__device__ __forceinline__ void unflatten(unsigned e, unsigned nj, unsigned nk, int &i, int &j, int &k) {
+    const unsigned t = e / nk;
+    k = (int)(e - t * nk);
+    const unsigned a = t / nj;
+    i = (int)a;
+    j = (int)(t - a * nj);
+}
+
 // Liquid fraction between two level-set samples (utility.h:162-170).
 __device__ __forceinline__ double liquid_fraction(double phi0, double phi1) {
 #pragma clang fp contract(off)

@@ -126,7 +126,8 @@ __global__ void k_curvature(GridDesc G, float *curv) {
     const size_t M = (size_t)G.w * G.h * G.d;
     const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= M) return;
-    const int k = (int)(c % G.d), j = (int)((c / G.d) % G.h), i = (int)(c / ((size_t)G.d * G.h));
+    int i, j, k;
+    unflatten((unsigned)c, (unsigned)G.h, (unsigned)G.d, i, j, k);
     const int im = clampi(i - 1, 0, G.w - 1), ip = clampi(i + 1, 0, G.w - 1);
     const int jm = clampi(j - 1, 0, G.h - 1), jp = clampi(j + 1, 0, G.h - 1);
     const int km = clampi(k - 1, 0, G.d - 1), kp = clampi(k + 1, 0, G.d - 1);
@@ -163,7 +164,8 @@ __global__ void k_surface_tension(GridDesc G, int dim, const float *curv, double
     const size_t f = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (f >= n) return;
     if (G.active[dim] && !G.active[dim][f]) return;
-    const int k = (int)(f % D), j = (int)((f / D) % H), i = (int)(f / ((size_t)D * H));
+    int i, j, k;
+    unflatten((unsigned)f, (unsigned)H, (unsigned)D, i, j, k);
     const double rho = face_rho(G, dim, i, j, k);
     if (rho && rho < 1.0) {
         const int i0 = clampi(i, 0, G.w - 1), j0 = clampi(j, 0, G.h - 1), k0 = clampi(k, 0, G.d - 1);
@@ -995,7 +997,8 @@ __global__ void k_scatter_pressure(GridDesc G, LevelDesc L, GridLevel GL, const 
     const size_t M = (size_t)G.w * G.h * G.d;
     const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= M) return;
-    const int k = (int)(c % G.d), j = (int)((c / G.d) % G.h), i = (int)(c / ((size_t)G.d * G.h));
+    int i, j, k;
+    unflatten((unsigned)c, (unsigned)G.h, (unsigned)G.d, i, j, k);
     const double v = level_value(L, GL, x, i, j, k);
     if (P) P[c] = (float)v;
     if (P64) P64[c] = v;
@@ -1008,7 +1011,8 @@ __global__ void k_export_system(GridDesc G, LevelDesc L, GridLevel GL, OpDesc<do
     const size_t M = (size_t)G.w * G.h * G.d;
     const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= M) return;
-    const int k = (int)(c % G.d), j = (int)((c / G.d) % G.h), i = (int)(c / ((size_t)G.d * G.h));
+    int i, j, k;
+    unflatten((unsigned)c, (unsigned)G.h, (unsigned)G.d, i, j, k);
     const bool stored = L.bmap[GL.bmap_off + ((i >> 3) * GL.nby + (j >> 3)) * GL.nbz + (k >> 3)] >= 0;
     const double dirc = stored ? level_value(L, GL, A.dir, i, j, k) : -1.0;
     const bool unk = dirc >= 0.0;
@@ -1065,7 +1069,8 @@ __global__ void k_velocity_update(GridDesc G, const float *__restrict__ P) {
     const size_t n = (size_t)W * H * D;
     const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= n) return;
-    const int k = (int)(e % D), j = (int)((e / D) % H), i = (int)(e / ((size_t)D * H));
+    int i, j, k;
+    unflatten((unsigned)e, (unsigned)H, (unsigned)D, i, j, k);
     const bool in_i = i < G.w, in_j = j < G.h, in_k = k < G.d;
     auto p_at = [&](int a, int b, int c) { return (a >= 0 && b >= 0 && c >= 0) ? P[cell_index(G, a, b, c)] : 0.0f; };
     const float pc = (in_i && in_j && in_k) ? P[cell_index(G, i, j, k)] : 0.0f;

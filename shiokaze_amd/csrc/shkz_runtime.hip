@@ -861,6 +861,8 @@ static int validate_grid(const shkz_grid *grid) {
         return fail(SHKZ_ERR_INVALID, "fluid and velocities are required");
     if (!(grid->dt > 0)) return fail(SHKZ_ERR_INVALID, "dt must be positive");
     if ((int64_t)w * h * d > (int64_t)1 << 31) return fail(SHKZ_ERR_INVALID, "grid too large for 32-bit brick ids");
+    if ((int64_t)(w + 1) * (h + 1) * (d + 1) >= (int64_t)1 << 32)
+        return fail(SHKZ_ERR_INVALID, "grid too large for 32-bit node indices");
     return SHKZ_OK;
 }
 
@@ -1903,20 +1905,19 @@ static int extrapolate_stages(shkz_ctx *C, const shkz_velocity_field *f, int wid
              (int)F.solid64, nodes, exist);
     }
     // byte models: a pass scans the activity bytes (neighbour reads hit the cache); the
-    // constraint reads activity, the saved and the live velocity and writes the velocity, plus
-    // the nodal solid once
+    // constraint scans the activity and reads the nodal solid once
     for (int p = 1; p <= width; ++p) K.go("extrap_pass", -1, 0, (double)total, grid, block, k_extrap_pass, F, p);
-    float *sv[3] = {vel[0], vel[1], vel[2]};
+    float *xs[3] = {nullptr, nullptr, nullptr};
     if (with_solid) {
-        const char *names[3] = {"vx_save0", "vx_save1", "vx_save2"};
-        for (int a = 0; a < 3; ++a) {
-            SHKZ_TRY(arena_get(AR, names[a], F.n[a], &sv[a]));
-            SHKZ_HIP(hipMemcpyAsync(sv[a], vel[a], F.n[a] * sizeof(float), hipMemcpyDeviceToDevice, s));
-        }
+        const char *names[3] = {"vx_side0", "vx_side1", "vx_side2"};
+        for (int a = 0; a < 3; ++a) SHKZ_TRY(arena_get(AR, names[a], F.n[a], &xs[a]));
     }
-    K.go("constrain", -1, 0, with_solid ? (double)total * 9 + (double)nodes * (F.solid64 ? 8 : 4) : (double)total,
-         grid, block, k_constrain, F, (const float *)sv[0], (const float *)sv[1], (const float *)sv[2],
-         (const int *)exist, with_solid ? 1 : 0);
+    // the side buffers are written and read only at constrained faces
+    K.go("constrain", -1, 0, with_solid ? (double)total + (double)nodes * (F.solid64 ? 8 : 4) : (double)total, grid,
+         block, k_constrain, F, xs[0], xs[1], xs[2], (const int *)exist, with_solid ? 1 : 0);
+    if (with_solid)
+        K.go("constrain_apply", -1, 0, (double)total, grid, block, k_constrain_apply, F, (const float *)xs[0],
+             (const float *)xs[1], (const float *)xs[2], (const int *)exist, 1);
     SHKZ_HIP(hipGetLastError());
     return SHKZ_OK;
 }
@@ -1924,6 +1925,8 @@ static int extrapolate_stages(shkz_ctx *C, const shkz_velocity_field *f, int wid
 static int validate_vfield(const shkz_velocity_field *f, int width, int constrain, bool device) {
     if (!f) return fail(SHKZ_ERR_INVALID, "NULL field");
     if (f->w <= 0 || f->h <= 0 || f->d <= 0) return fail(SHKZ_ERR_INVALID, "bad grid %dx%dx%d", f->w, f->h, f->d);
+    if ((int64_t)(f->w + 1) * (f->h + 1) * (f->d + 1) >= (int64_t)1 << 32)
+        return fail(SHKZ_ERR_INVALID, "grid too large for 32-bit indices");
     if (width < 0 || width > 254) return fail(SHKZ_ERR_INVALID, "width %d outside 0..254", width);
     if (!f->u || !f->v || !f->w_vel || !f->active_u || !f->active_v || !f->active_w)
         return fail(SHKZ_ERR_INVALID, "velocity and activity arrays are required");
@@ -1975,6 +1978,8 @@ static int validate_advect(const shkz_velocity_field *f, const shkz_advect_param
     if (!f || !p) return fail(SHKZ_ERR_INVALID, "NULL field or parameters");
     if (f->w < 2 || f->h < 2 || f->d < 2)
         return fail(SHKZ_ERR_INVALID, "advection needs at least 2 cells per axis (got %dx%dx%d)", f->w, f->h, f->d);
+    if ((int64_t)(f->w + 1) * (f->h + 1) * (f->d + 1) >= (int64_t)1 << 32)
+        return fail(SHKZ_ERR_INVALID, "grid too large for 32-bit indices");
     if (!f->u || !f->v || !f->w_vel || !f->active_u || !f->active_v || !f->active_w)
         return fail(SHKZ_ERR_INVALID, "velocity and activity arrays are required");
     if (!std::isfinite(p->dt)) return fail(SHKZ_ERR_INVALID, "dt is not finite");

@@ -58,7 +58,8 @@ __global__ void k_fill_fluid(ShkzSceneParams S, int W, int H, int D, double dx, 
 #pragma clang fp contract(off)
     const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= (size_t)W * H * D) return;
-    const int k = (int)(c % D), j = (int)((c / D) % H), i = (int)(c / ((size_t)D * H));
+    const unsigned t = (unsigned)c / (unsigned)D;  // 32-bit decode (boxes below 2^32 cells)
+    const int k = (int)((unsigned)c - t * (unsigned)D), j = (int)(t % (unsigned)H), i = (int)(t / (unsigned)H);
     const double p[3] = {(i + 0.5) * dx, (j + 0.5) * dx, (k + 0.5) * dx};
     double val = fluid_sdf(S, p);
     if (with_solid) val = fmax(val, -(solid_sdf(S, p) + dx));
@@ -71,7 +72,8 @@ __global__ void k_fill_solid(ShkzSceneParams S, int W, int H, int D, double dx, 
     const size_t n = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int W1 = W + 1, H1 = H + 1, D1 = D + 1;
     if (n >= (size_t)W1 * H1 * D1) return;
-    const int k = (int)(n % D1), j = (int)((n / D1) % H1), i = (int)(n / ((size_t)D1 * H1));
+    const unsigned t = (unsigned)n / (unsigned)D1;
+    const int k = (int)((unsigned)n - t * (unsigned)D1), j = (int)(t % (unsigned)H1), i = (int)(t / (unsigned)H1);
     const double p[3] = {i * dx, j * dx, k * dx};
     solid[n] = with_solid ? (float)solid_sdf(S, p) : 1.0f;
 }
@@ -98,6 +100,7 @@ extern "C" int shkz_scene_fill_device(const shkz_scene_desc *desc, void *fluid, 
     const int rc = shkz_scene_params_internal(desc, &S);
     if (rc != SHKZ_OK) return rc;
     const int W = desc->w, H = desc->h, D = desc->d;
+    if ((uint64_t)(W + 1) * (H + 1) * (D + 1) >= ((uint64_t)1 << 32)) return SHKZ_ERR_INVALID;  // 32-bit indices
     const double dx = std::min(1.0 / W, std::min(1.0 / H, 1.0 / D));
     const int with_solid = (S.n_solid > 0 || S.has_box) ? 1 : 0;
     if (has_solid) *has_solid = with_solid;
