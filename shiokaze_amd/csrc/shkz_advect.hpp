@@ -155,10 +155,9 @@ __device__ __forceinline__ double interp_level(const void *phi, int f64, const i
 __global__ void k_adv_face_fwd(VelField F, const float *in0, const float *in1, const float *in2, AdvScratch S,
                                const void *fluid, int fluid64, double dt, int trim, int minmax) {
 #pragma clang fp contract(off)
-    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     int a, i, j, k;
     size_t f;
-    if (!vf_decode(F, e, a, f, i, j, k)) return;
+    if (!vf_decode(F, a, f, i, j, k)) return;
     if (!F.act[a][f]) return;
     const float *const in[3] = {in0, in1, in2};
     float u[3];
@@ -212,10 +211,9 @@ __global__ void k_adv_face_fwd(VelField F, const float *in0, const float *in1, c
 // input field, read only at the thread's own face).
 __global__ void k_adv_face_bwd(VelField F, AdvScratch S, double dt_back) {
 #pragma clang fp contract(off)
-    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     int a, i, j, k;
     size_t f;
-    if (!vf_decode(F, e, a, f, i, j, k)) return;
+    if (!vf_decode(F, a, f, i, j, k)) return;
     if (!F.act[a][f]) return;
     const float v0 = S.v0[a][f];
     if (S.nb[a][f]) {
@@ -249,10 +247,9 @@ __global__ void k_adv_face_bwd(VelField F, AdvScratch S, double dt_back) {
 
 // semi-Lagrangian only: the forward result over the active faces
 __global__ void k_adv_face_copy(VelField F, AdvScratch S) {
-    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     int a, i, j, k;
     size_t f;
-    if (!vf_decode(F, e, a, f, i, j, k)) return;
+    if (!vf_decode(F, a, f, i, j, k)) return;
     if (F.act[a][f]) F.vel[a][f] = S.v0[a][f];
 }
 

@@ -13,8 +13,8 @@
 //                   needs no full copy (the reference samples a copy, velocity_save)
 //
 // Layout: the dense face arrays of the projection (u (w+1,h,d), v (w,h+1,d), w (w,h,d+1), z
-// fastest) with one activity byte per face.  One launch covers the three arrays (a flat index
-// over u, then v, then w).  A pass works in place: an inactive face reads only neighbours that
+// fastest) with one activity byte per face.  One launch covers the three arrays (blockIdx.y
+// selects the array).  A pass works in place: an inactive face reads only neighbours that
 // were active before the pass, which the activity byte records as the pass that activated it
 // (0 inactive, 1 active at the start, p+1 activated by pass p), so a pass needs no copy; the
 // constraint kernel turns the codes back into 0/1.
@@ -44,20 +44,21 @@ __device__ __forceinline__ void vf_shape(const VelField &F, int a, int s[3]) {
     s[2] = F.d + (a == 2);
 }
 
-// flat index over the three arrays -> (array, local index, i, j, k); false past the end
-__device__ __forceinline__ bool vf_decode(const VelField &F, size_t e, int &a, size_t &f, int &i, int &j, int &k) {
-    if (e < F.n[0]) {
-        a = 0;
-        f = e;
-    } else if (e < F.n[0] + F.n[1]) {
-        a = 1;
-        f = e - F.n[0];
-    } else if (e < F.n[0] + F.n[1] + F.n[2]) {
-        a = 2;
-        f = e - F.n[0] - F.n[1];
-    } else {
-        return false;
-    }
+// Face kernels launch a (ceil(max n / 256), 3) grid: blockIdx.y is the face array, so the array
+// index is uniform across a wavefront and every per-array field (extent, pointers) is a scalar
+// load.  (A flat index over the three arrays made it vary inside the wavefront that straddles two
+// arrays, and hipcc then lowered a dynamically indexed kernel-argument read with v_readfirstlane,
+// as if it were uniform: that wavefront's lanes read the first lane's array extent.)
+inline dim3 vf_grid(const size_t n[3]) {
+    const size_t m = n[0] > n[1] ? (n[0] > n[2] ? n[0] : n[2]) : (n[1] > n[2] ? n[1] : n[2]);
+    return dim3((unsigned)((m + 255) / 256), 3);
+}
+
+// (array, local index, i, j, k) of this thread; false past the end of its array
+__device__ __forceinline__ bool vf_decode(const VelField &F, int &a, size_t &f, int &i, int &j, int &k) {
+    a = (int)blockIdx.y;
+    f = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= F.n[a]) return false;
     int s[3];
     vf_shape(F, a, s);
     // 32-bit decode (each face array holds fewer than 2^32 faces, checked at entry)
@@ -73,10 +74,9 @@ __device__ __forceinline__ bool vf_decode(const VelField &F, size_t e, int &a, s
 // +x, -x, +y, -y, -z, +z (array_extrapolator3.h:57-72), and is marked activated by this pass.
 __global__ void k_extrap_pass(VelField F, int pass) {
 #pragma clang fp contract(off)
-    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     int a, i, j, k;
     size_t f;
-    if (!vf_decode(F, e, a, f, i, j, k)) return;
+    if (!vf_decode(F, a, f, i, j, k)) return;
     const uint8_t *A = F.act[a];
     if (A[f] != 0) return;
     int s[3];
@@ -245,10 +245,9 @@ constexpr uint8_t ACT_CONSTRAINED = 255;
 // ACT_CONSTRAINED.  The velocity arrays are only read here.
 __global__ void k_constrain(VelField F, float *X0, float *X1, float *X2, const int *exist, int constrain) {
 #pragma clang fp contract(off)
-    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     int a, i, j, k;
     size_t f;
-    if (!vf_decode(F, e, a, f, i, j, k)) return;
+    if (!vf_decode(F, a, f, i, j, k)) return;
     const int t = F.act[a][f];
     if (!t) return;
     if (constrain && F.solid && *exist) {
@@ -288,10 +287,9 @@ __global__ void k_constrain(VelField F, float *X0, float *X1, float *X2, const i
 // solid present) active faces on the domain boundary pointing out of the domain become 0.
 __global__ void k_constrain_apply(VelField F, const float *X0, const float *X1, const float *X2, const int *exist,
                                   int constrain) {
-    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     int a, i, j, k;
     size_t f;
-    if (!vf_decode(F, e, a, f, i, j, k)) return;
+    if (!vf_decode(F, a, f, i, j, k)) return;
     const int t = F.act[a][f];
     const int pi[3] = {i, j, k}, N[3] = {F.w, F.h, F.d};
     const bool boundary = pi[a] == 0 || pi[a] == N[a];
