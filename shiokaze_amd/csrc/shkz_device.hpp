@@ -42,12 +42,39 @@ struct GridLevel {
 
 // Per stored brick: its six neighbour bricks (NB_*, -1 = none), its grid and flags, in one
 // 32-byte record (one scalar load per workgroup).
-enum { BM_COARSEST = 1 };  // the brick's grid ends on this level (its coarse solve handles it)
+enum {
+    BM_COARSEST = 1,  // the brick's grid ends on this level (its coarse solve handles it)
+    BM_UNIFORM = 2,   // level 0: the operator is one constant on the brick (UniCoef)
+};
 struct __align__(32) BrickMeta {
     int nbr[6];
     int grid;
     int flags;
 };
+
+// A level-0 brick on which every cell is an unknown without a Dirichlet term and every face of
+// every cell (the high faces of its last layers included) has the same conductance g -- the
+// interior of the liquid away from surfaces and solids, most of the domain.  The smoothing and
+// operator kernels take g and the inverse diagonal from this record instead of streaming the
+// four operator arrays and idiag: the same values, so the same results, with 5 fewer arrays read.
+struct __align__(32) UniCoef {
+    double g64, id64;  // FP64 conductance and inverse diagonal (FP64 smoothers)
+    float g32, id32;   // the FP32 copies of the mixed-precision V-cycle
+    float pad[2];
+};
+
+template <class T>
+struct UniT {
+    T g, id;
+};
+__device__ __forceinline__ void uni_pick(const UniCoef &u, UniT<double> &o) {
+    o.g = u.g64;
+    o.id = u.id64;
+}
+__device__ __forceinline__ void uni_pick(const UniCoef &u, UniT<float> &o) {
+    o.g = u.g32;
+    o.id = u.id32;
+}
 
 // One multigrid level of a batch of grids: the stored bricks of every grid, concatenated
 // (grid-major, Morton order inside a grid).  A single grid is a batch of one.
@@ -62,7 +89,15 @@ struct LevelDesc {
     const GridLevel *gl;
     const int *child64;      // brick -> the 4x4x4 finer-level bricks 2B-1 .. 2B+2 per axis (-1 = none)
     const int *parent8;      // brick -> the 2x2x2 coarser-level bricks (B+1)/2 - 1 + {0,1} per axis
+    const UniCoef *uni;      // level 0: constants of the BM_UNIFORM bricks (nullptr elsewhere)
 };
+
+template <class T>
+__device__ __forceinline__ UniT<T> uni_load(const LevelDesc &L, int br) {
+    UniT<T> o;
+    uni_pick(L.uni[br], o);
+    return o;
+}
 
 template <class T>
 struct OpDesc {

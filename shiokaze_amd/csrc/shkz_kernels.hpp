@@ -24,6 +24,8 @@
 namespace shkz {
 
 // ------------------------------------------------------------------------------ helpers
+constexpr int UNIFORM_COUNT_BIT = 1 << 20;  // brick counts: the brick is BM_UNIFORM (counts <= 512)
+
 struct Nbrs {
     long base;
     int n[6];
@@ -73,6 +75,21 @@ __device__ __forceinline__ T offdiag_sum(const OpDesc<T> &A, const T *__restrict
     const T gxp = ld(A.gx, idx[NB_XP]), gyp = ld(A.gy, idx[NB_YP]), gzp = ld(A.gz, idx[NB_ZP]);
     return A.gx[c] * ld(v, idx[NB_XM]) + gxp * ld(v, idx[NB_XP]) + A.gy[c] * ld(v, idx[NB_YM]) +
            gyp * ld(v, idx[NB_YP]) + A.gz[c] * ld(v, idx[NB_ZM]) + gzp * ld(v, idx[NB_ZP]);
+}
+
+// apply_row and offdiag_sum on a BM_UNIFORM brick: every coupling is g and the Dirichlet sum 0
+// (the same operations in the same order, so the same values).
+template <class T>
+__device__ __forceinline__ T offdiag_uniform(T g, const T *__restrict__ v, const long idx[6]) {
+    return g * ld(v, idx[NB_XM]) + g * ld(v, idx[NB_XP]) + g * ld(v, idx[NB_YM]) + g * ld(v, idx[NB_YP]) +
+           g * ld(v, idx[NB_ZM]) + g * ld(v, idx[NB_ZP]);
+}
+template <class T>
+__device__ __forceinline__ T apply_row_uniform(T g, const T *__restrict__ p, long c, const long idx[6], T pc) {
+    (void)c;
+    const T off = offdiag_uniform(g, p, idx);
+    const T diag = T(0) + ((g + g) + (g + g) + (g + g));
+    return diag * pc - off;
 }
 
 __device__ __forceinline__ int brick_or_idle(const LevelDesc &L) {
@@ -345,13 +362,53 @@ __global__ __launch_bounds__(BLOCK) void k_idiag(LevelDesc L, OpDesc<T> A, T *__
     idiag[c] = out;
 }
 
-// Number of unknowns (dir >= 0) in every stored brick.
+// Marks the level-0 bricks whose operator is one constant (BM_UNIFORM) and records the constant
+// and the inverse diagonal the smoothers use (from whichever idiag array the level keeps; 0 when
+// none is built).  Runs after the operator and idiag kernels; the meta records were uploaded
+// without the flag.
+__global__ __launch_bounds__(BLOCK) void k_uniform(LevelDesc L, OpDesc<double> A, const double *__restrict__ id64,
+                                                   const float *__restrict__ id32, BrickMeta *meta, UniCoef *uni) {
+    __shared__ double gref, iref64;
+    __shared__ float iref32;
+    const int br = brick_or_idle(L);
+    if (br < 0) return;
+    const int t = threadIdx.x;
+    const long c = (long)br * BRICK_CELLS + t;
+    if (t == 0) {
+        gref = A.gx[c];
+        iref64 = id64 ? id64[c] : 0.0;
+        iref32 = id32 ? id32[c] : 0.0f;
+    }
+    __syncthreads();
+    const double G = gref;
+    const BrickMeta m = L.meta[br];
+    const Nbrs N = nbrs_of(m, br);
+    long idx[6];
+    nb_index(N, t, idx);
+    bool ok = G > 0.0 && A.dir[c] == 0.0 && A.gx[c] == G && A.gy[c] == G && A.gz[c] == G;
+    ok = ok && idx[NB_XP] >= 0 && idx[NB_YP] >= 0 && idx[NB_ZP] >= 0;
+    ok = ok && A.gx[idx[NB_XP] >= 0 ? idx[NB_XP] : c] == G && A.gy[idx[NB_YP] >= 0 ? idx[NB_YP] : c] == G &&
+         A.gz[idx[NB_ZP] >= 0 ? idx[NB_ZP] : c] == G;
+    ok = ok && (!id64 || id64[c] == iref64) && (!id32 || id32[c] == iref32);
+    if (__syncthreads_and(ok) && t == 0) {
+        meta[br].flags = m.flags | BM_UNIFORM;
+        UniCoef u;
+        u.g64 = G;
+        u.id64 = iref64;
+        u.g32 = (float)G;
+        u.id32 = iref32;
+        u.pad[0] = u.pad[1] = 0.0f;
+        uni[br] = u;
+    }
+}
+
+// Number of unknowns (dir >= 0) in every stored brick, plus UNIFORM_COUNT_BIT on BM_UNIFORM bricks.
 template <class T>
 __global__ __launch_bounds__(BLOCK) void k_brick_counts(LevelDesc L, const T *__restrict__ dir, int *counts) {
     const int br = brick_or_idle(L);
     if (br < 0) return;
     const int n = __syncthreads_count(dir[(long)br * BRICK_CELLS + threadIdx.x] >= 0);
-    if (threadIdx.x == 0) counts[br] = n;
+    if (threadIdx.x == 0) counts[br] = n | ((L.meta[br].flags & BM_UNIFORM) ? UNIFORM_COUNT_BIT : 0);
 }
 
 // ------------------------------------------------------------------------------ Krylov (batch)
@@ -367,14 +424,21 @@ __global__ __launch_bounds__(BLOCK) void k_defect(LevelDesc L, OpDesc<T> A, cons
     if (br < 0) return;
     const int t = threadIdx.x;
     const long c = (long)br * BRICK_CELLS + t;
-    const T dirc = A.dir[c];
+    const BrickMeta m = L.meta[br];
     T out = 0;
-    if (dirc >= 0) {
-        const Nbrs N = load_nbrs(L, br);
+    if (m.flags & BM_UNIFORM) {
+        const UniT<T> u = uni_load<T>(L, br);
         long idx[6];
-        nb_index(N, t, idx);
-        T diag;
-        out = b[c] - apply_row(A, x, c, idx, dirc, x[c], diag);
+        nb_index(nbrs_of(m, br), t, idx);
+        out = b[c] - apply_row_uniform(u.g, x, c, idx, x[c]);
+    } else {
+        const T dirc = A.dir[c];
+        if (dirc >= 0) {
+            long idx[6];
+            nb_index(nbrs_of(m, br), t, idx);
+            T diag;
+            out = b[c] - apply_row(A, x, c, idx, dirc, x[c], diag);
+        }
     }
     r[c] = out;
     double mx = fabs((double)out), sq = (double)out * (double)out;
@@ -410,17 +474,26 @@ __global__ __launch_bounds__(BLOCK) void k_apply_dot(LevelDesc L, OpDesc<T> A, c
     if (ctl->all_done || (L.ngrids > 1 && S[m.grid].done)) return;
     const int t = threadIdx.x;
     const long c = (long)br * BRICK_CELLS + t;
-    const T dirc = A.dir[c];
     T out = 0;
     double part = 0.0;
-    if (dirc >= 0) {
-        const Nbrs N = nbrs_of(m, br);
+    if (m.flags & BM_UNIFORM) {
+        const UniT<T> u = uni_load<T>(L, br);
         long idx[6];
-        nb_index(N, t, idx);
+        nb_index(nbrs_of(m, br), t, idx);
         const T pc = p[c];
-        T diag;
-        out = apply_row(A, p, c, idx, dirc, pc, diag);
+        out = apply_row_uniform(u.g, p, c, idx, pc);
         part = (double)pc * (double)out;
+    } else {
+        const T dirc = A.dir[c];
+        if (dirc >= 0) {
+            const Nbrs N = nbrs_of(m, br);
+            long idx[6];
+            nb_index(N, t, idx);
+            const T pc = p[c];
+            T diag;
+            out = apply_row(A, p, c, idx, dirc, pc, diag);
+            part = (double)pc * (double)out;
+        }
     }
     q[c] = out;
     part = block_sum(part, lds);
@@ -685,6 +758,21 @@ __global__ __launch_bounds__(BLOCK) void k_smooth2(LevelDesc L, OpDesc<T> A, con
     if (all_done(Q) || skip_smooth(Q, L, m)) return;
     const int t = threadIdx.x;
     const long c = (long)br * BRICK_CELLS + t;
+    if (m.flags & BM_UNIFORM) {
+        const UniT<T> u = uni_load<T>(L, br);
+        const Nbrs N = nbrs_of(m, br);
+        long idx[6];
+        nb_index(N, t, idx);
+        // neighbours in the brick share its inverse diagonal; the others are read
+        auto z1 = [&](long i) {
+            return i >= 0 ? omega * ((i >= N.base && i < N.base + BRICK_CELLS) ? u.id : idiag[i]) * rhs[i] : T(0);
+        };
+        const T off = u.g * z1(idx[NB_XM]) + u.g * z1(idx[NB_XP]) + u.g * z1(idx[NB_YM]) + u.g * z1(idx[NB_YP]) +
+                      u.g * z1(idx[NB_ZM]) + u.g * z1(idx[NB_ZP]);
+        const T r = rhs[c];
+        z[c] = (T(1) - omega) * (omega * u.id * r) + omega * u.id * (r + off);
+        return;
+    }
     const T id = idiag[c];
     T out = 0;
     if (id > 0) {
@@ -714,16 +802,25 @@ __global__ __launch_bounds__(BLOCK) void k_jacobi(LevelDesc L, OpDesc<T> A, cons
     if (all_done(Q) || skip_smooth(Q, L, m)) return;
     const int t = threadIdx.x;
     const long c = (long)br * BRICK_CELLS + t;
-    const T id = idiag[c];
     T out = 0;
     double part = 0.0;
-    if (id > 0) {
-        const Nbrs N = nbrs_of(m, br);
+    if (m.flags & BM_UNIFORM) {
+        const UniT<T> u = uni_load<T>(L, br);
         long idx[6];
-        nb_index(N, t, idx);
+        nb_index(nbrs_of(m, br), t, idx);
         const T r = rhs[c];
-        out = (T(1) - omega) * zin[c] + omega * id * (r + offdiag_sum(A, zin, c, idx));
+        out = (T(1) - omega) * zin[c] + omega * u.id * (r + offdiag_uniform(u.g, zin, idx));
         if (DOT) part = (double)r * (double)out;
+    } else {
+        const T id = idiag[c];
+        if (id > 0) {
+            const Nbrs N = nbrs_of(m, br);
+            long idx[6];
+            nb_index(N, t, idx);
+            const T r = rhs[c];
+            out = (T(1) - omega) * zin[c] + omega * id * (r + offdiag_sum(A, zin, c, idx));
+            if (DOT) part = (double)r * (double)out;
+        }
     }
     zout[c] = out;
     if (DOT) {
@@ -742,14 +839,21 @@ __global__ __launch_bounds__(BLOCK) void k_residual(LevelDesc L, OpDesc<T> A, co
     if (all_done(Q) || skip_smooth(Q, L, m)) return;
     const int t = threadIdx.x;
     const long c = (long)br * BRICK_CELLS + t;
-    const T dirc = A.dir[c];
     T out = 0;
-    if (dirc >= 0) {
-        const Nbrs N = nbrs_of(m, br);
+    if (m.flags & BM_UNIFORM) {
+        const UniT<T> u = uni_load<T>(L, br);
         long idx[6];
-        nb_index(N, t, idx);
-        T diag;
-        out = rhs[c] - apply_row(A, z, c, idx, dirc, z[c], diag);
+        nb_index(nbrs_of(m, br), t, idx);
+        out = rhs[c] - apply_row_uniform(u.g, z, c, idx, z[c]);
+    } else {
+        const T dirc = A.dir[c];
+        if (dirc >= 0) {
+            const Nbrs N = nbrs_of(m, br);
+            long idx[6];
+            nb_index(N, t, idx);
+            T diag;
+            out = rhs[c] - apply_row(A, z, c, idx, dirc, z[c], diag);
+        }
     }
     res[c] = out;
 }
@@ -833,7 +937,7 @@ __global__ __launch_bounds__(BLOCK) void k_prolong(LevelDesc Lf, OpDesc<T> Af, c
     }
     __syncthreads();
     const long c = (long)br * BRICK_CELLS + t;
-    if (!(Af.dir[c] >= 0)) return;
+    if (!(m.flags & BM_UNIFORM) && !(Af.dir[c] >= 0)) return;
     const int li = t >> 6, lj = (t >> 3) & 7, lk = t & 7;
     const int a0 = (li >> 1) + 1, b0 = (lj >> 1) + 1, c0 = (lk >> 1) + 1;
     const int a1 = (li & 1) ? a0 + 1 : a0 - 1, b1 = (lj & 1) ? b0 + 1 : b0 - 1, c1 = (lk & 1) ? c0 + 1 : c0 - 1;

@@ -192,6 +192,7 @@ struct LevelHost {
     void *dir = nullptr, *gx = nullptr, *gy = nullptr, *gz = nullptr, *idiag = nullptr;
     void *rhs = nullptr, *z = nullptr, *tmp = nullptr, *res = nullptr;
     int64_t unknowns = 0;
+    int64_t uniform_cells = 0;  // cells of BM_UNIFORM bricks (level 0)
     std::vector<int64_t> unknowns_grid;
 };
 
@@ -484,6 +485,12 @@ static void resolve_timing(shkz_ctx *C) {
     C->pending.clear();
 }
 
+// Byte model: the bytes a level-0 kernel does not stream on BM_UNIFORM bricks (`per_cell` per
+// cell), subtracted from its per-unknown figure; 0 on other levels.
+static double usave(const shkz_ctx *C, int level, double per_cell) {
+    return level == 0 ? per_cell * (double)C->levels[0].uniform_cells : 0.0;
+}
+
 // ------------------------------------------------------------------------ solve plumbing
 struct Problem {
     int ngrids = 0;
@@ -610,15 +617,15 @@ static T *vcycle(shkz_ctx *C, Launcher &K, Problem &P, const T *rhs0, T *z0, T *
             K.go(f ? "smooth_first_L0" : "smooth_first_Lc", l, 3 * es, 0, g, b,
                  f ? k_smooth_first<T, 1> : k_smooth_first<T, 0>, L.desc, id, rhs, z, om, Q);
         } else {
-            K.go(f ? "smooth2_L0" : "smooth2_Lc", l, 6 * es, 0, g, b, f ? k_smooth2<T, 1> : k_smooth2<T, 0>, L.desc, A,
+            K.go(f ? "smooth2_L0" : "smooth2_Lc", l, 6 * es, -usave(C, l, 4 * es), g, b, f ? k_smooth2<T, 1> : k_smooth2<T, 0>, L.desc, A,
                  id, rhs, z, om, Q);
             for (int s = 2; s < pre; ++s) {
-                K.go(f ? "jacobi_L0" : "jacobi_Lc", l, 7 * es, 0, g, b, f ? k_jacobi<T, 1, 0> : k_jacobi<T, 0, 0>,
-                     L.desc, A, id, rhs, (const T *)cur, oth, om, Q, P.red);
+                K.go(f ? "jacobi_L0" : "jacobi_Lc", l, 7 * es, -usave(C, l, 4 * es), g, b,
+                     f ? k_jacobi<T, 1, 0> : k_jacobi<T, 0, 0>, L.desc, A, id, rhs, (const T *)cur, oth, om, Q, P.red);
                 std::swap(cur, oth);
             }
         }
-        K.go(f ? "residual_L0" : "residual_Lc", l, 7 * es, 0, g, b, f ? k_residual<T, 1> : k_residual<T, 0>, L.desc, A,
+        K.go(f ? "residual_L0" : "residual_Lc", l, 7 * es, -usave(C, l, 4 * es), g, b, f ? k_residual<T, 1> : k_residual<T, 0>, L.desc, A,
              rhs, (const T *)cur, (T *)L.res, Q);
         LevelHost &Lc = C->levels[l + 1];
         K.go(f ? "restrict_L0" : "restrict_Lc", l, es, 2 * es * (double)Lc.unknowns, dim3(brick_grid(Lc.nbricks)), b,
@@ -635,7 +642,7 @@ static T *vcycle(shkz_ctx *C, Launcher &K, Problem &P, const T *rhs0, T *z0, T *
         const dim3 g(brick_grid(L.nbricks)), b(BLOCK);
         const bool f = l == 0;
         const int post = f ? P.post : P.post_c;
-        K.go(f ? "prolong_L0" : "prolong_Lc", l, 3 * es, es * (double)Lc.unknowns, g, b,
+        K.go(f ? "prolong_L0" : "prolong_Lc", l, 3 * es, es * (double)Lc.unknowns - usave(C, l, es), g, b,
              f ? k_prolong<T, 1> : k_prolong<T, 0>, L.desc, A, (const T *)zcur[l + 1], zcur[l], Q);
         T *cur = zcur[l], *oth = zoth[l];
         if (mpost[l]) {
@@ -648,11 +655,11 @@ static T *vcycle(shkz_ctx *C, Launcher &K, Problem &P, const T *rhs0, T *z0, T *
         for (int s = 0; s < post; ++s) {
             const bool last_dot = f && dot_partials && s == post - 1;
             if (last_dot)
-                K.go("jacobi_dot_L0", l, 7 * es, 0, g, b, k_jacobi<T, 1, 1>, L.desc, A, id, rhs, (const T *)cur, oth, om,
-                     Q, P.red);
+                K.go("jacobi_dot_L0", l, 7 * es, -usave(C, l, 4 * es), g, b, k_jacobi<T, 1, 1>, L.desc, A, id, rhs,
+                     (const T *)cur, oth, om, Q, P.red);
             else
-                K.go(f ? "jacobi_L0" : "jacobi_Lc", l, 7 * es, 0, g, b, f ? k_jacobi<T, 1, 0> : k_jacobi<T, 0, 0>,
-                     L.desc, A, id, rhs, (const T *)cur, oth, om, Q, P.red);
+                K.go(f ? "jacobi_L0" : "jacobi_Lc", l, 7 * es, -usave(C, l, 4 * es), g, b,
+                     f ? k_jacobi<T, 1, 0> : k_jacobi<T, 0, 0>, L.desc, A, id, rhs, (const T *)cur, oth, om, Q, P.red);
             std::swap(cur, oth);
         }
         zcur[l] = cur;
@@ -721,10 +728,10 @@ static std::string graph_key(shkz_ctx *C, const Problem &P, const void *const *v
     const int nl = precond ? P.nlev : 1;
     for (int l = 0; l < nl; ++l) {
         const LevelHost &L = C->levels[l];
-        std::snprintf(buf, sizeof(buf), "|%d %d %p %p %p %p %p %p %p %p %p %p %p %p %p %p %p", L.nbricks,
+        std::snprintf(buf, sizeof(buf), "|%d %d %p %p %p %p %p %p %p %p %p %p %p %p %p %p %p %p", L.nbricks,
                       L.any_coarsest ? 1 : 0, (const void *)L.desc.bmap, L.dir, L.gx, L.gy, L.gz, L.idiag, L.rhs, L.z,
                       L.tmp, L.res, (const void *)L.desc.bxyz, (const void *)L.desc.meta, (const void *)L.desc.gl,
-                      (const void *)L.desc.child64, (const void *)L.desc.parent8);
+                      (const void *)L.desc.child64, (const void *)L.desc.parent8, (const void *)L.desc.uni);
         k += buf;
     }
     return k;
@@ -758,7 +765,7 @@ static int krylov(shkz_ctx *C, Launcher &K, Problem &P, T *x, T *r, T *p, T *q, 
          1);
 
     auto iteration = [&]() {
-        K.go("apply_dot", 0, 6 * es, 0, g, b, k_apply_dot<T>, L0.desc, A, (const T *)p, q, (const SolveScalars *)S,
+        K.go("apply_dot", 0, 6 * es, -usave(C, 0, 4 * es), g, b, k_apply_dot<T>, L0.desc, A, (const T *)p, q, (const SolveScalars *)S,
              (const BatchCtl *)ctl, P.red);
         finalize(K, P, L0.desc, S, ctl, FIN_PQ);
         K.go("cg_update", 0, 6 * es, 0, g, b, k_cg_update<T>, L0.desc, x, r, (const T *)p, (const T *)q,
@@ -921,7 +928,7 @@ static int solve_phase(shkz_ctx *C, Launcher &K, Problem &P, const shkz_params *
         // no unknowns anywhere: nothing to solve
     } else if (!P.mixed) {
         // one FP64 Krylov solve per grid: CG (the reference iteration) or MGPCG
-        K.go("defect", 0, 7 * 8, 0, gb, bb, k_defect<double>, F0.desc, A64, (const double *)P.b64, (const double *)P.x64,
+        K.go("defect", 0, 7 * 8, -usave(C, 0, 32), gb, bb, k_defect<double>, F0.desc, A64, (const double *)P.b64, (const double *)P.x64,
              P.r64, P.red);
         finalize(K, P, F0.desc, P.S_out, P.ctl, FIN_DEFECT_FIRST, residual);
         BatchCtl hc{};
@@ -941,7 +948,7 @@ static int solve_phase(shkz_ctx *C, Launcher &K, Problem &P, const shkz_params *
         K.go("init_scalars", -1, 0, 0, dim3((unsigned)((ngrids + 255) / 256)), dim3(256), k_init_scalars, P.S_in,
              ngrids, norm, maxit);
         for (int outer = 0; outer < 64; ++outer) {
-            K.go("defect", 0, 7 * 8, 0, gb, bb, k_defect<double>, F0.desc, A64, (const double *)P.b64,
+            K.go("defect", 0, 7 * 8, -usave(C, 0, 32), gb, bb, k_defect<double>, F0.desc, A64, (const double *)P.b64,
                  (const double *)P.x64, P.r64, P.red);
             if (outer == 0)
                 finalize(K, P, F0.desc, P.S_out, nullptr, FIN_DEFECT_FIRST, residual);
@@ -975,7 +982,7 @@ static int solve_phase(shkz_ctx *C, Launcher &K, Problem &P, const shkz_params *
     }
     // true FP64 residual of the returned pressure, per grid
     if (F0.nbricks > 0) {
-        K.go("defect", 0, 7 * 8, 0, gb, bb, k_defect<double>, F0.desc, A64, (const double *)P.b64, (const double *)P.x64,
+        K.go("defect", 0, 7 * 8, -usave(C, 0, 32), gb, bb, k_defect<double>, F0.desc, A64, (const double *)P.b64, (const double *)P.x64,
              P.r64, P.red);
         finalize(K, P, F0.desc, P.S_fin, nullptr, FIN_DEFECT);
         SHKZ_TRY(read_scalars(C, s, P.S_fin, ngrids, hf));
@@ -1403,6 +1410,7 @@ static int project_stages(shkz_ctx *C, const shkz_grid *grids, int ngrids, const
             L.desc.gl = reinterpret_cast<const GridLevel *>(dmeta + o.gl);
             L.desc.child64 = L.child64.empty() ? nullptr : dmeta + o.c64;
             L.desc.parent8 = L.parent8.empty() ? nullptr : dmeta + o.p8;
+            L.desc.uni = nullptr;
         }
         // the pinned staging stays untouched until this call's next synchronisation
         SHKZ_HIP(hipMemcpyAsync(dmeta, host, total * sizeof(int), hipMemcpyHostToDevice, s));
@@ -1540,6 +1548,18 @@ static int project_stages(shkz_ctx *C, const shkz_grid *grids, int ngrids, const
                 K.go("idiag", -1, 0, 0, g, b, k_idiag<float>, L.desc, op_of<float>(L), (float *)L.idiag);
         }
     }
+    // level-0 bricks with a constant operator (BM_UNIFORM): the smoothers and operator kernels
+    // read their constants instead of the operator arrays
+    {
+        LevelHost &L = C->levels[0];
+        UniCoef *uni = nullptr;
+        SHKZ_TRY(arena_get(AR, "uni", (size_t)std::max(L.nbricks, 1), &uni));
+        L.desc.uni = uni;
+        const bool vc = multigrid && L.idiag;
+        K.go("uniform", -1, 0, 0, dim3(brick_grid(L.nbricks)), dim3(BLOCK), k_uniform, L.desc,
+             OpDesc<double>{P.dir64, P.gx64, P.gy64, P.gz64}, vc && P.fp64_levels ? (const double *)L.idiag : nullptr,
+             vc && !P.fp64_levels ? (const float *)L.idiag : nullptr, const_cast<BrickMeta *>(L.desc.meta), uni);
+    }
     // unknowns per level and grid (results + byte model of the kernel timers)
     {
         int total = 0;
@@ -1573,9 +1593,12 @@ static int project_stages(shkz_ctx *C, const shkz_grid *grids, int ngrids, const
             LevelHost &L = C->levels[l];
             L.unknowns_grid.assign(ngrids, 0);
             L.unknowns = 0;
+            L.uniform_cells = 0;
             for (int bk = 0; bk < L.nbricks; ++bk) {
-                L.unknowns_grid[L.bgrid[bk]] += hc[off + bk];
-                L.unknowns += hc[off + bk];
+                const int n = hc[off + bk] & (UNIFORM_COUNT_BIT - 1);
+                L.unknowns_grid[L.bgrid[bk]] += n;
+                L.unknowns += n;
+                if (hc[off + bk] & UNIFORM_COUNT_BIT) L.uniform_cells += BRICK_CELLS;
             }
             off += L.nbricks;
         }
