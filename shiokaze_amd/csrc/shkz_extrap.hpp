@@ -3,8 +3,10 @@
 // (src/utility/macutility3.cpp:89-93), which macliquid3::extend_both (macliquid3.cpp:309-319) and
 // macsmoke3 (macsmoke3.cpp:294) call on the velocity the projection left.
 //
-//   k_extrap_pass   one pass of array_extrapolator3::extrapolate (array_extrapolator3.h:47-80,
-//                   over dilate3.h:43-101) on all three face arrays
+//   k_pack_activity,
+//   k_extrap_pass,
+//   k_unpack_activity  array_extrapolator3::extrapolate (array_extrapolator3.h:47-80, over
+//                   dilate3.h:43-101) on all three face arrays, on bit-packed activity
 //   k_constrain,
 //   k_constrain_apply  macutility3::constrain_velocity (macutility3.cpp:61-88) with the
 //                   interpolation of array_interpolator3.h:40-97, array_derivative3.h:40-103 and
@@ -14,10 +16,8 @@
 //
 // Layout: the dense face arrays of the projection (u (w+1,h,d), v (w,h+1,d), w (w,h,d+1), z
 // fastest) with one activity byte per face.  One launch covers the three arrays (blockIdx.y
-// selects the array).  A pass works in place: an inactive face reads only neighbours that
-// were active before the pass, which the activity byte records as the pass that activated it
-// (0 inactive, 1 active at the start, p+1 activated by pass p), so a pass needs no copy; the
-// constraint kernel turns the codes back into 0/1.
+// selects the array).  The extrapolation passes ping-pong two bit-packed copies of the activity
+// and write the values of the faces they activate in place (no thread of a pass reads them).
 //
 // Arithmetic matches the reference's types operation by operation (no contraction): float sums
 // of float velocities, interpolation into the array's element type after a double product, vec3d
@@ -69,39 +69,101 @@ __device__ __forceinline__ bool vf_decode(const VelField &F, int &a, size_t &f, 
     return true;
 }
 
-// One extrapolation pass (1-based `pass`): every inactive face with an in-bounds neighbour that
-// was active before the pass takes the mean of those neighbours, summed in float in the order
-// +x, -x, +y, -y, -z, +z (array_extrapolator3.h:57-72), and is marked activated by this pass.
-__global__ void k_extrap_pass(VelField F, int pass) {
-#pragma clang fp contract(off)
-    int a, i, j, k;
-    size_t f;
-    if (!vf_decode(F, a, f, i, j, k)) return;
-    const uint8_t *A = F.act[a];
-    if (A[f] != 0) return;
+// The extrapolation works on bit-packed activity: each row (i, j) of a face array is a run of
+// 32-bit words along k, bit b of word kw = face k = 32 kw + b.  A pass is then one thread per word:
+// it forms the six neighbour masks from the word, its two neighbours along k and the words of the
+// rows +-1 in j and i, and activates the inactive faces with an active neighbour.
+struct BitField {
+    uint32_t *bits[3];
+    int W[3];  // words per row
+};
+
+__device__ __forceinline__ int vf_rows(const VelField &F, int a) { return (F.w + (a == 0)) * (F.h + (a == 1)); }
+
+// activity bytes -> bits: one lane per face, a wavefront ballot forms two words
+__global__ void k_pack_activity(VelField F, BitField B) {
+    const int a = (int)blockIdx.y;
     int s[3];
     vf_shape(F, a, s);
-    const size_t sx = (size_t)s[1] * s[2], sy = (size_t)s[2];
-    const float *U = F.vel[a];
-    const bool ok[6] = {i + 1 < s[0], i > 0, j + 1 < s[1], j > 0, k > 0, k + 1 < s[2]};
-    const long off[6] = {(long)sx, -(long)sx, (long)sy, -(long)sy, -1, 1};
-    float sum = 0.0f;
-    int weight = 0;
-#pragma unroll
-    for (int q = 0; q < 6; ++q) {
-        if (!ok[q]) continue;
-        const size_t g = (size_t)((long)f + off[q]);
-        const int t = A[g];
-        if (t != 0 && t <= pass) {
-            sum += U[g];
-            weight++;
-        }
+    const int W = B.W[a];
+    const size_t lane = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t w = lane >> 5;
+    const size_t nw = (size_t)vf_rows(F, a) * W;
+    bool on = false;
+    if (w < nw) {
+        const unsigned r = (unsigned)(w / (unsigned)W), kw = (unsigned)(w - (size_t)r * W);
+        const int k = (int)(kw * 32 + (lane & 31));
+        if (k < s[2]) on = F.act[a][(size_t)r * s[2] + k] != 0;
     }
-    if (weight) {
+    const unsigned long long m = __ballot(on);
+    if (w < nw && (lane & 31) == 0) B.bits[a][w] = (uint32_t)((threadIdx.x & 32) ? (m >> 32) : m);
+}
+
+// One extrapolation pass: Bo = Bi plus every inactive face with an in-bounds neighbour active in
+// Bi, which takes the mean of those neighbours, summed in float in the order +x, -x, +y, -y, -z,
+// +z (array_extrapolator3.h:57-72).  The faces a pass activates are inactive in Bi, so no thread
+// of the pass reads their values.
+__global__ void k_extrap_pass(VelField F, BitField Bi, BitField Bo) {
+#pragma clang fp contract(off)
+    const int a = (int)blockIdx.y;
+    int s[3];
+    vf_shape(F, a, s);
+    const int W = Bi.W[a];
+    const size_t w = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t nw = (size_t)vf_rows(F, a) * W;
+    if (w >= nw) return;
+    const unsigned r = (unsigned)(w / (unsigned)W), kw = (unsigned)(w - (size_t)r * W);
+    const int i = (int)(r / (unsigned)s[1]), j = (int)(r - (unsigned)i * s[1]);
+    const uint32_t *B = Bi.bits[a];
+    const uint32_t A = B[w];
+    const uint32_t nxt = kw + 1 < (unsigned)W ? B[w + 1] : 0u, prv = kw > 0 ? B[w - 1] : 0u;
+    const uint32_t kp = (A >> 1) | (nxt << 31);  // bit b: face k+1 active
+    const uint32_t km = (A << 1) | (prv >> 31);  // bit b: face k-1 active
+    const uint32_t jp = j + 1 < s[1] ? B[w + W] : 0u, jm = j > 0 ? B[w - W] : 0u;
+    const size_t sxw = (size_t)s[1] * W;
+    const uint32_t ip = i + 1 < s[0] ? B[w + sxw] : 0u, im = i > 0 ? B[w - sxw] : 0u;
+    const int nbits = s[2] - (int)kw * 32;
+    const uint32_t valid = nbits >= 32 ? 0xffffffffu : ((1u << nbits) - 1u);
+    uint32_t cand = (kp | km | jp | jm | ip | im) & ~A & valid;
+    Bo.bits[a][w] = A | cand;
+    if (!cand) return;
+    float *U = F.vel[a];
+    const size_t sx = (size_t)s[1] * s[2], sy = (size_t)s[2];
+    const size_t f0 = (size_t)r * s[2] + (size_t)kw * 32;
+    while (cand) {
+        const int b = __builtin_ctz(cand);
+        cand &= cand - 1;
+        const size_t f = f0 + b;
+        float sum = 0.0f;
+        int weight = 0;
+        if ((ip >> b) & 1u) { sum += U[f + sx]; weight++; }
+        if ((im >> b) & 1u) { sum += U[f - sx]; weight++; }
+        if ((jp >> b) & 1u) { sum += U[f + sy]; weight++; }
+        if ((jm >> b) & 1u) { sum += U[f - sy]; weight++; }
+        if ((km >> b) & 1u) { sum += U[f - 1]; weight++; }
+        if ((kp >> b) & 1u) { sum += U[f + 1]; weight++; }
         // float / int in the reference; the double quotient rounded once to float is the
         // correctly rounded float quotient
-        F.vel[a][f] = (float)((double)sum / (double)weight);
-        F.act[a][f] = (uint8_t)(pass + 1);
+        U[f] = (float)((double)sum / (double)weight);
+    }
+}
+
+// bits -> activity bytes for the faces the passes activated (final & ~initial)
+__global__ void k_unpack_activity(VelField F, BitField B0, BitField Bf) {
+    const int a = (int)blockIdx.y;
+    int s[3];
+    vf_shape(F, a, s);
+    const int W = Bf.W[a];
+    const size_t w = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= (size_t)vf_rows(F, a) * W) return;
+    uint32_t fresh = Bf.bits[a][w] & ~B0.bits[a][w];
+    if (!fresh) return;
+    const unsigned r = (unsigned)(w / (unsigned)W), kw = (unsigned)(w - (size_t)r * W);
+    const size_t f0 = (size_t)r * s[2] + (size_t)kw * 32;
+    while (fresh) {
+        const int b = __builtin_ctz(fresh);
+        fresh &= fresh - 1;
+        F.act[a][f0 + b] = 1;
     }
 }
 

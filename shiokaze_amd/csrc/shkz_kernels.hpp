@@ -1146,8 +1146,9 @@ __device__ __forceinline__ void update_face(const GridDesc &G, int dim, int i, i
     const int pi[3] = {i, j, k};
     const int N[3] = {G.w, G.h, G.d};
     float *U = G.vel[dim];
-    const float area = face_area(G, dim, i, j, k);
+    // the area only matters where rho is nonzero: most faces of the domain have air on both sides
     const double rho = face_rho(G, dim, i, j, k);
+    const float area = rho != 0.0 ? face_area(G, dim, i, j, k) : 0.0f;
     uint8_t keep = 1;
     if (area != 0.0f && rho != 0.0) {
         if (pi[dim] == 0 || pi[dim] == N[dim]) {

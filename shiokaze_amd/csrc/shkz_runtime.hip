@@ -1929,7 +1929,35 @@ static int extrapolate_stages(shkz_ctx *C, const shkz_velocity_field *f, int wid
     }
     // byte models: a pass scans the activity bytes (neighbour reads hit the cache); the
     // constraint scans the activity and reads the nodal solid once
-    for (int p = 1; p <= width; ++p) K.go("extrap_pass", -1, 0, (double)total, grid, block, k_extrap_pass, F, p);
+    if (width > 0) {
+        // bit-packed activity: the initial bits and two ping-pong copies
+        BitField B[3];
+        size_t words = 0, wmax = 0;
+        int W[3];
+        for (int a = 0; a < 3; ++a) {
+            W[a] = (f->d + (a == 2) + 31) / 32;
+            const size_t nw = (size_t)(f->w + (a == 0)) * (f->h + (a == 1)) * W[a];
+            words += nw;
+            wmax = std::max(wmax, nw);
+        }
+        uint32_t *bits = nullptr;
+        SHKZ_TRY(arena_get(AR, "vx_bits", 3 * words, &bits));
+        size_t off = 0;
+        for (int c = 0; c < 3; ++c)
+            for (int a = 0; a < 3; ++a) {
+                B[c].bits[a] = bits + off;
+                B[c].W[a] = W[a];
+                off += (size_t)(f->w + (a == 0)) * (f->h + (a == 1)) * W[a];
+            }
+        const dim3 gpack((unsigned)((wmax * 32 + 255) / 256), 3), gword((unsigned)((wmax + 255) / 256), 3);
+        // byte models: the pack reads the activity bytes and writes the bits; a pass reads and
+        // writes the bits (the values of the faces it activates are few)
+        K.go("extrap_pack", -1, 0, (double)total + 4.0 * words, gpack, block, k_pack_activity, F, B[0]);
+        for (int p = 1; p <= width; ++p)
+            K.go("extrap_pass", -1, 0, 8.0 * words, gword, block, k_extrap_pass, F, B[p == 1 ? 0 : 1 + (p & 1)],
+                 B[1 + ((p + 1) & 1)]);
+        K.go("extrap_unpack", -1, 0, 8.0 * words, gword, block, k_unpack_activity, F, B[0], B[1 + ((width + 1) & 1)]);
+    }
     float *xs[3] = {nullptr, nullptr, nullptr};
     if (with_solid) {
         const char *names[3] = {"vx_side0", "vx_side1", "vx_side2"};
