@@ -965,11 +965,16 @@ struct SmoothK {
     static constexpr int CPT = (NR + THREADS - 1) / THREADS;  // cube cells per thread
 };
 
-template <class T, int K, int FROM_ZERO>
+//
+// RES: the cube carries one more ghost layer, so that after the K sweeps the brick's neighbours are
+// exact too, and the kernel also writes the residual rhs - A z of the brick (what k_residual
+// computes next, in the same arithmetic): one launch fewer per V-cycle on the level.
+template <class T, int K, int FROM_ZERO, int RES = 0>
 __global__ __launch_bounds__(1024) void k_smooth_k(LevelDesc L, OpDesc<T> A, const T *__restrict__ idiag,
                                                    const T *__restrict__ rhs, const T *__restrict__ zin,
-                                                   T *__restrict__ zout, T omega, Gate Q) {
-    using C = SmoothK<T, K>;
+                                                   T *__restrict__ zout, T omega, Gate Q, T *__restrict__ res) {
+    constexpr int GH = K + RES;  // ghost cells per side
+    using C = SmoothK<T, GH>;
     constexpr int R = C::R, NR = C::NR, CPT = C::CPT;
     __shared__ T zs[2][NR];
     __shared__ T gxs[NR], gys[NR], gzs[NR];
@@ -997,8 +1002,8 @@ __global__ __launch_bounds__(1024) void k_smooth_k(LevelDesc L, OpDesc<T> A, con
         rr[q] = 0;
         if (e >= NR) continue;
         const int a = e / (R * R), b = (e / R) % R, c = e % R;
-        // cube offset a is brick-local cell a - K, in brick (-1, 0 or +1) + local (0..7)
-        const int xa = a - K + BRICK, xb = b - K + BRICK, xc = c - K + BRICK;  // 0 .. 23
+        // cube offset a is brick-local cell a - GH, in brick (-1, 0 or +1) + local (0..7)
+        const int xa = a - GH + BRICK, xb = b - GH + BRICK, xc = c - GH + BRICK;  // 0 .. 23
         const int bi = xa >> 3, bj = xb >> 3, bk = xc >> 3;
         const int id = nb27[(bi * 3 + bj) * 3 + bk];
         T g0 = 0, g1 = 0, g2 = 0, z0 = 0;
@@ -1042,7 +1047,24 @@ __global__ __launch_bounds__(1024) void k_smooth_k(LevelDesc L, OpDesc<T> A, con
     }
     if (t < BRICK_CELLS) {
         const int li = t >> 6, lj = (t >> 3) & 7, lk = t & 7;
-        zout[(long)br * BRICK_CELLS + t] = zs[K & 1][((li + K) * R + (lj + K)) * R + (lk + K)];
+        const T *zz = zs[K & 1];
+        const int e = ((li + GH) * R + (lj + GH)) * R + (lk + GH);
+        const long c = (long)br * BRICK_CELLS + t;
+        zout[c] = zz[e];
+        if (RES) {
+            // apply_row on the cube (the neighbours of the brick are exact after the K sweeps)
+            const T dirc = A.dir[c];
+            T out = 0;
+            if (dirc >= 0) {
+                const T gxm = gxs[e], gym = gys[e], gzm = gzs[e];
+                const T gxp = gxs[e + R * R], gyp = gys[e + R], gzp = gzs[e + 1];
+                const T off = gxm * zz[e - R * R] + gxp * zz[e + R * R] + gym * zz[e - R] + gyp * zz[e + R] +
+                              gzm * zz[e - 1] + gzp * zz[e + 1];
+                const T diag = dirc + ((gxm + gxp) + (gym + gyp) + (gzm + gzp));
+                out = rhs[c] - (diag * zz[e] - off);
+            }
+            res[c] = out;
+        }
     }
 }
 

@@ -533,33 +533,52 @@ static bool smooth_k_supported(int k) {
     return k >= 2 && k <= (sizeof(T) == 4 ? 5 : 3);
 }
 
-// k damped-Jacobi sweeps of one coarse level in one launch (from zero or from zin).
+// Sweep counts k_smooth_k's residual variant is built for (one more ghost layer in LDS).
+template <class T>
+static bool smooth_k_res_supported(int k) {
+    return k >= 2 && k <= (sizeof(T) == 4 ? 4 : 2);
+}
+
+template <class T, int KK, int FZ, int RES>
+static void smooth_k_go(Launcher &K, int l, dim3 g, const LevelDesc &L, OpDesc<T> A, const T *id, const T *rhs,
+                        const T *zin, T *zout, T om, Gate Q, T *res) {
+    // per unknown, the arrays it reads once and writes once (dir and res with the residual)
+    const double bytes = (RES ? 9.0 : 7.0) * sizeof(T);
+    K.go(RES ? "smooth_k_res_Lc" : "smooth_k_Lc", l, bytes, 0, g, dim3(1024), k_smooth_k<T, KK, FZ, RES>, L, A, id,
+         rhs, zin, zout, om, Q, res);
+}
+
+// k damped-Jacobi sweeps of one coarse level in one launch (from zero or from zin); with res
+// (pre-smoothing from zero, k within smooth_k_res_supported), also the residual of the result.
 template <class T>
 static void smooth_k(Launcher &K, int l, int k, bool from_zero, const LevelDesc &L, OpDesc<T> A, const T *id,
-                     const T *rhs, const T *zin, T *zout, T om, Gate Q) {
-    const dim3 g(brick_grid(L.nbricks)), b(1024);
-    const double bytes = 7.0 * sizeof(T);  // per unknown, the arrays it reads once and writes once
+                     const T *rhs, const T *zin, T *zout, T om, Gate Q, T *res = nullptr) {
+    const dim3 g(brick_grid(L.nbricks));
+    if (res && from_zero) {
+        if (k == 2) smooth_k_go<T, 2, 1, 1>(K, l, g, L, A, id, rhs, zin, zout, om, Q, res);
+        if constexpr (sizeof(T) == 4) {
+            if (k == 3) smooth_k_go<T, 3, 1, 1>(K, l, g, L, A, id, rhs, zin, zout, om, Q, res);
+            if (k == 4) smooth_k_go<T, 4, 1, 1>(K, l, g, L, A, id, rhs, zin, zout, om, Q, res);
+        }
+        return;
+    }
     switch (k) {
     case 2:
-        if (from_zero) K.go("smooth_k_Lc", l, bytes, 0, g, b, k_smooth_k<T, 2, 1>, L, A, id, rhs, zin, zout, om, Q);
-        else K.go("smooth_k_Lc", l, bytes, 0, g, b, k_smooth_k<T, 2, 0>, L, A, id, rhs, zin, zout, om, Q);
+        if (from_zero) smooth_k_go<T, 2, 1, 0>(K, l, g, L, A, id, rhs, zin, zout, om, Q, nullptr);
+        else smooth_k_go<T, 2, 0, 0>(K, l, g, L, A, id, rhs, zin, zout, om, Q, nullptr);
         break;
     case 3:
-        if (from_zero) K.go("smooth_k_Lc", l, bytes, 0, g, b, k_smooth_k<T, 3, 1>, L, A, id, rhs, zin, zout, om, Q);
-        else K.go("smooth_k_Lc", l, bytes, 0, g, b, k_smooth_k<T, 3, 0>, L, A, id, rhs, zin, zout, om, Q);
+        if (from_zero) smooth_k_go<T, 3, 1, 0>(K, l, g, L, A, id, rhs, zin, zout, om, Q, nullptr);
+        else smooth_k_go<T, 3, 0, 0>(K, l, g, L, A, id, rhs, zin, zout, om, Q, nullptr);
         break;
     default:
         if constexpr (sizeof(T) == 4) {
             if (k == 4) {
-                if (from_zero)
-                    K.go("smooth_k_Lc", l, bytes, 0, g, b, k_smooth_k<T, 4, 1>, L, A, id, rhs, zin, zout, om, Q);
-                else
-                    K.go("smooth_k_Lc", l, bytes, 0, g, b, k_smooth_k<T, 4, 0>, L, A, id, rhs, zin, zout, om, Q);
+                if (from_zero) smooth_k_go<T, 4, 1, 0>(K, l, g, L, A, id, rhs, zin, zout, om, Q, nullptr);
+                else smooth_k_go<T, 4, 0, 0>(K, l, g, L, A, id, rhs, zin, zout, om, Q, nullptr);
             } else if (k == 5) {
-                if (from_zero)
-                    K.go("smooth_k_Lc", l, bytes, 0, g, b, k_smooth_k<T, 5, 1>, L, A, id, rhs, zin, zout, om, Q);
-                else
-                    K.go("smooth_k_Lc", l, bytes, 0, g, b, k_smooth_k<T, 5, 0>, L, A, id, rhs, zin, zout, om, Q);
+                if (from_zero) smooth_k_go<T, 5, 1, 0>(K, l, g, L, A, id, rhs, zin, zout, om, Q, nullptr);
+                else smooth_k_go<T, 5, 0, 0>(K, l, g, L, A, id, rhs, zin, zout, om, Q, nullptr);
             }
         }
         break;
@@ -611,8 +630,11 @@ static T *vcycle(shkz_ctx *C, Launcher &K, Problem &P, const T *rhs0, T *z0, T *
             break;
         }
         T *cur = z, *oth = tmp;
+        bool res_done = false;
         if (mpre[l]) {
-            smooth_k<T>(K, l, pre, true, L.desc, A, id, rhs, (const T *)nullptr, z, om, Q);
+            res_done = smooth_k_res_supported<T>(pre);
+            smooth_k<T>(K, l, pre, true, L.desc, A, id, rhs, (const T *)nullptr, z, om, Q,
+                        res_done ? (T *)L.res : nullptr);
         } else if (pre <= 1) {
             K.go(f ? "smooth_first_L0" : "smooth_first_Lc", l, 3 * es, 0, g, b,
                  f ? k_smooth_first<T, 1> : k_smooth_first<T, 0>, L.desc, id, rhs, z, om, Q);
@@ -625,8 +647,9 @@ static T *vcycle(shkz_ctx *C, Launcher &K, Problem &P, const T *rhs0, T *z0, T *
                 std::swap(cur, oth);
             }
         }
-        K.go(f ? "residual_L0" : "residual_Lc", l, 7 * es, -usave(C, l, 4 * es), g, b, f ? k_residual<T, 1> : k_residual<T, 0>, L.desc, A,
-             rhs, (const T *)cur, (T *)L.res, Q);
+        if (!res_done)
+            K.go(f ? "residual_L0" : "residual_Lc", l, 7 * es, -usave(C, l, 4 * es), g, b,
+                 f ? k_residual<T, 1> : k_residual<T, 0>, L.desc, A, rhs, (const T *)cur, (T *)L.res, Q);
         LevelHost &Lc = C->levels[l + 1];
         K.go(f ? "restrict_L0" : "restrict_Lc", l, es, 2 * es * (double)Lc.unknowns, dim3(brick_grid(Lc.nbricks)), b,
              f ? k_restrict<T, 1> : k_restrict<T, 0>, Lc.desc, op_of<T>(Lc), (const T *)L.res, (T *)Lc.rhs, Q);
