@@ -115,6 +115,38 @@ __device__ __forceinline__ float interp_arr(const float *U, const uint8_t *A, co
     return val;
 }
 
+// The 8 corner values of c (inactive entries 0 when A is given), and the interpolation and the
+// bounds computed from them: the value pass and the min/max pass share their stencil unless the
+// two departure points (float and double arithmetic) fall in different cells.
+__device__ __forceinline__ void load8(const float *U, const uint8_t *A, const int s[3], const Corner &c, float v[8]) {
+#pragma unroll
+    for (int n = 0; n < 8; ++n) {
+        const size_t q = corner_index(c, n, s);
+        v[n] = A ? face_val(U, A, q) : U[q];
+    }
+}
+__device__ __forceinline__ float interp8(const Corner &c, const float v[8]) {
+#pragma clang fp contract(off)
+    double w[8];
+    interp_weights(c, w);
+    float val = 0.0f;
+#pragma unroll
+    for (int n = 0; n < 8; ++n)
+        if (w[n] != 0.0) val = acc<float>(val, (double)v[n] * w[n]);
+    return val;
+}
+__device__ __forceinline__ void bounds8(const float v[8], double &lo, double &hi) {
+#pragma unroll
+    for (int n = 0; n < 8; ++n) {
+        const double x = (double)v[n];
+        lo = x < lo ? x : lo;  // std::min
+        hi = hi < x ? x : hi;  // std::max
+    }
+}
+__device__ __forceinline__ bool same_cell(const Corner &a, const Corner &b) {
+    return a.i == b.i && a.j == b.j && a.k == b.k;
+}
+
 // min and max of the 8 corner values at p (interpolate_coef's stencil), from the given initial
 // bounds (the reference's numeric_limits initialisers)
 __device__ __forceinline__ void corner_bounds(const float *U, const uint8_t *A, const int s[3], const double p[3],
@@ -167,6 +199,8 @@ __global__ void k_adv_face_fwd(VelField F, const float *in0, const float *in1, c
     vf_shape(F, a, s);
     const int pc[3] = {i, j, k};
     float value;
+    float v8[8];
+    Corner cv;
     if (!empty) {
         double p[3];
 #pragma unroll
@@ -175,7 +209,9 @@ __global__ void k_adv_face_fwd(VelField F, const float *in0, const float *in1, c
             const float sft = (float)((double)t / F.dx);  // /dx on a float vector
             p[b] = (double)pc[b] - (double)sft;
         }
-        value = interp_arr(in[a], F.act[a], s, p);
+        cv = corner_of(s, p);
+        load8(in[a], F.act[a], s, cv, v8);
+        value = interp8(cv, v8);
     } else {
         value = face_val(in[a], F.act[a], f);
     }
@@ -192,7 +228,9 @@ __global__ void k_adv_face_fwd(VelField F, const float *in0, const float *in1, c
             fp[b] = (pc[b] + 0.5 * (a != b)) - sft;
         }
         double dlo = DBL_MAX, dhi = DBL_MIN;
-        corner_bounds(in[a], F.act[a], s, p, dlo, dhi);
+        const Corner cm = corner_of(s, p);
+        if (!same_cell(cm, cv)) load8(in[a], F.act[a], s, cm, v8);
+        bounds8(v8, dlo, dhi);
         lo = (float)dlo;
         hi = (float)dhi;
     } else {
@@ -276,11 +314,18 @@ __global__ void k_adv_cell_fwd(VelField F, const float *q, const uint8_t *aq, Ad
     double p[3];
 #pragma unroll
     for (int b = 0; b < 3; ++b) p[b] = (double)pc[b] - (dt * u[b]) / F.dx;
-    S.v0[0][c] = empty ? q[c] : interp_arr(q, nullptr, s, p);
+    float v8[8];
+    const Corner cv = corner_of(s, p);
+    if (!empty) {
+        load8(q, nullptr, s, cv, v8);
+        S.v0[0][c] = interp8(cv, v8);
+    } else {
+        S.v0[0][c] = q[c];
+    }
     if (!minmax) return;
     if (!empty) {
         double lo = (double)FLT_MAX, hi = (double)FLT_MIN;
-        corner_bounds(q, nullptr, s, p, lo, hi);
+        bounds8(v8, lo, hi);
         S.lo[0][c] = (float)lo;
         S.hi[0][c] = (float)hi;
         S.nb[0][c] = interp_level(fluid, fluid64, s, p) > -F.dx * (double)trim ? 1 : 0;
