@@ -2,8 +2,8 @@
 // for macpressuresolver3 (src/projection/macpressuresolver3.cpp).  Select it with
 //     Projection=macpressuresolver3_mi355x
 // It implements macproject3_interface (include/shiokaze/projection/macproject3_interface.h:38-102)
-// on top of the C ABI in include/shkz_mi355x.h.  Built inside a Shiokaze tree;
-// it reads the same configuration keys as macpressuresolver3 (:274-280), the pcg solver
+// on top of the C ABI in include/shkz_mi355x.h.  It is built inside a Shiokaze tree, next to the
+// other projection modules, linked against libshkz_mi355x.so; it reads the same configuration keys as macpressuresolver3 (:274-280), the pcg solver
 // (src/linsolver/pcg.cpp:39-44) and macutility3 (src/utility/macutility3.cpp:408-412).
 #include <shiokaze/projection/macproject3_interface.h>
 #include <shiokaze/array/array3.h>
