@@ -77,31 +77,6 @@ __device__ __forceinline__ T offdiag_sum(const OpDesc<T> &A, const T *__restrict
            gyp * ld(v, idx[NB_YP]) + A.gz[c] * ld(v, idx[NB_ZM]) + gzp * ld(v, idx[NB_ZP]);
 }
 
-// The six neighbour values of local cell t: from the block's LDS copy `s` of the brick's 512
-// values inside the brick, from far(idx) (a global read) across its faces.  Staging the brick in
-// LDS turns five of the six neighbour reads of an interior cell into LDS reads.
-template <class T, class Far>
-__device__ __forceinline__ void nb_values(const T *s, int t, const long idx[6], Far far, T v[6]) {
-    const int li = t >> 6, lj = (t >> 3) & 7, lk = t & 7;
-    v[NB_XM] = li > 0 ? s[t - 64] : far(idx[NB_XM]);
-    v[NB_XP] = li < 7 ? s[t + 64] : far(idx[NB_XP]);
-    v[NB_YM] = lj > 0 ? s[t - 8] : far(idx[NB_YM]);
-    v[NB_YP] = lj < 7 ? s[t + 8] : far(idx[NB_YP]);
-    v[NB_ZM] = lk > 0 ? s[t - 1] : far(idx[NB_ZM]);
-    v[NB_ZP] = lk < 7 ? s[t + 1] : far(idx[NB_ZP]);
-}
-
-// The high-face couplings of local cell t (the low-face couplings of its +x/+y/+z neighbours),
-// from the LDS copies inside the brick.
-template <class T>
-__device__ __forceinline__ void high_couplings(const OpDesc<T> &A, const T *gxs, const T *gys, const T *gzs, int t,
-                                               const long idx[6], T &gxp, T &gyp, T &gzp) {
-    const int li = t >> 6, lj = (t >> 3) & 7, lk = t & 7;
-    gxp = li < 7 ? gxs[t + 64] : ld(A.gx, idx[NB_XP]);
-    gyp = lj < 7 ? gys[t + 8] : ld(A.gy, idx[NB_YP]);
-    gzp = lk < 7 ? gzs[t + 1] : ld(A.gz, idx[NB_ZP]);
-}
-
 // apply_row and offdiag_sum on a BM_UNIFORM brick: every coupling is g and the Dirichlet sum 0
 // (the same operations in the same order, so the same values).
 template <class T>
@@ -777,41 +752,38 @@ template <class T, int FINEST>
 __global__ __launch_bounds__(BLOCK) void k_smooth2(LevelDesc L, OpDesc<T> A, const T *__restrict__ idiag,
                                                    const T *__restrict__ rhs, T *__restrict__ z, T omega,
                                                    Gate Q) {
-    __shared__ T z1s[BRICK_CELLS], gxs[BRICK_CELLS], gys[BRICK_CELLS], gzs[BRICK_CELLS];
     const int br = brick_or_idle(L);
     if (br < 0) return;
     const BrickMeta m = L.meta[br];
     if (all_done(Q) || skip_smooth(Q, L, m)) return;
     const int t = threadIdx.x;
     const long c = (long)br * BRICK_CELLS + t;
-    const bool uni = m.flags & BM_UNIFORM;
-    UniT<T> u{};
-    if (uni) u = uni_load<T>(L, br);
-    // the first sweep from zero at every cell of the brick, omega * idiag * rhs, staged in LDS
-    const T r = rhs[c];
-    const T id = uni ? u.id : idiag[c];
-    z1s[t] = omega * id * r;
-    if (!uni) {
-        gxs[t] = A.gx[c];
-        gys[t] = A.gy[c];
-        gzs[t] = A.gz[c];
+    if (m.flags & BM_UNIFORM) {
+        const UniT<T> u = uni_load<T>(L, br);
+        const Nbrs N = nbrs_of(m, br);
+        long idx[6];
+        nb_index(N, t, idx);
+        // neighbours in the brick share its inverse diagonal; the others are read
+        auto z1 = [&](long i) {
+            return i >= 0 ? omega * ((i >= N.base && i < N.base + BRICK_CELLS) ? u.id : idiag[i]) * rhs[i] : T(0);
+        };
+        const T off = u.g * z1(idx[NB_XM]) + u.g * z1(idx[NB_XP]) + u.g * z1(idx[NB_YM]) + u.g * z1(idx[NB_YP]) +
+                      u.g * z1(idx[NB_ZM]) + u.g * z1(idx[NB_ZP]);
+        const T r = rhs[c];
+        z[c] = (T(1) - omega) * (omega * u.id * r) + omega * u.id * (r + off);
+        return;
     }
-    __syncthreads();
-    const Nbrs N = nbrs_of(m, br);
-    long idx[6];
-    nb_index(N, t, idx);
-    auto far = [&](long i) { return i >= 0 ? omega * idiag[i] * rhs[i] : T(0); };
-    T v[6];
-    nb_values(z1s, t, idx, far, v);
+    const T id = idiag[c];
     T out = 0;
-    if (uni) {
-        const T off = u.g * v[NB_XM] + u.g * v[NB_XP] + u.g * v[NB_YM] + u.g * v[NB_YP] + u.g * v[NB_ZM] + u.g * v[NB_ZP];
-        out = (T(1) - omega) * (omega * u.id * r) + omega * u.id * (r + off);
-    } else if (id > 0) {
-        T gxp, gyp, gzp;
-        high_couplings(A, gxs, gys, gzs, t, idx, gxp, gyp, gzp);
-        const T off = gxs[t] * v[NB_XM] + gxp * v[NB_XP] + gys[t] * v[NB_YM] + gyp * v[NB_YP] + gzs[t] * v[NB_ZM] +
-                      gzp * v[NB_ZP];
+    if (id > 0) {
+        const Nbrs N = nbrs_of(m, br);
+        long idx[6];
+        nb_index(N, t, idx);
+        auto z1 = [&](long i) { return i >= 0 ? omega * idiag[i] * rhs[i] : T(0); };
+        const T gxp = ld(A.gx, idx[NB_XP]), gyp = ld(A.gy, idx[NB_YP]), gzp = ld(A.gz, idx[NB_ZP]);
+        const T off = A.gx[c] * z1(idx[NB_XM]) + gxp * z1(idx[NB_XP]) + A.gy[c] * z1(idx[NB_YM]) +
+                      gyp * z1(idx[NB_YP]) + A.gz[c] * z1(idx[NB_ZM]) + gzp * z1(idx[NB_ZP]);
+        const T r = rhs[c];
         out = (T(1) - omega) * (omega * id * r) + omega * id * (r + off);
     }
     z[c] = out;
@@ -824,46 +796,29 @@ __global__ __launch_bounds__(BLOCK) void k_jacobi(LevelDesc L, OpDesc<T> A, cons
                                                   const T *__restrict__ rhs, const T *__restrict__ zin,
                                                   T *__restrict__ zout, T omega, Gate Q, RedBuf R) {
     __shared__ double lds[32];
-    __shared__ T zs[BRICK_CELLS], gxs[BRICK_CELLS], gys[BRICK_CELLS], gzs[BRICK_CELLS];
     const int br = brick_or_idle(L);
     if (br < 0) return;
     const BrickMeta m = L.meta[br];
     if (all_done(Q) || skip_smooth(Q, L, m)) return;
     const int t = threadIdx.x;
     const long c = (long)br * BRICK_CELLS + t;
-    const bool uni = m.flags & BM_UNIFORM;
-    const T zc = zin[c];
-    zs[t] = zc;
-    if (!uni) {
-        gxs[t] = A.gx[c];
-        gys[t] = A.gy[c];
-        gzs[t] = A.gz[c];
-    }
-    __syncthreads();
     T out = 0;
     double part = 0.0;
-    long idx[6];
-    nb_index(nbrs_of(m, br), t, idx);
-    auto far = [&](long i) { return ld(zin, i); };
-    if (uni) {
+    if (m.flags & BM_UNIFORM) {
         const UniT<T> u = uni_load<T>(L, br);
-        T v[6];
-        nb_values(zs, t, idx, far, v);
+        long idx[6];
+        nb_index(nbrs_of(m, br), t, idx);
         const T r = rhs[c];
-        const T off = u.g * v[NB_XM] + u.g * v[NB_XP] + u.g * v[NB_YM] + u.g * v[NB_YP] + u.g * v[NB_ZM] + u.g * v[NB_ZP];
-        out = (T(1) - omega) * zc + omega * u.id * (r + off);
+        out = (T(1) - omega) * zin[c] + omega * u.id * (r + offdiag_uniform(u.g, zin, idx));
         if (DOT) part = (double)r * (double)out;
     } else {
         const T id = idiag[c];
         if (id > 0) {
-            T v[6];
-            nb_values(zs, t, idx, far, v);
-            T gxp, gyp, gzp;
-            high_couplings(A, gxs, gys, gzs, t, idx, gxp, gyp, gzp);
+            const Nbrs N = nbrs_of(m, br);
+            long idx[6];
+            nb_index(N, t, idx);
             const T r = rhs[c];
-            const T off = gxs[t] * v[NB_XM] + gxp * v[NB_XP] + gys[t] * v[NB_YM] + gyp * v[NB_YP] +
-                          gzs[t] * v[NB_ZM] + gzp * v[NB_ZP];
-            out = (T(1) - omega) * zc + omega * id * (r + off);
+            out = (T(1) - omega) * zin[c] + omega * id * (r + offdiag_sum(A, zin, c, idx));
             if (DOT) part = (double)r * (double)out;
         }
     }
