@@ -791,15 +791,36 @@ __global__ __launch_bounds__(BLOCK) void k_smooth2(LevelDesc L, OpDesc<T> A, con
 
 // zout = (1 - omega) zin + omega idiag (rhs + offdiag(zin)).  DOT: also rhs.zout partials (the
 // finest level's last post-smoothing sweep yields r.z of the preconditioned CG for free).
+template <class T, int DOT>
+__device__ __forceinline__ void jacobi_brick(const LevelDesc &L, const OpDesc<T> &A, const T *__restrict__ idiag,
+                                             const T *__restrict__ rhs, const T *__restrict__ zin,
+                                             T *__restrict__ zout, T omega, const RedBuf &R, int br, const BrickMeta &m,
+                                             double *lds);
+
+// Persistent form: the grid holds at most PERSIST_BLOCKS workgroups, each looping over virtual
+// block ids vb = blockIdx.x + k * gridDim.x (gridDim.x a multiple of 8, so vb keeps its XCD).
+constexpr int PERSIST_BLOCKS = 1024;  // 4 resident 512-thread workgroups on each of 256 CUs
+
 template <class T, int FINEST, int DOT>
 __global__ __launch_bounds__(BLOCK) void k_jacobi(LevelDesc L, OpDesc<T> A, const T *__restrict__ idiag,
                                                   const T *__restrict__ rhs, const T *__restrict__ zin,
-                                                  T *__restrict__ zout, T omega, Gate Q, RedBuf R) {
+                                                  T *__restrict__ zout, T omega, Gate Q, RedBuf R, int nvirt) {
     __shared__ double lds[32];
-    const int br = brick_or_idle(L);
-    if (br < 0) return;
-    const BrickMeta m = L.meta[br];
-    if (all_done(Q) || skip_smooth(Q, L, m)) return;
+    if (all_done(Q)) return;
+    for (int vb = blockIdx.x; vb < nvirt; vb += gridDim.x) {
+        const int br = xcd_brick(vb, nvirt);
+        if (br >= L.nbricks) continue;
+        const BrickMeta m = L.meta[br];
+        if (skip_smooth(Q, L, m)) continue;
+        jacobi_brick<T, DOT>(L, A, idiag, rhs, zin, zout, omega, R, br, m, lds);
+    }
+}
+
+template <class T, int DOT>
+__device__ __forceinline__ void jacobi_brick(const LevelDesc &L, const OpDesc<T> &A, const T *__restrict__ idiag,
+                                             const T *__restrict__ rhs, const T *__restrict__ zin,
+                                             T *__restrict__ zout, T omega, const RedBuf &R, int br, const BrickMeta &m,
+                                             double *lds) {
     const int t = threadIdx.x;
     const long c = (long)br * BRICK_CELLS + t;
     T out = 0;

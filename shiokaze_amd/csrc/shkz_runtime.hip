@@ -611,6 +611,7 @@ static T *vcycle(shkz_ctx *C, Launcher &K, Problem &P, const T *rhs0, T *z0, T *
         const OpDesc<T> A = op_of<T>(L);
         const T *id = (const T *)L.idiag;
         const dim3 g(brick_grid(L.nbricks)), b(BLOCK);
+        const dim3 gp(std::min<unsigned>(g.x, PERSIST_BLOCKS));  // persistent kernels
         const bool f = l == 0;
         const int pre = f ? P.pre : P.pre_c, post = f ? P.post : P.post_c;
         const bool last = l == nl - 1;
@@ -642,8 +643,9 @@ static T *vcycle(shkz_ctx *C, Launcher &K, Problem &P, const T *rhs0, T *z0, T *
             K.go(f ? "smooth2_L0" : "smooth2_Lc", l, 6 * es, -usave(C, l, 4 * es), g, b, f ? k_smooth2<T, 1> : k_smooth2<T, 0>, L.desc, A,
                  id, rhs, z, om, Q);
             for (int s = 2; s < pre; ++s) {
-                K.go(f ? "jacobi_L0" : "jacobi_Lc", l, 7 * es, -usave(C, l, 4 * es), g, b,
-                     f ? k_jacobi<T, 1, 0> : k_jacobi<T, 0, 0>, L.desc, A, id, rhs, (const T *)cur, oth, om, Q, P.red);
+                K.go(f ? "jacobi_L0" : "jacobi_Lc", l, 7 * es, -usave(C, l, 4 * es), gp, b,
+                     f ? k_jacobi<T, 1, 0> : k_jacobi<T, 0, 0>, L.desc, A, id, rhs, (const T *)cur, oth, om, Q, P.red,
+                     (int)g.x);
                 std::swap(cur, oth);
             }
         }
@@ -663,6 +665,7 @@ static T *vcycle(shkz_ctx *C, Launcher &K, Problem &P, const T *rhs0, T *z0, T *
         const OpDesc<T> A = op_of<T>(L);
         const T *id = (const T *)L.idiag;
         const dim3 g(brick_grid(L.nbricks)), b(BLOCK);
+        const dim3 gp(std::min<unsigned>(g.x, PERSIST_BLOCKS));  // persistent kernels
         const bool f = l == 0;
         const int post = f ? P.post : P.post_c;
         K.go(f ? "prolong_L0" : "prolong_Lc", l, 3 * es, es * (double)Lc.unknowns - usave(C, l, es), g, b,
@@ -678,11 +681,12 @@ static T *vcycle(shkz_ctx *C, Launcher &K, Problem &P, const T *rhs0, T *z0, T *
         for (int s = 0; s < post; ++s) {
             const bool last_dot = f && dot_partials && s == post - 1;
             if (last_dot)
-                K.go("jacobi_dot_L0", l, 7 * es, -usave(C, l, 4 * es), g, b, k_jacobi<T, 1, 1>, L.desc, A, id, rhs,
-                     (const T *)cur, oth, om, Q, P.red);
+                K.go("jacobi_dot_L0", l, 7 * es, -usave(C, l, 4 * es), gp, b, k_jacobi<T, 1, 1>, L.desc, A, id, rhs,
+                     (const T *)cur, oth, om, Q, P.red, (int)g.x);
             else
-                K.go(f ? "jacobi_L0" : "jacobi_Lc", l, 7 * es, -usave(C, l, 4 * es), g, b,
-                     f ? k_jacobi<T, 1, 0> : k_jacobi<T, 0, 0>, L.desc, A, id, rhs, (const T *)cur, oth, om, Q, P.red);
+                K.go(f ? "jacobi_L0" : "jacobi_Lc", l, 7 * es, -usave(C, l, 4 * es), gp, b,
+                     f ? k_jacobi<T, 1, 0> : k_jacobi<T, 0, 0>, L.desc, A, id, rhs, (const T *)cur, oth, om, Q, P.red,
+                     (int)g.x);
             std::swap(cur, oth);
         }
         zcur[l] = cur;
