@@ -297,7 +297,7 @@ __global__ void k_adv_face_copy(VelField F, AdvScratch S) {
 __global__ void k_adv_cell_fwd(VelField F, const float *q, const uint8_t *aq, AdvScratch S, const void *fluid,
                                int fluid64, double dt, int trim, float background, int minmax) {
 #pragma clang fp contract(off)
-    const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t c = (size_t)xcd_brick(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     const size_t M = (size_t)F.w * F.h * F.d;
     if (c >= M) return;
     if (aq && !aq[c]) {
@@ -339,7 +339,7 @@ __global__ void k_adv_cell_fwd(VelField F, const float *q, const uint8_t *aq, Ad
 // Backward pass from S.v0[0] fused with the final MacCormack step, written over q.
 __global__ void k_adv_cell_bwd(VelField F, float *q, const uint8_t *aq, AdvScratch S, double dt_back) {
 #pragma clang fp contract(off)
-    const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t c = (size_t)xcd_brick(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     const size_t M = (size_t)F.w * F.h * F.d;
     if (c >= M) return;
     if (aq && !aq[c]) return;

@@ -26,6 +26,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "shkz_device.hpp"
+
 namespace shkz {
 
 struct VelField {
@@ -51,13 +53,13 @@ __device__ __forceinline__ void vf_shape(const VelField &F, int a, int s[3]) {
 // as if it were uniform: that wavefront's lanes read the first lane's array extent.)
 inline dim3 vf_grid(const size_t n[3]) {
     const size_t m = n[0] > n[1] ? (n[0] > n[2] ? n[0] : n[2]) : (n[1] > n[2] ? n[1] : n[2]);
-    return dim3((unsigned)((m + 255) / 256), 3);
+    return dim3((unsigned)(((m + 255) / 256 + 7) / 8 * 8), 3);
 }
 
 // (array, local index, i, j, k) of this thread; false past the end of its array
 __device__ __forceinline__ bool vf_decode(const VelField &F, int &a, size_t &f, int &i, int &j, int &k) {
     a = (int)blockIdx.y;
-    f = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    f = (size_t)xcd_brick(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;  // XCD-contiguous slabs
     if (f >= F.n[a]) return false;
     int s[3];
     vf_shape(F, a, s);

@@ -141,7 +141,7 @@ __global__ void k_any_negative(const void *a, int is64, size_t n, int *flag) {
 // float level sets are summed in float as the reference sums Real values.
 __global__ void k_curvature(GridDesc G, float *curv) {
     const size_t M = (size_t)G.w * G.h * G.d;
-    const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t c = (size_t)xcd_brick(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (c >= M) return;
     int i, j, k;
     unflatten((unsigned)c, (unsigned)G.h, (unsigned)G.d, i, j, k);
@@ -178,7 +178,7 @@ __global__ void k_curvature(GridDesc G, float *curv) {
 __global__ void k_surface_tension(GridDesc G, int dim, const float *curv, double kappa) {
     const int W = G.w + (dim == 0), H = G.h + (dim == 1), D = G.d + (dim == 2);
     const size_t n = (size_t)W * H * D;
-    const size_t f = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t f = (size_t)xcd_brick(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (f >= n) return;
     if (G.active[dim] && !G.active[dim][f]) return;
     int i, j, k;
@@ -200,7 +200,8 @@ __global__ void k_surface_tension(GridDesc G, int dim, const float *curv, double
 // cells).  One block per brick position of the dense grid; writes cell flags
 // (bit0 liquid, bit1 unknown) and whether the brick holds an unknown.
 __global__ __launch_bounds__(BLOCK) void k_flags(GridDesc G, int nbx, int nby, int nbz, uint8_t *cflag, uint8_t *bflag) {
-    const int bid = blockIdx.x;
+    const int bid = xcd_brick(blockIdx.x, gridDim.x);  // each XCD takes a contiguous slab
+    if (bid >= nbx * nby * nbz) return;
     const int bz = bid % nbz, by = (bid / nbz) % nby, bx = bid / (nbz * nby);
     const int t = threadIdx.x;
     const int i = bx * BRICK + (t >> 6), j = by * BRICK + ((t >> 3) & 7), k = bz * BRICK + (t & 7);
@@ -1142,7 +1143,7 @@ __global__ __launch_bounds__(BLOCK) void k_coarse_solve(LevelDesc L, OpDesc<T> A
 __global__ void k_scatter_pressure(GridDesc G, LevelDesc L, GridLevel GL, const double *__restrict__ x, float *P,
                                    double *P64) {
     const size_t M = (size_t)G.w * G.h * G.d;
-    const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t c = (size_t)xcd_brick(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (c >= M) return;
     int i, j, k;
     unflatten((unsigned)c, (unsigned)G.h, (unsigned)G.d, i, j, k);
@@ -1215,7 +1216,7 @@ __device__ __forceinline__ void update_face(const GridDesc &G, int dim, int i, i
 __global__ void k_velocity_update(GridDesc G, const float *__restrict__ P) {
     const int W = G.w + 1, H = G.h + 1, D = G.d + 1;
     const size_t n = (size_t)W * H * D;
-    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t e = (size_t)xcd_brick(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (e >= n) return;
     int i, j, k;
     unflatten((unsigned)e, (unsigned)H, (unsigned)D, i, j, k);

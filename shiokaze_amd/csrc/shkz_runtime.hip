@@ -466,6 +466,9 @@ struct Launcher {
 };
 
 static inline unsigned brick_grid(int nbricks) { return (unsigned)(((nbricks + 7) / 8) * 8); }
+// Blocks of `per` threads over n items, a multiple of 8 (kernels remap block ids XCD-aware and
+// bounds-check the index).
+static inline unsigned xcd_blocks(size_t n, unsigned per) { return (unsigned)((((n + per - 1) / per) + 7) / 8 * 8); }
 
 // Resolves pending timing events into totals (after a stream synchronisation).
 static void resolve_timing(shkz_ctx *C) {
@@ -681,7 +684,8 @@ static T *vcycle(shkz_ctx *C, Launcher &K, Problem &P, const T *rhs0, T *z0, T *
         for (int s = 0; s < post; ++s) {
             const bool last_dot = f && dot_partials && s == post - 1;
             if (last_dot)
-                K.go("jacobi_dot_L0", l, 7 * es, -usave(C, l, 4 * es), gp, b, k_jacobi<T, 1, 1>, L.desc, A, id, rhs,
+                // one brick per workgroup: the reduction of the dot variant costs two barriers per brick
+                K.go("jacobi_dot_L0", l, 7 * es, -usave(C, l, 4 * es), g, b, k_jacobi<T, 1, 1>, L.desc, A, id, rhs,
                      (const T *)cur, oth, om, Q, P.red, (int)g.x);
             else
                 K.go(f ? "jacobi_L0" : "jacobi_Lc", l, 7 * es, -usave(C, l, 4 * es), gp, b,
@@ -1320,11 +1324,11 @@ static int project_stages(shkz_ctx *C, const shkz_grid *grids, int ngrids, const
             for (int q = 0; q < ngrids; ++q) maxM = std::max(maxM, (size_t)grids[q].w * grids[q].h * grids[q].d);
             float *curv = nullptr;
             SHKZ_TRY(arena_get(AR, "curvature", maxM, &curv));
-            K.go("curvature", -1, 0, (double)J.M * ((G.phi64 ? 8 : 4) + 4), dim3((unsigned)((J.M + 255) / 256)),
-                 dim3(256), k_curvature, G, curv);
+            K.go("curvature", -1, 0, (double)J.M * ((G.phi64 ? 8 : 4) + 4), dim3(xcd_blocks(J.M, 256)), dim3(256),
+                 k_curvature, G, curv);
             for (int dim = 0; dim < (flat ? 2 : 3); ++dim) {
                 const size_t nf = (size_t)(w + (dim == 0)) * (h + (dim == 1)) * (d + (dim == 2));
-                K.go("surface_tension", -1, 0, (double)nf * 8, dim3((unsigned)((nf + 255) / 256)), dim3(256),
+                K.go("surface_tension", -1, 0, (double)nf * 8, dim3(xcd_blocks(nf, 256)), dim3(256),
                      k_surface_tension, G, dim, (const float *)curv, prm->surface_tension);
             }
         }
@@ -1332,7 +1336,7 @@ static int project_stages(shkz_ctx *C, const shkz_grid *grids, int ngrids, const
         const int npos = J.l0.nbx * J.l0.nby * J.l0.nbz;
         J.cflag = cflag_all + cflag_off[g];
         J.bflag = bflag_all + bflag_off[g];
-        K.go("flags", -1, 0, (double)J.M * ((G.phi64 ? 8 : 4) + 1 + (grid->solid ? 4 : 0)), dim3((unsigned)npos),
+        K.go("flags", -1, 0, (double)J.M * ((G.phi64 ? 8 : 4) + 1 + (grid->solid ? 4 : 0)), dim3(brick_grid(npos)),
              dim3(BLOCK), k_flags, G, J.l0.nbx, J.l0.nby, J.l0.nbz, J.cflag, J.bflag);
     }
     {
@@ -1694,15 +1698,15 @@ static int project_stages(shkz_ctx *C, const shkz_grid *grids, int ngrids, const
         const GridLevel &GL = F0.gl[g];
         // the velocity update reads a dense float pressure: the caller's, or the scratch array
         float *P32 = grid->pressure ? grid->pressure : pdense;
-        K.go("scatter_pressure", -1, 0, (double)J.M * (4 + (grid->pressure64 ? 8 : 0)),
-             dim3((unsigned)((J.M + 255) / 256)), dim3(256), k_scatter_pressure, J.G, F0.desc, GL,
+        K.go("scatter_pressure", -1, 0, (double)J.M * (4 + (grid->pressure64 ? 8 : 0)), dim3(xcd_blocks(J.M, 256)),
+             dim3(256), k_scatter_pressure, J.G, F0.desc, GL,
              (const double *)P.x64, P32, grid->pressure64);
         if (J.scene)
-            K.go("scatter_pressure", -1, 0, (double)J.M * 8, dim3((unsigned)((J.M + 255) / 256)), dim3(256),
+            K.go("scatter_pressure", -1, 0, (double)J.M * 8, dim3(xcd_blocks(J.M, 256)), dim3(256),
                  k_scatter_pressure, J.G, F0.desc, GL, (const double *)P.x64, (float *)nullptr, J.scene->p);
         const size_t ne = (size_t)(grid->w + 1) * (grid->h + 1) * (grid->d + 1);
         K.go("velocity_update", -1, 0, (double)J.M * (24 + 4 + (J.G.phi64 ? 8 : 4) + (grid->solid ? 4 : 0)),
-             dim3((unsigned)((ne + 255) / 256)), dim3(256), k_velocity_update, J.G, (const float *)P32);
+             dim3(xcd_blocks(ne, 256)), dim3(256), k_velocity_update, J.G, (const float *)P32);
         if (J.scene) J.scene->has_pressure = true;
         res[g].iterations = total_iters[g];
         res[g].outer_iterations = outer_passes[g];
@@ -2111,7 +2115,7 @@ static int advect_scalar_stages(shkz_ctx *C, const shkz_velocity_field *f, float
     Launcher K{C, s};
     VelField F = vel_field_of(f, vel, act);
     const size_t M = (size_t)f->w * f->h * f->d;
-    const dim3 grid((unsigned)((M + 255) / 256)), block(256);
+    const dim3 grid(xcd_blocks(M, 256)), block(256);
     AdvScratch S = {};
     const bool mc = p->maccormack != 0;
     SHKZ_TRY(arena_get(AR, "adv_q0", M, &S.v0[0]));
