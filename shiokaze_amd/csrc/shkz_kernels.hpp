@@ -744,7 +744,8 @@ __global__ __launch_bounds__(BLOCK) void k_smooth_first(LevelDesc L, const T *__
     const BrickMeta m = L.meta[br];
     if (all_done(Q) || skip_smooth(Q, L, m)) return;
     const long c = (long)br * BRICK_CELLS + threadIdx.x;
-    z[c] = omega * idiag[c] * rhs[c];
+    const T id = (m.flags & BM_UNIFORM) ? uni_load<T>(L, br).id : idiag[c];
+    z[c] = omega * id * rhs[c];
 }
 
 // Two damped-Jacobi sweeps from a zero guess in one pass: the first sweep's values at the
@@ -807,6 +808,14 @@ __global__ __launch_bounds__(BLOCK) void k_jacobi(LevelDesc L, OpDesc<T> A, cons
                                                   const T *__restrict__ rhs, const T *__restrict__ zin,
                                                   T *__restrict__ zout, T omega, Gate Q, RedBuf R, int nvirt) {
     __shared__ double lds[32];
+    if (DOT) {  // one brick per workgroup (its reduction costs two barriers per brick)
+        const int br = brick_or_idle(L);
+        if (br < 0) return;
+        const BrickMeta m = L.meta[br];
+        if (all_done(Q) || skip_smooth(Q, L, m)) return;
+        jacobi_brick<T, DOT>(L, A, idiag, rhs, zin, zout, omega, R, br, m, lds);
+        return;
+    }
     if (all_done(Q)) return;
     for (int vb = blockIdx.x; vb < nvirt; vb += gridDim.x) {
         const int br = xcd_brick(vb, nvirt);

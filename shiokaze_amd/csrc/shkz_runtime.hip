@@ -524,8 +524,10 @@ static OpDesc<T> op_of(const LevelHost &L) {
 // post-smoothing, given the ping-pong of the sweeps: 0 = z, 1 = tmp.
 // multi_pre / multi_post: the level smooths with one k_smooth_k launch (which writes the other
 // vector once) instead of per-sweep launches.
-static int final_slot(int pre, int post, bool multi_pre, bool multi_post) {
-    const int after_pre = (multi_pre || pre <= 1) ? 0 : ((pre - 2) & 1);
+static int final_slot(int pre, int post, bool multi_pre, bool multi_post, bool finest) {
+    // pre-smoothing ends after pre - 1 swaps on the finest level (one sweep from zero, then plain
+    // sweeps), after pre - 2 elsewhere (the fused two-sweep kernel, then plain sweeps)
+    const int after_pre = (multi_pre || pre <= 1) ? 0 : (finest ? ((pre - 1) & 1) : ((pre - 2) & 1));
     const int post_swaps = multi_post ? (post > 0 ? 1 : 0) : post;
     return after_pre ^ (post_swaps & 1);
 }
@@ -620,7 +622,7 @@ static T *vcycle(shkz_ctx *C, Launcher &K, Problem &P, const T *rhs0, T *z0, T *
         const bool last = l == nl - 1;
         if (L.any_coarsest) {
             // grids ending here: their z goes where the other grids' smoothed z ends up
-            T *dst = (last || final_slot(pre, post, mpre[l], mpost[l]) == 0) ? z : tmp;
+            T *dst = (last || final_slot(pre, post, mpre[l], mpost[l], f) == 0) ? z : tmp;
             if (f && dot_partials)
                 K.go("coarse_solve_dot", l, 6 * es, 0, dim3((unsigned)P.ngrids), b, k_coarse_solve<T, 1>, L.desc, A,
                      rhs, dst, P.coarse_sweeps, om, Q, P.red);
@@ -639,9 +641,17 @@ static T *vcycle(shkz_ctx *C, Launcher &K, Problem &P, const T *rhs0, T *z0, T *
             res_done = smooth_k_res_supported<T>(pre);
             smooth_k<T>(K, l, pre, true, L.desc, A, id, rhs, (const T *)nullptr, z, om, Q,
                         res_done ? (T *)L.res : nullptr);
-        } else if (pre <= 1) {
-            K.go(f ? "smooth_first_L0" : "smooth_first_Lc", l, 3 * es, 0, g, b,
+        } else if (pre <= 1 || f) {
+            // finest level: one sweep from zero, then plain sweeps (cheaper there than the fused
+            // two-sweep kernel, whose on-the-fly neighbour values cost 12 extra loads per cell)
+            K.go(f ? "smooth_first_L0" : "smooth_first_Lc", l, 3 * es, -usave(C, l, es), g, b,
                  f ? k_smooth_first<T, 1> : k_smooth_first<T, 0>, L.desc, id, rhs, z, om, Q);
+            for (int s = 1; s < pre; ++s) {
+                K.go(f ? "jacobi_L0" : "jacobi_Lc", l, 7 * es, -usave(C, l, 4 * es), gp, b,
+                     f ? k_jacobi<T, 1, 0> : k_jacobi<T, 0, 0>, L.desc, A, id, rhs, (const T *)cur, oth, om, Q, P.red,
+                     (int)g.x);
+                std::swap(cur, oth);
+            }
         } else {
             K.go(f ? "smooth2_L0" : "smooth2_Lc", l, 6 * es, -usave(C, l, 4 * es), g, b, f ? k_smooth2<T, 1> : k_smooth2<T, 0>, L.desc, A,
                  id, rhs, z, om, Q);
