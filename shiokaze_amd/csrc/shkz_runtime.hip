@@ -21,6 +21,8 @@
 #include "shkz_serve.hpp"
 #include "shkz_error.hpp"
 
+#include <hip/hip_ext.h>
+
 #include <algorithm>
 #include <atomic>
 #include <chrono>
@@ -444,24 +446,26 @@ struct Launcher {
         if (hipEventCreate(&e) != hipSuccess) return nullptr;
         return e;
     }
-    // Launches kernel<<<grid, block>>> on the work stream; with kernel timing on (and outside a
-    // graph capture) brackets it with events.  The byte model (per unknown of `level`, plus fixed)
-    // feeds the per-kernel achieved-bandwidth statistics.
+    // Launches kernel<<<grid, block>>> on the work stream.  With kernel timing on (and outside a
+    // graph capture) the launch carries a start and a stop event that the dispatch itself stamps
+    // (hipExtLaunchKernelGGL), so the elapsed time is the kernel's own execution, as a kernel trace
+    // reports it, without the cost of separate event packets.  The byte model (per unknown of
+    // `level`, plus fixed) feeds the per-kernel achieved-bandwidth statistics.
     template <class K, class... Args>
     void go(const char *name, int level, double bytes_per_unknown, double bytes_fixed, dim3 grid, dim3 block, K kernel,
             Args... args) {
         if (grid.x == 0) return;
-        hipEvent_t e0 = nullptr, e1 = nullptr;
         if (C->timing && !capturing) {
-            e0 = event();
-            e1 = event();
-            if (e0) (void)hipEventRecord(e0, s);
+            hipEvent_t e0 = event(), e1 = event();
+            if (e0 && e1) {
+                hipExtLaunchKernelGGL(kernel, grid, block, 0, s, e0, e1, 0, args...);
+                C->pending.push_back({kernel_id(name), level, bytes_per_unknown, bytes_fixed, e0, e1});
+                return;
+            }
+            if (e0) C->event_pool.push_back(e0);
+            if (e1) C->event_pool.push_back(e1);
         }
         kernel<<<grid, block, 0, s>>>(args...);
-        if (C->timing && !capturing && e0 && e1) {
-            (void)hipEventRecord(e1, s);
-            C->pending.push_back({kernel_id(name), level, bytes_per_unknown, bytes_fixed, e0, e1});
-        }
     }
 };
 
