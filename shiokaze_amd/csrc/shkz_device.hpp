@@ -90,6 +90,7 @@ struct LevelDesc {
     const int *child64;      // brick -> the 4x4x4 finer-level bricks 2B-1 .. 2B+2 per axis (-1 = none)
     const int *parent8;      // brick -> the 2x2x2 coarser-level bricks (B+1)/2 - 1 + {0,1} per axis
     const UniCoef *uni;      // level 0: constants of the BM_UNIFORM bricks (nullptr elsewhere)
+    const int *nb27;         // coarse levels: brick -> the 3x3x3 bricks B-1 .. B+1 per axis (-1 = none)
 };
 
 template <class T>

@@ -901,23 +901,38 @@ __global__ __launch_bounds__(BLOCK) void k_restrict(LevelDesc Lc, OpDesc<T> Ac, 
     __shared__ int bid[64];
     const int br = brick_or_idle(Lc);
     if (br < 0) return;
+    const int t = threadIdx.x;
+    const long c = (long)br * BRICK_CELLS + t;
+    // loads that depend on the brick only, issued before the gate's loads resolve
+    const T dirc = Ac.dir[c];
+    const int cid = t < 64 ? Lc.child64[64 * br + t] : -1;
     const BrickMeta m = Lc.meta[br];
     if (all_done(Q) || grid_done(Q, Lc, m)) return;
-    const int t = threadIdx.x;
-    if (t < 64) bid[t] = Lc.child64[64 * br + t];
+    if (t < 64) bid[t] = cid;
     __syncthreads();
-    for (int e = t; e < E * E * E; e += BLOCK) {
-        const int c = e % E, b = (e / E) % E, a = e / (E * E);
-        // tile offset a is fine cell 16B - 1 + a: fine brick 2B - 1 + ((a + 7) >> 3), local (a + 7) & 7
-        const int slot = (((a + 7) >> 3) * 4 + ((b + 7) >> 3)) * 4 + ((c + 7) >> 3);
-        const int id = bid[slot];
-        tile[e] = id >= 0 ? res[(long)id * BRICK_CELLS + ((((a + 7) & 7) * 8 + ((b + 7) & 7)) * 8 + ((c + 7) & 7))]
-                          : T(0);
+    // all NQ loads of a thread are issued before the first LDS store (one memory latency, not NQ)
+    constexpr int NQ = (E * E * E + BLOCK - 1) / BLOCK;
+    T v[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        const int e = t + q * BLOCK;
+        v[q] = T(0);
+        if (e < E * E * E) {
+            const int c3 = e % E, b = (e / E) % E, a = e / (E * E);
+            // tile offset a is fine cell 16B - 1 + a: fine brick 2B - 1 + ((a + 7) >> 3), local (a + 7) & 7
+            const int slot = (((a + 7) >> 3) * 4 + ((b + 7) >> 3)) * 4 + ((c3 + 7) >> 3);
+            const int id = bid[slot];
+            if (id >= 0) v[q] = res[(long)id * BRICK_CELLS + ((((a + 7) & 7) * 8 + ((b + 7) & 7)) * 8 + ((c3 + 7) & 7))];
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        const int e = t + q * BLOCK;
+        if (e < E * E * E) tile[e] = v[q];
     }
     __syncthreads();
-    const long c = (long)br * BRICK_CELLS + t;
     T out = 0;
-    if (Ac.dir[c] >= 0) {
+    if (dirc >= 0) {
         const int li = t >> 6, lj = (t >> 3) & 7, lk = t & 7;
         const T w[4] = {T(0.25), T(0.75), T(0.75), T(0.25)};
         // a flat (2D) level has one cell layer: the transfer is bilinear, weight 1 on that layer
@@ -951,13 +966,18 @@ __global__ __launch_bounds__(BLOCK) void k_prolong(LevelDesc Lf, OpDesc<T> Af, c
     __shared__ int bid[8];
     const int br = brick_or_idle(Lf);
     if (br < 0) return;
-    const BrickMeta m = Lf.meta[br];
-    if (all_done(Q) || skip_smooth(Q, Lf, m)) return;
     const int t = threadIdx.x;
-    if (t < 8) bid[t] = Lf.parent8[8 * br + t];
+    const long c = (long)br * BRICK_CELLS + t;
+    // loads that depend on the brick only, issued before the gate's loads resolve
+    const T zf0 = zf[c];
+    const int pid = t < 8 ? Lf.parent8[8 * br + t] : -1;
     // with B = 2M + p, tile offset a is coarse cell 8M + 4p - 1 + a: coarse brick
     // M - 1 + p + slot with slot = ((4p + 7 + a) >> 3) - p, local (4p + 7 + a) & 7
     const int px = Lf.bxyz[3 * br] & 1, py = Lf.bxyz[3 * br + 1] & 1, pz = Lf.bxyz[3 * br + 2] & 1;
+    const BrickMeta m = Lf.meta[br];
+    if (all_done(Q) || skip_smooth(Q, Lf, m)) return;
+    const bool upd = (m.flags & BM_UNIFORM) || Af.dir[c] >= 0;
+    if (t < 8) bid[t] = pid;
     __syncthreads();
     if (t < E * E * E) {
         const int c = t % E, b = (t / E) % E, a = t / (E * E);
@@ -967,8 +987,7 @@ __global__ __launch_bounds__(BLOCK) void k_prolong(LevelDesc Lf, OpDesc<T> Af, c
         tile[t] = id >= 0 ? zc[(long)id * BRICK_CELLS + (((ia & 7) * 8 + (ib & 7)) * 8 + (ic & 7))] : T(0);
     }
     __syncthreads();
-    const long c = (long)br * BRICK_CELLS + t;
-    if (!(m.flags & BM_UNIFORM) && !(Af.dir[c] >= 0)) return;
+    if (!upd) return;
     const int li = t >> 6, lj = (t >> 3) & 7, lk = t & 7;
     const int a0 = (li >> 1) + 1, b0 = (lj >> 1) + 1, c0 = (lk >> 1) + 1;
     const int a1 = (li & 1) ? a0 + 1 : a0 - 1, b1 = (lj & 1) ? b0 + 1 : b0 - 1, c1 = (lk & 1) ? c0 + 1 : c0 - 1;
@@ -978,7 +997,7 @@ __global__ __launch_bounds__(BLOCK) void k_prolong(LevelDesc Lf, OpDesc<T> Af, c
     const T nk = Lf.flat ? T(1) : n, fk = Lf.flat ? T(0) : f;
     const T v = n * (n * (nk * at(a0, b0, c0) + fk * at(a0, b0, c1)) + f * (nk * at(a0, b1, c0) + fk * at(a0, b1, c1))) +
                 f * (n * (nk * at(a1, b0, c0) + fk * at(a1, b0, c1)) + f * (nk * at(a1, b1, c0) + fk * at(a1, b1, c1)));
-    zf[c] += v;
+    zf[c] = zf0 + v;
 }
 
 // K damped-Jacobi sweeps in one launch (coarse levels, where every launch is latency-bound).  The
@@ -1012,18 +1031,11 @@ __global__ __launch_bounds__(1024) void k_smooth_k(LevelDesc L, OpDesc<T> A, con
     __shared__ int nb27[27];
     const int br = brick_or_idle(L);
     if (br < 0) return;
+    const int t = threadIdx.x;
+    const int nid = t < 27 ? L.nb27[27 * br + t] : -1;  // issued before the gate's loads resolve
     const BrickMeta m = L.meta[br];
     if (all_done(Q) || skip_smooth(Q, L, m)) return;
-    const int t = threadIdx.x;
-    if (t < 27) {
-        const int dx = t / 9 - 1, dy = (t / 3) % 3 - 1, dz = t % 3 - 1;
-        const GridLevel G = L.gl[m.grid];
-        const int x = L.bxyz[3 * br] + dx, y = L.bxyz[3 * br + 1] + dy, z = L.bxyz[3 * br + 2] + dz;
-        int id = -1;
-        if (x >= 0 && y >= 0 && z >= 0 && x < G.nbx && y < G.nby && z < G.nbz)
-            id = L.bmap[G.bmap_off + (x * G.nby + y) * G.nbz + z];
-        nb27[t] = id;
-    }
+    if (t < 27) nb27[t] = nid;
     __syncthreads();
     T rid[CPT], rr[CPT];
 #pragma unroll

@@ -189,6 +189,7 @@ struct LevelHost {
     std::vector<GridLevel> gl;
     std::vector<BrickMeta> meta;
     std::vector<int> child64, parent8;  // restriction / prolongation brick tables (multigrid only)
+    std::vector<int> nb27;              // 3x3x3 brick neighbourhoods (coarse levels only)
     LevelDesc desc{};
     // operator and V-cycle vectors (element type = level precision)
     void *dir = nullptr, *gx = nullptr, *gy = nullptr, *gz = nullptr, *idiag = nullptr;
@@ -259,6 +260,7 @@ static void build_level(LevelHost &L, const std::vector<GridLevelHost> &grids) {
     }
     L.child64.clear();
     L.parent8.clear();
+    L.nb27.clear();
 }
 
 // Brick of grid g at brick coordinates (x, y, z) of level L, -1 outside the grid or unstored.
@@ -292,6 +294,18 @@ static void link_levels(LevelHost &F, LevelHost &Cl) {
             for (int b = 0; b < 2; ++b)
                 for (int c = 0; c < 2; ++c)
                     F.parent8[8 * (size_t)id + (a * 2 + b) * 2 + c] = brick_at(Cl, g, x0 + a, y0 + b, z0 + c);
+    }
+}
+
+// Every brick's 3x3x3 neighbourhood (bricks B-1 .. B+1 per axis of its grid, -1 where none is
+// stored), in the order k_smooth_k stages it: ((dx + 1) * 3 + (dy + 1)) * 3 + (dz + 1).
+static void neighbourhoods(LevelHost &L) {
+    L.nb27.assign((size_t)27 * L.nbricks, -1);
+    for (int id = 0; id < L.nbricks; ++id) {
+        const int g = L.bgrid[id];
+        const int X = L.bxyz[3 * id], Y = L.bxyz[3 * id + 1], Z = L.bxyz[3 * id + 2];
+        for (int t = 0; t < 27; ++t)
+            L.nb27[27 * (size_t)id + t] = brick_at(L, g, X + t / 9 - 1, Y + (t / 3) % 3 - 1, Z + t % 3 - 1);
     }
 }
 
@@ -773,10 +787,11 @@ static std::string graph_key(shkz_ctx *C, const Problem &P, const void *const *v
     const int nl = precond ? P.nlev : 1;
     for (int l = 0; l < nl; ++l) {
         const LevelHost &L = C->levels[l];
-        std::snprintf(buf, sizeof(buf), "|%d %d %p %p %p %p %p %p %p %p %p %p %p %p %p %p %p %p", L.nbricks,
+        std::snprintf(buf, sizeof(buf), "|%d %d %p %p %p %p %p %p %p %p %p %p %p %p %p %p %p %p %p", L.nbricks,
                       L.any_coarsest ? 1 : 0, (const void *)L.desc.bmap, L.dir, L.gx, L.gy, L.gz, L.idiag, L.rhs, L.z,
                       L.tmp, L.res, (const void *)L.desc.bxyz, (const void *)L.desc.meta, (const void *)L.desc.gl,
-                      (const void *)L.desc.child64, (const void *)L.desc.parent8, (const void *)L.desc.uni);
+                      (const void *)L.desc.child64, (const void *)L.desc.parent8, (const void *)L.desc.uni,
+                      (const void *)L.desc.nb27);
         k += buf;
     }
     return k;
@@ -1410,12 +1425,13 @@ static int project_stages(shkz_ctx *C, const shkz_grid *grids, int ngrids, const
         max_bricks = std::max(max_bricks, (int)brick_grid(L.nbricks));
     }
     for (int l = 0; l + 1 < P.nlev; ++l) link_levels(C->levels[l], C->levels[l + 1]);
+    for (int l = 1; l < P.nlev; ++l) neighbourhoods(C->levels[l]);
     {
-        // every level's tables in one upload: per level bmap | bxyz | meta | gl | child64 | parent8,
-        // each 32-byte aligned
+        // every level's tables in one upload: per level bmap | bxyz | meta | gl | child64 | parent8 |
+        // nb27, each 32-byte aligned
         auto al = [](size_t n) { return (n + 7) & ~(size_t)7; };
         struct Offs {
-            size_t base, xyz, meta, gl, c64, p8;
+            size_t base, xyz, meta, gl, c64, p8, n27;
         };
         std::vector<Offs> offs(P.nlev);
         size_t total = 0;
@@ -1428,7 +1444,8 @@ static int project_stages(shkz_ctx *C, const shkz_grid *grids, int ngrids, const
             o.gl = o.meta + al(L.meta.size() * (sizeof(BrickMeta) / sizeof(int)));
             o.c64 = o.gl + al(L.gl.size() * (sizeof(GridLevel) / sizeof(int)));
             o.p8 = o.c64 + al(L.child64.size());
-            total = o.p8 + al(L.parent8.size());
+            o.n27 = o.p8 + al(L.parent8.size());
+            total = o.n27 + al(L.nb27.size());
         }
         total += 8;
         int *dmeta = nullptr;
@@ -1446,6 +1463,7 @@ static int project_stages(shkz_ctx *C, const shkz_grid *grids, int ngrids, const
             std::memcpy(host + o.gl, L.gl.data(), L.gl.size() * sizeof(GridLevel));
             std::copy(L.child64.begin(), L.child64.end(), host + o.c64);
             std::copy(L.parent8.begin(), L.parent8.end(), host + o.p8);
+            std::copy(L.nb27.begin(), L.nb27.end(), host + o.n27);
             L.desc.nbricks = L.nbricks;
             L.desc.ngrids = ngrids;
             L.desc.flat = flat ? 1 : 0;
@@ -1455,6 +1473,7 @@ static int project_stages(shkz_ctx *C, const shkz_grid *grids, int ngrids, const
             L.desc.gl = reinterpret_cast<const GridLevel *>(dmeta + o.gl);
             L.desc.child64 = L.child64.empty() ? nullptr : dmeta + o.c64;
             L.desc.parent8 = L.parent8.empty() ? nullptr : dmeta + o.p8;
+            L.desc.nb27 = L.nb27.empty() ? nullptr : dmeta + o.n27;
             L.desc.uni = nullptr;
         }
         // the pinned staging stays untouched until this call's next synchronisation
