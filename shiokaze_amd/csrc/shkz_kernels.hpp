@@ -472,29 +472,30 @@ __global__ __launch_bounds__(BLOCK) void k_apply_dot(LevelDesc L, OpDesc<T> A, c
     const int br = brick_or_idle(L);
     if (br < 0) return;
     const BrickMeta m = L.meta[br];
-    if (ctl->all_done || (L.ngrids > 1 && S[m.grid].done)) return;
     const int t = threadIdx.x;
     const long c = (long)br * BRICK_CELLS + t;
+    // single grid: p is loaded before the gate resolves; batch: finished grids load nothing
+    if (L.ngrids > 1 && (ctl->all_done || S[m.grid].done)) return;
+    const T pc = p[c];
+    if (ctl->all_done) return;
     T out = 0;
     double part = 0.0;
     if (m.flags & BM_UNIFORM) {
         const UniT<T> u = uni_load<T>(L, br);
         long idx[6];
         nb_index(nbrs_of(m, br), t, idx);
-        const T pc = p[c];
         out = apply_row_uniform(u.g, p, c, idx, pc);
         part = (double)pc * (double)out;
     } else {
+        // every load is issued before the unknown test (the values of other cells are discarded)
+        const Nbrs N = nbrs_of(m, br);
+        long idx[6];
+        nb_index(N, t, idx);
         const T dirc = A.dir[c];
-        if (dirc >= 0) {
-            const Nbrs N = nbrs_of(m, br);
-            long idx[6];
-            nb_index(N, t, idx);
-            const T pc = p[c];
-            T diag;
-            out = apply_row(A, p, c, idx, dirc, pc, diag);
-            part = (double)pc * (double)out;
-        }
+        T diag;
+        const T v = apply_row(A, p, c, idx, dirc, pc, diag);
+        out = dirc >= 0 ? v : T(0);
+        part = dirc >= 0 ? (double)pc * (double)v : 0.0;
     }
     q[c] = out;
     part = block_sum(part, lds);
@@ -510,12 +511,16 @@ __global__ __launch_bounds__(BLOCK) void k_cg_update(LevelDesc L, T *__restrict_
     __shared__ double lds[32];
     const int br = brick_or_idle(L);
     if (br < 0) return;
+    const long c = (long)br * BRICK_CELLS + threadIdx.x;
+    // A single grid issues the vectors' loads before the gate's loads resolve (they are wasted
+    // only on an iteration past convergence); in a batch, bricks of finished grids load nothing.
+    if (L.ngrids > 1 && (ctl->all_done || S[grid_of(L, br)].done)) return;
+    const T xv = x[c], pv = p[c], r0 = r[c], qv = q[c];
     const SolveScalars &Sg = S[grid_of(L, br)];
     if (ctl->all_done || Sg.done) return;
-    const long c = (long)br * BRICK_CELLS + threadIdx.x;
     const T alpha = (T)Sg.alpha;
-    x[c] += alpha * p[c];
-    const T rv = r[c] - alpha * q[c];
+    x[c] = xv + alpha * pv;
+    const T rv = r0 - alpha * qv;
     r[c] = rv;
     double mx = fabs((double)rv), sq = (double)rv * (double)rv;
     block_reduce2<true>(mx, sq, lds);
@@ -541,10 +546,13 @@ __global__ __launch_bounds__(BLOCK) void k_update_p(LevelDesc L, const T *__rest
                                                     const SolveScalars *S, const BatchCtl *ctl, int first) {
     const int br = brick_or_idle(L);
     if (br < 0) return;
+    const long c = (long)br * BRICK_CELLS + threadIdx.x;
+    // single grid: loads before the gate resolves; batch: finished grids load nothing
+    if (L.ngrids > 1 && (ctl->all_done || S[grid_of(L, br)].done)) return;
+    const T zv = z[c], pv = first ? T(0) : p[c];
     const SolveScalars &Sg = S[grid_of(L, br)];
     if (ctl->all_done || Sg.done) return;
-    const long c = (long)br * BRICK_CELLS + threadIdx.x;
-    p[c] = first ? z[c] : z[c] + (T)Sg.beta * p[c];
+    p[c] = first ? zv : zv + (T)Sg.beta * pv;
 }
 
 // x64 += (double) corr
@@ -843,15 +851,15 @@ __device__ __forceinline__ void jacobi_brick(const LevelDesc &L, const OpDesc<T>
         out = (T(1) - omega) * zin[c] + omega * u.id * (r + offdiag_uniform(u.g, zin, idx));
         if (DOT) part = (double)r * (double)out;
     } else {
+        // every load is issued before the unknown test (the values of other cells are discarded)
+        const Nbrs N = nbrs_of(m, br);
+        long idx[6];
+        nb_index(N, t, idx);
         const T id = idiag[c];
-        if (id > 0) {
-            const Nbrs N = nbrs_of(m, br);
-            long idx[6];
-            nb_index(N, t, idx);
-            const T r = rhs[c];
-            out = (T(1) - omega) * zin[c] + omega * id * (r + offdiag_sum(A, zin, c, idx));
-            if (DOT) part = (double)r * (double)out;
-        }
+        const T r = rhs[c];
+        const T v = (T(1) - omega) * zin[c] + omega * id * (r + offdiag_sum(A, zin, c, idx));
+        out = id > 0 ? v : T(0);
+        if (DOT) part = id > 0 ? (double)r * (double)v : 0.0;
     }
     zout[c] = out;
     if (DOT) {
@@ -877,14 +885,14 @@ __global__ __launch_bounds__(BLOCK) void k_residual(LevelDesc L, OpDesc<T> A, co
         nb_index(nbrs_of(m, br), t, idx);
         out = rhs[c] - apply_row_uniform(u.g, z, c, idx, z[c]);
     } else {
+        // every load is issued before the unknown test (the values of other cells are discarded)
+        const Nbrs N = nbrs_of(m, br);
+        long idx[6];
+        nb_index(N, t, idx);
         const T dirc = A.dir[c];
-        if (dirc >= 0) {
-            const Nbrs N = nbrs_of(m, br);
-            long idx[6];
-            nb_index(N, t, idx);
-            T diag;
-            out = rhs[c] - apply_row(A, z, c, idx, dirc, z[c], diag);
-        }
+        T diag;
+        const T v = rhs[c] - apply_row(A, z, c, idx, dirc, z[c], diag);
+        out = dirc >= 0 ? v : T(0);
     }
     res[c] = out;
 }
