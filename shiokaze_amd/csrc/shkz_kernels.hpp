@@ -1027,24 +1027,79 @@ struct SmoothK {
 // RES: the cube carries one more ghost layer, so that after the K sweeps the brick's neighbours are
 // exact too, and the kernel also writes the residual rhs - A z of the brick (what k_residual
 // computes next, in the same arithmetic): one launch fewer per V-cycle on the level.
-template <class T, int K, int FROM_ZERO, int RES = 0>
+//
+// PRO (post-smoothing, with zin): the level's z first receives the trilinear prolongation of the
+// next coarser level's correction zcoarse, as k_prolong adds it (same arithmetic), on every cell
+// of the cube: one launch fewer per V-cycle on the level.  The block stages the coarse cells the
+// cube needs, in the (up to) 2x2x2 coarse bricks of its parent table.
+template <class T, int K>
+struct ProTile {
+    static constexpr int R = BRICK + 2 * K;
+    static constexpr int CT = R / 2 + 2;  // coarse cells per axis
+    static constexpr int N = CT * CT * CT;
+};
+
+template <class T, int K, int FROM_ZERO, int RES = 0, int PRO = 0>
 __global__ __launch_bounds__(1024) void k_smooth_k(LevelDesc L, OpDesc<T> A, const T *__restrict__ idiag,
                                                    const T *__restrict__ rhs, const T *__restrict__ zin,
-                                                   T *__restrict__ zout, T omega, Gate Q, T *__restrict__ res) {
+                                                   T *__restrict__ zout, T omega, Gate Q, T *__restrict__ res,
+                                                   const T *__restrict__ zcoarse) {
+    static_assert(!(PRO && (FROM_ZERO || RES)), "the prolongation joins post-smoothing only");
     constexpr int GH = K + RES;  // ghost cells per side
     using C = SmoothK<T, GH>;
     constexpr int R = C::R, NR = C::NR, CPT = C::CPT;
+    using PT = ProTile<T, GH>;
     __shared__ T zs[2][NR];
     __shared__ T gxs[NR], gys[NR], gzs[NR];
+    __shared__ T ctile[PRO ? PT::N : 1];
     __shared__ int nb27[27];
+    __shared__ int pb[8];
     const int br = brick_or_idle(L);
     if (br < 0) return;
     const int t = threadIdx.x;
-    const int nid = t < 27 ? L.nb27[27 * br + t] : -1;  // issued before the gate's loads resolve
+    // loads that depend on the brick only, issued before the gate's loads resolve
+    const int nid = t < 27 ? L.nb27[27 * br + t] : -1;
+    const int pid = (PRO && t < 8) ? L.parent8[8 * br + t] : -1;
+    int px = 0, py = 0, pz = 0;
+    if (PRO) {
+        px = L.bxyz[3 * br] & 1;
+        py = L.bxyz[3 * br + 1] & 1;
+        pz = L.bxyz[3 * br + 2] & 1;
+    }
     const BrickMeta m = L.meta[br];
     if (all_done(Q) || skip_smooth(Q, L, m)) return;
     if (t < 27) nb27[t] = nid;
+    if (PRO && t < 8) pb[t] = pid;
     __syncthreads();
+    // Coarse cells are indexed relative to coarse cell 8M - 8 of the brick B = 2M + p (per axis):
+    // cube offset a is fine cell 16M + 8p + a - GH, i.e. fi = 16 + 8p + a - GH relative to fine
+    // cell 16M - 16; its coarse cells are fi >> 1 (weight 3/4) and the one beside it toward fi's
+    // parity side (1/4).  The tile starts at coarse index cmin = ((16 + 8p - GH) >> 1) - 1; coarse
+    // index I lies in parent slot (I >> 3) - p, local cell I & 7.
+    const int cmx = ((16 + 8 * px - GH) >> 1) - 1, cmy = ((16 + 8 * py - GH) >> 1) - 1,
+              cmz = ((16 + 8 * pz - GH) >> 1) - 1;
+    if (PRO) {
+        constexpr int CT = PT::CT, NQ = (PT::N + C::THREADS - 1) / C::THREADS;
+        T cv[NQ];
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+            const int e = t + q * C::THREADS;
+            cv[q] = T(0);
+            if (e < PT::N) {
+                const int tc = e % CT, tb = (e / CT) % CT, ta = e / (CT * CT);
+                const int Ia = cmx + ta, Ib = cmy + tb, Ic = cmz + tc;
+                const int slot = (((Ia >> 3) - px) * 2 + ((Ib >> 3) - py)) * 2 + ((Ic >> 3) - pz);
+                const int id = pb[slot];
+                if (id >= 0) cv[q] = zcoarse[(long)id * BRICK_CELLS + (((Ia & 7) * 8 + (Ib & 7)) * 8 + (Ic & 7))];
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+            const int e = t + q * C::THREADS;
+            if (e < PT::N) ctile[e] = cv[q];
+        }
+        __syncthreads();
+    }
     T rid[CPT], rr[CPT];
 #pragma unroll
     for (int q = 0; q < CPT; ++q) {
@@ -1066,6 +1121,22 @@ __global__ __launch_bounds__(1024) void k_smooth_k(LevelDesc L, OpDesc<T> A, con
             g1 = A.gy[cell];
             g2 = A.gz[cell];
             if (!FROM_ZERO) z0 = zin[cell];
+            if (PRO && A.dir[cell] >= 0) {
+                // k_prolong's correction of this cell
+                constexpr int CT = PT::CT;
+                const int fa = 16 + 8 * px + a - GH, fb = 16 + 8 * py + b - GH, fc = 16 + 8 * pz + c - GH;
+                const int a0 = (fa >> 1) - cmx, b0 = (fb >> 1) - cmy, c0 = (fc >> 1) - cmz;
+                const int a1 = (fa & 1) ? a0 + 1 : a0 - 1, b1 = (fb & 1) ? b0 + 1 : b0 - 1,
+                          c1 = (fc & 1) ? c0 + 1 : c0 - 1;
+                auto at = [&](int i, int j, int k) { return ctile[(i * CT + j) * CT + k]; };
+                const T n = T(0.75), f = T(0.25);
+                const T nk = L.flat ? T(1) : n, fk = L.flat ? T(0) : f;
+                const T v = n * (n * (nk * at(a0, b0, c0) + fk * at(a0, b0, c1)) +
+                                 f * (nk * at(a0, b1, c0) + fk * at(a0, b1, c1))) +
+                            f * (n * (nk * at(a1, b0, c0) + fk * at(a1, b0, c1)) +
+                                 f * (nk * at(a1, b1, c0) + fk * at(a1, b1, c1)));
+                z0 = z0 + v;
+            }
         }
         gxs[e] = g0;
         gys[e] = g1;

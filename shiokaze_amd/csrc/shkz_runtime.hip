@@ -562,21 +562,35 @@ static bool smooth_k_res_supported(int k) {
     return k >= 2 && k <= (sizeof(T) == 4 ? 4 : 2);
 }
 
-template <class T, int KK, int FZ, int RES>
+template <class T, int KK, int FZ, int RES, int PRO = 0>
 static void smooth_k_go(Launcher &K, int l, dim3 g, const LevelDesc &L, OpDesc<T> A, const T *id, const T *rhs,
-                        const T *zin, T *zout, T om, Gate Q, T *res) {
-    // per unknown, the arrays it reads once and writes once (dir and res with the residual)
-    const double bytes = (RES ? 9.0 : 7.0) * sizeof(T);
-    K.go(RES ? "smooth_k_res_Lc" : "smooth_k_Lc", l, bytes, 0, g, dim3(1024), k_smooth_k<T, KK, FZ, RES>, L, A, id,
-         rhs, zin, zout, om, Q, res);
+                        const T *zin, T *zout, T om, Gate Q, T *res, const T *zcoarse = nullptr,
+                        double coarse_unknowns = 0) {
+    // per unknown, the arrays it reads once and writes once (dir and res with the residual, dir
+    // with the prolongation, which also reads the coarse correction once)
+    const double bytes = (RES ? 9.0 : PRO ? 8.0 : 7.0) * sizeof(T);
+    K.go(RES ? "smooth_k_res_Lc" : PRO ? "smooth_k_pro_Lc" : "smooth_k_Lc", l, bytes, coarse_unknowns * sizeof(T), g,
+         dim3(1024), k_smooth_k<T, KK, FZ, RES, PRO>, L, A, id, rhs, zin, zout, om, Q, res, zcoarse);
 }
 
 // k damped-Jacobi sweeps of one coarse level in one launch (from zero or from zin); with res
-// (pre-smoothing from zero, k within smooth_k_res_supported), also the residual of the result.
+// (pre-smoothing from zero, k within smooth_k_res_supported), also the residual of the result;
+// with zcoarse (post-smoothing), zin first receives the prolongation of the coarser level's
+// correction zcoarse (coarse_unknowns of them, for the byte model).
 template <class T>
 static void smooth_k(Launcher &K, int l, int k, bool from_zero, const LevelDesc &L, OpDesc<T> A, const T *id,
-                     const T *rhs, const T *zin, T *zout, T om, Gate Q, T *res = nullptr) {
+                     const T *rhs, const T *zin, T *zout, T om, Gate Q, T *res = nullptr,
+                     const T *zcoarse = nullptr, double coarse_unknowns = 0) {
     const dim3 g(brick_grid(L.nbricks));
+    if (zcoarse && !from_zero) {
+        if (k == 2) smooth_k_go<T, 2, 0, 0, 1>(K, l, g, L, A, id, rhs, zin, zout, om, Q, nullptr, zcoarse, coarse_unknowns);
+        if (k == 3) smooth_k_go<T, 3, 0, 0, 1>(K, l, g, L, A, id, rhs, zin, zout, om, Q, nullptr, zcoarse, coarse_unknowns);
+        if constexpr (sizeof(T) == 4) {
+            if (k == 4) smooth_k_go<T, 4, 0, 0, 1>(K, l, g, L, A, id, rhs, zin, zout, om, Q, nullptr, zcoarse, coarse_unknowns);
+            if (k == 5) smooth_k_go<T, 5, 0, 0, 1>(K, l, g, L, A, id, rhs, zin, zout, om, Q, nullptr, zcoarse, coarse_unknowns);
+        }
+        return;
+    }
     if (res && from_zero) {
         if (k == 2) smooth_k_go<T, 2, 1, 1>(K, l, g, L, A, id, rhs, zin, zout, om, Q, res);
         if constexpr (sizeof(T) == 4) {
@@ -699,16 +713,18 @@ static T *vcycle(shkz_ctx *C, Launcher &K, Problem &P, const T *rhs0, T *z0, T *
         const dim3 gp(std::min<unsigned>(g.x, PERSIST_BLOCKS));  // persistent kernels
         const bool f = l == 0;
         const int post = f ? P.post : P.post_c;
-        K.go(f ? "prolong_L0" : "prolong_Lc", l, 3 * es, es * (double)Lc.unknowns - usave(C, l, es), g, b,
-             f ? k_prolong<T, 1> : k_prolong<T, 0>, L.desc, A, (const T *)zcur[l + 1], zcur[l], Q);
         T *cur = zcur[l], *oth = zoth[l];
         if (mpost[l]) {
-            smooth_k<T>(K, l, post, false, L.desc, A, id, rhs, (const T *)cur, oth, om, Q);
+            // the prolongation joins the multi-sweep launch
+            smooth_k<T>(K, l, post, false, L.desc, A, id, rhs, (const T *)cur, oth, om, Q, nullptr,
+                        (const T *)zcur[l + 1], (double)Lc.unknowns);
             std::swap(cur, oth);
             zcur[l] = cur;
             zoth[l] = oth;
             continue;
         }
+        K.go(f ? "prolong_L0" : "prolong_Lc", l, 3 * es, es * (double)Lc.unknowns - usave(C, l, es), g, b,
+             f ? k_prolong<T, 1> : k_prolong<T, 0>, L.desc, A, (const T *)zcur[l + 1], zcur[l], Q);
         for (int s = 0; s < post; ++s) {
             const bool last_dot = f && dot_partials && s == post - 1;
             if (last_dot)
