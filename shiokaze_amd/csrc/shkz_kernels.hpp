@@ -622,21 +622,21 @@ __global__ __launch_bounds__(1024) void k_finalize(const double *__restrict__ pa
                            op == FIN_INNER_INIT || op == FIN_OUTER);
     double a = 0.0, b = 0.0;
     const double2 *pp = reinterpret_cast<const double2 *>(partials);
-    int i = G.brick_start + threadIdx.x;
-    for (; i + 3 * (int)blockDim.x < G.brick_end; i += 4 * blockDim.x) {
-        double2 v[4];
+    // up to 16 loads in flight per thread (16K bricks per pass); past the grid's end a thread adds
+    // zeros, which change neither the sums nor the maxima (of norms)
+    constexpr int U = 16;
+    for (int i = G.brick_start + threadIdx.x; i < G.brick_end; i += U * blockDim.x) {
+        double2 v[U];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) v[u] = pp[i + u * blockDim.x];
+        for (int u = 0; u < U; ++u) {
+            const int j = i + u * (int)blockDim.x;
+            v[u] = j < G.brick_end ? pp[j] : make_double2(0.0, 0.0);
+        }
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
+        for (int u = 0; u < U; ++u) {
             a = a_is_max ? fmax(a, v[u].x) : a + v[u].x;
             b += v[u].y;
         }
-    }
-    for (; i < G.brick_end; i += blockDim.x) {
-        const double2 v = pp[i];
-        a = a_is_max ? fmax(a, v.x) : a + v.x;
-        b += v.y;
     }
     if (a_is_max) block_reduce2<true>(a, b, lds);
     else block_reduce2<false>(a, b, lds);
@@ -1233,6 +1233,103 @@ __global__ __launch_bounds__(BLOCK) void k_coarse_solve(LevelDesc L, OpDesc<T> A
     z[c] = zo;
     if (DOT) {
         const double part = block_sum((double)r * (double)zo, lds);
+        store_partials(R, G.brick_start, part, 0.0);
+    }
+}
+
+// FP32 coarse solve with one wavefront per grid: lane t keeps the cell row (t >> 3, t & 7, 0..7)
+// of the brick and its couplings in registers, and each sweep exchanges rows through LDS (16-byte
+// accesses, no multi-wave barrier).  The per-cell arithmetic is k_coarse_solve's.
+template <int DOT>
+__global__ __launch_bounds__(64) void k_coarse_solve_w(LevelDesc L, OpDesc<float> A, const float *__restrict__ rhs,
+                                                       float *__restrict__ z, int sweeps, float omega, Gate Q,
+                                                       RedBuf R) {
+    __shared__ float4 zs[2][BRICK_CELLS / 4];
+    if (all_done(Q)) return;
+    const int g = blockIdx.x;
+    const GridLevel G = L.gl[g];
+    if (!G.coarsest || G.brick_end <= G.brick_start || (Q.S && Q.S[g].done)) return;
+    const int t = threadIdx.x, li = t >> 3, lj = t & 7;
+    const int o = li * 64 + lj * 8;  // the row's first cell in the brick
+    const long row = (long)G.brick_start * BRICK_CELLS + o;
+    auto ld8 = [&](const float *a, long i, float v[8]) {
+        const float4 x = *reinterpret_cast<const float4 *>(a + i), y = *reinterpret_cast<const float4 *>(a + i + 4);
+        v[0] = x.x; v[1] = x.y; v[2] = x.z; v[3] = x.w; v[4] = y.x; v[5] = y.y; v[6] = y.z; v[7] = y.w;
+    };
+    float dirc[8], gxm[8], gym[8], gzm[8], gxp[8], gyp[8], r[8];
+    ld8(A.dir, row, dirc);
+    ld8(A.gx, row, gxm);
+    ld8(A.gy, row, gym);
+    ld8(A.gz, row, gzm);
+    ld8(rhs, row, r);
+    if (li < 7) ld8(A.gx, row + 64, gxp);
+    else
+        for (int k = 0; k < 8; ++k) gxp[k] = 0.0f;
+    if (lj < 7) ld8(A.gy, row + 8, gyp);
+    else
+        for (int k = 0; k < 8; ++k) gyp[k] = 0.0f;
+    float diag[8], inv[8], zc[8];
+    bool unk[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const float gzp = k < 7 ? gzm[k + 1] : 0.0f;
+        diag[k] = dirc[k] + ((gxm[k] + gxp[k]) + (gym[k] + gyp[k]) + (gzm[k] + gzp));
+        unk[k] = dirc[k] >= 0 && diag[k] > 0;
+        inv[k] = unk[k] ? omega / diag[k] : 0.0f;
+        zc[k] = unk[k] ? inv[k] * r[k] : 0.0f;
+    }
+    auto put = [&](int buf) {
+        zs[buf][o / 4] = make_float4(zc[0], zc[1], zc[2], zc[3]);
+        zs[buf][o / 4 + 1] = make_float4(zc[4], zc[5], zc[6], zc[7]);
+    };
+    auto get = [&](int buf, int off, float v[8]) {
+        const float4 x = zs[buf][off / 4], y = zs[buf][off / 4 + 1];
+        v[0] = x.x; v[1] = x.y; v[2] = x.z; v[3] = x.w; v[4] = y.x; v[5] = y.y; v[6] = y.z; v[7] = y.w;
+    };
+    put(0);
+    __syncthreads();
+    int cur = 0;
+    for (int s = 1; s < sweeps; ++s) {
+        float xm[8], xp[8], ym[8], yp[8];
+        if (li > 0) get(cur, o - 64, xm);
+        else
+            for (int k = 0; k < 8; ++k) xm[k] = 0.0f;
+        if (li < 7) get(cur, o + 64, xp);
+        else
+            for (int k = 0; k < 8; ++k) xp[k] = 0.0f;
+        if (lj > 0) get(cur, o - 8, ym);
+        else
+            for (int k = 0; k < 8; ++k) ym[k] = 0.0f;
+        if (lj < 7) get(cur, o + 8, yp);
+        else
+            for (int k = 0; k < 8; ++k) yp[k] = 0.0f;
+        float nz[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            float out = 0.0f;
+            if (unk[k]) {
+                const float gzp = k < 7 ? gzm[k + 1] : 0.0f;
+                const float zzm = k > 0 ? zc[k - 1] : 0.0f, zzp = k < 7 ? zc[k + 1] : 0.0f;
+                const float off = gxm[k] * xm[k] + gxp[k] * xp[k] + gym[k] * ym[k] + gyp[k] * yp[k] + gzm[k] * zzm +
+                                  gzp * zzp;
+                out = zc[k] + inv[k] * (r[k] - (diag[k] * zc[k] - off));
+            }
+            nz[k] = out;
+        }
+#pragma unroll
+        for (int k = 0; k < 8; ++k) zc[k] = nz[k];
+        cur ^= 1;
+        put(cur);
+        __syncthreads();
+    }
+    float4 *zo = reinterpret_cast<float4 *>(z + row);
+    zo[0] = make_float4(zc[0], zc[1], zc[2], zc[3]);
+    zo[1] = make_float4(zc[4], zc[5], zc[6], zc[7]);
+    if (DOT) {
+        double part = 0.0;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) part += (double)r[k] * (double)zc[k];
+        part = wave_sum(part);
         store_partials(R, G.brick_start, part, 0.0);
     }
 }

@@ -655,7 +655,15 @@ static T *vcycle(shkz_ctx *C, Launcher &K, Problem &P, const T *rhs0, T *z0, T *
         if (L.any_coarsest) {
             // grids ending here: their z goes where the other grids' smoothed z ends up
             T *dst = (last || final_slot(pre, post, mpre[l], mpost[l], f) == 0) ? z : tmp;
-            if (f && dot_partials)
+            if constexpr (sizeof(T) == 4) {
+                // one wavefront per grid
+                if (f && dot_partials)
+                    K.go("coarse_solve_dot", l, 6 * es, 0, dim3((unsigned)P.ngrids), dim3(64), k_coarse_solve_w<1>,
+                         L.desc, A, rhs, dst, P.coarse_sweeps, om, Q, P.red);
+                else
+                    K.go("coarse_solve", l, 6 * es, 0, dim3((unsigned)P.ngrids), dim3(64), k_coarse_solve_w<0>, L.desc,
+                         A, rhs, dst, P.coarse_sweeps, om, Q, P.red);
+            } else if (f && dot_partials)
                 K.go("coarse_solve_dot", l, 6 * es, 0, dim3((unsigned)P.ngrids), b, k_coarse_solve<T, 1>, L.desc, A,
                      rhs, dst, P.coarse_sweeps, om, Q, P.red);
             else
