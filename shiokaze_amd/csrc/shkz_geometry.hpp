@@ -50,6 +50,13 @@ __device__ __forceinline__ double solid_at(const GridDesc &G, int i, int j, int 
 }
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
+// The grid's two "has a negative value" flags (liquid, solid), read once per thread: the face
+// functions would otherwise reload them for every face, behind every store that might alias them.
+struct Exist {
+    bool liquid, solid;
+};
+__device__ __forceinline__ Exist exist_of(const GridDesc &G) { return Exist{G.exist[0] != 0, G.exist[1] != 0}; }
+
 // (i, j, k) of flat index e of a z-fastest (ni, nj, nk) box, in 32-bit arithmetic: every dense box
 // here holds fewer than 2^32 entries (checked at entry), and 64-bit division has no hardware
 // support (a long software sequence per thread).
@@ -75,6 +82,11 @@ __device__ __forceinline__ double liquid_fraction(double phi0, double phi1) {
 // polygon of negative corners and edge crossings, shoelace sum (utility.h:179-214).
 __device__ __forceinline__ double negative_area(const double v[4]) {
 #pragma clang fp contract(off)
+    // no negative corner: no vertex and no crossing (a product of two non-negative samples is not
+    // negative), the sum is 0; all four negative: the unit square, whose shoelace sum is exactly 2
+    const bool n0 = v[0] < 0.0, n1 = v[1] < 0.0, n2 = v[2] < 0.0, n3 = v[3] < 0.0;
+    if (!(n0 || n1 || n2 || n3)) return 0.0;
+    if (n0 && n1 && n2 && n3) return 1.0;
     const int cx[4] = {0, 1, 1, 0};
     const int cy[4] = {0, 0, 1, 1};
     double px[8], py[8];
@@ -106,13 +118,13 @@ __device__ __forceinline__ double negative_area(const double v[4]) {
 }
 
 // Open fraction of face (dim; i,j,k), float as stored by the reference.
-__device__ __forceinline__ float face_area(const GridDesc &G, int dim, int i, int j, int k) {
+__device__ __forceinline__ float face_area(const GridDesc &G, const Exist &E, int dim, int i, int j, int k) {
 #pragma clang fp contract(off)
     const int N[3] = {G.w, G.h, G.d};
     const int pi[3] = {i, j, k};
     const bool boundary = (pi[dim] == 0 || pi[dim] == N[dim]);
     double a;
-    if (G.solid && G.exist[1]) {
+    if (G.solid && E.solid) {
         if (boundary) {
             a = 0.0;
         } else if (G.flat) {
@@ -151,10 +163,10 @@ __device__ __forceinline__ float face_area(const GridDesc &G, int dim, int i, in
 
 // Liquid fraction of face (dim; i,j,k) between cell (i,j,k) and (i,j,k)-e_dim, clamped into the
 // grid, float as stored by the reference.
-__device__ __forceinline__ float face_rho(const GridDesc &G, int dim, int i, int j, int k) {
+__device__ __forceinline__ float face_rho(const GridDesc &G, const Exist &E, int dim, int i, int j, int k) {
 #pragma clang fp contract(off)
     double r;
-    if (G.exist[0]) {
+    if (E.liquid) {
         const int i0 = clampi(i, 0, G.w - 1), j0 = clampi(j, 0, G.h - 1), k0 = clampi(k, 0, G.d - 1);
         const int i1 = clampi(i - (dim == 0), 0, G.w - 1), j1 = clampi(j - (dim == 1), 0, G.h - 1),
                   k1 = clampi(k - (dim == 2), 0, G.d - 1);
@@ -166,6 +178,13 @@ __device__ __forceinline__ float face_rho(const GridDesc &G, int dim, int i, int
     float rf = (float)r;
     if (!G.so_fluid && rf) rf = 1.0f;
     return rf;
+}
+
+__device__ __forceinline__ float face_area(const GridDesc &G, int dim, int i, int j, int k) {
+    return face_area(G, exist_of(G), dim, i, j, k);
+}
+__device__ __forceinline__ float face_rho(const GridDesc &G, int dim, int i, int j, int k) {
+    return face_rho(G, exist_of(G), dim, i, j, k);
 }
 
 }  // namespace shkz

@@ -183,7 +183,7 @@ __global__ void k_surface_tension(GridDesc G, int dim, const float *curv, double
     if (G.active[dim] && !G.active[dim][f]) return;
     int i, j, k;
     unflatten((unsigned)f, (unsigned)H, (unsigned)D, i, j, k);
-    const double rho = face_rho(G, dim, i, j, k);
+    const double rho = face_rho(G, exist_of(G), dim, i, j, k);
     if (rho && rho < 1.0) {
         const int i0 = clampi(i, 0, G.w - 1), j0 = clampi(j, 0, G.h - 1), k0 = clampi(k, 0, G.d - 1);
         const int i1 = clampi(i - (dim == 0), 0, G.w - 1), j1 = clampi(j - (dim == 1), 0, G.h - 1),
@@ -206,6 +206,7 @@ __global__ __launch_bounds__(BLOCK) void k_flags(GridDesc G, int nbx, int nby, i
     const int t = threadIdx.x;
     const int i = bx * BRICK + (t >> 6), j = by * BRICK + ((t >> 3) & 7), k = bz * BRICK + (t & 7);
     int unknown = 0;
+    const Exist E = exist_of(G);
     if (i < G.w && j < G.h && k < G.d) {
         const bool liquid = phi_at(G, i, j, k) < 0.0;
         if (liquid) {
@@ -216,7 +217,7 @@ __global__ __launch_bounds__(BLOCK) void k_flags(GridDesc G, int nbx, int nby, i
                 if (!(phi_at(G, qi, qj, qk) < 0.0)) continue;
                 const int dim = nq >> 1;
                 const int fi = (nq & 1) ? i : qi, fj = (nq & 1) ? j : qj, fk = (nq & 1) ? k : qk;
-                if (face_area(G, dim, fi, fj, fk) != 0.0f && face_rho(G, dim, fi, fj, fk) != 0.0f) unknown = 1;
+                if (face_area(G, E, dim, fi, fj, fk) != 0.0f && face_rho(G, E, dim, fi, fj, fk) != 0.0f) unknown = 1;
             }
         }
         cflag[cell_index(G, i, j, k)] = (uint8_t)((liquid ? 1 : 0) | (unknown ? 2 : 0));
@@ -247,6 +248,7 @@ __global__ __launch_bounds__(BLOCK) void k_operator(GridDesc G, LevelDesc L, Gri
     const bool unknown = inside && (cflag[cell_index(G, i, j, k)] & 2);
     double dir = -1.0, g[3] = {0.0, 0.0, 0.0}, rhs = 0.0, x0 = 0.0;
     if (unknown) {
+        const Exist E = exist_of(G);
         const double dt = G.dt, dx = G.dx;
         const int di[6] = {1, -1, 0, 0, 0, 0}, dj[6] = {0, 0, 1, -1, 0, 0}, dk[6] = {0, 0, 0, 0, 1, -1};
         const int sgn[6] = {1, -1, 1, -1, 1, -1};
@@ -256,9 +258,9 @@ __global__ __launch_bounds__(BLOCK) void k_operator(GridDesc G, LevelDesc L, Gri
             if (qi < 0 || qi >= G.w || qj < 0 || qj >= G.h || qk < 0 || qk >= G.d) continue;
             const int dim = nq >> 1;
             const int fi = (nq & 1) ? i : qi, fj = (nq & 1) ? j : qj, fk = (nq & 1) ? k : qk;
-            const double area = face_area(G, dim, fi, fj, fk);
+            const double area = face_area(G, E, dim, fi, fj, fk);
             if (area) {
-                const double rho = face_rho(G, dim, fi, fj, fk);
+                const double rho = face_rho(G, E, dim, fi, fj, fk);
                 if (rho) {
                     const double value = dt * area / (dx * dx * rho);
                     if (phi_at(G, qi, qj, qk) < 0.0) {
@@ -1376,7 +1378,8 @@ __global__ void k_export_system(GridDesc G, LevelDesc L, GridLevel GL, OpDesc<do
 // (i,j,k) in one thread (each level-set sample is read once for the three faces): open faces on
 // the domain boundary become 0, interior open faces lose dt*(P1-P0)/(rho*dx) with P stored in
 // float and P1-P0 taken in float; all other (active) faces are switched off (0).
-__device__ __forceinline__ void update_face(const GridDesc &G, int dim, int i, int j, int k, float p1, float p0) {
+__device__ __forceinline__ void update_face(const GridDesc &G, const Exist &E, int dim, int i, int j, int k, float p1,
+                                            float p0) {
 #pragma clang fp contract(off)
     const size_t f = face_index(G, dim, i, j, k);
     uint8_t *aout = G.active_out[dim];
@@ -1388,8 +1391,8 @@ __device__ __forceinline__ void update_face(const GridDesc &G, int dim, int i, i
     const int N[3] = {G.w, G.h, G.d};
     float *U = G.vel[dim];
     // the area only matters where rho is nonzero: most faces of the domain have air on both sides
-    const double rho = face_rho(G, dim, i, j, k);
-    const float area = rho != 0.0 ? face_area(G, dim, i, j, k) : 0.0f;
+    const double rho = face_rho(G, E, dim, i, j, k);
+    const float area = rho != 0.0 ? face_area(G, E, dim, i, j, k) : 0.0f;
     uint8_t keep = 1;
     if (area != 0.0f && rho != 0.0) {
         if (pi[dim] == 0 || pi[dim] == N[dim]) {
@@ -1420,9 +1423,10 @@ __global__ void k_velocity_update(GridDesc G, const float *__restrict__ P) {
     const bool in_i = i < G.w, in_j = j < G.h, in_k = k < G.d;
     auto p_at = [&](int a, int b, int c) { return (a >= 0 && b >= 0 && c >= 0) ? P[cell_index(G, a, b, c)] : 0.0f; };
     const float pc = (in_i && in_j && in_k) ? P[cell_index(G, i, j, k)] : 0.0f;
-    if (in_j && in_k) update_face(G, 0, i, j, k, pc, i > 0 ? p_at(i - 1, j, k) : 0.0f);
-    if (in_i && in_k) update_face(G, 1, i, j, k, pc, j > 0 ? p_at(i, j - 1, k) : 0.0f);
-    if (in_i && in_j && !G.flat) update_face(G, 2, i, j, k, pc, k > 0 ? p_at(i, j, k - 1) : 0.0f);
+    const Exist E = exist_of(G);
+    if (in_j && in_k) update_face(G, E, 0, i, j, k, pc, i > 0 ? p_at(i - 1, j, k) : 0.0f);
+    if (in_i && in_k) update_face(G, E, 1, i, j, k, pc, j > 0 ? p_at(i, j - 1, k) : 0.0f);
+    if (in_i && in_j && !G.flat) update_face(G, E, 2, i, j, k, pc, k > 0 ? p_at(i, j, k - 1) : 0.0f);
 }
 
 }  // namespace shkz
