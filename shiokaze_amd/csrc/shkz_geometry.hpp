@@ -30,22 +30,24 @@ struct GridDesc {
     int flat;            // a 2D grid (w, h) held as d = 1: solid nodes (w+1, h+1), no z faces
 };
 
+// Dense indices in 32-bit arithmetic (in-range, non-negative coordinates): every dense box holds
+// fewer than 2^32 entries (checked at entry), and a 64-bit multiply is several instructions.
 __device__ __forceinline__ size_t cell_index(const GridDesc &G, int i, int j, int k) {
-    return ((size_t)i * G.h + j) * G.d + k;
+    return ((unsigned)i * (unsigned)G.h + (unsigned)j) * (unsigned)G.d + (unsigned)k;
 }
 __device__ __forceinline__ size_t face_index(const GridDesc &G, int dim, int i, int j, int k) {
-    const size_t H = G.h + (dim == 1), D = G.d + (dim == 2);
-    return ((size_t)i * H + j) * D + k;
+    const unsigned H = G.h + (dim == 1), D = G.d + (dim == 2);
+    return ((unsigned)i * H + (unsigned)j) * D + (unsigned)k;
 }
 __device__ __forceinline__ size_t node_index(const GridDesc &G, int i, int j, int k) {
-    return ((size_t)i * (G.h + 1) + j) * (G.d + 1) + k;
+    return ((unsigned)i * (unsigned)(G.h + 1) + (unsigned)j) * (unsigned)(G.d + 1) + (unsigned)k;
 }
 __device__ __forceinline__ double phi_at(const GridDesc &G, int i, int j, int k) {
     const size_t c = cell_index(G, i, j, k);
     return G.phi64 ? ((const double *)G.phi)[c] : (double)((const float *)G.phi)[c];
 }
 __device__ __forceinline__ double solid_at(const GridDesc &G, int i, int j, int k) {
-    const size_t n = G.flat ? (size_t)i * (G.h + 1) + j : node_index(G, i, j, k);
+    const size_t n = G.flat ? (size_t)((unsigned)i * (unsigned)(G.h + 1) + (unsigned)j) : node_index(G, i, j, k);
     return G.solid64 ? ((const double *)G.solid)[n] : (double)((const float *)G.solid)[n];
 }
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
