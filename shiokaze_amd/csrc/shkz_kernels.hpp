@@ -870,6 +870,174 @@ __device__ __forceinline__ void jacobi_brick(const LevelDesc &L, const OpDesc<T>
     }
 }
 
+// ---- FP32 sweeps on cell rows: four consecutive cells (one float4 along z) per thread, 128
+// threads per brick.  Lane q of a brick holds the cells (q >> 4, (q >> 1) & 7, 4h .. 4h + 3) with
+// h = q & 1; the two lanes of a row are neighbours in the wavefront and swap their edge values, so
+// only the row's two end cells read a value across the brick's z faces.  A quarter of the load
+// instructions of the one-cell form, and 16-byte accesses.
+constexpr int ROW4_THREADS = BRICK_CELLS / 4;
+
+__device__ __forceinline__ float4 ld4(const float *__restrict__ a, long i) {
+    return *reinterpret_cast<const float4 *>(a + i);
+}
+__device__ __forceinline__ float4 ld4z(const float *__restrict__ a, long i) {
+    return i >= 0 ? ld4(a, i) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+}
+__device__ __forceinline__ void unpack4(const float4 v, float o[4]) {
+    o[0] = v.x;
+    o[1] = v.y;
+    o[2] = v.z;
+    o[3] = v.w;
+}
+
+// The six neighbour rows of lane q's cells: x and y neighbours as whole rows (index of their first
+// cell, -1 where no brick is stored), z neighbours as the row shifted by one cell.
+struct Row4Nbrs {
+    long c, ixm, ixp, iym, iyp, iz;  // iz: the end cell across the z face (k = -1 for h = 0, 8 for h = 1)
+    int h;
+};
+__device__ __forceinline__ Row4Nbrs row4_nbrs(const BrickMeta &m, int br, int q) {
+    Row4Nbrs N;
+    const int li = q >> 4, lj = (q >> 1) & 7, h = q & 1;
+    const int o = li * 64 + lj * 8 + 4 * h;
+    N.h = h;
+    N.c = (long)br * BRICK_CELLS + o;
+    N.ixm = li > 0 ? N.c - 64 : (m.nbr[NB_XM] >= 0 ? (long)m.nbr[NB_XM] * BRICK_CELLS + o + 448 : -1);
+    N.ixp = li < 7 ? N.c + 64 : (m.nbr[NB_XP] >= 0 ? (long)m.nbr[NB_XP] * BRICK_CELLS + o - 448 : -1);
+    N.iym = lj > 0 ? N.c - 8 : (m.nbr[NB_YM] >= 0 ? (long)m.nbr[NB_YM] * BRICK_CELLS + o + 56 : -1);
+    N.iyp = lj < 7 ? N.c + 8 : (m.nbr[NB_YP] >= 0 ? (long)m.nbr[NB_YP] * BRICK_CELLS + o - 56 : -1);
+    const int nz = h ? m.nbr[NB_ZP] : m.nbr[NB_ZM];
+    N.iz = nz >= 0 ? (long)nz * BRICK_CELLS + li * 64 + lj * 8 + (h ? 0 : 7) : -1;
+    return N;
+}
+
+// The values of v at the six neighbours of the lane's four cells (0 where no brick is stored).
+struct Row4Vals {
+    float c[4], xm[4], xp[4], ym[4], yp[4], zm[4], zp[4];
+};
+__device__ __forceinline__ Row4Vals row4_vals(const float *__restrict__ v, const Row4Nbrs &N) {
+    Row4Vals o;
+    const float4 c4 = ld4(v, N.c);
+    unpack4(c4, o.c);
+    unpack4(ld4z(v, N.ixm), o.xm);
+    unpack4(ld4z(v, N.ixp), o.xp);
+    unpack4(ld4z(v, N.iym), o.ym);
+    unpack4(ld4z(v, N.iyp), o.yp);
+    const float end = N.iz >= 0 ? v[N.iz] : 0.0f;
+    // lane swap: h = 0 receives the cell k = 4 of its partner, h = 1 the cell k = 3
+    const float swap = __shfl_xor(N.h ? c4.x : c4.w, 1, 64);
+    o.zm[0] = N.h ? swap : end;
+    o.zp[3] = N.h ? end : swap;
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        o.zm[j + 1] = o.c[j];
+        o.zp[j] = o.c[j + 1];
+    }
+    return o;
+}
+
+// The couplings of the lane's cells: low faces from the cell itself, high faces from the next cell
+// along each axis (0 where no brick is stored).
+struct Row4Coef {
+    float gxm[4], gym[4], gzm[4], gxp[4], gyp[4], gzp[4];
+};
+__device__ __forceinline__ Row4Coef row4_coef(const OpDesc<float> &A, const Row4Nbrs &N) {
+    Row4Coef o;
+    const float4 gz4 = ld4(A.gz, N.c);
+    unpack4(ld4(A.gx, N.c), o.gxm);
+    unpack4(ld4(A.gy, N.c), o.gym);
+    unpack4(gz4, o.gzm);
+    unpack4(ld4z(A.gx, N.ixp), o.gxp);
+    unpack4(ld4z(A.gy, N.iyp), o.gyp);
+    const float end = (N.h && N.iz >= 0) ? A.gz[N.iz] : 0.0f;  // the coupling across k = 8
+    const float swap = __shfl_xor(gz4.x, 1, 64);               // h = 0: the partner's k = 4
+#pragma unroll
+    for (int j = 0; j < 3; ++j) o.gzp[j] = o.gzm[j + 1];
+    o.gzp[3] = N.h ? end : swap;
+    return o;
+}
+
+// jacobi_brick on the lane's four cells (same per-cell arithmetic); DOT adds rhs.zout of the
+// unknowns, in cell order, to part.
+template <int DOT>
+__device__ __forceinline__ void jacobi_row4(const LevelDesc &L, const OpDesc<float> &A, const float *__restrict__ idiag,
+                                            const float *__restrict__ rhs, const float *__restrict__ zin,
+                                            float *__restrict__ zout, float omega, int br, const BrickMeta &m, int q,
+                                            double &part) {
+    const Row4Nbrs N = row4_nbrs(m, br, q);
+    const Row4Vals z = row4_vals(zin, N);
+    float r[4], out[4];
+    unpack4(ld4(rhs, N.c), r);
+    if (m.flags & BM_UNIFORM) {
+        const UniT<float> u = uni_load<float>(L, br);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const float off = u.g * z.xm[j] + u.g * z.xp[j] + u.g * z.ym[j] + u.g * z.yp[j] + u.g * z.zm[j] +
+                              u.g * z.zp[j];
+            out[j] = (1.0f - omega) * z.c[j] + omega * u.id * (r[j] + off);
+            if (DOT) part += (double)r[j] * (double)out[j];
+        }
+    } else {
+        const Row4Coef g = row4_coef(A, N);
+        float id[4];
+        unpack4(ld4(idiag, N.c), id);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const float off = g.gxm[j] * z.xm[j] + g.gxp[j] * z.xp[j] + g.gym[j] * z.ym[j] + g.gyp[j] * z.yp[j] +
+                              g.gzm[j] * z.zm[j] + g.gzp[j] * z.zp[j];
+            const float v = (1.0f - omega) * z.c[j] + omega * id[j] * (r[j] + off);
+            out[j] = id[j] > 0 ? v : 0.0f;
+            if (DOT) part += id[j] > 0 ? (double)r[j] * (double)v : 0.0;
+        }
+    }
+    *reinterpret_cast<float4 *>(zout + N.c) = make_float4(out[0], out[1], out[2], out[3]);
+}
+
+// Persistent FP32 sweep: 512-thread workgroups take four consecutive bricks per virtual block
+// (virtual blocks vb = blockIdx.x + k * gridDim.x < nvirt, nvirt a multiple of 8, XCD-preserving);
+// each wavefront works alone on half a brick (no barrier).
+template <int FINEST>
+__global__ __launch_bounds__(BLOCK) void k_jacobi4(LevelDesc L, OpDesc<float> A, const float *__restrict__ idiag,
+                                                   const float *__restrict__ rhs, const float *__restrict__ zin,
+                                                   float *__restrict__ zout, float omega, Gate Q, int nvirt) {
+    if (all_done(Q)) return;
+    const int slot = threadIdx.x / ROW4_THREADS, q = threadIdx.x % ROW4_THREADS;
+    for (int vb = blockIdx.x; vb < nvirt; vb += gridDim.x) {
+        const int br = 4 * xcd_brick(vb, nvirt) + slot;
+        if (br >= L.nbricks) continue;
+        const BrickMeta m = L.meta[br];
+        if (skip_smooth(Q, L, m)) continue;
+        double part = 0.0;
+        jacobi_row4<0>(L, A, idiag, rhs, zin, zout, omega, br, m, q, part);
+    }
+}
+
+// The finest level's last post-smoothing sweep with the rhs.z partial of each brick: one brick
+// per 128-thread workgroup.
+__global__ __launch_bounds__(ROW4_THREADS) void k_jacobi4_dot(LevelDesc L, OpDesc<float> A,
+                                                              const float *__restrict__ idiag,
+                                                              const float *__restrict__ rhs,
+                                                              const float *__restrict__ zin, float *__restrict__ zout,
+                                                              float omega, Gate Q, RedBuf R) {
+    __shared__ double lds[ROW4_THREADS / 64];
+    const int br = brick_or_idle(L);
+    if (br < 0) return;
+    const BrickMeta m = L.meta[br];
+    if (all_done(Q) || skip_smooth(Q, L, m)) return;
+    double part = 0.0;
+    jacobi_row4<1>(L, A, idiag, rhs, zin, zout, omega, br, m, threadIdx.x, part);
+    part = wave_sum(part);
+    if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = part;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double sum = 0.0;
+#pragma unroll
+        for (int w = 0; w < ROW4_THREADS / 64; ++w) sum += lds[w];
+        R.partials[2 * br + 0] = sum;
+        R.partials[2 * br + 1] = 0.0;
+    }
+}
+
 // res = rhs - A z
 template <class T, int FINEST>
 __global__ __launch_bounds__(BLOCK) void k_residual(LevelDesc L, OpDesc<T> A, const T *__restrict__ rhs,

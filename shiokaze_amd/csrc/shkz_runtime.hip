@@ -622,6 +622,35 @@ static void smooth_k(Launcher &K, int l, int k, bool from_zero, const LevelDesc 
     }
 }
 
+// One damped-Jacobi sweep zout = J(zin) of level l (dot: with the rhs.zout partials of the finest
+// level's last sweep).  FP32 levels sweep four cells per thread (k_jacobi4*), FP64 levels one.
+template <class T>
+static void jacobi_sweep(shkz_ctx *C, Launcher &K, int l, bool dot, const LevelDesc &L, OpDesc<T> A, const T *id,
+                         const T *rhs, const T *zin, T *zout, T om, Gate Q, RedBuf red) {
+    const bool f = l == 0;
+    const double es = (double)sizeof(T);
+    const char *name = dot ? "jacobi_dot_L0" : f ? "jacobi_L0" : "jacobi_Lc";
+    const unsigned nb = brick_grid(L.nbricks);
+    if constexpr (sizeof(T) == 4) {
+        if (dot) {
+            K.go(name, l, 7 * es, -usave(C, l, 4 * es), dim3(nb), dim3(ROW4_THREADS), k_jacobi4_dot, L, A, id, rhs, zin,
+                 zout, om, Q, red);
+        } else {
+            // four bricks per virtual block, a multiple of 8 of them
+            const unsigned nvirt = (unsigned)((((L.nbricks + 3) / 4) + 7) / 8 * 8);
+            K.go(name, l, 7 * es, -usave(C, l, 4 * es), dim3(nvirt), dim3(BLOCK),
+                 f ? k_jacobi4<1> : k_jacobi4<0>, L, A, id, rhs, zin, zout, om, Q, (int)nvirt);
+        }
+    } else if (dot) {
+        // one brick per workgroup: the reduction of the dot variant costs two barriers per brick
+        K.go(name, l, 7 * es, -usave(C, l, 4 * es), dim3(nb), dim3(BLOCK), k_jacobi<T, 1, 1>, L, A, id, rhs, zin, zout,
+             om, Q, red, (int)nb);
+    } else {
+        K.go(name, l, 7 * es, -usave(C, l, 4 * es), dim3(std::min<unsigned>(nb, PERSIST_BLOCKS)), dim3(BLOCK),
+             f ? k_jacobi<T, 1, 0> : k_jacobi<T, 0, 0>, L, A, id, rhs, zin, zout, om, Q, red, (int)nb);
+    }
+}
+
 // One V-cycle z = M r on the level-0 vectors of every grid not done; returns the pointer holding
 // z.  Each grid descends to its own coarsest level (one brick), solved there by k_coarse_solve;
 // the smoothing kernels skip the bricks of grids that end on their level.  With dot_partials, the
@@ -687,18 +716,14 @@ static T *vcycle(shkz_ctx *C, Launcher &K, Problem &P, const T *rhs0, T *z0, T *
             K.go(f ? "smooth_first_L0" : "smooth_first_Lc", l, 3 * es, -usave(C, l, es), g, b,
                  f ? k_smooth_first<T, 1> : k_smooth_first<T, 0>, L.desc, id, rhs, z, om, Q);
             for (int s = 1; s < pre; ++s) {
-                K.go(f ? "jacobi_L0" : "jacobi_Lc", l, 7 * es, -usave(C, l, 4 * es), gp, b,
-                     f ? k_jacobi<T, 1, 0> : k_jacobi<T, 0, 0>, L.desc, A, id, rhs, (const T *)cur, oth, om, Q, P.red,
-                     (int)g.x);
+                jacobi_sweep<T>(C, K, l, false, L.desc, A, id, rhs, (const T *)cur, oth, om, Q, P.red);
                 std::swap(cur, oth);
             }
         } else {
             K.go(f ? "smooth2_L0" : "smooth2_Lc", l, 6 * es, -usave(C, l, 4 * es), g, b, f ? k_smooth2<T, 1> : k_smooth2<T, 0>, L.desc, A,
                  id, rhs, z, om, Q);
             for (int s = 2; s < pre; ++s) {
-                K.go(f ? "jacobi_L0" : "jacobi_Lc", l, 7 * es, -usave(C, l, 4 * es), gp, b,
-                     f ? k_jacobi<T, 1, 0> : k_jacobi<T, 0, 0>, L.desc, A, id, rhs, (const T *)cur, oth, om, Q, P.red,
-                     (int)g.x);
+                jacobi_sweep<T>(C, K, l, false, L.desc, A, id, rhs, (const T *)cur, oth, om, Q, P.red);
                 std::swap(cur, oth);
             }
         }
@@ -735,14 +760,7 @@ static T *vcycle(shkz_ctx *C, Launcher &K, Problem &P, const T *rhs0, T *z0, T *
              f ? k_prolong<T, 1> : k_prolong<T, 0>, L.desc, A, (const T *)zcur[l + 1], zcur[l], Q);
         for (int s = 0; s < post; ++s) {
             const bool last_dot = f && dot_partials && s == post - 1;
-            if (last_dot)
-                // one brick per workgroup: the reduction of the dot variant costs two barriers per brick
-                K.go("jacobi_dot_L0", l, 7 * es, -usave(C, l, 4 * es), g, b, k_jacobi<T, 1, 1>, L.desc, A, id, rhs,
-                     (const T *)cur, oth, om, Q, P.red, (int)g.x);
-            else
-                K.go(f ? "jacobi_L0" : "jacobi_Lc", l, 7 * es, -usave(C, l, 4 * es), gp, b,
-                     f ? k_jacobi<T, 1, 0> : k_jacobi<T, 0, 0>, L.desc, A, id, rhs, (const T *)cur, oth, om, Q, P.red,
-                     (int)g.x);
+            jacobi_sweep<T>(C, K, l, last_dot, L.desc, A, id, rhs, (const T *)cur, oth, om, Q, P.red);
             std::swap(cur, oth);
         }
         zcur[l] = cur;
