@@ -1038,6 +1038,178 @@ __global__ __launch_bounds__(ROW4_THREADS) void k_jacobi4_dot(LevelDesc L, OpDes
     }
 }
 
+// (A v) on the lane's four cells, with the unknown mask: apply_row / apply_row_uniform per cell.
+__device__ __forceinline__ void apply_row4(const LevelDesc &L, const OpDesc<float> &A, int br, const BrickMeta &m,
+                                           const Row4Nbrs &N, const Row4Vals &v, float av[4], bool unk[4]) {
+    if (m.flags & BM_UNIFORM) {
+        const UniT<float> u = uni_load<float>(L, br);
+        const float diag = 0.0f + ((u.g + u.g) + (u.g + u.g) + (u.g + u.g));
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const float off = u.g * v.xm[j] + u.g * v.xp[j] + u.g * v.ym[j] + u.g * v.yp[j] + u.g * v.zm[j] +
+                              u.g * v.zp[j];
+            av[j] = diag * v.c[j] - off;
+            unk[j] = true;
+        }
+    } else {
+        const Row4Coef g = row4_coef(A, N);
+        float dir[4];
+        unpack4(ld4(A.dir, N.c), dir);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const float off = g.gxm[j] * v.xm[j] + g.gxp[j] * v.xp[j] + g.gym[j] * v.ym[j] + g.gyp[j] * v.yp[j] +
+                              g.gzm[j] * v.zm[j] + g.gzp[j] * v.zp[j];
+            const float diag = dir[j] + ((g.gxm[j] + g.gxp[j]) + (g.gym[j] + g.gyp[j]) + (g.gzm[j] + g.gzp[j]));
+            av[j] = diag * v.c[j] - off;
+            unk[j] = dir[j] >= 0;
+        }
+    }
+}
+
+// Brick and lane of a 512-thread workgroup that takes four consecutive bricks (block ids remapped
+// XCD-aware; gridDim.x a multiple of 8); br >= L.nbricks for an idle slot.
+__device__ __forceinline__ void row4_slot(const LevelDesc &L, int &br, int &q) {
+    br = 4 * xcd_brick(blockIdx.x, gridDim.x) + (int)(threadIdx.x / ROW4_THREADS);
+    q = threadIdx.x % ROW4_THREADS;
+}
+
+// Sum / max over the two wavefronts of a 128-thread workgroup (result valid in every thread).
+template <bool AMAX>
+__device__ __forceinline__ void row4_reduce2(double &a, double &b, double *lds) {
+    a = AMAX ? wave_max(a) : wave_sum(a);
+    b = wave_sum(b);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        lds[w] = a;
+        lds[2 + w] = b;
+    }
+    __syncthreads();
+    a = AMAX ? fmax(lds[0], lds[1]) : lds[0] + lds[1];
+    b = lds[2] + lds[3];
+}
+
+// res = rhs - A z (k_residual on rows).
+template <int FINEST>
+__global__ __launch_bounds__(BLOCK) void k_residual4(LevelDesc L, OpDesc<float> A, const float *__restrict__ rhs,
+                                                     const float *__restrict__ z, float *__restrict__ res, Gate Q) {
+    int br, q;
+    row4_slot(L, br, q);
+    if (br >= L.nbricks) return;
+    const BrickMeta m = L.meta[br];
+    if (all_done(Q) || skip_smooth(Q, L, m)) return;
+    const Row4Nbrs N = row4_nbrs(m, br, q);
+    const Row4Vals v = row4_vals(z, N);
+    float r[4], av[4], out[4];
+    bool unk[4];
+    unpack4(ld4(rhs, N.c), r);
+    apply_row4(L, A, br, m, N, v, av, unk);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) out[j] = unk[j] ? r[j] - av[j] : 0.0f;
+    *reinterpret_cast<float4 *>(res + N.c) = make_float4(out[0], out[1], out[2], out[3]);
+}
+
+// z = omega * idiag * rhs (k_smooth_first on rows).
+template <int FINEST>
+__global__ __launch_bounds__(BLOCK) void k_smooth_first4(LevelDesc L, const float *__restrict__ idiag,
+                                                         const float *__restrict__ rhs, float *__restrict__ z,
+                                                         float omega, Gate Q) {
+    int br, q;
+    row4_slot(L, br, q);
+    if (br >= L.nbricks) return;
+    const BrickMeta m = L.meta[br];
+    if (all_done(Q) || skip_smooth(Q, L, m)) return;
+    const long c = (long)br * BRICK_CELLS + 4 * q;
+    float r[4], id[4], out[4];
+    unpack4(ld4(rhs, c), r);
+    if (m.flags & BM_UNIFORM) {
+        const float u = uni_load<float>(L, br).id;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) id[j] = u;
+    } else {
+        unpack4(ld4(idiag, c), id);
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) out[j] = omega * id[j] * r[j];
+    *reinterpret_cast<float4 *>(z + c) = make_float4(out[0], out[1], out[2], out[3]);
+}
+
+// q = A p and p.q partials (k_apply_dot on rows): one brick per 128-thread workgroup.
+__global__ __launch_bounds__(ROW4_THREADS) void k_apply_dot4(LevelDesc L, OpDesc<float> A, const float *__restrict__ p,
+                                                             float *__restrict__ qv, const SolveScalars *S,
+                                                             const BatchCtl *ctl, RedBuf R) {
+    __shared__ double lds[4];
+    const int br = brick_or_idle(L);
+    if (br < 0) return;
+    const BrickMeta m = L.meta[br];
+    // a batch skips finished grids before loading anything; a single grid loads first
+    if (L.ngrids > 1 && (ctl->all_done || S[m.grid].done)) return;
+    const Row4Nbrs N = row4_nbrs(m, br, threadIdx.x);
+    const Row4Vals v = row4_vals(p, N);
+    if (ctl->all_done) return;
+    float av[4], out[4];
+    bool unk[4];
+    apply_row4(L, A, br, m, N, v, av, unk);
+    double part = 0.0, unused = 0.0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        out[j] = unk[j] ? av[j] : 0.0f;
+        part += unk[j] ? (double)v.c[j] * (double)av[j] : 0.0;
+    }
+    *reinterpret_cast<float4 *>(qv + N.c) = make_float4(out[0], out[1], out[2], out[3]);
+    row4_reduce2<false>(part, unused, lds);
+    store_partials(R, br, part, 0.0);
+}
+
+// x += alpha p, r -= alpha q with ||r||_inf and r.r partials (k_cg_update on rows).
+__global__ __launch_bounds__(ROW4_THREADS) void k_cg_update4(LevelDesc L, float *__restrict__ x, float *__restrict__ r,
+                                                             const float *__restrict__ p, const float *__restrict__ qv,
+                                                             const SolveScalars *S, const BatchCtl *ctl, RedBuf R) {
+    __shared__ double lds[4];
+    const int br = brick_or_idle(L);
+    if (br < 0) return;
+    const long c = (long)br * BRICK_CELLS + 4 * threadIdx.x;
+    if (L.ngrids > 1 && (ctl->all_done || S[grid_of(L, br)].done)) return;
+    float xv[4], pv[4], rv[4], qq[4];
+    unpack4(ld4(x, c), xv);
+    unpack4(ld4(p, c), pv);
+    unpack4(ld4(r, c), rv);
+    unpack4(ld4(qv, c), qq);
+    const SolveScalars &Sg = S[grid_of(L, br)];
+    if (ctl->all_done || Sg.done) return;
+    const float alpha = (float)Sg.alpha;
+    double mx = 0.0, sq = 0.0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        xv[j] = xv[j] + alpha * pv[j];
+        rv[j] = rv[j] - alpha * qq[j];
+        mx = fmax(mx, fabs((double)rv[j]));
+        sq += (double)rv[j] * (double)rv[j];
+    }
+    *reinterpret_cast<float4 *>(x + c) = make_float4(xv[0], xv[1], xv[2], xv[3]);
+    *reinterpret_cast<float4 *>(r + c) = make_float4(rv[0], rv[1], rv[2], rv[3]);
+    row4_reduce2<true>(mx, sq, lds);
+    store_partials(R, br, mx, sq);
+}
+
+// p = z + beta p (first: p = z), k_update_p on rows.
+__global__ __launch_bounds__(BLOCK) void k_update_p4(LevelDesc L, const float *__restrict__ z, float *__restrict__ p,
+                                                     const SolveScalars *S, const BatchCtl *ctl, int first) {
+    int br, q;
+    row4_slot(L, br, q);
+    if (br >= L.nbricks) return;
+    const long c = (long)br * BRICK_CELLS + 4 * q;
+    if (L.ngrids > 1 && (ctl->all_done || S[grid_of(L, br)].done)) return;
+    float zv[4], pv[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    unpack4(ld4(z, c), zv);
+    if (!first) unpack4(ld4(p, c), pv);
+    const SolveScalars &Sg = S[grid_of(L, br)];
+    if (ctl->all_done || Sg.done) return;
+    const float beta = (float)Sg.beta;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) pv[j] = first ? zv[j] : zv[j] + beta * pv[j];
+    *reinterpret_cast<float4 *>(p + c) = make_float4(pv[0], pv[1], pv[2], pv[3]);
+}
+
 // res = rhs - A z
 template <class T, int FINEST>
 __global__ __launch_bounds__(BLOCK) void k_residual(LevelDesc L, OpDesc<T> A, const T *__restrict__ rhs,

@@ -622,6 +622,9 @@ static void smooth_k(Launcher &K, int l, int k, bool from_zero, const LevelDesc 
     }
 }
 
+// Workgroups of the row kernels that take four bricks each (a multiple of 8 of them).
+static inline unsigned row4_grid(int nbricks) { return (unsigned)((((nbricks + 3) / 4) + 7) / 8 * 8); }
+
 // One damped-Jacobi sweep zout = J(zin) of level l (dot: with the rhs.zout partials of the finest
 // level's last sweep).  FP32 levels sweep four cells per thread (k_jacobi4*), FP64 levels one.
 template <class T>
@@ -637,7 +640,7 @@ static void jacobi_sweep(shkz_ctx *C, Launcher &K, int l, bool dot, const LevelD
                  zout, om, Q, red);
         } else {
             // four bricks per virtual block, a multiple of 8 of them
-            const unsigned nvirt = (unsigned)((((L.nbricks + 3) / 4) + 7) / 8 * 8);
+            const unsigned nvirt = row4_grid(L.nbricks);
             K.go(name, l, 7 * es, -usave(C, l, 4 * es), dim3(nvirt), dim3(BLOCK),
                  f ? k_jacobi4<1> : k_jacobi4<0>, L, A, id, rhs, zin, zout, om, Q, (int)nvirt);
         }
@@ -713,8 +716,12 @@ static T *vcycle(shkz_ctx *C, Launcher &K, Problem &P, const T *rhs0, T *z0, T *
         } else if (pre <= 1 || f) {
             // finest level: one sweep from zero, then plain sweeps (cheaper there than the fused
             // two-sweep kernel, whose on-the-fly neighbour values cost 12 extra loads per cell)
-            K.go(f ? "smooth_first_L0" : "smooth_first_Lc", l, 3 * es, -usave(C, l, es), g, b,
-                 f ? k_smooth_first<T, 1> : k_smooth_first<T, 0>, L.desc, id, rhs, z, om, Q);
+            if constexpr (sizeof(T) == 4)
+                K.go(f ? "smooth_first_L0" : "smooth_first_Lc", l, 3 * es, -usave(C, l, es), dim3(row4_grid(L.nbricks)),
+                     b, f ? k_smooth_first4<1> : k_smooth_first4<0>, L.desc, id, rhs, z, om, Q);
+            else
+                K.go(f ? "smooth_first_L0" : "smooth_first_Lc", l, 3 * es, -usave(C, l, es), g, b,
+                     f ? k_smooth_first<T, 1> : k_smooth_first<T, 0>, L.desc, id, rhs, z, om, Q);
             for (int s = 1; s < pre; ++s) {
                 jacobi_sweep<T>(C, K, l, false, L.desc, A, id, rhs, (const T *)cur, oth, om, Q, P.red);
                 std::swap(cur, oth);
@@ -727,9 +734,14 @@ static T *vcycle(shkz_ctx *C, Launcher &K, Problem &P, const T *rhs0, T *z0, T *
                 std::swap(cur, oth);
             }
         }
-        if (!res_done)
-            K.go(f ? "residual_L0" : "residual_Lc", l, 7 * es, -usave(C, l, 4 * es), g, b,
-                 f ? k_residual<T, 1> : k_residual<T, 0>, L.desc, A, rhs, (const T *)cur, (T *)L.res, Q);
+        if (!res_done) {
+            if constexpr (sizeof(T) == 4)
+                K.go(f ? "residual_L0" : "residual_Lc", l, 7 * es, -usave(C, l, 4 * es), dim3(row4_grid(L.nbricks)), b,
+                     f ? k_residual4<1> : k_residual4<0>, L.desc, A, rhs, (const T *)cur, (T *)L.res, Q);
+            else
+                K.go(f ? "residual_L0" : "residual_Lc", l, 7 * es, -usave(C, l, 4 * es), g, b,
+                     f ? k_residual<T, 1> : k_residual<T, 0>, L.desc, A, rhs, (const T *)cur, (T *)L.res, Q);
+        }
         LevelHost &Lc = C->levels[l + 1];
         K.go(f ? "restrict_L0" : "restrict_Lc", l, es, 2 * es * (double)Lc.unknowns, dim3(brick_grid(Lc.nbricks)), b,
              f ? k_restrict<T, 1> : k_restrict<T, 0>, Lc.desc, op_of<T>(Lc), (const T *)L.res, (T *)Lc.rhs, Q);
@@ -854,6 +866,15 @@ static int krylov(shkz_ctx *C, Launcher &K, Problem &P, T *x, T *r, T *p, T *q, 
     const OpDesc<T> A = op_of<T>(L0);
     const double es = (double)sizeof(T);
     const Gate Q{ctl, S};
+    // FP32: the row kernels (four cells per thread)
+    auto update_p = [&](double bytes, const T *zv, int first) {
+        if constexpr (sizeof(T) == 4)
+            K.go("update_p", 0, bytes, 0, dim3(row4_grid(L0.nbricks)), b, k_update_p4, L0.desc, zv, p,
+                 (const SolveScalars *)S, (const BatchCtl *)ctl, first);
+        else
+            K.go("update_p", 0, bytes, 0, g, b, k_update_p<T>, L0.desc, zv, p, (const SolveScalars *)S,
+                 (const BatchCtl *)ctl, first);
+    };
     // the finest level's last post-smoothing sweep forms r.z when there is one
     const bool fused_dot = precond && P.nlev > 1 && P.post > 0;
     // first direction
@@ -863,15 +884,22 @@ static int krylov(shkz_ctx *C, Launcher &K, Problem &P, T *x, T *r, T *p, T *q, 
         K.go("dot_rz", 0, 2 * es, 0, g, b, k_dot_rz<T>, L0.desc, (const T *)r, zp, (const SolveScalars *)S,
              (const BatchCtl *)ctl, P.red);
     finalize(K, P, L0.desc, S, ctl, FIN_RZ_FIRST);
-    K.go("update_p", 0, 2 * es, 0, g, b, k_update_p<T>, L0.desc, zp, p, (const SolveScalars *)S, (const BatchCtl *)ctl,
-         1);
+    update_p(2 * es, zp, 1);
 
     auto iteration = [&]() {
-        K.go("apply_dot", 0, 6 * es, -usave(C, 0, 4 * es), g, b, k_apply_dot<T>, L0.desc, A, (const T *)p, q, (const SolveScalars *)S,
-             (const BatchCtl *)ctl, P.red);
+        if constexpr (sizeof(T) == 4)
+            K.go("apply_dot", 0, 6 * es, -usave(C, 0, 4 * es), g, dim3(ROW4_THREADS), k_apply_dot4, L0.desc, A,
+                 (const T *)p, q, (const SolveScalars *)S, (const BatchCtl *)ctl, P.red);
+        else
+            K.go("apply_dot", 0, 6 * es, -usave(C, 0, 4 * es), g, b, k_apply_dot<T>, L0.desc, A, (const T *)p, q,
+                 (const SolveScalars *)S, (const BatchCtl *)ctl, P.red);
         finalize(K, P, L0.desc, S, ctl, FIN_PQ);
-        K.go("cg_update", 0, 6 * es, 0, g, b, k_cg_update<T>, L0.desc, x, r, (const T *)p, (const T *)q,
-             (const SolveScalars *)S, (const BatchCtl *)ctl, P.red);
+        if constexpr (sizeof(T) == 4)
+            K.go("cg_update", 0, 6 * es, 0, g, dim3(ROW4_THREADS), k_cg_update4, L0.desc, x, r, (const T *)p,
+                 (const T *)q, (const SolveScalars *)S, (const BatchCtl *)ctl, P.red);
+        else
+            K.go("cg_update", 0, 6 * es, 0, g, b, k_cg_update<T>, L0.desc, x, r, (const T *)p, (const T *)q,
+                 (const SolveScalars *)S, (const BatchCtl *)ctl, P.red);
         finalize(K, P, L0.desc, S, ctl, precond ? FIN_UPDATE : FIN_UPDATE_CG);
         if (precond) {
             const T *zz = vcycle<T>(C, K, P, r, z, tmp, Q, fused_dot);
@@ -879,11 +907,9 @@ static int krylov(shkz_ctx *C, Launcher &K, Problem &P, T *x, T *r, T *p, T *q, 
                 K.go("dot_rz", 0, 2 * es, 0, g, b, k_dot_rz<T>, L0.desc, (const T *)r, zz, (const SolveScalars *)S,
                      (const BatchCtl *)ctl, P.red);
             finalize(K, P, L0.desc, S, ctl, FIN_RZ);
-            K.go("update_p", 0, 3 * es, 0, g, b, k_update_p<T>, L0.desc, zz, p, (const SolveScalars *)S,
-                 (const BatchCtl *)ctl, 0);
+            update_p(3 * es, zz, 0);
         } else {
-            K.go("update_p", 0, 3 * es, 0, g, b, k_update_p<T>, L0.desc, (const T *)r, p, (const SolveScalars *)S,
-                 (const BatchCtl *)ctl, 0);
+            update_p(3 * es, (const T *)r, 0);
         }
     };
     hipGraphExec_t exec = nullptr;
