@@ -121,7 +121,7 @@ def kernel_roofline(stats, inst_steps, config):
     total_bytes = sum(v["bytes"] for v in kern.values())
     roofline["all_kernels_gbs"] = round(total_bytes / (total_ms * 1e-3) / 1e9, 1)
     log("[bench] kernel time share:")
-    for k, v in sorted(kern.items(), key=lambda kv: -kv[1]["total_ms"])[:14]:
+    for k, v in sorted(kern.items(), key=lambda kv: -kv[1]["total_ms"])[:32]:
         gbs = v["bytes"] / (v["total_ms"] * 1e-3) / 1e9 if v["total_ms"] else 0
         log(f"    {k:18s} {v['total_ms'] / inst_steps:8.3f} ms/step  {v['launches'] / inst_steps:7.1f} launches/step"
             f"  {gbs:8.1f} GB/s")

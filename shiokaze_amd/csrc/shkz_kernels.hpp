@@ -1350,6 +1350,83 @@ __global__ __launch_bounds__(BLOCK) void k_prolong(LevelDesc Lf, OpDesc<T> Af, c
     zf[c] = zf0 + v;
 }
 
+// k_prolong on cell rows (FP32): 512-thread workgroups take four fine bricks, each staging its own
+// 6^3 coarse tile; lane q adds the correction to its four cells with k_prolong's arithmetic (cells
+// that are not unknowns are written back unchanged).
+template <int FINEST>
+__global__ __launch_bounds__(BLOCK) void k_prolong4(LevelDesc Lf, OpDesc<float> Af, const float *__restrict__ zc,
+                                                    float *__restrict__ zf, Gate Q) {
+    constexpr int E = 6, NT = E * E * E;
+    __shared__ float tile[BLOCK / ROW4_THREADS][NT];
+    __shared__ int bid[BLOCK / ROW4_THREADS][8];
+    if (all_done(Q)) return;
+    int br, q;
+    row4_slot(Lf, br, q);
+    const int slot = threadIdx.x / ROW4_THREADS;
+    // every thread reaches the barriers; a slot past the end or of a skipped brick only idles
+    bool work = br < Lf.nbricks;
+    BrickMeta m{};
+    if (work) {
+        m = Lf.meta[br];
+        work = !skip_smooth(Q, Lf, m);
+    }
+    const long c = (long)br * BRICK_CELLS + 4 * q;
+    float z0[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    int px = 0, py = 0, pz = 0;
+    if (work) {
+        unpack4(ld4(zf, c), z0);
+        px = Lf.bxyz[3 * br] & 1;
+        py = Lf.bxyz[3 * br + 1] & 1;
+        pz = Lf.bxyz[3 * br + 2] & 1;
+        if (q < 8) bid[slot][q] = Lf.parent8[8 * br + q];
+    }
+    __syncthreads();
+    // with B = 2M + p, tile offset a is coarse cell 8M + 4p - 1 + a: coarse brick
+    // M - 1 + p + slot with slot = ((4p + 7 + a) >> 3) - p, local (4p + 7 + a) & 7
+    if (work) {
+        float v[2];
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            const int e = q + u * ROW4_THREADS;
+            v[u] = 0.0f;
+            if (e < NT) {
+                const int cc = e % E, b = (e / E) % E, a = e / (E * E);
+                const int ia = 4 * px + 7 + a, ib = 4 * py + 7 + b, ic = 4 * pz + 7 + cc;
+                const int s8 = (((ia >> 3) - px) * 2 + ((ib >> 3) - py)) * 2 + ((ic >> 3) - pz);
+                const int id = bid[slot][s8];
+                if (id >= 0) v[u] = zc[(long)id * BRICK_CELLS + (((ia & 7) * 8 + (ib & 7)) * 8 + (ic & 7))];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            const int e = q + u * ROW4_THREADS;
+            if (e < NT) tile[slot][e] = v[u];
+        }
+    }
+    __syncthreads();
+    if (!work) return;
+    float dir[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    const bool uni = m.flags & BM_UNIFORM;
+    if (!uni) unpack4(ld4(Af.dir, c), dir);
+    const int li = q >> 4, lj = (q >> 1) & 7, h = q & 1;
+    const int a0 = (li >> 1) + 1, b0 = (lj >> 1) + 1;
+    const int a1 = (li & 1) ? a0 + 1 : a0 - 1, b1 = (lj & 1) ? b0 + 1 : b0 - 1;
+    auto at = [&](int a, int b, int cc) { return tile[slot][(a * E + b) * E + cc]; };
+    const float n = 0.75f, f = 0.25f;
+    // a flat (2D) level has one cell layer: weight 1 on it, none on the empty neighbour layer
+    const float nk = Lf.flat ? 1.0f : n, fk = Lf.flat ? 0.0f : f;
+    float out[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int lk = 4 * h + j;
+        const int c0 = (lk >> 1) + 1, c1 = (lk & 1) ? c0 + 1 : c0 - 1;
+        const float v = n * (n * (nk * at(a0, b0, c0) + fk * at(a0, b0, c1)) + f * (nk * at(a0, b1, c0) + fk * at(a0, b1, c1))) +
+                        f * (n * (nk * at(a1, b0, c0) + fk * at(a1, b0, c1)) + f * (nk * at(a1, b1, c0) + fk * at(a1, b1, c1)));
+        out[j] = (uni || dir[j] >= 0) ? z0[j] + v : z0[j];
+    }
+    *reinterpret_cast<float4 *>(zf + c) = make_float4(out[0], out[1], out[2], out[3]);
+}
+
 // K damped-Jacobi sweeps in one launch (coarse levels, where every launch is latency-bound).  The
 // block of brick B stages the cube of cells within distance K of B (bricks B-1 .. B+1 per axis,
 // found through the grid's brick map) and runs all K sweeps on it in LDS; values near the cube's

@@ -768,8 +768,13 @@ static T *vcycle(shkz_ctx *C, Launcher &K, Problem &P, const T *rhs0, T *z0, T *
             zoth[l] = oth;
             continue;
         }
-        K.go(f ? "prolong_L0" : "prolong_Lc", l, 3 * es, es * (double)Lc.unknowns - usave(C, l, es), g, b,
-             f ? k_prolong<T, 1> : k_prolong<T, 0>, L.desc, A, (const T *)zcur[l + 1], zcur[l], Q);
+        if constexpr (sizeof(T) == 4)
+            K.go(f ? "prolong_L0" : "prolong_Lc", l, 3 * es, es * (double)Lc.unknowns - usave(C, l, es),
+                 dim3(row4_grid(L.nbricks)), b, f ? k_prolong4<1> : k_prolong4<0>, L.desc, A, (const T *)zcur[l + 1],
+                 zcur[l], Q);
+        else
+            K.go(f ? "prolong_L0" : "prolong_Lc", l, 3 * es, es * (double)Lc.unknowns - usave(C, l, es), g, b,
+                 f ? k_prolong<T, 1> : k_prolong<T, 0>, L.desc, A, (const T *)zcur[l + 1], zcur[l], Q);
         for (int s = 0; s < post; ++s) {
             const bool last_dot = f && dot_partials && s == post - 1;
             jacobi_sweep<T>(C, K, l, last_dot, L.desc, A, id, rhs, (const T *)cur, oth, om, Q, P.red);
