@@ -993,17 +993,17 @@ __device__ __forceinline__ void jacobi_row4(const LevelDesc &L, const OpDesc<flo
     *reinterpret_cast<float4 *>(zout + N.c) = make_float4(out[0], out[1], out[2], out[3]);
 }
 
-// Persistent FP32 sweep: 512-thread workgroups take four consecutive bricks per virtual block
-// (virtual blocks vb = blockIdx.x + k * gridDim.x < nvirt, nvirt a multiple of 8, XCD-preserving);
-// each wavefront works alone on half a brick (no barrier).
+// FP32 sweep: workgroups take blockDim.x / 128 consecutive bricks per virtual block (virtual blocks
+// vb = blockIdx.x + k * gridDim.x < nvirt, nvirt a multiple of 8, XCD-preserving); each wavefront
+// works alone on half a brick (no barrier).
 template <int FINEST>
 __global__ __launch_bounds__(BLOCK) void k_jacobi4(LevelDesc L, OpDesc<float> A, const float *__restrict__ idiag,
                                                    const float *__restrict__ rhs, const float *__restrict__ zin,
                                                    float *__restrict__ zout, float omega, Gate Q, int nvirt) {
     if (all_done(Q)) return;
-    const int slot = threadIdx.x / ROW4_THREADS, q = threadIdx.x % ROW4_THREADS;
+    const int bpb = blockDim.x / ROW4_THREADS, slot = threadIdx.x / ROW4_THREADS, q = threadIdx.x % ROW4_THREADS;
     for (int vb = blockIdx.x; vb < nvirt; vb += gridDim.x) {
-        const int br = 4 * xcd_brick(vb, nvirt) + slot;
+        const int br = bpb * xcd_brick(vb, nvirt) + slot;
         if (br >= L.nbricks) continue;
         const BrickMeta m = L.meta[br];
         if (skip_smooth(Q, L, m)) continue;
@@ -1066,10 +1066,10 @@ __device__ __forceinline__ void apply_row4(const LevelDesc &L, const OpDesc<floa
     }
 }
 
-// Brick and lane of a 512-thread workgroup that takes four consecutive bricks (block ids remapped
+// Brick and lane of a workgroup that takes blockDim.x / 128 consecutive bricks (block ids remapped
 // XCD-aware; gridDim.x a multiple of 8); br >= L.nbricks for an idle slot.
 __device__ __forceinline__ void row4_slot(const LevelDesc &L, int &br, int &q) {
-    br = 4 * xcd_brick(blockIdx.x, gridDim.x) + (int)(threadIdx.x / ROW4_THREADS);
+    br = (int)(blockDim.x / ROW4_THREADS) * xcd_brick(blockIdx.x, gridDim.x) + (int)(threadIdx.x / ROW4_THREADS);
     q = threadIdx.x % ROW4_THREADS;
 }
 
@@ -1563,6 +1563,23 @@ __global__ __launch_bounds__(1024) void k_smooth_k(LevelDesc L, OpDesc<T> A, con
         zs[0][e] = z0;
     }
     __syncthreads();
+    // each thread keeps its cells' six couplings and own value in registers: a sweep reads only
+    // the six neighbour values from LDS
+    T gxm[CPT], gxp[CPT], gym[CPT], gyp[CPT], gzm[CPT], gzp[CPT], zr[CPT];
+#pragma unroll
+    for (int q = 0; q < CPT; ++q) {
+        const int e = t + q * C::THREADS;
+        gxm[q] = gxp[q] = gym[q] = gyp[q] = gzm[q] = gzp[q] = zr[q] = T(0);
+        if (e >= NR) continue;
+        const int a = e / (R * R), b = (e / R) % R, c = e % R;
+        gxm[q] = gxs[e];
+        gym[q] = gys[e];
+        gzm[q] = gzs[e];
+        gxp[q] = a < R - 1 ? gxs[e + R * R] : T(0);
+        gyp[q] = b < R - 1 ? gys[e + R] : T(0);
+        gzp[q] = c < R - 1 ? gzs[e + 1] : T(0);
+        zr[q] = zs[0][e];
+    }
 #pragma unroll 1
     for (int s = 0; s < K; ++s) {
         const T *zc = zs[s & 1];
@@ -1577,12 +1594,11 @@ __global__ __launch_bounds__(1024) void k_smooth_k(LevelDesc L, OpDesc<T> A, con
                 const T zxm = a > 0 ? zc[e - R * R] : T(0), zxp = a < R - 1 ? zc[e + R * R] : T(0);
                 const T zym = b > 0 ? zc[e - R] : T(0), zyp = b < R - 1 ? zc[e + R] : T(0);
                 const T zzm = c > 0 ? zc[e - 1] : T(0), zzp = c < R - 1 ? zc[e + 1] : T(0);
-                const T gxp = a < R - 1 ? gxs[e + R * R] : T(0), gyp = b < R - 1 ? gys[e + R] : T(0),
-                        gzp = c < R - 1 ? gzs[e + 1] : T(0);
-                const T off = gxs[e] * zxm + gxp * zxp + gys[e] * zym + gyp * zyp + gzs[e] * zzm + gzp * zzp;
-                out = (T(1) - omega) * zc[e] + omega * rid[q] * (rr[q] + off);
+                const T off = gxm[q] * zxm + gxp[q] * zxp + gym[q] * zym + gyp[q] * zyp + gzm[q] * zzm + gzp[q] * zzp;
+                out = (T(1) - omega) * zr[q] + omega * rid[q] * (rr[q] + off);
             }
             zn[e] = out;
+            zr[q] = out;
         }
         __syncthreads();
     }

@@ -624,6 +624,15 @@ static void smooth_k(Launcher &K, int l, int k, bool from_zero, const LevelDesc 
 
 // Workgroups of the row kernels that take four bricks each (a multiple of 8 of them).
 static inline unsigned row4_grid(int nbricks) { return (unsigned)((((nbricks + 3) / 4) + 7) / 8 * 8); }
+// Launch shape of the row kernels on a level: four bricks per 512-thread workgroup on the finest
+// level, one brick per 128-thread workgroup on the (smaller) coarse levels, for more workgroups.
+struct Row4Launch {
+    dim3 grid, block;
+};
+static inline Row4Launch row4_launch(int nbricks, bool finest) {
+    if (finest) return Row4Launch{dim3(row4_grid(nbricks)), dim3(BLOCK)};
+    return Row4Launch{dim3(brick_grid(nbricks)), dim3(ROW4_THREADS)};
+}
 
 // One damped-Jacobi sweep zout = J(zin) of level l (dot: with the rhs.zout partials of the finest
 // level's last sweep).  FP32 levels sweep four cells per thread (k_jacobi4*), FP64 levels one.
@@ -639,10 +648,9 @@ static void jacobi_sweep(shkz_ctx *C, Launcher &K, int l, bool dot, const LevelD
             K.go(name, l, 7 * es, -usave(C, l, 4 * es), dim3(nb), dim3(ROW4_THREADS), k_jacobi4_dot, L, A, id, rhs, zin,
                  zout, om, Q, red);
         } else {
-            // four bricks per virtual block, a multiple of 8 of them
-            const unsigned nvirt = row4_grid(L.nbricks);
-            K.go(name, l, 7 * es, -usave(C, l, 4 * es), dim3(nvirt), dim3(BLOCK),
-                 f ? k_jacobi4<1> : k_jacobi4<0>, L, A, id, rhs, zin, zout, om, Q, (int)nvirt);
+            const Row4Launch R4 = row4_launch(L.nbricks, f);
+            K.go(name, l, 7 * es, -usave(C, l, 4 * es), R4.grid, R4.block, f ? k_jacobi4<1> : k_jacobi4<0>, L, A, id,
+                 rhs, zin, zout, om, Q, (int)R4.grid.x);
         }
     } else if (dot) {
         // one brick per workgroup: the reduction of the dot variant costs two barriers per brick
@@ -717,8 +725,9 @@ static T *vcycle(shkz_ctx *C, Launcher &K, Problem &P, const T *rhs0, T *z0, T *
             // finest level: one sweep from zero, then plain sweeps (cheaper there than the fused
             // two-sweep kernel, whose on-the-fly neighbour values cost 12 extra loads per cell)
             if constexpr (sizeof(T) == 4)
-                K.go(f ? "smooth_first_L0" : "smooth_first_Lc", l, 3 * es, -usave(C, l, es), dim3(row4_grid(L.nbricks)),
-                     b, f ? k_smooth_first4<1> : k_smooth_first4<0>, L.desc, id, rhs, z, om, Q);
+                K.go(f ? "smooth_first_L0" : "smooth_first_Lc", l, 3 * es, -usave(C, l, es),
+                     row4_launch(L.nbricks, f).grid, row4_launch(L.nbricks, f).block,
+                     f ? k_smooth_first4<1> : k_smooth_first4<0>, L.desc, id, rhs, z, om, Q);
             else
                 K.go(f ? "smooth_first_L0" : "smooth_first_Lc", l, 3 * es, -usave(C, l, es), g, b,
                      f ? k_smooth_first<T, 1> : k_smooth_first<T, 0>, L.desc, id, rhs, z, om, Q);
@@ -736,8 +745,9 @@ static T *vcycle(shkz_ctx *C, Launcher &K, Problem &P, const T *rhs0, T *z0, T *
         }
         if (!res_done) {
             if constexpr (sizeof(T) == 4)
-                K.go(f ? "residual_L0" : "residual_Lc", l, 7 * es, -usave(C, l, 4 * es), dim3(row4_grid(L.nbricks)), b,
-                     f ? k_residual4<1> : k_residual4<0>, L.desc, A, rhs, (const T *)cur, (T *)L.res, Q);
+                K.go(f ? "residual_L0" : "residual_Lc", l, 7 * es, -usave(C, l, 4 * es), row4_launch(L.nbricks, f).grid,
+                     row4_launch(L.nbricks, f).block, f ? k_residual4<1> : k_residual4<0>, L.desc, A, rhs,
+                     (const T *)cur, (T *)L.res, Q);
             else
                 K.go(f ? "residual_L0" : "residual_Lc", l, 7 * es, -usave(C, l, 4 * es), g, b,
                      f ? k_residual<T, 1> : k_residual<T, 0>, L.desc, A, rhs, (const T *)cur, (T *)L.res, Q);
@@ -770,8 +780,8 @@ static T *vcycle(shkz_ctx *C, Launcher &K, Problem &P, const T *rhs0, T *z0, T *
         }
         if constexpr (sizeof(T) == 4)
             K.go(f ? "prolong_L0" : "prolong_Lc", l, 3 * es, es * (double)Lc.unknowns - usave(C, l, es),
-                 dim3(row4_grid(L.nbricks)), b, f ? k_prolong4<1> : k_prolong4<0>, L.desc, A, (const T *)zcur[l + 1],
-                 zcur[l], Q);
+                 row4_launch(L.nbricks, f).grid, row4_launch(L.nbricks, f).block, f ? k_prolong4<1> : k_prolong4<0>,
+                 L.desc, A, (const T *)zcur[l + 1], zcur[l], Q);
         else
             K.go(f ? "prolong_L0" : "prolong_Lc", l, 3 * es, es * (double)Lc.unknowns - usave(C, l, es), g, b,
                  f ? k_prolong<T, 1> : k_prolong<T, 0>, L.desc, A, (const T *)zcur[l + 1], zcur[l], Q);
