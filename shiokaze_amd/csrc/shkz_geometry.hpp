@@ -28,6 +28,8 @@ struct GridDesc {
     const uint8_t *active[3];
     uint8_t *active_out[3]; // optional: activity after the update (nullptr = not wanted)
     int flat;            // a 2D grid (w, h) held as d = 1: solid nodes (w+1, h+1), no z faces
+    int liquid;          // whether the liquid level set has a negative value: 0 = read exist[0],
+                         // 1 = no, 2 = yes (known on the host once the labelling has run)
 };
 
 // Dense indices in 32-bit arithmetic (in-range, non-negative coordinates): every dense box holds
@@ -52,12 +54,17 @@ __device__ __forceinline__ double solid_at(const GridDesc &G, int i, int j, int 
 }
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
-// The grid's two "has a negative value" flags (liquid, solid), read once per thread: the face
+// The grid's "has a negative value" flags (liquid, solid), read once per thread: the face
 // functions would otherwise reload them for every face, behind every store that might alias them.
+// The solid flag is always set: with no negative solid sample, every face's marching-squares
+// area is exactly 1 (no vertex, no crossing: a zero sum), the value of the reference's no-solid
+// branch, so the solid branch runs whenever a solid array is given and no scan of it is needed.
 struct Exist {
     bool liquid, solid;
 };
-__device__ __forceinline__ Exist exist_of(const GridDesc &G) { return Exist{G.exist[0] != 0, G.exist[1] != 0}; }
+__device__ __forceinline__ Exist exist_of(const GridDesc &G) {
+    return Exist{G.liquid ? G.liquid == 2 : G.exist[0] != 0, true};
+}
 
 // (i, j, k) of flat index e of a z-fastest (ni, nj, nk) box, in 32-bit arithmetic: every dense box
 // here holds fewer than 2^32 entries (checked at entry), and 64-bit division has no hardware

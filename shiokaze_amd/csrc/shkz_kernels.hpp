@@ -7,7 +7,7 @@
 //   k_flags            unknown labelling (:121-156) + which 8^3 bricks hold unknowns
 //   k_operator         matrix-free assembly (:159-199): conductances, Dirichlet sums, rhs
 //   k_coarsen, k_idiag multigrid operators (no counterpart: they stand in for MIC(0))
-//   k_defect ... k_update_p, V-cycle kernels
+//   k_defect4 ... k_update_p, V-cycle kernels
 //                      Krylov solve (stands in for PCGSolver::solve, pcg_solver.h:246-295)
 //   k_scatter_pressure m_pressure (:245-248)
 //   k_velocity_update  velocity update (:252-268)
@@ -126,15 +126,44 @@ __global__ void k_reset_ints(int *a, int n) {
     if (i < n) a[i] = 0;
 }
 
-// flag |= 1 when any of the n values is negative.
+// flag |= 1 when any of the n values is negative.  A 16-byte aligned array is read 16 bytes per
+// load, four loads in flight per thread.
 __global__ void k_any_negative(const void *a, int is64, size_t n, int *flag) {
     int found = 0;
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-        const double v = is64 ? ((const double *)a)[i] : (double)((const float *)a)[i];
-        found |= (v < 0.0);
+    const size_t tid = (size_t)blockIdx.x * blockDim.x + threadIdx.x, nt = (size_t)gridDim.x * blockDim.x;
+    if (((uintptr_t)a & 15) == 0) {
+        const size_t per = is64 ? 2 : 4;  // values per 16 bytes
+        const size_t nv = n / per;
+        const float4 *v4 = (const float4 *)a;
+        auto neg = [&](const float4 x) -> int {
+            if (is64) {
+                const double2 d = *reinterpret_cast<const double2 *>(&x);
+                return (d.x < 0.0) | (d.y < 0.0);
+            }
+            return (x.x < 0.0f) | (x.y < 0.0f) | (x.z < 0.0f) | (x.w < 0.0f);
+        };
+        size_t i = tid;
+        for (; i + 3 * nt < nv; i += 4 * nt) {
+            float4 x[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) x[u] = v4[i + u * nt];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) found |= neg(x[u]);
+        }
+        for (; i < nv; i += nt) found |= neg(v4[i]);
+        for (size_t j = nv * per + tid; j < n; j += nt)
+            found |= is64 ? (((const double *)a)[j] < 0.0) : (((const float *)a)[j] < 0.0f);
+    } else {
+        for (size_t i = tid; i < n; i += nt) {
+            const double v = is64 ? ((const double *)a)[i] : (double)((const float *)a)[i];
+            found |= (v < 0.0);
+        }
     }
     found = __syncthreads_or(found);
-    if (threadIdx.x == 0 && found) atomicOr(flag, 1);
+    // the flag only ever goes from 0 to 1: a plain store, skipped once it is seen set (thousands of
+    // blocks finding a negative value would otherwise queue read-modify-writes on one address)
+    if (threadIdx.x == 0 && found && __hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0)
+        __hip_atomic_store(flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // 7-point Laplacian of the level set with clamped neighbours (macpressuresolver3.cpp:92-100);
@@ -197,18 +226,21 @@ __global__ void k_surface_tension(GridDesc G, int dim, const float *curv, double
 
 // Unknown labelling (macpressuresolver3.cpp:121-156): a liquid cell is an unknown when an
 // in-bounds liquid neighbour shares a face with nonzero area (rho is 1 between two liquid
-// cells).  One block per brick position of the dense grid; writes cell flags
-// (bit0 liquid, bit1 unknown) and whether the brick holds an unknown.
+// cells).  One block per brick position of the dense grid; writes cell flags (bit0 liquid, bit1
+// unknown) and per brick bit0 = holds an unknown, bit1 = holds a liquid cell (so the host learns
+// levelset_exist() of the liquid, array_utility3.h:112-122, on the way).  The face fractions here
+// are only taken beside a liquid cell, so the liquid exists.
 __global__ __launch_bounds__(BLOCK) void k_flags(GridDesc G, int nbx, int nby, int nbz, uint8_t *cflag, uint8_t *bflag) {
     const int bid = xcd_brick(blockIdx.x, gridDim.x);  // each XCD takes a contiguous slab
     if (bid >= nbx * nby * nbz) return;
     const int bz = bid % nbz, by = (bid / nbz) % nby, bx = bid / (nbz * nby);
     const int t = threadIdx.x;
     const int i = bx * BRICK + (t >> 6), j = by * BRICK + ((t >> 3) & 7), k = bz * BRICK + (t & 7);
-    int unknown = 0;
-    const Exist E = exist_of(G);
+    int unknown = 0, any_liquid = 0;
+    const Exist E{true, true};
     if (i < G.w && j < G.h && k < G.d) {
         const bool liquid = phi_at(G, i, j, k) < 0.0;
+        any_liquid = liquid;
         if (liquid) {
             const int di[6] = {1, -1, 0, 0, 0, 0}, dj[6] = {0, 0, 1, -1, 0, 0}, dk[6] = {0, 0, 0, 0, 1, -1};
             for (int nq = 0; nq < 6 && !unknown; ++nq) {
@@ -222,8 +254,17 @@ __global__ __launch_bounds__(BLOCK) void k_flags(GridDesc G, int nbx, int nby, i
         }
         cflag[cell_index(G, i, j, k)] = (uint8_t)((liquid ? 1 : 0) | (unknown ? 2 : 0));
     }
-    unknown = __syncthreads_or(unknown);
-    if (t == 0) bflag[bid] = (uint8_t)unknown;
+    // both bits with one barrier: per-wavefront ballots combined in LDS
+    __shared__ int wbits[BLOCK / 64];
+    const bool wu = __ballot(unknown) != 0, wl = __ballot(any_liquid) != 0;
+    if ((t & 63) == 0) wbits[t >> 6] = (wu ? 1 : 0) | (wl ? 2 : 0);
+    __syncthreads();
+    if (t == 0) {
+        int v = 0;
+#pragma unroll
+        for (int w = 0; w < BLOCK / 64; ++w) v |= wbits[w];
+        bflag[bid] = (uint8_t)v;
+    }
 }
 
 // Matrix-free assembly on the stored bricks of one grid at level 0 (macpressuresolver3.cpp:159-199).
@@ -418,53 +459,6 @@ __global__ __launch_bounds__(BLOCK) void k_brick_counts(LevelDesc L, const T *__
 // Grid of brick br (no load in a batch of one).
 __device__ __forceinline__ int grid_of(const LevelDesc &L, int br) { return L.ngrids > 1 ? L.meta[br].grid : 0; }
 
-// r = b - A x with per-brick ||r||_inf and ||r||^2 partials (finalized by FIN_DEFECT*).
-template <class T>
-__global__ __launch_bounds__(BLOCK) void k_defect(LevelDesc L, OpDesc<T> A, const T *__restrict__ b,
-                                                  const T *__restrict__ x, T *__restrict__ r, RedBuf R) {
-    __shared__ double lds[32];
-    const int br = brick_or_idle(L);
-    if (br < 0) return;
-    const int t = threadIdx.x;
-    const long c = (long)br * BRICK_CELLS + t;
-    const BrickMeta m = L.meta[br];
-    T out = 0;
-    if (m.flags & BM_UNIFORM) {
-        const UniT<T> u = uni_load<T>(L, br);
-        long idx[6];
-        nb_index(nbrs_of(m, br), t, idx);
-        out = b[c] - apply_row_uniform(u.g, x, c, idx, x[c]);
-    } else {
-        const T dirc = A.dir[c];
-        if (dirc >= 0) {
-            long idx[6];
-            nb_index(nbrs_of(m, br), t, idx);
-            T diag;
-            out = b[c] - apply_row(A, x, c, idx, dirc, x[c], diag);
-        }
-    }
-    r[c] = out;
-    double mx = fabs((double)out), sq = (double)out * (double)out;
-    block_reduce2<true>(mx, sq, lds);
-    store_partials(R, br, mx, sq);
-}
-
-// Inner (FP32) solve set-up of the defect correction: r32 = (float) r64, c = 0, with ||r32||
-// partials (finalized by FIN_INNER_INIT).
-__global__ __launch_bounds__(BLOCK) void k_inner_init(LevelDesc L, const double *__restrict__ r64, float *__restrict__ r32,
-                                                      float *__restrict__ corr, RedBuf R) {
-    __shared__ double lds[32];
-    const int br = brick_or_idle(L);
-    if (br < 0) return;
-    const long c = (long)br * BRICK_CELLS + threadIdx.x;
-    const float v = (float)r64[c];
-    r32[c] = v;
-    corr[c] = 0.0f;
-    double mx = fabs((double)v), sq = (double)v * (double)v;
-    block_reduce2<true>(mx, sq, lds);
-    store_partials(R, br, mx, sq);
-}
-
 // q = A p and p.q partials (FIN_PQ forms alpha = (r.z) / (p.q)).
 template <class T>
 __global__ __launch_bounds__(BLOCK) void k_apply_dot(LevelDesc L, OpDesc<T> A, const T *__restrict__ p,
@@ -555,15 +549,6 @@ __global__ __launch_bounds__(BLOCK) void k_update_p(LevelDesc L, const T *__rest
     const SolveScalars &Sg = S[grid_of(L, br)];
     if (ctl->all_done || Sg.done) return;
     p[c] = first ? zv : zv + (T)Sg.beta * pv;
-}
-
-// x64 += (double) corr
-__global__ __launch_bounds__(BLOCK) void k_add_correction(LevelDesc L, double *__restrict__ x,
-                                                          const float *__restrict__ corr) {
-    const int br = brick_or_idle(L);
-    if (br < 0) return;
-    const long c = (long)br * BRICK_CELLS + threadIdx.x;
-    x[c] += (double)corr[c];
 }
 
 enum FinOp {
@@ -889,6 +874,31 @@ __device__ __forceinline__ void unpack4(const float4 v, float o[4]) {
     o[2] = v.z;
     o[3] = v.w;
 }
+// Four consecutive values (16-byte loads: one float4 or two double2), zeros where i < 0.
+__device__ __forceinline__ void load4(const float *__restrict__ a, long i, float o[4]) { unpack4(ld4(a, i), o); }
+__device__ __forceinline__ void load4(const double *__restrict__ a, long i, double o[4]) {
+    const double2 u = *reinterpret_cast<const double2 *>(a + i), w = *reinterpret_cast<const double2 *>(a + i + 2);
+    o[0] = u.x;
+    o[1] = u.y;
+    o[2] = w.x;
+    o[3] = w.y;
+}
+template <class T>
+__device__ __forceinline__ void load4z(const T *__restrict__ a, long i, T o[4]) {
+    if (i >= 0) {
+        load4(a, i, o);
+    } else {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) o[j] = T(0);
+    }
+}
+__device__ __forceinline__ void store4(float *__restrict__ a, long i, const float v[4]) {
+    *reinterpret_cast<float4 *>(a + i) = make_float4(v[0], v[1], v[2], v[3]);
+}
+__device__ __forceinline__ void store4(double *__restrict__ a, long i, const double v[4]) {
+    *reinterpret_cast<double2 *>(a + i) = make_double2(v[0], v[1]);
+    *reinterpret_cast<double2 *>(a + i + 2) = make_double2(v[2], v[3]);
+}
 
 // The six neighbour rows of lane q's cells: x and y neighbours as whole rows (index of their first
 // cell, -1 where no brick is stored), z neighbours as the row shifted by one cell.
@@ -912,20 +922,21 @@ __device__ __forceinline__ Row4Nbrs row4_nbrs(const BrickMeta &m, int br, int q)
 }
 
 // The values of v at the six neighbours of the lane's four cells (0 where no brick is stored).
+template <class T>
 struct Row4Vals {
-    float c[4], xm[4], xp[4], ym[4], yp[4], zm[4], zp[4];
+    T c[4], xm[4], xp[4], ym[4], yp[4], zm[4], zp[4];
 };
-__device__ __forceinline__ Row4Vals row4_vals(const float *__restrict__ v, const Row4Nbrs &N) {
-    Row4Vals o;
-    const float4 c4 = ld4(v, N.c);
-    unpack4(c4, o.c);
-    unpack4(ld4z(v, N.ixm), o.xm);
-    unpack4(ld4z(v, N.ixp), o.xp);
-    unpack4(ld4z(v, N.iym), o.ym);
-    unpack4(ld4z(v, N.iyp), o.yp);
-    const float end = N.iz >= 0 ? v[N.iz] : 0.0f;
+template <class T>
+__device__ __forceinline__ Row4Vals<T> row4_vals(const T *__restrict__ v, const Row4Nbrs &N) {
+    Row4Vals<T> o;
+    load4(v, N.c, o.c);
+    load4z(v, N.ixm, o.xm);
+    load4z(v, N.ixp, o.xp);
+    load4z(v, N.iym, o.ym);
+    load4z(v, N.iyp, o.yp);
+    const T end = N.iz >= 0 ? v[N.iz] : T(0);
     // lane swap: h = 0 receives the cell k = 4 of its partner, h = 1 the cell k = 3
-    const float swap = __shfl_xor(N.h ? c4.x : c4.w, 1, 64);
+    const T swap = __shfl_xor(N.h ? o.c[0] : o.c[3], 1, 64);
     o.zm[0] = N.h ? swap : end;
     o.zp[3] = N.h ? end : swap;
 #pragma unroll
@@ -938,19 +949,20 @@ __device__ __forceinline__ Row4Vals row4_vals(const float *__restrict__ v, const
 
 // The couplings of the lane's cells: low faces from the cell itself, high faces from the next cell
 // along each axis (0 where no brick is stored).
+template <class T>
 struct Row4Coef {
-    float gxm[4], gym[4], gzm[4], gxp[4], gyp[4], gzp[4];
+    T gxm[4], gym[4], gzm[4], gxp[4], gyp[4], gzp[4];
 };
-__device__ __forceinline__ Row4Coef row4_coef(const OpDesc<float> &A, const Row4Nbrs &N) {
-    Row4Coef o;
-    const float4 gz4 = ld4(A.gz, N.c);
-    unpack4(ld4(A.gx, N.c), o.gxm);
-    unpack4(ld4(A.gy, N.c), o.gym);
-    unpack4(gz4, o.gzm);
-    unpack4(ld4z(A.gx, N.ixp), o.gxp);
-    unpack4(ld4z(A.gy, N.iyp), o.gyp);
-    const float end = (N.h && N.iz >= 0) ? A.gz[N.iz] : 0.0f;  // the coupling across k = 8
-    const float swap = __shfl_xor(gz4.x, 1, 64);               // h = 0: the partner's k = 4
+template <class T>
+__device__ __forceinline__ Row4Coef<T> row4_coef(const OpDesc<T> &A, const Row4Nbrs &N) {
+    Row4Coef<T> o;
+    load4(A.gx, N.c, o.gxm);
+    load4(A.gy, N.c, o.gym);
+    load4(A.gz, N.c, o.gzm);
+    load4z(A.gx, N.ixp, o.gxp);
+    load4z(A.gy, N.iyp, o.gyp);
+    const T end = (N.h && N.iz >= 0) ? A.gz[N.iz] : T(0);  // the coupling across k = 8
+    const T swap = __shfl_xor(o.gzm[0], 1, 64);            // h = 0: the partner's k = 4
 #pragma unroll
     for (int j = 0; j < 3; ++j) o.gzp[j] = o.gzm[j + 1];
     o.gzp[3] = N.h ? end : swap;
@@ -965,7 +977,7 @@ __device__ __forceinline__ void jacobi_row4(const LevelDesc &L, const OpDesc<flo
                                             float *__restrict__ zout, float omega, int br, const BrickMeta &m, int q,
                                             double &part) {
     const Row4Nbrs N = row4_nbrs(m, br, q);
-    const Row4Vals z = row4_vals(zin, N);
+    const Row4Vals<float> z = row4_vals(zin, N);
     float r[4], out[4];
     unpack4(ld4(rhs, N.c), r);
     if (m.flags & BM_UNIFORM) {
@@ -978,7 +990,7 @@ __device__ __forceinline__ void jacobi_row4(const LevelDesc &L, const OpDesc<flo
             if (DOT) part += (double)r[j] * (double)out[j];
         }
     } else {
-        const Row4Coef g = row4_coef(A, N);
+        const Row4Coef<float> g = row4_coef(A, N);
         float id[4];
         unpack4(ld4(idiag, N.c), id);
 #pragma unroll
@@ -1039,27 +1051,27 @@ __global__ __launch_bounds__(ROW4_THREADS) void k_jacobi4_dot(LevelDesc L, OpDes
 }
 
 // (A v) on the lane's four cells, with the unknown mask: apply_row / apply_row_uniform per cell.
-__device__ __forceinline__ void apply_row4(const LevelDesc &L, const OpDesc<float> &A, int br, const BrickMeta &m,
-                                           const Row4Nbrs &N, const Row4Vals &v, float av[4], bool unk[4]) {
+template <class T>
+__device__ __forceinline__ void apply_row4(const LevelDesc &L, const OpDesc<T> &A, int br, const BrickMeta &m,
+                                           const Row4Nbrs &N, const Row4Vals<T> &v, T av[4], bool unk[4]) {
     if (m.flags & BM_UNIFORM) {
-        const UniT<float> u = uni_load<float>(L, br);
-        const float diag = 0.0f + ((u.g + u.g) + (u.g + u.g) + (u.g + u.g));
+        const UniT<T> u = uni_load<T>(L, br);
+        const T diag = T(0) + ((u.g + u.g) + (u.g + u.g) + (u.g + u.g));
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-            const float off = u.g * v.xm[j] + u.g * v.xp[j] + u.g * v.ym[j] + u.g * v.yp[j] + u.g * v.zm[j] +
-                              u.g * v.zp[j];
+            const T off = u.g * v.xm[j] + u.g * v.xp[j] + u.g * v.ym[j] + u.g * v.yp[j] + u.g * v.zm[j] + u.g * v.zp[j];
             av[j] = diag * v.c[j] - off;
             unk[j] = true;
         }
     } else {
-        const Row4Coef g = row4_coef(A, N);
-        float dir[4];
-        unpack4(ld4(A.dir, N.c), dir);
+        const Row4Coef<T> g = row4_coef(A, N);
+        T dir[4];
+        load4(A.dir, N.c, dir);
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-            const float off = g.gxm[j] * v.xm[j] + g.gxp[j] * v.xp[j] + g.gym[j] * v.ym[j] + g.gyp[j] * v.yp[j] +
-                              g.gzm[j] * v.zm[j] + g.gzp[j] * v.zp[j];
-            const float diag = dir[j] + ((g.gxm[j] + g.gxp[j]) + (g.gym[j] + g.gyp[j]) + (g.gzm[j] + g.gzp[j]));
+            const T off = g.gxm[j] * v.xm[j] + g.gxp[j] * v.xp[j] + g.gym[j] * v.ym[j] + g.gyp[j] * v.yp[j] +
+                          g.gzm[j] * v.zm[j] + g.gzp[j] * v.zp[j];
+            const T diag = dir[j] + ((g.gxm[j] + g.gxp[j]) + (g.gym[j] + g.gyp[j]) + (g.gzm[j] + g.gzp[j]));
             av[j] = diag * v.c[j] - off;
             unk[j] = dir[j] >= 0;
         }
@@ -1098,7 +1110,7 @@ __global__ __launch_bounds__(BLOCK) void k_residual4(LevelDesc L, OpDesc<float> 
     const BrickMeta m = L.meta[br];
     if (all_done(Q) || skip_smooth(Q, L, m)) return;
     const Row4Nbrs N = row4_nbrs(m, br, q);
-    const Row4Vals v = row4_vals(z, N);
+    const Row4Vals<float> v = row4_vals(z, N);
     float r[4], av[4], out[4];
     bool unk[4];
     unpack4(ld4(rhs, N.c), r);
@@ -1144,7 +1156,7 @@ __global__ __launch_bounds__(ROW4_THREADS) void k_apply_dot4(LevelDesc L, OpDesc
     // a batch skips finished grids before loading anything; a single grid loads first
     if (L.ngrids > 1 && (ctl->all_done || S[m.grid].done)) return;
     const Row4Nbrs N = row4_nbrs(m, br, threadIdx.x);
-    const Row4Vals v = row4_vals(p, N);
+    const Row4Vals<float> v = row4_vals(p, N);
     if (ctl->all_done) return;
     float av[4], out[4];
     bool unk[4];
@@ -1208,6 +1220,75 @@ __global__ __launch_bounds__(BLOCK) void k_update_p4(LevelDesc L, const float *_
 #pragma unroll
     for (int j = 0; j < 4; ++j) pv[j] = first ? zv[j] : zv[j] + beta * pv[j];
     *reinterpret_cast<float4 *>(p + c) = make_float4(pv[0], pv[1], pv[2], pv[3]);
+}
+
+// r = b - A x in FP64 with per-brick ||r||_inf and ||r||^2 partials (finalized by FIN_DEFECT*):
+// one brick per 128-thread workgroup, four cells per thread (two 16-byte loads per array).
+__global__ __launch_bounds__(ROW4_THREADS) void k_defect4(LevelDesc L, OpDesc<double> A, const double *__restrict__ b,
+                                                          const double *__restrict__ x, double *__restrict__ r,
+                                                          RedBuf R) {
+    __shared__ double lds[4];
+    const int br = brick_or_idle(L);
+    if (br < 0) return;
+    const BrickMeta m = L.meta[br];
+    const Row4Nbrs N = row4_nbrs(m, br, threadIdx.x);
+    const Row4Vals<double> v = row4_vals(x, N);
+    double bv[4], av[4], out[4];
+    bool unk[4];
+    load4(b, N.c, bv);
+    apply_row4(L, A, br, m, N, v, av, unk);
+    double mx = 0.0, sq = 0.0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        out[j] = unk[j] ? bv[j] - av[j] : 0.0;
+        mx = fmax(mx, fabs(out[j]));
+        sq += out[j] * out[j];
+    }
+    store4(r, N.c, out);
+    row4_reduce2<true>(mx, sq, lds);
+    store_partials(R, br, mx, sq);
+}
+
+// Inner (FP32) solve set-up of the defect correction: r32 = (float) r64, c = 0, with ||r32||
+// partials (finalized by FIN_INNER_INIT); four cells per thread.
+__global__ __launch_bounds__(ROW4_THREADS) void k_inner_init4(LevelDesc L, const double *__restrict__ r64,
+                                                              float *__restrict__ r32, float *__restrict__ corr,
+                                                              RedBuf R) {
+    __shared__ double lds[4];
+    const int br = brick_or_idle(L);
+    if (br < 0) return;
+    const long c = (long)br * BRICK_CELLS + 4 * threadIdx.x;
+    double rv[4];
+    float v[4];
+    const float zero[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    load4(r64, c, rv);
+    double mx = 0.0, sq = 0.0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        v[j] = (float)rv[j];
+        mx = fmax(mx, fabs((double)v[j]));
+        sq += (double)v[j] * (double)v[j];
+    }
+    store4(r32, c, v);
+    store4(corr, c, zero);
+    row4_reduce2<true>(mx, sq, lds);
+    store_partials(R, br, mx, sq);
+}
+
+// x64 += (double) corr on rows (four bricks per 512-thread workgroup).
+__global__ __launch_bounds__(BLOCK) void k_add_correction4(LevelDesc L, double *__restrict__ x,
+                                                           const float *__restrict__ corr) {
+    int br, q;
+    row4_slot(L, br, q);
+    if (br >= L.nbricks) return;
+    const long c = (long)br * BRICK_CELLS + 4 * q;
+    double xv[4];
+    float cv[4];
+    load4(x, c, xv);
+    load4(corr, c, cv);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) xv[j] += (double)cv[j];
+    store4(x, c, xv);
 }
 
 // res = rhs - A z

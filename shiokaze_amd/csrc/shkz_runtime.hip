@@ -2,7 +2,7 @@
 //
 // One projection of one grid is one macpressuresolver3::project (src/projection/
 // macpressuresolver3.cpp:50-272):
-//   fractions + labelling          k_any_negative, k_flags               (:61-156)
+//   fractions + labelling          k_flags (+ k_any_negative with surface tension) (:61-156)
 //   assembly (matrix-free)         k_operator, k_coarsen, k_idiag        (:159-199)
 //   volume correction, warm start  host PI controller, scene state       (:203-230, :239-242)
 //   linear solve                   CG / MGPCG / mixed-precision MGPCG    (:233-237 -> pcg.cpp:45-73)
@@ -1071,8 +1071,8 @@ static int solve_phase(shkz_ctx *C, Launcher &K, Problem &P, const shkz_params *
         // no unknowns anywhere: nothing to solve
     } else if (!P.mixed) {
         // one FP64 Krylov solve per grid: CG (the reference iteration) or MGPCG
-        K.go("defect", 0, 7 * 8, -usave(C, 0, 32), gb, bb, k_defect<double>, F0.desc, A64, (const double *)P.b64, (const double *)P.x64,
-             P.r64, P.red);
+        K.go("defect", 0, 7 * 8, -usave(C, 0, 32), gb, dim3(ROW4_THREADS), k_defect4, F0.desc, A64,
+             (const double *)P.b64, (const double *)P.x64, P.r64, P.red);
         finalize(K, P, F0.desc, P.S_out, P.ctl, FIN_DEFECT_FIRST, residual);
         BatchCtl hc{};
         SHKZ_TRY(read_scalars(C, s, P.S_out, ngrids, hs, P.ctl, &hc));
@@ -1091,15 +1091,15 @@ static int solve_phase(shkz_ctx *C, Launcher &K, Problem &P, const shkz_params *
         K.go("init_scalars", -1, 0, 0, dim3((unsigned)((ngrids + 255) / 256)), dim3(256), k_init_scalars, P.S_in,
              ngrids, norm, maxit);
         for (int outer = 0; outer < 64; ++outer) {
-            K.go("defect", 0, 7 * 8, -usave(C, 0, 32), gb, bb, k_defect<double>, F0.desc, A64, (const double *)P.b64,
-                 (const double *)P.x64, P.r64, P.red);
+            K.go("defect", 0, 7 * 8, -usave(C, 0, 32), gb, dim3(ROW4_THREADS), k_defect4, F0.desc, A64,
+                 (const double *)P.b64, (const double *)P.x64, P.r64, P.red);
             if (outer == 0)
                 finalize(K, P, F0.desc, P.S_out, nullptr, FIN_DEFECT_FIRST, residual);
             else
                 finalize(K, P, F0.desc, P.S_out, nullptr, FIN_OUTER, 0.0, 0, P.S_in);
             K.go("init_ctl", -1, 0, 0, dim3(1), dim3(64), k_init_ctl, P.ctl + 1, ngrids);
-            K.go("inner_init", 0, 16, 0, gb, bb, k_inner_init, F0.desc, (const double *)P.r64, (float *)P.kr,
-                 (float *)P.kx, P.red);
+            K.go("inner_init", 0, 16, 0, gb, dim3(ROW4_THREADS), k_inner_init4, F0.desc, (const double *)P.r64,
+                 (float *)P.kr, (float *)P.kx, P.red);
             finalize(K, P, F0.desc, P.S_in, P.ctl + 1, FIN_INNER_INIT, eta, 0, P.S_out);
             BatchCtl hc{};
             SHKZ_TRY(read_scalars(C, s, P.S_out, ngrids, hs, P.ctl + 1, &hc));
@@ -1115,7 +1115,8 @@ static int solve_phase(shkz_ctx *C, Launcher &K, Problem &P, const shkz_params *
                     if (!hs[g].done)
                         std::fprintf(stderr, "[shkz] grid %d outer %d: defect %.6e (tol %.3e) inner it %d rinf %.6e\n",
                                      g, outer, norm == 0 ? hs[g].rinf : hs[g].rl2, hs[g].tol, hi[g].iter, hi[g].rinf);
-            K.go("add_correction", 0, 20, 0, gb, bb, k_add_correction, F0.desc, P.x64, (const float *)P.kx);
+            K.go("add_correction", 0, 20, 0, dim3(row4_grid(F0.nbricks)), dim3(BLOCK), k_add_correction4, F0.desc,
+                 P.x64, (const float *)P.kx);
             if (outer == 63)
                 for (int g = 0; g < ngrids; ++g)
                     if (!hs[g].done) total_iters[g] += hi[g].iter;
@@ -1125,8 +1126,8 @@ static int solve_phase(shkz_ctx *C, Launcher &K, Problem &P, const shkz_params *
     }
     // true FP64 residual of the returned pressure, per grid
     if (F0.nbricks > 0) {
-        K.go("defect", 0, 7 * 8, -usave(C, 0, 32), gb, bb, k_defect<double>, F0.desc, A64, (const double *)P.b64, (const double *)P.x64,
-             P.r64, P.red);
+        K.go("defect", 0, 7 * 8, -usave(C, 0, 32), gb, dim3(ROW4_THREADS), k_defect4, F0.desc, A64,
+             (const double *)P.b64, (const double *)P.x64, P.r64, P.red);
         finalize(K, P, F0.desc, P.S_fin, nullptr, FIN_DEFECT);
         SHKZ_TRY(read_scalars(C, s, P.S_fin, ngrids, hf));
     }
@@ -1421,14 +1422,14 @@ static int project_stages(shkz_ctx *C, const shkz_grid *grids, int ngrids, const
         G.flat = flat ? 1 : 0;
         J.exist = exist_all + 4 * g;
         G.exist = J.exist;
-        const unsigned nbk = (unsigned)std::min<size_t>((J.M + 255) / 256, 4096);
-        K.go("any_negative", -1, 0, (double)J.M * (G.phi64 ? 8 : 4), dim3(nbk), dim3(256), k_any_negative, grid->fluid,
-             (int)G.phi64, J.M, J.exist + 0);
-        if (grid->solid) {
-            const size_t Nn = (size_t)(w + 1) * (h + 1) * (flat ? 1 : d + 1);
-            const unsigned nbs = (unsigned)std::min<size_t>((Nn + 255) / 256, 4096);
-            K.go("any_negative", -1, 0, (double)Nn * (G.solid64 ? 8 : 4), dim3(nbs), dim3(256), k_any_negative,
-                 grid->solid, (int)G.solid64, Nn, J.exist + 1);
+        // levelset_exist() of the liquid: the labelling finds it on the way (read back with the brick
+        // flags), unless surface tension needs it first (the solid needs no scan, as the face
+        // fractions show)
+        const bool scan_liquid = prm->surface_tension != 0.0;
+        if (scan_liquid) {
+            const unsigned nbk = (unsigned)std::min<size_t>((J.M + 255) / 256, 4096);
+            K.go("any_negative", -1, 0, (double)J.M * (G.phi64 ? 8 : 4), dim3(nbk), dim3(256), k_any_negative,
+                 grid->fluid, (int)G.phi64, J.M, J.exist + 0);
         }
         if (prm->surface_tension != 0.0) {
             // sized for the batch's largest grid once, so no buffer is reallocated under queued work
@@ -1460,7 +1461,15 @@ static int project_stages(shkz_ctx *C, const shkz_grid *grids, int ngrids, const
         for (int g = 0; g < ngrids; ++g) {
             GridJob &J = jobs[g];
             const uint8_t *src = (const uint8_t *)hflags + bflag_off[g];
-            J.l0.active.assign(src, src + (size_t)J.l0.nbx * J.l0.nby * J.l0.nbz);
+            const size_t npos = (size_t)J.l0.nbx * J.l0.nby * J.l0.nbz;
+            J.l0.active.resize(npos);
+            bool liquid = false;
+            for (size_t b = 0; b < npos; ++b) {
+                J.l0.active[b] = src[b] & 1;
+                liquid |= (src[b] & 2) != 0;
+            }
+            // the later kernels take it from here (with surface tension, the scan has set exist[0])
+            if (prm->surface_tension == 0.0) J.G.liquid = liquid ? 2 : 1;
         }
     }
     const double t_frac = now_ms();
