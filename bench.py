@@ -42,8 +42,8 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-SYMBOL = {"jacobi_L0": "k_jacobi<float, 1, 0>", "jacobi_dot_L0": "k_jacobi<float, 1, 1>",
-          "residual_L0": "k_residual<float, 1>", "smooth2_L0": "k_smooth2<float, 1>"}
+SYMBOL = {"jacobi_L0": "k_jacobi4<1>", "jacobi_dot_L0": "k_jacobi4_dot",
+          "residual_L0": "k_residual4<1>", "smooth2_L0": "k_smooth2<float, 1>"}
 
 
 def measured_traffic(kernel, config):

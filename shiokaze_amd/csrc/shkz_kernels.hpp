@@ -1708,10 +1708,14 @@ __global__ __launch_bounds__(1024) void k_smooth_k(LevelDesc L, OpDesc<T> A, con
 
 // Coarse solve of every grid whose last level L is (one brick): block g handles grid g with
 // `sweeps` damped-Jacobi sweeps from zero, in LDS.  DOT (finest level only): also the r.z partial
-// of the grid's brick, as k_jacobi<..., DOT> stores it for grids with coarser levels.
-template <class T, int DOT>
-__global__ __launch_bounds__(BLOCK) void k_coarse_solve(LevelDesc L, OpDesc<T> A, const T *__restrict__ rhs,
-                                                        T *__restrict__ z, int sweeps, T omega, Gate Q, RedBuf R) {
+// of the grid's brick, as k_jacobi<..., DOT> stores it for grids with coarser levels.  CPT cells per
+// thread (512 / CPT threads): fewer wavefronts make each sweep's barrier cheaper; every thread keeps
+// its cells' couplings and own values in registers.
+template <class T, int DOT, int CPT = 1>
+__global__ __launch_bounds__(BRICK_CELLS / CPT) void k_coarse_solve(LevelDesc L, OpDesc<T> A,
+                                                                    const T *__restrict__ rhs, T *__restrict__ z,
+                                                                    int sweeps, T omega, Gate Q, RedBuf R) {
+    constexpr int NT = BRICK_CELLS / CPT;
     __shared__ T zs[2][BRICK_CELLS];
     __shared__ double lds[32];
     if (all_done(Q)) return;
@@ -1719,133 +1723,61 @@ __global__ __launch_bounds__(BLOCK) void k_coarse_solve(LevelDesc L, OpDesc<T> A
     const GridLevel G = L.gl[g];
     if (!G.coarsest || G.brick_end <= G.brick_start || (Q.S && Q.S[g].done)) return;
     const int t = threadIdx.x;
-    const int li = t >> 6, lj = (t >> 3) & 7, lk = t & 7;
-    const long c = (long)G.brick_start * BRICK_CELLS + t;
-    const T dirc = A.dir[c];
-    const T gxm = A.gx[c], gym = A.gy[c], gzm = A.gz[c];
-    const T gxp = li < 7 ? A.gx[c + 64] : T(0), gyp = lj < 7 ? A.gy[c + 8] : T(0), gzp = lk < 7 ? A.gz[c + 1] : T(0);
-    const T diag = dirc + ((gxm + gxp) + (gym + gyp) + (gzm + gzp));
-    const bool unk = dirc >= 0 && diag > 0;
-    const T r = rhs[c];
-    const T inv = unk ? omega / diag : T(0);
-    zs[0][t] = unk ? inv * r : T(0);
+    const long base = (long)G.brick_start * BRICK_CELLS;
+    T gxm[CPT], gym[CPT], gzm[CPT], gxp[CPT], gyp[CPT], gzp[CPT], diag[CPT], inv[CPT], r[CPT], zr[CPT];
+    bool unk[CPT];
+#pragma unroll
+    for (int u = 0; u < CPT; ++u) {
+        const int e = t + u * NT;
+        const int li = e >> 6, lj = (e >> 3) & 7, lk = e & 7;
+        const long c = base + e;
+        const T dirc = A.dir[c];
+        gxm[u] = A.gx[c];
+        gym[u] = A.gy[c];
+        gzm[u] = A.gz[c];
+        gxp[u] = li < 7 ? A.gx[c + 64] : T(0);
+        gyp[u] = lj < 7 ? A.gy[c + 8] : T(0);
+        gzp[u] = lk < 7 ? A.gz[c + 1] : T(0);
+        diag[u] = dirc + ((gxm[u] + gxp[u]) + (gym[u] + gyp[u]) + (gzm[u] + gzp[u]));
+        unk[u] = dirc >= 0 && diag[u] > 0;
+        r[u] = rhs[c];
+        inv[u] = unk[u] ? omega / diag[u] : T(0);
+        zr[u] = unk[u] ? inv[u] * r[u] : T(0);
+        zs[0][e] = zr[u];
+    }
     __syncthreads();
     int cur = 0;
     for (int s = 1; s < sweeps; ++s) {
         const T *zz = zs[cur];
-        T out = 0;
-        if (unk) {
-            const T off = gxm * (li > 0 ? zz[t - 64] : T(0)) + gxp * (li < 7 ? zz[t + 64] : T(0)) +
-                          gym * (lj > 0 ? zz[t - 8] : T(0)) + gyp * (lj < 7 ? zz[t + 8] : T(0)) +
-                          gzm * (lk > 0 ? zz[t - 1] : T(0)) + gzp * (lk < 7 ? zz[t + 1] : T(0));
-            const T zc = zz[t];
-            out = zc + inv * (r - (diag * zc - off));
-        }
-        zs[cur ^ 1][t] = out;
-        __syncthreads();
-        cur ^= 1;
-    }
-    const T zo = zs[cur][t];
-    z[c] = zo;
-    if (DOT) {
-        const double part = block_sum((double)r * (double)zo, lds);
-        store_partials(R, G.brick_start, part, 0.0);
-    }
-}
-
-// FP32 coarse solve with one wavefront per grid: lane t keeps the cell row (t >> 3, t & 7, 0..7)
-// of the brick and its couplings in registers, and each sweep exchanges rows through LDS (16-byte
-// accesses, no multi-wave barrier).  The per-cell arithmetic is k_coarse_solve's.
-template <int DOT>
-__global__ __launch_bounds__(64) void k_coarse_solve_w(LevelDesc L, OpDesc<float> A, const float *__restrict__ rhs,
-                                                       float *__restrict__ z, int sweeps, float omega, Gate Q,
-                                                       RedBuf R) {
-    __shared__ float4 zs[2][BRICK_CELLS / 4];
-    if (all_done(Q)) return;
-    const int g = blockIdx.x;
-    const GridLevel G = L.gl[g];
-    if (!G.coarsest || G.brick_end <= G.brick_start || (Q.S && Q.S[g].done)) return;
-    const int t = threadIdx.x, li = t >> 3, lj = t & 7;
-    const int o = li * 64 + lj * 8;  // the row's first cell in the brick
-    const long row = (long)G.brick_start * BRICK_CELLS + o;
-    auto ld8 = [&](const float *a, long i, float v[8]) {
-        const float4 x = *reinterpret_cast<const float4 *>(a + i), y = *reinterpret_cast<const float4 *>(a + i + 4);
-        v[0] = x.x; v[1] = x.y; v[2] = x.z; v[3] = x.w; v[4] = y.x; v[5] = y.y; v[6] = y.z; v[7] = y.w;
-    };
-    float dirc[8], gxm[8], gym[8], gzm[8], gxp[8], gyp[8], r[8];
-    ld8(A.dir, row, dirc);
-    ld8(A.gx, row, gxm);
-    ld8(A.gy, row, gym);
-    ld8(A.gz, row, gzm);
-    ld8(rhs, row, r);
-    if (li < 7) ld8(A.gx, row + 64, gxp);
-    else
-        for (int k = 0; k < 8; ++k) gxp[k] = 0.0f;
-    if (lj < 7) ld8(A.gy, row + 8, gyp);
-    else
-        for (int k = 0; k < 8; ++k) gyp[k] = 0.0f;
-    float diag[8], inv[8], zc[8];
-    bool unk[8];
+        T out[CPT];
 #pragma unroll
-    for (int k = 0; k < 8; ++k) {
-        const float gzp = k < 7 ? gzm[k + 1] : 0.0f;
-        diag[k] = dirc[k] + ((gxm[k] + gxp[k]) + (gym[k] + gyp[k]) + (gzm[k] + gzp));
-        unk[k] = dirc[k] >= 0 && diag[k] > 0;
-        inv[k] = unk[k] ? omega / diag[k] : 0.0f;
-        zc[k] = unk[k] ? inv[k] * r[k] : 0.0f;
-    }
-    auto put = [&](int buf) {
-        zs[buf][o / 4] = make_float4(zc[0], zc[1], zc[2], zc[3]);
-        zs[buf][o / 4 + 1] = make_float4(zc[4], zc[5], zc[6], zc[7]);
-    };
-    auto get = [&](int buf, int off, float v[8]) {
-        const float4 x = zs[buf][off / 4], y = zs[buf][off / 4 + 1];
-        v[0] = x.x; v[1] = x.y; v[2] = x.z; v[3] = x.w; v[4] = y.x; v[5] = y.y; v[6] = y.z; v[7] = y.w;
-    };
-    put(0);
-    __syncthreads();
-    int cur = 0;
-    for (int s = 1; s < sweeps; ++s) {
-        float xm[8], xp[8], ym[8], yp[8];
-        if (li > 0) get(cur, o - 64, xm);
-        else
-            for (int k = 0; k < 8; ++k) xm[k] = 0.0f;
-        if (li < 7) get(cur, o + 64, xp);
-        else
-            for (int k = 0; k < 8; ++k) xp[k] = 0.0f;
-        if (lj > 0) get(cur, o - 8, ym);
-        else
-            for (int k = 0; k < 8; ++k) ym[k] = 0.0f;
-        if (lj < 7) get(cur, o + 8, yp);
-        else
-            for (int k = 0; k < 8; ++k) yp[k] = 0.0f;
-        float nz[8];
-#pragma unroll
-        for (int k = 0; k < 8; ++k) {
-            float out = 0.0f;
-            if (unk[k]) {
-                const float gzp = k < 7 ? gzm[k + 1] : 0.0f;
-                const float zzm = k > 0 ? zc[k - 1] : 0.0f, zzp = k < 7 ? zc[k + 1] : 0.0f;
-                const float off = gxm[k] * xm[k] + gxp[k] * xp[k] + gym[k] * ym[k] + gyp[k] * yp[k] + gzm[k] * zzm +
-                                  gzp * zzp;
-                out = zc[k] + inv[k] * (r[k] - (diag[k] * zc[k] - off));
+        for (int u = 0; u < CPT; ++u) {
+            const int e = t + u * NT;
+            const int li = e >> 6, lj = (e >> 3) & 7, lk = e & 7;
+            out[u] = T(0);
+            if (unk[u]) {
+                const T off = gxm[u] * (li > 0 ? zz[e - 64] : T(0)) + gxp[u] * (li < 7 ? zz[e + 64] : T(0)) +
+                              gym[u] * (lj > 0 ? zz[e - 8] : T(0)) + gyp[u] * (lj < 7 ? zz[e + 8] : T(0)) +
+                              gzm[u] * (lk > 0 ? zz[e - 1] : T(0)) + gzp[u] * (lk < 7 ? zz[e + 1] : T(0));
+                out[u] = zr[u] + inv[u] * (r[u] - (diag[u] * zr[u] - off));
             }
-            nz[k] = out;
         }
 #pragma unroll
-        for (int k = 0; k < 8; ++k) zc[k] = nz[k];
-        cur ^= 1;
-        put(cur);
+        for (int u = 0; u < CPT; ++u) {
+            zs[cur ^ 1][t + u * NT] = out[u];
+            zr[u] = out[u];
+        }
         __syncthreads();
+        cur ^= 1;
     }
-    float4 *zo = reinterpret_cast<float4 *>(z + row);
-    zo[0] = make_float4(zc[0], zc[1], zc[2], zc[3]);
-    zo[1] = make_float4(zc[4], zc[5], zc[6], zc[7]);
-    if (DOT) {
-        double part = 0.0;
+    double part = 0.0;
 #pragma unroll
-        for (int k = 0; k < 8; ++k) part += (double)r[k] * (double)zc[k];
-        part = wave_sum(part);
+    for (int u = 0; u < CPT; ++u) {
+        z[base + t + u * NT] = zr[u];
+        part += (double)r[u] * (double)zr[u];
+    }
+    if (DOT) {
+        part = block_sum(part, lds);
         store_partials(R, G.brick_start, part, 0.0);
     }
 }

@@ -695,20 +695,15 @@ static T *vcycle(shkz_ctx *C, Launcher &K, Problem &P, const T *rhs0, T *z0, T *
         if (L.any_coarsest) {
             // grids ending here: their z goes where the other grids' smoothed z ends up
             T *dst = (last || final_slot(pre, post, mpre[l], mpost[l], f) == 0) ? z : tmp;
-            if constexpr (sizeof(T) == 4) {
-                // one wavefront per grid
-                if (f && dot_partials)
-                    K.go("coarse_solve_dot", l, 6 * es, 0, dim3((unsigned)P.ngrids), dim3(64), k_coarse_solve_w<1>,
-                         L.desc, A, rhs, dst, P.coarse_sweeps, om, Q, P.red);
-                else
-                    K.go("coarse_solve", l, 6 * es, 0, dim3((unsigned)P.ngrids), dim3(64), k_coarse_solve_w<0>, L.desc,
-                         A, rhs, dst, P.coarse_sweeps, om, Q, P.red);
-            } else if (f && dot_partials)
-                K.go("coarse_solve_dot", l, 6 * es, 0, dim3((unsigned)P.ngrids), b, k_coarse_solve<T, 1>, L.desc, A,
-                     rhs, dst, P.coarse_sweeps, om, Q, P.red);
+            // FP32: two cells per thread (256 threads), FP64 one
+            constexpr int CPT = sizeof(T) == 4 ? 2 : 1;
+            const dim3 cb(BRICK_CELLS / CPT);
+            if (f && dot_partials)
+                K.go("coarse_solve_dot", l, 6 * es, 0, dim3((unsigned)P.ngrids), cb, k_coarse_solve<T, 1, CPT>, L.desc,
+                     A, rhs, dst, P.coarse_sweeps, om, Q, P.red);
             else
-                K.go("coarse_solve", l, 6 * es, 0, dim3((unsigned)P.ngrids), b, k_coarse_solve<T, 0>, L.desc, A, rhs,
-                     dst, P.coarse_sweeps, om, Q, P.red);
+                K.go("coarse_solve", l, 6 * es, 0, dim3((unsigned)P.ngrids), cb, k_coarse_solve<T, 0, CPT>, L.desc, A,
+                     rhs, dst, P.coarse_sweeps, om, Q, P.red);
         }
         if (last) {
             zcur[l] = z;
@@ -1056,7 +1051,7 @@ static int solve_phase(shkz_ctx *C, Launcher &K, Problem &P, const shkz_params *
     const double residual = prm->residual < 1e-30 ? 1e-30 : prm->residual;
     OpDesc<double> A64{P.dir64, P.gx64, P.gy64, P.gz64};
     const unsigned g0 = brick_grid(F0.nbricks);
-    const dim3 gb(g0), bb(BLOCK);
+    const dim3 gb(g0);
     std::vector<int> &total_iters = O.total_iters, &outer_passes = O.outer_passes, &status = O.status;
     total_iters.assign(ngrids, 0);
     outer_passes.assign(ngrids, 0);
