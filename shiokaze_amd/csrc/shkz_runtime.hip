@@ -413,7 +413,7 @@ struct shkz_ctx {
     struct Pinned {
         void *p = nullptr;
         size_t cap = 0;
-    } pin_flags, pin_meta, pin_counts;
+    } pin_flags, pin_meta0, pin_meta, pin_links, pin_counts;
     // timing
     bool timing = false;
     std::vector<std::string> kernel_names;
@@ -1499,8 +1499,9 @@ static int project_stages(shkz_ctx *C, const shkz_grid *grids, int ngrids, const
     }
     P.nlev = (int)glevels.size();
     C->levels.assign(P.nlev, LevelHost{});
-    int max_bricks = 8;
-    for (int l = 0; l < P.nlev; ++l) {
+    // Level 0 first: its tables go up and the assembly starts on the device while the host builds
+    // the coarser levels and the transfer tables.
+    auto finish_level = [&](int l) {
         LevelHost &L = C->levels[l];
         build_level(L, glevels[l]);
         if (!multigrid) {
@@ -1509,76 +1510,68 @@ static int project_stages(shkz_ctx *C, const shkz_grid *grids, int ngrids, const
             for (GridLevel &gl : L.gl) gl.coarsest = 0;
             for (BrickMeta &m : L.meta) m.flags = 0;
         }
-        max_bricks = std::max(max_bricks, (int)brick_grid(L.nbricks));
-    }
-    for (int l = 0; l + 1 < P.nlev; ++l) link_levels(C->levels[l], C->levels[l + 1]);
-    for (int l = 1; l < P.nlev; ++l) neighbourhoods(C->levels[l]);
-    {
-        // every level's tables in one upload: per level bmap | bxyz | meta | gl | child64 | parent8 |
-        // nb27, each 32-byte aligned
+    };
+    // Uploads the tables of levels [l0, l1) (bmap | bxyz | meta | gl, and with links also child64 |
+    // parent8 | nb27, each 32-byte aligned) from the pinned staging pin into the arena buffer name.
+    // A staging buffer stays untouched until this call's next synchronisation.
+    auto upload_tables = [&](int l0, int l1, bool links, const char *name, shkz_ctx::Pinned &pin,
+                             bool base) -> int {
         auto al = [](size_t n) { return (n + 7) & ~(size_t)7; };
         struct Offs {
             size_t base, xyz, meta, gl, c64, p8, n27;
         };
         std::vector<Offs> offs(P.nlev);
         size_t total = 0;
-        for (int l = 0; l < P.nlev; ++l) {
+        for (int l = l0; l < l1; ++l) {
             const LevelHost &L = C->levels[l];
             Offs &o = offs[l];
             o.base = total;
-            o.xyz = o.base + al(L.bmap.size());
-            o.meta = o.xyz + al(L.bxyz.size());
-            o.gl = o.meta + al(L.meta.size() * (sizeof(BrickMeta) / sizeof(int)));
-            o.c64 = o.gl + al(L.gl.size() * (sizeof(GridLevel) / sizeof(int)));
-            o.p8 = o.c64 + al(L.child64.size());
-            o.n27 = o.p8 + al(L.parent8.size());
-            total = o.n27 + al(L.nb27.size());
+            o.xyz = o.base + (base ? al(L.bmap.size()) : 0);
+            o.meta = o.xyz + (base ? al(L.bxyz.size()) : 0);
+            o.gl = o.meta + (base ? al(L.meta.size() * (sizeof(BrickMeta) / sizeof(int))) : 0);
+            o.c64 = o.gl + (base ? al(L.gl.size() * (sizeof(GridLevel) / sizeof(int))) : 0);
+            o.p8 = o.c64 + (links ? al(L.child64.size()) : 0);
+            o.n27 = o.p8 + (links ? al(L.parent8.size()) : 0);
+            total = o.n27 + (links ? al(L.nb27.size()) : 0);
         }
         total += 8;
         int *dmeta = nullptr;
-        SHKZ_TRY(arena_get(AR, "meta", total, &dmeta));
+        SHKZ_TRY(arena_get(AR, name, total, &dmeta));
         void *hmeta = nullptr;
-        SHKZ_TRY(ensure_pinned(C->pin_meta, total * sizeof(int), &hmeta));
+        SHKZ_TRY(ensure_pinned(pin, total * sizeof(int), &hmeta));
         int *host = (int *)hmeta;
         std::fill(host, host + total, -1);
-        for (int l = 0; l < P.nlev; ++l) {
+        for (int l = l0; l < l1; ++l) {
             LevelHost &L = C->levels[l];
             const Offs &o = offs[l];
-            std::copy(L.bmap.begin(), L.bmap.end(), host + o.base);
-            std::copy(L.bxyz.begin(), L.bxyz.end(), host + o.xyz);
-            std::memcpy(host + o.meta, L.meta.data(), L.meta.size() * sizeof(BrickMeta));
-            std::memcpy(host + o.gl, L.gl.data(), L.gl.size() * sizeof(GridLevel));
-            std::copy(L.child64.begin(), L.child64.end(), host + o.c64);
-            std::copy(L.parent8.begin(), L.parent8.end(), host + o.p8);
-            std::copy(L.nb27.begin(), L.nb27.end(), host + o.n27);
-            L.desc.nbricks = L.nbricks;
-            L.desc.ngrids = ngrids;
-            L.desc.flat = flat ? 1 : 0;
-            L.desc.bmap = dmeta + o.base;
-            L.desc.bxyz = dmeta + o.xyz;
-            L.desc.meta = reinterpret_cast<const BrickMeta *>(dmeta + o.meta);
-            L.desc.gl = reinterpret_cast<const GridLevel *>(dmeta + o.gl);
-            L.desc.child64 = L.child64.empty() ? nullptr : dmeta + o.c64;
-            L.desc.parent8 = L.parent8.empty() ? nullptr : dmeta + o.p8;
-            L.desc.nb27 = L.nb27.empty() ? nullptr : dmeta + o.n27;
-            L.desc.uni = nullptr;
+            if (base) {
+                std::copy(L.bmap.begin(), L.bmap.end(), host + o.base);
+                std::copy(L.bxyz.begin(), L.bxyz.end(), host + o.xyz);
+                std::memcpy(host + o.meta, L.meta.data(), L.meta.size() * sizeof(BrickMeta));
+                std::memcpy(host + o.gl, L.gl.data(), L.gl.size() * sizeof(GridLevel));
+                L.desc.nbricks = L.nbricks;
+                L.desc.ngrids = ngrids;
+                L.desc.flat = flat ? 1 : 0;
+                L.desc.bmap = dmeta + o.base;
+                L.desc.bxyz = dmeta + o.xyz;
+                L.desc.meta = reinterpret_cast<const BrickMeta *>(dmeta + o.meta);
+                L.desc.gl = reinterpret_cast<const GridLevel *>(dmeta + o.gl);
+                L.desc.uni = nullptr;
+            }
+            if (links) {
+                std::copy(L.child64.begin(), L.child64.end(), host + o.c64);
+                std::copy(L.parent8.begin(), L.parent8.end(), host + o.p8);
+                std::copy(L.nb27.begin(), L.nb27.end(), host + o.n27);
+                L.desc.child64 = L.child64.empty() ? nullptr : dmeta + o.c64;
+                L.desc.parent8 = L.parent8.empty() ? nullptr : dmeta + o.p8;
+                L.desc.nb27 = L.nb27.empty() ? nullptr : dmeta + o.n27;
+            }
         }
-        // the pinned staging stays untouched until this call's next synchronisation
         SHKZ_HIP(hipMemcpyAsync(dmeta, host, total * sizeof(int), hipMemcpyHostToDevice, s));
-    }
-    if (C->trace) {
-        std::fprintf(stderr, "[shkz] %d grid(s), levels:", ngrids);
-        for (int l = 0; l < P.nlev; ++l) std::fprintf(stderr, " %d%s", C->levels[l].nbricks, C->levels[l].any_coarsest ? "*" : "");
-        std::fprintf(stderr, " bricks\n");
-    }
-    // reduction scratch, scalars, control blocks
-    SHKZ_TRY(arena_get(AR, "partials", (size_t)2 * max_bricks, &P.red.partials));
-    SolveScalars *Sbuf = nullptr;
-    SHKZ_TRY(arena_get(AR, "scalars", (size_t)3 * ngrids, &Sbuf));
-    P.S_out = Sbuf;
-    P.S_in = Sbuf + ngrids;
-    P.S_fin = Sbuf + 2 * ngrids;
-    SHKZ_TRY(arena_get(AR, "ctl", 2, &P.ctl));
+        return SHKZ_OK;
+    };
+    finish_level(0);
+    SHKZ_TRY(upload_tables(0, 1, false, "meta0", C->pin_meta0, true));
 
     // ---------------------------------------------------------------- operator + vectors
     LevelHost &F0 = C->levels[0];
@@ -1606,6 +1599,55 @@ static int project_stages(shkz_ctx *C, const shkz_grid *grids, int ngrids, const
         F0.gy = P.gy64;
         F0.gz = P.gz64;
     }
+    // per grid: volume correction (macpressuresolver3.cpp:203-217), warm start, assembly
+    for (int g = 0; g < ngrids; ++g) {
+        GridJob &J = jobs[g];
+        const shkz_grid *grid = J.in;
+        if (grid->scene_id) SHKZ_TRY(C->scenes->acquire(grid->scene_id, grid->w, grid->h, grid->d, &J.scene));
+        if (prm->gain && prm->target_volume) {
+            double &yprev = J.scene ? J.scene->y_prev : C->y_prev;
+            const double xv = (prm->current_volume - prm->target_volume) / prm->target_volume;
+            const double y = yprev + xv * grid->dt;
+            yprev = y;
+            const double kp = prm->gain * 2.3 / (25.0 * 0.01);
+            const double ki = kp * kp / 16.0;
+            J.rhs_correct = -(kp * xv + ki * y) / (xv + 1.0);
+        }
+        res[g].rhs_correct = J.rhs_correct;
+        const double *warm = (prm->warm_start && J.scene && J.scene->has_pressure) ? J.scene->p : nullptr;
+        const GridLevel &GL = F0.gl[g];
+        const int nbg = GL.brick_end - GL.brick_start;
+        K.go("operator", -1, 0,
+             (double)nbg * BRICK_CELLS *
+                 ((J.G.phi64 ? 8 : 4) + (grid->solid ? 4 : 0) + 12 + 1 + 6 * 8 + (P.mixed ? 16 : 0)),
+             dim3(brick_grid(nbg)), dim3(BLOCK), k_operator, J.G, F0.desc, GL, (const uint8_t *)J.cflag, P.dir64,
+             P.gx64, P.gy64, P.gz64, dir32, gx32, gy32, gz32, P.b64, P.x64, warm, J.rhs_correct);
+    }
+    // the coarser levels and the transfer tables, built on the host while the assembly runs
+    int max_bricks = 8;
+    for (int l = 1; l < P.nlev; ++l) finish_level(l);
+    for (int l = 0; l < P.nlev; ++l) max_bricks = std::max(max_bricks, (int)brick_grid(C->levels[l].nbricks));
+    for (int l = 0; l + 1 < P.nlev; ++l) link_levels(C->levels[l], C->levels[l + 1]);
+    for (int l = 1; l < P.nlev; ++l) neighbourhoods(C->levels[l]);
+    {
+        // levels >= 1: their base tables; every level: its transfer and neighbourhood tables
+        SHKZ_TRY(upload_tables(1, P.nlev, false, "meta", C->pin_meta, true));
+        SHKZ_TRY(upload_tables(0, P.nlev, true, "meta_links", C->pin_links, false));
+    }
+    if (C->trace) {
+        std::fprintf(stderr, "[shkz] %d grid(s), levels:", ngrids);
+        for (int l = 0; l < P.nlev; ++l) std::fprintf(stderr, " %d%s", C->levels[l].nbricks, C->levels[l].any_coarsest ? "*" : "");
+        std::fprintf(stderr, " bricks\n");
+    }
+    // reduction scratch, scalars, control blocks
+    SHKZ_TRY(arena_get(AR, "partials", (size_t)2 * max_bricks, &P.red.partials));
+    SolveScalars *Sbuf = nullptr;
+    SHKZ_TRY(arena_get(AR, "scalars", (size_t)3 * ngrids, &Sbuf));
+    P.S_out = Sbuf;
+    P.S_in = Sbuf + ngrids;
+    P.S_fin = Sbuf + 2 * ngrids;
+    SHKZ_TRY(arena_get(AR, "ctl", 2, &P.ctl));
+
     const size_t es = P.fp64_levels ? 8 : 4;
     for (int l = 1; l < P.nlev; ++l) {
         LevelHost &L = C->levels[l];
@@ -1648,30 +1690,7 @@ static int project_stages(shkz_ctx *C, const shkz_grid *grids, int ngrids, const
         SHKZ_TRY(AR.get("kx32", n0 * 4, &P.kx));
     }
 
-    // per grid: volume correction (macpressuresolver3.cpp:203-217), warm start, assembly
-    for (int g = 0; g < ngrids; ++g) {
-        GridJob &J = jobs[g];
-        const shkz_grid *grid = J.in;
-        if (grid->scene_id) SHKZ_TRY(C->scenes->acquire(grid->scene_id, grid->w, grid->h, grid->d, &J.scene));
-        if (prm->gain && prm->target_volume) {
-            double &yprev = J.scene ? J.scene->y_prev : C->y_prev;
-            const double xv = (prm->current_volume - prm->target_volume) / prm->target_volume;
-            const double y = yprev + xv * grid->dt;
-            yprev = y;
-            const double kp = prm->gain * 2.3 / (25.0 * 0.01);
-            const double ki = kp * kp / 16.0;
-            J.rhs_correct = -(kp * xv + ki * y) / (xv + 1.0);
-        }
-        res[g].rhs_correct = J.rhs_correct;
-        const double *warm = (prm->warm_start && J.scene && J.scene->has_pressure) ? J.scene->p : nullptr;
-        const GridLevel &GL = F0.gl[g];
-        const int nbg = GL.brick_end - GL.brick_start;
-        K.go("operator", -1, 0,
-             (double)nbg * BRICK_CELLS *
-                 ((J.G.phi64 ? 8 : 4) + (grid->solid ? 4 : 0) + 12 + 1 + 6 * 8 + (P.mixed ? 16 : 0)),
-             dim3(brick_grid(nbg)), dim3(BLOCK), k_operator, J.G, F0.desc, GL, (const uint8_t *)J.cflag, P.dir64,
-             P.gx64, P.gy64, P.gz64, dir32, gx32, gy32, gz32, P.b64, P.x64, warm, J.rhs_correct);
-    }
+
     for (int l = 1; l < P.nlev; ++l) {
         LevelHost &Lf = C->levels[l - 1];
         LevelHost &Lc = C->levels[l];
