@@ -36,7 +36,12 @@ struct AdvScratch {
     uint8_t *nb[3]; // inside the narrow band: keep the semi-Lagrangian value
 };
 
-__device__ __forceinline__ float face_val(const float *U, const uint8_t *A, size_t q) { return A[q] ? U[q] : 0.0f; }
+// The value of an entry that reads 0 where inactive.  The value is loaded together with the
+// activity byte (a select, not a branch), so the two loads share one memory latency.
+__device__ __forceinline__ float face_val(const float *U, const uint8_t *A, size_t q) {
+    const float u = U[q];
+    return A[q] ? u : 0.0f;
+}
 
 __device__ __forceinline__ int clamp_i(int v, int n) { return v < 0 ? 0 : (v > n - 1 ? n - 1 : v); }
 
@@ -98,23 +103,6 @@ __device__ __forceinline__ void full_cell(const VelField &F, int i, int j, int k
     for (int a = 0; a < 3; ++a) out[a] = valid == 3 ? (double)v[a] : 0.0;
 }
 
-// interpolate<Real> over a float array of shape s; inactive entries read 0 when A is given
-__device__ __forceinline__ float interp_arr(const float *U, const uint8_t *A, const int s[3], const double p[3]) {
-#pragma clang fp contract(off)
-    const Corner c = corner_of(s, p);
-    double w[8];
-    interp_weights(c, w);
-    float val = 0.0f;
-#pragma unroll
-    for (int n = 0; n < 8; ++n)
-        if (w[n] != 0.0) {
-            const size_t q = corner_index(c, n, s);
-            const float u = A ? face_val(U, A, q) : U[q];
-            val = acc<float>(val, (double)u * w[n]);
-        }
-    return val;
-}
-
 // The 8 corner values of c (inactive entries 0 when A is given), and the interpolation and the
 // bounds computed from them: the value pass and the min/max pass share their stencil unless the
 // two departure points (float and double arithmetic) fall in different cells.
@@ -134,6 +122,14 @@ __device__ __forceinline__ float interp8(const Corner &c, const float v[8]) {
     for (int n = 0; n < 8; ++n)
         if (w[n] != 0.0) val = acc<float>(val, (double)v[n] * w[n]);
     return val;
+}
+// interpolate<Real> over a float array of shape s; inactive entries read 0 when A is given (the
+// eight corners are loaded at once)
+__device__ __forceinline__ float interp_arr(const float *U, const uint8_t *A, const int s[3], const double p[3]) {
+    const Corner c = corner_of(s, p);
+    float v[8];
+    load8(U, A, s, c, v);
+    return interp8(c, v);
 }
 __device__ __forceinline__ void bounds8(const float v[8], double &lo, double &hi) {
 #pragma unroll
@@ -167,17 +163,25 @@ __device__ __forceinline__ double interp_level(const void *phi, int f64, const i
     const Corner c = corner_of(s, p);
     double w[8];
     interp_weights(c, w);
+    // all eight corners are loaded at once (advection asks for at least 2 cells per axis, so they
+    // lie in the array); zero weights still skip the accumulation
     if (f64) {
+        double v[8];
+#pragma unroll
+        for (int n = 0; n < 8; ++n) v[n] = ((const double *)phi)[corner_index(c, n, s)];
         double val = 0.0;
 #pragma unroll
         for (int n = 0; n < 8; ++n)
-            if (w[n] != 0.0) val += ((const double *)phi)[corner_index(c, n, s)] * w[n];
+            if (w[n] != 0.0) val += v[n] * w[n];
         return val;
     }
+    float v[8];
+#pragma unroll
+    for (int n = 0; n < 8; ++n) v[n] = ((const float *)phi)[corner_index(c, n, s)];
     float val = 0.0f;
 #pragma unroll
     for (int n = 0; n < 8; ++n)
-        if (w[n] != 0.0) val = acc<float>(val, (double)((const float *)phi)[corner_index(c, n, s)] * w[n]);
+        if (w[n] != 0.0) val = acc<float>(val, (double)v[n] * w[n]);
     return (double)val;
 }
 

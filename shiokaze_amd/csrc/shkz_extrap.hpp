@@ -287,14 +287,19 @@ __device__ __forceinline__ double vel_sample(const VelField &F, const float *con
     const Corner c = corner_of(s, pos);
     double w[8];
     interp_weights(c, w);
+    // all eight corners are loaded at once (they lie in the array: the constraint asks for at
+    // least 2 cells per axis); zero weights still skip the accumulation
+    float u[8];
+#pragma unroll
+    for (int n = 0; n < 8; ++n) {
+        const size_t q = corner_index(c, n, s);
+        const float sv = Sv[b][q];
+        u[n] = F.act[b][q] ? sv : 0.0f;
+    }
     float val = 0.0f;
 #pragma unroll
     for (int n = 0; n < 8; ++n)
-        if (w[n] != 0.0) {
-            const size_t q = corner_index(c, n, s);
-            const float u = F.act[b][q] ? Sv[b][q] : 0.0f;
-            val = acc<float>(val, (double)u * w[n]);
-        }
+        if (w[n] != 0.0) val = acc<float>(val, (double)u[n] * w[n]);
     return (double)val;
 }
 
