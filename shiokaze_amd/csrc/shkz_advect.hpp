@@ -84,14 +84,17 @@ __device__ __forceinline__ void full_cell(const VelField &F, int i, int j, int k
         vf_shape(F, a, s);
         const size_t q0 = ((size_t)i * s[1] + j) * s[2] + k;
         const size_t q1 = ((size_t)(i + (a == 0)) * s[1] + (j + (a == 1))) * s[2] + (k + (a == 2));
+        // values and activity bytes load together
+        const float u0 = F.vel[a][q0], u1 = F.vel[a][q1];
+        const bool a0 = F.act[a][q0] != 0, a1 = F.act[a][q1] != 0;
         int wsum = 0;
         double value = 0.0;
-        if (F.act[a][q0]) {
-            value += (double)F.vel[a][q0];
+        if (a0) {
+            value += (double)u0;
             wsum++;
         }
-        if (F.act[a][q1]) {
-            value += (double)F.vel[a][q1];
+        if (a1) {
+            value += (double)u1;
             wsum++;
         }
         if (wsum == 2) {
