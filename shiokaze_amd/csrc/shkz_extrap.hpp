@@ -93,7 +93,7 @@ __global__ void k_pack_activity(VelField F, BitField B) {
     const size_t nw = (size_t)vf_rows(F, a) * W;
     bool on = false;
     if (w < nw) {
-        const unsigned r = (unsigned)(w / (unsigned)W), kw = (unsigned)(w - (size_t)r * W);
+        const unsigned r = (unsigned)w / (unsigned)W, kw = (unsigned)w - r * (unsigned)W;  // 32-bit: fewer than 2^32 words
         const int k = (int)(kw * 32 + (lane & 31));
         if (k < s[2]) on = F.act[a][(size_t)r * s[2] + k] != 0;
     }
@@ -114,7 +114,7 @@ __global__ void k_extrap_pass(VelField F, BitField Bi, BitField Bo) {
     const size_t w = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     const size_t nw = (size_t)vf_rows(F, a) * W;
     if (w >= nw) return;
-    const unsigned r = (unsigned)(w / (unsigned)W), kw = (unsigned)(w - (size_t)r * W);
+    const unsigned r = (unsigned)w / (unsigned)W, kw = (unsigned)w - r * (unsigned)W;  // 32-bit: fewer than 2^32 words
     const int i = (int)(r / (unsigned)s[1]), j = (int)(r - (unsigned)i * s[1]);
     const uint32_t *B = Bi.bits[a];
     const uint32_t A = B[w];
@@ -160,7 +160,7 @@ __global__ void k_unpack_activity(VelField F, BitField B0, BitField Bf) {
     if (w >= (size_t)vf_rows(F, a) * W) return;
     uint32_t fresh = Bf.bits[a][w] & ~B0.bits[a][w];
     if (!fresh) return;
-    const unsigned r = (unsigned)(w / (unsigned)W), kw = (unsigned)(w - (size_t)r * W);
+    const unsigned r = (unsigned)w / (unsigned)W, kw = (unsigned)w - r * (unsigned)W;  // 32-bit: fewer than 2^32 words
     const size_t f0 = (size_t)r * s[2] + (size_t)kw * 32;
     while (fresh) {
         const int b = __builtin_ctz(fresh);
@@ -317,9 +317,10 @@ __global__ void k_constrain(VelField F, float *X0, float *X1, float *X2, const i
     int a, i, j, k;
     size_t f;
     if (!vf_decode(F, a, f, i, j, k)) return;
+    const bool with_solid = constrain && F.solid && *exist;  // loaded with the activity byte
     const int t = F.act[a][f];
     if (!t) return;
-    if (constrain && F.solid && *exist) {
+    if (with_solid) {
         const float *const Sv[3] = {F.vel[0], F.vel[1], F.vel[2]};
         const double p[3] = {i + 0.5 * (a != 0), j + 0.5 * (a != 1), k + 0.5 * (a != 2)};
         double der[3] = {0.0, 0.0, 0.0};
