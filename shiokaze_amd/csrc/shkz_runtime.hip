@@ -145,19 +145,6 @@ static int arena_get(Arena &A, const std::string &name, size_t count, T **out) {
 }
 
 // ------------------------------------------------------------------------ hierarchy (host side)
-static inline uint64_t spread3(uint32_t v) {
-    uint64_t x = v & 0x1fffff;
-    x = (x | x << 32) & 0x1f00000000ffffULL;
-    x = (x | x << 16) & 0x1f0000ff0000ffULL;
-    x = (x | x << 8) & 0x100f00f00f00f00fULL;
-    x = (x | x << 4) & 0x10c30c30c30c30c3ULL;
-    x = (x | x << 2) & 0x1249249249249249ULL;
-    return x;
-}
-static inline uint64_t morton3(int x, int y, int z) {
-    return spread3((uint32_t)x) << 2 | spread3((uint32_t)y) << 1 | spread3((uint32_t)z);
-}
-
 // One grid at one level, before concatenation.  A grid that ended on a coarser level is empty here.
 struct GridLevelHost {
     int ni = 0, nj = 0, nk = 0, nbx = 0, nby = 0, nbz = 0;
@@ -201,6 +188,29 @@ struct LevelHost {
 
 // Concatenates the grids' active bricks (Morton order inside each grid) and builds the dense maps
 // and the neighbour table of the level.
+// Depth-first walk of the octree over one grid's brick positions, children in Morton order (x the
+// most significant bit of each triple): assigns consecutive brick ids to the active positions.
+struct MortonWalk {
+    const GridLevelHost &H;
+    LevelHost &L;
+    int bmap_off, grid, next;
+    void walk(int x0, int y0, int z0, int sz) {
+        if (x0 >= H.nbx || y0 >= H.nby || z0 >= H.nbz) return;
+        if (sz == 1) {
+            const int b = (x0 * H.nby + y0) * H.nbz + z0;
+            if (!H.active[b]) return;
+            L.bmap[bmap_off + b] = next++;
+            L.bxyz.push_back(x0);
+            L.bxyz.push_back(y0);
+            L.bxyz.push_back(z0);
+            L.bgrid.push_back(grid);
+            return;
+        }
+        const int h = sz / 2;
+        for (int c = 0; c < 8; ++c) walk(x0 + ((c >> 2) & 1) * h, y0 + ((c >> 1) & 1) * h, z0 + (c & 1) * h, h);
+    }
+};
+
 static void build_level(LevelHost &L, const std::vector<GridLevelHost> &grids) {
     const int G = (int)grids.size();
     L.gl.assign(G, GridLevel{});
@@ -221,23 +231,15 @@ static void build_level(LevelHost &L, const std::vector<GridLevelHost> &grids) {
         gl.bmap_off = (int)L.bmap.size();
         gl.brick_start = nb;
         const int npos = H.nbx * H.nby * H.nbz;
-        std::vector<std::pair<uint64_t, int>> keys;
-        for (int b = 0; b < npos; ++b)
-            if (H.active[b]) {
-                const int bz = b % H.nbz, by = (b / H.nbz) % H.nby, bx = b / (H.nbz * H.nby);
-                keys.push_back({morton3(bx, by, bz), b});
-            }
-        std::sort(keys.begin(), keys.end());
         L.bmap.resize(L.bmap.size() + npos, -1);
-        for (size_t q = 0; q < keys.size(); ++q) {
-            const int b = keys[q].second, id = nb + (int)q;
-            L.bmap[gl.bmap_off + b] = id;
-            L.bxyz.push_back(b / (H.nbz * H.nby));
-            L.bxyz.push_back((b / H.nbz) % H.nby);
-            L.bxyz.push_back(b % H.nbz);
-            L.bgrid.push_back(g);
-        }
-        nb += (int)keys.size();
+        // active positions in Morton order (x the most significant bit of each triple): a depth-
+        // first walk of the octree over the brick grid, skipping octants outside it, visits them
+        // in that order in time linear in the positions (no sort)
+        int side = 1;
+        while (side < H.nbx || side < H.nby || side < H.nbz) side *= 2;
+        MortonWalk W{H, L, gl.bmap_off, g, nb};
+        if (npos > 0) W.walk(0, 0, 0, side);
+        nb = W.next;
         gl.brick_end = nb;
         gl.coarsest = (npos > 0 && H.single()) ? 1 : 0;
         if (gl.coarsest && gl.brick_end > gl.brick_start) L.any_coarsest = true;
@@ -1447,6 +1449,7 @@ static int project_stages(shkz_ctx *C, const shkz_grid *grids, int ngrids, const
         K.go("flags", -1, 0, (double)J.M * ((G.phi64 ? 8 : 4) + 1 + (grid->solid ? 4 : 0)), dim3(brick_grid(npos)),
              dim3(BLOCK), k_flags, G, J.l0.nbx, J.l0.nby, J.l0.nbz, J.cflag, J.bflag);
     }
+    const double t_queued = now_ms();
     {
         void *hflags = nullptr;
         SHKZ_TRY(ensure_pinned(C->pin_flags, bflag_total, &hflags));
@@ -1539,8 +1542,7 @@ static int project_stages(shkz_ctx *C, const shkz_grid *grids, int ngrids, const
         SHKZ_TRY(arena_get(AR, name, total, &dmeta));
         void *hmeta = nullptr;
         SHKZ_TRY(ensure_pinned(pin, total * sizeof(int), &hmeta));
-        int *host = (int *)hmeta;
-        std::fill(host, host + total, -1);
+        int *host = (int *)hmeta;  // every table is written in full; only alignment padding is left unset
         for (int l = l0; l < l1; ++l) {
             LevelHost &L = C->levels[l];
             const Offs &o = offs[l];
@@ -1572,6 +1574,7 @@ static int project_stages(shkz_ctx *C, const shkz_grid *grids, int ngrids, const
     };
     finish_level(0);
     SHKZ_TRY(upload_tables(0, 1, false, "meta0", C->pin_meta0, true));
+    const double t_level0 = now_ms();
 
     // ---------------------------------------------------------------- operator + vectors
     LevelHost &F0 = C->levels[0];
@@ -1624,6 +1627,7 @@ static int project_stages(shkz_ctx *C, const shkz_grid *grids, int ngrids, const
              P.gx64, P.gy64, P.gz64, dir32, gx32, gy32, gz32, P.b64, P.x64, warm, J.rhs_correct);
     }
     // the coarser levels and the transfer tables, built on the host while the assembly runs
+    const double t_assembly = now_ms();
     int max_bricks = 8;
     for (int l = 1; l < P.nlev; ++l) finish_level(l);
     for (int l = 0; l < P.nlev; ++l) max_bricks = std::max(max_bricks, (int)brick_grid(C->levels[l].nbricks));
@@ -1635,6 +1639,11 @@ static int project_stages(shkz_ctx *C, const shkz_grid *grids, int ngrids, const
         SHKZ_TRY(upload_tables(0, P.nlev, true, "meta_links", C->pin_links, false));
     }
     if (C->trace) {
+        std::fprintf(stderr,
+                     "[shkz] host ms: queue labelling %.3f, wait for brick flags %.3f, level 0 %.3f, queue assembly "
+                     "%.3f, coarse levels and tables %.3f\n",
+                     t_queued - t_start, t_frac - t_queued, t_level0 - t_frac, t_assembly - t_level0,
+                     now_ms() - t_assembly);
         std::fprintf(stderr, "[shkz] %d grid(s), levels:", ngrids);
         for (int l = 0; l < P.nlev; ++l) std::fprintf(stderr, " %d%s", C->levels[l].nbricks, C->levels[l].any_coarsest ? "*" : "");
         std::fprintf(stderr, " bricks\n");
