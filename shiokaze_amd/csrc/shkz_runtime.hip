@@ -834,7 +834,7 @@ static int read_scalars(shkz_ctx *C, hipStream_t s, const SolveScalars *dS, int 
 
 static void finalize(Launcher &K, const Problem &P, const LevelDesc &L0, SolveScalars *S, BatchCtl *ctl, int op,
                      double param = 0.0, int iparam = 0, const SolveScalars *other = nullptr) {
-    K.go("finalize", -1, 0, 0, dim3((unsigned)P.ngrids), dim3(1024), k_finalize, (const double *)P.red.partials, L0, S,
+    K.go("finalize", -1, 0, 0, dim3((unsigned)P.ngrids), dim3(256), k_finalize, (const double *)P.red.partials, L0, S,
          ctl, op, param, iparam, other);
 }
 
