@@ -126,6 +126,33 @@ __device__ __forceinline__ double negative_area(const double v[4]) {
     return 0.5 * sum;
 }
 
+// Open fraction of an interior 3D face from its four solid corner samples (corner order of
+// macutility3.cpp:123-138 mapped to (0,0),(1,0),(1,1),(0,1)), float as stored by the reference.
+__device__ __forceinline__ float area_from_corners(const GridDesc &G, const double v[4]) {
+#pragma clang fp contract(off)
+    double a = 1.0 - negative_area(v);
+    if (a && a < G.eps_solid) a = G.eps_solid;
+    float af = (float)a;
+    if (!G.so_solid && af) af = 1.0f;
+    return af;
+}
+
+// Liquid fraction of a face from the level-set samples on its two sides, float as stored by the
+// reference (face_rho with the samples given).
+__device__ __forceinline__ float rho_from_samples(const GridDesc &G, const Exist &E, double phi0, double phi1) {
+#pragma clang fp contract(off)
+    double r;
+    if (E.liquid) {
+        r = liquid_fraction(phi0, phi1);
+        if (r && r < G.eps_fluid) r = G.eps_fluid;
+    } else {
+        r = 1.0;
+    }
+    float rf = (float)r;
+    if (!G.so_fluid && rf) rf = 1.0f;
+    return rf;
+}
+
 // Open fraction of face (dim; i,j,k), float as stored by the reference.
 __device__ __forceinline__ float face_area(const GridDesc &G, const Exist &E, int dim, int i, int j, int k) {
 #pragma clang fp contract(off)
@@ -159,7 +186,7 @@ __device__ __forceinline__ float face_area(const GridDesc &G, const Exist &E, in
                 v[2] = solid_at(G, i + 1, j + 1, k);
                 v[3] = solid_at(G, i, j + 1, k);
             }
-            a = 1.0 - negative_area(v);
+            return area_from_corners(G, v);
         }
         if (a && a < G.eps_solid) a = G.eps_solid;
     } else {
@@ -173,20 +200,11 @@ __device__ __forceinline__ float face_area(const GridDesc &G, const Exist &E, in
 // Liquid fraction of face (dim; i,j,k) between cell (i,j,k) and (i,j,k)-e_dim, clamped into the
 // grid, float as stored by the reference.
 __device__ __forceinline__ float face_rho(const GridDesc &G, const Exist &E, int dim, int i, int j, int k) {
-#pragma clang fp contract(off)
-    double r;
-    if (E.liquid) {
-        const int i0 = clampi(i, 0, G.w - 1), j0 = clampi(j, 0, G.h - 1), k0 = clampi(k, 0, G.d - 1);
-        const int i1 = clampi(i - (dim == 0), 0, G.w - 1), j1 = clampi(j - (dim == 1), 0, G.h - 1),
-                  k1 = clampi(k - (dim == 2), 0, G.d - 1);
-        r = liquid_fraction(phi_at(G, i0, j0, k0), phi_at(G, i1, j1, k1));
-        if (r && r < G.eps_fluid) r = G.eps_fluid;
-    } else {
-        r = 1.0;
-    }
-    float rf = (float)r;
-    if (!G.so_fluid && rf) rf = 1.0f;
-    return rf;
+    if (!E.liquid) return rho_from_samples(G, E, 0.0, 0.0);
+    const int i0 = clampi(i, 0, G.w - 1), j0 = clampi(j, 0, G.h - 1), k0 = clampi(k, 0, G.d - 1);
+    const int i1 = clampi(i - (dim == 0), 0, G.w - 1), j1 = clampi(j - (dim == 1), 0, G.h - 1),
+              k1 = clampi(k - (dim == 2), 0, G.d - 1);
+    return rho_from_samples(G, E, phi_at(G, i0, j0, k0), phi_at(G, i1, j1, k1));
 }
 
 __device__ __forceinline__ float face_area(const GridDesc &G, int dim, int i, int j, int k) {

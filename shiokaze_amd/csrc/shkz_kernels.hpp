@@ -288,7 +288,65 @@ __global__ __launch_bounds__(BLOCK) void k_operator(GridDesc G, LevelDesc L, Gri
     const bool inside = i < G.w && j < G.h && k < G.d;
     const bool unknown = inside && (cflag[cell_index(G, i, j, k)] & 2);
     double dir = -1.0, g[3] = {0.0, 0.0, 0.0}, rhs = 0.0, x0 = 0.0;
-    if (unknown) {
+    if (unknown && !G.flat) {
+        // 3D: the cell's 8 solid corners, 7 level-set samples and 6 face velocities are loaded at
+        // once; each face's area and liquid fraction then use the face functions' arithmetic
+        const Exist E = exist_of(G);
+        const double dt = G.dt, dx = G.dx;
+        const int di[6] = {1, -1, 0, 0, 0, 0}, dj[6] = {0, 0, 1, -1, 0, 0}, dk[6] = {0, 0, 0, 0, 1, -1};
+        const int sgn[6] = {1, -1, 1, -1, 1, -1};
+        double sc[8];  // solid at node (i + a, j + b, k + c), index (a * 2 + b) * 2 + c
+        if (G.solid) {
+#pragma unroll
+            for (int n = 0; n < 8; ++n) sc[n] = solid_at(G, i + (n >> 2), j + ((n >> 1) & 1), k + (n & 1));
+        }
+        double ph[6];  // level set at the six neighbours (clamped into the grid: out-of-grid ones go unused)
+        float uf[6];   // velocity at the six faces
+#pragma unroll
+        for (int nq = 0; nq < 6; ++nq) {
+            ph[nq] = phi_at(G, clampi(i + di[nq], 0, G.w - 1), clampi(j + dj[nq], 0, G.h - 1), clampi(k + dk[nq], 0, G.d - 1));
+            const int dim = nq >> 1, o = (nq & 1) ? 0 : 1;
+            uf[nq] = G.vel[dim][face_index(G, dim, i + (dim == 0) * o, j + (dim == 1) * o, k + (dim == 2) * o)];
+        }
+        const double pc = phi_at(G, i, j, k);
+        double dsum = 0.0;
+#pragma unroll
+        for (int nq = 0; nq < 6; ++nq) {
+            const int qi = i + di[nq], qj = j + dj[nq], qk = k + dk[nq];
+            if (qi < 0 || qi >= G.w || qj < 0 || qj >= G.h || qk < 0 || qk >= G.d) continue;
+            const int dim = nq >> 1, o = (nq & 1) ? 0 : 1;  // the face lies at offset o along dim
+            float area = 1.0f;  // an interior face without solid (face_area's no-solid branch)
+            if (G.solid) {
+                // face_area's corner order for the face of axis dim at offset o
+                double v[4];
+                auto n = [&](int a, int b, int c) { return sc[(a * 2 + b) * 2 + c]; };
+                if (dim == 0) {
+                    v[0] = n(o, 0, 0); v[1] = n(o, 1, 0); v[2] = n(o, 1, 1); v[3] = n(o, 0, 1);
+                } else if (dim == 1) {
+                    v[0] = n(0, o, 0); v[1] = n(1, o, 0); v[2] = n(1, o, 1); v[3] = n(0, o, 1);
+                } else {
+                    v[0] = n(0, 0, o); v[1] = n(1, 0, o); v[2] = n(1, 1, o); v[3] = n(0, 1, o);
+                }
+                area = area_from_corners(G, v);
+            }
+            if (area) {
+                // face_rho: the sample at the face's index cell first, then the one below it
+                const double rho = o ? rho_from_samples(G, E, ph[nq], pc) : rho_from_samples(G, E, pc, ph[nq]);
+                if (rho) {
+                    const double value = dt * (double)area / (dx * dx * rho);
+                    if (ph[nq] < 0.0) {
+                        if (nq & 1) g[dim] = value;
+                    } else {
+                        dsum += value;
+                    }
+                }
+                rhs += -sgn[nq] * (double)area * (double)uf[nq] / dx;
+            }
+        }
+        dir = dsum;
+        rhs += rhs_correct;
+        if (warm) x0 = warm[cell_index(G, i, j, k)];
+    } else if (unknown) {
         const Exist E = exist_of(G);
         const double dt = G.dt, dx = G.dx;
         const int di[6] = {1, -1, 0, 0, 0, 0}, dj[6] = {0, 0, 1, -1, 0, 0}, dk[6] = {0, 0, 0, 0, 1, -1};
