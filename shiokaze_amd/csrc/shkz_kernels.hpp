@@ -1883,7 +1883,7 @@ __global__ void k_export_system(GridDesc G, LevelDesc L, GridLevel GL, OpDesc<do
 // the domain boundary become 0, interior open faces lose dt*(P1-P0)/(rho*dx) with P stored in
 // float and P1-P0 taken in float; all other (active) faces are switched off (0).
 __device__ __forceinline__ void update_face(const GridDesc &G, const Exist &E, int dim, int i, int j, int k, float p1,
-                                            float p0) {
+                                            float p0, float rho_face) {
 #pragma clang fp contract(off)
     const size_t f = face_index(G, dim, i, j, k);
     uint8_t *aout = G.active_out[dim];
@@ -1895,7 +1895,7 @@ __device__ __forceinline__ void update_face(const GridDesc &G, const Exist &E, i
     const int N[3] = {G.w, G.h, G.d};
     float *U = G.vel[dim];
     // the area only matters where rho is nonzero: most faces of the domain have air on both sides
-    const double rho = face_rho(G, E, dim, i, j, k);
+    const double rho = rho_face;
     const float area = rho != 0.0 ? face_area(G, E, dim, i, j, k) : 0.0f;
     uint8_t keep = 1;
     if (area != 0.0f && rho != 0.0) {
@@ -1928,9 +1928,25 @@ __global__ void k_velocity_update(GridDesc G, const float *__restrict__ P) {
     auto p_at = [&](int a, int b, int c) { return (a >= 0 && b >= 0 && c >= 0) ? P[cell_index(G, a, b, c)] : 0.0f; };
     const float pc = (in_i && in_j && in_k) ? P[cell_index(G, i, j, k)] : 0.0f;
     const Exist E = exist_of(G);
-    if (in_j && in_k) update_face(G, E, 0, i, j, k, pc, i > 0 ? p_at(i - 1, j, k) : 0.0f);
-    if (in_i && in_k) update_face(G, E, 1, i, j, k, pc, j > 0 ? p_at(i, j - 1, k) : 0.0f);
-    if (in_i && in_j && !G.flat) update_face(G, E, 2, i, j, k, pc, k > 0 ? p_at(i, j, k - 1) : 0.0f);
+    // face_rho of the three faces from four level-set samples, read once: the (clamped) node cell
+    // and the (clamped) cell below it along each axis
+    const bool v0 = in_j && in_k, v1 = in_i && in_k, v2 = in_i && in_j && !G.flat;
+    float rho[3] = {1.0f, 1.0f, 1.0f};
+    if (E.liquid) {
+        const int ci = clampi(i, 0, G.w - 1), cj = clampi(j, 0, G.h - 1), ck = clampi(k, 0, G.d - 1);
+        const double s0 = phi_at(G, ci, cj, ck);
+        const double sx = v0 ? phi_at(G, clampi(i - 1, 0, G.w - 1), cj, ck) : 0.0;
+        const double sy = v1 ? phi_at(G, ci, clampi(j - 1, 0, G.h - 1), ck) : 0.0;
+        const double sz = v2 ? phi_at(G, ci, cj, clampi(k - 1, 0, G.d - 1)) : 0.0;
+        rho[0] = rho_from_samples(G, E, s0, sx);
+        rho[1] = rho_from_samples(G, E, s0, sy);
+        rho[2] = rho_from_samples(G, E, s0, sz);
+    } else {
+        rho[0] = rho[1] = rho[2] = rho_from_samples(G, E, 0.0, 0.0);
+    }
+    if (v0) update_face(G, E, 0, i, j, k, pc, i > 0 ? p_at(i - 1, j, k) : 0.0f, rho[0]);
+    if (v1) update_face(G, E, 1, i, j, k, pc, j > 0 ? p_at(i, j - 1, k) : 0.0f, rho[1]);
+    if (v2) update_face(G, E, 2, i, j, k, pc, k > 0 ? p_at(i, j, k - 1) : 0.0f, rho[2]);
 }
 
 }  // namespace shkz
