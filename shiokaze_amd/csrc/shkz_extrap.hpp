@@ -82,23 +82,36 @@ struct BitField {
 
 __device__ __forceinline__ int vf_rows(const VelField &F, int a) { return (F.w + (a == 0)) * (F.h + (a == 1)); }
 
-// activity bytes -> bits: one lane per face, a wavefront ballot forms two words
+// activity bytes -> bits: one lane per face and word bit, a wavefront ballot forms two words; each
+// wavefront packs PACK_GROUPS consecutive groups of 64 faces (its loads all issued first), so the
+// launch has a quarter of the wavefronts of one group each.
+constexpr int PACK_GROUPS = 4;
 __global__ void k_pack_activity(VelField F, BitField B) {
     const int a = (int)blockIdx.y;
     int s[3];
     vf_shape(F, a, s);
     const int W = B.W[a];
-    const size_t lane = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const size_t w = lane >> 5;
     const size_t nw = (size_t)vf_rows(F, a) * W;
-    bool on = false;
-    if (w < nw) {
-        const unsigned r = (unsigned)w / (unsigned)W, kw = (unsigned)w - r * (unsigned)W;  // 32-bit: fewer than 2^32 words
-        const int k = (int)(kw * 32 + (lane & 31));
-        if (k < s[2]) on = F.act[a][(size_t)r * s[2] + k] != 0;
+    const unsigned lane = threadIdx.x & 63;
+    const size_t wave = ((size_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    bool on[PACK_GROUPS];
+#pragma unroll
+    for (int g = 0; g < PACK_GROUPS; ++g) {
+        const size_t slot = (wave * PACK_GROUPS + g) * 64 + lane;
+        const size_t w = slot >> 5;
+        on[g] = false;
+        if (w < nw) {
+            const unsigned r = (unsigned)w / (unsigned)W, kw = (unsigned)w - r * (unsigned)W;  // 32-bit: fewer than 2^32 words
+            const int k = (int)(kw * 32 + (lane & 31));
+            if (k < s[2]) on[g] = F.act[a][(size_t)r * s[2] + k] != 0;
+        }
     }
-    const unsigned long long m = __ballot(on);
-    if (w < nw && (lane & 31) == 0) B.bits[a][w] = (uint32_t)((threadIdx.x & 32) ? (m >> 32) : m);
+#pragma unroll
+    for (int g = 0; g < PACK_GROUPS; ++g) {
+        const unsigned long long m = __ballot(on[g]);
+        const size_t w = ((wave * PACK_GROUPS + g) * 64 + lane) >> 5;
+        if (w < nw && (lane & 31) == 0) B.bits[a][w] = (uint32_t)((lane & 32) ? (m >> 32) : m);
+    }
 }
 
 // One extrapolation pass: Bo = Bi plus every inactive face with an in-bounds neighbour active in

@@ -2128,7 +2128,8 @@ static int extrapolate_stages(shkz_ctx *C, const shkz_velocity_field *f, int wid
                 B[c].W[a] = W[a];
                 off += (size_t)(f->w + (a == 0)) * (f->h + (a == 1)) * W[a];
             }
-        const dim3 gpack((unsigned)((wmax * 32 + 255) / 256), 3), gword((unsigned)((wmax + 255) / 256), 3);
+        const dim3 gpack((unsigned)((wmax * 32 + 256 * PACK_GROUPS - 1) / (256 * PACK_GROUPS)), 3),
+            gword((unsigned)((wmax + 255) / 256), 3);
         // byte models: the pack reads the activity bytes and writes the bits; a pass reads and
         // writes the bits (the values of the faces it activates are few)
         K.go("extrap_pack", -1, 0, (double)total + 4.0 * words, gpack, block, k_pack_activity, F, B[0]);
