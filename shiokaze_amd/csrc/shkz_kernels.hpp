@@ -442,6 +442,43 @@ __global__ __launch_bounds__(BLOCK) void k_coarsen(LevelDesc Lf, OpDesc<Ti> Af, 
     gz[c] = any ? (To)(0.5 * sg[2]) : (To)0;
 }
 
+// Brick (x, y, z) of grid g on a level, -1 outside the grid or where none is stored.
+__device__ __forceinline__ int brick_at(const LevelDesc &L, int g, int x, int y, int z) {
+    const GridLevel &gl = L.gl[g];
+    if (x < 0 || y < 0 || z < 0 || x >= gl.nbx || y >= gl.nby || z >= gl.nbz) return -1;
+    return L.bmap[gl.bmap_off + (x * gl.nby + y) * gl.nbz + z];
+}
+
+// Transfer and neighbourhood tables of one level L from the base tables of L and its neighbours
+// in the hierarchy (Lf one level finer, Lc one coarser; a table is skipped where its pointer is
+// null).  One thread per (brick, slot), 99 slots: 64 of child64 (the finer bricks 2B-1 .. 2B+2
+// per axis), 8 of parent8 (the coarser bricks (B+1)/2 - 1 + {0,1} per axis; left at -1 for a grid
+// that ends on this level) and 27 of nb27 (B-1 .. B+1 per axis, in k_smooth_k's order).
+constexpr int LINK_SLOTS = 64 + 8 + 27;
+__global__ __launch_bounds__(256) void k_links(LevelDesc L, LevelDesc Lf, LevelDesc Lc, int *child64, int *parent8,
+                                               int *nb27) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int br = t / LINK_SLOTS, q = t - br * LINK_SLOTS;
+    if (br >= L.nbricks) return;
+    const int g = L.meta[br].grid;
+    const int X = L.bxyz[3 * br], Y = L.bxyz[3 * br + 1], Z = L.bxyz[3 * br + 2];
+    if (q < 64) {
+        if (child64)
+            child64[64 * (size_t)br + q] = brick_at(Lf, g, 2 * X - 1 + (q >> 4), 2 * Y - 1 + ((q >> 2) & 3), 2 * Z - 1 + (q & 3));
+    } else if (q < 72) {
+        if (parent8) {
+            const int c = q - 64;
+            parent8[8 * (size_t)br + c] =
+                L.gl[g].coarsest ? -1
+                                 : brick_at(Lc, g, (X + 1) / 2 - 1 + (c >> 2), (Y + 1) / 2 - 1 + ((c >> 1) & 1),
+                                            (Z + 1) / 2 - 1 + (c & 1));
+        }
+    } else if (nb27) {
+        const int c = q - 72;
+        nb27[27 * (size_t)br + c] = brick_at(L, g, X + c / 9 - 1, Y + (c / 3) % 3 - 1, Z + c % 3 - 1);
+    }
+}
+
 // The smoothers use a stored inverse diagonal: damped Jacobi is
 //   z' = (1 - omega) z + omega * idiag * (rhs + sum_f g_f z_q),
 // and idiag = 0 marks a non-unknown (or a cell whose diagonal vanished).

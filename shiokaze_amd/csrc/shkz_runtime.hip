@@ -190,13 +190,18 @@ struct LevelHost {
 // and the neighbour table of the level.
 // Depth-first walk of the octree over one grid's brick positions, children in Morton order (x the
 // most significant bit of each triple): assigns consecutive brick ids to the active positions.
+// An octant of 2^m x 2^m x 2^m positions is skipped when the grid's mask m levels coarser (when
+// the hierarchy has it) shows no active position in it: a coarse position is active exactly when
+// one of its children is.
 struct MortonWalk {
+    const std::vector<std::vector<GridLevelHost>> &levels;
+    int level;  // the level of H in levels
     const GridLevelHost &H;
     LevelHost &L;
     int bmap_off, grid, next;
-    void walk(int x0, int y0, int z0, int sz) {
+    void walk(int x0, int y0, int z0, int m) {
         if (x0 >= H.nbx || y0 >= H.nby || z0 >= H.nbz) return;
-        if (sz == 1) {
+        if (m == 0) {
             const int b = (x0 * H.nby + y0) * H.nbz + z0;
             if (!H.active[b]) return;
             L.bmap[bmap_off + b] = next++;
@@ -206,12 +211,17 @@ struct MortonWalk {
             L.bgrid.push_back(grid);
             return;
         }
-        const int h = sz / 2;
-        for (int c = 0; c < 8; ++c) walk(x0 + ((c >> 2) & 1) * h, y0 + ((c >> 1) & 1) * h, z0 + (c & 1) * h, h);
+        if (level + m < (int)levels.size()) {
+            const GridLevelHost &C = levels[level + m][grid];
+            if (!C.active.empty() && !C.active[((x0 >> m) * C.nby + (y0 >> m)) * C.nbz + (z0 >> m)]) return;
+        }
+        const int h = 1 << (m - 1);
+        for (int c = 0; c < 8; ++c) walk(x0 + ((c >> 2) & 1) * h, y0 + ((c >> 1) & 1) * h, z0 + (c & 1) * h, m - 1);
     }
 };
 
-static void build_level(LevelHost &L, const std::vector<GridLevelHost> &grids) {
+static void build_level(LevelHost &L, const std::vector<std::vector<GridLevelHost>> &levels, int level) {
+    const std::vector<GridLevelHost> &grids = levels[level];
     const int G = (int)grids.size();
     L.gl.assign(G, GridLevel{});
     L.bmap.clear();
@@ -233,12 +243,12 @@ static void build_level(LevelHost &L, const std::vector<GridLevelHost> &grids) {
         const int npos = H.nbx * H.nby * H.nbz;
         L.bmap.resize(L.bmap.size() + npos, -1);
         // active positions in Morton order (x the most significant bit of each triple): a depth-
-        // first walk of the octree over the brick grid, skipping octants outside it, visits them
-        // in that order in time linear in the positions (no sort)
-        int side = 1;
-        while (side < H.nbx || side < H.nby || side < H.nbz) side *= 2;
-        MortonWalk W{H, L, gl.bmap_off, g, nb};
-        if (npos > 0) W.walk(0, 0, 0, side);
+        // first walk of the octree over the brick grid, skipping octants outside it or with no
+        // active position, visits them in that order (no sort)
+        int m = 0;
+        while ((1 << m) < H.nbx || (1 << m) < H.nby || (1 << m) < H.nbz) ++m;
+        MortonWalk W{levels, level, H, L, gl.bmap_off, g, nb};
+        if (npos > 0) W.walk(0, 0, 0, m);
         nb = W.next;
         gl.brick_end = nb;
         gl.coarsest = (npos > 0 && H.single()) ? 1 : 0;
@@ -428,6 +438,7 @@ struct shkz_ctx {
     hipEvent_t ev_in = nullptr, ev_out = nullptr;
     bool use_graphs = true;
     bool trace = false;
+    bool host_links = false;  // SHKZ_HOST_LINKS=1: transfer tables built on the host (a cross-check)
     std::string device_key;  // device identity of the tuning cache
     // replayable PCG iteration graphs, keyed by everything their kernel arguments depend on
     struct GraphSlot {
@@ -1491,10 +1502,11 @@ static int project_stages(shkz_ctx *C, const shkz_grid *grids, int ngrids, const
                 if (!H.coarsens()) continue;  // ended on this or a finer level
                 grid_level_dims(N, (H.ni + 1) / 2, (H.nj + 1) / 2, (H.nk + 1) / 2);
                 N.active.assign((size_t)N.nbx * N.nby * N.nbz, 0);
-                for (int b = 0; b < H.nbx * H.nby * H.nbz; ++b)
-                    if (H.active[b]) {
-                        const int bz = b % H.nbz, by = (b / H.nbz) % H.nby, bx = b / (H.nbz * H.nby);
-                        N.active[((size_t)(bx / 2) * N.nby + by / 2) * N.nbz + bz / 2] = 1;
+                const uint8_t *a = H.active.data();
+                for (int bx = 0; bx < H.nbx; ++bx)
+                    for (int by = 0; by < H.nby; ++by) {
+                        uint8_t *row = N.active.data() + ((size_t)(bx / 2) * N.nby + by / 2) * N.nbz;
+                        for (int bz = 0; bz < H.nbz; ++bz, ++a) row[bz / 2] |= *a;
                     }
             }
             glevels.push_back(std::move(next));
@@ -1506,7 +1518,7 @@ static int project_stages(shkz_ctx *C, const shkz_grid *grids, int ngrids, const
     // the coarser levels and the transfer tables.
     auto finish_level = [&](int l) {
         LevelHost &L = C->levels[l];
-        build_level(L, glevels[l]);
+        build_level(L, glevels, l);
         if (!multigrid) {
             // plain CG: no coarse solve anywhere
             L.any_coarsest = false;
@@ -1631,12 +1643,43 @@ static int project_stages(shkz_ctx *C, const shkz_grid *grids, int ngrids, const
     int max_bricks = 8;
     for (int l = 1; l < P.nlev; ++l) finish_level(l);
     for (int l = 0; l < P.nlev; ++l) max_bricks = std::max(max_bricks, (int)brick_grid(C->levels[l].nbricks));
-    for (int l = 0; l + 1 < P.nlev; ++l) link_levels(C->levels[l], C->levels[l + 1]);
-    for (int l = 1; l < P.nlev; ++l) neighbourhoods(C->levels[l]);
-    {
-        // levels >= 1: their base tables; every level: its transfer and neighbourhood tables
-        SHKZ_TRY(upload_tables(1, P.nlev, false, "meta", C->pin_meta, true));
+    // levels >= 1: their base tables
+    SHKZ_TRY(upload_tables(1, P.nlev, false, "meta", C->pin_meta, true));
+    if (C->host_links) {
+        for (int l = 0; l + 1 < P.nlev; ++l) link_levels(C->levels[l], C->levels[l + 1]);
+        for (int l = 1; l < P.nlev; ++l) neighbourhoods(C->levels[l]);
         SHKZ_TRY(upload_tables(0, P.nlev, true, "meta_links", C->pin_links, false));
+    } else {
+        // every level's transfer and neighbourhood tables, built on the device from the base
+        // tables: child64 on levels >= 1, parent8 on all but the coarsest, nb27 on levels >= 1
+        std::vector<size_t> o64(P.nlev), o8(P.nlev), o27(P.nlev);
+        size_t total = 0;
+        for (int l = 0; l < P.nlev; ++l) {
+            const size_t nb = (size_t)C->levels[l].nbricks;
+            o64[l] = total;
+            total += l >= 1 ? 64 * nb : 0;
+            o8[l] = total;
+            total += l + 1 < P.nlev ? 8 * nb : 0;
+            o27[l] = total;
+            total += l >= 1 ? 27 * nb : 0;
+        }
+        int *dl = nullptr;
+        SHKZ_TRY(arena_get(AR, "links", std::max<size_t>(total, 1), &dl));
+        for (int l = 0; l < P.nlev; ++l) {
+            LevelDesc &D = C->levels[l].desc;
+            const bool any = C->levels[l].nbricks > 0;
+            D.child64 = (l >= 1 && any) ? dl + o64[l] : nullptr;
+            D.parent8 = (l + 1 < P.nlev && any) ? dl + o8[l] : nullptr;
+            D.nb27 = (l >= 1 && any) ? dl + o27[l] : nullptr;
+        }
+        for (int l = 0; l < P.nlev; ++l) {
+            const LevelHost &L = C->levels[l];
+            if (L.nbricks == 0 || (!L.desc.child64 && !L.desc.parent8 && !L.desc.nb27)) continue;
+            const LevelDesc &Lf = C->levels[l > 0 ? l - 1 : l].desc, &Lc = C->levels[l + 1 < P.nlev ? l + 1 : l].desc;
+            K.go("links", -1, 0, 0, dim3((unsigned)(((size_t)L.nbricks * LINK_SLOTS + 255) / 256)), dim3(256), k_links,
+                 L.desc, Lf, Lc, const_cast<int *>(L.desc.child64), const_cast<int *>(L.desc.parent8),
+                 const_cast<int *>(L.desc.nb27));
+        }
     }
     if (C->trace) {
         std::fprintf(stderr,
@@ -2354,6 +2397,8 @@ int shkz_create(int device, uint64_t hbm_budget_bytes, shkz_ctx **out) {
     C->use_graphs = !(ng && ng[0] == '1');
     const char *tr = std::getenv("SHKZ_TRACE");
     C->trace = tr && tr[0] == '1';
+    const char *hl = std::getenv("SHKZ_HOST_LINKS");
+    C->host_links = hl && hl[0] == '1';
     *out = C;
     return SHKZ_OK;
 }
