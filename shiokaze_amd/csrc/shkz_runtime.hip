@@ -1070,6 +1070,8 @@ static int solve_phase(shkz_ctx *C, Launcher &K, Problem &P, const shkz_params *
     outer_passes.assign(ngrids, 0);
     status.assign(ngrids, SHKZ_OK);
     std::vector<SolveScalars> hs(ngrids), hi, hf;
+    // set when the last outer defect was taken at the returned pressure of every grid
+    bool final_measured = false;
     if (F0.nbricks > 0) {
         K.go("init_scalars", -1, 0, 0, dim3((unsigned)((ngrids + 255) / 256)), dim3(256), k_init_scalars, P.S_out,
              ngrids, norm, maxit);
@@ -1113,7 +1115,10 @@ static int solve_phase(shkz_ctx *C, Launcher &K, Problem &P, const shkz_params *
             SHKZ_TRY(read_scalars(C, s, P.S_out, ngrids, hs, P.ctl + 1, &hc));
             if (outer > 0)
                 for (int g = 0; g < ngrids; ++g) total_iters[g] = hs[g].iter;
-            if (hc.all_done) break;
+            if (hc.all_done) {
+                final_measured = true;
+                break;
+            }
             for (int g = 0; g < ngrids; ++g)
                 if (!hs[g].done) outer_passes[g] = outer + 1;
             SHKZ_TRY(krylov<float>(C, K, P, (float *)P.kx, (float *)P.kr, (float *)P.kp, (float *)P.kq, (float *)P.kz,
@@ -1132,8 +1137,12 @@ static int solve_phase(shkz_ctx *C, Launcher &K, Problem &P, const shkz_params *
         for (int g = 0; g < ngrids; ++g)
             if (hs[g].status) status[g] = SHKZ_ERR_NUMERIC;
     }
-    // true FP64 residual of the returned pressure, per grid
-    if (F0.nbricks > 0) {
+    // true FP64 residual of the returned pressure, per grid.  When the defect correction ended
+    // on a pass whose defect found every grid done, that defect was taken at the returned
+    // pressure (a grid done earlier only gained zero corrections since), so its norms are kept.
+    if (final_measured) {
+        hf = hs;
+    } else if (F0.nbricks > 0) {
         K.go("defect", 0, 7 * 8, -usave(C, 0, 32), gb, dim3(ROW4_THREADS), k_defect4, F0.desc, A64,
              (const double *)P.b64, (const double *)P.x64, P.r64, P.red);
         finalize(K, P, F0.desc, P.S_fin, nullptr, FIN_DEFECT);
