@@ -54,6 +54,21 @@ __device__ __forceinline__ double solid_at(const GridDesc &G, int i, int j, int 
 }
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
+// N samples of a float-or-double array at in-range indices, with one branch on the element type
+// for the group: the loads are issued together instead of one type test (and one wait) apiece.
+template <int N>
+__device__ __forceinline__ void load_samples(const void *base, bool is64, const size_t (&idx)[N], double (&out)[N]) {
+    if (is64) {
+        const double *a = (const double *)base;
+#pragma unroll
+        for (int q = 0; q < N; ++q) out[q] = a[idx[q]];
+    } else {
+        const float *a = (const float *)base;
+#pragma unroll
+        for (int q = 0; q < N; ++q) out[q] = (double)a[idx[q]];
+    }
+}
+
 // The grid's "has a negative value" flags (liquid, solid), read once per thread: the face
 // functions would otherwise reload them for every face, behind every store that might alias them.
 // The solid flag is always set: with no negative solid sample, every face's marching-squares
@@ -169,23 +184,12 @@ __device__ __forceinline__ float face_area(const GridDesc &G, const Exist &E, in
             a = 1.0 - liquid_fraction(solid_at(G, i, j, 0), solid_at(G, i + (dim != 0), j + (dim != 1), 0));
         } else {
             // corner order of macutility3.cpp:123-138 mapped to (0,0),(1,0),(1,1),(0,1)
+            // the face's second and third corner step along a and b (the two axes other than dim)
+            const int ai = dim != 0, aj = dim == 0, bj = dim == 2, bk = dim != 2;
+            const size_t idx[4] = {node_index(G, i, j, k), node_index(G, i + ai, j + aj, k),
+                                   node_index(G, i + ai, j + aj + bj, k + bk), node_index(G, i, j + bj, k + bk)};
             double v[4];
-            if (dim == 0) {
-                v[0] = solid_at(G, i, j, k);
-                v[1] = solid_at(G, i, j + 1, k);
-                v[2] = solid_at(G, i, j + 1, k + 1);
-                v[3] = solid_at(G, i, j, k + 1);
-            } else if (dim == 1) {
-                v[0] = solid_at(G, i, j, k);
-                v[1] = solid_at(G, i + 1, j, k);
-                v[2] = solid_at(G, i + 1, j, k + 1);
-                v[3] = solid_at(G, i, j, k + 1);
-            } else {
-                v[0] = solid_at(G, i, j, k);
-                v[1] = solid_at(G, i + 1, j, k);
-                v[2] = solid_at(G, i + 1, j + 1, k);
-                v[3] = solid_at(G, i, j + 1, k);
-            }
+            load_samples<4>(G.solid, G.solid64, idx, v);
             return area_from_corners(G, v);
         }
         if (a && a < G.eps_solid) a = G.eps_solid;
@@ -204,7 +208,10 @@ __device__ __forceinline__ float face_rho(const GridDesc &G, const Exist &E, int
     const int i0 = clampi(i, 0, G.w - 1), j0 = clampi(j, 0, G.h - 1), k0 = clampi(k, 0, G.d - 1);
     const int i1 = clampi(i - (dim == 0), 0, G.w - 1), j1 = clampi(j - (dim == 1), 0, G.h - 1),
               k1 = clampi(k - (dim == 2), 0, G.d - 1);
-    return rho_from_samples(G, E, phi_at(G, i0, j0, k0), phi_at(G, i1, j1, k1));
+    const size_t idx[2] = {cell_index(G, i0, j0, k0), cell_index(G, i1, j1, k1)};
+    double ph[2];
+    load_samples<2>(G.phi, G.phi64, idx, ph);
+    return rho_from_samples(G, E, ph[0], ph[1]);
 }
 
 __device__ __forceinline__ float face_area(const GridDesc &G, int dim, int i, int j, int k) {

@@ -1969,21 +1969,25 @@ __global__ void k_velocity_update(GridDesc G, const float *__restrict__ P) {
     // and the (clamped) cell below it along each axis
     const bool v0 = in_j && in_k, v1 = in_i && in_k, v2 = in_i && in_j && !G.flat;
     float rho[3] = {1.0f, 1.0f, 1.0f};
+    // the four samples and the three low-side pressures are loaded together; the clamped indices
+    // are in range for every node, and a face that does not exist never uses its values
+    const int ci = clampi(i, 0, G.w - 1), cj = clampi(j, 0, G.h - 1), ck = clampi(k, 0, G.d - 1);
+    const int ci1 = clampi(i - 1, 0, G.w - 1), cj1 = clampi(j - 1, 0, G.h - 1), ck1 = clampi(k - 1, 0, G.d - 1);
+    const size_t idx[4] = {cell_index(G, ci, cj, ck), cell_index(G, ci1, cj, ck), cell_index(G, ci, cj1, ck),
+                           cell_index(G, ci, cj, ck1)};
     if (E.liquid) {
-        const int ci = clampi(i, 0, G.w - 1), cj = clampi(j, 0, G.h - 1), ck = clampi(k, 0, G.d - 1);
-        const double s0 = phi_at(G, ci, cj, ck);
-        const double sx = v0 ? phi_at(G, clampi(i - 1, 0, G.w - 1), cj, ck) : 0.0;
-        const double sy = v1 ? phi_at(G, ci, clampi(j - 1, 0, G.h - 1), ck) : 0.0;
-        const double sz = v2 ? phi_at(G, ci, cj, clampi(k - 1, 0, G.d - 1)) : 0.0;
-        rho[0] = rho_from_samples(G, E, s0, sx);
-        rho[1] = rho_from_samples(G, E, s0, sy);
-        rho[2] = rho_from_samples(G, E, s0, sz);
+        double sm[4];
+        load_samples<4>(G.phi, G.phi64, idx, sm);
+        rho[0] = rho_from_samples(G, E, sm[0], v0 ? sm[1] : 0.0);
+        rho[1] = rho_from_samples(G, E, sm[0], v1 ? sm[2] : 0.0);
+        rho[2] = rho_from_samples(G, E, sm[0], v2 ? sm[3] : 0.0);
     } else {
         rho[0] = rho[1] = rho[2] = rho_from_samples(G, E, 0.0, 0.0);
     }
-    if (v0) update_face(G, E, 0, i, j, k, pc, i > 0 ? p_at(i - 1, j, k) : 0.0f, rho[0]);
-    if (v1) update_face(G, E, 1, i, j, k, pc, j > 0 ? p_at(i, j - 1, k) : 0.0f, rho[1]);
-    if (v2) update_face(G, E, 2, i, j, k, pc, k > 0 ? p_at(i, j, k - 1) : 0.0f, rho[2]);
+    const float px = P[idx[1]], py = P[idx[2]], pz = P[idx[3]];
+    if (v0) update_face(G, E, 0, i, j, k, pc, i > 0 ? px : 0.0f, rho[0]);
+    if (v1) update_face(G, E, 1, i, j, k, pc, j > 0 ? py : 0.0f, rho[1]);
+    if (v2) update_face(G, E, 2, i, j, k, pc, k > 0 ? pz : 0.0f, rho[2]);
 }
 
 }  // namespace shkz

@@ -1268,7 +1268,7 @@ static int autotune(shkz_ctx *C, Launcher &K, Problem &P, const shkz_params *prm
     auto trial = [&](const Knobs &k, double *ms) -> int {
         apply_knobs(P, k, prm);
         double best = 1e300;
-        for (int rep = 0; rep < 2; ++rep) {
+        for (int rep = 0; rep < 3; ++rep) {
             SHKZ_HIP(hipMemcpyAsync(P.x64, x0, n0 * sizeof(double), hipMemcpyDeviceToDevice, K.s));
             SHKZ_HIP(hipStreamSynchronize(K.s));
             SolveOut o;
@@ -1294,7 +1294,7 @@ static int autotune(shkz_ctx *C, Launcher &K, Problem &P, const shkz_params *prm
         if (C->trace)
             std::fprintf(stderr, "[shkz] tune pre/post %d/%d coarse %d/%d cs %d multi %d omega %.2f eta %.0e: %.3f ms\n",
                          k.pre, k.post, k.pre_c, k.post_c, k.coarse, k.multi, k.omega, k.eta, ms);
-        if (ms < 0.98 * cur_ms) {
+        if (ms < 0.99 * cur_ms) {
             cur = k;
             cur_ms = ms;
         }
