@@ -297,18 +297,26 @@ __global__ __launch_bounds__(BLOCK) void k_operator(GridDesc G, LevelDesc L, Gri
         const int sgn[6] = {1, -1, 1, -1, 1, -1};
         double sc[8];  // solid at node (i + a, j + b, k + c), index (a * 2 + b) * 2 + c
         if (G.solid) {
+            size_t nidx[8];
 #pragma unroll
-            for (int n = 0; n < 8; ++n) sc[n] = solid_at(G, i + (n >> 2), j + ((n >> 1) & 1), k + (n & 1));
+            for (int n = 0; n < 8; ++n) nidx[n] = node_index(G, i + (n >> 2), j + ((n >> 1) & 1), k + (n & 1));
+            load_samples<8>(G.solid, G.solid64, nidx, sc);
         }
-        double ph[6];  // level set at the six neighbours (clamped into the grid: out-of-grid ones go unused)
-        float uf[6];   // velocity at the six faces
+        // level set at the six neighbours (clamped into the grid: out-of-grid ones go unused) and
+        // at the cell (last)
+        size_t pidx[7];
+        float uf[6];  // velocity at the six faces
 #pragma unroll
         for (int nq = 0; nq < 6; ++nq) {
-            ph[nq] = phi_at(G, clampi(i + di[nq], 0, G.w - 1), clampi(j + dj[nq], 0, G.h - 1), clampi(k + dk[nq], 0, G.d - 1));
+            pidx[nq] = cell_index(G, clampi(i + di[nq], 0, G.w - 1), clampi(j + dj[nq], 0, G.h - 1),
+                                  clampi(k + dk[nq], 0, G.d - 1));
             const int dim = nq >> 1, o = (nq & 1) ? 0 : 1;
             uf[nq] = G.vel[dim][face_index(G, dim, i + (dim == 0) * o, j + (dim == 1) * o, k + (dim == 2) * o)];
         }
-        const double pc = phi_at(G, i, j, k);
+        pidx[6] = cell_index(G, i, j, k);
+        double ph[7];
+        load_samples<7>(G.phi, G.phi64, pidx, ph);
+        const double pc = ph[6];
         double dsum = 0.0;
 #pragma unroll
         for (int nq = 0; nq < 6; ++nq) {
