@@ -239,17 +239,23 @@ __global__ __launch_bounds__(BLOCK) void k_flags(GridDesc G, int nbx, int nby, i
     int unknown = 0, any_liquid = 0;
     const Exist E{true, true};
     if (i < G.w && j < G.h && k < G.d) {
-        const bool liquid = phi_at(G, i, j, k) < 0.0;
+        const double pc = phi_at(G, i, j, k);
+        const bool liquid = pc < 0.0;
         any_liquid = liquid;
         if (liquid) {
             const int di[6] = {1, -1, 0, 0, 0, 0}, dj[6] = {0, 0, 1, -1, 0, 0}, dk[6] = {0, 0, 0, 0, 1, -1};
             for (int nq = 0; nq < 6 && !unknown; ++nq) {
                 const int qi = i + di[nq], qj = j + dj[nq], qk = k + dk[nq];
                 if (qi < 0 || qi >= G.w || qj < 0 || qj >= G.h || qk < 0 || qk >= G.d) continue;
-                if (!(phi_at(G, qi, qj, qk) < 0.0)) continue;
+                const double pq = phi_at(G, qi, qj, qk);
+                if (!(pq < 0.0)) continue;
                 const int dim = nq >> 1;
                 const int fi = (nq & 1) ? i : qi, fj = (nq & 1) ? j : qj, fk = (nq & 1) ? k : qk;
-                if (face_area(G, E, dim, fi, fj, fk) != 0.0f && face_rho(G, E, dim, fi, fj, fk) != 0.0f) unknown = 1;
+                // face_rho of this interior face from the two samples in hand: the one at the
+                // face's index cell first (the neighbour for a high face, the cell for a low one)
+                if (face_area(G, E, dim, fi, fj, fk) != 0.0f &&
+                    ((nq & 1) ? rho_from_samples(G, E, pc, pq) : rho_from_samples(G, E, pq, pc)) != 0.0f)
+                    unknown = 1;
             }
         }
         cflag[cell_index(G, i, j, k)] = (uint8_t)((liquid ? 1 : 0) | (unknown ? 2 : 0));
