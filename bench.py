@@ -145,17 +145,6 @@ def mg_controls(r):
             "omega": r.mg_omega, "inner_residual": r.inner_residual, "tuned": r.tuned}
 
 
-def c4_scenes(S):
-    """64 cubes, sides drawn uniformly from {32,48,64,96,128,160} (numpy default_rng(3)), each a
-    seeded sphere-drop (scene seed 3000+i) with its own N(0,1) velocities."""
-    from concurrent.futures import ThreadPoolExecutor
-    sides = np.random.default_rng(3).choice(C4_SIDES, size=C4_GRIDS)
-    # the generator runs in the library (ctypes releases the GIL): synthesise in parallel
-    with ThreadPoolExecutor(max_workers=min(16, os.cpu_count() or 1)) as ex:
-        return list(ex.map(lambda a: S.make_scene(S.SCENE_RANDOM_DROP, int(a[1]), seed=3000 + a[0]),
-                           enumerate(sides)))
-
-
 def bench_batch(args):
     """C4: 64 uneven grids projected in one batched call per step (mixed precision, 1e-6), against
     the plain single-solve FP64 GPU path over the same grids one by one."""
@@ -164,7 +153,8 @@ def bench_batch(args):
 
     residual = 1e-6
     log(f"[bench] synthesising c4: {C4_GRIDS} grids")
-    scenes = c4_scenes(S)
+    from shiokaze_amd import loadgen
+    scenes = loadgen.c4_scenes(workers=min(16, os.cpu_count() or 1))
     cells = sum(sc.w * sc.h * sc.d for sc in scenes)
     log(f"[bench] sides {sorted(sc.w for sc in scenes)}; {cells / 1e6:.1f} M cells")
     dev = []

@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define SHKZ_ABI_VERSION 3
+#define SHKZ_ABI_VERSION 4
 
 /* ---------------------------------------------------------------- enums */
 enum shkz_status {
@@ -89,6 +89,10 @@ typedef struct shkz_grid {
     uint8_t *active_out_w;    /* faces and closed boundary faces with liquid inside stay active
                                  (macpressuresolver3.cpp:252-266).  The input of the velocity
                                  extrapolation (shkz_extrapolate_velocity). */
+    uint8_t *unknown_out;     /* optional out (cells): 1 on the cells that are unknowns of the
+                                 system (the reference's index_map actives, macpressuresolver3.cpp:
+                                 121-156), 0 elsewhere: the cells whose m_pressure the reference
+                                 sets (:245-248), zeros included */
 } shkz_grid;
 
 /* ---------------------------------------------------------------- parameters */
@@ -299,6 +303,8 @@ typedef struct shkz_grid2d {
     float *pressure;              /* optional outputs (cells) */
     double *pressure64;
     uint64_t scene_id;
+    uint8_t *unknown_out;         /* optional out (cells): 1 on the unknowns (macpressuresolver2.cpp
+                                     sets m_pressure on each of them, zeros included) */
 } shkz_grid2d;
 
 int shkz_project2d(shkz_ctx *ctx, const shkz_grid2d *grid, const shkz_params *params, shkz_result *result);

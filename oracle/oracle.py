@@ -93,6 +93,10 @@ def lib():
         L.oracle_pcg.argtypes = [ctypes.c_int64, _vp, _vp, _vp, _vp, _vp, ctypes.c_double, ctypes.c_int, ctypes.c_int,
                                  ctypes.c_double, ctypes.c_double, ctypes.POINTER(ctypes.c_int), _dp]
         L.oracle_pcg.restype = ctypes.c_int
+        L.oracle_pcg_snap.argtypes = [ctypes.c_int64, _vp, _vp, _vp, _vp, _vp, ctypes.c_double, ctypes.c_int,
+                                      ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.POINTER(ctypes.c_int), _dp,
+                                      ctypes.c_double, _vp, ctypes.POINTER(ctypes.c_int)]
+        L.oracle_pcg_snap.restype = ctypes.c_int
         L.oracle_velocity_update.argtypes = [ctypes.POINTER(OGrid), _vp, _vp * 3, _vp * 3, _vp]
         L.oracle_divergence.argtypes = [ctypes.POINTER(OGrid), _vp, _vp * 3, _vp]
         L.oracle_divergence.restype = ctypes.c_double
@@ -223,6 +227,22 @@ def pcg(sys, b=None, residual=1e-8, max_iterations=30000, mic_mode=MIC_NONE):
                           x.ctypes.data, residual, max_iterations, mic_mode, 0.97, 0.25, ctypes.byref(it),
                           ctypes.byref(rr))
     return x[:n], it.value, rr.value, bool(ok)
+
+
+def pcg_snap(sys, residual=1e-10, snap_residual=1e-6, max_iterations=30000, mic_mode=MIC_APPLIED, b=None):
+    """pcg to `residual` that also returns the iterate the same solver reaches at the first
+    iteration meeting `snap_residual`: (x, iterations, reresid, ok, x_snap, snap_iterations)."""
+    n = sys["n"]
+    b = sys["b"] if b is None else np.ascontiguousarray(b, np.float64)
+    x = np.zeros(max(n, 1), np.float64)
+    xs = np.zeros(max(n, 1), np.float64)
+    it = ctypes.c_int(0)
+    rr = ctypes.c_double(0)
+    sit = ctypes.c_int(-1)
+    ok = lib().oracle_pcg_snap(n, sys["rowptr"].ctypes.data, sys["col"].ctypes.data, sys["val"].ctypes.data,
+                               b.ctypes.data, x.ctypes.data, residual, max_iterations, mic_mode, 0.97, 0.25,
+                               ctypes.byref(it), ctypes.byref(rr), snap_residual, xs.ctypes.data, ctypes.byref(sit))
+    return x[:n], it.value, rr.value, bool(ok), xs[:n], sit.value
 
 
 def project(scene, params=None, pressure64=True):

@@ -613,10 +613,35 @@ static void mic_apply(const mic_factor *F, const double *x, double *out)
  * preconditioner output is overwritten by its input (pcg_solver.h:383): ORACLE_MIC_FAITHFUL
  * pays for the factor and both triangular solves and then uses z = r, exactly as the
  * reference does; ORACLE_MIC_APPLIED uses the factor; ORACLE_MIC_NONE skips it. */
+static int pcg_run(int64_t n, const int64_t *rowptr, const int64_t *col, const double *val, const double *b, double *x,
+                   double residual, int max_iterations, int mic_mode, double omega, double min_diag_ratio,
+                   int *iterations, double *reresid, double snap_residual, double *x_snap, int *snap_iterations);
+
 int oracle_pcg(int64_t n, const int64_t *rowptr, const int64_t *col, const double *val, const double *b, double *x,
                double residual, int max_iterations, int mic_mode, double omega, double min_diag_ratio,
                int *iterations, double *reresid)
 {
+    return pcg_run(n, rowptr, col, val, b, x, residual, max_iterations, mic_mode, omega, min_diag_ratio, iterations,
+                   reresid, 0.0, NULL, NULL);
+}
+
+/* oracle_pcg that also records the iterate at the first iteration whose ||r||_inf meets
+ * snap_residual * ||b||_inf (the answer the same solver returns at that looser tolerance) in
+ * x_snap and that iteration in *snap_iterations (-1 if never met). */
+int oracle_pcg_snap(int64_t n, const int64_t *rowptr, const int64_t *col, const double *val, const double *b,
+                    double *x, double residual, int max_iterations, int mic_mode, double omega,
+                    double min_diag_ratio, int *iterations, double *reresid, double snap_residual, double *x_snap,
+                    int *snap_iterations)
+{
+    return pcg_run(n, rowptr, col, val, b, x, residual, max_iterations, mic_mode, omega, min_diag_ratio, iterations,
+                   reresid, snap_residual, x_snap, snap_iterations);
+}
+
+static int pcg_run(int64_t n, const int64_t *rowptr, const int64_t *col, const double *val, const double *b, double *x,
+                   double residual, int max_iterations, int mic_mode, double omega, double min_diag_ratio,
+                   int *iterations, double *reresid, double snap_residual, double *x_snap, int *snap_iterations)
+{
+    if (snap_iterations) *snap_iterations = -1;
     if (residual < 1e-30) residual = 1e-30;
     memset(x, 0, (size_t)n * sizeof(double));
     if (n == 0) {
@@ -666,6 +691,10 @@ int oracle_pcg(int64_t n, const int64_t *rowptr, const int64_t *col, const doubl
         add_scaled(n, alpha, s, x);
         add_scaled(n, -alpha, z, r);
         res = abs_max(n, r);
+        if (x_snap && *snap_iterations < 0 && res <= snap_residual * res0) {
+            memcpy(x_snap, x, (size_t)n * sizeof(double));
+            *snap_iterations = it + 1;
+        }
         if (res <= tol) {
             *iterations = it + 1;
             *reresid = res / res0;

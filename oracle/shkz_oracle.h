@@ -103,6 +103,12 @@ void oracle_spmv(int64_t n, const int64_t *rowptr, const int64_t *col, const dou
 int oracle_pcg(int64_t n, const int64_t *rowptr, const int64_t *col, const double *val,
                const double *b, double *x, double residual, int max_iterations, int mic_mode,
                double omega, double min_diag_ratio, int *iterations, double *reresid);
+/* oracle_pcg that also keeps the iterate at the first iteration meeting snap_residual (relative to
+ * ||b||_inf) in x_snap, and that iteration count in *snap_iterations (-1 if never met). */
+int oracle_pcg_snap(int64_t n, const int64_t *rowptr, const int64_t *col, const double *val, const double *b,
+                    double *x, double residual, int max_iterations, int mic_mode, double omega,
+                    double min_diag_ratio, int *iterations, double *reresid, double snap_residual, double *x_snap,
+                    int *snap_iterations);
 
 /* macpressuresolver3.cpp:252-268 velocity update with pressure stored in float. */
 void oracle_velocity_update(const oracle_grid *g, const float *pressure, float *area[3], float *rho[3],

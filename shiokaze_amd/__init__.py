@@ -19,7 +19,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "lib", "libshkz_mi355x.so")
 
-ABI_VERSION = 3  # the layout of the structs below
+ABI_VERSION = 4  # the layout of the structs below
 F32, F64 = 0, 1
 SOLVER_MGPCG, SOLVER_CG = 0, 1
 PREC_MIXED, PREC_FP64 = 0, 1
@@ -43,6 +43,7 @@ class Grid(ctypes.Structure):
         ("pressure", _vp), ("pressure64", _vp),
         ("scene_id", ctypes.c_uint64),
         ("active_out_u", _vp), ("active_out_v", _vp), ("active_out_w", _vp),
+        ("unknown_out", _vp),
     ]
 
 
@@ -103,7 +104,7 @@ class Grid2D(ctypes.Structure):
         ("w", ctypes.c_int32), ("h", ctypes.c_int32), ("dx", ctypes.c_double), ("dt", ctypes.c_double),
         ("fluid", _vp), ("fluid_dtype", ctypes.c_int32), ("solid_dtype", ctypes.c_int32), ("solid", _vp),
         ("u", _vp), ("v", _vp), ("active_u", _vp), ("active_v", _vp), ("pressure", _vp), ("pressure64", _vp),
-        ("scene_id", ctypes.c_uint64),
+        ("scene_id", ctypes.c_uint64), ("unknown_out", _vp),
     ]
 
 
@@ -529,14 +530,19 @@ class Context:
 
     # ------------------------------------------------------------------ projection
     def project(self, scene: Scene, params: Params | None = None, pressure: bool = True, pressure64: bool = False,
-                scene_id: int = 0, active=None, active_out=None, check: bool = True):
+                scene_id: int = 0, active=None, active_out=None, unknown_out=None, check: bool = True):
         """Projects scene's velocities in place (host arrays).  Returns (result, P32, P64).
-        active_out: optional three uint8 face arrays that receive the post-projection activity."""
+        active_out: optional three uint8 face arrays that receive the post-projection activity;
+        unknown_out: optional uint8 cell array that receives the unknown mask."""
         g = self._grid(scene, scene_id, active)
         if active_out is not None:
             for a in active_out:
                 assert a.dtype == np.uint8 and a.flags["C_CONTIGUOUS"]
             g.active_out_u, g.active_out_v, g.active_out_w = (_ptr(a) for a in active_out)
+        if unknown_out is not None:
+            assert unknown_out.dtype == np.uint8 and unknown_out.flags["C_CONTIGUOUS"]
+            assert unknown_out.size == scene.w * scene.h * scene.d
+            g.unknown_out = _ptr(unknown_out)
         P = np.zeros((scene.w, scene.h, scene.d), np.float32) if pressure else None
         P64 = np.zeros((scene.w, scene.h, scene.d), np.float64) if pressure64 else None
         g.pressure = _ptr(P)

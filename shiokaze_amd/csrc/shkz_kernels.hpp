@@ -1894,8 +1894,9 @@ __global__ __launch_bounds__(BRICK_CELLS / CPT) void k_coarse_solve(LevelDesc L,
 // ------------------------------------------------------------------------------ output (one grid)
 // m_pressure: dense float (and optional FP64) pressure, 0 off the unknowns
 // (macpressuresolver3.cpp:245-248).
+// With unknown_out, also the unknown mask (the labelling's cell flag, bit 1).
 __global__ void k_scatter_pressure(GridDesc G, LevelDesc L, GridLevel GL, const double *__restrict__ x, float *P,
-                                   double *P64) {
+                                   double *P64, const uint8_t *__restrict__ cflag, uint8_t *unknown_out) {
     const size_t M = (size_t)G.w * G.h * G.d;
     const size_t c = (size_t)xcd_brick(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (c >= M) return;
@@ -1904,6 +1905,7 @@ __global__ void k_scatter_pressure(GridDesc G, LevelDesc L, GridLevel GL, const 
     const double v = level_value(L, GL, x, i, j, k);
     if (P) P[c] = (float)v;
     if (P64) P64[c] = v;
+    if (unknown_out) unknown_out[c] = (cflag[c] >> 1) & 1;
 }
 
 // Dense export of the level-0 system: diag (0 off the unknowns), the low-face couplings and the

@@ -864,8 +864,10 @@ static std::string graph_key(shkz_ctx *C, const Problem &P, const void *const *v
     const int nl = precond ? P.nlev : 1;
     for (int l = 0; l < nl; ++l) {
         const LevelHost &L = C->levels[l];
-        std::snprintf(buf, sizeof(buf), "|%d %d %p %p %p %p %p %p %p %p %p %p %p %p %p %p %p %p %p", L.nbricks,
-                      L.any_coarsest ? 1 : 0, (const void *)L.desc.bmap, L.dir, L.gx, L.gy, L.gz, L.idiag, L.rhs, L.z,
+        // the transfer kernels branch on flat (bilinear in 2D) and the reductions on ngrids
+        std::snprintf(buf, sizeof(buf), "|%d %d f%d g%d %p %p %p %p %p %p %p %p %p %p %p %p %p %p %p %p %p",
+                      L.nbricks, L.any_coarsest ? 1 : 0, L.desc.flat, L.desc.ngrids, (const void *)L.desc.bmap, L.dir,
+                      L.gx, L.gy, L.gz, L.idiag, L.rhs, L.z,
                       L.tmp, L.res, (const void *)L.desc.bxyz, (const void *)L.desc.meta, (const void *)L.desc.gl,
                       (const void *)L.desc.child64, (const void *)L.desc.parent8, (const void *)L.desc.uni,
                       (const void *)L.desc.nb27);
@@ -1898,12 +1900,14 @@ static int project_stages(shkz_ctx *C, const shkz_grid *grids, int ngrids, const
         const GridLevel &GL = F0.gl[g];
         // the velocity update reads a dense float pressure: the caller's, or the scratch array
         float *P32 = grid->pressure ? grid->pressure : pdense;
-        K.go("scatter_pressure", -1, 0, (double)J.M * (4 + (grid->pressure64 ? 8 : 0)), dim3(xcd_blocks(J.M, 256)),
-             dim3(256), k_scatter_pressure, J.G, F0.desc, GL,
-             (const double *)P.x64, P32, grid->pressure64);
+        uint8_t *unk_out = grid->unknown_out;
+        K.go("scatter_pressure", -1, 0, (double)J.M * (4 + (grid->pressure64 ? 8 : 0) + (unk_out ? 2 : 0)),
+             dim3(xcd_blocks(J.M, 256)), dim3(256), k_scatter_pressure, J.G, F0.desc, GL, (const double *)P.x64, P32,
+             grid->pressure64, (const uint8_t *)J.cflag, unk_out);
         if (J.scene)
             K.go("scatter_pressure", -1, 0, (double)J.M * 8, dim3(xcd_blocks(J.M, 256)), dim3(256),
-                 k_scatter_pressure, J.G, F0.desc, GL, (const double *)P.x64, (float *)nullptr, J.scene->p);
+                 k_scatter_pressure, J.G, F0.desc, GL, (const double *)P.x64, (float *)nullptr, J.scene->p,
+                 (const uint8_t *)nullptr, (uint8_t *)nullptr);
         const size_t ne = (size_t)(grid->w + 1) * (grid->h + 1) * (grid->d + 1);
         K.go("velocity_update", -1, 0, (double)J.M * (24 + 4 + (J.G.phi64 ? 8 : 4) + (grid->solid ? 4 : 0)),
              dim3(xcd_blocks(ne, 256)), dim3(256), k_velocity_update, J.G, (const float *)P32);
@@ -1945,6 +1949,7 @@ static int project_host(shkz_ctx *C, const shkz_grid *grids, int ngrids, const s
         size_t nf[3];
         float *dp;
         double *dp64;
+        uint8_t *dunk;
         size_t M;
     };
     std::vector<Back> back(ngrids);
@@ -2002,6 +2007,13 @@ static int project_host(shkz_ctx *C, const shkz_grid *grids, int ngrids, const s
         dg[g].active_out_u = B.dao[0];
         dg[g].active_out_v = B.dao[1];
         dg[g].active_out_w = B.dao[2];
+        B.dunk = nullptr;
+        if (grid->unknown_out) {
+            void *pp = nullptr;
+            SHKZ_TRY(AR.get("h_unk" + sfx, M, &pp));
+            B.dunk = (uint8_t *)pp;
+        }
+        dg[g].unknown_out = B.dunk;
         B.dp = nullptr;
         B.dp64 = nullptr;
         B.M = M;
@@ -2045,6 +2057,7 @@ static int project_host(shkz_ctx *C, const shkz_grid *grids, int ngrids, const s
             if (back[g].dao[dim]) SHKZ_HIP(hipMemcpy(hao[dim], back[g].dao[dim], back[g].nf[dim], hipMemcpyDeviceToHost));
         }
         if (grid->pressure) SHKZ_HIP(hipMemcpy(grid->pressure, back[g].dp, back[g].M * 4, hipMemcpyDeviceToHost));
+        if (back[g].dunk) SHKZ_HIP(hipMemcpy(grid->unknown_out, back[g].dunk, back[g].M, hipMemcpyDeviceToHost));
         if (grid->pressure64)
             SHKZ_HIP(hipMemcpy(grid->pressure64, back[g].dp64, back[g].M * 8, hipMemcpyDeviceToHost));
     }
@@ -2717,6 +2730,7 @@ static shkz_grid grid_of_2d(const shkz_grid2d *g) {
     q.pressure = g->pressure;
     q.pressure64 = g->pressure64;
     q.scene_id = g->scene_id;
+    q.unknown_out = g->unknown_out;
     return q;
 }
 

@@ -13,6 +13,7 @@
 #include <shiokaze/core/timer.h>
 
 #include "../../include/shkz_mi355x.h"
+#include "shkz_stage.hpp"
 
 #include <cstdint>
 #include <string>
@@ -35,28 +36,18 @@ protected:
         const size_t M = (size_t)w * h;
         // ---- densify (operator() yields the background / fill values where the sparse grid is
         // inactive, array2.h), as the reference's routines read them
-        m_phi.resize(M);
-        for (int i = 0; i < w; ++i)
-            for (int j = 0; j < h; ++j) m_phi[(size_t)i * h + j] = fluid(i, j);
+        shkz_host::densify(m_phi, w, h, 1, [&](int i, int j, int) { return fluid(i, j); });
         const bool has_solid = array_utility2::levelset_exist(solid) && solid.shape() == m_shape.nodal();
-        if (has_solid) {
-            m_solid.resize((size_t)(w + 1) * (h + 1));
-            for (int i = 0; i <= w; ++i)
-                for (int j = 0; j <= h; ++j) m_solid[(size_t)i * (h + 1) + j] = solid(i, j);
-        }
+        if (has_solid) shkz_host::densify(m_solid, w + 1, h + 1, 1, [&](int i, int j, int) { return solid(i, j); });
         for (int dim = 0; dim < 2; ++dim) {
             const shape2 fs = m_shape.face(dim);
-            m_vel[dim].resize(fs.count());
-            m_vel0[dim].resize(fs.count());
-            m_act[dim].resize(fs.count());
-            for (int i = 0; i < (int)fs.w; ++i)
-                for (int j = 0; j < (int)fs.h; ++j) {
-                    const size_t f = (size_t)i * fs.h + j;
-                    m_vel[dim][f] = m_vel0[dim][f] = velocity[dim](i, j);
-                    m_act[dim][f] = velocity[dim].active(i, j) ? 1 : 0;
-                }
+            const array2<Real> &u = velocity[dim];
+            shkz_host::densify(m_vel0[dim], fs.w, fs.h, 1, [&](int i, int j, int) { return u(i, j); });
+            shkz_host::densify(m_act[dim], fs.w, fs.h, 1, [&](int i, int j, int) { return u.active(i, j) ? 1 : 0; });
+            m_vel[dim] = m_vel0[dim];
         }
-        m_p.assign(M, 0.0f);
+        m_p.resize(M);
+        m_unknown.resize(M);
         shkz_grid2d g = {};
         g.w = w;
         g.h = h;
@@ -71,7 +62,8 @@ protected:
         g.active_u = m_act[0].data();
         g.active_v = m_act[1].data();
         g.pressure = m_p.data();
-        g.scene_id = reinterpret_cast<uintptr_t>(this);
+        g.unknown_out = m_unknown.data();
+        g.scene_id = scene_id();
         shkz_params p = m_params;
         p.surface_tension = surface_tension;
         p.current_volume = m_current_volume;
@@ -82,12 +74,13 @@ protected:
             console::dump("shkz_project2d failed (%d): %s\n", rc, shkz_last_error());
             throw std::runtime_error(shkz_last_error());
         }
-        // ---- write back: m_pressure on the unknowns, velocities with the 2D module's activity rule
+        // ---- write back: m_pressure on every unknown, zeros included (macpressuresolver2.cpp sets
+        // each of them), velocities with the 2D module's activity rule
         m_pressure.clear();
         for (int i = 0; i < w; ++i)
             for (int j = 0; j < h; ++j) {
-                const float v = m_p[(size_t)i * h + j];
-                if (v) m_pressure.set(i, j, v);
+                const size_t c = (size_t)i * h + j;
+                if (m_unknown[c]) m_pressure.set(i, j, m_p[c]);
             }
         const bool has_fluid = array_utility2::levelset_exist(fluid);
         for (int dim = 0; dim < 2; ++dim) {
@@ -169,7 +162,11 @@ protected:
         m_pressure.initialize(m_shape);
         m_target_volume = m_current_volume = 0.0;
         if (!m_ctx && shkz_create(m_device, 0, &m_ctx) != SHKZ_OK) throw std::runtime_error(shkz_last_error());
+        // m_y_prev = 0: dropping the scene's resident state resets the library's volume integrator
+        // (SHKZ_ERR_NOT_FOUND before the first projection is expected)
+        (void)shkz_release_scene(m_ctx, scene_id());
     }
+    uint64_t scene_id() const { return (uint64_t) reinterpret_cast<uintptr_t>(this); }
     virtual const array2<Real> *get_pressure() const override { return &m_pressure; }
 
 public:
@@ -186,7 +183,7 @@ protected:
     int m_device{0};
     double m_target_volume{0.0}, m_current_volume{0.0};
     std::vector<float> m_phi, m_solid, m_vel[2], m_vel0[2], m_p;
-    std::vector<uint8_t> m_act[2];
+    std::vector<uint8_t> m_act[2], m_unknown;
 };
 
 extern "C" module *create_instance() { return new macpressuresolver2_mi355x(); }

@@ -9,11 +9,11 @@
 #include <shiokaze/array/array3.h>
 #include <shiokaze/array/macarray3.h>
 #include <shiokaze/array/array_utility3.h>
-#include <shiokaze/utility/utility.h>
 #include <shiokaze/core/console.h>
 #include <shiokaze/core/timer.h>
 
 #include "../../include/shkz_mi355x.h"
+#include "shkz_stage.hpp"
 
 #include <cstdint>
 #include <string>
@@ -37,32 +37,23 @@ protected:
         console::dump(">>> Pressure Projection (MI355X) started...\n");
         const int w = m_shape.w, h = m_shape.h, d = m_shape.d;
         const size_t M = (size_t)w * h * d;
-        // ---- densify: operator() yields background / fill values where the sparse grid is
-        // inactive (array3.h:796-800), which is what the reference's routines read too.
-        m_phi.resize(M);
-        for (int i = 0; i < w; ++i)
-            for (int j = 0; j < h; ++j)
-                for (int k = 0; k < d; ++k) m_phi[((size_t)i * h + j) * d + k] = fluid(i, j, k);
+        // ---- densify on every host thread: operator() yields the background / fill value where
+        // the sparse grid is inactive (array3.h:796-800), which is what the reference's routines
+        // read too.  Every face is staged (inactive ones read their background value, as the
+        // reference's assembly reads them, macpressuresolver3.cpp:187-194); the write-back below
+        // visits only the faces the caller holds active.
+        shkz_host::densify(m_phi, w, h, d, [&](int i, int j, int k) { return fluid(i, j, k); });
         const bool has_solid = array_utility3::levelset_exist(solid) && solid.shape() == m_shape.nodal();
-        if (has_solid) {
-            m_solid.resize((size_t)(w + 1) * (h + 1) * (d + 1));
-            for (int i = 0; i <= w; ++i)
-                for (int j = 0; j <= h; ++j)
-                    for (int k = 0; k <= d; ++k) m_solid[((size_t)i * (h + 1) + j) * (d + 1) + k] = solid(i, j, k);
-        }
+        if (has_solid)
+            shkz_host::densify(m_solid, w + 1, h + 1, d + 1, [&](int i, int j, int k) { return solid(i, j, k); });
         for (int dim = 0; dim < 3; ++dim) {
             const shape3 fs = m_shape.face(dim);
-            m_vel[dim].resize(fs.count());
-            m_act[dim].resize(fs.count());
-            for (int i = 0; i < (int)fs.w; ++i)
-                for (int j = 0; j < (int)fs.h; ++j)
-                    for (int k = 0; k < (int)fs.d; ++k) {
-                        const size_t f = ((size_t)i * fs.h + j) * fs.d + k;
-                        m_vel[dim][f] = velocity[dim](i, j, k);
-                        m_act[dim][f] = velocity[dim].active(i, j, k) ? 1 : 0;
-                    }
+            const array3<Real> &u = velocity[dim];
+            shkz_host::densify(m_vel[dim], fs.w, fs.h, fs.d, [&](int i, int j, int k) { return u(i, j, k); });
+            m_act_out[dim].resize(fs.count());
         }
-        m_p.assign(M, 0.0f);
+        m_p.resize(M);
+        m_unknown.resize(M);
         // ---- project on the device
         shkz_grid g = {};
         g.w = w;
@@ -77,11 +68,12 @@ protected:
         g.u = m_vel[0].data();
         g.v = m_vel[1].data();
         g.w_vel = m_vel[2].data();
-        g.active_u = m_act[0].data();
-        g.active_v = m_act[1].data();
-        g.active_w = m_act[2].data();
         g.pressure = m_p.data();
-        g.scene_id = reinterpret_cast<uintptr_t>(this);
+        g.scene_id = scene_id();
+        g.active_out_u = m_act_out[0].data();
+        g.active_out_v = m_act_out[1].data();
+        g.active_out_w = m_act_out[2].data();
+        g.unknown_out = m_unknown.data();
         shkz_params p = m_params;
         p.surface_tension = surface_tension;
         p.current_volume = m_current_volume;
@@ -94,21 +86,23 @@ protected:
         }
         console::write(get_argument_name() + "_number_projection_iteration", r.iterations);
         console::dump("Done. Took %d iterations, Reresid=%e.\n", r.iterations, r.reresid);
-        // ---- write back: m_pressure on the unknowns, velocities on the faces the reference visits
+        // ---- write back: m_pressure on every unknown, zeros included (:245-248), serially as the
+        // reference's const_serial_actives; velocities on the faces the caller holds active (:252-266)
         m_pressure.clear();
         for (int i = 0; i < w; ++i)
             for (int j = 0; j < h; ++j)
                 for (int k = 0; k < d; ++k) {
-                    const float v = m_p[((size_t)i * h + j) * d + k];
-                    if (v) m_pressure.set(i, j, k, v);
+                    const size_t c = ((size_t)i * h + j) * d + k;
+                    if (m_unknown[c]) m_pressure.set(i, j, k, m_p[c]);
                 }
-        const bool has_fluid = array_utility3::levelset_exist(fluid);
         for (int dim = 0; dim < 3; ++dim) {
             const shape3 fs = m_shape.face(dim);
+            const std::vector<float> &vel = m_vel[dim];
+            const std::vector<uint8_t> &act = m_act_out[dim];
             velocity[dim].parallel_actives([&](int i, int j, int k, auto &it, int tn) {
                 const size_t f = ((size_t)i * fs.h + j) * fs.d + k;
-                if (face_open(dim, i, j, k, has_solid, has_fluid, solid) || boundary_liquid(dim, i, j, k))
-                    it.set(m_vel[dim][f]);
+                if (shkz_host::face_action(true, act[f]) == shkz_host::FACE_SET)
+                    it.set(vel[f]);
                 else
                     it.set_off();  // macpressuresolver3.cpp:266
             });
@@ -116,43 +110,9 @@ protected:
         console::dump("<<< Projection done. Took %s.\n", timer.stock("projection").c_str());
     }
 
-    // area > 0 and rho > 0 (macpressuresolver3.cpp:255), evaluated as the reference's
-    // macutility3 routines would on these level sets.
-    bool face_open(int dim, int i, int j, int k, bool has_solid, bool has_fluid, const array3<Real> &solid) const {
-        const int pi[3] = {i, j, k};
-        const unsigned N[3] = {m_shape.w, m_shape.h, m_shape.d};
-        const bool boundary = pi[dim] == 0 || pi[dim] == (int)N[dim];
-        bool area = !boundary;
-        if (has_solid && !boundary) {
-            double q[2][2];
-            if (dim == 0) {
-                q[0][0] = solid(i, j, k); q[1][0] = solid(i, j + 1, k); q[1][1] = solid(i, j + 1, k + 1); q[0][1] = solid(i, j, k + 1);
-            } else if (dim == 1) {
-                q[0][0] = solid(i, j, k); q[1][0] = solid(i + 1, j, k); q[1][1] = solid(i + 1, j, k + 1); q[0][1] = solid(i, j, k + 1);
-            } else {
-                q[0][0] = solid(i, j, k); q[1][0] = solid(i + 1, j, k); q[1][1] = solid(i + 1, j + 1, k); q[0][1] = solid(i, j + 1, k);
-            }
-            area = (1.0 - utility::get_area(q)) != 0.0;
-        } else if (has_solid && boundary) {
-            area = false;
-        }
-        if (!area) return false;
-        if (!has_fluid) return true;
-        const vec3i a = m_shape.clamp(i, j, k), b = m_shape.clamp(i - (dim == 0), j - (dim == 1), k - (dim == 2));
-        return m_phi[((size_t)a[0] * m_shape.h + a[1]) * m_shape.d + a[2]] < 0.0 ||
-               m_phi[((size_t)b[0] * m_shape.h + b[1]) * m_shape.d + b[2]] < 0.0;
-    }
-    // a boundary face with liquid on its inner side stays active at 0 (macpressuresolver3.cpp:264-265)
-    bool boundary_liquid(int dim, int i, int j, int k) const {
-        const int pi[3] = {i, j, k};
-        const unsigned N[3] = {m_shape.w, m_shape.h, m_shape.d};
-        if (pi[dim] == 0) return m_phi[((size_t)i * m_shape.h + j) * m_shape.d + k] < 0.0;
-        if (pi[dim] == (int)N[dim]) {
-            const int a = i - (dim == 0), b = j - (dim == 1), c = k - (dim == 2);
-            return m_phi[((size_t)a * m_shape.h + b) * m_shape.d + c] < 0.0;
-        }
-        return false;
-    }
+    // The library keys this module's resident state (warm-start pressure, volume integrator) by
+    // the module's address.
+    uint64_t scene_id() const { return (uint64_t) reinterpret_cast<uintptr_t>(this); }
 
     virtual void configure(configuration &config) override {
         shkz_default_params(&m_params);
@@ -191,6 +151,10 @@ protected:
         m_pressure.initialize(m_shape);
         m_target_volume = m_current_volume = 0.0;
         if (!m_ctx && shkz_create(m_device, 0, &m_ctx) != SHKZ_OK) throw std::runtime_error(shkz_last_error());
+        // m_y_prev = 0 (macpressuresolver3.cpp:290): dropping the scene's resident state resets the
+        // library's volume integrator (its warm-start pressure goes with it; SHKZ_ERR_NOT_FOUND
+        // before the first projection is expected)
+        (void)shkz_release_scene(m_ctx, scene_id());
     }
     virtual const array3<Real> *get_pressure() const override { return &m_pressure; }
 
@@ -208,7 +172,7 @@ protected:
     int m_device{0};
     double m_target_volume{0.0}, m_current_volume{0.0};
     std::vector<float> m_phi, m_solid, m_vel[3], m_p;
-    std::vector<uint8_t> m_act[3];
+    std::vector<uint8_t> m_act_out[3], m_unknown;
 };
 
 extern "C" module *create_instance() { return new macpressuresolver3_mi355x(); }

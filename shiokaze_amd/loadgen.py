@@ -19,9 +19,28 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from . import (F32, PENDING, SCENE_RANDOM_DROP, Context, ShkzError, default_params, last_error,
+from . import (F32, PENDING, SCENE_RANDOM_DROP, Context, ShkzError, default_params, last_error, make_scene,
                scene_fill_device)
 from .devmem import Stream, check, hip
+
+# Batched workload: 64 cubes with sides drawn uniformly from these (numpy default_rng(3)).
+C4_SIDES = (32, 48, 64, 96, 128, 160)
+C4_GRIDS = 64
+
+
+def c4_sides(n_grids: int = C4_GRIDS, seed: int = 3) -> list[int]:
+    return [int(a) for a in np.random.default_rng(seed).choice(C4_SIDES, size=n_grids)]
+
+
+def c4_scenes(n_grids: int = C4_GRIDS, seed: int = 3, workers: int = 16) -> list:
+    """The batched workload's host scenes: cube i has side c4_sides()[i] and is a seeded
+    sphere-drop (scene seed 1000*seed + i) with its own N(0,1) velocities."""
+    from concurrent.futures import ThreadPoolExecutor
+    sides = c4_sides(n_grids, seed)
+    # the generator runs in the library (ctypes releases the GIL): synthesise in parallel
+    with ThreadPoolExecutor(max_workers=max(1, workers)) as ex:
+        return list(ex.map(lambda a: make_scene(SCENE_RANDOM_DROP, a[1], seed=1000 * seed + a[0]),
+                           enumerate(sides)))
 
 
 @dataclass

@@ -9,6 +9,7 @@ Bars:
   * velocities: bit-identical to the oracle's velocity update applied to the GPU's own pressure;
   * divergence: |max div_gpu - max div_oracle| <= 1e-5 (the 1e-5 gate of the projection target).
 """
+import ctypes
 import sys
 
 import numpy as np
@@ -244,6 +245,54 @@ def test_warm_start_reaches_same_answer(ctx):
     A = O.assemble(sc2)
     assert rel_l2(O.gather(A, Q2), O.gather(A, Q3)) < 1e-5
     ctx.release_scene(77)
+
+
+def test_warm_start_matches_oracle_warm_start(ctx):
+    """The reference's iteration (plain FP64 CG) warm-started from the scene's previous pressure
+    takes the oracle's iteration count on the same warm-started system (macpressuresolver3.cpp:
+    221-242: b -= A p_prev, solve, x += p_prev) and reaches its answer; the mixed-precision MGPCG
+    warm start meets the tolerance relative to b - A p_prev on the oracle's assembly."""
+    say("warm start against the oracle's warm start")
+    sc1 = S.make_scene(S.SCENE_DROP_SOLIDS, 40, seed=4)
+    sc2 = S.make_scene(S.SCENE_DROP_SOLIDS, 40, seed=4, shift=(0.01, -0.02, 0.0), velocity_seed=17)
+    A2 = O.assemble(sc2)
+    for solver, prec, tol in ((S.SOLVER_CG, S.PREC_FP64, 1e-8), (S.SOLVER_MGPCG, S.PREC_MIXED, 1e-7)):
+        prm = S.default_params(residual=tol, solver=solver, precision=prec, warm_start=1)
+        r1, _, Q1 = ctx.project(sc1.copy(), prm, pressure64=True, scene_id=78)
+        r2, _, Q2 = ctx.project(sc2.copy(), prm, pressure64=True, scene_id=78)
+        ctx.release_scene(78)
+        assert r1.converged and r2.converged
+        b_eff = A2["b"] - O.spmv(A2, O.gather(A2, Q1))
+        rel = np.abs(A2["b"] - O.spmv(A2, O.gather(A2, Q2))).max() / np.abs(b_eff).max()
+        assert rel <= tol * 1.0001, rel
+        if solver == S.SOLVER_CG:
+            oprm = O.default_params(residual=tol, mic_mode=O.MIC_NONE)
+            warm = np.ascontiguousarray(Q1)
+            oprm.warm_start = warm.ctypes.data
+            r_o, P_o, P64_o = O.project(sc2.copy(), oprm)
+            say(f"  CG warm start: GPU {r2.iterations} iterations, oracle {r_o.iterations}")
+            assert abs(r2.iterations - r_o.iterations) <= max(2, 0.05 * r_o.iterations)
+            assert rel_l2(O.gather(A2, Q2), O.gather(A2, P64_o)) < 1e-6
+
+
+def test_volume_integrator_resets_with_the_scene(ctx):
+    """A scene's integrator accumulates x dt across projections (m_y_prev, macpressuresolver3.cpp:
+    208) and dropping the scene resets it, as post_initialize zeroes m_y_prev (:290)."""
+    say("volume integrator reset")
+    sc = S.make_scene(S.SCENE_DROP, 16, seed=2)
+    prm = S.default_params(residual=1e-6, gain=1.0, current_volume=0.9, target_volume=1.0)
+    c = [ctx.project(sc.copy(), prm, scene_id=91)[0].rhs_correct for _ in range(2)]
+    ctx.release_scene(91)
+    c.append(ctx.project(sc.copy(), prm, scene_id=91)[0].rhs_correct)
+    ctx.release_scene(91)
+    y = 0.0
+    expect = []
+    for _ in range(2):
+        yy = ctypes.c_double(y)
+        expect.append(O.lib().oracle_volume_correction(0.9, 1.0, sc.dt, 1.0, ctypes.byref(yy)))
+        y = yy.value
+    assert c[0] == expect[0] and c[1] == expect[1] and c[1] != c[0]
+    assert c[2] == c[0]
 
 
 def test_determinism(ctx):
