@@ -34,8 +34,6 @@ CONFIGS = {
     "c3": ("spheres_box", 256, 2, 1e-6),
 }
 HBM_PEAK_GBS = 8000.0
-C4_SIDES = (32, 48, 64, 96, 128, 160)
-C4_GRIDS = 64
 
 
 def log(*a):
@@ -62,39 +60,91 @@ def measured_traffic(kernel, config):
     return None
 
 
-def cpu_baseline(scene, n_iter_needed, residual):
-    """Reference-faithful CPU projection (the oracle restating macpressuresolver3 + pcg with MIC(0)
-    computed and discarded), timed on a bounded sample: the full fractions/labelling/assembly and
-    velocity update, plus 20 and 40 CG iterations to separate the per-iteration cost; the solve is
-    extrapolated to the iteration count the same CG iteration needs on this scene."""
+STAGES = ("Projection_solid_fluid_fractions", "Projection_build_highres_linsystem", "Projection_linsolve",
+          "Projection_update_velocity", "Projection_projection")
+
+
+def host_info():
+    """CPU model (as the reference's UI logs it, src/ui/ui.cpp:583-589) and the oracle's compiler."""
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    import subprocess
+    try:
+        cc = subprocess.run(["gcc", "--version"], capture_output=True, text=True, timeout=10).stdout.splitlines()[0]
+    except (OSError, IndexError, subprocess.SubprocessError):
+        cc = "gcc"
+    return model, f"{cc}; -O3 -fopenmp -ffp-contract=off (oracle/Makefile)"
+
+
+def oracle_stages(r):
+    return dict(zip(STAGES, (round(r.t_fractions, 4), round(r.t_build, 4), round(r.t_linsolve, 4),
+                             round(r.t_update, 4), round(r.t_projection, 4))))
+
+
+def cpu_baseline(scene, n_iter_needed, residual, mic_iterations=None):
+    """Reference-faithful CPU projection: the oracle restating macpressuresolver3 + pcg, with the
+    MIC(0) factor computed and applied but its output discarded (pcg_solver.h:379-384), so the
+    iterates are plain CG's, the reference's threading (fractions, assembly and update on all
+    threads; labelling, CG and scatter serial), timed per stage under the reference's names.
+
+    Bounded sample: two whole projections of the same scene stopped after 20 and 40 CG iterations
+    separate the per-iteration cost from the fixed cost; the solve is extrapolated to the iteration
+    count of the same CG iteration on this scene (measured by the GPU's FP64 run of it).  The CPU's
+    own iteration count is measured on the reference's CPU-runnable configuration (64^3 drop,
+    FP64, 1e-8), solved to completion.  A second line prices the intended algorithm (the MIC(0)
+    output applied) at the iteration count the oracle's own MIC(0)-PCG needs on this scene."""
+    import shiokaze_amd as S
     from oracle import oracle as O
-    t0 = time.perf_counter()
-    A = O.assemble(scene)
-    t_build = time.perf_counter() - t0
-    t1 = time.perf_counter()
-    O.pcg(A, residual=residual, max_iterations=20, mic_mode=O.MIC_FAITHFUL)
-    t20 = time.perf_counter() - t1
-    t1 = time.perf_counter()
-    O.pcg(A, residual=residual, max_iterations=40, mic_mode=O.MIC_FAITHFUL)
-    t40 = time.perf_counter() - t1
-    per_it = max((t40 - t20) / 20.0, 1e-9)
-    fixed = max(t20 - 20 * per_it, 0.0)
-    P = np.zeros((scene.w, scene.h, scene.d), np.float32)
-    work = scene.copy()
-    t2 = time.perf_counter()
-    O.velocity_update(work, P, A["area"], A["rho"])
-    t_upd = time.perf_counter() - t2
-    t_solve = fixed + n_iter_needed * per_it
-    total = t_build + t_solve + t_upd
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return {
-        "value": 1.0 / total, "unit": "solves/s", "cores": threads, "kind": "port",
-        "sample": (f"oracle (reference-faithful: MIC(0) factored and discarded, serial CG, threaded "
-                   f"fractions/assembly/update on {threads} threads) on the same {scene.w}x{scene.h}x{scene.d} scene: "
-                   f"build {t_build:.2f}s + update {t_upd:.2f}s measured, CG {per_it * 1e3:.1f} ms/iteration "
-                   f"(from 20 and 40 iterations) x {n_iter_needed} iterations (count of the same CG iteration run "
-                   f"to {residual:g} on the GPU) + factor {fixed:.2f}s = {total:.1f}s per projection"),
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+    runs = {}
+    for k in (20, 40):
+        prm = O.default_params(residual=residual, max_iterations=k, mic_mode=O.MIC_FAITHFUL, threads=threads)
+        r, _, _ = O.project(scene.copy(), prm, pressure64=False)
+        runs[k] = r
+    per_it = max((runs[40].t_linsolve - runs[20].t_linsolve) / 20.0, 1e-9)
+    fixed_solve = max(runs[20].t_linsolve - 20 * per_it, 0.0)  # matrix copy + MIC(0) factor
+    st = oracle_stages(runs[40])
+    st["Projection_linsolve"] = round(fixed_solve + n_iter_needed * per_it, 3)
+    total = (st["Projection_solid_fluid_fractions"] + st["Projection_build_highres_linsystem"] +
+             st["Projection_linsolve"] + st["Projection_update_velocity"])
+    st["Projection_projection"] = round(total, 3)
+    # the reference's CPU-runnable configuration, solved to completion
+    c1 = S.make_scene(S.SCENE_DROP, 64, seed=0)
+    r1, _, _ = O.project(c1, O.default_params(residual=1e-8, mic_mode=O.MIC_FAITHFUL, threads=threads),
+                         pressure64=False)
+    model, compiler = host_info()
+    out = {
+        "value": round(1.0 / total, 6), "unit": "solves/s", "cores": threads, "kind": "port",
+        "sample": (f"oracle, reference-faithful (MIC(0) factored and applied, output discarded: plain CG "
+                   f"iterates), on the same {scene.w}x{scene.h}x{scene.d} scene: whole projections stopped "
+                   f"after 20 and 40 CG iterations give {per_it * 1e3:.1f} ms per iteration and "
+                   f"{fixed_solve:.2f} s of matrix copy and factor; the solve is extrapolated to "
+                   f"{n_iter_needed} iterations (the same CG iteration to {residual:g}, counted by the GPU's "
+                   f"FP64 run of it); fractions, assembly and update threaded on {threads} threads, labelling, "
+                   f"CG and scatter serial, as in the reference"),
+        "cpu_model": model, "compiler": compiler,
+        "threads": {"Projection_solid_fluid_fractions": threads, "Projection_build_highres_linsystem":
+                    f"{threads} (labelling serial)", "Projection_linsolve": 1, "Projection_update_velocity": threads},
+        "stage_s": st, "cg_iterations": int(n_iter_needed), "cg_ms_per_iteration": round(per_it * 1e3, 3),
+        "c1_64_fp64_1e-8": {"iterations": int(r1.iterations), "reresid": r1.reresid, "converged": bool(r1.converged),
+                            "unknowns": int(r1.n_unknowns), "solves_per_s": round(1.0 / r1.t_projection, 4),
+                            "stage_s": oracle_stages(r1),
+                            "what": "the CPU's own CG iteration count, solved to completion"},
     }
+    if mic_iterations:
+        t_mic = (st["Projection_solid_fluid_fractions"] + st["Projection_build_highres_linsystem"] +
+                 fixed_solve + mic_iterations * per_it + st["Projection_update_velocity"])
+        out["intended_mic0"] = {
+            "value": round(1.0 / t_mic, 5), "unit": "solves/s", "iterations": int(mic_iterations),
+            "what": (f"PCG with the MIC(0) output applied (pcg_solver.h:89-223), {mic_iterations} iterations to "
+                     f"{residual:g} (the oracle's own run on this scene) at the same cost per iteration")}
+    return out
 
 
 def kernel_roofline(stats, inst_steps, config):
@@ -145,15 +195,16 @@ def mg_controls(r):
             "omega": r.mg_omega, "inner_residual": r.inner_residual, "tuned": r.tuned}
 
 
-def bench_batch(args):
-    """C4: 64 uneven grids projected in one batched call per step (mixed precision, 1e-6), against
-    the plain single-solve FP64 GPU path over the same grids one by one."""
+def batched_block(args, steps, single_rounds=1, kernel_timing=False):
+    """The batched workload: 64 uneven grids projected in one batched call per step (mixed
+    precision, 1e-6), against the plain single-solve FP64 GPU path over the same grids one by one.
+    Returns (block dict, kernel stats, instrumented steps)."""
     import shiokaze_amd as S
+    from shiokaze_amd import loadgen
     from shiokaze_amd.devmem import DeviceArray, Stream, device_synchronize
 
     residual = 1e-6
-    log(f"[bench] synthesising c4: {C4_GRIDS} grids")
-    from shiokaze_amd import loadgen
+    log(f"[bench] synthesising the batched workload: {loadgen.C4_GRIDS} grids")
     scenes = loadgen.c4_scenes(workers=min(16, os.cpu_count() or 1))
     cells = sum(sc.w * sc.h * sc.d for sc in scenes)
     log(f"[bench] sides {sorted(sc.w for sc in scenes)}; {cells / 1e6:.1f} M cells")
@@ -188,13 +239,13 @@ def bench_batch(args):
             out.append(ctx.project_batch_device([g], prm, stream=stream)[0])
         return out
 
-    def timed(fn, prm, steps, label):
-        for _ in range(args.warmup):
+    def timed(fn, prm, n, label):
+        for _ in range(max(1, args.warmup if fn is batched else 1)):
             fn(prm)
         st.synchronize()
         lat, res = [], None
         t0 = time.perf_counter()
-        for _ in range(steps):
+        for _ in range(n):
             ts = time.perf_counter()
             res = fn(prm)
             lat.append((time.perf_counter() - ts) * 1e3)
@@ -203,60 +254,65 @@ def bench_batch(args):
         worst = max(r.reresid_inf for r in res)
         conv = all(r.converged for r in res)
         its = [r.iterations for r in res]
-        log(f"[bench] {label}: {steps} x {len(res)} solves in {el:.3f}s -> {steps * len(res) / el:.1f} solves/s; "
+        log(f"[bench] {label}: {n} x {len(res)} solves in {el:.3f}s -> {n * len(res) / el:.1f} solves/s; "
             f"iterations {min(its)}..{max(its)}; worst reresid_inf {worst:.2e}; all converged {conv}")
-        log(f"[bench] {label}: controls {mg_controls(res[-1])}")
-        return steps * len(res) / el, el / steps * 1e3, lat, worst, conv
+        return n * len(res) / el, el / n * 1e3, lat, worst, conv
 
-    v_b, ms_b, lat_b, worst_b, conv_b = timed(batched, prm_mixed, args.steps, "batched mixed")
-    v_f, ms_f, _, worst_f, conv_f = timed(single, prm_fp64, max(1, args.steps // 2), "single FP64")
-    v_m, _, _, _, _ = timed(single, prm_mixed, max(1, args.steps // 2), "single mixed")
-    v_bf, _, _, _, _ = timed(batched, prm_fp64, max(1, args.steps // 2), "batched FP64")
+    v_b, ms_b, lat_b, worst_b, conv_b = timed(batched, prm_mixed, steps, "batched mixed")
+    v_f, ms_f, _, worst_f, conv_f = timed(single, prm_fp64, single_rounds, "single FP64")
     stats, inst_steps = {}, 0
-    if not args.no_kernel_timing:
+    if kernel_timing:
         ctx.set_kernel_timing(True)
         ctx.reset_kernel_stats()
-        inst_steps = max(1, min(args.steps, 3))
+        inst_steps = max(1, min(steps, 3))
         for _ in range(inst_steps):
             batched(prm_mixed)
         st.synchronize()
         stats = ctx.kernel_stats()
         ctx.set_kernel_timing(False)
-    roofline = kernel_roofline(stats, inst_steps, "c4")
     ctx.close()
-    out = {
-        "metric": f"batched pressure solves/sec ({C4_GRIDS} uneven grids to 1e-6 rel-res)",
-        "value": round(v_b, 3), "unit": "solves/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": round(ms_b, 3), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32+f64", "data": "synthetic",
-        "config": {"workload": f"{C4_GRIDS} sphere-drop cubes, sides from {list(C4_SIDES)} (seed 3), mgpcg mixed "
-                               f"to {residual:g} relative residual (inf norm), one batched call per step",
-                   "grids": C4_GRIDS, "cells": int(cells), "sides": sorted(int(sc.w) for sc in scenes),
-                   "worst_reresid_inf": worst_b, "all_converged": conv_b,
-                   "cell_updates_per_s": round(v_b / C4_GRIDS * cells, 1)},
+    block = {
+        "workload": (f"{loadgen.C4_GRIDS} sphere-drop cubes, sides from {list(loadgen.C4_SIDES)} (seed 3), "
+                     f"{cells / 1e6:.1f} M cells; mgpcg mixed to {residual:g} (inf norm), one batched call per step"),
+        "solves_per_s": round(v_b, 3), "ms_per_batch": round(ms_b, 3), "steps": steps,
+        "worst_reresid_inf": worst_b, "all_converged": conv_b, "cells": int(cells),
+        "sides": sorted(int(sc.w) for sc in scenes), "cell_updates_per_s": round(v_b / loadgen.C4_GRIDS * cells, 1),
         "single_fp64_solves_per_s": round(v_f, 3), "single_fp64_worst_reresid_inf": worst_f,
-        "single_fp64_all_converged": conv_f, "single_mixed_solves_per_s": round(v_m, 3),
-        "batched_fp64_solves_per_s": round(v_bf, 3), "speedup_vs_single_fp64": round(v_b / v_f, 3),
-        "latency_ms": {"p50": round(float(np.percentile(lat_b, 50)), 3),
-                       "p99": round(float(np.percentile(lat_b, 99)), 3)},
-        "roofline": roofline, "cpu_baseline": None,
+        "single_fp64_all_converged": conv_f, "speedup_vs_single_fp64": round(v_b / v_f, 3),
+        "batch_latency_ms": {"p50": round(float(np.percentile(lat_b, 50)), 3),
+                             "p99": round(float(np.percentile(lat_b, 99)), 3)},
+    }
+    return block, stats, inst_steps
+
+
+def bench_batch(args):
+    """--config c4: the batched workload as its own line."""
+    block, stats, inst_steps = batched_block(args, args.steps, max(1, args.steps // 2),
+                                             kernel_timing=not args.no_kernel_timing)
+    out = {
+        "metric": "batched pressure solves/sec (64 uneven grids to 1e-6 rel-res)",
+        "value": block["solves_per_s"], "unit": "solves/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": block["ms_per_batch"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32+f64", "data": "synthetic", "config": block,
+        "roofline": kernel_roofline(stats, inst_steps, "c4"), "cpu_baseline": None,
     }
     print(json.dumps(out), flush=True)
 
 
-def bench_serve(args):
-    """C5: Poisson-arrival stream of time-stepped projection requests against the server
-    (continuous batching, HBM-budget admission, warm start from resident pressure)."""
+def served_block(args, n_requests):
+    """The served workload: the Poisson-arrival stream (200 requests/s, seed 4) of 20-step
+    time-stepped requests against the server (continuous batching, HBM-budget admission, warm start
+    from resident pressure).  Returns the block dict."""
     import shiokaze_amd as S
     from shiokaze_amd import loadgen
 
-    specs = loadgen.make_requests(args.c5_requests, rate=200.0, seed=4)
+    specs = loadgen.make_requests(n_requests, rate=200.0, seed=4)
     steps = args.c5_steps
     # library default controls: tuning on the first batch of each new shape class would land in
     # the served latencies
     prm = S.default_params(residual=1e-6, warm_start=1)
     cells = [s.w * s.h * s.d for s in specs]
-    log(f"[bench] c5: {len(specs)} requests x {steps} steps, mean {np.mean(cells) / 1e6:.2f} M cells, "
+    log(f"[bench] served: {len(specs)} requests x {steps} steps, mean {np.mean(cells) / 1e6:.2f} M cells, "
         f"arrivals over {specs[-1].t_arrival:.2f}s")
     # warm-up: a few requests through a separate server (code paths, allocations, graphs)
     with S.Server(0, executors=args.c5_executors) as srv:
@@ -267,28 +323,84 @@ def bench_serve(args):
     ok = sum(1 for x in res["statuses"] if x == 0)
     value = res["solves"] / res["elapsed_s"]
     rl, sl = res["request_latency_ms"], res["solve_latency_ms"]
-    log(f"[bench] c5: {res['solves']} solves ({ok} ok) in {res['elapsed_s']:.2f}s -> {value:.1f} solves/s; "
+    offered = 200.0 * steps
+    log(f"[bench] served: {res['solves']} solves ({ok} ok) in {res['elapsed_s']:.2f}s -> {value:.1f} solves/s; "
         f"request p50 {np.percentile(rl, 50):.0f} ms p99 {np.percentile(rl, 99):.0f} ms; "
         f"solve p50 {np.percentile(sl, 50):.1f} ms p99 {np.percentile(sl, 99):.1f} ms; server {st}")
+    return {
+        "workload": (f"{len(specs)} requests (Poisson arrivals, 200/s, seed 4) x {steps} steps, sides U[48,192] "
+                     f"aspect <= 2, translated sphere-drop, warm start, mgpcg mixed to 1e-6"),
+        "requests": len(specs), "requests_full_stream": 10000,
+        "reduced": (f"the first {len(specs)} requests of the 10,000-request stream; the stream offers "
+                    f"{offered:.0f} solves/s, above this device's capacity, so request latency is queueing "
+                    f"and grows with the stream's length; solve latency (start of a request's batch to its "
+                    f"completion) does not") if len(specs) < 10000 else "full stream",
+        "steps_per_request": steps, "executors": args.c5_executors, "hbm_budget_bytes": 200 * 10 ** 9,
+        "solves": res["solves"], "ok_solves": ok, "elapsed_s": round(res["elapsed_s"], 3),
+        "solves_per_s": round(value, 3), "offered_solves_per_s": offered,
+        "mean_cells": float(np.mean(cells)), "cell_updates_per_s": round(sum(cells) * steps / res["elapsed_s"], 1),
+        "median_iterations": float(np.median(res["iterations"])) if res["iterations"] else None,
+        "request_latency_ms": {"p50": round(float(np.percentile(rl, 50)), 3),
+                               "p99": round(float(np.percentile(rl, 99)), 3)},
+        "solve_latency_ms": {"p50": round(float(np.percentile(sl, 50)), 3),
+                             "p99": round(float(np.percentile(sl, 99)), 3)},
+        "server": st,
+    }
+
+
+def bench_serve(args):
+    """--config c5: the served workload as its own line."""
+    block = served_block(args, args.c5_requests)
     out = {
         "metric": "served pressure solves/sec (Poisson 200 req/s, 20-step requests, 1e-6 rel-res)",
-        "value": round(value, 3), "unit": "solves/s", "n_gpus": 1, "steps": res["solves"], "warmup": 8,
-        "ms_per_step": round(res["elapsed_s"] / max(res["solves"], 1) * 1e3, 3), "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic",
-        "config": {"workload": f"{len(specs)} requests (Poisson arrivals, 200/s, seed 4) x {steps} steps, sides "
-                               f"U[48,192] aspect <= 2, translated sphere-drop, warm start, mgpcg mixed to 1e-6",
-                   "requests": len(specs), "steps_per_request": steps, "executors": args.c5_executors,
-                   "hbm_budget_bytes": 200 * 10 ** 9, "mean_cells": float(np.mean(cells)),
-                   "ok_solves": ok, "cell_updates_per_s": round(sum(cells) * steps / res["elapsed_s"], 1),
-                   "median_iterations": float(np.median(res["iterations"])) if res["iterations"] else None,
-                   "server": st},
-        "latency_ms": {"p50": round(float(np.percentile(rl, 50)), 3), "p99": round(float(np.percentile(rl, 99)), 3),
+        "value": block["solves_per_s"], "unit": "solves/s", "n_gpus": 1, "steps": block["solves"], "warmup": 8,
+        "ms_per_step": round(block["elapsed_s"] / max(block["solves"], 1) * 1e3, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic", "config": block,
+        "latency_ms": {"p50": block["request_latency_ms"]["p50"], "p99": block["request_latency_ms"]["p99"],
                        "what": "request (arrival to last step done)",
-                       "solve_p50": round(float(np.percentile(sl, 50)), 3),
-                       "solve_p99": round(float(np.percentile(sl, 99)), 3)},
+                       "solve_p50": block["solve_latency_ms"]["p50"], "solve_p99": block["solve_latency_ms"]["p99"]},
         "roofline": None, "cpu_baseline": None,
     }
     print(json.dumps(out), flush=True)
+
+
+def accuracy_block(ctx, sc, prm, residual):
+    """The returned pressure against the FP64 oracle: the reference's PCG with its MIC(0) factor
+    applied, solved to 1e-10 on the oracle's own assembly of the same scene; the reference PCG's
+    own answer at the benchmark tolerance (its iterate at the first iteration meeting it) for
+    scale; the maximum post-projection divergence over the unknowns for the GPU result and for the
+    oracle's update with the oracle pressure."""
+    from oracle import oracle as O
+    work = sc.copy()
+    r, P, P64 = ctx.project(work, prm, pressure64=True)
+    t0 = time.perf_counter()
+    A = O.assemble(sc)
+    x = O.gather(A, P64)
+    bmax = float(np.abs(A["b"]).max())
+    rel_res = float(np.abs(A["b"] - O.spmv(A, x)).max() / bmax)
+    t1 = time.perf_counter()
+    x_ref, it, rr, ok, x_snap, it_snap = O.pcg_snap(A, residual=1e-10, snap_residual=residual,
+                                                    mic_mode=O.MIC_APPLIED)
+    t_pcg = time.perf_counter() - t1
+    nref = float(np.linalg.norm(x_ref))
+    err = float(np.linalg.norm(x - x_ref) / nref)
+    err_ref = float(np.linalg.norm(x_snap - x_ref) / nref)
+    div_gpu, _ = O.divergence(work, A["index_map"], A["area"])
+    ref = sc.copy()
+    O.velocity_update(ref, O.scatter(A, x_ref, (sc.w, sc.h, sc.d)).astype(np.float32), A["area"], A["rho"])
+    div_ref, _ = O.divergence(ref, A["index_map"], A["area"])
+    log(f"[bench] accuracy: GPU residual {rel_res:.2e} on the oracle's assembly; pressure rel-L2 {err:.3e} "
+        f"(reference PCG at {residual:g}: {err_ref:.3e}); max|div| GPU {div_gpu:.4e} oracle {div_ref:.4e} "
+        f"(oracle MIC(0)-PCG {it} iterations, {t_pcg:.1f}s; check {time.perf_counter() - t0:.1f}s)")
+    return {
+        "oracle": f"FP64 MIC(0)-PCG (pcg_solver.h:89-295, factor applied) to 1e-10 relative residual: {it} iterations",
+        "gpu_reresid_inf_on_oracle_assembly": rel_res, "pressure_rel_l2": err,
+        "reference_pcg_rel_l2_at_tolerance": err_ref, "reference_pcg_iterations_to_tolerance": int(it_snap),
+        "max_div_gpu": float(div_gpu), "max_div_oracle": float(div_ref),
+        "div_delta_abs": float(abs(div_gpu - div_ref)), "div_delta_rel_b_inf": float(abs(div_gpu - div_ref) / bmax),
+        "b_inf": bmax, "div_gate": "|max div_gpu - max div_oracle| <= 1e-5 ||b||_inf",
+        "div_gate_met": bool(abs(div_gpu - div_ref) <= 1e-5 * bmax),
+    }, int(it_snap)
 
 
 def bench_post(args):
@@ -394,7 +506,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--config", default="c3", choices=sorted(CONFIGS) + ["c4", "c5", "post"])
     ap.add_argument("--post-width", type=int, default=3, help="post: extrapolation passes")
-    ap.add_argument("--c5-requests", type=int, default=200, help="c5: requests in the sample")
+    ap.add_argument("--c5-requests", type=int, default=1000, help="served: requests of the stream measured")
+    ap.add_argument("--batched-steps", type=int, default=5, help="default line: batched calls timed")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="default line: skip the FP64 rate, accuracy, batched and served blocks")
     ap.add_argument("--c5-steps", type=int, default=20, help="c5: steps per request")
     ap.add_argument("--c5-executors", type=int, default=2, help="c5: server executors")
     ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64"])
@@ -522,6 +637,31 @@ def main():
         "roofline": roofline,
         "cpu_baseline": None,
     }
+    mic_iterations = None
+    if args.config == "c3" and not args.no_extras and prec == S.PREC_MIXED and solver == S.SOLVER_MGPCG:
+        # the plain single-solve FP64 GPU path on the same scene (the comparison point of the
+        # batched/served claim), same timing protocol
+        prm64 = S.default_params(residual=residual, precision=S.PREC_FP64,
+                                 autotune=0 if (args.no_autotune or args.mg) else 1, **explicit_mg(args.mg))
+        saved = prm
+        prm = prm64
+        for _ in range(2):
+            r64 = step()
+        st.synchronize()
+        t0 = time.perf_counter()
+        n64 = 5
+        for _ in range(n64):
+            r64 = step()
+        st.synchronize()
+        el64 = time.perf_counter() - t0
+        prm = saved
+        out["fp64_single"] = {"solves_per_s": round(n64 / el64, 3), "ms_per_solve": round(el64 / n64 * 1e3, 3),
+                              "iterations": int(r64.iterations), "reresid_inf": r64.reresid_inf,
+                              "converged": bool(r64.converged), "mg_controls": mg_controls(r64),
+                              "what": "plain FP64 MGPCG, one projection per call, same scene and timing"}
+        log(f"[bench] FP64 single solve: {n64 / el64:.1f} solves/s ({r64.iterations} iterations)")
+        log("[bench] accuracy against the FP64 oracle ...")
+        out["accuracy"], mic_iterations = accuracy_block(ctx, sc, prm, residual)
     if not args.no_cpu_baseline:
         # iteration count of the reference's own (unpreconditioned) CG on this scene, from the GPU
         prm_cg = S.default_params(residual=residual, precision=S.PREC_FP64, solver=S.SOLVER_CG)
@@ -533,8 +673,23 @@ def main():
         out["config"]["reference_cg_iterations"] = int(rcg.iterations)
         out["config"]["gpu_reference_cg_ms"] = round(rcg.t_projection, 2)
         log("[bench] CPU baseline sample ...")
-        out["cpu_baseline"] = cpu_baseline(sc, rcg.iterations, residual)
+        out["cpu_baseline"] = cpu_baseline(sc, rcg.iterations, residual, mic_iterations)
     ctx.close()
+    for a in [d_fluid, d_solid] + src + vel:
+        if a is not None:
+            a.free()
+    if args.config == "c3" and not args.no_extras:
+        out["batched"], _, _ = batched_block(args, args.batched_steps)
+        if out.get("fp64_single"):
+            out["batched"]["speedup_vs_c3_fp64_cells_per_s"] = round(
+                out["batched"]["cell_updates_per_s"] / (out["fp64_single"]["solves_per_s"] * n ** 3), 3)
+        out["served"] = served_block(args, args.c5_requests)
+        out["latency_ms"]["served_request_p50"] = out["served"]["request_latency_ms"]["p50"]
+        out["latency_ms"]["served_request_p99"] = out["served"]["request_latency_ms"]["p99"]
+        out["latency_ms"]["served_solve_p50"] = out["served"]["solve_latency_ms"]["p50"]
+        out["latency_ms"]["served_solve_p99"] = out["served"]["solve_latency_ms"]["p99"]
+        out["latency_ms"]["what"] = ("p50/p99: closed loop of single projections of this scene; served_*: the "
+                                     "served stream (request = arrival to last step, solve = one projection)")
     print(json.dumps(out), flush=True)
 
 

@@ -490,13 +490,15 @@ static void add_scaled(int64_t n, double alpha, const double *x, double *y)
     for (int64_t i = 0; i < n; ++i) y[i] += alpha * x[i];
 }
 
-/* Column-stored lower factor L (strictly lower part by columns) plus inverse diagonal. */
+/* Incomplete Cholesky factor L (A ~ L L^T) with the pattern of A's lower triangle.  Column k of
+ * L holds the entries below its diagonal, stored in row k's upper part (A is symmetric), in
+ * ascending row order; invdiag[k] = 1 / L_kk. */
 typedef struct {
     int64_t n;
     int64_t *colstart; /* n+1 */
     int64_t *rowindex;
     double *value;
-    double *invdiag, *adiag;
+    double *invdiag;
 } mic_factor;
 
 static void mic_free(mic_factor *f)
@@ -505,106 +507,101 @@ static void mic_free(mic_factor *f)
     free(f->rowindex);
     free(f->value);
     free(f->invdiag);
-    free(f->adiag);
 }
 
-/* Modified incomplete Cholesky with zero fill-in, pcg_solver.h:89-189.  Column k of L is the
- * strictly-lower part of A's column k (= row k's upper part, A symmetric).  Sweeping k in
- * order: pick the pivot (falling back to A_kk when it has shrunk below min_diag_ratio*A_kk),
- * scale the column, then for each row j of the column subtract the rank-one update from the
- * entries of column j that exist in the pattern and collect the ones that do not ("missing",
- * including entries above column j's diagonal that A lacks) into omega-weighted diagonal
- * compensation. */
+/* Position of row r in column c of the factor (rows ascending), -1 if the pattern lacks it. */
+static int64_t mic_find(const mic_factor *F, int64_t c, int64_t r)
+{
+    for (int64_t q = F->colstart[c]; q < F->colstart[c + 1]; ++q) {
+        if (F->rowindex[q] == r) return q;
+        if (F->rowindex[q] > r) break;
+    }
+    return -1;
+}
+
+/* Modified incomplete Cholesky with zero fill-in, the factor pcg_solver.h:89-189 builds (omega =
+ * 0.97, pivots below min_diag_ratio of the original diagonal replaced by the original diagonal).
+ * Right-looking: column k takes its pivot from the running diagonal d, is scaled by 1/L_kk, and
+ * then updates the trailing matrix with the outer product of its entries c: for every pair of its
+ * rows j < l the product c_j c_l is subtracted from entry (l, j) where the pattern holds one, and
+ * otherwise (a dropped fill-in) omega times it is taken from both d_j and d_l, which keeps the row
+ * sums of L L^T those of A; each d_j loses c_j^2.  Rows whose original diagonal is 0 are left out
+ * (invdiag 0). */
 static void mic_factor_build(int64_t n, const int64_t *rowptr, const int64_t *col, const double *val,
                              double omega, double min_ratio, mic_factor *F)
 {
     F->n = n;
     F->colstart = (int64_t *)malloc((size_t)(n + 1) * sizeof(int64_t));
-    F->invdiag = (double *)calloc((size_t)n, sizeof(double));
-    F->adiag = (double *)calloc((size_t)n, sizeof(double));
-    int64_t cnt = 0;
+    F->invdiag = (double *)calloc((size_t)(n ? n : 1), sizeof(double));
+    double *a0 = (double *)calloc((size_t)(n ? n : 1), sizeof(double)); /* original diagonal */
+    double *d = (double *)calloc((size_t)(n ? n : 1), sizeof(double));  /* running diagonal */
+    int64_t total = 0;
     for (int64_t i = 0; i < n; ++i)
-        for (int64_t p = rowptr[i]; p < rowptr[i + 1]; ++p)
-            if (col[p] > i) ++cnt;
-    F->rowindex = (int64_t *)malloc((size_t)(cnt ? cnt : 1) * sizeof(int64_t));
-    F->value = (double *)malloc((size_t)(cnt ? cnt : 1) * sizeof(double));
+        for (int64_t p = rowptr[i]; p < rowptr[i + 1]; ++p) total += col[p] > i;
+    F->rowindex = (int64_t *)malloc((size_t)(total ? total : 1) * sizeof(int64_t));
+    F->value = (double *)malloc((size_t)(total ? total : 1) * sizeof(double));
     int64_t at = 0;
     for (int64_t i = 0; i < n; ++i) {
         F->colstart[i] = at;
         for (int64_t p = rowptr[i]; p < rowptr[i + 1]; ++p) {
-            if (col[p] > i) {
-                F->rowindex[at] = col[p];
-                F->value[at] = val[p];
-                ++at;
-            } else if (col[p] == i) {
-                F->invdiag[i] = F->adiag[i] = val[p];
+            if (col[p] == i) a0[i] = d[i] = val[p];
+            if (col[p] <= i) continue;
+            /* insertion keeps the column's rows ascending (CSR columns may come in any order) */
+            int64_t q = at++;
+            while (q > F->colstart[i] && F->rowindex[q - 1] > col[p]) {
+                F->rowindex[q] = F->rowindex[q - 1];
+                F->value[q] = F->value[q - 1];
+                --q;
             }
+            F->rowindex[q] = col[p];
+            F->value[q] = val[p];
         }
     }
     F->colstart[n] = at;
-
     for (int64_t k = 0; k < n; ++k) {
-        if (F->adiag[k] == 0) continue;
-        if (F->invdiag[k] < min_ratio * F->adiag[k]) F->invdiag[k] = 1 / sqrt(F->adiag[k]);
-        else F->invdiag[k] = 1 / sqrt(F->invdiag[k]);
-        const int64_t c0 = F->colstart[k], c1 = F->colstart[k + 1];
-        for (int64_t p = c0; p < c1; ++p) F->value[p] *= F->invdiag[k];
-        for (int64_t p = c0; p < c1; ++p) {
+        if (a0[k] == 0) continue;
+        const double pivot = d[k] < min_ratio * a0[k] ? a0[k] : d[k];
+        const double inv = 1.0 / sqrt(pivot);
+        F->invdiag[k] = inv;
+        const int64_t b0 = F->colstart[k], b1 = F->colstart[k + 1];
+        for (int64_t p = b0; p < b1; ++p) F->value[p] *= inv;
+        for (int64_t p = b0; p < b1; ++p) {
             const int64_t j = F->rowindex[p];
-            const double multiplier = F->value[p];
-            double missing = 0;
-            int64_t a = c0;
-            /* entries of column k above row j: present in A's row j? if not, they are missing */
-            int64_t bpos = rowptr[j];
-            while (a < c1 && F->rowindex[a] < j) {
-                const int64_t target = F->rowindex[a];
-                while (bpos < rowptr[j + 1]) {
-                    if (col[bpos] < target) ++bpos;
-                    else if (col[bpos] == target) break;
-                    else {
-                        missing += F->value[a];
-                        break;
-                    }
-                }
-                ++a;
-            }
-            if (a < c1 && F->rowindex[a] == j) F->invdiag[j] -= multiplier * F->value[a];
-            ++a;
-            int64_t b = F->colstart[j];
-            while (a < c1 && b < F->colstart[j + 1]) {
-                if (F->rowindex[b] < F->rowindex[a]) ++b;
-                else if (F->rowindex[b] == F->rowindex[a]) {
-                    F->value[b] -= multiplier * F->value[a];
-                    ++a;
-                    ++b;
+            const double cj = F->value[p];
+            d[j] -= cj * cj;
+            for (int64_t q = p + 1; q < b1; ++q) {
+                const int64_t l = F->rowindex[q];
+                const double prod = cj * F->value[q];
+                const int64_t e = mic_find(F, j, l);
+                if (e >= 0) {
+                    F->value[e] -= prod;
                 } else {
-                    missing += F->value[a];
-                    ++a;
+                    d[j] -= omega * prod;
+                    d[l] -= omega * prod;
                 }
             }
-            while (a < c1) {
-                missing += F->value[a];
-                ++a;
-            }
-            F->invdiag[j] -= omega * multiplier * missing;
         }
     }
+    free(a0);
+    free(d);
 }
 
-/* pcg_solver.h:196-207 and :211-223 */
-static void mic_apply(const mic_factor *F, const double *x, double *out)
+/* z = (L L^T)^-1 x, the preconditioner pcg_solver.h:196-223 applies: forward substitution with L
+ * column by column (each solved entry is scattered into the rows below it), then backward
+ * substitution with L^T, whose row c is L's column c. */
+static void mic_apply(const mic_factor *F, const double *x, double *z)
 {
     const int64_t n = F->n;
-    memcpy(out, x, (size_t)n * sizeof(double));
-    for (int64_t i = 0; i < n; ++i) {
-        out[i] *= F->invdiag[i];
-        for (int64_t p = F->colstart[i]; p < F->colstart[i + 1]; ++p) out[F->rowindex[p]] -= F->value[p] * out[i];
+    memcpy(z, x, (size_t)n * sizeof(double));
+    for (int64_t c = 0; c < n; ++c) {
+        z[c] *= F->invdiag[c];
+        const double zc = z[c];
+        for (int64_t p = F->colstart[c]; p < F->colstart[c + 1]; ++p) z[F->rowindex[p]] -= F->value[p] * zc;
     }
-    int64_t i = n;
-    while (i > 0) {
-        --i;
-        for (int64_t p = F->colstart[i]; p < F->colstart[i + 1]; ++p) out[i] -= F->value[p] * out[F->rowindex[p]];
-        out[i] *= F->invdiag[i];
+    for (int64_t c = n - 1; c >= 0; --c) {
+        double acc = z[c];
+        for (int64_t p = F->colstart[c]; p < F->colstart[c + 1]; ++p) acc -= F->value[p] * z[F->rowindex[p]];
+        z[c] = acc * F->invdiag[c];
     }
 }
 

@@ -439,12 +439,23 @@ struct shkz_ctx {
     bool use_graphs = true;
     bool trace = false;
     bool host_links = false;  // SHKZ_HOST_LINKS=1: transfer tables built on the host (a cross-check)
+    // mixed precision: each defect-correction pass after the first continues the previous pass's
+    // search direction (p = z + beta p with the replaced residual) instead of restarting CG
+    bool keep_direction = false;
     std::string device_key;  // device identity of the tuning cache
-    // replayable PCG iteration graphs, keyed by everything their kernel arguments depend on
+    // replayable PCG iteration graphs, keyed by everything their kernel arguments depend on; an
+    // exec is destroyed only to make room (least recently used, after the stream has drained) or
+    // with the context, so trying many controls (the autotuner) re-instantiates nothing it keeps
     struct GraphSlot {
         std::string key;
         hipGraphExec_t exec = nullptr;
-    } graphs[4];
+        uint64_t used = 0;
+    };
+    std::vector<GraphSlot> graphs;
+    uint64_t graph_clock = 0;
+    // iterations each solve of a shape class took (per defect-correction pass, slowest grid): the
+    // next solve of the class replays that many iterations before its first poll
+    std::unordered_map<std::string, std::vector<int>> iter_hist;
 };
 
 namespace shkz {
@@ -829,15 +840,20 @@ static int ensure_pinned(shkz_ctx::Pinned &b, size_t bytes, void **out) {
     return SHKZ_OK;
 }
 
-// Reads n per-grid scalars (and optionally a control block) back to the host.
+// Reads n per-grid scalars (and optionally a control block, and a second block of n scalars
+// into out2) back to the host in one synchronisation.
 static int read_scalars(shkz_ctx *C, hipStream_t s, const SolveScalars *dS, int n, std::vector<SolveScalars> &out,
-                        const BatchCtl *dctl = nullptr, BatchCtl *ctl_out = nullptr) {
-    SHKZ_TRY(ensure_host_scalars(C, n));
+                        const BatchCtl *dctl = nullptr, BatchCtl *ctl_out = nullptr, const SolveScalars *dS2 = nullptr,
+                        std::vector<SolveScalars> *out2 = nullptr) {
+    SHKZ_TRY(ensure_host_scalars(C, 2 * n));
     SHKZ_HIP(hipMemcpyAsync(C->host_scalars, dS, (size_t)n * sizeof(SolveScalars), hipMemcpyDeviceToHost, s));
+    if (dS2)
+        SHKZ_HIP(hipMemcpyAsync(C->host_scalars + n, dS2, (size_t)n * sizeof(SolveScalars), hipMemcpyDeviceToHost, s));
     if (dctl) SHKZ_HIP(hipMemcpyAsync(C->host_ctl, dctl, sizeof(BatchCtl), hipMemcpyDeviceToHost, s));
     SHKZ_HIP(hipStreamSynchronize(s));
     SHKZ_HIP(hipGetLastError());
     out.assign(C->host_scalars, C->host_scalars + n);
+    if (dS2 && out2) out2->assign(C->host_scalars + n, C->host_scalars + 2 * n);
     if (ctl_out) *ctl_out = *C->host_ctl;
     if (C->timing) resolve_timing(C);
     return SHKZ_OK;
@@ -882,9 +898,17 @@ static std::string graph_key(shkz_ctx *C, const Problem &P, const void *const *v
 // into a hipGraph and replayed; every kernel returns at once when all grids are done and skips the
 // bricks of a grid that is done, so a chunk of replays may overshoot by a few no-op iterations
 // without changing any result.  `last` receives the final per-grid scalars.
+//
+// first_chunk: iterations replayed before the first poll (the caller's prediction; results do not
+// depend on it).  S2/last2: another scalar block read back with every poll.
+//
+// keep_dir: the first direction continues the search direction p already in place (beta from the
+// new r.z over the previous one in S) instead of starting with p = z.
 template <class T>
 static int krylov(shkz_ctx *C, Launcher &K, Problem &P, T *x, T *r, T *p, T *q, T *z, T *tmp, SolveScalars *S,
-                  BatchCtl *ctl, bool precond, std::vector<SolveScalars> &last) {
+                  BatchCtl *ctl, bool precond, std::vector<SolveScalars> &last, int first_chunk = 4,
+                  const SolveScalars *S2 = nullptr, std::vector<SolveScalars> *last2 = nullptr,
+                  bool keep_dir = false) {
     LevelHost &L0 = C->levels[0];
     const unsigned nb = brick_grid(L0.nbricks);
     const dim3 g(nb), b(BLOCK);
@@ -908,8 +932,8 @@ static int krylov(shkz_ctx *C, Launcher &K, Problem &P, T *x, T *r, T *p, T *q, 
     if (!fused_dot)
         K.go("dot_rz", 0, 2 * es, 0, g, b, k_dot_rz<T>, L0.desc, (const T *)r, zp, (const SolveScalars *)S,
              (const BatchCtl *)ctl, P.red);
-    finalize(K, P, L0.desc, S, ctl, FIN_RZ_FIRST);
-    update_p(2 * es, zp, 1);
+    finalize(K, P, L0.desc, S, ctl, keep_dir ? FIN_RZ : FIN_RZ_FIRST);
+    update_p(keep_dir ? 3 * es : 2 * es, zp, keep_dir ? 0 : 1);
 
     auto iteration = [&]() {
         if constexpr (sizeof(T) == 4)
@@ -940,15 +964,23 @@ static int krylov(shkz_ctx *C, Launcher &K, Problem &P, T *x, T *r, T *p, T *q, 
     hipGraphExec_t exec = nullptr;
     if (C->use_graphs && !C->timing) {
         const void *vecs[] = {x, r, p, q, z, tmp, S, ctl};
-        const std::string key = graph_key<T>(C, P, vecs, 8, precond);
-        auto &slot = C->graphs[(sizeof(T) == 8 ? 2 : 0) + (precond ? 1 : 0)];
-        if (slot.exec && slot.key == key) {
-            exec = slot.exec;
-        } else {
-            if (slot.exec) {
-                SHKZ_HIP(hipGraphExecDestroy(slot.exec));
-                slot.exec = nullptr;
-                slot.key.clear();
+        const std::string key = std::string(sizeof(T) == 8 ? "d" : "f") + graph_key<T>(C, P, vecs, 8, precond);
+        for (auto &g : C->graphs)
+            if (g.exec && g.key == key) {
+                exec = g.exec;
+                g.used = ++C->graph_clock;
+                break;
+            }
+        if (!exec) {
+            constexpr size_t kMaxGraphs = 24;
+            if (C->graphs.size() >= kMaxGraphs) {
+                // evict the least recently used exec once every launch of it has completed
+                size_t lru = 0;
+                for (size_t i = 1; i < C->graphs.size(); ++i)
+                    if (C->graphs[i].used < C->graphs[lru].used) lru = i;
+                SHKZ_HIP(hipStreamSynchronize(K.s));
+                SHKZ_HIP(hipGraphExecDestroy(C->graphs[lru].exec));
+                C->graphs.erase(C->graphs.begin() + (long)lru);
             }
             hipGraph_t graph = nullptr;
             SHKZ_HIP(hipStreamBeginCapture(K.s, hipStreamCaptureModeThreadLocal));
@@ -956,24 +988,22 @@ static int krylov(shkz_ctx *C, Launcher &K, Problem &P, T *x, T *r, T *p, T *q, 
             iteration();
             K.capturing = false;
             SHKZ_HIP(hipStreamEndCapture(K.s, &graph));
-            hipError_t e = hipGraphInstantiate(&slot.exec, graph, nullptr, nullptr, 0);
+            hipGraphExec_t made = nullptr;
+            hipError_t e = hipGraphInstantiateWithFlags(&made, graph, 0);
             (void)hipGraphDestroy(graph);
-            if (e != hipSuccess) {
-                slot.exec = nullptr;
-                return fail(SHKZ_ERR_DEVICE, "hipGraphInstantiate failed: %s", hipGetErrorString(e));
-            }
-            slot.key = key;
-            exec = slot.exec;
+            if (e != hipSuccess) return fail(SHKZ_ERR_DEVICE, "hipGraphInstantiate failed: %s", hipGetErrorString(e));
+            C->graphs.push_back(shkz_ctx::GraphSlot{key, made, ++C->graph_clock});
+            exec = made;
         }
     }
-    int chunk = 4;
+    int chunk = std::max(1, std::min(first_chunk, 64));
     BatchCtl hc{};
     for (;;) {
         for (int it = 0; it < chunk; ++it) {
             if (exec) SHKZ_HIP(hipGraphLaunch(exec, K.s));
             else iteration();
         }
-        SHKZ_TRY(read_scalars(C, K.s, S, P.ngrids, last, ctl, &hc));
+        SHKZ_TRY(read_scalars(C, K.s, S, P.ngrids, last, ctl, &hc, S2, last2));
         if (hc.all_done) break;
         // size the next chunk from the slowest grid's observed contraction rate
         int next = 1;
@@ -1056,9 +1086,31 @@ struct SolveOut {
 // The linear solve of every grid of the built batch (macpressuresolver3.cpp:233-237): FP64
 // CG / MGPCG, or FP64 defect correction around FP32 MGPCG.  Starts from P.x64 and leaves the
 // solution there; fills the per-grid residual and convergence fields of res.
+static std::string tune_key(const shkz_ctx *C, const Problem &P, const shkz_params *prm);
+
+// Key of a solve's iteration history: the shape class and the controls in use.
+static std::string hist_key(const shkz_ctx *C, const Problem &P, const shkz_params *prm) {
+    char buf[160];
+    std::snprintf(buf, sizeof(buf), "|%d %d %d %d %d %d %.6g %.3g", P.pre, P.post, P.pre_c, P.post_c, P.coarse_sweeps,
+                  P.multi_max_bricks, P.omega, P.eta);
+    return tune_key(C, P, prm) + buf;
+}
+
 static int solve_phase(shkz_ctx *C, Launcher &K, Problem &P, const shkz_params *prm, bool multigrid, SolveOut &O,
                        shkz_result *res) {
     const int ngrids = P.ngrids;
+    // iterations the last solve of this class took per defect-correction pass (slowest grid): the
+    // first chunk of each Krylov solve, and whether a pass is expected to run a Krylov solve at all
+    // (then the host does not wait for the pass's set-up before queueing it).  Every kernel is
+    // gated on the device scalars, so a wrong guess only costs time, never a different result.
+    const std::string hkey = hist_key(C, P, prm);
+    const std::vector<int> hist = C->iter_hist.count(hkey) ? C->iter_hist[hkey] : std::vector<int>();
+    std::vector<int> seen;
+    auto seen_pass = [&](const std::vector<SolveScalars> &inner) {
+        int mx = 0;
+        for (const SolveScalars &v : inner) mx = std::max(mx, (int)v.iter);
+        seen.push_back(mx);
+    };
     hipStream_t s = K.s;
     LevelHost &F0 = C->levels[0];
     const int norm = prm->norm == SHKZ_NORM_L2 ? 1 : 0;
@@ -1087,10 +1139,12 @@ static int solve_phase(shkz_ctx *C, Launcher &K, Problem &P, const shkz_params *
              (const double *)P.b64, (const double *)P.x64, P.r64, P.red);
         finalize(K, P, F0.desc, P.S_out, P.ctl, FIN_DEFECT_FIRST, residual);
         BatchCtl hc{};
-        SHKZ_TRY(read_scalars(C, s, P.S_out, ngrids, hs, P.ctl, &hc));
-        if (!hc.all_done)
+        if (hist.empty() || hist[0] == 0) SHKZ_TRY(read_scalars(C, s, P.S_out, ngrids, hs, P.ctl, &hc));
+        if (!hc.all_done) {
             SHKZ_TRY(krylov<double>(C, K, P, P.x64, P.r64, (double *)P.kp, (double *)P.kq, (double *)P.kz,
-                                    (double *)P.ktmp, P.S_out, P.ctl, multigrid, hs));
+                                    (double *)P.ktmp, P.S_out, P.ctl, multigrid, hs, hist.empty() ? 4 : hist[0]));
+            seen_pass(hs);
+        }
         for (int g = 0; g < ngrids; ++g) {
             total_iters[g] = hs[g].iter;
             outer_passes[g] = F0.gl[g].brick_end > F0.gl[g].brick_start ? 1 : 0;
@@ -1114,17 +1168,37 @@ static int solve_phase(shkz_ctx *C, Launcher &K, Problem &P, const shkz_params *
                  (float *)P.kr, (float *)P.kx, P.red);
             finalize(K, P, F0.desc, P.S_in, P.ctl + 1, FIN_INNER_INIT, eta, 0, P.S_out);
             BatchCtl hc{};
-            SHKZ_TRY(read_scalars(C, s, P.S_out, ngrids, hs, P.ctl + 1, &hc));
-            if (outer > 0)
-                for (int g = 0; g < ngrids; ++g) total_iters[g] = hs[g].iter;
-            if (hc.all_done) {
-                final_measured = true;
-                break;
+            // the history expects an inner solve in this pass: queue it at once; the outer scalars
+            // come back with the Krylov solve's polls (a grid found done runs no-op kernels and
+            // adds a zero correction)
+            const bool expect_inner = outer < (int)hist.size() && hist[outer] > 0;
+            if (!expect_inner) {
+                SHKZ_TRY(read_scalars(C, s, P.S_out, ngrids, hs, P.ctl + 1, &hc));
+                if (outer > 0)
+                    for (int g = 0; g < ngrids; ++g) total_iters[g] = hs[g].iter;
+                if (hc.all_done) {
+                    final_measured = true;
+                    break;
+                }
             }
+            SHKZ_TRY(krylov<float>(C, K, P, (float *)P.kx, (float *)P.kr, (float *)P.kp, (float *)P.kq, (float *)P.kz,
+                                   (float *)P.ktmp, P.S_in, P.ctl + 1, true, hi, expect_inner ? hist[outer] : 4,
+                                   expect_inner ? P.S_out : nullptr, expect_inner ? &hs : nullptr,
+                                   C->keep_direction && outer > 0));
+            if (expect_inner) {
+                if (outer > 0)
+                    for (int g = 0; g < ngrids; ++g) total_iters[g] = hs[g].iter;
+                bool all = true;
+                for (int g = 0; g < ngrids; ++g) all = all && hs[g].done;
+                if (all) {
+                    // every grid was done at this pass's defect: what the poll would have found
+                    final_measured = true;
+                    break;
+                }
+            }
+            seen_pass(hi);
             for (int g = 0; g < ngrids; ++g)
                 if (!hs[g].done) outer_passes[g] = outer + 1;
-            SHKZ_TRY(krylov<float>(C, K, P, (float *)P.kx, (float *)P.kr, (float *)P.kp, (float *)P.kq, (float *)P.kz,
-                                   (float *)P.ktmp, P.S_in, P.ctl + 1, true, hi));
             if (C->trace)
                 for (int g = 0; g < ngrids; ++g)
                     if (!hs[g].done)
@@ -1139,6 +1213,7 @@ static int solve_phase(shkz_ctx *C, Launcher &K, Problem &P, const shkz_params *
         for (int g = 0; g < ngrids; ++g)
             if (hs[g].status) status[g] = SHKZ_ERR_NUMERIC;
     }
+    if (F0.nbricks > 0) C->iter_hist[hkey] = seen;
     // true FP64 residual of the returned pressure, per grid.  When the defect correction ended
     // on a pass whose defect found every grid done, that defect was taken at the returned
     // pressure (a grid done earlier only gained zero corrections since), so its norms are kept.
@@ -1915,6 +1990,7 @@ static int project_stages(shkz_ctx *C, const shkz_grid *grids, int ngrids, const
         res[g].iterations = total_iters[g];
         res[g].outer_iterations = outer_passes[g];
         res[g].status = status[g];
+        res[g].batch_size = ngrids;
     }
     if (C->timing) {
         SHKZ_HIP(hipStreamSynchronize(s));
@@ -2421,6 +2497,8 @@ int shkz_create(int device, uint64_t hbm_budget_bytes, shkz_ctx **out) {
     C->trace = tr && tr[0] == '1';
     const char *hl = std::getenv("SHKZ_HOST_LINKS");
     C->host_links = hl && hl[0] == '1';
+    const char *kd = std::getenv("SHKZ_KEEP_DIRECTION");
+    C->keep_direction = kd && kd[0] == '1';
     *out = C;
     return SHKZ_OK;
 }
@@ -2436,6 +2514,7 @@ void shkz_destroy(shkz_ctx *C) {
     for (hipEvent_t e : C->event_pool) (void)hipEventDestroy(e);
     for (auto &g : C->graphs)
         if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    C->graphs.clear();
     if (C->ev_in) (void)hipEventDestroy(C->ev_in);
     if (C->ev_out) (void)hipEventDestroy(C->ev_out);
     if (C->work) (void)hipStreamDestroy(C->work);
@@ -2443,7 +2522,7 @@ void shkz_destroy(shkz_ctx *C) {
     C->own_scenes.clear();
     if (C->host_scalars) (void)hipHostFree(C->host_scalars);
     if (C->host_ctl) (void)hipHostFree(C->host_ctl);
-    for (shkz_ctx::Pinned *b : {&C->pin_flags, &C->pin_meta, &C->pin_counts})
+    for (shkz_ctx::Pinned *b : {&C->pin_flags, &C->pin_meta0, &C->pin_meta, &C->pin_links, &C->pin_counts})
         if (b->p) (void)hipHostFree(b->p);
     delete C;
 }
