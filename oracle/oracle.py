@@ -445,3 +445,43 @@ def advect_scalar(w, h, d, vel, active, q, dt, active_q=None, fluid=None, dx=Non
     lib().oracle_advect_scalar(ctypes.byref(f), q.ctypes.data, _p(active_q), None if fl is None else fl.ctypes.data,
                                int(fl is not None and fl.dtype == np.float64), float(dt), int(bool(maccormack)),
                                int(trim), float(background))
+
+
+def permuted(sys, perm):
+    """The same system with its unknowns renumbered (unknown perm[i] becomes i): symmetric
+    permutation of the CSR matrix and of b."""
+    import scipy.sparse as sp
+    n = sys["n"]
+    A = sp.csr_matrix((sys["val"], sys["col"], sys["rowptr"]), shape=(n, n))
+    P = A[perm][:, perm].tocsr()
+    P.sort_indices()
+    return dict(n=n, rowptr=P.indptr.astype(np.int64), col=P.indices.astype(np.int64),
+                val=P.data.astype(np.float64), b=np.ascontiguousarray(sys["b"][perm], np.float64))
+
+
+def cg_count_spread(sys, residual=1e-8, n_orders=3, seed=0):
+    """Iteration counts of the reference iteration (plain CG, infinity-norm test) on the same
+    system under other unknown orders (reversed, then seeded random permutations).  Renumbering
+    changes nothing but the summation order of the dot products and norms, so the spread of these
+    counts is the sensitivity of the count to rounding alone."""
+    n = sys["n"]
+    rng = np.random.default_rng(seed)
+    counts = []
+    for k in range(n_orders):
+        perm = np.arange(n)[::-1].copy() if k == 0 else rng.permutation(n)
+        _, it, _, ok = pcg(permuted(sys, perm), residual=residual, mic_mode=MIC_NONE)
+        assert ok
+        counts.append(it)
+    return counts
+
+
+def cg_count_envelope(sys, residual=1e-8, n_orders=5):
+    """(lo, hi): the range of the reference iteration's counts over this unknown order and
+    n_orders others, widened on each side by half that range, and by at least 5% of this order's
+    count (and 5).  A count outside it differs by more than rounding accounts for."""
+    _, it, _, ok = pcg(sys, residual=residual, mic_mode=MIC_NONE)
+    assert ok
+    counts = [it] + cg_count_spread(sys, residual, n_orders)
+    lo, hi = min(counts), max(counts)
+    pad = max(5, int(0.05 * it), (hi - lo + 1) // 2)
+    return lo - pad, hi + pad

@@ -21,6 +21,13 @@
 #include "shkz_device.hpp"
 #include "shkz_geometry.hpp"
 
+// No floating-point contraction in any solver kernel: every a*b+c rounds twice, exactly as
+// written.  The same sweep, residual and prolongation are computed by different kernels depending
+// on a level's size (one launch per sweep, or several sweeps in LDS), and the size of a level
+// depends on the batch a grid is in; with the operations fixed by the source, every form gives the
+// same bits, so a grid's result does not depend on its batch.
+#pragma clang fp contract(off)
+
 namespace shkz {
 
 // ------------------------------------------------------------------------------ helpers
@@ -1310,6 +1317,104 @@ __global__ __launch_bounds__(ROW4_THREADS) void k_cg_update4(LevelDesc L, float 
     *reinterpret_cast<float4 *>(r + c) = make_float4(rv[0], rv[1], rv[2], rv[3]);
     row4_reduce2<true>(mx, sq, lds);
     store_partials(R, br, mx, sq);
+}
+
+// k_cg_update4 fused with the first pre-smoothing sweep of the next V-cycle on the finest level
+// (k_smooth_first4<1>): z = omega * idiag * r on the updated residual, on the bricks that sweep
+// covers (not those of grids ending on this level), with the same arithmetic.  A grid that this
+// update finds converged gets a z it never uses.
+__global__ __launch_bounds__(ROW4_THREADS) void k_cg_update4z(LevelDesc L, float *__restrict__ x,
+                                                              float *__restrict__ r, const float *__restrict__ p,
+                                                              const float *__restrict__ qv, const SolveScalars *S,
+                                                              const BatchCtl *ctl, RedBuf R,
+                                                              const float *__restrict__ idiag, float *__restrict__ z,
+                                                              float omega) {
+    __shared__ double lds[4];
+    const int br = brick_or_idle(L);
+    if (br < 0) return;
+    const long c = (long)br * BRICK_CELLS + 4 * threadIdx.x;
+    const BrickMeta m = L.meta[br];
+    const int g = L.ngrids > 1 ? m.grid : 0;
+    if (L.ngrids > 1 && (ctl->all_done || S[g].done)) return;
+    float xv[4], pv[4], rv[4], qq[4], id[4];
+    unpack4(ld4(x, c), xv);
+    unpack4(ld4(p, c), pv);
+    unpack4(ld4(r, c), rv);
+    unpack4(ld4(qv, c), qq);
+    if (m.flags & BM_UNIFORM) {
+        const float u = uni_load<float>(L, br).id;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) id[j] = u;
+    } else {
+        unpack4(ld4(idiag, c), id);
+    }
+    const SolveScalars &Sg = S[g];
+    if (ctl->all_done || Sg.done) return;
+    const float alpha = (float)Sg.alpha;
+    double mx = 0.0, sq = 0.0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        xv[j] = xv[j] + alpha * pv[j];
+        rv[j] = rv[j] - alpha * qq[j];
+        mx = fmax(mx, fabs((double)rv[j]));
+        sq += (double)rv[j] * (double)rv[j];
+    }
+    *reinterpret_cast<float4 *>(x + c) = make_float4(xv[0], xv[1], xv[2], xv[3]);
+    *reinterpret_cast<float4 *>(r + c) = make_float4(rv[0], rv[1], rv[2], rv[3]);
+    if (!(m.flags & BM_COARSEST)) {
+        float out[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) out[j] = omega * id[j] * rv[j];
+        *reinterpret_cast<float4 *>(z + c) = make_float4(out[0], out[1], out[2], out[3]);
+    }
+    row4_reduce2<true>(mx, sq, lds);
+    store_partials(R, br, mx, sq);
+}
+
+// k_update_p4 (p = z + beta p) fused into k_apply_dot4: the new direction of the lane's cells and
+// of their neighbours is formed on the fly from z and the previous direction (same arithmetic),
+// written to pnew (a different buffer than pold: other workgroups still read pold), and q = A pnew
+// with the pnew.q partials.
+__global__ __launch_bounds__(ROW4_THREADS) void k_apply_dot4p(LevelDesc L, OpDesc<float> A,
+                                                              const float *__restrict__ z,
+                                                              const float *__restrict__ pold,
+                                                              float *__restrict__ pnew, float *__restrict__ qv,
+                                                              const SolveScalars *S, const BatchCtl *ctl, RedBuf R) {
+    __shared__ double lds[4];
+    const int br = brick_or_idle(L);
+    if (br < 0) return;
+    const BrickMeta m = L.meta[br];
+    const int g = L.ngrids > 1 ? m.grid : 0;
+    if (L.ngrids > 1 && (ctl->all_done || S[g].done)) return;
+    const Row4Nbrs N = row4_nbrs(m, br, threadIdx.x);
+    const Row4Vals<float> zv = row4_vals(z, N);
+    const Row4Vals<float> po = row4_vals(pold, N);
+    if (ctl->all_done) return;
+    const float beta = (float)S[g].beta;
+    Row4Vals<float> v;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        v.c[j] = zv.c[j] + beta * po.c[j];
+        v.xm[j] = zv.xm[j] + beta * po.xm[j];
+        v.xp[j] = zv.xp[j] + beta * po.xp[j];
+        v.ym[j] = zv.ym[j] + beta * po.ym[j];
+        v.yp[j] = zv.yp[j] + beta * po.yp[j];
+        v.zm[j] = zv.zm[j] + beta * po.zm[j];
+        v.zp[j] = zv.zp[j] + beta * po.zp[j];
+    }
+    float av[4], out[4];
+    bool unk[4];
+    apply_row4(L, A, br, m, N, v, av, unk);
+    double part = 0.0, unused = 0.0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        out[j] = unk[j] ? av[j] : 0.0f;
+        part += unk[j] ? (double)v.c[j] * (double)av[j] : 0.0;
+    }
+    *reinterpret_cast<float4 *>(pnew + N.c) = make_float4(v.c[0], v.c[1], v.c[2], v.c[3]);
+    *reinterpret_cast<float4 *>(qv + N.c) = make_float4(out[0], out[1], out[2], out[3]);
+    row4_reduce2<false>(part, unused, lds);
+    store_partials(R, br, part, 0.0);
 }
 
 // p = z + beta p (first: p = z), k_update_p on rows.

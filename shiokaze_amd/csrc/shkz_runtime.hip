@@ -442,6 +442,14 @@ struct shkz_ctx {
     // mixed precision: each defect-correction pass after the first continues the previous pass's
     // search direction (p = z + beta p with the replaced residual) instead of restarting CG
     bool keep_direction = false;
+    // FP32 Krylov iterations with the direction update fused into the operator apply and the
+    // first finest-level sweep fused into the residual update (SHKZ_NO_FUSE=1: separate kernels;
+    // the results are the same)
+    bool fuse_krylov = true;
+    // coarse levels of at most this many stored bricks smooth with multi-sweep launches unless the
+    // autotuner picks another limit (SHKZ_MULTI_MAX_BRICKS overrides the default; the results do
+    // not depend on it)
+    int multi_max_bricks = 1024;
     std::string device_key;  // device identity of the tuning cache
     // replayable PCG iteration graphs, keyed by everything their kernel arguments depend on; an
     // exec is destroyed only to make room (least recently used, after the stream has drained) or
@@ -547,6 +555,7 @@ struct Problem {
     double *b64 = nullptr, *x64 = nullptr, *r64 = nullptr;
     // Krylov vectors of the level-0 solve (element type of the solve)
     void *kr = nullptr, *kp = nullptr, *kq = nullptr, *kx = nullptr, *kz = nullptr, *ktmp = nullptr;
+    void *kp2 = nullptr;  // FP32: the other direction buffer of the fused iteration (ping-pong)
     // per-grid scalars: outer (FP64 defect / FP64 solve), inner (FP32 solve), final check
     SolveScalars *S_out = nullptr, *S_in = nullptr, *S_fin = nullptr;
     BatchCtl *ctl = nullptr;  // [0] outer / FP64 solve, [1] inner solve
@@ -692,7 +701,8 @@ static void jacobi_sweep(shkz_ctx *C, Launcher &K, int l, bool dot, const LevelD
 // last post-smoothing sweep on the finest level (or the coarse solve of a one-brick grid) also
 // stores r.z partials in P.red.
 template <class T>
-static T *vcycle(shkz_ctx *C, Launcher &K, Problem &P, const T *rhs0, T *z0, T *tmp0, Gate Q, bool dot_partials) {
+static T *vcycle(shkz_ctx *C, Launcher &K, Problem &P, const T *rhs0, T *z0, T *tmp0, Gate Q, bool dot_partials,
+                 bool first_done = false) {
     const int nl = P.nlev;
     std::vector<T *> zcur(nl), zoth(nl);
     const T om = (T)P.omega;
@@ -743,7 +753,9 @@ static T *vcycle(shkz_ctx *C, Launcher &K, Problem &P, const T *rhs0, T *z0, T *
         } else if (pre <= 1 || f) {
             // finest level: one sweep from zero, then plain sweeps (cheaper there than the fused
             // two-sweep kernel, whose on-the-fly neighbour values cost 12 extra loads per cell)
-            if constexpr (sizeof(T) == 4)
+            if (f && first_done) {
+                // z = omega * idiag * rhs was written by the residual update (k_cg_update4z)
+            } else if constexpr (sizeof(T) == 4)
                 K.go(f ? "smooth_first_L0" : "smooth_first_Lc", l, 3 * es, -usave(C, l, es),
                      row4_launch(L.nbricks, f).grid, row4_launch(L.nbricks, f).block,
                      f ? k_smooth_first4<1> : k_smooth_first4<0>, L.desc, id, rhs, z, om, Q);
@@ -935,7 +947,48 @@ static int krylov(shkz_ctx *C, Launcher &K, Problem &P, T *x, T *r, T *p, T *q, 
     finalize(K, P, L0.desc, S, ctl, keep_dir ? FIN_RZ : FIN_RZ_FIRST);
     update_p(keep_dir ? 3 * es : 2 * es, zp, keep_dir ? 0 : 1);
 
+    // FP32 preconditioned form with fused kernels: the loop is rotated so that each iteration
+    // ends with the operator apply of the next direction, formed in the same kernel
+    // (k_apply_dot4p), and starts with the residual update that also performs the V-cycle's first
+    // sweep (k_cg_update4z); the direction alternates between two buffers, so one captured graph
+    // holds two iterations.  Same kernels' arithmetic, same results as the separate kernels.
+    const bool fused = sizeof(T) == 4 && precond && !keep_dir && C->fuse_krylov && P.kp2;
+    T *const p2 = (T *)P.kp2;
+    auto half = [&](T *pc, T *pn) {
+        if constexpr (sizeof(T) == 4) {
+            const bool zfirst = P.nlev > 1;
+            const LevelHost &F0 = C->levels[0];
+            if (zfirst)
+                K.go("cg_update_z", 0, 8 * es, -usave(C, 0, es), g, dim3(ROW4_THREADS), k_cg_update4z, L0.desc, x, r,
+                     (const T *)pc, (const T *)q, (const SolveScalars *)S, (const BatchCtl *)ctl, P.red,
+                     (const T *)F0.idiag, z, (T)P.omega);
+            else
+                K.go("cg_update", 0, 6 * es, 0, g, dim3(ROW4_THREADS), k_cg_update4, L0.desc, x, r, (const T *)pc,
+                     (const T *)q, (const SolveScalars *)S, (const BatchCtl *)ctl, P.red);
+            finalize(K, P, L0.desc, S, ctl, FIN_UPDATE);
+            const T *zz = vcycle<T>(C, K, P, r, z, tmp, Q, fused_dot, zfirst);
+            if (!fused_dot)
+                K.go("dot_rz", 0, 2 * es, 0, g, b, k_dot_rz<T>, L0.desc, (const T *)r, zz, (const SolveScalars *)S,
+                     (const BatchCtl *)ctl, P.red);
+            finalize(K, P, L0.desc, S, ctl, FIN_RZ);
+            K.go("apply_dot_p", 0, 8 * es, -usave(C, 0, 4 * es), g, dim3(ROW4_THREADS), k_apply_dot4p, L0.desc, A,
+                 zz, (const T *)pc, pn, q, (const SolveScalars *)S, (const BatchCtl *)ctl, P.red);
+            finalize(K, P, L0.desc, S, ctl, FIN_PQ);
+        }
+    };
+    if (fused) {
+        // the first direction's operator apply (the loop starts with the residual update)
+        if constexpr (sizeof(T) == 4)
+            K.go("apply_dot", 0, 6 * es, -usave(C, 0, 4 * es), g, dim3(ROW4_THREADS), k_apply_dot4, L0.desc, A,
+                 (const T *)p, q, (const SolveScalars *)S, (const BatchCtl *)ctl, P.red);
+        finalize(K, P, L0.desc, S, ctl, FIN_PQ);
+    }
     auto iteration = [&]() {
+        if (fused) {
+            half(p, p2);
+            half(p2, p);
+            return;
+        }
         if constexpr (sizeof(T) == 4)
             K.go("apply_dot", 0, 6 * es, -usave(C, 0, 4 * es), g, dim3(ROW4_THREADS), k_apply_dot4, L0.desc, A,
                  (const T *)p, q, (const SolveScalars *)S, (const BatchCtl *)ctl, P.red);
@@ -963,8 +1016,9 @@ static int krylov(shkz_ctx *C, Launcher &K, Problem &P, T *x, T *r, T *p, T *q, 
     };
     hipGraphExec_t exec = nullptr;
     if (C->use_graphs && !C->timing) {
-        const void *vecs[] = {x, r, p, q, z, tmp, S, ctl};
-        const std::string key = std::string(sizeof(T) == 8 ? "d" : "f") + graph_key<T>(C, P, vecs, 8, precond);
+        const void *vecs[] = {x, r, p, q, z, tmp, S, ctl, fused ? (const void *)p2 : nullptr};
+        const std::string key = std::string(sizeof(T) == 8 ? "d" : "f") + (fused ? "F" : "") +
+                                graph_key<T>(C, P, vecs, 9, precond);
         for (auto &g : C->graphs)
             if (g.exec && g.key == key) {
                 exec = g.exec;
@@ -996,10 +1050,11 @@ static int krylov(shkz_ctx *C, Launcher &K, Problem &P, T *x, T *r, T *p, T *q, 
             exec = made;
         }
     }
+    const int per_replay = fused ? 2 : 1;  // iterations per replay of `iteration`
     int chunk = std::max(1, std::min(first_chunk, 64));
     BatchCtl hc{};
     for (;;) {
-        for (int it = 0; it < chunk; ++it) {
+        for (int it = 0; it < chunk; it += per_replay) {
             if (exec) SHKZ_HIP(hipGraphLaunch(exec, K.s));
             else iteration();
         }
@@ -1461,6 +1516,7 @@ static int project_stages(shkz_ctx *C, const shkz_grid *grids, int ngrids, const
     P.coarse_sweeps = prm->mg_coarse_sweeps > 0 ? prm->mg_coarse_sweeps : 48;
     P.omega = prm->mg_omega > 0 ? prm->mg_omega : 0.9;
     P.eta = prm->inner_residual > 0 ? prm->inner_residual : 1e-3;
+    P.multi_max_bricks = C->multi_max_bricks;
     const bool multigrid = prm->solver == SHKZ_SOLVER_MGPCG;
 
     // ---------------------------------------------------------------- per grid: fractions + labelling
@@ -1826,6 +1882,7 @@ static int project_stages(shkz_ctx *C, const shkz_grid *grids, int ngrids, const
     if (P.mixed) {
         SHKZ_TRY(AR.get("kr32", n0 * 4, &P.kr));
         SHKZ_TRY(AR.get("kx32", n0 * 4, &P.kx));
+        SHKZ_TRY(AR.get("kp2", n0 * 4, &P.kp2));
     }
 
 
@@ -2499,6 +2556,10 @@ int shkz_create(int device, uint64_t hbm_budget_bytes, shkz_ctx **out) {
     C->host_links = hl && hl[0] == '1';
     const char *kd = std::getenv("SHKZ_KEEP_DIRECTION");
     C->keep_direction = kd && kd[0] == '1';
+    const char *nf = std::getenv("SHKZ_NO_FUSE");
+    C->fuse_krylov = !(nf && nf[0] == '1');
+    const char *mb = std::getenv("SHKZ_MULTI_MAX_BRICKS");
+    if (mb && mb[0]) C->multi_max_bricks = std::atoi(mb);
     *out = C;
     return SHKZ_OK;
 }

@@ -102,7 +102,8 @@ def test_2d_reference_cg_iteration(ctx):
     x_o, it_o, rr_o, ok = O.pcg(A, residual=1e-8, mic_mode=O.MIC_NONE)
     r, P, P64 = ctx.project2d(sc.copy(), S.default_params(residual=1e-8, solver=S.SOLVER_CG,
                                                           precision=S.PREC_FP64), pressure64=True)
-    assert r.converged and abs(r.iterations - it_o) <= max(5, 0.05 * it_o), (r.iterations, it_o)
+    lo, hi = O.cg_count_envelope(A, residual=1e-8)  # the oracle's counts under other unknown orders
+    assert r.converged and lo <= r.iterations <= hi, (r.iterations, it_o, lo, hi)
 
 
 def test_2d_surface_tension_volume_correction_and_full_oracle(ctx):

@@ -38,7 +38,8 @@ def test_reference_iteration_on_pressure_systems(ls, idx):
     x, r = ls.solve(A["rowptr"], A["col"], A["val"], A["b"], residual=1e-8)
     assert r.status == 0 and r.converged and ok
     # CG iterates depend on dot-product summation order (blocked here, serial in the oracle)
-    assert abs(r.iterations - it_o) <= max(5, 0.05 * it_o), (r.iterations, it_o)
+    lo, hi = O.cg_count_envelope(A, residual=1e-8)  # the oracle's counts under other unknown orders
+    assert lo <= r.iterations <= hi, (r.iterations, it_o, lo, hi)
     resid = A["b"] - O.spmv(A, x)
     assert np.abs(resid).max() <= 1e-8 * np.abs(A["b"]).max() * 1.0001
     assert r.reresid == pytest.approx(np.abs(resid).max() / np.abs(A["b"]).max(), rel=1e-6)

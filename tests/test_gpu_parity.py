@@ -109,14 +109,16 @@ def test_operator_matches_oracle(ctx, name):
 def test_reference_cg_iteration_matches_oracle(ctx, name):
     """SHKZ_SOLVER_CG runs the reference's iteration (plain CG, inf-norm test).  In floating point
     CG iterates depend on dot-product summation order (parallel here, serial in the oracle), so
-    the counts agree to a few percent, not exactly."""
+    the counts agree only as closely as the oracle's own counts under other unknown orders do: the
+    GPU's count must lie in the envelope of those (O.cg_count_envelope)."""
     say(f"cg {name}")
     sc = SCENES[name]()
     A = O.assemble(sc)
     x_o, it_o, rr_o, ok = O.pcg(A, residual=1e-8, mic_mode=O.MIC_NONE)
     r, P, P64, work = run_gpu(ctx, sc, solver=S.SOLVER_CG, precision=S.PREC_FP64, residual=1e-8)
     assert r.converged and r.status == 0
-    assert abs(r.iterations - it_o) <= max(5, 0.05 * it_o), (r.iterations, it_o)
+    lo, hi = O.cg_count_envelope(A, residual=1e-8)
+    assert lo <= r.iterations <= hi, (r.iterations, it_o, lo, hi)
     x_g = O.gather(A, P64)
     assert rel_l2(x_g, x_o) < 1e-6
 

@@ -9,7 +9,7 @@ scenes too); pressure equal to the FP64 oracle's up to one constant per connecte
 (rel-L2 1e-5 after removing each component's mean); the FP64 residual of the returned pressure
 meets the requested tolerance; velocities bit-identical to the oracle's update applied to the
 GPU pressure; the divergence gate of test_gpu_parity.py; the reference CG iteration's count
-within 5% of the oracle's."""
+within the envelope of the oracle's counts under other unknown orders (rounding alone)."""
 import sys
 
 import numpy as np
@@ -104,7 +104,8 @@ def test_smoke_reference_cg_iteration(ctx):
     assert ok
     r, P, P64 = ctx.project(sc.copy(), S.default_params(residual=1e-8, solver=S.SOLVER_CG, precision=S.PREC_FP64),
                             pressure64=True)
-    assert r.converged and abs(r.iterations - it_o) <= max(5, 0.05 * it_o), (r.iterations, it_o)
+    lo, hi = O.cg_count_envelope(A, residual=1e-8)  # the oracle's counts under other unknown orders
+    assert r.converged and lo <= r.iterations <= hi, (r.iterations, it_o, lo, hi)
 
 
 @pytest.mark.parametrize("prec", [S.PREC_FP64, S.PREC_MIXED])

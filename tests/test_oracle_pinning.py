@@ -282,3 +282,23 @@ def test_accuracy_known_answer_second_order():
     order = math.log2(e1 / e2)
     assert e2 < 2e-3
     assert order > 1.6, (e1, e2, order)
+
+
+def test_permuted_system_is_the_same_system():
+    """O.permuted renumbers the unknowns only: the permuted system's solution is the permuted
+    solution, and its matrix-vector product is the permuted product (so the spread of CG counts
+    over orders measures rounding alone)."""
+    A = O.assemble(S.make_scene(S.SCENE_DROP_SOLIDS, 16, seed=1))
+    n = A["n"]
+    rng = np.random.default_rng(3)
+    perm = rng.permutation(n)
+    Ap = O.permuted(A, perm)
+    v = rng.standard_normal(n)
+    assert np.allclose(O.spmv(Ap, v[perm]), O.spmv(A, v)[perm], rtol=0, atol=1e-12)
+    x, _, _, ok = O.pcg(A, residual=1e-12, mic_mode=O.MIC_NONE)
+    xp, _, _, okp = O.pcg(Ap, residual=1e-12, mic_mode=O.MIC_NONE)
+    assert ok and okp
+    assert np.linalg.norm(xp - x[perm]) <= 1e-9 * np.linalg.norm(x)
+    lo, hi = O.cg_count_envelope(A, residual=1e-8, n_orders=2)
+    _, it, _, _ = O.pcg(A, residual=1e-8, mic_mode=O.MIC_NONE)
+    assert lo < it < hi
