@@ -354,6 +354,24 @@ int shkz_linsolve_csr(shkz_linsolver *solver, int64_t n, const int64_t *rowptr, 
                       const double *val, const double *b, double *x, const shkz_linsolve_params *params,
                       shkz_linsolve_result *result);
 
+/* Allocation-free variant of shkz_linsolve_csr.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (tight serving loops, harnesses that forbid
+ * hipMalloc).  The system is given in device memory: rowptr (n+1) and col (nnz) as 32-bit
+ * indices, val (nnz), b (n); x (n, device) receives the solution.  The indices are trusted (they
+ * are not read back to be checked).  Same iteration, kernels and launch shapes as
+ * shkz_linsolve_csr, so the same result, launched directly on `stream` (no graph).
+ * The working vectors live in one fixed device array inside the library, sized for at most
+ * SHKZ_LS_NOALLOC_MAX_ROWS rows (about 10.6 MB; each GPU the library runs on holds its own copy
+ * until the process ends); larger systems fail with SHKZ_ERR_INVALID.  No call allocates or frees
+ * device memory.  Every call on one GPU shares that GPU's array: calls that overlap on one GPU
+ * (from any solver object or thread) race and must be serialised by the caller.  Each call
+ * initialises every part of the array it reads, so nothing carries over between calls.
+ * Returns when the solve is complete (the scalars are read back between chunks of iterations). */
+#define SHKZ_LS_NOALLOC_MAX_ROWS (1 << 18)
+int shkz_linsolve_csr_device_noalloc(shkz_linsolver *solver, int32_t n, const int32_t *rowptr, const int32_t *col,
+                                     const double *val, const double *b, double *x,
+                                     const shkz_linsolve_params *params, shkz_linsolve_result *result, void *stream);
+
 /* ------------------------------------------------------- server (continuous batching)
  * No counterpart in the reference (its driver projects one grid per time step on one thread,
  * ui.cpp:703-710).  A server owns `executors` workspaces, each with its own HIP stream; every

@@ -165,7 +165,7 @@ EXPORTS = [
     "shkz_scene_fill2d", "shkz_linsolve_default_params", "shkz_linsolver_create", "shkz_linsolver_destroy",
     "shkz_linsolve_csr", "shkz_extrapolate_velocity", "shkz_extrapolate_velocity_device",
     "shkz_advect_default_params", "shkz_advect_velocity", "shkz_advect_velocity_device", "shkz_advect_scalar",
-    "shkz_advect_scalar_device",
+    "shkz_advect_scalar_device", "shkz_linsolve_csr_device_noalloc",
 ]
 
 _lib = None
@@ -223,6 +223,8 @@ def lib():
     L.shkz_linsolver_destroy.argtypes = [_vp]
     L.shkz_linsolve_csr.argtypes = [_vp, ctypes.c_int64, _vp, _vp, _vp, _vp, _vp, ctypes.POINTER(LinsolveParams),
                                     ctypes.POINTER(LinsolveResult)]
+    L.shkz_linsolve_csr_device_noalloc.argtypes = [_vp, ctypes.c_int32, _vp, _vp, _vp, _vp, _vp,
+                                                   ctypes.POINTER(LinsolveParams), ctypes.POINTER(LinsolveResult), _vp]
     L.shkz_extrapolate_velocity.argtypes = [_vp, ctypes.POINTER(VelocityField), ctypes.c_int32, ctypes.c_int32]
     L.shkz_extrapolate_velocity_device.argtypes = [_vp, ctypes.POINTER(VelocityField), ctypes.c_int32, ctypes.c_int32,
                                                    _vp]

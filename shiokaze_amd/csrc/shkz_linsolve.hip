@@ -272,15 +272,24 @@ int ls_get(shkz_linsolver::Buf &b, size_t bytes, void **out) {
         if (rc_ != SHKZ_OK) return rc_;                                                                                \
     } while (0)
 
-// The solve on device arrays already in the solver's buffers (b filled); x receives the result.
-int ls_solve_device(shkz_linsolver *L, int n, const shkz_linsolve_params *prm, shkz_linsolve_result *res) {
-    hipStream_t s = L->stream;
-    const int *rp = (const int *)L->rowptr.p, *cl = (const int *)L->col.p;
-    const double *vl = (const double *)L->val.p;
-    double *b = (double *)L->b.p, *x = (double *)L->x.p, *r = (double *)L->r.p, *z = (double *)L->z.p,
-           *p = (double *)L->p.p, *q = (double *)L->q.p, *id = (double *)L->idiag.p;
-    double *part = (double *)L->partials.p;
-    LsScalars *S = (LsScalars *)L->scalars.p;
+// Device arrays of one solve.
+struct LsArrays {
+    const int *rowptr, *col;
+    const double *val, *b;
+    double *x, *r, *z, *p, *q, *idiag, *partials;
+    LsScalars *S;
+};
+
+// The solve on device arrays (b filled); x receives the result.  With `graph`, one iteration is
+// captured once into a hipGraph kept by the solver and replayed; without, the same kernels are
+// launched directly (same launch shapes, so the same results).  Scalars come back through the
+// solver's pinned buffer.
+int ls_solve_device(shkz_linsolver *L, hipStream_t s, const LsArrays &D, int n, const shkz_linsolve_params *prm,
+                    shkz_linsolve_result *res, bool graph) {
+    const int *rp = D.rowptr, *cl = D.col;
+    const double *vl = D.val, *b = D.b;
+    double *x = D.x, *r = D.r, *z = D.z, *p = D.p, *q = D.q, *id = D.idiag, *part = D.partials;
+    LsScalars *S = D.S;
     const bool jacobi = prm->preconditioner == SHKZ_LS_JACOBI;
     const double *M = jacobi ? id : nullptr;
     double *zz = jacobi ? z : r;  // without a preconditioner z is r itself
@@ -297,24 +306,26 @@ int ls_solve_device(shkz_linsolver *L, int n, const shkz_linsolve_params *prm, s
         k_ls_finalize<<<1, 1024, 0, s>>>(part, (int)nb_v, S, LS_FIN_UPDATE, 0.0, 0);
         k_ls_direction<<<nb_v, LS_BLOCK, 0, s>>>(n, zz, p, S, 0);
     };
-    // one captured iteration, reused while the arrays and sizes stay the same
-    std::vector<const void *> key = {rp, cl, vl, x, r, z, p, q, id, part, S};
-    if (!L->exec || L->exec_key != key || L->exec_n != n || L->exec_prec != (int)jacobi) {
-        if (L->exec) (void)hipGraphExecDestroy(L->exec);
-        L->exec = nullptr;
-        hipGraph_t graph = nullptr;
-        LS_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-        iteration();
-        LS_HIP(hipStreamEndCapture(s, &graph));
-        const hipError_t e = hipGraphInstantiate(&L->exec, graph, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(graph);
-        if (e != hipSuccess) {
+    if (graph) {
+        // one captured iteration, reused while the arrays and sizes stay the same
+        std::vector<const void *> key = {rp, cl, vl, x, r, z, p, q, id, part, S};
+        if (!L->exec || L->exec_key != key || L->exec_n != n || L->exec_prec != (int)jacobi) {
+            if (L->exec) (void)hipGraphExecDestroy(L->exec);
             L->exec = nullptr;
-            return shkz_fail(SHKZ_ERR_DEVICE, "hipGraphInstantiate failed: %s", hipGetErrorString(e));
+            hipGraph_t g = nullptr;
+            LS_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            iteration();
+            LS_HIP(hipStreamEndCapture(s, &g));
+            const hipError_t e = hipGraphInstantiate(&L->exec, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (e != hipSuccess) {
+                L->exec = nullptr;
+                return shkz_fail(SHKZ_ERR_DEVICE, "hipGraphInstantiate failed: %s", hipGetErrorString(e));
+            }
+            L->exec_key = key;
+            L->exec_n = n;
+            L->exec_prec = (int)jacobi;
         }
-        L->exec_key = key;
-        L->exec_n = n;
-        L->exec_prec = (int)jacobi;
     }
     int chunk = 8;
     for (;;) {
@@ -330,7 +341,10 @@ int ls_solve_device(shkz_linsolver *L, int n, const shkz_linsolve_params *prm, s
             if (rate > 0 && rate < 1) next = (int)std::ceil(std::log(h.tol / h.rinf) / std::log(rate));
         }
         chunk = std::max(1, std::min({next, h.maxit - h.iter, 256}));
-        for (int it = 0; it < chunk; ++it) LS_HIP(hipGraphLaunch(L->exec, s));
+        for (int it = 0; it < chunk; ++it) {
+            if (graph) LS_HIP(hipGraphLaunch(L->exec, s));
+            else iteration();
+        }
     }
     LS_HIP(hipGetLastError());
     const LsScalars h = *L->host_s;
@@ -339,6 +353,38 @@ int ls_solve_device(shkz_linsolver *L, int n, const shkz_linsolve_params *prm, s
     res->reresid = h.r0 > 0 ? h.rinf / h.r0 : 0.0;
     res->status = h.status ? SHKZ_ERR_NUMERIC : SHKZ_OK;
     return res->status == SHKZ_OK ? SHKZ_OK : shkz_fail(SHKZ_ERR_NUMERIC, "non-finite value met in the linear solve");
+}
+
+// Scratch of the allocation-free entry point: a fixed device array (each GPU the library runs on
+// holds its own copy until the process ends).  Layout in doubles: r, z, p, q, idiag (each
+// SHKZ_LS_NOALLOC_MAX_ROWS), the block partials (two per SpMV block of the largest system), then
+// the scalar record.
+constexpr size_t NA_ROWS = SHKZ_LS_NOALLOC_MAX_ROWS;
+constexpr size_t NA_PARTIALS = 2 * ((NA_ROWS + LS_ROWS - 1) / LS_ROWS);
+constexpr size_t NA_SCALARS = (sizeof(LsScalars) + sizeof(double) - 1) / sizeof(double);
+constexpr size_t NA_DOUBLES = 5 * NA_ROWS + NA_PARTIALS + NA_SCALARS;
+static_assert(NA_ROWS % LS_ROWS == 0 && LS_BLOCK % LS_ROWS == 0, "partials cover both kernels' blocks");
+static __device__ __align__(256) double scratch[NA_DOUBLES];
+
+// Per-device address of `scratch` (the array's name is a host-side stand-in, not a device
+// address), looked up once per device with that device current.
+constexpr int NA_MAX_DEVICES = 64;
+std::mutex g_na_mu;
+double *g_na_ptr[NA_MAX_DEVICES] = {};
+
+int noalloc_scratch(int device, double **out) {
+    if (device < 0 || device >= NA_MAX_DEVICES) return shkz_fail(SHKZ_ERR_INVALID, "device %d out of range", device);
+    std::lock_guard<std::mutex> lock(g_na_mu);
+    if (!g_na_ptr[device]) {
+        void *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            return shkz_fail(SHKZ_ERR_DEVICE, "hipGetSymbolAddress(scratch) failed on device %d: %s", device,
+                             hipGetErrorString(e));
+        g_na_ptr[device] = (double *)ptr;
+    }
+    *out = g_na_ptr[device];
+    return SHKZ_OK;
 }
 
 }  // namespace
@@ -367,6 +413,14 @@ int shkz_linsolver_create(int device, shkz_linsolver **out) {
         if (L->stream) (void)hipStreamDestroy(L->stream);
         delete L;
         return shkz_fail(SHKZ_ERR_DEVICE, "stream or pinned buffer creation failed");
+    }
+    double *na = nullptr;
+    const int rc = noalloc_scratch(device, &na);
+    if (rc != SHKZ_OK) {
+        (void)hipStreamDestroy(L->stream);
+        (void)hipHostFree(L->host_s);
+        delete L;
+        return rc;
     }
     *out = L;
     return SHKZ_OK;
@@ -434,12 +488,47 @@ int shkz_linsolve_csr(shkz_linsolver *L, int64_t n, const int64_t *rowptr, const
     LS_HIP(hipMemcpyAsync(L->b.p, b, n * sizeof(double), hipMemcpyHostToDevice, L->stream));
     LS_HIP(hipStreamSynchronize(L->stream));  // the host staging vectors go out of scope
     const double t1 = now_ms();
-    const int rc = ls_solve_device(L, (int)n, prm, result);
+    const LsArrays D{(const int *)L->rowptr.p, (const int *)L->col.p, (const double *)L->val.p, (const double *)L->b.p,
+                     (double *)L->x.p, (double *)L->r.p, (double *)L->z.p, (double *)L->p.p, (double *)L->q.p,
+                     (double *)L->idiag.p, (double *)L->partials.p, (LsScalars *)L->scalars.p};
+    const int rc = ls_solve_device(L, L->stream, D, (int)n, prm, result, true);
     LS_HIP(hipMemcpyAsync(x, L->x.p, n * sizeof(double), hipMemcpyDeviceToHost, L->stream));
     LS_HIP(hipStreamSynchronize(L->stream));
     const double t2 = now_ms();
     result->t_upload = t1 - t0;
     result->t_solve = t2 - t1;
+    return rc;
+}
+
+int shkz_linsolve_csr_device_noalloc(shkz_linsolver *L, int32_t n, const int32_t *rowptr, const int32_t *col,
+                                     const double *val, const double *b, double *x,
+                                     const shkz_linsolve_params *params, shkz_linsolve_result *result, void *stream) {
+    if (!L || !result) return shkz_fail(SHKZ_ERR_INVALID, "NULL solver or result");
+    std::memset(result, 0, sizeof(*result));
+    if (n < 0 || (n > 0 && (!rowptr || !col || !val || !b || !x)))
+        return shkz_fail(SHKZ_ERR_INVALID, "NULL argument or negative size");
+    if ((size_t)n > NA_ROWS)
+        return shkz_fail(SHKZ_ERR_INVALID, "%d rows exceed the allocation-free solver's limit of %zu rows", (int)n,
+                         NA_ROWS);
+    shkz_linsolve_params def;
+    shkz_linsolve_default_params(&def);
+    const shkz_linsolve_params *prm = params ? params : &def;
+    if (prm->preconditioner != SHKZ_LS_NONE && prm->preconditioner != SHKZ_LS_JACOBI)
+        return shkz_fail(SHKZ_ERR_INVALID, "unknown preconditioner %d", prm->preconditioner);
+    std::lock_guard<std::mutex> lock(L->mu);
+    LS_HIP(hipSetDevice(L->device));
+    const double t0 = now_ms();
+    if (n == 0) {
+        result->converged = 1;
+        return SHKZ_OK;
+    }
+    double *base = nullptr;
+    LS_TRY(noalloc_scratch(L->device, &base));
+    const hipStream_t s = (hipStream_t)stream;
+    const LsArrays D{rowptr, col, val, b, x, base, base + NA_ROWS, base + 2 * NA_ROWS, base + 3 * NA_ROWS,
+                     base + 4 * NA_ROWS, base + 5 * NA_ROWS, (LsScalars *)(base + 5 * NA_ROWS + NA_PARTIALS)};
+    const int rc = ls_solve_device(L, s, D, (int)n, prm, result, false);
+    result->t_solve = now_ms() - t0;
     return rc;
 }
 
