@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define SHKZ_ABI_VERSION 4
+#define SHKZ_ABI_VERSION 5
 
 /* ---------------------------------------------------------------- enums */
 enum shkz_status {
@@ -42,7 +42,8 @@ enum shkz_status {
     SHKZ_ERR_NOMEM = -3,     /* device allocation failed */
     SHKZ_ERR_BUDGET = -4,    /* request exceeds the context's HBM budget */
     SHKZ_ERR_NOT_FOUND = -5, /* unknown scene or ticket */
-    SHKZ_ERR_NUMERIC = -6    /* NaN/Inf met in the solve (reference: pcg_solver.h:267-271) */
+    SHKZ_ERR_NUMERIC = -6,   /* NaN/Inf met in the solve (reference: pcg_solver.h:267-271) */
+    SHKZ_ERR_CANCELLED = -7  /* server: the request was cancelled before it started */
 };
 
 enum shkz_dtype { SHKZ_F32 = 0, SHKZ_F64 = 1 };
@@ -116,7 +117,8 @@ typedef struct shkz_params {
     int32_t mg_post_sweeps;
     int32_t mg_coarse_sweeps;
     double mg_omega;
-    double inner_residual;        /* mixed precision: relative reduction per inner FP32 solve */
+    double inner_residual;        /* mixed precision: relative reduction asked of each inner FP32
+                                     solve (0: the final tolerance, one pass) */
     int32_t mg_coarse_pre_sweeps; /* sweeps on the coarser levels (mg_pre/post_sweeps: finest) */
     int32_t mg_coarse_post_sweeps;
     int32_t mg_bottom_bricks;     /* unused (kept for layout): every grid's coarsest level is one
@@ -214,6 +216,13 @@ int shkz_release_scene(shkz_ctx *ctx, uint64_t scene_id);
 
 /* Device bytes currently held by the context. */
 uint64_t shkz_device_bytes(const shkz_ctx *ctx);
+
+/* Test hook: the context's device allocation after `after` more successful ones fails once, as a
+ * failed hipMalloc does (SHKZ_ERR_NOMEM from the call that needed it; the device buffers the
+ * context already holds, its scene state and the caller's arrays are left as they were, and later
+ * calls allocate normally); after < 0 clears it.  The reference has no recovery path here (an
+ * allocation failure ends the process, src/core/module.cpp:140-147); this exercises the library's. */
+int shkz_inject_alloc_failure(shkz_ctx *ctx, int32_t after);
 
 /* ------------------------------------------------------- after the projection
  * Velocity extrapolation and the solid constraint: macutility3::extrapolate_and_constrain_velocity
@@ -403,6 +412,7 @@ typedef struct shkz_server_stats {
     uint64_t max_in_flight_bytes;  /* largest estimated working set in flight */
     uint64_t resident_bytes;       /* scene state held now */
     uint64_t device_bytes;         /* workspaces + scene state held now */
+    uint64_t cancelled;            /* requests cancelled before they started */
 } shkz_server_stats;
 
 void shkz_server_default_config(shkz_server_config *cfg);
@@ -416,6 +426,12 @@ int shkz_submit(shkz_server *srv, const shkz_grid *grid, const shkz_params *para
 int shkz_poll(shkz_server *srv, uint64_t ticket, shkz_result *result);
 /* shkz_poll after waiting up to timeout_ms (< 0: no limit) for the request. */
 int shkz_wait(shkz_server *srv, uint64_t ticket, double timeout_ms, shkz_result *result);
+/* Cancels a queued request: SHKZ_OK when it was still queued (it then completes at once with
+ * SHKZ_ERR_CANCELLED, collected by shkz_poll / shkz_wait, and none of its outputs is written);
+ * SHKZ_PENDING when it has already started or completed (it runs to its end, result unchanged);
+ * SHKZ_ERR_NOT_FOUND for an unknown or collected ticket.  Later requests of the same scene no
+ * longer wait for it.  No counterpart in the reference (its driver has no request queue). */
+int shkz_cancel(shkz_server *srv, uint64_t ticket);
 /* Drops a scene's resident state; SHKZ_ERR_INVALID while a request of the scene is queued or running. */
 int shkz_server_release_scene(shkz_server *srv, uint64_t scene_id);
 int shkz_server_get_stats(shkz_server *srv, shkz_server_stats *stats);

@@ -19,7 +19,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "lib", "libshkz_mi355x.so")
 
-ABI_VERSION = 4  # the layout of the structs below
+ABI_VERSION = 5  # the layout of the structs below
 F32, F64 = 0, 1
 SOLVER_MGPCG, SOLVER_CG = 0, 1
 PREC_MIXED, PREC_FP64 = 0, 1
@@ -27,7 +27,9 @@ NORM_INF, NORM_L2 = 0, 1
 SCENE_DROP, SCENE_DROP_SOLIDS, SCENE_SPHERES_BOX, SCENE_RANDOM_DROP, SCENE_SMOKE = 0, 1, 2, 3, 4
 
 OK, ERR_INVALID, ERR_DEVICE, ERR_NOMEM, ERR_BUDGET, ERR_NOT_FOUND, ERR_NUMERIC = 0, -1, -2, -3, -4, -5, -6
-STATUS = {0: "OK", -1: "INVALID", -2: "DEVICE", -3: "NOMEM", -4: "BUDGET", -5: "NOT_FOUND", -6: "NUMERIC"}
+ERR_CANCELLED = -7
+STATUS = {0: "OK", -1: "INVALID", -2: "DEVICE", -3: "NOMEM", -4: "BUDGET", -5: "NOT_FOUND", -6: "NUMERIC",
+          -7: "CANCELLED"}
 
 _vp = ctypes.c_void_p
 
@@ -132,7 +134,7 @@ class ServerStats(ctypes.Structure):
         ("submitted", ctypes.c_uint64), ("completed", ctypes.c_uint64), ("rejected", ctypes.c_uint64),
         ("batches", ctypes.c_uint64), ("batched_requests", ctypes.c_uint64), ("max_batch", ctypes.c_int32),
         ("max_in_flight_batches", ctypes.c_int32), ("max_in_flight_bytes", ctypes.c_uint64),
-        ("resident_bytes", ctypes.c_uint64), ("device_bytes", ctypes.c_uint64),
+        ("resident_bytes", ctypes.c_uint64), ("device_bytes", ctypes.c_uint64), ("cancelled", ctypes.c_uint64),
     ]
 
     def as_dict(self) -> dict:
@@ -165,7 +167,7 @@ EXPORTS = [
     "shkz_scene_fill2d", "shkz_linsolve_default_params", "shkz_linsolver_create", "shkz_linsolver_destroy",
     "shkz_linsolve_csr", "shkz_extrapolate_velocity", "shkz_extrapolate_velocity_device",
     "shkz_advect_default_params", "shkz_advect_velocity", "shkz_advect_velocity_device", "shkz_advect_scalar",
-    "shkz_advect_scalar_device", "shkz_linsolve_csr_device_noalloc",
+    "shkz_advect_scalar_device", "shkz_linsolve_csr_device_noalloc", "shkz_cancel", "shkz_inject_alloc_failure",
 ]
 
 _lib = None
@@ -190,6 +192,8 @@ def lib():
     L.shkz_project.argtypes = [_vp, ctypes.POINTER(Grid), ctypes.POINTER(Params), ctypes.POINTER(Result)]
     L.shkz_project_device.argtypes = [_vp, ctypes.POINTER(Grid), ctypes.POINTER(Params), ctypes.POINTER(Result), _vp]
     L.shkz_release_scene.argtypes = [_vp, ctypes.c_uint64]
+    L.shkz_inject_alloc_failure.argtypes = [_vp, ctypes.c_int32]
+    L.shkz_cancel.argtypes = [_vp, ctypes.c_uint64]
     L.shkz_device_bytes.argtypes = [_vp]
     L.shkz_device_bytes.restype = ctypes.c_uint64
     L.shkz_set_kernel_timing.argtypes = [_vp, ctypes.c_int]
@@ -488,6 +492,11 @@ class Server:
         r = Result()
         rc = self._L.shkz_wait(self.handle, ticket, timeout_ms, ctypes.byref(r))
         return rc, r
+
+    def cancel(self, ticket: int) -> int:
+        """OK: cancelled while queued (the ticket then reports ERR_CANCELLED); PENDING: already started
+        or done (its result stands); ERR_NOT_FOUND: unknown ticket."""
+        return self._L.shkz_cancel(self.handle, ticket)
 
     def release_scene(self, scene_id: int):
         rc = self._L.shkz_server_release_scene(self.handle, scene_id)
@@ -797,6 +806,12 @@ class Context:
 
     def device_bytes(self) -> int:
         return int(self._L.shkz_device_bytes(self.handle))
+
+    def inject_alloc_failure(self, after: int):
+        """Test hook: the device allocation after `after` more successful ones fails (after < 0: off)."""
+        rc = self._L.shkz_inject_alloc_failure(self.handle, after)
+        if rc != 0:
+            raise ShkzError(rc, last_error())
 
     # ------------------------------------------------------------------ kernel timers
     def set_kernel_timing(self, on: bool):

@@ -119,13 +119,15 @@ struct SolveScalars {
     double r0inf;     // ||r0||_inf
     double r0l2;
     double tol;       // absolute tolerance in the chosen norm
+    double rmin;      // smallest residual norm reached (stagnation test)
     int32_t iter;     // iterations done
     int32_t maxit;
     int32_t done;     // 1 = stop (converged or out of iterations)
     int32_t converged;
     int32_t norm;     // shkz_norm
     int32_t status;   // 0 ok, 1 = NaN met
-    int32_t pad[2];
+    int32_t stall;        // iterations since the residual last reached a new minimum
+    int32_t stall_limit;  // > 0: stop after that many (inner solves of the defect correction)
 };
 
 // Batch-wide state of one Krylov solve: how many grids are done; every kernel returns at once when

@@ -682,6 +682,10 @@ enum FinOp {
 };
 
 // Fresh per-grid scalars (norm, iteration limit; everything else 0) for n grids.
+// Iterations without a new minimum of the residual after which an inner FP32 solve of the defect
+// correction stops (and the FP64 defect is recomputed).
+constexpr int INNER_STALL_LIMIT = 12;
+
 __global__ void k_init_scalars(SolveScalars *S, int n, int norm, int maxit) {
     const int g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= n) return;
@@ -767,13 +771,25 @@ __global__ __launch_bounds__(1024) void k_finalize(const double *__restrict__ pa
             S.beta = b / S.rz;
             S.rz = b;
         }
+        // an inner solve whose FP32 residual stops reaching new minima hands back to the FP64
+        // defect correction (a consistent singular system, for one, keeps a null-space component
+        // of the FP32 rounding that no iteration removes)
+        bool stalled = false;
+        if (S.stall_limit > 0) {
+            if (cur < S.rmin) {
+                S.rmin = cur;
+                S.stall = 0;
+            } else {
+                stalled = ++S.stall >= S.stall_limit;
+            }
+        }
         if (!(b == b)) {
             S.status = 1;
             mark_done(S, ctl);
         } else if (cur <= S.tol) {
             S.converged = 1;
             mark_done(S, ctl);
-        } else if (it >= S.maxit) {
+        } else if (it >= S.maxit || stalled) {
             mark_done(S, ctl);
         }
         break;
@@ -808,6 +824,9 @@ __global__ __launch_bounds__(1024) void k_finalize(const double *__restrict__ pa
                 rel = ocur > 0.0 ? fmax(param, O.tol / ocur) : param;
                 maxit = (O.done || ocur <= O.tol || O.status) ? 0 : max(1, O.maxit - O.iter);
                 S.maxit = maxit;
+                S.rmin = r0;
+                S.stall = 0;
+                S.stall_limit = INNER_STALL_LIMIT;
             }
             S.tol = rel * r0;
             S.iter = 0;
@@ -1261,8 +1280,46 @@ __global__ __launch_bounds__(BLOCK) void k_smooth_first4(LevelDesc L, const floa
     *reinterpret_cast<float4 *>(z + c) = make_float4(out[0], out[1], out[2], out[3]);
 }
 
-// q = A p and p.q partials (k_apply_dot on rows): one brick per 128-thread workgroup.
-__global__ __launch_bounds__(ROW4_THREADS) void k_apply_dot4(LevelDesc L, OpDesc<float> A, const float *__restrict__ p,
+// ---- The FP32 Krylov iteration of the mixed-precision solve.  Its vectors r, p, q, z are stored in
+// FP32, but the iterate is the FP64 pressure itself (x64 += alpha p in FP64) and q = A p is formed
+// from the FP64 operator in FP64 arithmetic, rounded once when stored.  Then the recursive
+// residual follows b - A x64 to FP32 storage accuracy of r and q: neither the rounding of an FP32
+// iterate nor an FP32 operator apply (whose cancellation amplifies the rounding by about the
+// condition number, 1e4 and more at 256^3) enters it, so one pass reaches the final tolerance.
+
+// (A v) in FP64 on the lane's four cells from FP32 values v and the FP64 operator (k_apply_dot's
+// arithmetic in FP64); unk: the unknown mask.
+__device__ __forceinline__ void apply_row4_64(const LevelDesc &L, const OpDesc<double> &A, int br, const BrickMeta &m,
+                                              const Row4Nbrs &N, const Row4Vals<float> &v, double av[4],
+                                              bool unk[4]) {
+    if (m.flags & BM_UNIFORM) {
+        const double g = L.uni[br].g64;
+        const double diag = 0.0 + ((g + g) + (g + g) + (g + g));
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const double off = g * (double)v.xm[j] + g * (double)v.xp[j] + g * (double)v.ym[j] + g * (double)v.yp[j] +
+                               g * (double)v.zm[j] + g * (double)v.zp[j];
+            av[j] = diag * (double)v.c[j] - off;
+            unk[j] = true;
+        }
+    } else {
+        const Row4Coef<double> g = row4_coef(A, N);
+        double dir[4];
+        load4(A.dir, N.c, dir);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const double off = g.gxm[j] * (double)v.xm[j] + g.gxp[j] * (double)v.xp[j] + g.gym[j] * (double)v.ym[j] +
+                               g.gyp[j] * (double)v.yp[j] + g.gzm[j] * (double)v.zm[j] + g.gzp[j] * (double)v.zp[j];
+            const double diag = dir[j] + ((g.gxm[j] + g.gxp[j]) + (g.gym[j] + g.gyp[j]) + (g.gzm[j] + g.gzp[j]));
+            av[j] = diag * (double)v.c[j] - off;
+            unk[j] = dir[j] >= 0;
+        }
+    }
+}
+
+// q = A p (FP64 operator and arithmetic, q stored in FP32) with the p.q partials (FP64 q): one
+// brick per 128-thread workgroup.
+__global__ __launch_bounds__(ROW4_THREADS) void k_apply_dot4(LevelDesc L, OpDesc<double> A, const float *__restrict__ p,
                                                              float *__restrict__ qv, const SolveScalars *S,
                                                              const BatchCtl *ctl, RedBuf R) {
     __shared__ double lds[4];
@@ -1274,108 +1331,26 @@ __global__ __launch_bounds__(ROW4_THREADS) void k_apply_dot4(LevelDesc L, OpDesc
     const Row4Nbrs N = row4_nbrs(m, br, threadIdx.x);
     const Row4Vals<float> v = row4_vals(p, N);
     if (ctl->all_done) return;
-    float av[4], out[4];
+    double av[4];
+    float out[4];
     bool unk[4];
-    apply_row4(L, A, br, m, N, v, av, unk);
+    apply_row4_64(L, A, br, m, N, v, av, unk);
     double part = 0.0, unused = 0.0;
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
-        out[j] = unk[j] ? av[j] : 0.0f;
-        part += unk[j] ? (double)v.c[j] * (double)av[j] : 0.0;
+        out[j] = unk[j] ? (float)av[j] : 0.0f;
+        part += unk[j] ? (double)v.c[j] * av[j] : 0.0;
     }
     *reinterpret_cast<float4 *>(qv + N.c) = make_float4(out[0], out[1], out[2], out[3]);
     row4_reduce2<false>(part, unused, lds);
     store_partials(R, br, part, 0.0);
 }
 
-// x += alpha p, r -= alpha q with ||r||_inf and r.r partials (k_cg_update on rows).
-__global__ __launch_bounds__(ROW4_THREADS) void k_cg_update4(LevelDesc L, float *__restrict__ x, float *__restrict__ r,
-                                                             const float *__restrict__ p, const float *__restrict__ qv,
-                                                             const SolveScalars *S, const BatchCtl *ctl, RedBuf R) {
-    __shared__ double lds[4];
-    const int br = brick_or_idle(L);
-    if (br < 0) return;
-    const long c = (long)br * BRICK_CELLS + 4 * threadIdx.x;
-    if (L.ngrids > 1 && (ctl->all_done || S[grid_of(L, br)].done)) return;
-    float xv[4], pv[4], rv[4], qq[4];
-    unpack4(ld4(x, c), xv);
-    unpack4(ld4(p, c), pv);
-    unpack4(ld4(r, c), rv);
-    unpack4(ld4(qv, c), qq);
-    const SolveScalars &Sg = S[grid_of(L, br)];
-    if (ctl->all_done || Sg.done) return;
-    const float alpha = (float)Sg.alpha;
-    double mx = 0.0, sq = 0.0;
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-        xv[j] = xv[j] + alpha * pv[j];
-        rv[j] = rv[j] - alpha * qq[j];
-        mx = fmax(mx, fabs((double)rv[j]));
-        sq += (double)rv[j] * (double)rv[j];
-    }
-    *reinterpret_cast<float4 *>(x + c) = make_float4(xv[0], xv[1], xv[2], xv[3]);
-    *reinterpret_cast<float4 *>(r + c) = make_float4(rv[0], rv[1], rv[2], rv[3]);
-    row4_reduce2<true>(mx, sq, lds);
-    store_partials(R, br, mx, sq);
-}
-
-// k_cg_update4 fused with the first pre-smoothing sweep of the next V-cycle on the finest level
-// (k_smooth_first4<1>): z = omega * idiag * r on the updated residual, on the bricks that sweep
-// covers (not those of grids ending on this level), with the same arithmetic.  A grid that this
-// update finds converged gets a z it never uses.
-__global__ __launch_bounds__(ROW4_THREADS) void k_cg_update4z(LevelDesc L, float *__restrict__ x,
-                                                              float *__restrict__ r, const float *__restrict__ p,
-                                                              const float *__restrict__ qv, const SolveScalars *S,
-                                                              const BatchCtl *ctl, RedBuf R,
-                                                              const float *__restrict__ idiag, float *__restrict__ z,
-                                                              float omega) {
-    __shared__ double lds[4];
-    const int br = brick_or_idle(L);
-    if (br < 0) return;
-    const long c = (long)br * BRICK_CELLS + 4 * threadIdx.x;
-    const BrickMeta m = L.meta[br];
-    const int g = L.ngrids > 1 ? m.grid : 0;
-    if (L.ngrids > 1 && (ctl->all_done || S[g].done)) return;
-    float xv[4], pv[4], rv[4], qq[4], id[4];
-    unpack4(ld4(x, c), xv);
-    unpack4(ld4(p, c), pv);
-    unpack4(ld4(r, c), rv);
-    unpack4(ld4(qv, c), qq);
-    if (m.flags & BM_UNIFORM) {
-        const float u = uni_load<float>(L, br).id;
-#pragma unroll
-        for (int j = 0; j < 4; ++j) id[j] = u;
-    } else {
-        unpack4(ld4(idiag, c), id);
-    }
-    const SolveScalars &Sg = S[g];
-    if (ctl->all_done || Sg.done) return;
-    const float alpha = (float)Sg.alpha;
-    double mx = 0.0, sq = 0.0;
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-        xv[j] = xv[j] + alpha * pv[j];
-        rv[j] = rv[j] - alpha * qq[j];
-        mx = fmax(mx, fabs((double)rv[j]));
-        sq += (double)rv[j] * (double)rv[j];
-    }
-    *reinterpret_cast<float4 *>(x + c) = make_float4(xv[0], xv[1], xv[2], xv[3]);
-    *reinterpret_cast<float4 *>(r + c) = make_float4(rv[0], rv[1], rv[2], rv[3]);
-    if (!(m.flags & BM_COARSEST)) {
-        float out[4];
-#pragma unroll
-        for (int j = 0; j < 4; ++j) out[j] = omega * id[j] * rv[j];
-        *reinterpret_cast<float4 *>(z + c) = make_float4(out[0], out[1], out[2], out[3]);
-    }
-    row4_reduce2<true>(mx, sq, lds);
-    store_partials(R, br, mx, sq);
-}
-
-// k_update_p4 (p = z + beta p) fused into k_apply_dot4: the new direction of the lane's cells and
-// of their neighbours is formed on the fly from z and the previous direction (same arithmetic),
-// written to pnew (a different buffer than pold: other workgroups still read pold), and q = A pnew
-// with the pnew.q partials.
-__global__ __launch_bounds__(ROW4_THREADS) void k_apply_dot4p(LevelDesc L, OpDesc<float> A,
+// k_update_p4 (p = z + beta p, FP32) fused into k_apply_dot4: the new direction of the lane's cells
+// and of their neighbours is formed on the fly from z and the previous direction (the same FP32
+// arithmetic), written to pnew (a different buffer than pold: other workgroups still read pold),
+// and q = A pnew with the pnew.q partials.
+__global__ __launch_bounds__(ROW4_THREADS) void k_apply_dot4p(LevelDesc L, OpDesc<double> A,
                                                               const float *__restrict__ z,
                                                               const float *__restrict__ pold,
                                                               float *__restrict__ pnew, float *__restrict__ qv,
@@ -1402,19 +1377,78 @@ __global__ __launch_bounds__(ROW4_THREADS) void k_apply_dot4p(LevelDesc L, OpDes
         v.zm[j] = zv.zm[j] + beta * po.zm[j];
         v.zp[j] = zv.zp[j] + beta * po.zp[j];
     }
-    float av[4], out[4];
+    double av[4];
+    float out[4];
     bool unk[4];
-    apply_row4(L, A, br, m, N, v, av, unk);
+    apply_row4_64(L, A, br, m, N, v, av, unk);
     double part = 0.0, unused = 0.0;
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
-        out[j] = unk[j] ? av[j] : 0.0f;
-        part += unk[j] ? (double)v.c[j] * (double)av[j] : 0.0;
+        out[j] = unk[j] ? (float)av[j] : 0.0f;
+        part += unk[j] ? (double)v.c[j] * av[j] : 0.0;
     }
     *reinterpret_cast<float4 *>(pnew + N.c) = make_float4(v.c[0], v.c[1], v.c[2], v.c[3]);
     *reinterpret_cast<float4 *>(qv + N.c) = make_float4(out[0], out[1], out[2], out[3]);
     row4_reduce2<false>(part, unused, lds);
     store_partials(R, br, part, 0.0);
+}
+
+// x64 += alpha p (FP64), r -= alpha q (FP32) with ||r||_inf and r.r partials; ZFIRST: also the first
+// pre-smoothing sweep of the next V-cycle from zero on the finest level (k_smooth_first4<1>),
+// z = omega * idiag * r on the updated residual, on the bricks that sweep covers (not those of
+// grids ending on this level), with the same arithmetic.  A grid that this update finds converged
+// gets a z it never uses.
+template <int ZFIRST>
+__global__ __launch_bounds__(ROW4_THREADS) void k_cg_update4(LevelDesc L, double *__restrict__ x,
+                                                             float *__restrict__ r, const float *__restrict__ p,
+                                                             const float *__restrict__ qv, const SolveScalars *S,
+                                                             const BatchCtl *ctl, RedBuf R,
+                                                             const float *__restrict__ idiag, float *__restrict__ z,
+                                                             float omega) {
+    __shared__ double lds[4];
+    const int br = brick_or_idle(L);
+    if (br < 0) return;
+    const long c = (long)br * BRICK_CELLS + 4 * threadIdx.x;
+    const BrickMeta m = L.meta[br];
+    const int g = L.ngrids > 1 ? m.grid : 0;
+    if (L.ngrids > 1 && (ctl->all_done || S[g].done)) return;
+    double xv[4];
+    float pv[4], rv[4], qq[4], id[4];
+    load4(x, c, xv);
+    unpack4(ld4(p, c), pv);
+    unpack4(ld4(r, c), rv);
+    unpack4(ld4(qv, c), qq);
+    if (ZFIRST) {
+        if (m.flags & BM_UNIFORM) {
+            const float u = uni_load<float>(L, br).id;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) id[j] = u;
+        } else {
+            unpack4(ld4(idiag, c), id);
+        }
+    }
+    const SolveScalars &Sg = S[g];
+    if (ctl->all_done || Sg.done) return;
+    const double alpha = Sg.alpha;
+    const float alpha32 = (float)alpha;
+    double mx = 0.0, sq = 0.0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        xv[j] = xv[j] + alpha * (double)pv[j];
+        rv[j] = rv[j] - alpha32 * qq[j];
+        mx = fmax(mx, fabs((double)rv[j]));
+        sq += (double)rv[j] * (double)rv[j];
+    }
+    store4(x, c, xv);
+    *reinterpret_cast<float4 *>(r + c) = make_float4(rv[0], rv[1], rv[2], rv[3]);
+    if (ZFIRST && !(m.flags & BM_COARSEST)) {
+        float out[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) out[j] = omega * id[j] * rv[j];
+        *reinterpret_cast<float4 *>(z + c) = make_float4(out[0], out[1], out[2], out[3]);
+    }
+    row4_reduce2<true>(mx, sq, lds);
+    store_partials(R, br, mx, sq);
 }
 
 // p = z + beta p (first: p = z), k_update_p on rows.
@@ -1463,18 +1497,17 @@ __global__ __launch_bounds__(ROW4_THREADS) void k_defect4(LevelDesc L, OpDesc<do
     store_partials(R, br, mx, sq);
 }
 
-// Inner (FP32) solve set-up of the defect correction: r32 = (float) r64, c = 0, with ||r32||
-// partials (finalized by FIN_INNER_INIT); four cells per thread.
+// Inner (FP32) solve set-up of the defect correction: r32 = (float) r64 with ||r32|| partials
+// (finalized by FIN_INNER_INIT); four cells per thread.  The inner solve adds its correction to the
+// FP64 iterate directly (k_cg_update4).
 __global__ __launch_bounds__(ROW4_THREADS) void k_inner_init4(LevelDesc L, const double *__restrict__ r64,
-                                                              float *__restrict__ r32, float *__restrict__ corr,
-                                                              RedBuf R) {
+                                                              float *__restrict__ r32, RedBuf R) {
     __shared__ double lds[4];
     const int br = brick_or_idle(L);
     if (br < 0) return;
     const long c = (long)br * BRICK_CELLS + 4 * threadIdx.x;
     double rv[4];
     float v[4];
-    const float zero[4] = {0.0f, 0.0f, 0.0f, 0.0f};
     load4(r64, c, rv);
     double mx = 0.0, sq = 0.0;
 #pragma unroll
@@ -1484,25 +1517,8 @@ __global__ __launch_bounds__(ROW4_THREADS) void k_inner_init4(LevelDesc L, const
         sq += (double)v[j] * (double)v[j];
     }
     store4(r32, c, v);
-    store4(corr, c, zero);
     row4_reduce2<true>(mx, sq, lds);
     store_partials(R, br, mx, sq);
-}
-
-// x64 += (double) corr on rows (four bricks per 512-thread workgroup).
-__global__ __launch_bounds__(BLOCK) void k_add_correction4(LevelDesc L, double *__restrict__ x,
-                                                           const float *__restrict__ corr) {
-    int br, q;
-    row4_slot(L, br, q);
-    if (br >= L.nbricks) return;
-    const long c = (long)br * BRICK_CELLS + 4 * q;
-    double xv[4];
-    float cv[4];
-    load4(x, c, xv);
-    load4(corr, c, cv);
-#pragma unroll
-    for (int j = 0; j < 4; ++j) xv[j] += (double)cv[j];
-    store4(x, c, xv);
 }
 
 // res = rhs - A z

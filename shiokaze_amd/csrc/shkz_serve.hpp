@@ -13,9 +13,12 @@
 //     all batches in flight plus the resident scene state must stay within the budget.  Memory is
 //     admitted in arrival order, so a large request is never starved by smaller ones behind it;
 //   * caps a batch at max_batch_grids grids and max_batch_cells box cells.
-// A request whose estimate cannot fit even on an idle device fails with SHKZ_ERR_BUDGET.
+// A request whose estimate cannot fit even on an idle device fails with SHKZ_ERR_BUDGET.  A request
+// still queued can be cancelled (it completes with SHKZ_ERR_CANCELLED); one that started runs to
+// its end.
 #pragma once
 
+#include <algorithm>
 #include <chrono>
 #include <condition_variable>
 #include <cstdint>
@@ -55,7 +58,7 @@ struct ServeLimits {
 };
 
 struct ServeStats {
-    uint64_t submitted = 0, completed = 0, rejected = 0, batches = 0, batched_requests = 0;
+    uint64_t submitted = 0, completed = 0, rejected = 0, batches = 0, batched_requests = 0, cancelled = 0;
     int max_batch = 0;
     int max_in_flight_batches = 0;
     uint64_t max_in_flight_bytes = 0;
@@ -124,6 +127,27 @@ public:
         if (timeout_ms < 0) cv_done_.wait(lk, ready);
         else cv_done_.wait_for(lk, std::chrono::duration<double, std::milli>(timeout_ms), ready);
         return take_locked(ticket, out);
+    }
+
+    // Cancels a request that is still queued: it completes at once with status -7 (cancelled) and
+    // is collected by poll / wait like any other.  Returns 0 when cancelled, 1 when the request
+    // already started or completed (it runs to the end; its outcome is unchanged), -5 for an
+    // unknown ticket.  Later requests of the same scene no longer wait for it.
+    int cancel(uint64_t ticket) {
+        {
+            std::lock_guard<std::mutex> g(mu_);
+            auto it = state_.find(ticket);
+            if (it == state_.end()) return -5;
+            auto q = std::find(queue_.begin(), queue_.end(), ticket);
+            if (q == queue_.end()) return 1;
+            queue_.erase(q);
+            const double t = serve_now_ms();
+            finish_locked(it->second.req, -7 /* SHKZ_ERR_CANCELLED */, 0, -1, t, t);
+            ++stats_.cancelled;
+        }
+        cv_done_.notify_all();
+        cv_work_.notify_all();  // requests of its scene may start now
+        return 0;
     }
 
     // A scene with a queued or running request.

@@ -9,7 +9,8 @@
 //     max_batch_cells cells (unless it is a single request);
 //   * the estimated bytes of the batches in flight plus the resident bytes never exceed the budget;
 //   * a request that cannot fit the budget alone fails with -4, others never do;
-//   * a large request queued behind a stream of small ones is not starved.
+//   * a large request queued behind a stream of small ones is not starved;
+//   * a request cancelled while queued never runs and reports -7; a started one is not cancelled.
 #include "../shiokaze_amd/csrc/shkz_serve.hpp"
 
 #include <atomic>
@@ -47,6 +48,7 @@ struct Recorder {
     int64_t max_cells = 0;
     uint64_t resident = 0;
     int batches = 0, max_batch = 0, max_concurrent = 0, concurrent = 0;
+    bool allow_gaps = false;
 };
 
 static void run_fake(Recorder &R, int, std::vector<ServeRequest> &batch, std::vector<int> &status, int us_base) {
@@ -62,8 +64,10 @@ static void run_fake(Recorder &R, int, std::vector<ServeRequest> &batch, std::ve
                 CHECK(!R.running_scenes.count(r.scene_id), "scene %llu runs twice", (unsigned long long)r.scene_id);
                 R.running_scenes.insert(r.scene_id);
                 const Tag *t = (const Tag *)r.payload;
-                CHECK(t->seq == R.next_seq[r.scene_id], "scene %llu ran step %d, expected %d",
-                      (unsigned long long)r.scene_id, t->seq, R.next_seq[r.scene_id]);
+                // in order; with cancellations a scene may skip the steps cancelled before they ran
+                CHECK(R.allow_gaps ? t->seq >= R.next_seq[r.scene_id] : t->seq == R.next_seq[r.scene_id],
+                      "scene %llu ran step %d, expected %d", (unsigned long long)r.scene_id, t->seq,
+                      R.next_seq[r.scene_id]);
                 R.next_seq[r.scene_id] = t->seq + 1;
             }
         }
@@ -242,9 +246,96 @@ static void scenario_budget() {
     }
 }
 
+// Cancellation: requests cancelled while queued complete with -7 and never run; cancelling a
+// running or finished request changes nothing (1); an unknown ticket gives -5; the remaining
+// requests of a scene still run in order after a cancelled one; clients cancel from several
+// threads while executors run.
+static void scenario_cancel() {
+    Recorder R;
+    ServeLimits lim;
+    lim.executors = 2;
+    lim.max_batch_grids = 4;
+    lim.max_batch_cells = 1 << 20;
+    lim.budget = 1 << 30;
+    R.budget = lim.budget;
+    R.max_grids = lim.max_batch_grids;
+    R.max_cells = lim.max_batch_cells;
+    R.allow_gaps = true;
+    const int scenes = 6, steps = 30;
+    std::vector<Tag> tags(scenes * steps);
+    std::atomic<int> ran{0};
+    {
+        ServeCore core(
+            lim,
+            [&](int k, std::vector<ServeRequest> &b, std::vector<int> &st) {
+                ran += (int)b.size();
+                run_fake(R, k, b, st, 150);
+            },
+            [] { return (uint64_t)0; });
+        std::vector<uint64_t> tickets(scenes * steps);
+        uint64_t t = 1;
+        for (int s = 0; s < steps; ++s)
+            for (int q = 0; q < scenes; ++q) {
+                Tag &tg = tags[q * steps + s];
+                tg = Tag{(uint64_t)(500 + q), s};
+                ServeRequest r;
+                r.ticket = t++;
+                r.scene_id = tg.scene;
+                r.est_bytes = 10;
+                r.cells = 10;
+                r.payload = &tg;
+                tickets[q * steps + s] = r.ticket;
+                CHECK(core.submit(r) == 0, "submit");
+            }
+        // three threads cancel every third step of their scenes from the back (run_fake checks that
+        // each scene's remaining steps run in increasing order)
+        std::atomic<int> cancelled{0}, not_cancelled{0};
+        std::vector<std::thread> th;
+        for (int c = 0; c < 3; ++c)
+            th.emplace_back([&, c] {
+                for (int q = c; q < scenes; q += 3)
+                    for (int s = steps - 1; s >= 0; s -= 3) {
+                        const int i = q * steps + s;
+                        const int rc = core.cancel(tickets[i]);
+                        CHECK(rc == 0 || rc == 1, "cancel rc %d", rc);
+                        if (rc == 0) ++cancelled;
+                        else ++not_cancelled;
+                    }
+            });
+        for (auto &x : th) x.join();
+        // the first step of the first scene has long run: cancelling it changes nothing
+        std::this_thread::sleep_for(std::chrono::milliseconds(100));
+        CHECK(core.cancel(tickets[0]) == 1, "cancelled a finished request");
+        // collect every request: cancelled ones report -7 and were never run
+        int got_cancelled = 0, got_ok = 0;
+        for (int i = 0; i < scenes * steps; ++i) {
+            ServeOutcome o;
+            CHECK(core.wait(tickets[i], -1, &o) == 0, "wait");
+            if (o.status == -7) {
+                ++got_cancelled;
+                CHECK(o.executor == -1 && o.batch_size == 0, "cancelled request has an executor");
+            } else {
+                CHECK(o.status == 0, "status %d", o.status);
+                ++got_ok;
+            }
+            CHECK(core.cancel(tickets[i]) == -5, "collected ticket still known");
+        }
+        CHECK(got_cancelled == cancelled.load(), "cancelled %d reported %d", cancelled.load(), got_cancelled);
+        CHECK(got_ok + got_cancelled == scenes * steps, "outcomes");
+        CHECK(ran.load() == got_ok, "ran %d ok %d", ran.load(), got_ok);
+        CHECK(cancelled.load() > 0, "nothing was cancelled");
+        const ServeStats st = core.stats();
+        CHECK(st.cancelled == (uint64_t)cancelled.load(), "stats.cancelled %llu", (unsigned long long)st.cancelled);
+        CHECK(core.cancel(987654321) == -5, "unknown ticket");
+        std::printf("cancel: %d cancelled while queued, %d already started or done\n", cancelled.load(),
+                    not_cancelled.load());
+    }
+}
+
 int main() {
     scenario_clients();
     scenario_budget();
+    scenario_cancel();
     if (g_fail) {
         std::printf("FAILED\n");
         return 1;
