@@ -179,20 +179,26 @@ def kernel_roofline(stats, inst_steps, config):
 
 
 def explicit_mg(spec):
-    """Params fields of an explicit --mg pre,post,coarse_pre,coarse_post,coarse_sweeps,omega,eta."""
+    """Params fields of an explicit --mg pre,post,coarse_pre,coarse_post,coarse_sweeps,omega,eta[,smoother]
+    (smoother: 1 = damped Jacobi, 2 = brick-local Gauss-Seidel on the finest level)."""
     if not spec:
         return {}
     v = spec.split(",")
-    return dict(mg_pre_sweeps=int(v[0]), mg_post_sweeps=int(v[1]), mg_coarse_pre_sweeps=int(v[2]),
-                mg_coarse_post_sweeps=int(v[3]), mg_coarse_sweeps=int(v[4]), mg_omega=float(v[5]),
-                inner_residual=float(v[6]))
+    out = dict(mg_pre_sweeps=int(v[0]), mg_post_sweeps=int(v[1]), mg_coarse_pre_sweeps=int(v[2]),
+               mg_coarse_post_sweeps=int(v[3]), mg_coarse_sweeps=int(v[4]), mg_omega=float(v[5]),
+               inner_residual=float(v[6]))
+    if len(v) > 7:
+        out["mg_smoother"] = int(v[7])
+    return out
 
 
 def mg_controls(r):
     """The multigrid controls a result reports (autotuned or default)."""
     return {"pre": r.mg_pre_sweeps, "post": r.mg_post_sweeps, "coarse_pre": r.mg_coarse_pre_sweeps,
             "coarse_post": r.mg_coarse_post_sweeps, "coarse_solve_sweeps": r.mg_coarse_sweeps,
-            "omega": r.mg_omega, "inner_residual": r.inner_residual, "tuned": r.tuned}
+            "omega": r.mg_omega, "inner_residual": r.inner_residual,
+            "smoother": {1: "jacobi", 2: "brick-local gauss-seidel"}.get(r.mg_smoother, r.mg_smoother),
+            "tuned": r.tuned}
 
 
 def batched_block(args, steps, single_rounds=1, kernel_timing=False):

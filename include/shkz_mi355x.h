@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define SHKZ_ABI_VERSION 5
+#define SHKZ_ABI_VERSION 6
 
 /* ---------------------------------------------------------------- enums */
 enum shkz_status {
@@ -59,6 +59,15 @@ enum shkz_solver {
 enum shkz_precision {
     SHKZ_PREC_MIXED = 0, /* FP32 V-cycle / inner CG inside an FP64 defect-correction loop */
     SHKZ_PREC_FP64 = 1   /* every vector and operator in FP64 */
+};
+
+/* Finest-level smoother of the V-cycle.  Coarser levels always use damped Jacobi. */
+enum shkz_smoother {
+    SHKZ_SMOOTH_DEFAULT = 0, /* brick-local in mixed precision, Jacobi in FP64 (or the tuner's choice) */
+    SHKZ_SMOOTH_JACOBI = 1,  /* damped Jacobi, one sweep per mg_pre/post_sweeps */
+    SHKZ_SMOOTH_BRICK = 2    /* brick-local symmetric Gauss-Seidel (block Jacobi across 8^3 bricks, two
+                                red-black sweeps inside each), one application per mg_pre/post_sweeps;
+                                mixed precision only (FP64 solves use Jacobi) */
 };
 
 enum shkz_norm {
@@ -128,7 +137,8 @@ typedef struct shkz_params {
                                      (repeated solves of the built system; the returned projection
                                      uses the best controls).  Cached per process, and in the file
                                      named by the environment variable SHKZ_TUNE_CACHE if set. */
-    int32_t reserved[4];
+    int32_t mg_smoother;          /* finest-level smoother (shkz_smoother; 0 = library default / tuned) */
+    int32_t reserved[3];
 } shkz_params;
 
 /* ---------------------------------------------------------------- result */
@@ -161,6 +171,8 @@ typedef struct shkz_result {
     int32_t tuned;                /* 0 = defaults / given, 1 = from the tuning cache, 2 = tuned now */
     double mg_omega;
     double inner_residual;
+    int32_t mg_smoother;          /* shkz_smoother used on the finest level */
+    int32_t pad;
 } shkz_result;
 
 /* ---------------------------------------------------------------- context */

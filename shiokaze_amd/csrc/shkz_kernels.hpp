@@ -1185,6 +1185,168 @@ __global__ __launch_bounds__(ROW4_THREADS) void k_jacobi4_dot(LevelDesc L, OpDes
     }
 }
 
+// ---- Brick-local smoother (finest level, FP32): z += M_b (rhs - A z), where M_b applies
+// LOCAL_SGS_SWEEPS symmetric red-black Gauss-Seidel sweeps from zero to the brick's own operator
+// (its couplings to cells in other bricks dropped, the diagonal whole), i.e. the half-sweeps red,
+// black, red, ..., red.  Across bricks it is block Jacobi; inside a brick, Gauss-Seidel.  M_b is
+// symmetric, so the V-cycle built on it is a symmetric preconditioner.  One pass over the level's
+// arrays per application, like one Jacobi sweep, smoothing about as much as three.
+constexpr int LOCAL_SGS_SWEEPS = 2;
+
+// The couplings of the lane's four cells to cells of the same brick (0 across the brick's faces):
+// from the loaded Row4Coef of the level (whose high couplings may come from neighbour bricks).
+__device__ __forceinline__ Row4Coef<float> row4_local(const Row4Coef<float> &g, int q) {
+    const int li = q >> 4, lj = (q >> 1) & 7, h = q & 1;
+    Row4Coef<float> o;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        o.gxm[j] = li > 0 ? g.gxm[j] : 0.0f;
+        o.gxp[j] = li < 7 ? g.gxp[j] : 0.0f;
+        o.gym[j] = lj > 0 ? g.gym[j] : 0.0f;
+        o.gyp[j] = lj < 7 ? g.gyp[j] : 0.0f;
+        const int lk = 4 * h + j;
+        o.gzm[j] = lk > 0 ? g.gzm[j] : 0.0f;
+        o.gzp[j] = lk < 7 ? g.gzp[j] : 0.0f;
+    }
+    return o;
+}
+__device__ __forceinline__ Row4Coef<float> row4_local_uniform(float g, int q) {
+    Row4Coef<float> u;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) u.gxm[j] = u.gxp[j] = u.gym[j] = u.gyp[j] = u.gzm[j] = u.gzp[j] = g;
+    return row4_local(u, q);
+}
+
+// e = M_b r on the brick of a 128-thread workgroup (lane q: cells 4q .. 4q+3 of the brick, the
+// storage order).  lds: 512 floats.  Every lane of the workgroup must call it (barriers).
+__device__ __forceinline__ void local_sgs4(const Row4Coef<float> &c, const float id[4], const float r[4], float e[4],
+                                           float *lds, int q) {
+    const int li = q >> 4, lj = (q >> 1) & 7, h = q & 1;
+    const int base = 4 * q;
+    const int col0 = (li + lj) & 1;  // colour of the lane's cell j is col0 ^ (j & 1) (4h is even)
+    // red half-sweep from zero: no neighbour values yet
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        e[j] = ((col0 ^ (j & 1)) == 0) ? id[j] * r[j] : 0.0f;
+        if ((col0 ^ (j & 1)) == 0) lds[base + j] = e[j];
+    }
+    __syncthreads();
+#pragma unroll 1
+    for (int s = 1; s <= 2 * LOCAL_SGS_SWEEPS; ++s) {
+        const int colour = s & 1;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            if ((col0 ^ (j & 1)) != colour) continue;
+            const int at = base + j, lk = 4 * h + j;
+            const float xm = li > 0 ? lds[at - 64] : 0.0f, xp = li < 7 ? lds[at + 64] : 0.0f;
+            const float ym = lj > 0 ? lds[at - 8] : 0.0f, yp = lj < 7 ? lds[at + 8] : 0.0f;
+            const float zm = lk > 0 ? lds[at - 1] : 0.0f, zp = lk < 7 ? lds[at + 1] : 0.0f;
+            const float nb = c.gxm[j] * xm + c.gxp[j] * xp + c.gym[j] * ym + c.gyp[j] * yp + c.gzm[j] * zm +
+                             c.gzp[j] * zp;
+            e[j] = id[j] * (r[j] + nb);
+            lds[at] = e[j];
+        }
+        __syncthreads();
+    }
+}
+
+// Pre-smoothing from zero on the finest level: z = M_b rhs (one brick per 128-thread workgroup).
+__global__ __launch_bounds__(ROW4_THREADS) void k_pre_local4(LevelDesc L, OpDesc<float> A,
+                                                             const float *__restrict__ idiag,
+                                                             const float *__restrict__ rhs, float *__restrict__ z,
+                                                             Gate Q) {
+    __shared__ float lds[BRICK_CELLS];
+    const int br = brick_or_idle(L);
+    if (br < 0) return;
+    const BrickMeta m = L.meta[br];
+    if (all_done(Q) || skip_smooth(Q, L, m)) return;
+    const int q = threadIdx.x;
+    const Row4Nbrs N = row4_nbrs(m, br, q);
+    float r[4], id[4], e[4];
+    unpack4(ld4(rhs, N.c), r);
+    Row4Coef<float> c;
+    if (m.flags & BM_UNIFORM) {
+        const UniT<float> u = uni_load<float>(L, br);
+        c = row4_local_uniform(u.g, q);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) id[j] = u.id;
+    } else {
+        c = row4_local(row4_coef(A, N), q);
+        unpack4(ld4(idiag, N.c), id);
+    }
+    local_sgs4(c, id, r, e, lds, q);
+    *reinterpret_cast<float4 *>(z + N.c) = make_float4(e[0], e[1], e[2], e[3]);
+}
+
+// Post-smoothing (and further pre-smoothing) on the finest level: zout = zin + M_b (rhs - A zin);
+// DOT: also the rhs.zout partials of the unknowns (the last post-smoothing pass yields r.z of the
+// preconditioned CG).  One brick per 128-thread workgroup.
+template <int DOT>
+__global__ __launch_bounds__(ROW4_THREADS) void k_post_local4(LevelDesc L, OpDesc<float> A,
+                                                              const float *__restrict__ idiag,
+                                                              const float *__restrict__ rhs,
+                                                              const float *__restrict__ zin, float *__restrict__ zout,
+                                                              Gate Q, RedBuf R) {
+    __shared__ float lds[BRICK_CELLS];
+    __shared__ double red[2];
+    const int br = brick_or_idle(L);
+    if (br < 0) return;
+    const BrickMeta m = L.meta[br];
+    if (all_done(Q) || skip_smooth(Q, L, m)) return;
+    const int q = threadIdx.x;
+    const Row4Nbrs N = row4_nbrs(m, br, q);
+    const Row4Vals<float> v = row4_vals(zin, N);
+    float r[4], id[4], e[4], res[4];
+    unpack4(ld4(rhs, N.c), r);
+    Row4Coef<float> c;
+    bool unk[4];
+    if (m.flags & BM_UNIFORM) {
+        const UniT<float> u = uni_load<float>(L, br);
+        const float diag = 0.0f + ((u.g + u.g) + (u.g + u.g) + (u.g + u.g));
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const float off = u.g * v.xm[j] + u.g * v.xp[j] + u.g * v.ym[j] + u.g * v.yp[j] + u.g * v.zm[j] +
+                              u.g * v.zp[j];
+            res[j] = r[j] - (diag * v.c[j] - off);
+            id[j] = u.id;
+            unk[j] = true;
+        }
+        c = row4_local_uniform(u.g, q);
+    } else {
+        const Row4Coef<float> g = row4_coef(A, N);
+        float dir[4];
+        load4(A.dir, N.c, dir);
+        unpack4(ld4(idiag, N.c), id);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const float off = g.gxm[j] * v.xm[j] + g.gxp[j] * v.xp[j] + g.gym[j] * v.ym[j] + g.gyp[j] * v.yp[j] +
+                              g.gzm[j] * v.zm[j] + g.gzp[j] * v.zp[j];
+            const float diag = dir[j] + ((g.gxm[j] + g.gxp[j]) + (g.gym[j] + g.gyp[j]) + (g.gzm[j] + g.gzp[j]));
+            unk[j] = dir[j] >= 0;
+            res[j] = unk[j] ? r[j] - (diag * v.c[j] - off) : 0.0f;
+        }
+        c = row4_local(g, q);
+    }
+    local_sgs4(c, id, res, e, lds, q);
+    float out[4];
+    double part = 0.0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        out[j] = unk[j] ? v.c[j] + e[j] : 0.0f;
+        if (DOT) part += unk[j] ? (double)r[j] * (double)out[j] : 0.0;
+    }
+    *reinterpret_cast<float4 *>(zout + N.c) = make_float4(out[0], out[1], out[2], out[3]);
+    if (DOT) {
+        part = wave_sum(part);
+        if ((q & 63) == 0) red[q >> 6] = part;
+        __syncthreads();
+        if (q == 0) {
+            R.partials[2 * br + 0] = red[0] + red[1];
+            R.partials[2 * br + 1] = 0.0;
+        }
+    }
+}
+
 // (A v) on the lane's four cells, with the unknown mask: apply_row / apply_row_uniform per cell.
 template <class T>
 __device__ __forceinline__ void apply_row4(const LevelDesc &L, const OpDesc<T> &A, int br, const BrickMeta &m,
@@ -1394,17 +1556,17 @@ __global__ __launch_bounds__(ROW4_THREADS) void k_apply_dot4p(LevelDesc L, OpDes
 }
 
 // x64 += alpha p (FP64), r -= alpha q (FP32) with ||r||_inf and r.r partials; ZFIRST: also the first
-// pre-smoothing sweep of the next V-cycle from zero on the finest level (k_smooth_first4<1>),
-// z = omega * idiag * r on the updated residual, on the bricks that sweep covers (not those of
-// grids ending on this level), with the same arithmetic.  A grid that this update finds converged
-// gets a z it never uses.
+// pre-smoothing step of the next V-cycle from zero on the finest level, on the bricks it covers
+// (not those of grids ending on this level), with the same arithmetic: 1 = a damped Jacobi sweep
+// (k_smooth_first4<1>), z = omega * idiag * r; 2 = the brick-local smoother (k_pre_local4), whose
+// brick is the workgroup's.  A grid that this update finds converged gets a z it never uses.
 template <int ZFIRST>
 __global__ __launch_bounds__(ROW4_THREADS) void k_cg_update4(LevelDesc L, double *__restrict__ x,
                                                              float *__restrict__ r, const float *__restrict__ p,
                                                              const float *__restrict__ qv, const SolveScalars *S,
                                                              const BatchCtl *ctl, RedBuf R,
                                                              const float *__restrict__ idiag, float *__restrict__ z,
-                                                             float omega) {
+                                                             float omega, OpDesc<float> A32) {
     __shared__ double lds[4];
     const int br = brick_or_idle(L);
     if (br < 0) return;
@@ -1418,13 +1580,16 @@ __global__ __launch_bounds__(ROW4_THREADS) void k_cg_update4(LevelDesc L, double
     unpack4(ld4(p, c), pv);
     unpack4(ld4(r, c), rv);
     unpack4(ld4(qv, c), qq);
+    Row4Coef<float> cf;  // ZFIRST == 2: the brick's own couplings
     if (ZFIRST) {
         if (m.flags & BM_UNIFORM) {
-            const float u = uni_load<float>(L, br).id;
+            const UniT<float> u = uni_load<float>(L, br);
 #pragma unroll
-            for (int j = 0; j < 4; ++j) id[j] = u;
+            for (int j = 0; j < 4; ++j) id[j] = u.id;
+            if (ZFIRST == 2) cf = row4_local_uniform(u.g, threadIdx.x);
         } else {
             unpack4(ld4(idiag, c), id);
+            if (ZFIRST == 2) cf = row4_local(row4_coef(A32, row4_nbrs(m, br, threadIdx.x)), threadIdx.x);
         }
     }
     const SolveScalars &Sg = S[g];
@@ -1441,11 +1606,18 @@ __global__ __launch_bounds__(ROW4_THREADS) void k_cg_update4(LevelDesc L, double
     }
     store4(x, c, xv);
     *reinterpret_cast<float4 *>(r + c) = make_float4(rv[0], rv[1], rv[2], rv[3]);
-    if (ZFIRST && !(m.flags & BM_COARSEST)) {
+    if (ZFIRST == 1 && !(m.flags & BM_COARSEST)) {
         float out[4];
 #pragma unroll
         for (int j = 0; j < 4; ++j) out[j] = omega * id[j] * rv[j];
         *reinterpret_cast<float4 *>(z + c) = make_float4(out[0], out[1], out[2], out[3]);
+    }
+    if (ZFIRST == 2 && !(m.flags & BM_COARSEST)) {
+        // the brick-local smoother's pre-smoothing from zero (k_pre_local4) on the updated residual
+        __shared__ float zl[BRICK_CELLS];
+        float e[4];
+        local_sgs4(cf, id, rv, e, zl, threadIdx.x);
+        *reinterpret_cast<float4 *>(z + c) = make_float4(e[0], e[1], e[2], e[3]);
     }
     row4_reduce2<true>(mx, sq, lds);
     store_partials(R, br, mx, sq);
@@ -1936,6 +2108,61 @@ __global__ __launch_bounds__(1024) void k_smooth_k(LevelDesc L, OpDesc<T> A, con
     }
 }
 
+// Connected components of the coarsest brick of every grid ending on level L (block g: grid g):
+// unknowns joined by a nonzero coupling, labelled by min-label propagation in LDS.  code[cell] =
+// -1 off the unknowns, -2 in a component with a Dirichlet term (a nonsingular block), else the
+// smallest cell index of its component: a pure-Neumann component, singular with the constants of
+// the component as its null space (an all-fluid box, a pocket enclosed by solid).  Built once per
+// projection; k_coarse_solve makes the right-hand side of such a component consistent.
+template <class T>
+__global__ __launch_bounds__(BRICK_CELLS) void k_coarse_components(LevelDesc L, OpDesc<T> A, int16_t *code) {
+    __shared__ int lab[BRICK_CELLS];
+    __shared__ double dsum[BRICK_CELLS];
+    const int g = blockIdx.x;
+    const GridLevel G = L.gl[g];
+    if (!G.coarsest || G.brick_end <= G.brick_start) return;
+    const int e = threadIdx.x;
+    const long base = (long)G.brick_start * BRICK_CELLS;
+    const long c = base + e;
+    const int li = e >> 6, lj = (e >> 3) & 7, lk = e & 7;
+    const T dirc = A.dir[c];
+    const bool unk = dirc >= 0;
+    // couplings to the six neighbours (the brick is the whole grid here: nothing outside it)
+    const bool cxm = li > 0 && A.gx[c] > T(0), cxp = li < 7 && A.gx[c + 64] > T(0);
+    const bool cym = lj > 0 && A.gy[c] > T(0), cyp = lj < 7 && A.gy[c + 8] > T(0);
+    const bool czm = lk > 0 && A.gz[c] > T(0), czp = lk < 7 && A.gz[c + 1] > T(0);
+    lab[e] = unk ? e : BRICK_CELLS;
+    __syncthreads();
+    // labels only decrease, so the propagation reaches the unique fixpoint (the smallest index of
+    // each component) whatever the interleaving; at most 512 rounds
+    for (int it = 0; it < BRICK_CELLS; ++it) {
+        int m = lab[e];
+        if (unk) {
+            if (cxm) m = min(m, lab[e - 64]);
+            if (cxp) m = min(m, lab[e + 64]);
+            if (cym) m = min(m, lab[e - 8]);
+            if (cyp) m = min(m, lab[e + 8]);
+            if (czm) m = min(m, lab[e - 1]);
+            if (czp) m = min(m, lab[e + 1]);
+        }
+        const bool changed = m < lab[e];
+        __syncthreads();
+        if (changed) lab[e] = m;
+        if (!__syncthreads_or(changed)) break;
+    }
+    // Dirichlet sum of each component, summed by its root in cell order
+    dsum[e] = unk ? (double)dirc : 0.0;
+    __syncthreads();
+    double d = 0.0;
+    if (unk && lab[e] == e)
+        for (int k = e; k < BRICK_CELLS; ++k)
+            if (lab[k] == e) d += dsum[k];
+    __syncthreads();
+    dsum[e] = d;
+    __syncthreads();
+    code[c] = !unk ? (int16_t)-1 : (dsum[lab[e]] > 0.0 ? (int16_t)-2 : (int16_t)lab[e]);
+}
+
 // Coarse solve of every grid whose last level L is (one brick): block g handles grid g with
 // `sweeps` damped-Jacobi sweeps from zero, in LDS.  DOT (finest level only): also the r.z partial
 // of the grid's brick, as k_jacobi<..., DOT> stores it for grids with coarser levels.  CPT cells per
@@ -1944,10 +2171,12 @@ __global__ __launch_bounds__(1024) void k_smooth_k(LevelDesc L, OpDesc<T> A, con
 template <class T, int DOT, int CPT = 1>
 __global__ __launch_bounds__(BRICK_CELLS / CPT) void k_coarse_solve(LevelDesc L, OpDesc<T> A,
                                                                     const T *__restrict__ rhs, T *__restrict__ z,
-                                                                    int sweeps, T omega, Gate Q, RedBuf R) {
+                                                                    int sweeps, T omega, Gate Q, RedBuf R,
+                                                                    const int16_t *__restrict__ code) {
     constexpr int NT = BRICK_CELLS / CPT;
     __shared__ T zs[2][BRICK_CELLS];
     __shared__ double lds[32];
+    __shared__ int16_t cc[BRICK_CELLS];
     if (all_done(Q)) return;
     const int g = blockIdx.x;
     const GridLevel G = L.gl[g];
@@ -1972,8 +2201,48 @@ __global__ __launch_bounds__(BRICK_CELLS / CPT) void k_coarse_solve(LevelDesc L,
         unk[u] = dirc >= 0 && diag[u] > 0;
         r[u] = rhs[c];
         inv[u] = unk[u] ? omega / diag[u] : T(0);
-        zr[u] = unk[u] ? inv[u] * r[u] : T(0);
-        zs[0][e] = zr[u];
+        cc[e] = code[c];
+    }
+    // A pure-Neumann component (k_coarse_components) is singular: its right-hand side is made
+    // consistent (its mean over the component removed) so that the sweeps do not grow the
+    // component's constant mode, which the Krylov iteration would carry into the pressure.
+    bool sing = false;
+#pragma unroll
+    for (int u = 0; u < CPT; ++u) sing = sing || cc[t + u * NT] >= 0;
+    T rs[CPT];  // the right-hand side the sweeps solve (r itself, or made consistent)
+#pragma unroll
+    for (int u = 0; u < CPT; ++u) rs[u] = r[u];
+    if (__syncthreads_or(sing)) {
+#pragma unroll
+        for (int u = 0; u < CPT; ++u) zs[0][t + u * NT] = r[u];
+        __syncthreads();
+        // the root of each singular component sums it in cell order and stores its mean
+#pragma unroll
+        for (int u = 0; u < CPT; ++u) {
+            const int e = t + u * NT;
+            if (cc[e] == e) {
+                double sum = 0.0;
+                int cnt = 0;
+                for (int k = e; k < BRICK_CELLS; ++k)
+                    if (cc[k] == e) {
+                        sum += (double)zs[0][k];
+                        ++cnt;
+                    }
+                zs[1][e] = (T)(sum / cnt);
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < CPT; ++u) {
+            const int e = t + u * NT;
+            if (cc[e] >= 0) rs[u] = r[u] - zs[1][cc[e]];
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int u = 0; u < CPT; ++u) {
+        zr[u] = unk[u] ? inv[u] * rs[u] : T(0);
+        zs[0][t + u * NT] = zr[u];
     }
     __syncthreads();
     int cur = 0;
@@ -1989,7 +2258,7 @@ __global__ __launch_bounds__(BRICK_CELLS / CPT) void k_coarse_solve(LevelDesc L,
                 const T off = gxm[u] * (li > 0 ? zz[e - 64] : T(0)) + gxp[u] * (li < 7 ? zz[e + 64] : T(0)) +
                               gym[u] * (lj > 0 ? zz[e - 8] : T(0)) + gyp[u] * (lj < 7 ? zz[e + 8] : T(0)) +
                               gzm[u] * (lk > 0 ? zz[e - 1] : T(0)) + gzp[u] * (lk < 7 ? zz[e + 1] : T(0));
-                out[u] = zr[u] + inv[u] * (r[u] - (diag[u] * zr[u] - off));
+                out[u] = zr[u] + inv[u] * (rs[u] - (diag[u] * zr[u] - off));
             }
         }
 #pragma unroll

@@ -190,6 +190,7 @@ struct LevelHost {
     // operator and V-cycle vectors (element type = level precision)
     void *dir = nullptr, *gx = nullptr, *gy = nullptr, *gz = nullptr, *idiag = nullptr;
     void *rhs = nullptr, *z = nullptr, *tmp = nullptr, *res = nullptr;
+    int16_t *ccode = nullptr;  // coarsest bricks: component codes (k_coarse_components)
     int64_t unknowns = 0;
     int64_t uniform_cells = 0;  // cells of BM_UNIFORM bricks (level 0)
     std::vector<int64_t> unknowns_grid;
@@ -575,6 +576,7 @@ struct Problem {
     BatchCtl *ctl = nullptr;  // [0] outer / FP64 solve, [1] inner solve
     RedBuf red{};
     int pre = 4, post = 4, pre_c = 4, post_c = 4, coarse_sweeps = 48;
+    int smoother = SHKZ_SMOOTH_JACOBI;  // finest level (SHKZ_SMOOTH_BRICK only with an FP32 V-cycle)
     double omega = 0.9;
     int multi_max_bricks = 1024;  // coarse levels up to this size smooth with multi-sweep launches
     double eta = 0.0;  // mixed precision: relative defect reduction asked of each inner solve (0: to the tolerance)
@@ -718,6 +720,7 @@ template <class T>
 static T *vcycle(shkz_ctx *C, Launcher &K, Problem &P, const T *rhs0, T *z0, T *tmp0, Gate Q, bool dot_partials,
                  bool first_done = false) {
     const int nl = P.nlev;
+    const bool brick_smoother = sizeof(T) == 4 && P.smoother == SHKZ_SMOOTH_BRICK;
     std::vector<T *> zcur(nl), zoth(nl);
     const T om = (T)P.omega;
     const double es = (double)sizeof(T);
@@ -748,10 +751,11 @@ static T *vcycle(shkz_ctx *C, Launcher &K, Problem &P, const T *rhs0, T *z0, T *
             const dim3 cb(BRICK_CELLS / CPT);
             if (f && dot_partials)
                 K.go("coarse_solve_dot", l, 6 * es, 0, dim3((unsigned)P.ngrids), cb, k_coarse_solve<T, 1, CPT>, L.desc,
-                     A, rhs, dst, P.coarse_sweeps, om, Q, P.red);
+                     A, rhs, dst, P.coarse_sweeps, om, Q, P.red, (const int16_t *)L.ccode);
             else
                 K.go("coarse_solve", l, 6 * es, 0, dim3((unsigned)P.ngrids), cb, k_coarse_solve<T, 0, CPT>, L.desc, A,
-                     rhs, dst, P.coarse_sweeps, om, Q, P.red);
+                     rhs, dst, P.coarse_sweeps, om, Q, P.red,
+                     (const int16_t *)L.ccode);
         }
         if (last) {
             zcur[l] = z;
@@ -764,6 +768,19 @@ static T *vcycle(shkz_ctx *C, Launcher &K, Problem &P, const T *rhs0, T *z0, T *
             res_done = smooth_k_res_supported<T>(pre);
             smooth_k<T>(K, l, pre, true, L.desc, A, id, rhs, (const T *)nullptr, z, om, Q,
                         res_done ? (T *)L.res : nullptr);
+        } else if (f && brick_smoother) {
+            // brick-local smoother: z = M_b rhs from zero (done by the residual update when
+            // first_done), then further applications z += M_b (rhs - A z)
+            if constexpr (sizeof(T) == 4) {
+                if (!first_done)
+                    K.go("pre_local_L0", 0, 6 * es, -usave(C, 0, 4 * es), dim3(brick_grid(L.nbricks)), dim3(ROW4_THREADS),
+                         k_pre_local4, L.desc, A, id, rhs, z, Q);
+                for (int s = 1; s < pre; ++s) {
+                    K.go("post_local_L0", 0, 8 * es, -usave(C, 0, 5 * es), dim3(brick_grid(L.nbricks)),
+                         dim3(ROW4_THREADS), k_post_local4<0>, L.desc, A, id, rhs, (const T *)cur, oth, Q, P.red);
+                    std::swap(cur, oth);
+                }
+            }
         } else if (pre <= 1 || f) {
             // finest level: one sweep from zero, then plain sweeps (cheaper there than the fused
             // two-sweep kernel, whose on-the-fly neighbour values cost 12 extra loads per cell)
@@ -832,6 +849,19 @@ static T *vcycle(shkz_ctx *C, Launcher &K, Problem &P, const T *rhs0, T *z0, T *
                  f ? k_prolong<T, 1> : k_prolong<T, 0>, L.desc, A, (const T *)zcur[l + 1], zcur[l], Q);
         for (int s = 0; s < post; ++s) {
             const bool last_dot = f && dot_partials && s == post - 1;
+            if constexpr (sizeof(T) == 4) {
+                if (f && brick_smoother) {
+                    const dim3 gb(brick_grid(L.nbricks));
+                    if (last_dot)
+                        K.go("post_local_dot_L0", 0, 8 * es, -usave(C, 0, 5 * es), gb, dim3(ROW4_THREADS),
+                             k_post_local4<1>, L.desc, A, id, rhs, (const T *)cur, oth, Q, P.red);
+                    else
+                        K.go("post_local_L0", 0, 8 * es, -usave(C, 0, 5 * es), gb, dim3(ROW4_THREADS),
+                             k_post_local4<0>, L.desc, A, id, rhs, (const T *)cur, oth, Q, P.red);
+                    std::swap(cur, oth);
+                    continue;
+                }
+            }
             jacobi_sweep<T>(C, K, l, last_dot, L.desc, A, id, rhs, (const T *)cur, oth, om, Q, P.red);
             std::swap(cur, oth);
         }
@@ -895,9 +925,9 @@ template <class T>
 static std::string graph_key(shkz_ctx *C, const Problem &P, const void *const *vecs, int nvec, bool precond) {
     std::string k;
     char buf[256];
-    std::snprintf(buf, sizeof(buf), "T%zu pc%d g%d nl%d pre%d post%d prc%d poc%d cs%d mm%d om%.17g red%p|", sizeof(T),
-                  precond ? 1 : 0, P.ngrids, P.nlev, P.pre, P.post, P.pre_c, P.post_c, P.coarse_sweeps,
-                  P.multi_max_bricks, P.omega, (void *)P.red.partials);
+    std::snprintf(buf, sizeof(buf), "T%zu pc%d g%d nl%d pre%d post%d prc%d poc%d cs%d mm%d sm%d om%.17g red%p|",
+                  sizeof(T), precond ? 1 : 0, P.ngrids, P.nlev, P.pre, P.post, P.pre_c, P.post_c, P.coarse_sweeps,
+                  P.multi_max_bricks, P.smoother, P.omega, (void *)P.red.partials);
     k += buf;
     for (int i = 0; i < nvec; ++i) {
         std::snprintf(buf, sizeof(buf), "%p,", vecs[i]);
@@ -907,12 +937,12 @@ static std::string graph_key(shkz_ctx *C, const Problem &P, const void *const *v
     for (int l = 0; l < nl; ++l) {
         const LevelHost &L = C->levels[l];
         // the transfer kernels branch on flat (bilinear in 2D) and the reductions on ngrids
-        std::snprintf(buf, sizeof(buf), "|%d %d f%d g%d %p %p %p %p %p %p %p %p %p %p %p %p %p %p %p %p %p",
+        std::snprintf(buf, sizeof(buf), "|%d %d f%d g%d %p %p %p %p %p %p %p %p %p %p %p %p %p %p %p %p %p %p",
                       L.nbricks, L.any_coarsest ? 1 : 0, L.desc.flat, L.desc.ngrids, (const void *)L.desc.bmap, L.dir,
                       L.gx, L.gy, L.gz, L.idiag, L.rhs, L.z,
                       L.tmp, L.res, (const void *)L.desc.bxyz, (const void *)L.desc.meta, (const void *)L.desc.gl,
                       (const void *)L.desc.child64, (const void *)L.desc.parent8, (const void *)L.desc.uni,
-                      (const void *)L.desc.nb27);
+                      (const void *)L.desc.nb27, (const void *)L.ccode);
         k += buf;
     }
     return k;
@@ -976,14 +1006,19 @@ static int krylov(shkz_ctx *C, Launcher &K, Problem &P, T *x, T *r, T *p, T *q, 
         if constexpr (sizeof(T) == 4) {
             const bool zfirst = P.nlev > 1;
             const LevelHost &F0 = C->levels[0];
-            if (zfirst)
+            const OpDesc<float> A32 = op_of<float>(F0);
+            if (zfirst && P.smoother == SHKZ_SMOOTH_BRICK)
+                K.go("cg_update_z", 0, 52, -usave(C, 0, 16), g, dim3(ROW4_THREADS), k_cg_update4<2>, L0.desc, x64, r,
+                     (const T *)pc, (const T *)q, (const SolveScalars *)S, (const BatchCtl *)ctl, P.red,
+                     (const T *)F0.idiag, z, (T)P.omega, A32);
+            else if (zfirst)
                 K.go("cg_update_z", 0, 40, -usave(C, 0, 4), g, dim3(ROW4_THREADS), k_cg_update4<1>, L0.desc, x64, r,
                      (const T *)pc, (const T *)q, (const SolveScalars *)S, (const BatchCtl *)ctl, P.red,
-                     (const T *)F0.idiag, z, (T)P.omega);
+                     (const T *)F0.idiag, z, (T)P.omega, A32);
             else
                 K.go("cg_update", 0, 32, 0, g, dim3(ROW4_THREADS), k_cg_update4<0>, L0.desc, x64, r, (const T *)pc,
                      (const T *)q, (const SolveScalars *)S, (const BatchCtl *)ctl, P.red, (const T *)nullptr,
-                     (T *)nullptr, (T)0);
+                     (T *)nullptr, (T)0, A32);
             finalize(K, P, L0.desc, S, ctl, FIN_UPDATE);
             const T *zz = vcycle<T>(C, K, P, r, z, tmp, Q, fused_dot, zfirst);
             if (!fused_dot)
@@ -1018,7 +1053,7 @@ static int krylov(shkz_ctx *C, Launcher &K, Problem &P, T *x, T *r, T *p, T *q, 
         if constexpr (sizeof(T) == 4)
             K.go("cg_update", 0, 32, 0, g, dim3(ROW4_THREADS), k_cg_update4<0>, L0.desc, x64, r, (const T *)p,
                  (const T *)q, (const SolveScalars *)S, (const BatchCtl *)ctl, P.red, (const T *)nullptr, (T *)nullptr,
-                 (T)0);
+                 (T)0, op_of<float>(L0));
         else
             K.go("cg_update", 0, 6 * es, 0, g, b, k_cg_update<T>, L0.desc, x, r, (const T *)p, (const T *)q,
                  (const SolveScalars *)S, (const BatchCtl *)ctl, P.red);
@@ -1167,8 +1202,8 @@ static std::string tune_key(const shkz_ctx *C, const Problem &P, const shkz_para
 // Key of a solve's iteration history: the shape class and the controls in use.
 static std::string hist_key(const shkz_ctx *C, const Problem &P, const shkz_params *prm) {
     char buf[160];
-    std::snprintf(buf, sizeof(buf), "|%d %d %d %d %d %d %.6g %.3g", P.pre, P.post, P.pre_c, P.post_c, P.coarse_sweeps,
-                  P.multi_max_bricks, P.omega, P.eta);
+    std::snprintf(buf, sizeof(buf), "|%d %d %d %d %d %d %d %.6g %.3g", P.pre, P.post, P.pre_c, P.post_c,
+                  P.coarse_sweeps, P.multi_max_bricks, P.smoother, P.omega, P.eta);
     return tune_key(C, P, prm) + buf;
 }
 
@@ -1326,10 +1361,11 @@ static int solve_phase(shkz_ctx *C, Launcher &K, Problem &P, const shkz_params *
 struct Knobs {
     int pre, post, pre_c, post_c, coarse, multi;
     double omega, eta;
+    int smoother;
 };
 
 static Knobs knobs_of(const Problem &P) {
-    return Knobs{P.pre, P.post, P.pre_c, P.post_c, P.coarse_sweeps, P.multi_max_bricks, P.omega, P.eta};
+    return Knobs{P.pre, P.post, P.pre_c, P.post_c, P.coarse_sweeps, P.multi_max_bricks, P.omega, P.eta, P.smoother};
 }
 
 // Sets P's controls to k except where the caller fixed a control in prm.
@@ -1341,6 +1377,7 @@ static void apply_knobs(Problem &P, const Knobs &k, const shkz_params *prm) {
     if (prm->mg_coarse_sweeps <= 0) P.coarse_sweeps = k.coarse;
     if (!(prm->mg_omega > 0)) P.omega = k.omega;
     if (!(prm->inner_residual > 0)) P.eta = k.eta;
+    if (prm->mg_smoother <= 0 && (k.smoother == SHKZ_SMOOTH_JACOBI || P.mixed)) P.smoother = k.smoother;
     P.multi_max_bricks = k.multi;
 }
 
@@ -1364,10 +1401,10 @@ static void tune_load_locked() {
     while (std::fgets(line, sizeof(line), f)) {
         char key[320];
         Knobs k;
-        if (std::sscanf(line, "%319s %d %d %d %d %d %d %lf %lf", key, &k.pre, &k.post, &k.pre_c, &k.post_c, &k.coarse,
-                        &k.multi, &k.omega, &k.eta) == 9 &&
+        if (std::sscanf(line, "%319s %d %d %d %d %d %d %lf %lf %d", key, &k.pre, &k.post, &k.pre_c, &k.post_c,
+                        &k.coarse, &k.multi, &k.omega, &k.eta, &k.smoother) == 10 &&
             k.pre > 0 && k.post > 0 && k.pre_c > 0 && k.post_c > 0 && k.coarse > 0 && k.multi >= 0 && k.omega > 0 &&
-            k.omega <= 1.0 && k.eta >= 0)
+            k.omega <= 1.0 && k.eta >= 0 && (k.smoother == SHKZ_SMOOTH_JACOBI || k.smoother == SHKZ_SMOOTH_BRICK))
             g_tune.map[key] = k;
     }
     std::fclose(f);
@@ -1389,8 +1426,8 @@ static void tune_store(const std::string &key, const Knobs &k) {
     if (g_tune.path.empty()) return;
     FILE *f = std::fopen(g_tune.path.c_str(), "a");
     if (!f) return;
-    std::fprintf(f, "%s %d %d %d %d %d %d %.17g %.17g\n", key.c_str(), k.pre, k.post, k.pre_c, k.post_c, k.coarse,
-                 k.multi, k.omega, k.eta);
+    std::fprintf(f, "%s %d %d %d %d %d %d %.17g %.17g %d\n", key.c_str(), k.pre, k.post, k.pre_c, k.post_c, k.coarse,
+                 k.multi, k.omega, k.eta, k.smoother);
     std::fclose(f);
 }
 
@@ -1403,7 +1440,7 @@ static std::string tune_key(const shkz_ctx *C, const Problem &P, const shkz_para
     const int gclass = P.ngrids == 1 ? 0 : (int)std::ceil(std::log2((double)P.ngrids) / 3.0);
     const int fixed = (prm->mg_pre_sweeps > 0) | (prm->mg_post_sweeps > 0) << 1 | (prm->mg_coarse_pre_sweeps > 0) << 2 |
                       (prm->mg_coarse_post_sweeps > 0) << 3 | (prm->mg_coarse_sweeps > 0) << 4 |
-                      (prm->mg_omega > 0) << 5 | (prm->inner_residual > 0) << 6;
+                      (prm->mg_omega > 0) << 5 | (prm->inner_residual > 0) << 6 | (prm->mg_smoother > 0) << 7;
     char buf[320];
     std::snprintf(buf, sizeof(buf), "%s|s%d|p%d|n%d|r%d|b%d|l%d|g%d|f%d", C->device_key.c_str(), prm->solver,
                   prm->precision, prm->norm, rclass, bclass, P.nlev, gclass, fixed);
@@ -1451,9 +1488,18 @@ static int autotune(shkz_ctx *C, Launcher &K, Problem &P, const shkz_params *prm
         }
         return SHKZ_OK;
     };
-    if (prm->mg_pre_sweeps <= 0 && prm->mg_post_sweeps <= 0)
-        for (int v : {2, 3, 4, 6}) {
+    const bool sweeps_free = prm->mg_pre_sweeps <= 0 && prm->mg_post_sweeps <= 0;
+    if (P.mixed && prm->mg_smoother <= 0) {
+        // the other smoother family, with its own sweep count
+        Knobs k = cur;
+        k.smoother = cur.smoother == SHKZ_SMOOTH_BRICK ? SHKZ_SMOOTH_JACOBI : SHKZ_SMOOTH_BRICK;
+        if (sweeps_free) k.pre = k.post = k.smoother == SHKZ_SMOOTH_BRICK ? 1 : 3;
+        SHKZ_TRY(consider(k));
+    }
+    if (sweeps_free)
+        for (int v : {1, 2, 3, 4, 6}) {
             if (v == cur.pre && v == cur.post) continue;
+            if (cur.smoother == SHKZ_SMOOTH_BRICK ? v > 2 : v < 2) continue;
             Knobs k = cur;
             k.pre = k.post = v;
             SHKZ_TRY(consider(k));
@@ -1528,10 +1574,14 @@ static int project_stages(shkz_ctx *C, const shkz_grid *grids, int ngrids, const
     P.ngrids = ngrids;
     P.mixed = (prm->solver == SHKZ_SOLVER_MGPCG && prm->precision == SHKZ_PREC_MIXED);
     P.fp64_levels = !P.mixed;
-    P.pre = prm->mg_pre_sweeps > 0 ? prm->mg_pre_sweeps : 4;
-    P.post = prm->mg_post_sweeps > 0 ? prm->mg_post_sweeps : 4;
-    P.pre_c = prm->mg_coarse_pre_sweeps > 0 ? prm->mg_coarse_pre_sweeps : P.pre;
-    P.post_c = prm->mg_coarse_post_sweeps > 0 ? prm->mg_coarse_post_sweeps : P.post;
+    // finest-level smoother: the brick-local one needs the FP32 V-cycle of the mixed mode
+    P.smoother = prm->mg_smoother > 0 ? prm->mg_smoother : (P.mixed ? SHKZ_SMOOTH_BRICK : SHKZ_SMOOTH_JACOBI);
+    if (!P.mixed) P.smoother = SHKZ_SMOOTH_JACOBI;
+    const bool brick = P.smoother == SHKZ_SMOOTH_BRICK;
+    P.pre = prm->mg_pre_sweeps > 0 ? prm->mg_pre_sweeps : (brick ? 1 : 4);
+    P.post = prm->mg_post_sweeps > 0 ? prm->mg_post_sweeps : (brick ? 1 : 4);
+    P.pre_c = prm->mg_coarse_pre_sweeps > 0 ? prm->mg_coarse_pre_sweeps : (brick ? 4 : P.pre);
+    P.post_c = prm->mg_coarse_post_sweeps > 0 ? prm->mg_coarse_post_sweeps : (brick ? 4 : P.post);
     P.coarse_sweeps = prm->mg_coarse_sweeps > 0 ? prm->mg_coarse_sweeps : 48;
     P.omega = prm->mg_omega > 0 ? prm->mg_omega : 0.9;
     // 0: each inner solve goes to the final tolerance (one defect-correction pass unless the FP32
@@ -1922,6 +1972,22 @@ static int project_stages(shkz_ctx *C, const shkz_grid *grids, int ngrids, const
                  (float *)Lc.dir, (float *)Lc.gx, (float *)Lc.gy, (float *)Lc.gz);
         }
     }
+    // components of the coarsest bricks (the coarse solve makes a pure-Neumann component's
+    // right-hand side consistent)
+    if (multigrid) {
+        for (int l = 0; l < P.nlev; ++l) {
+            LevelHost &L = C->levels[l];
+            L.ccode = nullptr;
+            if (!L.any_coarsest) continue;
+            SHKZ_TRY(arena_get(AR, "ccode_l" + std::to_string(l), (size_t)L.nbricks * BRICK_CELLS, &L.ccode));
+            if (P.fp64_levels)
+                K.go("coarse_components", -1, 0, 0, dim3((unsigned)P.ngrids), dim3(BRICK_CELLS),
+                     k_coarse_components<double>, L.desc, op_of<double>(L), L.ccode);
+            else
+                K.go("coarse_components", -1, 0, 0, dim3((unsigned)P.ngrids), dim3(BRICK_CELLS),
+                     k_coarse_components<float>, L.desc, op_of<float>(L), L.ccode);
+        }
+    }
     // inverse diagonals for the smoothers
     if (multigrid) {
         for (int l = 0; l < P.nlev; ++l) {
@@ -2037,6 +2103,7 @@ static int project_stages(shkz_ctx *C, const shkz_grid *grids, int ngrids, const
         res[g].mg_coarse_post_sweeps = P.post_c;
         res[g].mg_coarse_sweeps = P.coarse_sweeps;
         res[g].mg_omega = P.omega;
+        res[g].mg_smoother = P.smoother;
         res[g].inner_residual = P.eta;
         res[g].tuned = tuned;
     }
