@@ -1190,8 +1190,11 @@ __global__ __launch_bounds__(ROW4_THREADS) void k_jacobi4_dot(LevelDesc L, OpDes
 // (its couplings to cells in other bricks dropped, the diagonal whole), i.e. the half-sweeps red,
 // black, red, ..., red.  Across bricks it is block Jacobi; inside a brick, Gauss-Seidel.  M_b is
 // symmetric, so the V-cycle built on it is a symmetric preconditioner.  One pass over the level's
-// arrays per application, like one Jacobi sweep, smoothing about as much as three.
-constexpr int LOCAL_SGS_SWEEPS = 2;
+// arrays per application, like one Jacobi sweep, smoothing about as much as three.  The coarse
+// levels use it too (three sweeps per application): there the pre-smoothing from zero runs inside
+// the restriction that forms the level's right-hand side (k_restrict_pre).
+constexpr int LOCAL_SGS_SWEEPS = 2;         // finest level
+constexpr int LOCAL_SGS_SWEEPS_COARSE = 3;  // coarse levels (one application per smoothing step)
 
 // The couplings of the lane's four cells to cells of the same brick (0 across the brick's faces):
 // from the loaded Row4Coef of the level (whose high couplings may come from neighbour bricks).
@@ -1218,36 +1221,63 @@ __device__ __forceinline__ Row4Coef<float> row4_local_uniform(float g, int q) {
 }
 
 // e = M_b r on the brick of a 128-thread workgroup (lane q: cells 4q .. 4q+3 of the brick, the
-// storage order).  lds: 512 floats.  Every lane of the workgroup must call it (barriers).
+// storage order).  lds: 512 floats.  Every lane of the workgroup must call it (barriers).  The
+// half-sweep loop is unrolled and each cell's value is a plain register (a select per half-sweep),
+// so nothing spills to scratch.
+template <int K>
 __device__ __forceinline__ void local_sgs4(const Row4Coef<float> &c, const float id[4], const float r[4], float e[4],
                                            float *lds, int q) {
     const int li = q >> 4, lj = (q >> 1) & 7, h = q & 1;
     const int base = 4 * q;
     const int col0 = (li + lj) & 1;  // colour of the lane's cell j is col0 ^ (j & 1) (4h is even)
+    float e0, e1, e2, e3;
     // red half-sweep from zero: no neighbour values yet
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-        e[j] = ((col0 ^ (j & 1)) == 0) ? id[j] * r[j] : 0.0f;
-        if ((col0 ^ (j & 1)) == 0) lds[base + j] = e[j];
+    {
+        const bool r0 = col0 == 0;  // cells 0 and 2 are red when col0 == 0, cells 1 and 3 otherwise
+        e0 = r0 ? id[0] * r[0] : 0.0f;
+        e1 = r0 ? 0.0f : id[1] * r[1];
+        e2 = r0 ? id[2] * r[2] : 0.0f;
+        e3 = r0 ? 0.0f : id[3] * r[3];
+        if (r0) {
+            lds[base + 0] = e0;
+            lds[base + 2] = e2;
+        } else {
+            lds[base + 1] = e1;
+            lds[base + 3] = e3;
+        }
     }
     __syncthreads();
-#pragma unroll 1
-    for (int s = 1; s <= 2 * LOCAL_SGS_SWEEPS; ++s) {
-        const int colour = s & 1;
+    auto cell = [&](int j, float cur) -> float {
+        const int at = base + j, lk = 4 * h + j;
+        const float xm = li > 0 ? lds[at - 64] : 0.0f, xp = li < 7 ? lds[at + 64] : 0.0f;
+        const float ym = lj > 0 ? lds[at - 8] : 0.0f, yp = lj < 7 ? lds[at + 8] : 0.0f;
+        const float zm = lk > 0 ? lds[at - 1] : 0.0f, zp = lk < 7 ? lds[at + 1] : 0.0f;
+        const float nb = c.gxm[j] * xm + c.gxp[j] * xp + c.gym[j] * ym + c.gyp[j] * yp + c.gzm[j] * zm + c.gzp[j] * zp;
+        (void)cur;
+        return id[j] * (r[j] + nb);
+    };
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            if ((col0 ^ (j & 1)) != colour) continue;
-            const int at = base + j, lk = 4 * h + j;
-            const float xm = li > 0 ? lds[at - 64] : 0.0f, xp = li < 7 ? lds[at + 64] : 0.0f;
-            const float ym = lj > 0 ? lds[at - 8] : 0.0f, yp = lj < 7 ? lds[at + 8] : 0.0f;
-            const float zm = lk > 0 ? lds[at - 1] : 0.0f, zp = lk < 7 ? lds[at + 1] : 0.0f;
-            const float nb = c.gxm[j] * xm + c.gxp[j] * xp + c.gym[j] * ym + c.gyp[j] * yp + c.gzm[j] * zm +
-                             c.gzp[j] * zp;
-            e[j] = id[j] * (r[j] + nb);
-            lds[at] = e[j];
+    for (int s = 1; s <= 2 * K; ++s) {
+        // this half-sweep updates the cells of colour s & 1: cells 0 and 2 when it equals col0
+        // (a half-sweep reads only cells of the other colour, so its stores race with no read)
+        const bool even = (s & 1) == col0;
+        if (even) {
+            e0 = cell(0, e0);
+            e2 = cell(2, e2);
+            lds[base + 0] = e0;
+            lds[base + 2] = e2;
+        } else {
+            e1 = cell(1, e1);
+            e3 = cell(3, e3);
+            lds[base + 1] = e1;
+            lds[base + 3] = e3;
         }
         __syncthreads();
     }
+    e[0] = e0;
+    e[1] = e1;
+    e[2] = e2;
+    e[3] = e3;
 }
 
 // Pre-smoothing from zero on the finest level: z = M_b rhs (one brick per 128-thread workgroup).
@@ -1274,14 +1304,14 @@ __global__ __launch_bounds__(ROW4_THREADS) void k_pre_local4(LevelDesc L, OpDesc
         c = row4_local(row4_coef(A, N), q);
         unpack4(ld4(idiag, N.c), id);
     }
-    local_sgs4(c, id, r, e, lds, q);
+    local_sgs4<LOCAL_SGS_SWEEPS>(c, id, r, e, lds, q);
     *reinterpret_cast<float4 *>(z + N.c) = make_float4(e[0], e[1], e[2], e[3]);
 }
 
 // Post-smoothing (and further pre-smoothing) on the finest level: zout = zin + M_b (rhs - A zin);
 // DOT: also the rhs.zout partials of the unknowns (the last post-smoothing pass yields r.z of the
 // preconditioned CG).  One brick per 128-thread workgroup.
-template <int DOT>
+template <int DOT, int K = LOCAL_SGS_SWEEPS>
 __global__ __launch_bounds__(ROW4_THREADS) void k_post_local4(LevelDesc L, OpDesc<float> A,
                                                               const float *__restrict__ idiag,
                                                               const float *__restrict__ rhs,
@@ -1327,7 +1357,7 @@ __global__ __launch_bounds__(ROW4_THREADS) void k_post_local4(LevelDesc L, OpDes
         }
         c = row4_local(g, q);
     }
-    local_sgs4(c, id, res, e, lds, q);
+    local_sgs4<K>(c, id, res, e, lds, q);
     float out[4];
     double part = 0.0;
 #pragma unroll
@@ -1616,7 +1646,7 @@ __global__ __launch_bounds__(ROW4_THREADS) void k_cg_update4(LevelDesc L, double
         // the brick-local smoother's pre-smoothing from zero (k_pre_local4) on the updated residual
         __shared__ float zl[BRICK_CELLS];
         float e[4];
-        local_sgs4(cf, id, rv, e, zl, threadIdx.x);
+        local_sgs4<LOCAL_SGS_SWEEPS>(cf, id, rv, e, zl, threadIdx.x);
         *reinterpret_cast<float4 *>(z + c) = make_float4(e[0], e[1], e[2], e[3]);
     }
     row4_reduce2<true>(mx, sq, lds);
@@ -1726,6 +1756,104 @@ __global__ __launch_bounds__(BLOCK) void k_residual(LevelDesc L, OpDesc<T> A, co
 // 2I-1..2I+2 per axis with weights 1/4, 3/4, 3/4, 1/4.  One block per coarse brick B stages the
 // 18^3 fine residuals it needs (fine cells 16B-1 .. 16B+16 per axis, in the fine bricks 2B-1 ..
 // 2B+2 that the brick's child table lists) in LDS.
+// M_b r for one cell per thread of a 512-thread workgroup holding one brick (cell t, storage
+// order): K symmetric red-black Gauss-Seidel sweeps from zero on the brick's own operator (c: the
+// cell's couplings to cells of the brick, 0 across its faces), as local_sgs4.  lds: 512 floats.
+template <int K>
+__device__ __forceinline__ float local_sgs1(const float c[6], float id, float r, float *lds, int t) {
+    const int li = t >> 6, lj = (t >> 3) & 7, lk = t & 7;
+    const int colour = (li + lj + lk) & 1;
+    float e = colour == 0 ? id * r : 0.0f;
+    if (colour == 0) lds[t] = e;
+    __syncthreads();
+#pragma unroll
+    for (int s = 1; s <= 2 * K; ++s) {
+        if (colour == (s & 1)) {
+            const float xm = li > 0 ? lds[t - 64] : 0.0f, xp = li < 7 ? lds[t + 64] : 0.0f;
+            const float ym = lj > 0 ? lds[t - 8] : 0.0f, yp = lj < 7 ? lds[t + 8] : 0.0f;
+            const float zm = lk > 0 ? lds[t - 1] : 0.0f, zp = lk < 7 ? lds[t + 1] : 0.0f;
+            const float nb = c[0] * xm + c[1] * xp + c[2] * ym + c[3] * yp + c[4] * zm + c[5] * zp;
+            e = id * (r + nb);
+            lds[t] = e;
+        }
+        __syncthreads();
+    }
+    return e;
+}
+
+// k_restrict (FP32) that also runs the coarse level's pre-smoothing from zero with the brick-local
+// smoother: zc = M_b rhs_c on the coarse brick (skipped on bricks of grids that end on the coarse
+// level: their coarse solve starts from the right-hand side).
+template <int FINEST>
+__global__ __launch_bounds__(BLOCK) void k_restrict_pre(LevelDesc Lc, OpDesc<float> Ac, const float *__restrict__ idc,
+                                                        const float *__restrict__ res, float *__restrict__ rhs_c,
+                                                        float *__restrict__ zc, Gate Q) {
+    constexpr int E = 18;
+    __shared__ float tile[E * E * E];
+    __shared__ int bid[64];
+    const int br = brick_or_idle(Lc);
+    if (br < 0) return;
+    const int t = threadIdx.x;
+    const long c = (long)br * BRICK_CELLS + t;
+    const float dirc = Ac.dir[c];
+    const int cid = t < 64 ? Lc.child64[64 * br + t] : -1;
+    const BrickMeta m = Lc.meta[br];
+    if (all_done(Q) || grid_done(Q, Lc, m)) return;
+    if (t < 64) bid[t] = cid;
+    __syncthreads();
+    constexpr int NQ = (E * E * E + BLOCK - 1) / BLOCK;
+    float v[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        const int e = t + q * BLOCK;
+        v[q] = 0.0f;
+        if (e < E * E * E) {
+            const int c3 = e % E, b = (e / E) % E, a = e / (E * E);
+            const int slot = (((a + 7) >> 3) * 4 + ((b + 7) >> 3)) * 4 + ((c3 + 7) >> 3);
+            const int id = bid[slot];
+            if (id >= 0) v[q] = res[(long)id * BRICK_CELLS + ((((a + 7) & 7) * 8 + ((b + 7) & 7)) * 8 + ((c3 + 7) & 7))];
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        const int e = t + q * BLOCK;
+        if (e < E * E * E) tile[e] = v[q];
+    }
+    __syncthreads();
+    const int li = t >> 6, lj = (t >> 3) & 7, lk = t & 7;
+    float out = 0.0f;
+    if (dirc >= 0) {
+        const float w[4] = {0.25f, 0.75f, 0.75f, 0.25f};
+        const float wk[4] = {Lc.flat ? 0.0f : w[0], Lc.flat ? 1.0f : w[1], Lc.flat ? 0.0f : w[2],
+                             Lc.flat ? 0.0f : w[3]};
+        float acc = 0.0f;
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+            float acc_a = 0.0f;
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                const float *row = &tile[((2 * li + a) * E + (2 * lj + b)) * E + 2 * lk];
+                const float acc_b = wk[0] * row[0] + wk[1] * row[1] + wk[2] * row[2] + wk[3] * row[3];
+                acc_a += w[b] * acc_b;
+            }
+            acc += w[a] * acc_a;
+        }
+        out = acc;
+    }
+    rhs_c[c] = out;
+    if (m.flags & BM_COARSEST) return;  // the whole block: the flag is the brick's
+    __syncthreads();                     // the tile becomes the smoother's scratch
+    float cf[6];
+    cf[0] = li > 0 ? Ac.gx[c] : 0.0f;
+    cf[1] = li < 7 ? Ac.gx[c + 64] : 0.0f;
+    cf[2] = lj > 0 ? Ac.gy[c] : 0.0f;
+    cf[3] = lj < 7 ? Ac.gy[c + 8] : 0.0f;
+    cf[4] = lk > 0 ? Ac.gz[c] : 0.0f;
+    cf[5] = lk < 7 ? Ac.gz[c + 1] : 0.0f;
+    const float e = local_sgs1<LOCAL_SGS_SWEEPS_COARSE>(cf, idc[c], out, tile, t);
+    zc[c] = e;
+}
+
 template <class T, int FINEST>
 __global__ __launch_bounds__(BLOCK) void k_restrict(LevelDesc Lc, OpDesc<T> Ac, const T *__restrict__ res,
                                                     T *__restrict__ rhs_c, Gate Q) {
@@ -2133,11 +2261,13 @@ __global__ __launch_bounds__(BRICK_CELLS) void k_coarse_components(LevelDesc L, 
     const bool czm = lk > 0 && A.gz[c] > T(0), czp = lk < 7 && A.gz[c + 1] > T(0);
     lab[e] = unk ? e : BRICK_CELLS;
     __syncthreads();
-    // labels only decrease, so the propagation reaches the unique fixpoint (the smallest index of
-    // each component) whatever the interleaving; at most 512 rounds
+    // labels only decrease and always name a cell of the same component, so the propagation (with
+    // pointer jumping) reaches the unique fixpoint, the smallest index of each component; at most
+    // 512 rounds
     for (int it = 0; it < BRICK_CELLS; ++it) {
         int m = lab[e];
         if (unk) {
+            m = min(m, lab[m]);  // pointer jumping: the label of the cell this one points to
             if (cxm) m = min(m, lab[e - 64]);
             if (cxp) m = min(m, lab[e + 64]);
             if (cym) m = min(m, lab[e - 8]);

@@ -724,9 +724,10 @@ static T *vcycle(shkz_ctx *C, Launcher &K, Problem &P, const T *rhs0, T *z0, T *
     std::vector<T *> zcur(nl), zoth(nl);
     const T om = (T)P.omega;
     const double es = (double)sizeof(T);
-    // coarse levels small enough smooth with one multi-sweep launch per smoothing step
+    // coarse levels small enough smooth with one multi-sweep launch per smoothing step (Jacobi);
+    // with the brick-local smoother every level uses it, whatever its size
     std::vector<char> mpre(nl, 0), mpost(nl, 0);
-    for (int l = 1; l < nl; ++l) {
+    for (int l = 1; l < nl && !brick_smoother; ++l) {
         const bool small = C->levels[l].nbricks <= P.multi_max_bricks;
         mpre[l] = small && smooth_k_supported<T>(P.pre_c);
         mpost[l] = small && smooth_k_supported<T>(P.post_c);
@@ -745,7 +746,8 @@ static T *vcycle(shkz_ctx *C, Launcher &K, Problem &P, const T *rhs0, T *z0, T *
         const bool last = l == nl - 1;
         if (L.any_coarsest) {
             // grids ending here: their z goes where the other grids' smoothed z ends up
-            T *dst = (last || final_slot(pre, post, mpre[l], mpost[l], f) == 0) ? z : tmp;
+            // (the brick-local smoother swaps as the finest level's Jacobi does on every level)
+            T *dst = (last || final_slot(pre, post, mpre[l], mpost[l], f || brick_smoother) == 0) ? z : tmp;
             // one cell per thread: measured faster per sweep than two (19.3 against 13 us for 48 sweeps)
             constexpr int CPT = 1;
             const dim3 cb(BRICK_CELLS / CPT);
@@ -768,16 +770,21 @@ static T *vcycle(shkz_ctx *C, Launcher &K, Problem &P, const T *rhs0, T *z0, T *
             res_done = smooth_k_res_supported<T>(pre);
             smooth_k<T>(K, l, pre, true, L.desc, A, id, rhs, (const T *)nullptr, z, om, Q,
                         res_done ? (T *)L.res : nullptr);
-        } else if (f && brick_smoother) {
-            // brick-local smoother: z = M_b rhs from zero (done by the residual update when
-            // first_done), then further applications z += M_b (rhs - A z)
+        } else if (brick_smoother) {
+            // brick-local smoother: z = M_b rhs from zero (on the finest level done by the residual
+            // update when first_done; on coarse levels by the restriction that formed rhs), then
+            // further applications z += M_b (rhs - A z)
             if constexpr (sizeof(T) == 4) {
-                if (!first_done)
+                if (f && !first_done)
                     K.go("pre_local_L0", 0, 6 * es, -usave(C, 0, 4 * es), dim3(brick_grid(L.nbricks)), dim3(ROW4_THREADS),
                          k_pre_local4, L.desc, A, id, rhs, z, Q);
                 for (int s = 1; s < pre; ++s) {
-                    K.go("post_local_L0", 0, 8 * es, -usave(C, 0, 5 * es), dim3(brick_grid(L.nbricks)),
-                         dim3(ROW4_THREADS), k_post_local4<0>, L.desc, A, id, rhs, (const T *)cur, oth, Q, P.red);
+                    if (f)
+                        K.go("post_local_L0", 0, 8 * es, -usave(C, 0, 5 * es), dim3(brick_grid(L.nbricks)),
+                             dim3(ROW4_THREADS), k_post_local4<0>, L.desc, A, id, rhs, (const T *)cur, oth, Q, P.red);
+                    else
+                        K.go("post_local_Lc", l, 8 * es, 0, dim3(brick_grid(L.nbricks)), dim3(ROW4_THREADS),
+                             k_post_local4<0, LOCAL_SGS_SWEEPS_COARSE>, L.desc, A, id, rhs, (const T *)cur, oth, Q, P.red);
                     std::swap(cur, oth);
                 }
             }
@@ -815,6 +822,17 @@ static T *vcycle(shkz_ctx *C, Launcher &K, Problem &P, const T *rhs0, T *z0, T *
                      f ? k_residual<T, 1> : k_residual<T, 0>, L.desc, A, rhs, (const T *)cur, (T *)L.res, Q);
         }
         LevelHost &Lc = C->levels[l + 1];
+        if constexpr (sizeof(T) == 4) {
+            if (brick_smoother) {
+                // the coarse level's pre-smoothing from zero joins its restriction
+                K.go(f ? "restrict_pre_L0" : "restrict_pre_Lc", l, es, 4 * es * (double)Lc.unknowns,
+                     dim3(brick_grid(Lc.nbricks)), b, f ? k_restrict_pre<1> : k_restrict_pre<0>, Lc.desc, op_of<T>(Lc),
+                     (const T *)Lc.idiag, (const T *)L.res, (T *)Lc.rhs, (T *)Lc.z, Q);
+                zcur[l] = cur;
+                zoth[l] = oth;
+                continue;
+            }
+        }
         K.go(f ? "restrict_L0" : "restrict_Lc", l, es, 2 * es * (double)Lc.unknowns, dim3(brick_grid(Lc.nbricks)), b,
              f ? k_restrict<T, 1> : k_restrict<T, 0>, Lc.desc, op_of<T>(Lc), (const T *)L.res, (T *)Lc.rhs, Q);
         zcur[l] = cur;
@@ -850,14 +868,18 @@ static T *vcycle(shkz_ctx *C, Launcher &K, Problem &P, const T *rhs0, T *z0, T *
         for (int s = 0; s < post; ++s) {
             const bool last_dot = f && dot_partials && s == post - 1;
             if constexpr (sizeof(T) == 4) {
-                if (f && brick_smoother) {
+                if (brick_smoother) {
                     const dim3 gb(brick_grid(L.nbricks));
                     if (last_dot)
                         K.go("post_local_dot_L0", 0, 8 * es, -usave(C, 0, 5 * es), gb, dim3(ROW4_THREADS),
                              k_post_local4<1>, L.desc, A, id, rhs, (const T *)cur, oth, Q, P.red);
-                    else
+                    else if (f)
                         K.go("post_local_L0", 0, 8 * es, -usave(C, 0, 5 * es), gb, dim3(ROW4_THREADS),
                              k_post_local4<0>, L.desc, A, id, rhs, (const T *)cur, oth, Q, P.red);
+                    else
+                        K.go("post_local_Lc", l, 8 * es, 0, gb, dim3(ROW4_THREADS),
+                             k_post_local4<0, LOCAL_SGS_SWEEPS_COARSE>, L.desc, A, id, rhs, (const T *)cur, oth, Q,
+                             P.red);
                     std::swap(cur, oth);
                     continue;
                 }
@@ -1494,6 +1516,8 @@ static int autotune(shkz_ctx *C, Launcher &K, Problem &P, const shkz_params *prm
         Knobs k = cur;
         k.smoother = cur.smoother == SHKZ_SMOOTH_BRICK ? SHKZ_SMOOTH_JACOBI : SHKZ_SMOOTH_BRICK;
         if (sweeps_free) k.pre = k.post = k.smoother == SHKZ_SMOOTH_BRICK ? 1 : 3;
+        if (prm->mg_coarse_pre_sweeps <= 0 && prm->mg_coarse_post_sweeps <= 0)
+            k.pre_c = k.post_c = k.smoother == SHKZ_SMOOTH_BRICK ? 1 : 4;
         SHKZ_TRY(consider(k));
     }
     if (sweeps_free)
@@ -1505,8 +1529,9 @@ static int autotune(shkz_ctx *C, Launcher &K, Problem &P, const shkz_params *prm
             SHKZ_TRY(consider(k));
         }
     if (prm->mg_coarse_pre_sweeps <= 0 && prm->mg_coarse_post_sweeps <= 0 && P.nlev > 2)
-        for (int v : {2, 3, 4, 5, 6}) {
+        for (int v : {1, 2, 3, 4, 5, 6}) {
             if (v == cur.pre_c && v == cur.post_c) continue;
+            if (cur.smoother == SHKZ_SMOOTH_BRICK ? v > 2 : v < 2) continue;
             Knobs k = cur;
             k.pre_c = k.post_c = v;
             SHKZ_TRY(consider(k));
@@ -1580,8 +1605,8 @@ static int project_stages(shkz_ctx *C, const shkz_grid *grids, int ngrids, const
     const bool brick = P.smoother == SHKZ_SMOOTH_BRICK;
     P.pre = prm->mg_pre_sweeps > 0 ? prm->mg_pre_sweeps : (brick ? 1 : 4);
     P.post = prm->mg_post_sweeps > 0 ? prm->mg_post_sweeps : (brick ? 1 : 4);
-    P.pre_c = prm->mg_coarse_pre_sweeps > 0 ? prm->mg_coarse_pre_sweeps : (brick ? 4 : P.pre);
-    P.post_c = prm->mg_coarse_post_sweeps > 0 ? prm->mg_coarse_post_sweeps : (brick ? 4 : P.post);
+    P.pre_c = prm->mg_coarse_pre_sweeps > 0 ? prm->mg_coarse_pre_sweeps : (brick ? 1 : P.pre);
+    P.post_c = prm->mg_coarse_post_sweeps > 0 ? prm->mg_coarse_post_sweeps : (brick ? 1 : P.post);
     P.coarse_sweeps = prm->mg_coarse_sweeps > 0 ? prm->mg_coarse_sweeps : 48;
     P.omega = prm->mg_omega > 0 ? prm->mg_omega : 0.9;
     // 0: each inner solve goes to the final tolerance (one defect-correction pass unless the FP32
