@@ -41,7 +41,9 @@ def log(*a):
 
 
 SYMBOL = {"jacobi_L0": "k_jacobi4<1>", "jacobi_dot_L0": "k_jacobi4_dot",
-          "residual_L0": "k_residual4<1>", "smooth2_L0": "k_smooth2<float, 1>"}
+          "residual_L0": "k_residual4<1>", "smooth2_L0": "k_smooth2<float, 1>",
+          "cg_update_z": "k_cg_update4<2>", "post_local_L0": "k_post_local4<0, 2>",
+          "post_local_dot_L0": "k_post_local4<1, 2>", "apply_dot_p": "k_apply_dot4p"}
 
 
 def measured_traffic(kernel, config):
@@ -624,6 +626,13 @@ def main():
     roofline = kernel_roofline(stats, inst_steps, args.config)
 
     value = args.steps / elapsed
+    if roofline and stats:
+        # the whole projection: every kernel's algorithmic bytes per step over the timed step
+        step_bytes = sum(v["bytes"] for v in stats.values()) / inst_steps
+        gbs = step_bytes / (elapsed / args.steps) / 1e9
+        roofline["projection"] = {"algorithmic_bytes_per_step": int(step_bytes), "achieved": round(gbs, 1),
+                                  "frac": round(gbs / HBM_PEAK_GBS, 4),
+                                  "what": "all kernels' algorithmic bytes per projection / wall time per projection"}
     out = {
         "metric": "pressure solves/sec @256^3 to 1e-6 rel-res" if args.config == "c3" else
                   f"pressure solves/sec @{n}^3 to {residual:g} rel-res",
