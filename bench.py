@@ -416,13 +416,14 @@ def bench_post(args):
     post-projection activity, all resident in HBM: velocity extrapolation over `--post-width`
     passes and the solid constraint (shkz_extrapolate_velocity_device), then MacCormack advection
     of the liquid level set and of the velocity (shkz_advect_scalar_device,
-    shkz_advect_velocity_device) with dt at CFL 2.  Each step restores the inputs from device
-    copies (inside the timed region)."""
+    shkz_advect_velocity_device) with dt at CFL 2, trilinear or (--weno) WENO sampling.  Each step
+    restores the inputs from device copies (inside the timed region)."""
     import shiokaze_amd as S
     from shiokaze_amd.devmem import DeviceArray, Stream, device_synchronize
     kind_name, n, seed, residual = CONFIGS["c3"]
     width = args.post_width
-    log(f"[bench] post-projection step: c3 {n}^3 seed={seed}, width {width}")
+    weno = bool(args.weno)
+    log(f"[bench] post-projection step: c3 {n}^3 seed={seed}, width {width}, {'WENO' if weno else 'trilinear'}")
     sc = S.make_scene(S.SCENE_SPHERES_BOX, n, seed=seed)
     shapes = [(n + 1, n, n), (n, n + 1, n), (n, n, n + 1)]
     d_fluid, d_solid = DeviceArray(sc.fluid), DeviceArray(sc.solid)
@@ -448,8 +449,8 @@ def bench_post(args):
             a.copy_from(b, st.handle)
         ctx.extrapolate_velocity_device(n, n, n, vel, act, solid=d_solid, solid_dtype=S.F32, width=width,
                                         stream=st.handle)
-        ctx.advect_scalar_device(n, n, n, vel, act, phi, dt, stream=st.handle)
-        ctx.advect_velocity_device(n, n, n, vel, act, phi, S.F32, dt, stream=st.handle)
+        ctx.advect_scalar_device(n, n, n, vel, act, phi, dt, stream=st.handle, weno=weno)
+        ctx.advect_velocity_device(n, n, n, vel, act, phi, S.F32, dt, stream=st.handle, weno=weno)
 
     for _ in range(args.warmup):
         step()
@@ -473,21 +474,40 @@ def bench_post(args):
         ctx.set_kernel_timing(False)
     roofline = kernel_roofline(stats, inst, "post")
     faces = sum(a * b * c for a, b, c in shapes)
+    weno_fp64 = None
+    if weno and stats:
+        # FP64 roofline of the WENO kernels: one sample is 43 six-point interpolations (36 along x,
+        # 6 along y, 1 along z) of 174 flops each (divisions counted once) plus the linear weights
+        # of the three axes (36 flops); a pass samples each active entry whose velocity is not zero
+        # and, backward, outside the narrow band -- counted here as every active face and every
+        # cell, an upper bound
+        wk = {k: v for k, v in stats.items() if k.endswith("_weno")}
+        t_ms = sum(v["total_ms"] for v in wk.values()) / inst
+        active_after = int(sum(int(a.download().sum()) for a in act))
+        samples = 2 * (active_after + n * n * n)
+        flops = samples * (43 * 174 + 36)
+        weno_fp64 = {"bound": "fp64", "kernels": sorted(wk), "ms_per_step": round(t_ms, 4),
+                     "flops_per_step_upper": int(flops), "achieved_upper": round(flops / (t_ms * 1e-3) / 1e12, 2),
+                     "peak": 78.6, "unit": "TFLOP/s",
+                     "frac_upper": round(flops / (t_ms * 1e-3) / 1e12 / 78.6, 4)}
     stage_ms = {}
     for k, v in stats.items():
         stage_ms[k] = round(v["total_ms"] / max(inst, 1), 4)
+    interp = "WENO" if weno else "trilinear"
     out = {"metric": f"post-projection steps/sec @{n}^3 (extrapolation width {width}, solid constraint, "
-                     f"level-set and velocity advection)",
+                     f"level-set and velocity advection" + (", WENO)" if weno else ")"),
            "value": round(args.steps / elapsed, 3), "unit": "steps/s", "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "ms_per_step": round(elapsed / args.steps * 1e3, 3), "higher_is_better": True,
            "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
            "config": {"workload": f"{n}^3 ({kind_name}, seed {seed}) projected velocity: {width} extrapolation "
                                   f"passes, the solid constraint, MacCormack advection of the level set and the "
-                                  f"velocity at CFL 2; inputs restored on the device each step",
+                                  f"velocity at CFL 2 ({interp} sampling); inputs restored on the device each step",
                       "faces": int(faces), "faces_activated": activated, "kernel_ms_per_step": stage_ms,
                       "active_faces_before": int(sum(int(a.sum()) for a in act0_host)),
                       "projection_iterations": int(r.iterations)},
            "roofline": roofline, "cpu_baseline": None}
+    if weno_fp64:
+        out["weno_fp64"] = weno_fp64
     if not args.no_cpu_baseline:
         from oracle import oracle as O
         vh = [v.copy() for v in vel0_host]
@@ -496,8 +516,8 @@ def bench_post(args):
         t = time.perf_counter()
         O.extrapolate(n, n, n, vh, ah, width)
         O.constrain_velocity(n, n, n, vh, ah, sc.solid)
-        O.advect_scalar(n, n, n, vh, ah, ph, dt)
-        O.advect_velocity(n, n, n, vh, ah, ph, dt)
+        O.advect_scalar(n, n, n, vh, ah, ph, dt, weno=weno)
+        O.advect_velocity(n, n, n, vh, ah, ph, dt, weno=weno)
         el = time.perf_counter() - t
         threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count()
         out["cpu_baseline"] = {"value": round(1.0 / el, 4), "unit": "steps/s", "cores": threads, "kind": "port",
@@ -514,6 +534,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--config", default="c3", choices=sorted(CONFIGS) + ["c4", "c5", "post"])
     ap.add_argument("--post-width", type=int, default=3, help="post: extrapolation passes")
+    ap.add_argument("--weno", action="store_true", help="post: WENO sampling in the advection")
     ap.add_argument("--c5-requests", type=int, default=1000, help="served: requests of the stream measured")
     ap.add_argument("--batched-steps", type=int, default=5, help="default line: batched calls timed")
     ap.add_argument("--no-extras", action="store_true",

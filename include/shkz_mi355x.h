@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define SHKZ_ABI_VERSION 6
+#define SHKZ_ABI_VERSION 7
 
 /* ---------------------------------------------------------------- enums */
 enum shkz_status {
@@ -269,8 +269,10 @@ int shkz_extrapolate_velocity_device(shkz_ctx *ctx, const shkz_velocity_field *f
                                      int32_t constrain, void *stream);
 
 /* Advection between projections: macadvection3 (src/advection/macadvection3.cpp) with its
- * defaults (MacCormack, trilinear interpolation, TrimNarrowBand 1; the WENO option is not
- * provided).
+ * defaults (MacCormack, trilinear interpolation, TrimNarrowBand 1) and its options (:56-60):
+ * semi-Lagrangian instead of MacCormack, TrimNarrowBand, and WENO sampling (weno = 1: the value
+ * passes sample with WENO3::interpolate, order 6, include/shiokaze/math/WENO3.h:105-135, in
+ * double; MacCormack's bounds and the narrow-band test stay trilinear, :96-118,207-231).
  *  - shkz_advect_velocity*: advect_vector(u, u, fluid, dt) (:47-53 -> advect_u, :140-177): each
  *    active face moves back along its own full velocity (convert_to_full, macarray3.h:350-372);
  *    MacCormack's correction is clamped to the forward stencil's min/max and replaced by the
@@ -291,7 +293,8 @@ typedef struct shkz_advect_params {
     const void *fluid;            /* liquid level set (cells) for the narrow-band test */
     int32_t fluid_dtype;          /* SHKZ_F32 or SHKZ_F64 */
     float background;             /* scalars: see above */
-    int32_t reserved[4];
+    int32_t weno;                 /* 0 (default) trilinear, 1 WENO (config key "WENO", :59) */
+    int32_t reserved[3];
 } shkz_advect_params;
 
 void shkz_advect_default_params(shkz_advect_params *params);

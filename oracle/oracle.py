@@ -120,9 +120,13 @@ def lib():
         L.oracle_constrain_velocity.argtypes = [ctypes.POINTER(OVField)]
         L.oracle_constrain_velocity.restype = ctypes.c_int
         L.oracle_advect_velocity.argtypes = [ctypes.POINTER(OVField), _vp, ctypes.c_int, ctypes.c_double, ctypes.c_int,
-                                             ctypes.c_int]
+                                             ctypes.c_int, ctypes.c_int]
         L.oracle_advect_scalar.argtypes = [ctypes.POINTER(OVField), _vp, _vp, _vp, ctypes.c_int, ctypes.c_double,
-                                           ctypes.c_int, ctypes.c_int, ctypes.c_float]
+                                           ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_int]
+        L.oracle_weno6.argtypes = [ctypes.c_double, _vp]
+        L.oracle_weno6.restype = ctypes.c_double
+        L.oracle_weno3.argtypes = [_vp, _vp, _vp]
+        L.oracle_weno3.restype = ctypes.c_double
         _lib = L
     return _lib
 
@@ -142,6 +146,8 @@ def ref():
         R.ref_spmv.restype = ctypes.c_int64
         R.ref_interp_coef.argtypes = [_vp, _vp, _vp, _vp]
         R.ref_deriv_coef.argtypes = [_vp, _vp, _vp, _vp]
+        R.ref_weno6.argtypes = [ctypes.c_double, _vp]
+        R.ref_weno6.restype = ctypes.c_double
         _ref = R
     return _ref
 
@@ -428,23 +434,45 @@ def constrain_velocity(w, h, d, vel, active, solid, dx=None):
 
 
 # ------------------------------------------------------------------------------ advection
-def advect_velocity(w, h, d, vel, active, fluid, dt, dx=None, maccormack=True, trim=1):
+def advect_velocity(w, h, d, vel, active, fluid, dt, dx=None, maccormack=True, trim=1, weno=False):
     """macadvection3::advect_vector(u, u, fluid, dt) in place on vel (float32 faces, uint8 activity)."""
     f = make_vfield(w, h, d, vel, active, None, dx)
     fl = np.ascontiguousarray(fluid)
     lib().oracle_advect_velocity(ctypes.byref(f), fl.ctypes.data, int(fl.dtype == np.float64), float(dt),
-                                 int(bool(maccormack)), int(trim))
+                                 int(bool(maccormack)), int(trim), int(bool(weno)))
 
 
 def advect_scalar(w, h, d, vel, active, q, dt, active_q=None, fluid=None, dx=None, maccormack=True, trim=1,
-                  background=0.0):
+                  background=0.0, weno=False):
     """macadvection3::advect_scalar(q, velocity, fluid, dt) in place on q (float32 cells)."""
     assert q.dtype == np.float32 and q.flags["C_CONTIGUOUS"]
     f = make_vfield(w, h, d, vel, active, None, dx)
     fl = None if fluid is None else np.ascontiguousarray(fluid)
     lib().oracle_advect_scalar(ctypes.byref(f), q.ctypes.data, _p(active_q), None if fl is None else fl.ctypes.data,
                                int(fl is not None and fl.dtype == np.float64), float(dt), int(bool(maccormack)),
-                               int(trim), float(background))
+                               int(trim), float(background), int(bool(weno)))
+
+
+def weno6(x, f):
+    """WENO::interp6 restated: six samples at nodes -2..3, x in [0, 1]."""
+    v = np.ascontiguousarray(f, dtype=np.float64)
+    assert v.shape == (6,)
+    return lib().oracle_weno6(float(x), v.ctypes.data)
+
+
+def ref_weno6(x, f):
+    """The reference's own WENO::interp6, compiled from its header."""
+    v = np.ascontiguousarray(f, dtype=np.float64)
+    assert v.shape == (6,)
+    return ref().ref_weno6(float(x), v.ctypes.data)
+
+
+def weno3(U, p):
+    """WENO3::interpolate(array, p) (order 6) restated, on a float32 array."""
+    U = np.ascontiguousarray(U, dtype=np.float32)
+    s = np.array(U.shape, np.int32)
+    pp = np.ascontiguousarray(p, dtype=np.float64)
+    return lib().oracle_weno3(U.ctypes.data, s.ctypes.data, pp.ctypes.data)
 
 
 def permuted(sys, perm):

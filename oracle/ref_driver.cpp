@@ -9,12 +9,14 @@
 //                                        include/shiokaze/array/array_interpolator3.h:40-62
 //   array_derivative3::derivative_interpolate_coef
 //                                        include/shiokaze/array/array_derivative3.h:40-85
+//   WENO::interp6                        include/shiokaze/math/WENO.h:55-88
 // The reference's solver header (pcg_solver.h) needs GSL's cblas headers, which this image
 // lacks, so it is not built here.
 #include <shiokaze/utility/utility.h>
 #include <pcgsolver/sparse_matrix.h>
 #include <shiokaze/array/array_interpolator3.h>
 #include <shiokaze/array/array_derivative3.h>
+#include <shiokaze/math/WENO.h>
 #include <cstdint>
 #include <vector>
 
@@ -72,3 +74,6 @@ extern "C" void ref_deriv_coef(const int shape[3], const double p[3], int idx[24
             coef[8 * a + n] = c[a][n];
         }
 }
+
+// Six-point WENO interpolation of samples at nodes -2..3 at x in [0, 1].
+extern "C" double ref_weno6(double x, const double v[6]) { return WENO::interp6(x, v); }

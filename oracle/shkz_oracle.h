@@ -174,13 +174,18 @@ int64_t oracle_extrapolate(const oracle_vfield *f, int width);
 int oracle_constrain_velocity(const oracle_vfield *f);
 
 /* ---------------------------------------- advection (shkz_oracle_adv.c): macadvection3.cpp
- * advect_vector(u, u, fluid, dt) in place on f->vel (f->active required; solid unused) */
+ * advect_vector(u, u, fluid, dt) in place on f->vel (f->active required; solid unused); weno: the
+ * WENO option (six-point WENO sampling instead of trilinear) */
 void oracle_advect_velocity(const oracle_vfield *f, const void *fluid, int fluid_f64, double dt, int maccormack,
-                            int trim);
+                            int trim, int weno);
 /* advect_scalar(q, velocity f, fluid, dt) in place on q (cells); active_q NULL = all active;
  * fluid NULL = a copy of q; background = the value inactive cells of the backward pass read */
 void oracle_advect_scalar(const oracle_vfield *f, float *q, const uint8_t *active_q, const void *fluid, int fluid_f64,
-                          double dt, int maccormack, int trim, float background);
+                          double dt, int maccormack, int trim, float background, int weno);
+/* WENO::interp6 (WENO.h:55-88): samples f at nodes -2..3, x in [0, 1] */
+double oracle_weno6(double x, const double f[6]);
+/* WENO3::interpolate(array, p) with order 6 (WENO3.h:105-135) on a float array of shape s */
+double oracle_weno3(const float *U, const int s[3], const double p[3]);
 
 #ifdef __cplusplus
 }

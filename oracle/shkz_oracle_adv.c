@@ -6,6 +6,9 @@
  *   macadvection3::advect_scalar / advect_cell         :38-45,239-288
  *   macadvection3::advect_semiLagrangian_cell          :179-237
  *   defaults MacCormack on, WENO off, TrimNarrowBand 1  :290-294
+ *   the WENO option (config key "WENO")                :59,85-88,200-203
+ *   WENO3::interpolate (order 6) on array3<Real>       include/shiokaze/math/WENO3.h:105-135
+ *   WENO::interp6                                      include/shiokaze/math/WENO.h:55-88
  *   macarray3::convert_to_full (face and cell forms)   include/shiokaze/array/macarray3.h:305-372
  *   array_interpolator3::interpolate_coef/interpolate  include/shiokaze/array/array_interpolator3.h:40-97
  *   vec arithmetic                                     include/shiokaze/math/vec.h:133-136,208-230,401-403
@@ -114,6 +117,100 @@ static void full_at_face(const oracle_vfield *f, float *const U[3], const int di
     }
 }
 
+/* Six-point WENO interpolation (Macdonald & Ruuth, J. Sci. Comput. 35 (2008) 219-240, the method
+ * WENO::interp6 implements): f holds samples at the nodes -2..3, x lies in [0, 1].  Three cubics,
+ * each through four consecutive nodes r-2..r+1 (Newton form), are blended with weights
+ * gamma_r / (eps + beta_r^2) normalised to sum 1, where gamma_r are the linear weights that make
+ * the blend the quintic through all six nodes and beta_r the smoothness indicators, quadratic
+ * forms in the samples (180 * beta_r tabulated below term by term: coefficient, sample a, sample
+ * b).  Each product, sum and quotient is formed in the reference's order, so the result is the
+ * same double. */
+typedef struct {
+    double c;
+    int a, b;
+} WenoTerm;
+static const WenoTerm weno_beta[3][10] = {
+    {{814., 3, 3}, {4326., 2, 2}, {2976., 1, 1}, {244., 0, 0}, {-3579., 2, 3},
+     {-6927., 2, 1}, {1854., 2, 0}, {2634., 3, 1}, {-683., 3, 0}, {-1659., 1, 0}},
+    {{1986., 3, 3}, {1986., 2, 2}, {244., 1, 1}, {244., 4, 4}, {1074., 2, 4},
+     {-3777., 2, 3}, {-1269., 2, 1}, {1074., 3, 1}, {-1269., 4, 3}, {-293., 4, 1}},
+    {{814., 2, 2}, {4326., 3, 3}, {2976., 4, 4}, {244., 5, 5}, {-683., 2, 5},
+     {2634., 2, 4}, {-3579., 2, 3}, {-6927., 3, 4}, {1854., 3, 5}, {-1659., 4, 5}},
+};
+
+static inline double weno_term(const WenoTerm *t, const double f[6])
+{
+    return t->a == t->b ? t->c * (f[t->a] * f[t->a]) : (t->c * f[t->a]) * f[t->b];
+}
+
+double oracle_weno6(double x, const double f[6])
+{
+    const double gamma[3] = {(2.0 - x) * (3.0 - x) / 20.0, (3.0 - x) * (x + 2.0) / 10.0,
+                             (x + 2.0) * (x + 1.0) / 20.0};
+    double alpha[3], poly[3], total = 0.0;
+    for (int r = 0; r < 3; ++r) {
+        double beta = weno_term(&weno_beta[r][0], f);
+        for (int t = 1; t < 10; ++t) beta += weno_term(&weno_beta[r][t], f);
+        beta /= 180.0;
+        /* the cubic through nodes r-2 .. r+1 */
+        const double *g = f + r;
+        const double e0 = x - (double)(r - 2), e1 = x - (double)(r - 1), e2 = x - (double)r;
+        const double d1 = g[1] - g[0];
+        const double d2 = g[2] - 2. * g[1] + g[0];
+        const double d3 = g[3] - 3. * g[2] + 3. * g[1] - g[0];
+        poly[r] = g[0] + d1 * e0 + d2 * e0 * e1 / 2.0 + d3 * e0 * e1 * e2 / 6.0;
+        alpha[r] = gamma[r] / (DBL_EPSILON + beta * beta);
+        total += alpha[r];
+    }
+    double out = 0.0;
+    for (int r = 0; r < 3; ++r) {
+        const double t = alpha[r] / total * poly[r];
+        out = r ? out + t : t;
+    }
+    return out;
+}
+
+/* WENO3::interpolate(array, p) with order 6 on a float array of shape S: p is clamped into the
+ * array, the base index i = min(x, n-2) (truncated), and the 6x6x6 samples at i-2..i+3 (clamped
+ * per axis) are reduced along x, then y, then z.  Inactive entries read 0 when A is given. */
+static double weno3_f(const float *U, const uint8_t *A, const Shape *S, const double p[3])
+{
+    double c[3], fr[3];
+    int b[3];
+    for (int a = 0; a < 3; ++a) {
+        c[a] = amax(0.0, amin(S->s[a] - 1.0, p[a]));
+        b[a] = (int)amin(c[a], S->s[a] - 2.0);
+        fr[a] = c[a] - b[a];
+    }
+    double vz[6];
+    for (int kk = 0; kk < 6; ++kk) {
+        double vy[6];
+        for (int jj = 0; jj < 6; ++jj) {
+            double vx[6];
+            for (int ii = 0; ii < 6; ++ii) {
+                const size_t q = sidx(S, iclamp(b[0] + ii - 2, S->s[0]), iclamp(b[1] + jj - 2, S->s[1]),
+                                      iclamp(b[2] + kk - 2, S->s[2]));
+                vx[ii] = A ? fval(U, A, q) : U[q];
+            }
+            vy[jj] = oracle_weno6(fr[0], vx);
+        }
+        vz[kk] = oracle_weno6(fr[1], vy);
+    }
+    return oracle_weno6(fr[2], vz);
+}
+
+double oracle_weno3(const float *U, const int s[3], const double p[3])
+{
+    const Shape S = {{s[0], s[1], s[2]}};
+    return weno3_f(U, NULL, &S, p);
+}
+
+/* the value a semi-Lagrangian pass samples at p: trilinear (interpolate<Real>) or WENO */
+static inline float sample_f(const float *U, const uint8_t *A, const Shape *S, const double p[3], int weno)
+{
+    return weno ? (float)weno3_f(U, A, S, p) : interp_f(U, A, S, p);
+}
+
 typedef struct {
     float lo, hi;
     uint8_t nb;
@@ -122,7 +219,7 @@ typedef struct {
 /* advect_semiLagrangian_u for every active face: in -> out (float), minmax optional.
  * The full velocity comes from `in` itself (the field being advected, :72-73). */
 static void sl_faces(const oracle_vfield *f, float *const in[3], float *const out[3], MinMax *const mm[3],
-                     const void *fluid, int fluid_f64, double dt, int trim)
+                     const void *fluid, int fluid_f64, double dt, int trim, int weno)
 {
     const double dx = f->dx;
     const Shape Sc = {{f->w, f->h, f->d}};
@@ -145,7 +242,7 @@ static void sl_faces(const oracle_vfield *f, float *const in[3], float *const ou
                             const float sft = (float)((double)t / dx);   /* /m_dx on vec3r */
                             p[a] = (double)pc[a] - (double)sft;
                         }
-                        out[dim][e] = interp_f(in[dim], f->active[dim], &S, p);
+                        out[dim][e] = sample_f(in[dim], f->active[dim], &S, p, weno);
                     } else {
                         out[dim][e] = fval(in[dim], f->active[dim], e);
                     }
@@ -183,7 +280,7 @@ static void sl_faces(const oracle_vfield *f, float *const in[3], float *const ou
 
 /* macadvection3::advect_vector(u, u, fluid, dt) (advect_u, :140-177) in place on f->vel. */
 void oracle_advect_velocity(const oracle_vfield *f, const void *fluid, int fluid_f64, double dt, int maccormack,
-                            int trim)
+                            int trim, int weno)
 {
     float *v0[3], *v1[3], *vin[3];
     MinMax *mm[3];
@@ -199,10 +296,10 @@ void oracle_advect_velocity(const oracle_vfield *f, const void *fluid, int fluid
         mm[a] = (MinMax *)calloc(n, sizeof(MinMax));
     }
     if (!maccormack) {
-        sl_faces(f, vin, f->vel, NULL, fluid, fluid_f64, dt, trim);
+        sl_faces(f, vin, f->vel, NULL, fluid, fluid_f64, dt, trim, weno);
     } else {
-        sl_faces(f, vin, v0, mm, fluid, fluid_f64, dt, trim);
-        sl_faces(f, v0, v1, NULL, fluid, fluid_f64, -dt, trim);
+        sl_faces(f, vin, v0, mm, fluid, fluid_f64, dt, trim, weno);
+        sl_faces(f, v0, v1, NULL, fluid, fluid_f64, -dt, trim, weno);
         for (int dim = 0; dim < 3; ++dim) {
             const Shape S = face_shape3(f, dim);
             const size_t n = (size_t)S.s[0] * S.s[1] * S.s[2];
@@ -262,7 +359,7 @@ static void full_at_cell(const oracle_vfield *f, int i, int j, int k, float out[
 
 /* advect_semiLagrangian_cell for every active cell */
 static void sl_cells(const oracle_vfield *f, const float *in, float *out, const uint8_t *act, MinMax *mm,
-                     const void *fluid, int fluid_f64, double dt, int trim)
+                     const void *fluid, int fluid_f64, double dt, int trim, int weno)
 {
     const double dx = f->dx;
     const Shape S = {{f->w, f->h, f->d}};
@@ -279,7 +376,7 @@ static void sl_cells(const oracle_vfield *f, const float *in, float *out, const 
                 double p[3];
                 const int pc[3] = {i, j, k};
                 for (int a = 0; a < 3; ++a) p[a] = (double)pc[a] - (dt * u[a]) / dx;  /* vec3d arithmetic */
-                out[c] = empty ? in[c] : interp_f(in, NULL, &S, p);
+                out[c] = empty ? in[c] : sample_f(in, NULL, &S, p, weno);
                 if (!mm) continue;
                 MinMax r;
                 if (!empty) {
@@ -310,7 +407,7 @@ static void sl_cells(const oracle_vfield *f, const float *in, float *out, const 
  * pass is a fresh array of q's type (:246), whose inactive cells read its background value
  * `background` (no fill information), as in the reference. */
 void oracle_advect_scalar(const oracle_vfield *f, float *q, const uint8_t *active_q, const void *fluid, int fluid_f64,
-                          double dt, int maccormack, int trim, float background)
+                          double dt, int maccormack, int trim, float background, int weno)
 {
     const size_t n = (size_t)f->w * f->h * f->d;
     float *qin = (float *)malloc(n * sizeof(float));
@@ -320,7 +417,7 @@ void oracle_advect_scalar(const oracle_vfield *f, float *q, const uint8_t *activ
         fluid_f64 = 0;
     }
     if (!maccormack) {
-        sl_cells(f, qin, q, active_q, NULL, fluid, fluid_f64, dt, trim);
+        sl_cells(f, qin, q, active_q, NULL, fluid, fluid_f64, dt, trim, weno);
         free(qin);
         return;
     }
@@ -328,8 +425,8 @@ void oracle_advect_scalar(const oracle_vfield *f, float *q, const uint8_t *activ
     MinMax *mm = (MinMax *)calloc(n, sizeof(MinMax));
     for (size_t c = 0; c < n; ++c) q0[c] = (active_q && !active_q[c]) ? background : qin[c];
     memcpy(q1, q0, n * sizeof(float));
-    sl_cells(f, qin, q0, active_q, mm, fluid, fluid_f64, dt, trim);
-    sl_cells(f, q0, q1, active_q, NULL, fluid, fluid_f64, -dt, trim);
+    sl_cells(f, qin, q0, active_q, mm, fluid, fluid_f64, dt, trim, weno);
+    sl_cells(f, q0, q1, active_q, NULL, fluid, fluid_f64, -dt, trim, weno);
     for (size_t c = 0; c < n; ++c) {
         if (active_q && !active_q[c]) continue;
         if (mm[c].nb) {

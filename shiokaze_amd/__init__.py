@@ -19,7 +19,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "lib", "libshkz_mi355x.so")
 
-ABI_VERSION = 6  # the layout of the structs below
+ABI_VERSION = 7  # the layout of the structs below
 F32, F64 = 0, 1
 SOLVER_MGPCG, SOLVER_CG = 0, 1
 PREC_MIXED, PREC_FP64 = 0, 1
@@ -99,7 +99,7 @@ class VelocityField(ctypes.Structure):
 class AdvectParams(ctypes.Structure):
     _fields_ = [("dt", ctypes.c_double), ("maccormack", ctypes.c_int32), ("trim_narrowband", ctypes.c_int32),
                 ("fluid", _vp), ("fluid_dtype", ctypes.c_int32), ("background", ctypes.c_float),
-                ("reserved", ctypes.c_int32 * 4)]
+                ("weno", ctypes.c_int32), ("reserved", ctypes.c_int32 * 3)]
 
 
 class Grid2D(ctypes.Structure):
@@ -661,10 +661,11 @@ class Context:
         return rc
 
     @staticmethod
-    def _advect_params(dt, fluid, fluid_dtype, maccormack, trim, background, device):
+    def _advect_params(dt, fluid, fluid_dtype, maccormack, trim, background, device, weno=False):
         p = AdvectParams()
         lib().shkz_advect_default_params(ctypes.byref(p))
         p.dt = dt
+        p.weno = int(bool(weno))
         p.maccormack = int(bool(maccormack))
         p.trim_narrowband = int(trim)
         p.background = background
@@ -685,19 +686,20 @@ class Context:
         return VelocityField(w=w, h=h, d=d, dx=dx, u=conv(vel[0]), v=conv(vel[1]), w_vel=conv(vel[2]),
                              active_u=conv(active[0]), active_v=conv(active[1]), active_w=conv(active[2]))
 
-    def advect_velocity(self, w, h, d, vel, active, fluid, dt, maccormack=True, trim=1, dx=0.0, check=True) -> int:
+    def advect_velocity(self, w, h, d, vel, active, fluid, dt, maccormack=True, trim=1, dx=0.0, check=True,
+                        weno=False) -> int:
         """macadvection3::advect_vector(u, u, fluid, dt) on host arrays, in place on vel."""
         f = self._vfield(w, h, d, vel, active, dx, False)
-        p = self._advect_params(dt, fluid, None, maccormack, trim, 0.0, False)
+        p = self._advect_params(dt, fluid, None, maccormack, trim, 0.0, False, weno)
         rc = self._L.shkz_advect_velocity(self.handle, ctypes.byref(f), ctypes.byref(p))
         if rc != 0 and check:
             raise ShkzError(rc, last_error())
         return rc
 
     def advect_velocity_device(self, w, h, d, vel, active, fluid, fluid_dtype, dt, maccormack=True, trim=1, dx=0.0,
-                               stream=None, check=True) -> int:
+                               stream=None, check=True, weno=False) -> int:
         f = self._vfield(w, h, d, vel, active, dx, True)
-        p = self._advect_params(dt, fluid, fluid_dtype, maccormack, trim, 0.0, True)
+        p = self._advect_params(dt, fluid, fluid_dtype, maccormack, trim, 0.0, True, weno)
         rc = self._L.shkz_advect_velocity_device(self.handle, ctypes.byref(f), ctypes.byref(p),
                                                  None if stream is None else int(stream))
         if rc != 0 and check:
@@ -705,20 +707,21 @@ class Context:
         return rc
 
     def advect_scalar(self, w, h, d, vel, active, q, dt, active_q=None, fluid=None, maccormack=True, trim=1,
-                      background=0.0, dx=0.0, check=True) -> int:
+                      background=0.0, dx=0.0, check=True, weno=False) -> int:
         """macadvection3::advect_scalar(q, velocity, fluid, dt) on host arrays, in place on q (float32)."""
         assert q.dtype == np.float32 and q.flags["C_CONTIGUOUS"]
         f = self._vfield(w, h, d, vel, active, dx, False)
-        p = self._advect_params(dt, fluid, None, maccormack, trim, background, False)
+        p = self._advect_params(dt, fluid, None, maccormack, trim, background, False, weno)
         rc = self._L.shkz_advect_scalar(self.handle, ctypes.byref(f), _ptr(q), _ptr(active_q), ctypes.byref(p))
         if rc != 0 and check:
             raise ShkzError(rc, last_error())
         return rc
 
     def advect_scalar_device(self, w, h, d, vel, active, q, dt, active_q=None, fluid=None, fluid_dtype=F32,
-                             maccormack=True, trim=1, background=0.0, dx=0.0, stream=None, check=True) -> int:
+                             maccormack=True, trim=1, background=0.0, dx=0.0, stream=None, check=True,
+                             weno=False) -> int:
         f = self._vfield(w, h, d, vel, active, dx, True)
-        p = self._advect_params(dt, fluid, fluid_dtype, maccormack, trim, background, True)
+        p = self._advect_params(dt, fluid, fluid_dtype, maccormack, trim, background, True, weno)
         rc = self._L.shkz_advect_scalar_device(self.handle, ctypes.byref(f), _dp(q), _dp(active_q), ctypes.byref(p),
                                                None if stream is None else int(stream))
         if rc != 0 and check:

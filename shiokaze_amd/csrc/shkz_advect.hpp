@@ -1,7 +1,7 @@
-// shkz_advect.hpp -- advection on the device: macadvection3 (src/advection/macadvection3.cpp) with
-// its defaults (MacCormack, trilinear interpolation, TrimNarrowBand 1), for the face velocity
-// (advect_vector -> advect_u, :47-53,140-177) and for cell scalars such as the liquid level set
-// (advect_scalar -> advect_cell, :38-45,239-288).
+// shkz_advect.hpp -- advection on the device: macadvection3 (src/advection/macadvection3.cpp)
+// (MacCormack or semi-Lagrangian, trilinear or WENO sampling, TrimNarrowBand), for the face
+// velocity (advect_vector -> advect_u, :47-53,140-177) and for cell scalars such as the liquid
+// level set (advect_scalar -> advect_cell, :38-45,239-288).
 //
 //   k_adv_face_fwd   forward semi-Lagrangian pass of advect_semiLagrangian_u (:68-137) with the
 //                    MacCormack min/max and narrow-band record
@@ -21,6 +21,11 @@
 // pass of the face version computes the departure point from a float vector (vec3r), its min/max
 // pass from a double one (vec3d); interpolation accumulates in the array's type; MacCormack's
 // correction is 0.5 times a float difference.
+//
+// The WENO option (config key "WENO", :59) samples the value passes with WENO3::interpolate
+// (include/shiokaze/math/WENO3.h:105-135, order 6) in double and stores the float; the MacCormack
+// bounds and the narrow-band test stay trilinear (:96-118,207-231).  The kernels take it as a
+// template flag, so the default trilinear kernels keep their register budget.
 #pragma once
 #include <float.h>
 
@@ -188,9 +193,118 @@ __device__ __forceinline__ double interp_level(const void *phi, int f64, const i
     return (double)val;
 }
 
+// Six-point WENO interpolation (Macdonald & Ruuth, J. Sci. Comput. 35 (2008) 219-240; the
+// reference's WENO::interp6, include/shiokaze/math/WENO.h:55-88).  f holds the samples at the
+// nodes -2..3 and x in [0, 1] the position.  The three cubics through four consecutive nodes
+// (Newton form) are blended with weights gamma_r / (eps + beta_r^2) normalised to sum 1: gamma_r
+// the linear weights (which depend on x only and are passed in, computed once per axis), beta_r
+// the smoothness indicators, quadratic forms in the samples whose terms (coefficient, sample a,
+// sample b; 180 beta_r) are tabulated in the reference's order.  Every operation is the
+// reference's, in its order, so the double is the same.
+namespace weno {
+struct Term {
+    double c;
+    int a, b;
+};
+constexpr Term BETA[3][10] = {
+    {{814., 3, 3}, {4326., 2, 2}, {2976., 1, 1}, {244., 0, 0}, {-3579., 2, 3},
+     {-6927., 2, 1}, {1854., 2, 0}, {2634., 3, 1}, {-683., 3, 0}, {-1659., 1, 0}},
+    {{1986., 3, 3}, {1986., 2, 2}, {244., 1, 1}, {244., 4, 4}, {1074., 2, 4},
+     {-3777., 2, 3}, {-1269., 2, 1}, {1074., 3, 1}, {-1269., 4, 3}, {-293., 4, 1}},
+    {{814., 2, 2}, {4326., 3, 3}, {2976., 4, 4}, {244., 5, 5}, {-683., 2, 5},
+     {2634., 2, 4}, {-3579., 2, 3}, {-6927., 3, 4}, {1854., 3, 5}, {-1659., 4, 5}},
+};
+}  // namespace weno
+
+__device__ __forceinline__ void weno_gamma(double x, double g[3]) {
+#pragma clang fp contract(off)
+    g[0] = (2.0 - x) * (3.0 - x) / 20.0;
+    g[1] = (3.0 - x) * (x + 2.0) / 10.0;
+    g[2] = (x + 2.0) * (x + 1.0) / 20.0;
+}
+
+__device__ __forceinline__ double weno6(const double g[3], double x, const double f[6]) {
+#pragma clang fp contract(off)
+    double alpha[3], poly[3], total = 0.0;
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+        double beta = 0.0;
+#pragma unroll
+        for (int t = 0; t < 10; ++t) {
+            const weno::Term T = weno::BETA[r][t];
+            const double v = T.a == T.b ? T.c * (f[T.a] * f[T.a]) : (T.c * f[T.a]) * f[T.b];
+            beta = t ? beta + v : v;
+        }
+        beta = beta / 180.0;
+        const double e0 = x - (double)(r - 2), e1 = x - (double)(r - 1), e2 = x - (double)r;
+        const double d1 = f[r + 1] - f[r];
+        const double d2 = f[r + 2] - 2. * f[r + 1] + f[r];
+        const double d3 = f[r + 3] - 3. * f[r + 2] + 3. * f[r + 1] - f[r];
+        poly[r] = f[r] + d1 * e0 + d2 * e0 * e1 / 2.0 + d3 * e0 * e1 * e2 / 6.0;
+        alpha[r] = g[r] / (DBL_EPSILON + beta * beta);
+        total += alpha[r];
+    }
+    const double t0 = alpha[0] / total * poly[0], t1 = alpha[1] / total * poly[1], t2 = alpha[2] / total * poly[2];
+    return t0 + t1 + t2;
+}
+
+__device__ __forceinline__ double dmin_std(double a, double b) { return b < a ? b : a; }  // std::min
+__device__ __forceinline__ double dmax_std(double a, double b) { return a < b ? b : a; }  // std::max
+
+// WENO3::interpolate(array, p) with order 6 over a float array of shape s: p clamped into the
+// array, base index min(x, n-2) truncated, the 6x6x6 samples at base-2 .. base+3 (clamped per
+// axis) reduced along x, then y, then z.  Inactive entries read 0 when A is given.  The y and z
+// partials enter 6-deep shift registers, so the two outer loops stay rolled (one copy of the
+// x-row code) and index nothing dynamically.
+__device__ double weno3_arr(const float *U, const uint8_t *A, const int s[3], const double p[3]) {
+#pragma clang fp contract(off)
+    int b[3];
+    double fr[3], g[3][3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const double c = dmax_std(0.0, dmin_std(s[a] - 1.0, p[a]));
+        b[a] = (int)dmin_std(c, s[a] - 2.0);
+        fr[a] = c - b[a];
+        weno_gamma(fr[a], g[a]);
+    }
+    double vz[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+#pragma unroll 1
+    for (int kk = 0; kk < 6; ++kk) {
+        const int k = clamp_i(b[2] + kk - 2, s[2]);
+        double vy[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+#pragma unroll 1
+        for (int jj = 0; jj < 6; ++jj) {
+            const int j = clamp_i(b[1] + jj - 2, s[1]);
+            double vx[6];
+#pragma unroll
+            for (int ii = 0; ii < 6; ++ii) {
+                const size_t q = ((size_t)clamp_i(b[0] + ii - 2, s[0]) * s[1] + j) * s[2] + k;
+                vx[ii] = A ? (double)face_val(U, A, q) : (double)U[q];
+            }
+            const double r = weno6(g[0], fr[0], vx);
+#pragma unroll
+            for (int n = 0; n < 5; ++n) vy[n] = vy[n + 1];
+            vy[5] = r;
+        }
+        const double r = weno6(g[1], fr[1], vy);
+#pragma unroll
+        for (int n = 0; n < 5; ++n) vz[n] = vz[n + 1];
+        vz[5] = r;
+    }
+    return weno6(g[2], fr[2], vz);
+}
+
+// The value a semi-Lagrangian pass samples at p: trilinear (interpolate<Real>) or WENO.
+template <int WENO>
+__device__ __forceinline__ float sample_arr(const float *U, const uint8_t *A, const int s[3], const double p[3]) {
+    if (WENO) return (float)weno3_arr(U, A, s, p);
+    return interp_arr(U, A, s, p);
+}
+
 // Forward pass over every active face: S.v0 = the semi-Lagrangian value; with `minmax`, the
 // MacCormack bounds and the narrow-band flag.  `in` is the field being advected (its own full
 // velocity moves it, :72-73).
+template <int WENO>
 __global__ void k_adv_face_fwd(VelField F, const float *in0, const float *in1, const float *in2, AdvScratch S,
                                const void *fluid, int fluid64, double dt, int trim, int minmax) {
 #pragma clang fp contract(off)
@@ -216,9 +330,13 @@ __global__ void k_adv_face_fwd(VelField F, const float *in0, const float *in1, c
             const float sft = (float)((double)t / F.dx);  // /dx on a float vector
             p[b] = (double)pc[b] - (double)sft;
         }
-        cv = corner_of(s, p);
-        load8(in[a], F.act[a], s, cv, v8);
-        value = interp8(cv, v8);
+        if (WENO) {
+            value = (float)weno3_arr(in[a], F.act[a], s, p);
+        } else {
+            cv = corner_of(s, p);
+            load8(in[a], F.act[a], s, cv, v8);
+            value = interp8(cv, v8);
+        }
     } else {
         value = face_val(in[a], F.act[a], f);
     }
@@ -236,7 +354,7 @@ __global__ void k_adv_face_fwd(VelField F, const float *in0, const float *in1, c
         }
         double dlo = DBL_MAX, dhi = DBL_MIN;
         const Corner cm = corner_of(s, p);
-        if (!same_cell(cm, cv)) load8(in[a], F.act[a], s, cm, v8);
+        if (WENO || !same_cell(cm, cv)) load8(in[a], F.act[a], s, cm, v8);
         bounds8(v8, dlo, dhi);
         lo = (float)dlo;
         hi = (float)dhi;
@@ -254,6 +372,7 @@ __global__ void k_adv_face_fwd(VelField F, const float *in0, const float *in1, c
 
 // Backward pass from S.v0 (-dt) fused with MacCormack's final step, written over F.vel (the
 // input field, read only at the thread's own face).
+template <int WENO>
 __global__ void k_adv_face_bwd(VelField F, AdvScratch S, double dt_back) {
 #pragma clang fp contract(off)
     int a, i, j, k;
@@ -280,7 +399,7 @@ __global__ void k_adv_face_bwd(VelField F, AdvScratch S, double dt_back) {
             const float sft = (float)((double)t / F.dx);
             p[b] = (double)pc[b] - (double)sft;
         }
-        v1 = interp_arr(in[a], F.act[a], s, p);
+        v1 = sample_arr<WENO>(in[a], F.act[a], s, p);
     } else {
         v1 = v0;
     }
@@ -301,6 +420,7 @@ __global__ void k_adv_face_copy(VelField F, AdvScratch S) {
 // Forward pass over the cells of q (advected by F): S.v0[0] = the semi-Lagrangian value, or
 // `background` on inactive cells (what the backward pass reads there); with `minmax`, bounds
 // and the narrow-band flag against `fluid` (cells).
+template <int WENO>
 __global__ void k_adv_cell_fwd(VelField F, const float *q, const uint8_t *aq, AdvScratch S, const void *fluid,
                                int fluid64, double dt, int trim, float background, int minmax) {
 #pragma clang fp contract(off)
@@ -324,14 +444,19 @@ __global__ void k_adv_cell_fwd(VelField F, const float *q, const uint8_t *aq, Ad
     float v8[8];
     const Corner cv = corner_of(s, p);
     if (!empty) {
-        load8(q, nullptr, s, cv, v8);
-        S.v0[0][c] = interp8(cv, v8);
+        if (WENO) {
+            S.v0[0][c] = (float)weno3_arr(q, nullptr, s, p);
+        } else {
+            load8(q, nullptr, s, cv, v8);
+            S.v0[0][c] = interp8(cv, v8);
+        }
     } else {
         S.v0[0][c] = q[c];
     }
     if (!minmax) return;
     if (!empty) {
         double lo = (double)FLT_MAX, hi = (double)FLT_MIN;
+        if (WENO) load8(q, nullptr, s, cv, v8);
         bounds8(v8, lo, hi);
         S.lo[0][c] = (float)lo;
         S.hi[0][c] = (float)hi;
@@ -344,6 +469,7 @@ __global__ void k_adv_cell_fwd(VelField F, const float *q, const uint8_t *aq, Ad
 }
 
 // Backward pass from S.v0[0] fused with the final MacCormack step, written over q.
+template <int WENO>
 __global__ void k_adv_cell_bwd(VelField F, float *q, const uint8_t *aq, AdvScratch S, double dt_back) {
 #pragma clang fp contract(off)
     const size_t c = (size_t)xcd_brick(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
@@ -366,7 +492,7 @@ __global__ void k_adv_cell_bwd(VelField F, float *q, const uint8_t *aq, AdvScrat
         double p[3];
 #pragma unroll
         for (int b = 0; b < 3; ++b) p[b] = (double)pc[b] - (dt_back * u[b]) / F.dx;
-        v1 = interp_arr(S.v0[0], nullptr, s, p);
+        v1 = sample_arr<WENO>(S.v0[0], nullptr, s, p);
     } else {
         v1 = v0;
     }

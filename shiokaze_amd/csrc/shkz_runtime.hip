@@ -2529,6 +2529,7 @@ static int validate_advect(const shkz_velocity_field *f, const shkz_advect_param
         return fail(SHKZ_ERR_INVALID, "velocity and activity arrays are required");
     if (!std::isfinite(p->dt)) return fail(SHKZ_ERR_INVALID, "dt is not finite");
     if (p->trim_narrowband < 0) return fail(SHKZ_ERR_INVALID, "negative TrimNarrowBand");
+    if (p->weno != 0 && p->weno != 1) return fail(SHKZ_ERR_INVALID, "weno must be 0 or 1 (got %d)", p->weno);
     if (!scalar && !p->fluid) return fail(SHKZ_ERR_INVALID, "velocity advection needs the liquid level set");
     if (p->fluid && p->fluid_dtype != SHKZ_F32 && p->fluid_dtype != SHKZ_F64)
         return fail(SHKZ_ERR_INVALID, "unknown fluid dtype %d", p->fluid_dtype);
@@ -2560,10 +2561,18 @@ static int advect_velocity_stages(shkz_ctx *C, const shkz_velocity_field *f, con
     const int f64 = p->fluid_dtype == SHKZ_F64;
     // byte models per face: forward reads the face and its activity (neighbours hit the cache)
     // and writes v0 (+ bounds and flag); backward reads v0, bounds, flag, the input and writes it
-    K.go("adv_face_fwd", -1, 0, (double)total * (mc ? 18 : 9), grid, block, k_adv_face_fwd, F, (const float *)vel[0],
-         (const float *)vel[1], (const float *)vel[2], S, fluid, f64, p->dt, (int)p->trim_narrowband, mc ? 1 : 0);
-    if (mc)
-        K.go("adv_face_bwd", -1, 0, (double)total * 22, grid, block, k_adv_face_bwd, F, S, -p->dt);
+    if (p->weno)
+        K.go("adv_face_fwd_weno", -1, 0, (double)total * (mc ? 18 : 9), grid, block, k_adv_face_fwd<1>, F,
+             (const float *)vel[0], (const float *)vel[1], (const float *)vel[2], S, fluid, f64, p->dt,
+             (int)p->trim_narrowband, mc ? 1 : 0);
+    else
+        K.go("adv_face_fwd", -1, 0, (double)total * (mc ? 18 : 9), grid, block, k_adv_face_fwd<0>, F,
+             (const float *)vel[0], (const float *)vel[1], (const float *)vel[2], S, fluid, f64, p->dt,
+             (int)p->trim_narrowband, mc ? 1 : 0);
+    if (mc && p->weno)
+        K.go("adv_face_bwd_weno", -1, 0, (double)total * 22, grid, block, k_adv_face_bwd<1>, F, S, -p->dt);
+    else if (mc)
+        K.go("adv_face_bwd", -1, 0, (double)total * 22, grid, block, k_adv_face_bwd<0>, F, S, -p->dt);
     else
         K.go("adv_face_copy", -1, 0, (double)total * 9, grid, block, k_adv_face_copy, F, S);
     SHKZ_HIP(hipGetLastError());
@@ -2591,10 +2600,16 @@ static int advect_scalar_stages(shkz_ctx *C, const shkz_velocity_field *f, float
         fluid = q;
         fluid64 = 0;
     }
-    K.go("adv_cell_fwd", -1, 0, (double)M * (mc ? 18 : 8), grid, block, k_adv_cell_fwd, F, (const float *)q, aq, S,
-         fluid, fluid64, p->dt, (int)p->trim_narrowband, p->background, mc ? 1 : 0);
-    if (mc)
-        K.go("adv_cell_bwd", -1, 0, (double)M * 21, grid, block, k_adv_cell_bwd, F, q, aq, S, -p->dt);
+    if (p->weno)
+        K.go("adv_cell_fwd_weno", -1, 0, (double)M * (mc ? 18 : 8), grid, block, k_adv_cell_fwd<1>, F,
+             (const float *)q, aq, S, fluid, fluid64, p->dt, (int)p->trim_narrowband, p->background, mc ? 1 : 0);
+    else
+        K.go("adv_cell_fwd", -1, 0, (double)M * (mc ? 18 : 8), grid, block, k_adv_cell_fwd<0>, F, (const float *)q,
+             aq, S, fluid, fluid64, p->dt, (int)p->trim_narrowband, p->background, mc ? 1 : 0);
+    if (mc && p->weno)
+        K.go("adv_cell_bwd_weno", -1, 0, (double)M * 21, grid, block, k_adv_cell_bwd<1>, F, q, aq, S, -p->dt);
+    else if (mc)
+        K.go("adv_cell_bwd", -1, 0, (double)M * 21, grid, block, k_adv_cell_bwd<0>, F, q, aq, S, -p->dt);
     else
         K.go("adv_cell_copy", -1, 0, (double)M * 8, grid, block, k_adv_cell_copy, M, q, aq, S);
     SHKZ_HIP(hipGetLastError());

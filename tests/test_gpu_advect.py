@@ -6,10 +6,13 @@ macadvection3 with its defaults) against the oracle, bit for bit:
   * the liquid level set advected by that velocity with the narrow band measured on itself (the
     level-set tracker's call), on a narrow band of active cells with a background value, and a
     scalar with an explicit level set;
+  * the WENO option (six-point WENO sampling in the value passes) on the velocity and on the level
+    set, MacCormack and semi-Lagrangian, with a narrow band of active cells;
   * the whole step chained on the device on one stream (projection, extrapolation and
     constraint, velocity and level-set advection) equals the host entries;
   * invalid input is refused.
 """
+import ctypes
 import sys
 
 import numpy as np
@@ -117,6 +120,71 @@ def test_scalar_with_explicit_level_set(ctx):
         assert same(qg, qo)[0]
 
 
+@pytest.mark.parametrize("case", [0, 2])
+@pytest.mark.parametrize("mc", [True, False])
+def test_weno_velocity_advection_matches_oracle(ctx, case, mc):
+    kind, n, seed = CASES[case]
+    say(f"advect velocity WENO case={case} mc={mc}")
+    sc, vel, act = stepped(ctx, kind, n, seed)
+    dt = 1.5 * sc.dx / max(float(np.abs(v).max()) for v in vel)
+    fluid = sc.fluid.astype(np.float32)
+    vg = [v.copy() for v in vel]
+    ctx.advect_velocity(sc.w, sc.h, sc.d, vg, act, fluid, dt, maccormack=mc, weno=True)
+    vo = [v.copy() for v in vel]
+    O.advect_velocity(sc.w, sc.h, sc.d, vo, act, fluid, dt, maccormack=mc, weno=True)
+    for a in range(3):
+        ok, info = same(vg[a], vo[a])
+        assert ok, (a, info)
+    vt = [v.copy() for v in vel]
+    ctx.advect_velocity(sc.w, sc.h, sc.d, vt, act, fluid, dt, maccormack=mc)
+    assert any(not np.array_equal(x, y) for x, y in zip(vg, vt))
+
+
+@pytest.mark.parametrize("mc", [True, False])
+def test_weno_level_set_advection_matches_oracle(ctx, mc):
+    say(f"advect level set WENO mc={mc}")
+    sc, vel, act = stepped(ctx, S.SCENE_SPHERES_BOX, 36, 2)
+    dt = 2.0 * sc.dx / max(float(np.abs(v).max()) for v in vel)
+    phi = sc.fluid.astype(np.float32)
+    qg, qo = phi.copy(), phi.copy()
+    ctx.advect_scalar(sc.w, sc.h, sc.d, vel, act, qg, dt, maccormack=mc, weno=True)
+    O.advect_scalar(sc.w, sc.h, sc.d, vel, act, qo, dt, maccormack=mc, weno=True)
+    assert same(qg, qo)[0]
+    band = (np.abs(phi) < 3 * sc.dx).astype(np.uint8)
+    qg, qo = phi.copy(), phi.copy()
+    ctx.advect_scalar(sc.w, sc.h, sc.d, vel, act, qg, dt, active_q=band, maccormack=mc, background=0.125, weno=True)
+    O.advect_scalar(sc.w, sc.h, sc.d, vel, act, qo, dt, active_q=band, maccormack=mc, background=0.125, weno=True)
+    assert same(qg, qo)[0]
+    assert np.array_equal(qg[band == 0], phi[band == 0])
+
+
+def test_weno_on_device_arrays_matches_host_entry(ctx):
+    say("advect WENO device entries")
+    from shiokaze_amd.devmem import DeviceArray, Stream
+    sc, vel, act = stepped(ctx, S.SCENE_DROP_SOLIDS, 32, 4)
+    dt = 1.0 * sc.dx / max(float(np.abs(v).max()) for v in vel)
+    phi = sc.fluid.astype(np.float32)
+    dv = [DeviceArray(v) for v in vel]
+    da = [DeviceArray(a) for a in act]
+    dphi = DeviceArray(phi)
+    st = Stream()
+    ctx.advect_scalar_device(sc.w, sc.h, sc.d, dv, da, dphi, dt, stream=st.handle, weno=True)
+    ctx.advect_velocity_device(sc.w, sc.h, sc.d, dv, da, dphi, S.F32, dt, stream=st.handle, weno=True)
+    st.synchronize()
+    gv = [v.download() for v in dv]
+    gphi = dphi.download()
+    st.destroy()
+    for a in dv + da + [dphi]:
+        a.free()
+    hv = [v.copy() for v in vel]
+    hphi = phi.copy()
+    ctx.advect_scalar(sc.w, sc.h, sc.d, hv, act, hphi, dt, weno=True)
+    ctx.advect_velocity(sc.w, sc.h, sc.d, hv, act, hphi, dt, weno=True)
+    assert np.array_equal(gphi, hphi)
+    for a in range(3):
+        assert np.array_equal(gv[a], hv[a])
+
+
 def test_whole_step_chained_on_device(ctx):
     """Projection, extrapolation and constraint, velocity and level-set advection on device
     arrays, queued on one stream without host synchronisation, equal the host entries."""
@@ -168,4 +236,12 @@ def test_invalid_input_is_refused(ctx):
     act = [np.ones(s, np.uint8) for s in shapes]
     assert ctx.advect_velocity(w, h, d, vel, act, None, 0.1, check=False) == S.ERR_INVALID
     assert ctx.advect_velocity(w, h, d, vel, act, np.zeros((w, h, d), np.float32), float("nan"), check=False) == \
+        S.ERR_INVALID
+    q = np.zeros((w, h, d), np.float32)
+    p = S.AdvectParams()
+    S.lib().shkz_advect_default_params(ctypes.byref(p))
+    p.dt, p.weno = 0.1, 2  # the flag is 0 or 1
+    p.fluid, p.fluid_dtype = None, S.F32
+    f = ctx._vfield(w, h, d, vel, act, 0.0, False)
+    assert S.lib().shkz_advect_scalar(ctx.handle, ctypes.byref(f), q.ctypes.data, None, ctypes.byref(p)) == \
         S.ERR_INVALID
