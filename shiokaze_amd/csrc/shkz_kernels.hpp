@@ -2498,7 +2498,6 @@ __global__ void k_velocity_update(GridDesc G, const float *__restrict__ P) {
     int i, j, k;
     unflatten((unsigned)e, (unsigned)H, (unsigned)D, i, j, k);
     const bool in_i = i < G.w, in_j = j < G.h, in_k = k < G.d;
-    auto p_at = [&](int a, int b, int c) { return (a >= 0 && b >= 0 && c >= 0) ? P[cell_index(G, a, b, c)] : 0.0f; };
     const float pc = (in_i && in_j && in_k) ? P[cell_index(G, i, j, k)] : 0.0f;
     const Exist E = exist_of(G);
     // face_rho of the three faces from four level-set samples, read once: the (clamped) node cell

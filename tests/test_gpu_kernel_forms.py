@@ -7,7 +7,8 @@
   launch per sweep everywhere, a large limit = multi-sweep launches on every coarse level).
 * The FP32 Krylov iteration runs either with the direction update fused into the operator apply
   and the first V-cycle sweep fused into the residual update, or with separate kernels
-  (SHKZ_NO_FUSE=1); both must give bit-identical projections.
+  (SHKZ_NO_FUSE=1); both must give bit-identical projections, alone and in a batch whose grids end
+  on different levels, with several V-cycle shapes of the brick-local smoother.
 """
 import os
 import sys
@@ -75,11 +76,46 @@ def test_multi_sweep_and_per_sweep_smoothing_agree(name):
 
 
 @pytest.mark.parametrize("name", ["drop_solids_96_mixed", "spheres_box_128_mixed"])
-def test_fused_and_separate_krylov_kernels_agree(name):
+def test_fused_and_separate_kernels_agree(name):
     kind, n, seed, prec = CASES[name]
-    say(f"krylov forms {name}")
+    say(f"fused kernel forms {name}")
     sc = S.make_scene(kind, n, seed=seed)
     prm = S.default_params(residual=1e-6, precision=prec)
     fused = project_with_env(sc, prm, {})
     separate = project_with_env(sc, prm, {"SHKZ_NO_FUSE": "1"})
     assert_same(fused, separate)
+
+
+@pytest.mark.parametrize("mg", [(2, 2, 2, 2), (1, 1, 1, 1), (2, 1, 1, 2)])
+def test_fused_and_separate_kernels_agree_in_a_batch(mg):
+    """Grids of 8^3 to 112^3 cells end on levels 0 to 4; the brick-local smoother with the
+    given pre/post applications on the finest and the coarse levels."""
+    say(f"fused kernel forms, batch, V-cycle {mg}")
+    scenes = [S.make_scene(S.SCENE_DROP_SOLIDS, 112, seed=1), S.make_scene(S.SCENE_RANDOM_DROP, 17, 30, 22, seed=3),
+              S.make_scene(S.SCENE_DROP, 8, seed=4), S.make_scene(S.SCENE_SPHERES_BOX, 64, seed=2),
+              S.make_scene(S.SCENE_DROP, 40, seed=0)]
+    prm = S.default_params(residual=1e-6, precision=S.PREC_MIXED)
+    prm.mg_smoother = S.SMOOTH_BRICK
+    prm.mg_pre_sweeps, prm.mg_post_sweeps, prm.mg_coarse_pre_sweeps, prm.mg_coarse_post_sweeps = mg
+    out = []
+    for env in ({}, {"SHKZ_NO_FUSE": "1"}):
+        old = {k: os.environ.get(k) for k in env}
+        os.environ.update(env)
+        try:
+            with S.Context(0) as ctx:
+                work = [sc.copy() for sc in scenes]
+                res, _, P64 = ctx.project_batch(work, prm, pressure64=True)
+        finally:
+            for k, v in old.items():
+                if v is None:
+                    os.environ.pop(k, None)
+                else:
+                    os.environ[k] = v
+        out.append((res, P64, work))
+    (ra, Pa, wa), (rb, Pb, wb) = out
+    for g in range(len(scenes)):
+        assert ra[g].status == 0 and ra[g].converged and rb[g].converged, g
+        assert ra[g].iterations == rb[g].iterations and ra[g].reresid_inf == rb[g].reresid_inf, g
+        assert np.array_equal(Pa[g].view(np.uint64), Pb[g].view(np.uint64)), g
+        for x, y in ((wa[g].u, wb[g].u), (wa[g].v, wb[g].v), (wa[g].wv, wb[g].wv)):
+            assert np.array_equal(x.view(np.uint32), y.view(np.uint32)), g
