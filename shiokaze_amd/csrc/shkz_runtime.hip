@@ -181,7 +181,7 @@ static void grid_level_dims(GridLevelHost &L, int ni, int nj, int nk) {
 struct LevelHost {
     int nbricks = 0;
     bool any_coarsest = false;  // some grid ends on this level
-    std::vector<int> bmap, bxyz, bnbr, bgrid;
+    std::vector<int> bmap, bxyz, bgrid;
     std::vector<GridLevel> gl;
     std::vector<BrickMeta> meta;
     std::vector<int> child64, parent8;  // restriction / prolongation brick tables (multigrid only)
@@ -196,6 +196,28 @@ struct LevelHost {
     std::vector<int64_t> unknowns_grid;
 };
 
+// n default levels, whose host tables reuse the storage of the levels in `levels` (emptied).
+static void recycle_levels(std::vector<LevelHost> &levels, int n) {
+    std::vector<LevelHost> out(n);
+    for (int l = 0; l < n && l < (int)levels.size(); ++l) {
+        LevelHost &F = out[l], &O = levels[l];
+        auto take = [](auto &dst, auto &src) {
+            dst = std::move(src);
+            dst.clear();
+        };
+        take(F.bmap, O.bmap);
+        take(F.bxyz, O.bxyz);
+        take(F.bgrid, O.bgrid);
+        take(F.gl, O.gl);
+        take(F.meta, O.meta);
+        take(F.child64, O.child64);
+        take(F.parent8, O.parent8);
+        take(F.nb27, O.nb27);
+        take(F.unknowns_grid, O.unknowns_grid);
+    }
+    levels.swap(out);
+}
+
 // Concatenates the grids' active bricks (Morton order inside each grid) and builds the dense maps
 // and the neighbour table of the level.
 // Depth-first walk of the octree over one grid's brick positions, children in Morton order (x the
@@ -207,23 +229,36 @@ struct MortonWalk {
     const std::vector<std::vector<GridLevelHost>> &levels;
     int level;  // the level of H in levels
     const GridLevelHost &H;
-    LevelHost &L;
-    int bmap_off, grid, next;
+    int *bmap;   // the grid's dense map
+    int *xyz;    // the level's brick positions (3 per brick)
+    int *bgrid;  // the level's brick -> grid
+    int grid, next;
+    void leaf(int x, int y, int z) {
+        const int b = (x * H.nby + y) * H.nbz + z;
+        if (!H.active[b]) return;
+        bmap[b] = next;
+        xyz[3 * next] = x;
+        xyz[3 * next + 1] = y;
+        xyz[3 * next + 2] = z;
+        bgrid[next] = grid;
+        ++next;
+    }
     void walk(int x0, int y0, int z0, int m) {
         if (x0 >= H.nbx || y0 >= H.nby || z0 >= H.nbz) return;
         if (m == 0) {
-            const int b = (x0 * H.nby + y0) * H.nbz + z0;
-            if (!H.active[b]) return;
-            L.bmap[bmap_off + b] = next++;
-            L.bxyz.push_back(x0);
-            L.bxyz.push_back(y0);
-            L.bxyz.push_back(z0);
-            L.bgrid.push_back(grid);
+            leaf(x0, y0, z0);
             return;
         }
         if (level + m < (int)levels.size()) {
             const GridLevelHost &C = levels[level + m][grid];
             if (!C.active.empty() && !C.active[((x0 >> m) * C.nby + (y0 >> m)) * C.nbz + (z0 >> m)]) return;
+        }
+        if (m == 1) {  // the eight positions of the octant, in Morton order
+            for (int c = 0; c < 8; ++c) {
+                const int x = x0 + ((c >> 2) & 1), y = y0 + ((c >> 1) & 1), z = z0 + (c & 1);
+                if (x < H.nbx && y < H.nby && z < H.nbz) leaf(x, y, z);
+            }
+            return;
         }
         const int h = 1 << (m - 1);
         for (int c = 0; c < 8; ++c) walk(x0 + ((c >> 2) & 1) * h, y0 + ((c >> 1) & 1) * h, z0 + (c & 1) * h, m - 1);
@@ -234,11 +269,19 @@ static void build_level(LevelHost &L, const std::vector<std::vector<GridLevelHos
     const std::vector<GridLevelHost> &grids = levels[level];
     const int G = (int)grids.size();
     L.gl.assign(G, GridLevel{});
-    L.bmap.clear();
-    L.bxyz.clear();
-    L.bgrid.clear();
     L.any_coarsest = false;
+    // sizes first: the dense maps of all grids, and the active positions (the level's bricks)
+    size_t nmap = 0, nact = 0;
+    for (int g = 0; g < G; ++g) {
+        const GridLevelHost &H = grids[g];
+        nmap += (size_t)H.nbx * H.nby * H.nbz;
+        for (uint8_t a : H.active) nact += a != 0;
+    }
+    L.bmap.assign(nmap, -1);
+    L.bxyz.resize(3 * nact);
+    L.bgrid.resize(nact);
     int nb = 0;
+    size_t off = 0;
     for (int g = 0; g < G; ++g) {
         const GridLevelHost &H = grids[g];
         GridLevel &gl = L.gl[g];
@@ -248,37 +291,46 @@ static void build_level(LevelHost &L, const std::vector<std::vector<GridLevelHos
         gl.nbx = H.nbx;
         gl.nby = H.nby;
         gl.nbz = H.nbz;
-        gl.bmap_off = (int)L.bmap.size();
+        gl.bmap_off = (int)off;
         gl.brick_start = nb;
         const int npos = H.nbx * H.nby * H.nbz;
-        L.bmap.resize(L.bmap.size() + npos, -1);
         // active positions in Morton order (x the most significant bit of each triple): a depth-
         // first walk of the octree over the brick grid, skipping octants outside it or with no
         // active position, visits them in that order (no sort)
         int m = 0;
         while ((1 << m) < H.nbx || (1 << m) < H.nby || (1 << m) < H.nbz) ++m;
-        MortonWalk W{levels, level, H, L, gl.bmap_off, g, nb};
+        MortonWalk W{levels, level, H, L.bmap.data() + off, L.bxyz.data(), L.bgrid.data(), g, nb};
         if (npos > 0) W.walk(0, 0, 0, m);
         nb = W.next;
+        off += (size_t)npos;
         gl.brick_end = nb;
         gl.coarsest = (npos > 0 && H.single()) ? 1 : 0;
         if (gl.coarsest && gl.brick_end > gl.brick_start) L.any_coarsest = true;
     }
     L.nbricks = nb;
-    L.bnbr.assign(6 * (size_t)nb, -1);
-    L.meta.assign(nb, BrickMeta{});
-    const int off[6][3] = {{-1, 0, 0}, {1, 0, 0}, {0, -1, 0}, {0, 1, 0}, {0, 0, -1}, {0, 0, 1}};
-    for (int id = 0; id < nb; ++id) {
-        const GridLevel &gl = L.gl[L.bgrid[id]];
-        BrickMeta &m = L.meta[id];
-        for (int d = 0; d < 6; ++d) {
-            const int x = L.bxyz[3 * id] + off[d][0], y = L.bxyz[3 * id + 1] + off[d][1], z = L.bxyz[3 * id + 2] + off[d][2];
-            if (x >= 0 && y >= 0 && z >= 0 && x < gl.nbx && y < gl.nby && z < gl.nbz)
-                L.bnbr[6 * id + d] = L.bmap[gl.bmap_off + (x * gl.nby + y) * gl.nbz + z];
-            m.nbr[d] = L.bnbr[6 * id + d];
+    // the neighbour table, grid by grid (raw pointers: no aliasing between the map and the records)
+    L.meta.resize(nb);
+    const int *__restrict__ xyz = L.bxyz.data();
+    BrickMeta *__restrict__ meta = L.meta.data();
+    for (int g = 0; g < G; ++g) {
+        const GridLevel &gl = L.gl[g];
+        const int *__restrict__ gm = L.bmap.data() + gl.bmap_off;
+        const int sy = gl.nbz, sx = gl.nby * gl.nbz;
+        const int flags = gl.coarsest ? BM_COARSEST : 0;
+        for (int id = gl.brick_start; id < gl.brick_end; ++id) {
+            const int x = xyz[3 * id], y = xyz[3 * id + 1], z = xyz[3 * id + 2];
+            const int p = x * sx + y * sy + z;
+            BrickMeta m;
+            m.nbr[NB_XM] = x > 0 ? gm[p - sx] : -1;
+            m.nbr[NB_XP] = x + 1 < gl.nbx ? gm[p + sx] : -1;
+            m.nbr[NB_YM] = y > 0 ? gm[p - sy] : -1;
+            m.nbr[NB_YP] = y + 1 < gl.nby ? gm[p + sy] : -1;
+            m.nbr[NB_ZM] = z > 0 ? gm[p - 1] : -1;
+            m.nbr[NB_ZP] = z + 1 < gl.nbz ? gm[p + 1] : -1;
+            m.grid = g;
+            m.flags = flags;
+            meta[id] = m;
         }
-        m.grid = L.bgrid[id];
-        m.flags = gl.coarsest ? BM_COARSEST : 0;
     }
     L.child64.clear();
     L.parent8.clear();
@@ -1091,50 +1143,68 @@ static int krylov(shkz_ctx *C, Launcher &K, Problem &P, T *x, T *r, T *p, T *q, 
             update_p(3 * es, (const T *)r, 0);
         }
     };
-    hipGraphExec_t exec = nullptr;
+    // the iteration (fused: each of the two half-iterations, which alternate the direction
+    // buffers) captured once into a hipGraph and replayed
+    auto graph_for = [&](const std::string &key, auto &&body, hipGraphExec_t *out) -> int {
+        for (auto &g : C->graphs)
+            if (g.exec && g.key == key) {
+                g.used = ++C->graph_clock;
+                *out = g.exec;
+                return SHKZ_OK;
+            }
+        constexpr size_t kMaxGraphs = 24;
+        if (C->graphs.size() >= kMaxGraphs) {
+            // evict the least recently used exec once every launch of it has completed
+            size_t lru = 0;
+            for (size_t i = 1; i < C->graphs.size(); ++i)
+                if (C->graphs[i].used < C->graphs[lru].used) lru = i;
+            SHKZ_HIP(hipStreamSynchronize(K.s));
+            SHKZ_HIP(hipGraphExecDestroy(C->graphs[lru].exec));
+            C->graphs.erase(C->graphs.begin() + (long)lru);
+        }
+        hipGraph_t graph = nullptr;
+        SHKZ_HIP(hipStreamBeginCapture(K.s, hipStreamCaptureModeThreadLocal));
+        K.capturing = true;
+        body();
+        K.capturing = false;
+        SHKZ_HIP(hipStreamEndCapture(K.s, &graph));
+        hipGraphExec_t made = nullptr;
+        hipError_t e = hipGraphInstantiateWithFlags(&made, graph, 0);
+        (void)hipGraphDestroy(graph);
+        if (e != hipSuccess) return fail(SHKZ_ERR_DEVICE, "hipGraphInstantiate failed: %s", hipGetErrorString(e));
+        C->graphs.push_back(shkz_ctx::GraphSlot{key, made, ++C->graph_clock});
+        *out = made;
+        return SHKZ_OK;
+    };
+    hipGraphExec_t exec = nullptr, exec2 = nullptr;
     if (C->use_graphs && !C->timing) {
         const void *vecs[] = {sizeof(T) == 4 ? (const void *)x64 : (const void *)x, r, p, q, z, tmp, S, ctl,
                               fused ? (const void *)p2 : nullptr, P.dir64, P.gx64, P.gy64, P.gz64};
         const std::string key = std::string(sizeof(T) == 8 ? "d" : "f") + (fused ? "F" : "") +
                                 graph_key<T>(C, P, vecs, 13, precond);
-        for (auto &g : C->graphs)
-            if (g.exec && g.key == key) {
-                exec = g.exec;
-                g.used = ++C->graph_clock;
-                break;
-            }
-        if (!exec) {
-            constexpr size_t kMaxGraphs = 24;
-            if (C->graphs.size() >= kMaxGraphs) {
-                // evict the least recently used exec once every launch of it has completed
-                size_t lru = 0;
-                for (size_t i = 1; i < C->graphs.size(); ++i)
-                    if (C->graphs[i].used < C->graphs[lru].used) lru = i;
-                SHKZ_HIP(hipStreamSynchronize(K.s));
-                SHKZ_HIP(hipGraphExecDestroy(C->graphs[lru].exec));
-                C->graphs.erase(C->graphs.begin() + (long)lru);
-            }
-            hipGraph_t graph = nullptr;
-            SHKZ_HIP(hipStreamBeginCapture(K.s, hipStreamCaptureModeThreadLocal));
-            K.capturing = true;
-            iteration();
-            K.capturing = false;
-            SHKZ_HIP(hipStreamEndCapture(K.s, &graph));
-            hipGraphExec_t made = nullptr;
-            hipError_t e = hipGraphInstantiateWithFlags(&made, graph, 0);
-            (void)hipGraphDestroy(graph);
-            if (e != hipSuccess) return fail(SHKZ_ERR_DEVICE, "hipGraphInstantiate failed: %s", hipGetErrorString(e));
-            C->graphs.push_back(shkz_ctx::GraphSlot{key, made, ++C->graph_clock});
-            exec = made;
+        if (fused) {
+            // one graph per half-iteration (the second one a later find cannot evict: the first
+            // was used last but one), so that a solve replays exactly the iterations it asks for
+            SHKZ_TRY(graph_for(key + "A", [&]() { half(p, p2); }, &exec));
+            SHKZ_TRY(graph_for(key + "B", [&]() { half(p2, p); }, &exec2));
+        } else {
+            SHKZ_TRY(graph_for(key, iteration, &exec));
         }
     }
-    const int per_replay = fused ? 2 : 1;  // iterations per replay of `iteration`
     int chunk = std::max(1, std::min(first_chunk, 64));
+    int phase = 0;  // fused: the half-iteration that comes next (0: direction in p, 1: in p2)
     BatchCtl hc{};
     for (;;) {
-        for (int it = 0; it < chunk; it += per_replay) {
-            if (exec) SHKZ_HIP(hipGraphLaunch(exec, K.s));
-            else iteration();
+        for (int it = 0; it < chunk; ++it) {
+            if (!fused) {
+                if (exec) SHKZ_HIP(hipGraphLaunch(exec, K.s));
+                else iteration();
+                continue;
+            }
+            if (exec) SHKZ_HIP(hipGraphLaunch(phase ? exec2 : exec, K.s));
+            else if (phase) half(p2, p);
+            else half(p, p2);
+            phase ^= 1;
         }
         SHKZ_TRY(read_scalars(C, K.s, S, P.ngrids, last, ctl, &hc, S2, last2));
         if (hc.all_done) break;
@@ -1751,7 +1821,9 @@ static int project_stages(shkz_ctx *C, const shkz_grid *grids, int ngrids, const
         }
     }
     P.nlev = (int)glevels.size();
-    C->levels.assign(P.nlev, LevelHost{});
+    // fresh levels that keep the previous projection's table storage: the tables of a 256^3 grid
+    // take about a megabyte, whose pages would otherwise be mapped anew (faulted in) on every call
+    recycle_levels(C->levels, P.nlev);
     // Level 0 first: its tables go up and the assembly starts on the device while the host builds
     // the coarser levels and the transfer tables.
     auto finish_level = [&](int l) {
@@ -1822,7 +1894,9 @@ static int project_stages(shkz_ctx *C, const shkz_grid *grids, int ngrids, const
         SHKZ_HIP(hipMemcpyAsync(dmeta, host, total * sizeof(int), hipMemcpyHostToDevice, s));
         return SHKZ_OK;
     };
+    const double t_hier = now_ms();
     finish_level(0);
+    const double t_built0 = now_ms();
     SHKZ_TRY(upload_tables(0, 1, false, "meta0", C->pin_meta0, true));
     const double t_level0 = now_ms();
 
@@ -1921,9 +1995,10 @@ static int project_stages(shkz_ctx *C, const shkz_grid *grids, int ngrids, const
     }
     if (C->trace) {
         std::fprintf(stderr,
-                     "[shkz] host ms: queue labelling %.3f, wait for brick flags %.3f, level 0 %.3f, queue assembly "
-                     "%.3f, coarse levels and tables %.3f\n",
-                     t_queued - t_start, t_frac - t_queued, t_level0 - t_frac, t_assembly - t_level0,
+                     "[shkz] host ms: queue labelling %.3f, wait for brick flags %.3f, level 0 %.3f (masks %.3f, tables %.3f), "
+                     "queue assembly %.3f, coarse levels and tables %.3f\n",
+                     t_queued - t_start, t_frac - t_queued, t_level0 - t_frac, t_hier - t_frac, t_built0 - t_hier,
+                     t_assembly - t_level0,
                      now_ms() - t_assembly);
         std::fprintf(stderr, "[shkz] %d grid(s), levels:", ngrids);
         for (int l = 0; l < P.nlev; ++l) std::fprintf(stderr, " %d%s", C->levels[l].nbricks, C->levels[l].any_coarsest ? "*" : "");
