@@ -181,8 +181,9 @@ def kernel_roofline(stats, inst_steps, config):
 
 
 def explicit_mg(spec):
-    """Params fields of an explicit --mg pre,post,coarse_pre,coarse_post,coarse_sweeps,omega,eta[,smoother]
-    (smoother: 1 = damped Jacobi, 2 = brick-local Gauss-Seidel on the finest level)."""
+    """Params fields of an explicit --mg pre,post,coarse_pre,coarse_post,coarse_sweeps,omega,eta[,smoother[,
+    local_sweeps]] (smoother: 1 = damped Jacobi, 2 = brick-local Gauss-Seidel on the finest level;
+    local_sweeps: its sweeps per finest-level application, 2 or 3)."""
     if not spec:
         return {}
     v = spec.split(",")
@@ -191,6 +192,8 @@ def explicit_mg(spec):
                inner_residual=float(v[6]))
     if len(v) > 7:
         out["mg_smoother"] = int(v[7])
+    if len(v) > 8:
+        out["mg_local_sweeps"] = int(v[8])
     return out
 
 
@@ -200,7 +203,7 @@ def mg_controls(r):
             "coarse_post": r.mg_coarse_post_sweeps, "coarse_solve_sweeps": r.mg_coarse_sweeps,
             "omega": r.mg_omega, "inner_residual": r.inner_residual,
             "smoother": {1: "jacobi", 2: "brick-local gauss-seidel"}.get(r.mg_smoother, r.mg_smoother),
-            "tuned": r.tuned}
+            "local_sweeps": r.mg_local_sweeps if r.mg_smoother == 2 else None, "tuned": r.tuned}
 
 
 def batched_block(args, steps, single_rounds=1, kernel_timing=False):

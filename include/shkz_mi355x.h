@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define SHKZ_ABI_VERSION 7
+#define SHKZ_ABI_VERSION 8
 
 /* ---------------------------------------------------------------- enums */
 enum shkz_status {
@@ -138,7 +138,9 @@ typedef struct shkz_params {
                                      uses the best controls).  Cached per process, and in the file
                                      named by the environment variable SHKZ_TUNE_CACHE if set. */
     int32_t mg_smoother;          /* finest-level smoother (shkz_smoother; 0 = library default / tuned) */
-    int32_t reserved[3];
+    int32_t mg_local_sweeps;      /* brick-local smoother: symmetric Gauss-Seidel sweeps per finest-level
+                                     application, 2 or 3 (0 = library default 2 / tuned) */
+    int32_t reserved[2];
 } shkz_params;
 
 /* ---------------------------------------------------------------- result */
@@ -172,7 +174,7 @@ typedef struct shkz_result {
     double mg_omega;
     double inner_residual;
     int32_t mg_smoother;          /* shkz_smoother used on the finest level */
-    int32_t pad;
+    int32_t mg_local_sweeps;      /* its sweeps per finest-level application (brick-local smoother) */
 } shkz_result;
 
 /* ---------------------------------------------------------------- context */

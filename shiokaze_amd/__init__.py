@@ -19,7 +19,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "lib", "libshkz_mi355x.so")
 
-ABI_VERSION = 7  # the layout of the structs below
+ABI_VERSION = 8  # the layout of the structs below
 F32, F64 = 0, 1
 SOLVER_MGPCG, SOLVER_CG = 0, 1
 PREC_MIXED, PREC_FP64 = 0, 1
@@ -62,7 +62,7 @@ class Params(ctypes.Structure):
         ("mg_omega", ctypes.c_double), ("inner_residual", ctypes.c_double),
         ("mg_coarse_pre_sweeps", ctypes.c_int32), ("mg_coarse_post_sweeps", ctypes.c_int32),
         ("mg_bottom_bricks", ctypes.c_int32), ("autotune", ctypes.c_int32),
-        ("mg_smoother", ctypes.c_int32), ("reserved", ctypes.c_int32 * 3),
+        ("mg_smoother", ctypes.c_int32), ("mg_local_sweeps", ctypes.c_int32), ("reserved", ctypes.c_int32 * 2),
     ]
 
 
@@ -80,7 +80,7 @@ class Result(ctypes.Structure):
         ("mg_coarse_pre_sweeps", ctypes.c_int32), ("mg_coarse_post_sweeps", ctypes.c_int32),
         ("mg_coarse_sweeps", ctypes.c_int32), ("tuned", ctypes.c_int32),
         ("mg_omega", ctypes.c_double), ("inner_residual", ctypes.c_double),
-        ("mg_smoother", ctypes.c_int32), ("pad", ctypes.c_int32),
+        ("mg_smoother", ctypes.c_int32), ("mg_local_sweeps", ctypes.c_int32),
     ]
 
     def as_dict(self) -> dict:

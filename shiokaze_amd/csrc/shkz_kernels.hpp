@@ -1193,7 +1193,7 @@ __global__ __launch_bounds__(ROW4_THREADS) void k_jacobi4_dot(LevelDesc L, OpDes
 // arrays per application, like one Jacobi sweep, smoothing about as much as three.  The coarse
 // levels use it too (three sweeps per application): there the pre-smoothing from zero runs inside
 // the restriction that forms the level's right-hand side (k_restrict_pre).
-constexpr int LOCAL_SGS_SWEEPS = 2;         // finest level
+constexpr int LOCAL_SGS_SWEEPS = 2;         // finest level: the default (2 or 3, a tuned control)
 constexpr int LOCAL_SGS_SWEEPS_COARSE = 3;  // coarse levels (one application per smoothing step)
 
 // The couplings of the lane's four cells to cells of the same brick (0 across the brick's faces):
@@ -1280,7 +1280,9 @@ __device__ __forceinline__ void local_sgs4(const Row4Coef<float> &c, const float
     e[3] = e3;
 }
 
-// Pre-smoothing from zero on the finest level: z = M_b rhs (one brick per 128-thread workgroup).
+// Pre-smoothing from zero on the finest level: z = M_b rhs (one brick per 128-thread workgroup), K
+// sweeps.
+template <int K>
 __global__ __launch_bounds__(ROW4_THREADS) void k_pre_local4(LevelDesc L, OpDesc<float> A,
                                                              const float *__restrict__ idiag,
                                                              const float *__restrict__ rhs, float *__restrict__ z,
@@ -1304,7 +1306,7 @@ __global__ __launch_bounds__(ROW4_THREADS) void k_pre_local4(LevelDesc L, OpDesc
         c = row4_local(row4_coef(A, N), q);
         unpack4(ld4(idiag, N.c), id);
     }
-    local_sgs4<LOCAL_SGS_SWEEPS>(c, id, r, e, lds, q);
+    local_sgs4<K>(c, id, r, e, lds, q);
     *reinterpret_cast<float4 *>(z + N.c) = make_float4(e[0], e[1], e[2], e[3]);
 }
 
@@ -1588,9 +1590,9 @@ __global__ __launch_bounds__(ROW4_THREADS) void k_apply_dot4p(LevelDesc L, OpDes
 // x64 += alpha p (FP64), r -= alpha q (FP32) with ||r||_inf and r.r partials; ZFIRST: also the first
 // pre-smoothing step of the next V-cycle from zero on the finest level, on the bricks it covers
 // (not those of grids ending on this level), with the same arithmetic: 1 = a damped Jacobi sweep
-// (k_smooth_first4<1>), z = omega * idiag * r; 2 = the brick-local smoother (k_pre_local4), whose
-// brick is the workgroup's.  A grid that this update finds converged gets a z it never uses.
-template <int ZFIRST>
+// (k_smooth_first4<1>), z = omega * idiag * r; 2 = the brick-local smoother (k_pre_local4<K>),
+// whose brick is the workgroup's.  A grid that this update finds converged gets a z it never uses.
+template <int ZFIRST, int K = LOCAL_SGS_SWEEPS>
 __global__ __launch_bounds__(ROW4_THREADS) void k_cg_update4(LevelDesc L, double *__restrict__ x,
                                                              float *__restrict__ r, const float *__restrict__ p,
                                                              const float *__restrict__ qv, const SolveScalars *S,
@@ -1646,7 +1648,7 @@ __global__ __launch_bounds__(ROW4_THREADS) void k_cg_update4(LevelDesc L, double
         // the brick-local smoother's pre-smoothing from zero (k_pre_local4) on the updated residual
         __shared__ float zl[BRICK_CELLS];
         float e[4];
-        local_sgs4<LOCAL_SGS_SWEEPS>(cf, id, rv, e, zl, threadIdx.x);
+        local_sgs4<K>(cf, id, rv, e, zl, threadIdx.x);
         *reinterpret_cast<float4 *>(z + c) = make_float4(e[0], e[1], e[2], e[3]);
     }
     row4_reduce2<true>(mx, sq, lds);

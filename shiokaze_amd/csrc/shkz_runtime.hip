@@ -629,6 +629,7 @@ struct Problem {
     RedBuf red{};
     int pre = 4, post = 4, pre_c = 4, post_c = 4, coarse_sweeps = 48;
     int smoother = SHKZ_SMOOTH_JACOBI;  // finest level (SHKZ_SMOOTH_BRICK only with an FP32 V-cycle)
+    int local_k = LOCAL_SGS_SWEEPS;      // brick-local smoother: sweeps per finest-level application (2, 3)
     double omega = 0.9;
     int multi_max_bricks = 1024;  // coarse levels up to this size smooth with multi-sweep launches
     double eta = 0.0;  // mixed precision: relative defect reduction asked of each inner solve (0: to the tolerance)
@@ -827,13 +828,15 @@ static T *vcycle(shkz_ctx *C, Launcher &K, Problem &P, const T *rhs0, T *z0, T *
             // update when first_done; on coarse levels by the restriction that formed rhs), then
             // further applications z += M_b (rhs - A z)
             if constexpr (sizeof(T) == 4) {
+                const bool k3 = P.local_k == 3;
                 if (f && !first_done)
                     K.go("pre_local_L0", 0, 6 * es, -usave(C, 0, 4 * es), dim3(brick_grid(L.nbricks)), dim3(ROW4_THREADS),
-                         k_pre_local4, L.desc, A, id, rhs, z, Q);
+                         k3 ? k_pre_local4<3> : k_pre_local4<2>, L.desc, A, id, rhs, z, Q);
                 for (int s = 1; s < pre; ++s) {
                     if (f)
                         K.go("post_local_L0", 0, 8 * es, -usave(C, 0, 5 * es), dim3(brick_grid(L.nbricks)),
-                             dim3(ROW4_THREADS), k_post_local4<0>, L.desc, A, id, rhs, (const T *)cur, oth, Q, P.red);
+                             dim3(ROW4_THREADS), k3 ? k_post_local4<0, 3> : k_post_local4<0, 2>, L.desc, A, id, rhs,
+                             (const T *)cur, oth, Q, P.red);
                     else
                         K.go("post_local_Lc", l, 8 * es, 0, dim3(brick_grid(L.nbricks)), dim3(ROW4_THREADS),
                              k_post_local4<0, LOCAL_SGS_SWEEPS_COARSE>, L.desc, A, id, rhs, (const T *)cur, oth, Q, P.red);
@@ -922,12 +925,15 @@ static T *vcycle(shkz_ctx *C, Launcher &K, Problem &P, const T *rhs0, T *z0, T *
             if constexpr (sizeof(T) == 4) {
                 if (brick_smoother) {
                     const dim3 gb(brick_grid(L.nbricks));
+                    const bool k3 = P.local_k == 3;
                     if (last_dot)
                         K.go("post_local_dot_L0", 0, 8 * es, -usave(C, 0, 5 * es), gb, dim3(ROW4_THREADS),
-                             k_post_local4<1>, L.desc, A, id, rhs, (const T *)cur, oth, Q, P.red);
+                             k3 ? k_post_local4<1, 3> : k_post_local4<1, 2>, L.desc, A, id, rhs, (const T *)cur, oth, Q,
+                             P.red);
                     else if (f)
                         K.go("post_local_L0", 0, 8 * es, -usave(C, 0, 5 * es), gb, dim3(ROW4_THREADS),
-                             k_post_local4<0>, L.desc, A, id, rhs, (const T *)cur, oth, Q, P.red);
+                             k3 ? k_post_local4<0, 3> : k_post_local4<0, 2>, L.desc, A, id, rhs, (const T *)cur, oth, Q,
+                             P.red);
                     else
                         K.go("post_local_Lc", l, 8 * es, 0, gb, dim3(ROW4_THREADS),
                              k_post_local4<0, LOCAL_SGS_SWEEPS_COARSE>, L.desc, A, id, rhs, (const T *)cur, oth, Q,
@@ -999,9 +1005,9 @@ template <class T>
 static std::string graph_key(shkz_ctx *C, const Problem &P, const void *const *vecs, int nvec, bool precond) {
     std::string k;
     char buf[256];
-    std::snprintf(buf, sizeof(buf), "T%zu pc%d g%d nl%d pre%d post%d prc%d poc%d cs%d mm%d sm%d om%.17g red%p|",
+    std::snprintf(buf, sizeof(buf), "T%zu pc%d g%d nl%d pre%d post%d prc%d poc%d cs%d mm%d sm%d lk%d om%.17g red%p|",
                   sizeof(T), precond ? 1 : 0, P.ngrids, P.nlev, P.pre, P.post, P.pre_c, P.post_c, P.coarse_sweeps,
-                  P.multi_max_bricks, P.smoother, P.omega, (void *)P.red.partials);
+                  P.multi_max_bricks, P.smoother, P.local_k, P.omega, (void *)P.red.partials);
     k += buf;
     for (int i = 0; i < nvec; ++i) {
         std::snprintf(buf, sizeof(buf), "%p,", vecs[i]);
@@ -1082,7 +1088,8 @@ static int krylov(shkz_ctx *C, Launcher &K, Problem &P, T *x, T *r, T *p, T *q, 
             const LevelHost &F0 = C->levels[0];
             const OpDesc<float> A32 = op_of<float>(F0);
             if (zfirst && P.smoother == SHKZ_SMOOTH_BRICK)
-                K.go("cg_update_z", 0, 52, -usave(C, 0, 16), g, dim3(ROW4_THREADS), k_cg_update4<2>, L0.desc, x64, r,
+                K.go("cg_update_z", 0, 52, -usave(C, 0, 16), g, dim3(ROW4_THREADS),
+                     P.local_k == 3 ? k_cg_update4<2, 3> : k_cg_update4<2, 2>, L0.desc, x64, r,
                      (const T *)pc, (const T *)q, (const SolveScalars *)S, (const BatchCtl *)ctl, P.red,
                      (const T *)F0.idiag, z, (T)P.omega, A32);
             else if (zfirst)
@@ -1294,8 +1301,8 @@ static std::string tune_key(const shkz_ctx *C, const Problem &P, const shkz_para
 // Key of a solve's iteration history: the shape class and the controls in use.
 static std::string hist_key(const shkz_ctx *C, const Problem &P, const shkz_params *prm) {
     char buf[160];
-    std::snprintf(buf, sizeof(buf), "|%d %d %d %d %d %d %d %.6g %.3g", P.pre, P.post, P.pre_c, P.post_c,
-                  P.coarse_sweeps, P.multi_max_bricks, P.smoother, P.omega, P.eta);
+    std::snprintf(buf, sizeof(buf), "|%d %d %d %d %d %d %d %d %.6g %.3g", P.pre, P.post, P.pre_c, P.post_c,
+                  P.coarse_sweeps, P.multi_max_bricks, P.smoother, P.local_k, P.omega, P.eta);
     return tune_key(C, P, prm) + buf;
 }
 
@@ -1454,10 +1461,12 @@ struct Knobs {
     int pre, post, pre_c, post_c, coarse, multi;
     double omega, eta;
     int smoother;
+    int local_k;
 };
 
 static Knobs knobs_of(const Problem &P) {
-    return Knobs{P.pre, P.post, P.pre_c, P.post_c, P.coarse_sweeps, P.multi_max_bricks, P.omega, P.eta, P.smoother};
+    return Knobs{P.pre,   P.post, P.pre_c,    P.post_c, P.coarse_sweeps, P.multi_max_bricks,
+                 P.omega, P.eta,  P.smoother, P.local_k};
 }
 
 // Sets P's controls to k except where the caller fixed a control in prm.
@@ -1470,6 +1479,7 @@ static void apply_knobs(Problem &P, const Knobs &k, const shkz_params *prm) {
     if (!(prm->mg_omega > 0)) P.omega = k.omega;
     if (!(prm->inner_residual > 0)) P.eta = k.eta;
     if (prm->mg_smoother <= 0 && (k.smoother == SHKZ_SMOOTH_JACOBI || P.mixed)) P.smoother = k.smoother;
+    if (prm->mg_local_sweeps <= 0) P.local_k = k.local_k;
     P.multi_max_bricks = k.multi;
 }
 
@@ -1493,10 +1503,11 @@ static void tune_load_locked() {
     while (std::fgets(line, sizeof(line), f)) {
         char key[320];
         Knobs k;
-        if (std::sscanf(line, "%319s %d %d %d %d %d %d %lf %lf %d", key, &k.pre, &k.post, &k.pre_c, &k.post_c,
-                        &k.coarse, &k.multi, &k.omega, &k.eta, &k.smoother) == 10 &&
+        if (std::sscanf(line, "%319s %d %d %d %d %d %d %lf %lf %d %d", key, &k.pre, &k.post, &k.pre_c, &k.post_c,
+                        &k.coarse, &k.multi, &k.omega, &k.eta, &k.smoother, &k.local_k) == 11 &&
             k.pre > 0 && k.post > 0 && k.pre_c > 0 && k.post_c > 0 && k.coarse > 0 && k.multi >= 0 && k.omega > 0 &&
-            k.omega <= 1.0 && k.eta >= 0 && (k.smoother == SHKZ_SMOOTH_JACOBI || k.smoother == SHKZ_SMOOTH_BRICK))
+            k.omega <= 1.0 && k.eta >= 0 && (k.smoother == SHKZ_SMOOTH_JACOBI || k.smoother == SHKZ_SMOOTH_BRICK) &&
+            (k.local_k == 2 || k.local_k == 3))
             g_tune.map[key] = k;
     }
     std::fclose(f);
@@ -1518,8 +1529,8 @@ static void tune_store(const std::string &key, const Knobs &k) {
     if (g_tune.path.empty()) return;
     FILE *f = std::fopen(g_tune.path.c_str(), "a");
     if (!f) return;
-    std::fprintf(f, "%s %d %d %d %d %d %d %.17g %.17g %d\n", key.c_str(), k.pre, k.post, k.pre_c, k.post_c, k.coarse,
-                 k.multi, k.omega, k.eta, k.smoother);
+    std::fprintf(f, "%s %d %d %d %d %d %d %.17g %.17g %d %d\n", key.c_str(), k.pre, k.post, k.pre_c, k.post_c,
+                 k.coarse, k.multi, k.omega, k.eta, k.smoother, k.local_k);
     std::fclose(f);
 }
 
@@ -1532,7 +1543,8 @@ static std::string tune_key(const shkz_ctx *C, const Problem &P, const shkz_para
     const int gclass = P.ngrids == 1 ? 0 : (int)std::ceil(std::log2((double)P.ngrids) / 3.0);
     const int fixed = (prm->mg_pre_sweeps > 0) | (prm->mg_post_sweeps > 0) << 1 | (prm->mg_coarse_pre_sweeps > 0) << 2 |
                       (prm->mg_coarse_post_sweeps > 0) << 3 | (prm->mg_coarse_sweeps > 0) << 4 |
-                      (prm->mg_omega > 0) << 5 | (prm->inner_residual > 0) << 6 | (prm->mg_smoother > 0) << 7;
+                      (prm->mg_omega > 0) << 5 | (prm->inner_residual > 0) << 6 | (prm->mg_smoother > 0) << 7 |
+                      (prm->mg_local_sweeps > 0) << 8;
     char buf[320];
     std::snprintf(buf, sizeof(buf), "%s|s%d|p%d|n%d|r%d|b%d|l%d|g%d|f%d", C->device_key.c_str(), prm->solver,
                   prm->precision, prm->norm, rclass, bclass, P.nlev, gclass, fixed);
@@ -1572,8 +1584,10 @@ static int autotune(shkz_ctx *C, Launcher &K, Problem &P, const shkz_params *prm
         double ms = 1e300;
         SHKZ_TRY(trial(k, &ms));
         if (C->trace)
-            std::fprintf(stderr, "[shkz] tune pre/post %d/%d coarse %d/%d cs %d multi %d omega %.2f eta %.0e: %.3f ms\n",
-                         k.pre, k.post, k.pre_c, k.post_c, k.coarse, k.multi, k.omega, k.eta, ms);
+            std::fprintf(stderr,
+                         "[shkz] tune sm %d pre/post %d/%d (x%d) coarse %d/%d cs %d multi %d omega %.2f eta %.0e: %.3f "
+                         "ms\n",
+                         k.smoother, k.pre, k.post, k.local_k, k.pre_c, k.post_c, k.coarse, k.multi, k.omega, k.eta, ms);
         if (ms < 0.99 * cur_ms) {
             cur = k;
             cur_ms = ms;
@@ -1627,6 +1641,20 @@ static int autotune(shkz_ctx *C, Launcher &K, Problem &P, const shkz_params *prm
             k.multi = v;
             SHKZ_TRY(consider(k));
         }
+    if (cur.smoother == SHKZ_SMOOTH_BRICK && prm->mg_local_sweeps <= 0) {
+        // sweeps per finest-level application of the brick-local smoother, then the sweep counts
+        // once more around the choice
+        Knobs k = cur;
+        k.local_k = cur.local_k == 2 ? 3 : 2;
+        SHKZ_TRY(consider(k));
+        if (sweeps_free && cur.local_k == k.local_k)
+            for (int v : {1, 2}) {
+                if (v == cur.pre && v == cur.post) continue;
+                Knobs k2 = cur;
+                k2.pre = k2.post = v;
+                SHKZ_TRY(consider(k2));
+            }
+    }
     if (P.mixed && !(prm->inner_residual > 0))
         for (double v : {0.0, 1e-2, 1e-3, 1e-4}) {
             if (v == cur.eta) continue;
@@ -1648,6 +1676,8 @@ static int project_stages(shkz_ctx *C, const shkz_grid *grids, int ngrids, const
     if (prm->solver != SHKZ_SOLVER_MGPCG && prm->solver != SHKZ_SOLVER_CG) return fail(SHKZ_ERR_INVALID, "unknown solver %d", prm->solver);
     if (prm->precision != SHKZ_PREC_MIXED && prm->precision != SHKZ_PREC_FP64)
         return fail(SHKZ_ERR_INVALID, "unknown precision %d", prm->precision);
+    if (prm->mg_local_sweeps != 0 && prm->mg_local_sweeps != 2 && prm->mg_local_sweeps != 3)
+        return fail(SHKZ_ERR_INVALID, "mg_local_sweeps must be 0, 2 or 3 (got %d)", prm->mg_local_sweeps);
     const bool flat = is_flat(&grids[0]);
     for (int g = 1; g < ngrids; ++g)
         if (is_flat(&grids[g]) != flat) return fail(SHKZ_ERR_INVALID, "a batch cannot mix 2D and 3D grids");
@@ -1675,6 +1705,7 @@ static int project_stages(shkz_ctx *C, const shkz_grid *grids, int ngrids, const
     const bool brick = P.smoother == SHKZ_SMOOTH_BRICK;
     P.pre = prm->mg_pre_sweeps > 0 ? prm->mg_pre_sweeps : (brick ? 1 : 4);
     P.post = prm->mg_post_sweeps > 0 ? prm->mg_post_sweeps : (brick ? 1 : 4);
+    P.local_k = prm->mg_local_sweeps > 0 ? prm->mg_local_sweeps : LOCAL_SGS_SWEEPS;
     P.pre_c = prm->mg_coarse_pre_sweeps > 0 ? prm->mg_coarse_pre_sweeps : (brick ? 1 : P.pre);
     P.post_c = prm->mg_coarse_post_sweeps > 0 ? prm->mg_coarse_post_sweeps : (brick ? 1 : P.post);
     P.coarse_sweeps = prm->mg_coarse_sweeps > 0 ? prm->mg_coarse_sweeps : 48;
@@ -2204,6 +2235,7 @@ static int project_stages(shkz_ctx *C, const shkz_grid *grids, int ngrids, const
         res[g].mg_coarse_sweeps = P.coarse_sweeps;
         res[g].mg_omega = P.omega;
         res[g].mg_smoother = P.smoother;
+        res[g].mg_local_sweeps = P.local_k;
         res[g].inner_residual = P.eta;
         res[g].tuned = tuned;
     }
