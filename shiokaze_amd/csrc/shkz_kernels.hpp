@@ -286,10 +286,24 @@ __global__ __launch_bounds__(BLOCK) void k_flags(GridDesc G, int nbx, int nby, i
 // liquid (then q is an unknown too); with area>0 the rhs gains -sgn*area*u/dx.  Stored: the
 // low-face couplings gx/gy/gz and dir = the sum of conductances to non-unknown neighbours
 // (so diag = dir + sum of the six couplings); dir = -1 marks a non-unknown.
+//
+// The same block also finishes the brick for the solver: the inverse diagonal the smoothers use
+// (k_idiag's arithmetic on the stored FP32 (id32) or FP64 (id64) values, each coupling of a high
+// face being the neighbour's low one, formed here from the same samples), the brick's unknown count
+// (k_brick_counts), and whether its operator is one constant (k_uniform: then BM_UNIFORM and its
+// UniCoef record).
+struct OperatorFinish {
+    float *id32;      // FP32 inverse diagonal (the mixed V-cycle) or nullptr
+    double *id64;     // FP64 inverse diagonal (FP64 V-cycle) or nullptr
+    BrickMeta *meta;  // level-0 brick records (BM_UNIFORM is or-ed in)
+    UniCoef *uni;
+    int *counts;      // unknowns per brick (| UNIFORM_COUNT_BIT)
+};
 __global__ __launch_bounds__(BLOCK) void k_operator(GridDesc G, LevelDesc L, GridLevel GL, const uint8_t *__restrict__ cflag,
                                                     double *dir64, double *gx64, double *gy64, double *gz64,
                                                     float *dir32, float *gx32, float *gy32, float *gz32, double *b,
-                                                    double *x, const double *__restrict__ warm, double rhs_correct) {
+                                                    double *x, const double *__restrict__ warm, double rhs_correct,
+                                                    OperatorFinish FIN) {
 #pragma clang fp contract(off)
     const int local = xcd_brick(blockIdx.x, gridDim.x);
     if (local >= GL.brick_end - GL.brick_start) return;
@@ -301,6 +315,7 @@ __global__ __launch_bounds__(BLOCK) void k_operator(GridDesc G, LevelDesc L, Gri
     const bool inside = i < G.w && j < G.h && k < G.d;
     const bool unknown = inside && (cflag[cell_index(G, i, j, k)] & 2);
     double dir = -1.0, g[3] = {0.0, 0.0, 0.0}, rhs = 0.0, x0 = 0.0;
+    double gp[3] = {0.0, 0.0, 0.0};  // couplings of the high faces (the neighbours' low ones)
     if (unknown && !G.flat) {
         // 3D: the cell's 8 solid corners, 7 level-set samples and 6 face velocities are loaded at
         // once; each face's area and liquid fraction then use the face functions' arithmetic
@@ -357,6 +372,7 @@ __global__ __launch_bounds__(BLOCK) void k_operator(GridDesc G, LevelDesc L, Gri
                     const double value = dt * (double)area / (dx * dx * rho);
                     if (ph[nq] < 0.0) {
                         if (nq & 1) g[dim] = value;
+                        else gp[dim] = value;
                     } else {
                         dsum += value;
                     }
@@ -385,6 +401,7 @@ __global__ __launch_bounds__(BLOCK) void k_operator(GridDesc G, LevelDesc L, Gri
                     const double value = dt * area / (dx * dx * rho);
                     if (phi_at(G, qi, qj, qk) < 0.0) {
                         if (nq & 1) g[dim] = value;
+                        else gp[dim] = value;
                     } else {
                         dsum += value;
                     }
@@ -415,6 +432,48 @@ __global__ __launch_bounds__(BLOCK) void k_operator(GridDesc G, LevelDesc L, Gri
         gx32[c] = (float)g[0];
         gy32[c] = (float)g[1];
         gz32[c] = (float)g[2];
+    }
+    // k_idiag: diag = dir + the six couplings, in the stored type
+    float i32 = 0.0f;
+    double i64 = 0.0;
+    if (dir >= 0) {
+        if (FIN.id32) {
+            const float d = (float)dir + (((float)g[0] + (float)gp[0]) + ((float)g[1] + (float)gp[1]) +
+                                          ((float)g[2] + (float)gp[2]));
+            if (d > 0) i32 = 1.0f / d;
+        }
+        if (FIN.id64) {
+            const double d = dir + ((g[0] + gp[0]) + (g[1] + gp[1]) + (g[2] + gp[2]));
+            if (d > 0) i64 = 1.0 / d;
+        }
+    }
+    if (FIN.id32) FIN.id32[c] = i32;
+    if (FIN.id64) FIN.id64[c] = i64;
+    // k_brick_counts and k_uniform: the brick's first cell gives the reference values
+    __shared__ double gref, iref64;
+    __shared__ float iref32;
+    if (t == 0) {
+        gref = g[0];
+        iref64 = i64;
+        iref32 = i32;
+    }
+    const int n = __syncthreads_count(dir >= 0);
+    const double Gu = gref;
+    const bool ok = Gu > 0.0 && dir == 0.0 && g[0] == Gu && g[1] == Gu && g[2] == Gu && gp[0] == Gu && gp[1] == Gu &&
+                    gp[2] == Gu && (!FIN.id64 || i64 == iref64) && (!FIN.id32 || i32 == iref32);
+    const bool uniform = __syncthreads_and(ok);
+    if (t == 0) {
+        if (uniform) {
+            FIN.meta[br].flags = L.meta[br].flags | BM_UNIFORM;
+            UniCoef u;
+            u.g64 = Gu;
+            u.id64 = iref64;
+            u.g32 = (float)Gu;
+            u.id32 = iref32;
+            u.pad[0] = u.pad[1] = 0.0f;
+            FIN.uni[br] = u;
+        }
+        FIN.counts[br] = n | (uniform ? UNIFORM_COUNT_BIT : 0);
     }
 }
 

@@ -1957,6 +1957,27 @@ static int project_stages(shkz_ctx *C, const shkz_grid *grids, int ngrids, const
         F0.gy = P.gy64;
         F0.gz = P.gz64;
     }
+    // the assembly also writes level 0's inverse diagonal (the V-cycle's), the brick counts and the
+    // BM_UNIFORM records
+    const size_t es = P.fp64_levels ? 8 : 4;
+    // stored bricks of all levels (the coarse levels are built later: their active positions)
+    int count_total = 0;
+    for (const std::vector<GridLevelHost> &lv : glevels)
+        for (const GridLevelHost &H : lv)
+            for (uint8_t a : H.active) count_total += a != 0;
+    int *counts = nullptr;
+    SHKZ_TRY(arena_get(AR, "counts", (size_t)std::max(count_total, 1), &counts));
+    {
+        void *pp = nullptr;
+        SHKZ_TRY(AR.get("idiag_l0", n0 * es, &pp));
+        F0.idiag = multigrid ? pp : nullptr;
+        UniCoef *uni = nullptr;
+        SHKZ_TRY(arena_get(AR, "uni", (size_t)std::max(F0.nbricks, 1), &uni));
+        F0.desc.uni = uni;
+    }
+    const OperatorFinish FIN{multigrid && !P.fp64_levels ? (float *)F0.idiag : nullptr,
+                             multigrid && P.fp64_levels ? (double *)F0.idiag : nullptr,
+                             const_cast<BrickMeta *>(F0.desc.meta), const_cast<UniCoef *>(F0.desc.uni), counts};
     // per grid: volume correction (macpressuresolver3.cpp:203-217), warm start, assembly
     for (int g = 0; g < ngrids; ++g) {
         GridJob &J = jobs[g];
@@ -1979,7 +2000,7 @@ static int project_stages(shkz_ctx *C, const shkz_grid *grids, int ngrids, const
              (double)nbg * BRICK_CELLS *
                  ((J.G.phi64 ? 8 : 4) + (grid->solid ? 4 : 0) + 12 + 1 + 6 * 8 + (P.mixed ? 16 : 0)),
              dim3(brick_grid(nbg)), dim3(BLOCK), k_operator, J.G, F0.desc, GL, (const uint8_t *)J.cflag, P.dir64,
-             P.gx64, P.gy64, P.gz64, dir32, gx32, gy32, gz32, P.b64, P.x64, warm, J.rhs_correct);
+             P.gx64, P.gy64, P.gz64, dir32, gx32, gy32, gz32, P.b64, P.x64, warm, J.rhs_correct, FIN);
     }
     // the coarser levels and the transfer tables, built on the host while the assembly runs
     const double t_assembly = now_ms();
@@ -2044,7 +2065,6 @@ static int project_stages(shkz_ctx *C, const shkz_grid *grids, int ngrids, const
     P.S_fin = Sbuf + 2 * ngrids;
     SHKZ_TRY(arena_get(AR, "ctl", 2, &P.ctl));
 
-    const size_t es = P.fp64_levels ? 8 : 4;
     for (int l = 1; l < P.nlev; ++l) {
         LevelHost &L = C->levels[l];
         const size_t n = (size_t)std::max(L.nbricks, 1) * BRICK_CELLS;
@@ -2073,8 +2093,6 @@ static int project_stages(shkz_ctx *C, const shkz_grid *grids, int ngrids, const
         void *pp = nullptr;
         SHKZ_TRY(AR.get("res_l0", n0 * es, &pp));
         F0.res = pp;
-        SHKZ_TRY(AR.get("idiag_l0", n0 * es, &pp));
-        F0.idiag = pp;
     }
     const size_t ks = P.mixed ? 4 : 8;
     SHKZ_TRY(AR.get("kp", n0 * ks, &P.kp));
@@ -2119,9 +2137,9 @@ static int project_stages(shkz_ctx *C, const shkz_grid *grids, int ngrids, const
                      k_coarse_components<float>, L.desc, op_of<float>(L), L.ccode);
         }
     }
-    // inverse diagonals for the smoothers
+    // inverse diagonals for the smoothers (level 0: written by the assembly)
     if (multigrid) {
-        for (int l = 0; l < P.nlev; ++l) {
+        for (int l = 1; l < P.nlev; ++l) {
             LevelHost &L = C->levels[l];
             const dim3 g(brick_grid(L.nbricks)), b(BLOCK);
             if (P.fp64_levels)
@@ -2130,31 +2148,19 @@ static int project_stages(shkz_ctx *C, const shkz_grid *grids, int ngrids, const
                 K.go("idiag", -1, 0, 0, g, b, k_idiag<float>, L.desc, op_of<float>(L), (float *)L.idiag);
         }
     }
-    // level-0 bricks with a constant operator (BM_UNIFORM): the smoothers and operator kernels
-    // read their constants instead of the operator arrays
-    {
-        LevelHost &L = C->levels[0];
-        UniCoef *uni = nullptr;
-        SHKZ_TRY(arena_get(AR, "uni", (size_t)std::max(L.nbricks, 1), &uni));
-        L.desc.uni = uni;
-        const bool vc = multigrid && L.idiag;
-        K.go("uniform", -1, 0, 0, dim3(brick_grid(L.nbricks)), dim3(BLOCK), k_uniform, L.desc,
-             OpDesc<double>{P.dir64, P.gx64, P.gy64, P.gz64}, vc && P.fp64_levels ? (const double *)L.idiag : nullptr,
-             vc && !P.fp64_levels ? (const float *)L.idiag : nullptr, const_cast<BrickMeta *>(L.desc.meta), uni);
-    }
+    // level-0 bricks with a constant operator (BM_UNIFORM, marked by the assembly): the smoothers
+    // and operator kernels read their constants instead of the operator arrays
     // unknowns per level and grid (results + byte model of the kernel timers)
     {
         int total = 0;
-        for (const LevelHost &L : C->levels) total += L.nbricks;
-        int *cnt = nullptr;
-        SHKZ_TRY(arena_get(AR, "counts", (size_t)std::max(total, 1), &cnt));
-        int off = 0;
-        for (int l = 0; l < P.nlev; ++l) {
+        for (int l = 0; l < P.nlev; ++l) total += C->levels[l].nbricks;
+        if (total != count_total) return fail(SHKZ_ERR_DEVICE, "internal: %d stored bricks, %d counted", total, count_total);
+        int *cnt = counts;
+        int off = C->levels[0].nbricks;  // level 0: counted by the assembly
+        for (int l = 1; l < P.nlev; ++l) {
             LevelHost &L = C->levels[l];
             const dim3 g(brick_grid(L.nbricks)), b(BLOCK);
-            if (l == 0)
-                K.go("count", -1, 0, 0, g, b, k_brick_counts<double>, L.desc, (const double *)P.dir64, cnt + off);
-            else if (P.fp64_levels)
+            if (P.fp64_levels)
                 K.go("count", -1, 0, 0, g, b, k_brick_counts<double>, L.desc, (const double *)L.dir, cnt + off);
             else
                 K.go("count", -1, 0, 0, g, b, k_brick_counts<float>, L.desc, (const float *)L.dir, cnt + off);
