@@ -1306,28 +1306,40 @@ __device__ __forceinline__ void local_sgs4(const Row4Coef<float> &c, const float
         }
     }
     __syncthreads();
-    auto cell = [&](int j, float cur) -> float {
-        const int at = base + j, lk = 4 * h + j;
+    // cell j's update from its x neighbours (other lanes, LDS), its y neighbours ym, yp and its z
+    // neighbours zm, zp: the lane's own cells j - 1 and j + 1 come from registers (the values LDS
+    // holds), only the row's end cells read the partner lane's value
+    auto cell = [&](int j, float ym, float yp, float zm, float zp) -> float {
+        const int at = base + j;
         const float xm = li > 0 ? lds[at - 64] : 0.0f, xp = li < 7 ? lds[at + 64] : 0.0f;
-        const float ym = lj > 0 ? lds[at - 8] : 0.0f, yp = lj < 7 ? lds[at + 8] : 0.0f;
-        const float zm = lk > 0 ? lds[at - 1] : 0.0f, zp = lk < 7 ? lds[at + 1] : 0.0f;
         const float nb = c.gxm[j] * xm + c.gxp[j] * xp + c.gym[j] * ym + c.gyp[j] * yp + c.gzm[j] * zm + c.gzp[j] * zp;
-        (void)cur;
         return id[j] * (r[j] + nb);
+    };
+    // the y neighbours' cells: lanes q - 2 and q + 2 of the same 16-lane row (the same x layer),
+    // moved across lanes by DPP with every lane active; lj = 0 and lj = 7 read 0 (outside the brick)
+    auto ymv = [](float v) {
+        return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x112, 0xF, 0xF, true));
+    };
+    auto ypv = [](float v) {
+        return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x102, 0xF, 0xF, true));
     };
 #pragma unroll
     for (int s = 1; s <= 2 * K; ++s) {
         // this half-sweep updates the cells of colour s & 1: cells 0 and 2 when it equals col0
         // (a half-sweep reads only cells of the other colour, so its stores race with no read)
         const bool even = (s & 1) == col0;
+        const float ym0 = ymv(e0), ym1 = ymv(e1), ym2 = ymv(e2), ym3 = ymv(e3);
+        const float yp0 = ypv(e0), yp1 = ypv(e1), yp2 = ypv(e2), yp3 = ypv(e3);
         if (even) {
-            e0 = cell(0, e0);
-            e2 = cell(2, e2);
+            // cell 0 (k = 4h): k - 1 is the partner's cell 3 when h = 1, outside the brick when h = 0
+            e0 = cell(0, ym0, yp0, h ? lds[base - 1] : 0.0f, e1);
+            e2 = cell(2, ym2, yp2, e1, e3);
             lds[base + 0] = e0;
             lds[base + 2] = e2;
         } else {
-            e1 = cell(1, e1);
-            e3 = cell(3, e3);
+            // cell 3 (k = 4h + 3): k + 1 is the partner's cell 0 when h = 0, outside when h = 1
+            e1 = cell(1, ym1, yp1, e0, e2);
+            e3 = cell(3, ym3, yp3, e2, h ? 0.0f : lds[base + 4]);
             lds[base + 1] = e1;
             lds[base + 3] = e3;
         }
@@ -1369,10 +1381,11 @@ __global__ __launch_bounds__(ROW4_THREADS) void k_pre_local4(LevelDesc L, OpDesc
     *reinterpret_cast<float4 *>(z + N.c) = make_float4(e[0], e[1], e[2], e[3]);
 }
 
-// Post-smoothing (and further pre-smoothing) on the finest level: zout = zin + M_b (rhs - A zin);
+// Post-smoothing (and further pre-smoothing) with the brick-local smoother: zout = zin + M_b (rhs - A zin);
 // DOT: also the rhs.zout partials of the unknowns (the last post-smoothing pass yields r.z of the
-// preconditioned CG).  One brick per 128-thread workgroup.
-template <int DOT, int K = LOCAL_SGS_SWEEPS>
+// preconditioned CG).  One brick per 128-thread workgroup.  FINEST: 1 on the finest level, 0 on
+// the coarse ones (the same code; distinct symbols for profiling).
+template <int DOT, int K = LOCAL_SGS_SWEEPS, int FINEST = 1>
 __global__ __launch_bounds__(ROW4_THREADS) void k_post_local4(LevelDesc L, OpDesc<float> A,
                                                               const float *__restrict__ idiag,
                                                               const float *__restrict__ rhs,

@@ -839,7 +839,7 @@ static T *vcycle(shkz_ctx *C, Launcher &K, Problem &P, const T *rhs0, T *z0, T *
                              (const T *)cur, oth, Q, P.red);
                     else
                         K.go("post_local_Lc", l, 8 * es, 0, dim3(brick_grid(L.nbricks)), dim3(ROW4_THREADS),
-                             k_post_local4<0, LOCAL_SGS_SWEEPS_COARSE>, L.desc, A, id, rhs, (const T *)cur, oth, Q, P.red);
+                             k_post_local4<0, LOCAL_SGS_SWEEPS_COARSE, 0>, L.desc, A, id, rhs, (const T *)cur, oth, Q, P.red);
                     std::swap(cur, oth);
                 }
             }
@@ -936,7 +936,7 @@ static T *vcycle(shkz_ctx *C, Launcher &K, Problem &P, const T *rhs0, T *z0, T *
                              P.red);
                     else
                         K.go("post_local_Lc", l, 8 * es, 0, gb, dim3(ROW4_THREADS),
-                             k_post_local4<0, LOCAL_SGS_SWEEPS_COARSE>, L.desc, A, id, rhs, (const T *)cur, oth, Q,
+                             k_post_local4<0, LOCAL_SGS_SWEEPS_COARSE, 0>, L.desc, A, id, rhs, (const T *)cur, oth, Q,
                              P.red);
                     std::swap(cur, oth);
                     continue;
