@@ -42,8 +42,8 @@ def log(*a):
 
 SYMBOL = {"jacobi_L0": "k_jacobi4<1>", "jacobi_dot_L0": "k_jacobi4_dot",
           "residual_L0": "k_residual4<1>", "smooth2_L0": "k_smooth2<float, 1>",
-          "cg_update_z": "k_cg_update4<2>", "post_local_L0": "k_post_local4<0, 2>",
-          "post_local_dot_L0": "k_post_local4<1, 2>", "apply_dot_p": "k_apply_dot4p"}
+          "cg_update_z": "k_cg_update4<2, 3>", "post_local_L0": "k_post_local4<0, 3, 1>",
+          "post_local_dot_L0": "k_post_local4<1, 3, 1>", "apply_dot_p": "k_apply_dot4p"}
 
 
 def measured_traffic(kernel, config):

@@ -290,8 +290,9 @@ __global__ __launch_bounds__(BLOCK) void k_flags(GridDesc G, int nbx, int nby, i
 // The same block also finishes the brick for the solver: the inverse diagonal the smoothers use
 // (k_idiag's arithmetic on the stored FP32 (id32) or FP64 (id64) values, each coupling of a high
 // face being the neighbour's low one, formed here from the same samples), the brick's unknown count
-// (k_brick_counts), and whether its operator is one constant (k_uniform: then BM_UNIFORM and its
-// UniCoef record).
+// (k_brick_counts' count, which the coarse levels still launch), and whether its operator is one
+// constant: every cell an unknown, dir = 0, all six couplings and the inverse diagonal equal to the
+// brick's first cell's (then BM_UNIFORM and its UniCoef record).
 struct OperatorFinish {
     float *id32;      // FP32 inverse diagonal (the mixed V-cycle) or nullptr
     double *id64;     // FP64 inverse diagonal (FP64 V-cycle) or nullptr
@@ -449,7 +450,7 @@ __global__ __launch_bounds__(BLOCK) void k_operator(GridDesc G, LevelDesc L, Gri
     }
     if (FIN.id32) FIN.id32[c] = i32;
     if (FIN.id64) FIN.id64[c] = i64;
-    // k_brick_counts and k_uniform: the brick's first cell gives the reference values
+    // unknown count and the uniformity vote: the brick's first cell gives the reference values
     __shared__ double gref, iref64;
     __shared__ float iref32;
     if (t == 0) {
