@@ -776,28 +776,39 @@ __device__ __forceinline__ void mark_done(SolveScalars &S, BatchCtl *ctl) {
 
 // One block per grid combines the partials of the grid's bricks in brick order and updates the
 // grid's scalars.  Grids that finished are left alone (their kernels stored no partials).
+// Every load is issued before the done test resolves: a single grid's brick range is the level's
+// (no dependent load of its GridLevel), and the grid's scalars are copied in the same round trip,
+// so the update after the reduction reads no memory (a chain of four dependent round trips became
+// one).  The summation order is the brick order as before.
 __global__ __launch_bounds__(1024) void k_finalize(const double *__restrict__ partials, LevelDesc L0,
                                                    SolveScalars *Sarr, BatchCtl *ctl, int op, double param, int iparam,
                                                    const SolveScalars *outer) {
     __shared__ double lds[32];
     const int g = blockIdx.x;
     SolveScalars &S = Sarr[g];
+    const SolveScalars s0 = S;
+    const bool stop = s0.done || (ctl && ctl->all_done);
+    int bs = 0, be = L0.nbricks;
+    if (L0.ngrids > 1) {
+        const GridLevel G = L0.gl[g];
+        bs = G.brick_start;
+        be = G.brick_end;
+    }
     const bool always = (op == FIN_DEFECT_FIRST || op == FIN_DEFECT || op == FIN_INNER_INIT);
-    if (!always && (S.done || (ctl && ctl->all_done))) return;
-    const GridLevel G = L0.gl[g];
     const bool a_is_max = (op == FIN_UPDATE || op == FIN_UPDATE_CG || op == FIN_DEFECT_FIRST || op == FIN_DEFECT ||
                            op == FIN_INNER_INIT || op == FIN_OUTER);
     double a = 0.0, b = 0.0;
     const double2 *pp = reinterpret_cast<const double2 *>(partials);
     // up to 16 loads in flight per thread (16K bricks per pass); past the grid's end a thread adds
-    // zeros, which change neither the sums nor the maxima (of norms)
+    // zeros, which change neither the sums nor the maxima (of norms).  A finished grid's partials
+    // are stale but in bounds, and its result is dropped below.
     constexpr int U = 16;
-    for (int i = G.brick_start + threadIdx.x; i < G.brick_end; i += U * blockDim.x) {
+    for (int i = bs + threadIdx.x; i < be; i += U * blockDim.x) {
         double2 v[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             const int j = i + u * (int)blockDim.x;
-            v[u] = j < G.brick_end ? pp[j] : make_double2(0.0, 0.0);
+            v[u] = j < be ? pp[j] : make_double2(0.0, 0.0);
         }
 #pragma unroll
         for (int u = 0; u < U; ++u) {
@@ -805,13 +816,14 @@ __global__ __launch_bounds__(1024) void k_finalize(const double *__restrict__ pa
             b += v[u].y;
         }
     }
+    if (!always && stop) return;
     if (a_is_max) block_reduce2<true>(a, b, lds);
     else block_reduce2<false>(a, b, lds);
     if (threadIdx.x != 0) return;
     switch (op) {
     case FIN_PQ: {
         S.pq = a;
-        const double al = S.rz / a;
+        const double al = s0.rz / a;
         S.alpha = al;
         if (!(al == al) || al - al != 0.0) {
             S.status = 1;
@@ -824,39 +836,41 @@ __global__ __launch_bounds__(1024) void k_finalize(const double *__restrict__ pa
         const double rl2 = sqrt(b);
         S.rinf = a;
         S.rl2 = rl2;
-        const int it = S.iter + 1;
+        const int it = s0.iter + 1;
         S.iter = it;
-        const double cur = S.norm == 0 ? a : rl2;
+        const double cur = s0.norm == 0 ? a : rl2;
         if (op == FIN_UPDATE_CG) {
-            S.beta = b / S.rz;
+            S.beta = b / s0.rz;
             S.rz = b;
         }
         // an inner solve whose FP32 residual stops reaching new minima hands back to the FP64
         // defect correction (a consistent singular system, for one, keeps a null-space component
         // of the FP32 rounding that no iteration removes)
         bool stalled = false;
-        if (S.stall_limit > 0) {
-            if (cur < S.rmin) {
+        if (s0.stall_limit > 0) {
+            if (cur < s0.rmin) {
                 S.rmin = cur;
                 S.stall = 0;
             } else {
-                stalled = ++S.stall >= S.stall_limit;
+                const int st = s0.stall + 1;
+                S.stall = st;
+                stalled = st >= s0.stall_limit;
             }
         }
         if (!(b == b)) {
             S.status = 1;
             mark_done(S, ctl);
-        } else if (cur <= S.tol) {
+        } else if (cur <= s0.tol) {
             S.converged = 1;
             mark_done(S, ctl);
-        } else if (it >= S.maxit || stalled) {
+        } else if (it >= s0.maxit || stalled) {
             mark_done(S, ctl);
         }
         break;
     }
     case FIN_RZ_FIRST:
     case FIN_RZ: {
-        S.beta = op == FIN_RZ_FIRST ? 0.0 : a / S.rz;
+        S.beta = op == FIN_RZ_FIRST ? 0.0 : a / s0.rz;
         S.rz = a;
         if (!(a == a)) {
             S.status = 1;
@@ -875,9 +889,9 @@ __global__ __launch_bounds__(1024) void k_finalize(const double *__restrict__ pa
         if (op != FIN_DEFECT) {
             S.r0inf = a;
             S.r0l2 = rl2;
-            const double r0 = S.norm == 0 ? a : rl2;
+            const double r0 = s0.norm == 0 ? a : rl2;
             double rel = param;
-            int maxit = S.maxit;
+            int maxit = s0.maxit;
             if (op == FIN_INNER_INIT) {
                 const SolveScalars &O = outer[g];
                 const double ocur = O.norm == 0 ? O.rinf : O.rl2;
@@ -900,19 +914,20 @@ __global__ __launch_bounds__(1024) void k_finalize(const double *__restrict__ pa
     }
     case FIN_OUTER: {
         const SolveScalars &I = outer[g];
-        const double prev = S.norm == 0 ? S.rinf : S.rl2;
+        const double prev = s0.norm == 0 ? s0.rinf : s0.rl2;
         const double rl2 = sqrt(b);
         S.rinf = a;
         S.rl2 = rl2;
-        S.iter += I.iter;
-        const double cur = S.norm == 0 ? a : rl2;
+        const int it = s0.iter + I.iter;
+        S.iter = it;
+        const double cur = s0.norm == 0 ? a : rl2;
         if (!(b == b) || I.status) {
             S.status = 1;
             mark_done(S, ctl);
-        } else if (cur <= S.tol) {
+        } else if (cur <= s0.tol) {
             S.converged = 1;
             mark_done(S, ctl);
-        } else if (S.iter >= S.maxit || cur > 0.9 * prev) {
+        } else if (it >= s0.maxit || cur > 0.9 * prev) {
             mark_done(S, ctl);
         }
         break;
@@ -1280,76 +1295,136 @@ __device__ __forceinline__ Row4Coef<float> row4_local_uniform(float g, int q) {
     return row4_local(u, q);
 }
 
+// One cell of a lane in the brick-local sweep: what its update needs besides the neighbour values
+// (its x couplings and LDS slots, y couplings, three z weights, inverse diagonal and right-hand side).
+struct SgsCell {
+    float ax_m, ax_p, ay_m, ay_p;  // couplings to the x and y neighbours (0 across the brick's faces)
+    float za, zb, zs;              // weights of the z terms: the two own cells of the other group and
+                                   // the partner lane's swapped value (see local_sgs4)
+    float id, r;
+    int xm, xp, st;                // LDS slots: x neighbours (clamped in-bounds where the coupling is
+                                   // 0) and the cell's own slot
+};
+
 // e = M_b r on the brick of a 128-thread workgroup (lane q: cells 4q .. 4q+3 of the brick, the
-// storage order).  lds: 512 floats.  Every lane of the workgroup must call it (barriers).  The
-// half-sweep loop is unrolled and each cell's value is a plain register (a select per half-sweep),
-// so nothing spills to scratch.
+// storage order).  lds: 512 floats.  Every lane of the workgroup must call it (barriers, DPP).
+//
+// Cell j of the lane has colour col0 ^ (j & 1) (col0 = (li + lj) & 1), so each half-sweep updates
+// two cells of every lane: group P (j = col0, col0 + 2) in the even half-sweeps (the first one, from
+// zero, included), group Q (j = 1 - col0, 3 - col0) in the odd ones.  Held by group, the update has
+// no divergent branch (adjacent lanes have opposite col0, so a branch on the colour would run both
+// sides in every wavefront), and the neighbours of an updated cell are fixed registers:
+//  - y: the lane two below / above in the same 16-lane row has the opposite col0, so the same cell
+//    there belongs to the other group at the same position: one DPP row shift per value (0 outside
+//    the row: lj = 0 / 7, outside the brick);
+//  - z: the lane's own cells of the other group, and at the row's ends the partner lane's
+//    (q ^ 1, same col0) by a DPP lane swap; the weights za, zb, zs place the two couplings;
+//  - x: lanes q -+ 16 (other wavefront at li = 3 / 4), through LDS.
+// A coupling across the brick's faces is 0 and its value is read from an in-bounds slot that holds
+// a sweep value, so the product is a zero.  The sums use fused multiply-adds in a fixed order.
+constexpr int DPP_ROW_SHR2 = 0x112, DPP_ROW_SHL2 = 0x102, DPP_SWAP_PAIRS = 0xB1;  // quad_perm [1,0,3,2]
+template <int CTRL>
+__device__ __forceinline__ float sgs_dpp(float v) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL, 0xF, 0xF, true));
+}
+
+__device__ __forceinline__ float sgs_update(const SgsCell &c, const float *lds, float ym, float yp, float z0, float z1,
+                                            float zsw) {
+    float acc = c.r;
+    acc = __builtin_fmaf(c.ax_m, lds[c.xm], acc);
+    acc = __builtin_fmaf(c.ax_p, lds[c.xp], acc);
+    acc = __builtin_fmaf(c.ay_m, ym, acc);
+    acc = __builtin_fmaf(c.ay_p, yp, acc);
+    acc = __builtin_fmaf(c.za, z0, acc);
+    acc = __builtin_fmaf(c.zb, z1, acc);
+    acc = __builtin_fmaf(c.zs, zsw, acc);
+    return c.id * acc;
+}
+
 template <int K>
 __device__ __forceinline__ void local_sgs4(const Row4Coef<float> &c, const float id[4], const float r[4], float e[4],
                                            float *lds, int q) {
-    const int li = q >> 4, lj = (q >> 1) & 7, h = q & 1;
+    const int li = q >> 4, lj = (q >> 1) & 7;
     const int base = 4 * q;
-    const int col0 = (li + lj) & 1;  // colour of the lane's cell j is col0 ^ (j & 1) (4h is even)
-    float e0, e1, e2, e3;
-    // red half-sweep from zero: no neighbour values yet
-    {
-        const bool r0 = col0 == 0;  // cells 0 and 2 are red when col0 == 0, cells 1 and 3 otherwise
-        e0 = r0 ? id[0] * r[0] : 0.0f;
-        e1 = r0 ? 0.0f : id[1] * r[1];
-        e2 = r0 ? id[2] * r[2] : 0.0f;
-        e3 = r0 ? 0.0f : id[3] * r[3];
-        if (r0) {
-            lds[base + 0] = e0;
-            lds[base + 2] = e2;
-        } else {
-            lds[base + 1] = e1;
-            lds[base + 3] = e3;
-        }
+    const bool c1 = ((li + lj) & 1) != 0;  // col0
+    // The lane's choice between its cells is made on loaded values (one select per field), never on
+    // an array index: an index chosen per lane would keep the arrays in memory (LDS, addressed
+    // through the dispatch packet), on the critical path of every launch.
+    float gxm[4], gxp[4], gym[4], gyp[4], gzm[4], gzp[4], idv[4], rv[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        gxm[j] = c.gxm[j];
+        gxp[j] = c.gxp[j];
+        gym[j] = c.gym[j];
+        gyp[j] = c.gyp[j];
+        gzm[j] = c.gzm[j];
+        gzp[j] = c.gzp[j];
+        idv[j] = id[j];
+        rv[j] = r[j];
     }
+    // cell ja when col0 = 0, jb when col0 = 1, with the z weights of each case
+    auto cell_of = [&](int ja, float za_a, float zb_a, float zs_a, int jb, float za_b, float zb_b, float zs_b) {
+        SgsCell o;
+        o.ax_m = c1 ? gxm[jb] : gxm[ja];
+        o.ax_p = c1 ? gxp[jb] : gxp[ja];
+        o.ay_m = c1 ? gym[jb] : gym[ja];
+        o.ay_p = c1 ? gyp[jb] : gyp[ja];
+        o.za = c1 ? za_b : za_a;
+        o.zb = c1 ? zb_b : zb_a;
+        o.zs = c1 ? zs_b : zs_a;
+        o.id = c1 ? idv[jb] : idv[ja];
+        o.r = c1 ? rv[jb] : rv[ja];
+        const int j = c1 ? jb : ja;
+        o.st = base + j;
+        o.xm = li > 0 ? base + j - 64 : base + j + 64;
+        o.xp = li < 7 ? base + j + 64 : base + j - 64;
+        return o;
+    };
+    // group P: cells 0, 2 (col0 = 0) or 1, 3 (col0 = 1); group Q: the others.  For col0 = 0:
+    //   P0 = cell 0: z- partner's cell 3 (= its Q1), z+ cell 1 (Q0)
+    //   P1 = cell 2: z- cell 1 (Q0), z+ cell 3 (Q1)
+    //   Q0 = cell 1: z- cell 0 (P0), z+ cell 2 (P1)
+    //   Q1 = cell 3: z- cell 2 (P1), z+ partner's cell 0 (= its P0)
+    // and for col0 = 1:
+    //   P0 = cell 1: z- cell 0 (Q0), z+ cell 2 (Q1)
+    //   P1 = cell 3: z- cell 2 (Q1), z+ partner's cell 0 (= its Q0)
+    //   Q0 = cell 0: z- partner's cell 3 (= its P1), z+ cell 1 (P0)
+    //   Q1 = cell 2: z- cell 1 (P0), z+ cell 3 (P1)
+    // P0 and P1 take the swap of Q1 and Q0, Q0 and Q1 the swap of P1 and P0.
+    const SgsCell P0 = cell_of(0, gzp[0], 0.0f, gzm[0], 1, gzm[1], gzp[1], 0.0f);
+    const SgsCell P1 = cell_of(2, gzm[2], gzp[2], 0.0f, 3, 0.0f, gzm[3], gzp[3]);
+    const SgsCell Q0 = cell_of(1, gzm[1], gzp[1], 0.0f, 0, gzp[0], 0.0f, gzm[0]);
+    const SgsCell Q1 = cell_of(3, 0.0f, gzm[3], gzp[3], 2, gzm[2], gzp[2], 0.0f);
+    // the first half-sweep, from zero: no neighbour values yet
+    float p0 = P0.id * P0.r, p1 = P1.id * P1.r, q0 = 0.0f, q1 = 0.0f;
+    lds[P0.st] = p0;
+    lds[P1.st] = p1;
     __syncthreads();
-    // cell j's update from its x neighbours (other lanes, LDS), its y neighbours ym, yp and its z
-    // neighbours zm, zp: the lane's own cells j - 1 and j + 1 come from registers (the values LDS
-    // holds), only the row's end cells read the partner lane's value
-    auto cell = [&](int j, float ym, float yp, float zm, float zp) -> float {
-        const int at = base + j;
-        const float xm = li > 0 ? lds[at - 64] : 0.0f, xp = li < 7 ? lds[at + 64] : 0.0f;
-        const float nb = c.gxm[j] * xm + c.gxp[j] * xp + c.gym[j] * ym + c.gyp[j] * yp + c.gzm[j] * zm + c.gzp[j] * zp;
-        return id[j] * (r[j] + nb);
-    };
-    // the y neighbours' cells: lanes q - 2 and q + 2 of the same 16-lane row (the same x layer),
-    // moved across lanes by DPP with every lane active; lj = 0 and lj = 7 read 0 (outside the brick)
-    auto ymv = [](float v) {
-        return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x112, 0xF, 0xF, true));
-    };
-    auto ypv = [](float v) {
-        return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x102, 0xF, 0xF, true));
-    };
 #pragma unroll
     for (int s = 1; s <= 2 * K; ++s) {
-        // this half-sweep updates the cells of colour s & 1: cells 0 and 2 when it equals col0
-        // (a half-sweep reads only cells of the other colour, so its stores race with no read)
-        const bool even = (s & 1) == col0;
-        const float ym0 = ymv(e0), ym1 = ymv(e1), ym2 = ymv(e2), ym3 = ymv(e3);
-        const float yp0 = ypv(e0), yp1 = ypv(e1), yp2 = ypv(e2), yp3 = ypv(e3);
-        if (even) {
-            // cell 0 (k = 4h): k - 1 is the partner's cell 3 when h = 1, outside the brick when h = 0
-            e0 = cell(0, ym0, yp0, h ? lds[base - 1] : 0.0f, e1);
-            e2 = cell(2, ym2, yp2, e1, e3);
-            lds[base + 0] = e0;
-            lds[base + 2] = e2;
+        if (s & 1) {
+            const float ym0 = sgs_dpp<DPP_ROW_SHR2>(p0), yp0 = sgs_dpp<DPP_ROW_SHL2>(p0);
+            const float ym1 = sgs_dpp<DPP_ROW_SHR2>(p1), yp1 = sgs_dpp<DPP_ROW_SHL2>(p1);
+            const float sw0 = sgs_dpp<DPP_SWAP_PAIRS>(p0), sw1 = sgs_dpp<DPP_SWAP_PAIRS>(p1);
+            q0 = sgs_update(Q0, lds, ym0, yp0, p0, p1, sw1);
+            q1 = sgs_update(Q1, lds, ym1, yp1, p0, p1, sw0);
+            lds[Q0.st] = q0;
+            lds[Q1.st] = q1;
         } else {
-            // cell 3 (k = 4h + 3): k + 1 is the partner's cell 0 when h = 0, outside when h = 1
-            e1 = cell(1, ym1, yp1, e0, e2);
-            e3 = cell(3, ym3, yp3, e2, h ? 0.0f : lds[base + 4]);
-            lds[base + 1] = e1;
-            lds[base + 3] = e3;
+            const float ym0 = sgs_dpp<DPP_ROW_SHR2>(q0), yp0 = sgs_dpp<DPP_ROW_SHL2>(q0);
+            const float ym1 = sgs_dpp<DPP_ROW_SHR2>(q1), yp1 = sgs_dpp<DPP_ROW_SHL2>(q1);
+            const float sw0 = sgs_dpp<DPP_SWAP_PAIRS>(q0), sw1 = sgs_dpp<DPP_SWAP_PAIRS>(q1);
+            p0 = sgs_update(P0, lds, ym0, yp0, q0, q1, sw1);
+            p1 = sgs_update(P1, lds, ym1, yp1, q0, q1, sw0);
+            lds[P0.st] = p0;
+            lds[P1.st] = p1;
         }
         __syncthreads();
     }
-    e[0] = e0;
-    e[1] = e1;
-    e[2] = e2;
-    e[3] = e3;
+    e[0] = c1 ? q0 : p0;
+    e[1] = c1 ? p0 : q0;
+    e[2] = c1 ? q1 : p1;
+    e[3] = c1 ? p1 : q1;
 }
 
 // Pre-smoothing from zero on the finest level: z = M_b rhs (one brick per 128-thread workgroup), K
@@ -2320,7 +2395,7 @@ __global__ __launch_bounds__(1024) void k_smooth_k(LevelDesc L, OpDesc<T> A, con
 template <class T>
 __global__ __launch_bounds__(BRICK_CELLS) void k_coarse_components(LevelDesc L, OpDesc<T> A, int16_t *code) {
     __shared__ int lab[BRICK_CELLS];
-    __shared__ double dsum[BRICK_CELLS];
+    __shared__ int hasd[BRICK_CELLS];
     const int g = blockIdx.x;
     const GridLevel G = L.gl[g];
     if (!G.coarsest || G.brick_end <= G.brick_start) return;
@@ -2355,17 +2430,15 @@ __global__ __launch_bounds__(BRICK_CELLS) void k_coarse_components(LevelDesc L, 
         if (changed) lab[e] = m;
         if (!__syncthreads_or(changed)) break;
     }
-    // Dirichlet sum of each component, summed by its root in cell order
-    dsum[e] = unk ? (double)dirc : 0.0;
+    // whether each component has a Dirichlet term: dir of an unknown is a sum of conductances
+    // (non-negative), so the component's sum of dir is positive exactly when one of its cells has
+    // dir > 0 (a sum of non-negative numbers rounds to 0 only when every term is 0); each such cell
+    // marks its root (concurrent stores of the same value)
+    hasd[e] = 0;
     __syncthreads();
-    double d = 0.0;
-    if (unk && lab[e] == e)
-        for (int k = e; k < BRICK_CELLS; ++k)
-            if (lab[k] == e) d += dsum[k];
+    if (unk && dirc > T(0)) hasd[lab[e]] = 1;
     __syncthreads();
-    dsum[e] = d;
-    __syncthreads();
-    code[c] = !unk ? (int16_t)-1 : (dsum[lab[e]] > 0.0 ? (int16_t)-2 : (int16_t)lab[e]);
+    code[c] = !unk ? (int16_t)-1 : (hasd[lab[e]] ? (int16_t)-2 : (int16_t)lab[e]);
 }
 
 // Coarse solve of every grid whose last level L is (one brick): block g handles grid g with
