@@ -327,7 +327,7 @@ __global__ void k_adv_face_fwd(VelField F, const float *in0, const float *in1, c
 #pragma unroll
         for (int b = 0; b < 3; ++b) {
             const float t = (float)((double)u[b] * dt);   // dt*u on a float vector
-            const float sft = (float)((double)t / F.dx);  // /dx on a float vector
+            const float sft = (float)vf_div_dx(F, (double)t);  // /dx on a float vector
             p[b] = (double)pc[b] - (double)sft;
         }
         if (WENO) {
@@ -348,7 +348,7 @@ __global__ void k_adv_face_fwd(VelField F, const float *in0, const float *in1, c
         double p[3];
 #pragma unroll
         for (int b = 0; b < 3; ++b) {
-            const double sft = (dt * (double)u[b]) / F.dx;  // dt*u/dx on a double vector
+            const double sft = vf_div_dx(F, dt * (double)u[b]);  // dt*u/dx on a double vector
             p[b] = (double)pc[b] - sft;
             fp[b] = (pc[b] + 0.5 * (a != b)) - sft;
         }
@@ -396,7 +396,7 @@ __global__ void k_adv_face_bwd(VelField F, AdvScratch S, double dt_back) {
 #pragma unroll
         for (int b = 0; b < 3; ++b) {
             const float t = (float)((double)u[b] * dt_back);
-            const float sft = (float)((double)t / F.dx);
+            const float sft = (float)vf_div_dx(F, (double)t);
             p[b] = (double)pc[b] - (double)sft;
         }
         v1 = sample_arr<WENO>(in[a], F.act[a], s, p);
@@ -440,7 +440,7 @@ __global__ void k_adv_cell_fwd(VelField F, const float *q, const uint8_t *aq, Ad
     const int pc[3] = {i, j, k};
     double p[3];
 #pragma unroll
-    for (int b = 0; b < 3; ++b) p[b] = (double)pc[b] - (dt * u[b]) / F.dx;
+    for (int b = 0; b < 3; ++b) p[b] = (double)pc[b] - vf_div_dx(F, dt * u[b]);
     float v8[8];
     const Corner cv = corner_of(s, p);
     if (!empty) {
@@ -491,7 +491,7 @@ __global__ void k_adv_cell_bwd(VelField F, float *q, const uint8_t *aq, AdvScrat
     if (u[0] != 0.0 || u[1] != 0.0 || u[2] != 0.0) {
         double p[3];
 #pragma unroll
-        for (int b = 0; b < 3; ++b) p[b] = (double)pc[b] - (dt_back * u[b]) / F.dx;
+        for (int b = 0; b < 3; ++b) p[b] = (double)pc[b] - vf_div_dx(F, dt_back * u[b]);
         v1 = sample_arr<WENO>(S.v0[0], nullptr, s, p);
     } else {
         v1 = v0;

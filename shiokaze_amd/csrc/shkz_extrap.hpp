@@ -38,7 +38,15 @@ struct VelField {
     float *vel[3];
     uint8_t *act[3];
     size_t n[3];          // faces per array
+    // dx a power of two (1/256 for a unit box of 256 cells): x / dx then equals x * inv_dx bit for
+    // bit (both round the same exact value once; the inverse is a power of two), and the kernels
+    // multiply instead of dividing (an FP64 division is about eleven instructions here)
+    int dx_pow2;
+    double inv_dx;
 };
+
+// x / F.dx, exactly as the division rounds.
+__device__ __forceinline__ double vf_div_dx(const VelField &F, double x) { return F.dx_pow2 ? x * F.inv_dx : x / F.dx; }
 
 __device__ __forceinline__ void vf_shape(const VelField &F, int a, int s[3]) {
     s[0] = F.w + (a == 0);
@@ -343,7 +351,7 @@ __global__ void k_constrain(VelField F, float *X0, float *X1, float *X2, const i
         if (inside) {
             double nrm[3];
 #pragma unroll
-            for (int b = 0; b < 3; ++b) nrm[b] = der[b] / F.dx;
+            for (int b = 0; b < 3; ++b) nrm[b] = vf_div_dx(F, der[b]);
             double n2 = 0.0;
 #pragma unroll
             for (int b = 0; b < 3; ++b) n2 += nrm[b] * nrm[b];

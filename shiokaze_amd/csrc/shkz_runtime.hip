@@ -2502,6 +2502,16 @@ struct shkz_server {
 
 // ============================================================================ C ABI
 // ============================================================================ after the projection
+// dx of a face field, and whether it is a power of two far from the exponent range's ends (then
+// its inverse is exact and the kernels multiply by it)
+static void vf_set_dx(VelField &F, const shkz_velocity_field *f) {
+    F.dx = f->dx > 0 ? f->dx : std::min(1.0 / f->w, std::min(1.0 / f->h, 1.0 / f->d));
+    int ex = 0;
+    const double mant = std::frexp(F.dx, &ex);
+    F.dx_pow2 = (mant == 0.5 && ex > -400 && ex < 400) ? 1 : 0;
+    F.inv_dx = 1.0 / F.dx;
+}
+
 // macutility3::extrapolate_and_constrain_velocity (macutility3.cpp:89-93) on the work stream:
 // `width` in-place extrapolation passes over the three face arrays, then the constraint (which
 // also normalises the activity codes of the passes).  vel/act/solid are device arrays.
@@ -2514,7 +2524,7 @@ static int extrapolate_stages(shkz_ctx *C, const shkz_velocity_field *f, int wid
     F.h = f->h;
     F.d = f->d;
     F.solid64 = f->solid_dtype == SHKZ_F64;
-    F.dx = f->dx > 0 ? f->dx : std::min(1.0 / f->w, std::min(1.0 / f->h, 1.0 / f->d));
+    vf_set_dx(F, f);
     F.solid = solid;
     for (int a = 0; a < 3; ++a) {
         F.vel[a] = vel[a];
@@ -2622,7 +2632,7 @@ static VelField vel_field_of(const shkz_velocity_field *f, float *const vel[3], 
     F.h = f->h;
     F.d = f->d;
     F.solid64 = f->solid_dtype == SHKZ_F64;
-    F.dx = f->dx > 0 ? f->dx : std::min(1.0 / f->w, std::min(1.0 / f->h, 1.0 / f->d));
+    vf_set_dx(F, f);
     F.solid = nullptr;
     for (int a = 0; a < 3; ++a) {
         F.vel[a] = vel[a];
