@@ -424,7 +424,7 @@ template <int WENO>
 __global__ void k_adv_cell_fwd(VelField F, const float *q, const uint8_t *aq, AdvScratch S, const void *fluid,
                                int fluid64, double dt, int trim, float background, int minmax) {
 #pragma clang fp contract(off)
-    const size_t c = (size_t)xcd_brick(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    const size_t c = (size_t)xcd_brick(blockIdx.x, gridDim.x) * VF_BLOCK + threadIdx.x;
     const size_t M = (size_t)F.w * F.h * F.d;
     if (c >= M) return;
     if (aq && !aq[c]) {
@@ -472,7 +472,7 @@ __global__ void k_adv_cell_fwd(VelField F, const float *q, const uint8_t *aq, Ad
 template <int WENO>
 __global__ void k_adv_cell_bwd(VelField F, float *q, const uint8_t *aq, AdvScratch S, double dt_back) {
 #pragma clang fp contract(off)
-    const size_t c = (size_t)xcd_brick(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    const size_t c = (size_t)xcd_brick(blockIdx.x, gridDim.x) * VF_BLOCK + threadIdx.x;
     const size_t M = (size_t)F.w * F.h * F.d;
     if (c >= M) return;
     if (aq && !aq[c]) return;
@@ -503,7 +503,7 @@ __global__ void k_adv_cell_bwd(VelField F, float *q, const uint8_t *aq, AdvScrat
 }
 
 __global__ void k_adv_cell_copy(size_t M, float *q, const uint8_t *aq, AdvScratch S) {
-    const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t c = (size_t)blockIdx.x * VF_BLOCK + threadIdx.x;
     if (c < M && (!aq || aq[c])) q[c] = S.v0[0][c];
 }
 

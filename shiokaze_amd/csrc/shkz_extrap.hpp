@@ -59,6 +59,11 @@ __device__ __forceinline__ void vf_shape(const VelField &F, int a, int s[3]) {
 // load.  (A flat index over the three arrays made it vary inside the wavefront that straddles two
 // arrays, and hipcc then lowered a dynamically indexed kernel-argument read with v_readfirstlane,
 // as if it were uniform: that wavefront's lanes read the first lane's array extent.)
+// Threads per block of every face, cell and activity-word kernel here.  The kernels index with
+// this constant, not blockDim.x: the workgroup size is a 16-bit field of the hidden kernel
+// arguments, which hipcc reads with a vector load, one more memory round trip before a wave can
+// form its index.
+constexpr int VF_BLOCK = 256;
 inline dim3 vf_grid(const size_t n[3]) {
     const size_t m = n[0] > n[1] ? (n[0] > n[2] ? n[0] : n[2]) : (n[1] > n[2] ? n[1] : n[2]);
     return dim3((unsigned)(((m + 255) / 256 + 7) / 8 * 8), 3);
@@ -67,7 +72,7 @@ inline dim3 vf_grid(const size_t n[3]) {
 // (array, local index, i, j, k) of this thread; false past the end of its array
 __device__ __forceinline__ bool vf_decode(const VelField &F, int &a, size_t &f, int &i, int &j, int &k) {
     a = (int)blockIdx.y;
-    f = (size_t)xcd_brick(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;  // XCD-contiguous slabs
+    f = (size_t)xcd_brick(blockIdx.x, gridDim.x) * VF_BLOCK + threadIdx.x;  // XCD-contiguous slabs
     if (f >= F.n[a]) return false;
     int s[3];
     vf_shape(F, a, s);
@@ -101,7 +106,7 @@ __global__ void k_pack_activity(VelField F, BitField B) {
     const int W = B.W[a];
     const size_t nw = (size_t)vf_rows(F, a) * W;
     const unsigned lane = threadIdx.x & 63;
-    const size_t wave = ((size_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const size_t wave = ((size_t)blockIdx.x * VF_BLOCK + threadIdx.x) >> 6;
     bool on[PACK_GROUPS];
 #pragma unroll
     for (int g = 0; g < PACK_GROUPS; ++g) {
@@ -132,7 +137,7 @@ __global__ void k_extrap_pass(VelField F, BitField Bi, BitField Bo) {
     int s[3];
     vf_shape(F, a, s);
     const int W = Bi.W[a];
-    const size_t w = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t w = (size_t)blockIdx.x * VF_BLOCK + threadIdx.x;
     const size_t nw = (size_t)vf_rows(F, a) * W;
     if (w >= nw) return;
     const unsigned r = (unsigned)w / (unsigned)W, kw = (unsigned)w - r * (unsigned)W;  // 32-bit: fewer than 2^32 words
@@ -177,7 +182,7 @@ __global__ void k_unpack_activity(VelField F, BitField B0, BitField Bf) {
     int s[3];
     vf_shape(F, a, s);
     const int W = Bf.W[a];
-    const size_t w = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t w = (size_t)blockIdx.x * VF_BLOCK + threadIdx.x;
     if (w >= (size_t)vf_rows(F, a) * W) return;
     uint32_t fresh = Bf.bits[a][w] & ~B0.bits[a][w];
     if (!fresh) return;
@@ -268,7 +273,8 @@ __device__ __forceinline__ T acc(T v, double term) {
     return (T)((double)v + term);
 }
 
-// The solid sample and its derivative at p (nodal array), in the level set's type T.
+// The solid sample and its derivative at p (nodal array), in the level set's type T.  Returns the
+// interpolated value where the point is inside (negative); 0 where it is not.
 template <class T>
 __device__ __forceinline__ double solid_sample(const VelField &F, const double p[3], double der[3], bool &inside) {
 #pragma clang fp contract(off)
@@ -278,6 +284,16 @@ __device__ __forceinline__ double solid_sample(const VelField &F, const double p
     T v[8];
 #pragma unroll
     for (int n = 0; n < 8; ++n) v[n] = S[corner_index(c, n, s)];
+    // with no negative corner the interpolation is a sum of non-negative products (a NaN corner
+    // makes it NaN): never below 0 whatever the rounding, so the point is outside and the weights
+    // are not needed (most faces lie away from the solid)
+    bool any_neg = false;
+#pragma unroll
+    for (int n = 0; n < 8; ++n) any_neg = any_neg || v[n] < T(0);
+    if (!any_neg) {
+        inside = false;
+        return 0.0;
+    }
     double w[8];
     interp_weights(c, w);
     T val = T(0);

@@ -1555,10 +1555,15 @@ __device__ __forceinline__ void apply_row4(const LevelDesc &L, const OpDesc<T> &
     }
 }
 
-// Brick and lane of a workgroup that takes blockDim.x / 128 consecutive bricks (block ids remapped
-// XCD-aware; gridDim.x a multiple of 8); br >= L.nbricks for an idle slot.
+// Brick and lane of a workgroup that takes BPB consecutive bricks (block ids remapped XCD-aware;
+// gridDim.x a multiple of 8); br >= L.nbricks for an idle slot.  Row-4 launches give the finest
+// level BLOCK threads (four bricks) and coarse levels ROW4_THREADS (one brick): the kernels' FINEST
+// flag fixes BPB at compile time (blockDim.x is a 16-bit hidden kernel argument that hipcc reads
+// with a vector load, one more memory round trip at the start of every wave).
+template <int FINEST>
 __device__ __forceinline__ void row4_slot(const LevelDesc &L, int &br, int &q) {
-    br = (int)(blockDim.x / ROW4_THREADS) * xcd_brick(blockIdx.x, gridDim.x) + (int)(threadIdx.x / ROW4_THREADS);
+    constexpr int BPB = FINEST ? BLOCK / ROW4_THREADS : 1;
+    br = BPB * xcd_brick(blockIdx.x, gridDim.x) + (int)(threadIdx.x / ROW4_THREADS);
     q = threadIdx.x % ROW4_THREADS;
 }
 
@@ -1582,7 +1587,7 @@ template <int FINEST>
 __global__ __launch_bounds__(BLOCK) void k_residual4(LevelDesc L, OpDesc<float> A, const float *__restrict__ rhs,
                                                      const float *__restrict__ z, float *__restrict__ res, Gate Q) {
     int br, q;
-    row4_slot(L, br, q);
+    row4_slot<FINEST>(L, br, q);
     if (br >= L.nbricks) return;
     const BrickMeta m = L.meta[br];
     if (all_done(Q) || skip_smooth(Q, L, m)) return;
@@ -1603,7 +1608,7 @@ __global__ __launch_bounds__(BLOCK) void k_smooth_first4(LevelDesc L, const floa
                                                          const float *__restrict__ rhs, float *__restrict__ z,
                                                          float omega, Gate Q) {
     int br, q;
-    row4_slot(L, br, q);
+    row4_slot<FINEST>(L, br, q);
     if (br >= L.nbricks) return;
     const BrickMeta m = L.meta[br];
     if (all_done(Q) || skip_smooth(Q, L, m)) return;
@@ -1807,7 +1812,7 @@ __global__ __launch_bounds__(ROW4_THREADS) void k_cg_update4(LevelDesc L, double
 __global__ __launch_bounds__(BLOCK) void k_update_p4(LevelDesc L, const float *__restrict__ z, float *__restrict__ p,
                                                      const SolveScalars *S, const BatchCtl *ctl, int first) {
     int br, q;
-    row4_slot(L, br, q);
+    row4_slot<1>(L, br, q);  // the finest level only (row-4 launch of BLOCK threads)
     if (br >= L.nbricks) return;
     const long c = (long)br * BRICK_CELLS + 4 * q;
     if (L.ngrids > 1 && (ctl->all_done || S[grid_of(L, br)].done)) return;
@@ -2122,7 +2127,7 @@ __global__ __launch_bounds__(BLOCK) void k_prolong4(LevelDesc Lf, OpDesc<float> 
     __shared__ int bid[BLOCK / ROW4_THREADS][8];
     if (all_done(Q)) return;
     int br, q;
-    row4_slot(Lf, br, q);
+    row4_slot<FINEST>(Lf, br, q);
     const int slot = threadIdx.x / ROW4_THREADS;
     // every thread reaches the barriers; a slot past the end or of a skipped brick only idles
     bool work = br < Lf.nbricks;

@@ -2533,7 +2533,7 @@ static int extrapolate_stages(shkz_ctx *C, const shkz_velocity_field *f, int wid
     }
     const size_t total = F.n[0] + F.n[1] + F.n[2];
     const size_t nodes = (size_t)(f->w + 1) * (f->h + 1) * (f->d + 1);
-    const dim3 grid = vf_grid(F.n), block(256);
+    const dim3 grid = vf_grid(F.n), block(VF_BLOCK);
     int *exist = nullptr;
     SHKZ_TRY(arena_get(AR, "vx_exist", 1, &exist));
     K.go("reset", -1, 0, 4.0, dim3(1), dim3(256), k_reset_ints, exist, 1);
@@ -2666,7 +2666,7 @@ static int advect_velocity_stages(shkz_ctx *C, const shkz_velocity_field *f, con
     Launcher K{C, s};
     VelField F = vel_field_of(f, vel, act);
     const size_t total = F.n[0] + F.n[1] + F.n[2];
-    const dim3 grid = vf_grid(F.n), block(256);
+    const dim3 grid = vf_grid(F.n), block(VF_BLOCK);
     AdvScratch S;
     const char *nv[3] = {"adv_v0", "adv_v1", "adv_v2"}, *nl[3] = {"adv_lo0", "adv_lo1", "adv_lo2"},
                *nh[3] = {"adv_hi0", "adv_hi1", "adv_hi2"}, *nn[3] = {"adv_nb0", "adv_nb1", "adv_nb2"};
@@ -2710,7 +2710,7 @@ static int advect_scalar_stages(shkz_ctx *C, const shkz_velocity_field *f, float
     Launcher K{C, s};
     VelField F = vel_field_of(f, vel, act);
     const size_t M = (size_t)f->w * f->h * f->d;
-    const dim3 grid(xcd_blocks(M, 256)), block(256);
+    const dim3 grid(xcd_blocks(M, VF_BLOCK)), block(VF_BLOCK);
     AdvScratch S = {};
     const bool mc = p->maccormack != 0;
     SHKZ_TRY(arena_get(AR, "adv_q0", M, &S.v0[0]));
