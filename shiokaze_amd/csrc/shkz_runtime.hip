@@ -2112,11 +2112,10 @@ static int project_stages(shkz_ctx *C, const shkz_grid *grids, int ngrids, const
         if (P.fp64_levels) {
             K.go("coarsen", -1, 0, 0, g, b, k_coarsen<double, double>, Lf.desc, op_of<double>(Lf), Lc.desc,
                  (double *)Lc.dir, (double *)Lc.gx, (double *)Lc.gy, (double *)Lc.gz);
-        } else if (l == 1) {
-            OpDesc<double> A64{P.dir64, P.gx64, P.gy64, P.gz64};
-            K.go("coarsen", -1, 0, 0, g, b, k_coarsen<double, float>, Lf.desc, A64, Lc.desc, (float *)Lc.dir,
-                 (float *)Lc.gx, (float *)Lc.gy, (float *)Lc.gz);
         } else {
+            // mixed: level 1 too is formed from the FP32 copy of the level-0 operator the assembly
+            // writes (the same sums in double of values already rounded to FP32, stored in FP32;
+            // half the bytes of the FP64 operator)
             K.go("coarsen", -1, 0, 0, g, b, k_coarsen<float, float>, Lf.desc, op_of<float>(Lf), Lc.desc,
                  (float *)Lc.dir, (float *)Lc.gx, (float *)Lc.gy, (float *)Lc.gz);
         }
