@@ -290,7 +290,7 @@ __global__ __launch_bounds__(BLOCK) void k_flags(GridDesc G, int nbx, int nby, i
 // The same block also finishes the brick for the solver: the inverse diagonal the smoothers use
 // (k_idiag's arithmetic on the stored FP32 (id32) or FP64 (id64) values, each coupling of a high
 // face being the neighbour's low one, formed here from the same samples), the brick's unknown count
-// (k_brick_counts' count, which the coarse levels still launch), and whether its operator is one
+// (the count k_idiag forms on the coarse levels), and whether its operator is one
 // constant: every cell an unknown, dir = 0, all six couplings and the inverse diagonal equal to the
 // brick's first cell's (then BM_UNIFORM and its UniCoef record).
 struct OperatorFinish {
@@ -563,8 +563,10 @@ __global__ __launch_bounds__(256) void k_links(LevelDesc L, LevelDesc Lf, LevelD
 // The smoothers use a stored inverse diagonal: damped Jacobi is
 //   z' = (1 - omega) z + omega * idiag * (rhs + sum_f g_f z_q),
 // and idiag = 0 marks a non-unknown (or a cell whose diagonal vanished).
+// With counts, the same block also counts the brick's unknowns (dir >= 0), plus UNIFORM_COUNT_BIT
+// on BM_UNIFORM bricks.
 template <class T>
-__global__ __launch_bounds__(BLOCK) void k_idiag(LevelDesc L, OpDesc<T> A, T *__restrict__ idiag) {
+__global__ __launch_bounds__(BLOCK) void k_idiag(LevelDesc L, OpDesc<T> A, T *__restrict__ idiag, int *counts) {
     const int br = brick_or_idle(L);
     if (br < 0) return;
     const int t = threadIdx.x;
@@ -580,55 +582,10 @@ __global__ __launch_bounds__(BLOCK) void k_idiag(LevelDesc L, OpDesc<T> A, T *__
         if (diag > 0) out = T(1) / diag;
     }
     idiag[c] = out;
-}
-
-// Marks the level-0 bricks whose operator is one constant (BM_UNIFORM) and records the constant
-// and the inverse diagonal the smoothers use (from whichever idiag array the level keeps; 0 when
-// none is built).  Runs after the operator and idiag kernels; the meta records were uploaded
-// without the flag.
-__global__ __launch_bounds__(BLOCK) void k_uniform(LevelDesc L, OpDesc<double> A, const double *__restrict__ id64,
-                                                   const float *__restrict__ id32, BrickMeta *meta, UniCoef *uni) {
-    __shared__ double gref, iref64;
-    __shared__ float iref32;
-    const int br = brick_or_idle(L);
-    if (br < 0) return;
-    const int t = threadIdx.x;
-    const long c = (long)br * BRICK_CELLS + t;
-    if (t == 0) {
-        gref = A.gx[c];
-        iref64 = id64 ? id64[c] : 0.0;
-        iref32 = id32 ? id32[c] : 0.0f;
+    if (counts) {
+        const int n = __syncthreads_count(dirc >= 0);
+        if (t == 0) counts[br] = n | ((L.meta[br].flags & BM_UNIFORM) ? UNIFORM_COUNT_BIT : 0);
     }
-    __syncthreads();
-    const double G = gref;
-    const BrickMeta m = L.meta[br];
-    const Nbrs N = nbrs_of(m, br);
-    long idx[6];
-    nb_index(N, t, idx);
-    bool ok = G > 0.0 && A.dir[c] == 0.0 && A.gx[c] == G && A.gy[c] == G && A.gz[c] == G;
-    ok = ok && idx[NB_XP] >= 0 && idx[NB_YP] >= 0 && idx[NB_ZP] >= 0;
-    ok = ok && A.gx[idx[NB_XP] >= 0 ? idx[NB_XP] : c] == G && A.gy[idx[NB_YP] >= 0 ? idx[NB_YP] : c] == G &&
-         A.gz[idx[NB_ZP] >= 0 ? idx[NB_ZP] : c] == G;
-    ok = ok && (!id64 || id64[c] == iref64) && (!id32 || id32[c] == iref32);
-    if (__syncthreads_and(ok) && t == 0) {
-        meta[br].flags = m.flags | BM_UNIFORM;
-        UniCoef u;
-        u.g64 = G;
-        u.id64 = iref64;
-        u.g32 = (float)G;
-        u.id32 = iref32;
-        u.pad[0] = u.pad[1] = 0.0f;
-        uni[br] = u;
-    }
-}
-
-// Number of unknowns (dir >= 0) in every stored brick, plus UNIFORM_COUNT_BIT on BM_UNIFORM bricks.
-template <class T>
-__global__ __launch_bounds__(BLOCK) void k_brick_counts(LevelDesc L, const T *__restrict__ dir, int *counts) {
-    const int br = brick_or_idle(L);
-    if (br < 0) return;
-    const int n = __syncthreads_count(dir[(long)br * BRICK_CELLS + threadIdx.x] >= 0);
-    if (threadIdx.x == 0) counts[br] = n | ((L.meta[br].flags & BM_UNIFORM) ? UNIFORM_COUNT_BIT : 0);
 }
 
 // ------------------------------------------------------------------------------ Krylov (batch)

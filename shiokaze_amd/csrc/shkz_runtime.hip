@@ -2136,35 +2136,28 @@ static int project_stages(shkz_ctx *C, const shkz_grid *grids, int ngrids, const
                      k_coarse_components<float>, L.desc, op_of<float>(L), L.ccode);
         }
     }
-    // inverse diagonals for the smoothers (level 0: written by the assembly)
+    // unknowns per level and grid (results + byte model of the kernel timers): level 0 counted by
+    // the assembly, the coarse levels by their inverse-diagonal kernels below
+    int total = 0;
+    for (int l = 0; l < P.nlev; ++l) total += C->levels[l].nbricks;
+    if (total != count_total) return fail(SHKZ_ERR_DEVICE, "internal: %d stored bricks, %d counted", total, count_total);
+    // inverse diagonals for the smoothers (level 0: written by the assembly), with the brick counts
     if (multigrid) {
+        int off = C->levels[0].nbricks;
         for (int l = 1; l < P.nlev; ++l) {
             LevelHost &L = C->levels[l];
             const dim3 g(brick_grid(L.nbricks)), b(BLOCK);
             if (P.fp64_levels)
-                K.go("idiag", -1, 0, 0, g, b, k_idiag<double>, L.desc, op_of<double>(L), (double *)L.idiag);
+                K.go("idiag", -1, 0, 0, g, b, k_idiag<double>, L.desc, op_of<double>(L), (double *)L.idiag, counts + off);
             else
-                K.go("idiag", -1, 0, 0, g, b, k_idiag<float>, L.desc, op_of<float>(L), (float *)L.idiag);
+                K.go("idiag", -1, 0, 0, g, b, k_idiag<float>, L.desc, op_of<float>(L), (float *)L.idiag, counts + off);
+            off += L.nbricks;
         }
     }
     // level-0 bricks with a constant operator (BM_UNIFORM, marked by the assembly): the smoothers
     // and operator kernels read their constants instead of the operator arrays
-    // unknowns per level and grid (results + byte model of the kernel timers)
     {
-        int total = 0;
-        for (int l = 0; l < P.nlev; ++l) total += C->levels[l].nbricks;
-        if (total != count_total) return fail(SHKZ_ERR_DEVICE, "internal: %d stored bricks, %d counted", total, count_total);
         int *cnt = counts;
-        int off = C->levels[0].nbricks;  // level 0: counted by the assembly
-        for (int l = 1; l < P.nlev; ++l) {
-            LevelHost &L = C->levels[l];
-            const dim3 g(brick_grid(L.nbricks)), b(BLOCK);
-            if (P.fp64_levels)
-                K.go("count", -1, 0, 0, g, b, k_brick_counts<double>, L.desc, (const double *)L.dir, cnt + off);
-            else
-                K.go("count", -1, 0, 0, g, b, k_brick_counts<float>, L.desc, (const float *)L.dir, cnt + off);
-            off += L.nbricks;
-        }
         void *hcounts = nullptr;
         SHKZ_TRY(ensure_pinned(C->pin_counts, (size_t)std::max(total, 1) * sizeof(int), &hcounts));
         if (total) SHKZ_HIP(hipMemcpyAsync(hcounts, cnt, (size_t)total * sizeof(int), hipMemcpyDeviceToHost, s));
