@@ -1628,7 +1628,9 @@ static int autotune(shkz_ctx *C, Launcher &K, Problem &P, const shkz_params *prm
             SHKZ_TRY(consider(k));
         }
     if (prm->mg_coarse_sweeps <= 0)
-        for (int v : {16, 32, 48, 96}) {
+        // the bottom grid is one brick: a few sweeps can be enough there (at 256^3 four sweeps keep
+        // the iteration count of sixteen), and the tuner times the whole solve
+        for (int v : {4, 8, 16, 32, 48, 96}) {
             if (v == cur.coarse) continue;
             Knobs k = cur;
             k.coarse = v;
