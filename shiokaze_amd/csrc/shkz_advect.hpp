@@ -4,7 +4,9 @@
 // level set (advect_scalar -> advect_cell, :38-45,239-288).
 //
 //   k_adv_face_fwd   forward semi-Lagrangian pass of advect_semiLagrangian_u (:68-137) with the
-//                    MacCormack min/max and narrow-band record
+//                    MacCormack min/max and narrow-band record; its result holds 0 at inactive
+//                    faces, so the backward pass samples it without the activity bytes (half of
+//                    its loads)
 //   k_adv_face_bwd   backward pass from the forward result, fused with MacCormack's final
 //                    combination (:152-168), written in place over the input
 //   k_adv_cell_fwd,
@@ -50,7 +52,15 @@ __device__ __forceinline__ float face_val(const float *U, const uint8_t *A, size
 
 __device__ __forceinline__ int clamp_i(int v, int n) { return v < 0 ? 0 : (v > n - 1 ? n - 1 : v); }
 
-// macarray3::convert_to_full at face (dim; i,j,k) of the fields U (activity F.act).
+// Flat index of (i, j, k) in an array of shape s, in 32-bit arithmetic: the advection entry points
+// reject grids with (w+1)(h+1)(d+1) >= 2^32.
+__device__ __forceinline__ size_t flat3(int i, int j, int k, const int s[3]) {
+    return ((unsigned)i * (unsigned)s[1] + (unsigned)j) * (unsigned)s[2] + (unsigned)k;
+}
+
+// macarray3::convert_to_full at face (dim; i,j,k) of the fields U (activity F.act).  MASKED: U may
+// hold anything at inactive faces (read through the activity bytes); otherwise U holds 0 there.
+template <bool MASKED>
 __device__ __forceinline__ void full_face(const VelField &F, const float *const U[3], int dim, int i, int j, int k,
                                           float out[3]) {
 #pragma clang fp contract(off)
@@ -60,7 +70,8 @@ __device__ __forceinline__ void full_face(const VelField &F, const float *const 
         int s[3];
         vf_shape(F, ud, s);
         if (ud == dim) {
-            out[ud] = face_val(U[ud], F.act[ud], ((size_t)i * s[1] + j) * s[2] + k);
+            const size_t q = flat3(i, j, k, s);
+            out[ud] = MASKED ? face_val(U[ud], F.act[ud], q) : U[ud][q];
             continue;
         }
         double u = 0.0;
@@ -71,7 +82,8 @@ __device__ __forceinline__ void full_face(const VelField &F, const float *const 
                 int q[3];
 #pragma unroll
                 for (int a = 0; a < 3; ++a) q[a] = clamp_i(pc[a] - (a == dim) + ii * (a == dim) + jj * (a == ud), s[a]);
-                u += (double)face_val(U[ud], F.act[ud], ((size_t)q[0] * s[1] + q[1]) * s[2] + q[2]);
+                const size_t n = flat3(q[0], q[1], q[2], s);
+                u += (double)(MASKED ? face_val(U[ud], F.act[ud], n) : U[ud][n]);
             }
         u /= 4.0;
         out[ud] = (float)u;
@@ -87,8 +99,8 @@ __device__ __forceinline__ void full_cell(const VelField &F, int i, int j, int k
     for (int a = 0; a < 3; ++a) {
         int s[3];
         vf_shape(F, a, s);
-        const size_t q0 = ((size_t)i * s[1] + j) * s[2] + k;
-        const size_t q1 = ((size_t)(i + (a == 0)) * s[1] + (j + (a == 1))) * s[2] + (k + (a == 2));
+        const size_t q0 = flat3(i, j, k, s);
+        const size_t q1 = flat3(i + (a == 0), j + (a == 1), k + (a == 2), s);
         // values and activity bytes load together
         const float u0 = F.vel[a][q0], u1 = F.vel[a][q1];
         const bool a0 = F.act[a][q0] != 0, a1 = F.act[a][q1] != 0;
@@ -278,7 +290,7 @@ __device__ double weno3_arr(const float *U, const uint8_t *A, const int s[3], co
             double vx[6];
 #pragma unroll
             for (int ii = 0; ii < 6; ++ii) {
-                const size_t q = ((size_t)clamp_i(b[0] + ii - 2, s[0]) * s[1] + j) * s[2] + k;
+                const size_t q = flat3(clamp_i(b[0] + ii - 2, s[0]), j, k, s);
                 vx[ii] = A ? (double)face_val(U, A, q) : (double)U[q];
             }
             const double r = weno6(g[0], fr[0], vx);
@@ -311,10 +323,13 @@ __global__ void k_adv_face_fwd(VelField F, const float *in0, const float *in1, c
     int a, i, j, k;
     size_t f;
     if (!vf_decode(F, a, f, i, j, k)) return;
-    if (!F.act[a][f]) return;
+    if (!F.act[a][f]) {
+        S.v0[a][f] = 0.0f;  // what the activity mask reads there
+        return;
+    }
     const float *const in[3] = {in0, in1, in2};
     float u[3];
-    full_face(F, in, a, i, j, k, u);
+    full_face<true>(F, in, a, i, j, k, u);
     const bool empty = u[0] == 0.0f && u[1] == 0.0f && u[2] == 0.0f;
     int s[3];
     vf_shape(F, a, s);
@@ -386,7 +401,7 @@ __global__ void k_adv_face_bwd(VelField F, AdvScratch S, double dt_back) {
     }
     const float *const in[3] = {S.v0[0], S.v0[1], S.v0[2]};
     float u[3];
-    full_face(F, in, a, i, j, k, u);
+    full_face<false>(F, in, a, i, j, k, u);
     int s[3];
     vf_shape(F, a, s);
     const int pc[3] = {i, j, k};
@@ -399,7 +414,7 @@ __global__ void k_adv_face_bwd(VelField F, AdvScratch S, double dt_back) {
             const float sft = (float)vf_div_dx(F, (double)t);
             p[b] = (double)pc[b] - (double)sft;
         }
-        v1 = sample_arr<WENO>(in[a], F.act[a], s, p);
+        v1 = sample_arr<WENO>(in[a], nullptr, s, p);
     } else {
         v1 = v0;
     }

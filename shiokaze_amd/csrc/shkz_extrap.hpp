@@ -261,9 +261,11 @@ __device__ __forceinline__ void deriv_weights(const Corner &c, double dw[24]) {
     dw[23] = xi * yj;
 }
 
+// Flat index of corner n of c in an array of shape s.  32-bit arithmetic: the entry points that
+// sample face, cell and node arrays reject grids with (w+1)(h+1)(d+1) >= 2^32.
 __device__ __forceinline__ size_t corner_index(const Corner &c, int n, const int s[3]) {
-    const int i = c.i + (n & 1), j = c.j + ((n >> 1) & 1), k = c.k + ((n >> 2) & 1);
-    return ((size_t)i * s[1] + j) * s[2] + k;
+    const unsigned i = c.i + (n & 1), j = c.j + ((n >> 1) & 1), k = c.k + ((n >> 2) & 1);
+    return (i * (unsigned)s[1] + j) * (unsigned)s[2] + k;
 }
 
 // Accumulation in the array's element type T after a double product (value += array(idx)*coef).
