@@ -245,3 +245,9 @@ def test_invalid_input_is_refused(ctx):
     f = ctx._vfield(w, h, d, vel, act, 0.0, False)
     assert S.lib().shkz_advect_scalar(ctx.handle, ctypes.byref(f), q.ctypes.data, None, ctypes.byref(p)) == \
         S.ERR_INVALID
+    # the kernels index face, cell and node arrays in 32 bits: a grid with (w+1)(h+1)(d+1) >= 2^32
+    # (1701^3 = 4.9e9) is refused before any array is read, so the small arrays stand in for it
+    big = ctx._vfield(1700, 1700, 1700, vel, act, 0.0, False)
+    p.weno = 0
+    assert S.lib().shkz_advect_velocity(ctx.handle, ctypes.byref(big), ctypes.byref(p)) == S.ERR_INVALID
+    assert "32-bit" in S.last_error()
